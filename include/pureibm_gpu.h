@@ -1,0 +1,195 @@
+/*
+ * pureibm_gpu.h -- C-ABI of the MI355X (gfx950) per-time-step fluid solve of
+ * PUReIBM-PS2D-FD1D.  Every entry point is extern "C", takes plain pointers and
+ * sizes, and returns an int status (0 = ok, < 0 = error; pib_last_error()
+ * gives the text).  No exception, abort or STOP crosses this boundary.
+ *
+ * The entry points replace the reference's per-step Fortran module procedures,
+ * which share state through module globals (global_data.F90:66-125,
+ * workarrays.F90:64-80).  Here that state lives in an opaque context that owns
+ * the device copies; host arrays passed in are borrowed for the call only and
+ * are in the reference's Fortran (column-major) layout:
+ *   u, nl, onl, ff : complex(8) (nx+1, my2, mz, 3)   (global_data.F90:66-68)
+ *   p              : complex(8) (nx,   my2, mz)
+ *   ubcp, nlbcp, onlbcp, diffn, fr : real(8) (nx+1, my, mz, 3)
+ *   pbcp           : real(8) (nx, my, mz);  ppr : real(8) (nx, my, mz, 3)
+ *   xc             : real(8) (nbody, 3)  grid units, node i at position i
+ *   bndpts, cd     : real(8) (3, nbnd)   unit-sphere points and normals
+ * Complex arrays are interleaved (re, im) doubles.
+ *
+ * One context per realization; a context is not thread safe; every call on a
+ * context is issued on that context's own HIP stream.
+ */
+#ifndef PUREIBM_GPU_H
+#define PUREIBM_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define PIB_OK 0
+#define PIB_ERR_ARG (-1)          /* bad argument / size */
+#define PIB_ERR_UNSUPPORTED (-2)  /* option the GPU path does not implement */
+#define PIB_ERR_HIP (-3)          /* HIP runtime error */
+#define PIB_ERR_STATE (-4)        /* call out of order (e.g. step before set_particles) */
+#define PIB_ERR_BLOWUP (-5)       /* norm_drag > 1e6: boundary_condition.F90:874-884 */
+#define PIB_ERR_PIVOT (-6)        /* zero Thomas pivot: tridiagonal.F90:62,72-76 */
+
+/* precision modes */
+#define PIB_FP64 0        /* everything in FP64 */
+#define PIB_FP32_BULK 1   /* FFTs, nonlinear terms, IBM forcing in FP32; (0,0) mode,
+                             mean-pressure-gradient/umean update and per-particle
+                             force sums in FP64 */
+
+/* discretisation of the x-convective flux, nl_allflow.F90:96-294 */
+#define PIB_SCHEME_CENTER 0
+#define PIB_SCHEME_UPWIND 1
+#define PIB_LIMITER_NONE 0
+#define PIB_LIMITER_MINMOD 1
+
+/* Run constants: what initflo (initialize_flo.F90:61-946) derives from the
+ * namelists and keeps in global_data for the time loop. */
+typedef struct pib_config {
+  int nx, my, mz;        /* nx = mx-1 unique x planes; my, mz even (initialize.F90:345-352) */
+  int precision;         /* PIB_FP64 | PIB_FP32_BULK */
+  double dx;             /* dy = dz = dx (initialize_flo.F90:1411-1413) */
+  double vis;            /* VIS */
+  double cfl_max;        /* CFL_MAX: adaptive-dt bound (dependent_functions.F90:1150) */
+  int rk3;               /* RK3: 0 -> AB2/CN Euler coef (1/2,1/2,3/2,-1/2), 1 -> 3 stages */
+  int adaptive;          /* ADAPTIVE */
+  int discr_scheme;      /* PIB_SCHEME_* (DISCR_SCHEME) */
+  int limiter;           /* PIB_LIMITER_* (LIMITER) */
+  int revernorm;         /* REVERNORM */
+  int dorpr;             /* DORPR */
+  int dobnd;             /* DOBND: calc_bnd_forcing, not implemented -> must be 0 */
+  int include_frmeanfluid;
+  int set_mpg;           /* SET_MPG: mpg held fixed */
+  int soft_start;        /* initflo sets .true. (initialize_flo.F90:126) */
+  int reserved_i[2];
+  double ucharmod;       /* |uchar| (initialize_flo.F90:278-280) */
+  double char_length;    /* dia_phys */
+  double dia;            /* phase diameter 2*radbdy(1)*dx (initialize_flo.F90:186) */
+  double dr;             /* DR: reversal-point offset in cells */
+  double voldom;         /* doml(1)*doml(2)*doml(3) */
+  double reserved_d[8];
+} pib_config;
+
+/* Scalars the reference keeps in global_data between routine calls. */
+typedef struct pib_scalars {
+  double t, dt, cf, cforig;
+  double umean[3];       /* (ky,kz)=(0,0) mean carried separately from u */
+  double mpg[3];         /* mean pressure gradient */
+  double frmean[3], frmeanfluid[3], frame_accln[3];
+  double ufmean[3], ufmean_old[3], ufmean_des[3];
+  double usmean[3], usmean_des[3], usmean_act[3];
+  double dufmeandt[3], umean_temp[3], usmean_temp[3];
+  double meanslipmod, mixmeanslipmod;
+  double maxvolfrac;     /* count_solid/(nx*my*mz), set by pib_set_particles */
+  double mean_volfrac;   /* nbody*pi*dia^3/(6*vbox) (initialize_flo.F90:188,203-207) */
+  double pres_total[3], visc_total[3], visc_total_old[3];
+  double pres_drag, visc_drag, total_drag, norm_drag, norm_drag_chem;
+  double ferror, fold, total_drag_mpg;
+  double dtnew, umax_tmp, divmax;
+  double count_solid, count_fluid;
+  double ramp_frac_time; /* initialize_flo.F90:700, used by the cf soft start */
+  double reserved[15];
+} pib_scalars;
+
+/* field ids for pib_get_field / pib_set_field */
+#define PIB_F_U 0      /* complex (nx+1,my2,mz,3) */
+#define PIB_F_P 1      /* complex (nx,my2,mz) */
+#define PIB_F_NL 2     /* complex (nx+1,my2,mz,3) */
+#define PIB_F_ONL 3    /* complex (nx+1,my2,mz,3) */
+#define PIB_F_FF 4     /* complex (nx+1,my2,mz,3) */
+#define PIB_F_UBCP 5   /* real (nx+1,my,mz,3) */
+#define PIB_F_PBCP 6   /* real (nx,my,mz) */
+#define PIB_F_NLBCP 7  /* real (nx+1,my,mz,3) */
+#define PIB_F_ONLBCP 8 /* real (nx+1,my,mz,3) */
+#define PIB_F_DIFFN 9  /* real (nx+1,my,mz,3) */
+#define PIB_F_PPR 10   /* real (nx,my,mz,3) */
+#define PIB_F_FR 11    /* real (nx+1,my,mz,3): IBM forcing before frmean removal */
+#define PIB_F_OMEGA 12 /* real (nx+1,my,mz,3): vorticity from compute_omega */
+#define PIB_F_MASK 13  /* real (nx,my,mz): 1.0 fluid, 0.0 solid (fluid_atijk) */
+
+/* ---- lifecycle -------------------------------------------------------- */
+/* Replaces alloc_mem + the grid/wavenumber/coef part of initflo
+ * (initialize_flo.F90:2438-2645, 722-763, 861-885). Wavenumbers wy, wz, w2
+ * are built from my, mz, dx as initialize_flo.F90:735-763 does. */
+int pib_create(const pib_config* cfg, void** ctx);
+int pib_destroy(void* ctx);
+const char* pib_last_error(void* ctx);
+/* Device the library runs on and its name, for logging. */
+int pib_device_info(char* name, int len, int* cu_count);
+
+/* Unit-sphere point set (quadgener, geom_init.F90:44-369): bndpts and cd are
+ * (3, nbnd) column-major; nrpr = nbnd (initialize_flo.F90:473). */
+int pib_set_points(void* ctx, int nbnd, const double* bndpts, const double* cd);
+
+/* Particle set for a fixed bed.  Computes on the device the solid mask
+ * (grid_nodes_insphere, dependent_functions.F90:783-963), the neighbour sets
+ * of the soft-sphere coarse-grid search (collision_mod.F90:807-1062, pairs with
+ * |d|_min-image < 1.2*(r_a+r_b)*dx) and the active reversal pairs
+ * (update_nrpr_array/check_external_pt, dependent_functions.F90:966-1051).
+ * radbdy in grid units; radibdy = radbdy-dr, radobdy = radbdy+dr.
+ * velbdy (nbody,3) and angv (nbody,3) may be NULL (zero).
+ * count_solid/nrpr_active outputs may be NULL. */
+int pib_set_particles(void* ctx, int nbody, const double* xc, const double* radbdy,
+                      const double* velbdy, const double* angv,
+                      long long* count_solid, int* nrpr_active);
+
+/* State transfer in the reference layout (init, checkpoint, parity dumps). */
+int pib_set_field(void* ctx, int field, const double* host);
+int pib_get_field(void* ctx, int field, double* host);
+int pib_set_scalars(void* ctx, const pib_scalars* s);
+int pib_get_scalars(void* ctx, pib_scalars* s);
+/* Per-particle force(nbody,3), pres, visc, torq (boundary_condition.F90:430-435,
+ * 1613-1634); any pointer may be NULL. */
+int pib_get_forces(void* ctx, double* force, double* pres, double* visc, double* torq);
+
+/* ---- step entry points (dtibm_main_scal_flo.F90:426-527) ------------- */
+/* compute_new_timestep (dependent_functions.F90:1054-1288) incl. calc_pressure */
+int pib_compute_new_timestep(void* ctx, int rks);
+/* nonlinear (nl.F90:63-205): form_nl, sign, nl/onl -> physical, then bcset
+ * (boundary_condition.F90:467-900) when nbody > 0 */
+int pib_nonlinear(void* ctx, int rks);
+/* velstep (steptheflow.F90:52-160): pressure, ustep, divcorr per (ky,kz) mode,
+ * check_divergence, umean update */
+int pib_velstep(void* ctx, int rks);
+/* nsteps full steps: for each, rks = 1..itrmax of the three calls above.
+ * Issues no host synchronisation inside; returns after the stream drains. */
+int pib_step(void* ctx, int nsteps);
+/* Same work enqueued on the context stream without the final drain. */
+int pib_step_async(void* ctx, int nsteps);
+int pib_sync(void* ctx);
+
+/* ---- transform layer drop-in (fftw_interface.F90:256-338) ------------ */
+/* ff2rc over nplanes planes: a(my,mz) real -> b(my2,mz) complex, forward
+ * scaled by 1/(my*mz).  Plane q of a starts at a + q*lda (doubles), plane q
+ * of b at b + 2*q*ldb (doubles).  Host pointers; the context's precision
+ * selects FP64 or FP32 arithmetic. */
+int pib_ff2rc_planes(void* ctx, const double* a, long long lda, int nplanes,
+                     double* b, long long ldb);
+/* ff2cr: b(my2,mz) complex -> a(my,mz) real, unscaled; input preserved.
+ * Im of the ky=0 and ky=my/2 entries is ignored after the z pass. */
+int pib_ff2cr_planes(void* ctx, const double* b, long long ldb, int nplanes,
+                     double* a, long long lda);
+
+/* ---- cyclic tridiagonal drop-in (tridiagonal.F90:47-150 with the
+ * Sherman-Morrison wrapper of usteptime.F90:97-227) -------------------- */
+/* nsys independent systems of length n, system s, row i at [i*nsys + s]
+ * (wavenumber-major, coalesced).  a, b, c, r real; cyclic != 0 applies the
+ * reference's Sherman-Morrison corner treatment.  x receives the solution. */
+int pib_tridiag_batched(void* ctx, int n, int nsys, const double* a, const double* b,
+                        const double* c, const double* r, double* x, int cyclic);
+
+/* ---- timers ----------------------------------------------------------- */
+/* Milliseconds spent in each phase over the last pib_step call, measured with
+ * HIP events on the context stream: [0] compute_new_timestep, [1] nonlinear
+ * (form_nl + transforms), [2] bcset, [3] velstep. */
+int pib_phase_times(void* ctx, double* ms4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PUREIBM_GPU_H */

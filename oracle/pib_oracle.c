@@ -1,0 +1,1640 @@
+/*
+ * pib_oracle.c -- TEST INFRASTRUCTURE ONLY (parity checker + CPU baseline).
+ *
+ * Serial FP64 restatement of PUReIBM-PS2D-FD1D's per-time-step fluid solve in
+ * plain C.  Each routine follows the reference routine named in its comment
+ * (paths relative to PUReIBM_PS2D-FD1D/SRC/FLO), keeping the reference's loop
+ * order and expression association so that the restatement can be read
+ * against the Fortran line by line.  Arrays are held in the reference's
+ * column-major layout with 1-based index helpers.  Build with
+ * -ffp-contract=off so that every operation rounds as written.
+ *
+ * The only product-independent choices are (a) the FFT, a recursive
+ * mixed-radix DIT DFT with directly evaluated twiddles standing in for FFTW
+ * (whose 2-D c2r uses the row-column convention: inverse c2c along z, then
+ * c2r along y ignoring Im of the ky=0 and ky=my/2 entries), and (b) the
+ * neighbour search, evaluated by brute force over the 27 periodic images
+ * instead of the coarse-grid cell list (same pair set for R_LM < cell size).
+ */
+#include "pib_oracle.h"
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct { double re, im; } cpx;
+
+/* constants.F90:92-93 are single-precision literals widened to real(8) */
+#define SMALL_NUMBER ((double)1.0E-35f)
+#define LARGE_NUMBER ((double)1.0E+32f)
+
+static const double PI = 3.14159265358979323846264338327950288;
+
+static inline cpx C(double r, double i) { cpx z; z.re = r; z.im = i; return z; }
+static inline cpx cadd(cpx a, cpx b) { return C(a.re + b.re, a.im + b.im); }
+static inline cpx csub(cpx a, cpx b) { return C(a.re - b.re, a.im - b.im); }
+static inline cpx cscal(double s, cpx a) { return C(s * a.re, s * a.im); }
+static inline cpx cdivr(cpx a, double s) { return C(a.re / s, a.im / s); }
+/* (i*k) * z, the product of a pure-imaginary wavenumber with z */
+static inline cpx cik(double k, cpx z) { return C(-k * z.im, k * z.re); }
+static inline cpx cmul(cpx a, cpx b) {
+  return C(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+
+typedef struct orc {
+  pib_config cfg;
+  int nx, my, mz, my2, mx, mxf;
+  double dx, dx2, dx3, vis;
+  double coef[3][4];
+  int itrmax;
+  double *wy, *wz, *w2; /* wy, wz hold the imaginary parts; w2(my2,mz) */
+  cpx *u, *nl, *onl, *ff; /* (nx+1, my2, mz, 3) */
+  cpx *p;                 /* (nx, my2, mz) */
+  double *ubcp, *nlbc, *onlbc, *diffn, *fr; /* (nx+1, my, mz, 3); fr doubles as omega */
+  double *omega_copy;     /* last omega, kept for parity dumps */
+  double *pbcp;           /* (nx, my, mz) */
+  double *ppr;            /* (nx, my, mz, 3) */
+  unsigned char* fluid;   /* (0:nx+1, my, mz) */
+  int nbody, nbnd;
+  double *xc, *radbdy, *radibdy, *radobdy, *velbdy, *angv;
+  double *bnd, *cd;
+  unsigned char* if_rev; /* (nbnd, nbody) */
+  int* nrpr_active;
+  double *pres, *visc, *torq, *force, *force_chem; /* (nbody, 3) */
+  long long count_solid, count_fluid;
+  pib_scalars s;
+  int have_points, have_particles;
+  /* plane work arrays */
+  double *ra, *rb;
+  cpx *ca, *cb;
+  char err[256];
+} orc;
+
+/* ---- index helpers (1-based, column-major as in the reference) -------- */
+#define IU(o, i, j, k, n) ((((size_t)((n)-1) * (o)->mz + ((k)-1)) * (o)->my2 + ((j)-1)) * ((o)->nx + 1) + ((i)-1))
+#define IP(o, i, j, k) ((((size_t)((k)-1)) * (o)->my2 + ((j)-1)) * (o)->nx + ((i)-1))
+#define IR(o, i, j, k, n) ((((size_t)((n)-1) * (o)->mz + ((k)-1)) * (o)->my + ((j)-1)) * ((o)->nx + 1) + ((i)-1))
+#define IPR(o, i, j, k) ((((size_t)((k)-1)) * (o)->my + ((j)-1)) * (o)->nx + ((i)-1))
+#define IPPR(o, i, j, k, n) ((((size_t)((n)-1) * (o)->mz + ((k)-1)) * (o)->my + ((j)-1)) * (o)->nx + ((i)-1))
+#define IFL(o, i, j, k) ((((size_t)((k)-1)) * (o)->my + ((j)-1)) * ((o)->nx + 2) + (i))
+#define XC(o, m, n) ((o)->xc[(size_t)((n)-1) * (o)->nbody + ((m)-1)])
+#define PB(o, a, m, n) ((o)->a[(size_t)((n)-1) * (o)->nbody + ((m)-1)])
+#define BND(o, n, l) ((o)->bnd[(size_t)((l)-1) * 3 + ((n)-1)])
+#define CD(o, n, l) ((o)->cd[(size_t)((l)-1) * 3 + ((n)-1)])
+
+static int fail(orc* o, int code, const char* msg) {
+  snprintf(o->err, sizeof o->err, "%s", msg);
+  return code;
+}
+
+/* ======================================================================
+ * FFT: recursive mixed-radix decimation in time.  out[k] = sum_n in[n*s]
+ * exp(sign*2*pi*i*n*k/n).  Twiddles are evaluated directly from the exact
+ * integer phase (r*k mod n) so that no error accumulates across levels.
+ * ====================================================================== */
+static int small_factor(int n) {
+  for (int p = 2; p * p <= n; ++p)
+    if (n % p == 0) return p;
+  return n;
+}
+static void dft_rec(const cpx* in, size_t s, cpx* out, int n, int sign, cpx* tmp) {
+  if (n == 1) { out[0] = in[0]; return; }
+  int p = small_factor(n);
+  int m = n / p;
+  /* sub-transforms of the p decimated sequences into tmp[r*m + k] */
+  for (int r = 0; r < p; ++r) dft_rec(in + r * s, s * p, tmp + (size_t)r * m, m, sign, out);
+  for (int q = 0; q < p; ++q) {
+    for (int k = 0; k < m; ++k) {
+      int kk = k + q * m;
+      double sr = 0.0, si = 0.0;
+      for (int r = 0; r < p; ++r) {
+        long long ph = ((long long)r * kk) % n;
+        double ang = (double)sign * 2.0 * PI * (double)ph / (double)n;
+        double c = cos(ang), sn = sin(ang);
+        cpx v = tmp[(size_t)r * m + k];
+        sr += v.re * c - v.im * sn;
+        si += v.re * sn + v.im * c;
+      }
+      out[kk] = C(sr, si);
+    }
+  }
+  /* copy back so callers that pass tmp-derived buffers keep a clean state */
+}
+static void fft_c(const cpx* in, cpx* out, int n, int sign) {
+  cpx* tmp = (cpx*)malloc(sizeof(cpx) * (size_t)n);
+  cpx* tmp2 = (cpx*)malloc(sizeof(cpx) * (size_t)n);
+  /* dft_rec uses its out argument as scratch for deeper levels */
+  memcpy(tmp2, in, sizeof(cpx) * (size_t)n);
+  dft_rec(tmp2, 1, out, n, sign, tmp);
+  free(tmp);
+  free(tmp2);
+}
+void orc_fft1d(int n, int sign, const double* in, double* out) {
+  fft_c((const cpx*)in, (cpx*)out, n, sign);
+}
+
+/* ff2rc, fftw_interface.F90:256-297: b = DFT2(a)/(my*mz), y is the halved axis */
+void orc_ff2rc(int my, int mz, const double* a, double* bout) {
+  int my2 = my / 2 + 1;
+  cpx* b = (cpx*)bout;
+  cpx* row = (cpx*)malloc(sizeof(cpx) * (size_t)(my > mz ? my : mz));
+  cpx* res = (cpx*)malloc(sizeof(cpx) * (size_t)(my > mz ? my : mz));
+  for (int k = 0; k < mz; ++k) {
+    for (int j = 0; j < my; ++j) row[j] = C(a[(size_t)k * my + j], 0.0);
+    fft_c(row, res, my, -1);
+    for (int j = 0; j < my2; ++j) b[(size_t)k * my2 + j] = res[j];
+  }
+  for (int j = 0; j < my2; ++j) {
+    for (int k = 0; k < mz; ++k) row[k] = b[(size_t)k * my2 + j];
+    fft_c(row, res, mz, -1);
+    for (int k = 0; k < mz; ++k) b[(size_t)k * my2 + j] = res[k];
+  }
+  double scale = (double)(float)(my * mz); /* float(mylo*mzlo) */
+  for (size_t q = 0; q < (size_t)my2 * mz; ++q) b[q] = cdivr(b[q], scale);
+  free(row);
+  free(res);
+}
+
+/* ff2cr, fftw_interface.F90:299-338: unscaled inverse, input preserved */
+void orc_ff2cr(int my, int mz, const double* bin, double* a) {
+  int my2 = my / 2 + 1;
+  const cpx* b = (const cpx*)bin;
+  cpx* t = (cpx*)malloc(sizeof(cpx) * (size_t)my2 * mz);
+  cpx* row = (cpx*)malloc(sizeof(cpx) * (size_t)(my > mz ? my : mz));
+  cpx* res = (cpx*)malloc(sizeof(cpx) * (size_t)(my > mz ? my : mz));
+  for (int j = 0; j < my2; ++j) {
+    for (int k = 0; k < mz; ++k) row[k] = b[(size_t)k * my2 + j];
+    fft_c(row, res, mz, +1);
+    for (int k = 0; k < mz; ++k) t[(size_t)k * my2 + j] = res[k];
+  }
+  for (int k = 0; k < mz; ++k) {
+    const cpx* tk = t + (size_t)k * my2;
+    row[0] = C(tk[0].re, 0.0);
+    row[my / 2] = C(tk[my / 2].re, 0.0);
+    for (int j = 1; j < my / 2; ++j) {
+      row[j] = tk[j];
+      row[my - j] = C(tk[j].re, -tk[j].im);
+    }
+    fft_c(row, res, my, +1);
+    for (int j = 0; j < my; ++j) a[(size_t)k * my + j] = res[j].re;
+  }
+  free(t);
+  free(row);
+  free(res);
+}
+
+/* ---- plane gather/scatter around the transforms ---------------------- */
+/* real plane i of a (nx+1 or nx, my, mz) array with leading dim ld */
+static void get_rplane(const orc* o, const double* f, int ld, int i, double* out) {
+  for (int k = 1; k <= o->mz; ++k)
+    for (int j = 1; j <= o->my; ++j)
+      out[(size_t)(k - 1) * o->my + (j - 1)] = f[((size_t)(k - 1) * o->my + (j - 1)) * ld + (i - 1)];
+}
+static void put_rplane(const orc* o, double* f, int ld, int i, const double* in) {
+  for (int k = 1; k <= o->mz; ++k)
+    for (int j = 1; j <= o->my; ++j)
+      f[((size_t)(k - 1) * o->my + (j - 1)) * ld + (i - 1)] = in[(size_t)(k - 1) * o->my + (j - 1)];
+}
+
+/* ======================================================================
+ * Thomas algorithm: tridag / tridag3, tridiagonal.F90:47-150
+ * ====================================================================== */
+int orc_tridag(const double* a, const double* b, const double* c, const double* r,
+               double* u, int n) {
+  double* gam = (double*)calloc((size_t)n + 1, sizeof(double));
+  int st = 0;
+  for (int i = 0; i < n; ++i) u[i] = 0.0;
+  if (b[0] == 0.0) st = PIB_ERR_PIVOT;
+  double bet = b[0];
+  u[0] = r[0] / bet;
+  for (int j = 1; j < n; ++j) {
+    gam[j] = c[j - 1] / bet;
+    bet = b[j] - a[j] * gam[j];
+    if (bet == 0.0) { bet = 1.0e-30; st = PIB_ERR_PIVOT; }
+    u[j] = (r[j] - a[j] * u[j - 1]) / bet;
+  }
+  for (int j = n - 2; j >= 0; --j) u[j] = u[j] - gam[j + 1] * u[j + 1];
+  free(gam);
+  return st;
+}
+static int tridag3(const double* a, const double* b, const double* c, const double* r1,
+                   const double* r2, const double* r3, double* u1, double* u2, double* u3,
+                   int n) {
+  double* gam = (double*)calloc((size_t)n + 1, sizeof(double));
+  int st = 0;
+  for (int i = 0; i < n; ++i) u1[i] = u2[i] = u3[i] = 0.0;
+  if (b[0] == 0.0) st = PIB_ERR_PIVOT;
+  double bet = b[0];
+  u1[0] = r1[0] / bet;
+  u2[0] = r2[0] / bet;
+  u3[0] = r3[0] / bet;
+  for (int j = 1; j < n; ++j) {
+    gam[j] = c[j - 1] / bet;
+    bet = b[j] - a[j] * gam[j];
+    if (bet == 0.0) { bet = 1.0e-30; st = PIB_ERR_PIVOT; }
+    u1[j] = (r1[j] - a[j] * u1[j - 1]) / bet;
+    u2[j] = (r2[j] - a[j] * u2[j - 1]) / bet;
+    u3[j] = (r3[j] - a[j] * u3[j - 1]) / bet;
+  }
+  for (int j = n - 2; j >= 0; --j) {
+    u1[j] = u1[j] - gam[j + 1] * u1[j + 1];
+    u2[j] = u2[j] - gam[j + 1] * u2[j + 1];
+    u3[j] = u3[j] - gam[j + 1] * u3[j + 1];
+  }
+  free(gam);
+  return st;
+}
+
+/* Sherman-Morrison wrapper: alpha = beta = 1, auxu = e_1 + e_n,
+ * auxv(1) = c(n)/auxu(n), auxv(n) = a(1)/auxu(1) (usteptime.F90:97-108,
+ * poisson.F90:150-161).  Complex rhs given as re/im; b is modified in place. */
+static int sm_solve(const double* a, double* b, const double* c, const double* rr,
+                    const double* ri, double* xr, double* xi, int n) {
+  double* auxu = (double*)calloc((size_t)n, sizeof(double));
+  double* auxv = (double*)calloc((size_t)n, sizeof(double));
+  double* auxz = (double*)calloc((size_t)n, sizeof(double));
+  double* yr = (double*)calloc((size_t)n, sizeof(double));
+  double* yi = (double*)calloc((size_t)n, sizeof(double));
+  auxu[0] = 1.0;
+  auxu[n - 1] = 1.0;
+  auxv[0] = c[n - 1] / auxu[n - 1];
+  auxv[n - 1] = a[0] / auxu[0];
+  b[0] = b[0] - auxu[0] * auxv[0];
+  b[n - 1] = b[n - 1] - auxu[n - 1] * auxv[n - 1];
+  int st = tridag3(a, b, c, rr, auxu, ri, yr, auxz, yi, n);
+  double vdotz = auxv[0] * auxz[0] + auxv[n - 1] * auxz[n - 1];
+  double vdotyr = auxv[0] * yr[0] + auxv[n - 1] * yr[n - 1];
+  double vdotyi = auxv[0] * yi[0] + auxv[n - 1] * yi[n - 1];
+  for (int i = 0; i < n; ++i) {
+    xr[i] = yr[i] - vdotyr * auxz[i] / (1.0 + vdotz);
+    xi[i] = yi[i] - vdotyi * auxz[i] / (1.0 + vdotz);
+  }
+  free(auxu); free(auxv); free(auxz); free(yr); free(yi);
+  return st;
+}
+int orc_cyclic_solve(const double* a, const double* b, const double* c, const double* r,
+                     double* x, int n) {
+  double* bb = (double*)malloc(sizeof(double) * (size_t)n);
+  double* zi = (double*)calloc((size_t)n, sizeof(double));
+  double* xi = (double*)calloc((size_t)n, sizeof(double));
+  memcpy(bb, b, sizeof(double) * (size_t)n);
+  int st = sm_solve(a, bb, c, r, zi, x, xi, n);
+  free(bb); free(zi); free(xi);
+  return st;
+}
+
+/* ======================================================================
+ * quadgener / generbnd / dcgen, geom_init.F90:44-369, 573-586
+ * ====================================================================== */
+int orc_quadgener(double r, double dr, double f1, double f2, int maxpts, double* bnd,
+                  double* cd) {
+  (void)dr; (void)f1;
+  double twopi = 2.0 * PI;
+  int imax = (int)lround(twopi * r / 4.0 * f2);
+  double dtheta = twopi / imax / 4.0;
+  double theta = 0.0;
+  int npt = 0;
+  int cap = 0;
+  /* first sweep: count */
+  {
+    double th = 0.0;
+    for (int i = 1; i <= imax + 1; ++i) {
+      int jmax = (int)lround(twopi * r * sin(th) / 4 * f2);
+      cap += jmax + 1;
+      th = th + dtheta;
+    }
+  }
+  double* nxb = (double*)malloc(sizeof(double) * (size_t)cap);
+  double* nyb = (double*)malloc(sizeof(double) * (size_t)cap);
+  double* nzb = (double*)malloc(sizeof(double) * (size_t)cap);
+  for (int i = 1; i <= imax + 1; ++i) {
+    int jmax = (int)lround(twopi * r * sin(theta) / 4 * f2);
+    double dphi = twopi / jmax / 4;
+    double phi = 0.0;
+    for (int j = 1; j <= jmax + 1; ++j) {
+      double z = r * cos(theta) / r;
+      if (fabs(z) < 1.0e-5) z = 0.0;
+      double y = r * sin(theta) * cos(phi) / r;
+      if (fabs(y) < 1.0e-5) y = 0.0;
+      double x = r * sin(theta) * sin(phi) / r;
+      if (fabs(x) < 1.0e-5) x = 0.0;
+      nxb[npt] = x;
+      nyb[npt] = y;
+      nzb[npt] = z;
+      ++npt;
+      phi = phi + dphi;
+    }
+    theta = theta + dtheta;
+  }
+  /* generbnd: octant then the seven mirror images in the reference order */
+  static const int sx[8] = {1, -1, 1, 1, -1, -1, 1, -1};
+  static const int sy[8] = {1, 1, -1, 1, -1, 1, -1, -1};
+  static const int sz[8] = {1, 1, 1, -1, 1, -1, -1, -1};
+  int count = 0;
+  for (int g = 0; g < 8; ++g) {
+    for (int i = 0; i < npt; ++i) {
+      int need = 1;
+      if (sx[g] < 0 && nxb[i] == 0.0) need = 0;
+      if (sy[g] < 0 && nyb[i] == 0.0) need = 0;
+      if (sz[g] < 0 && nzb[i] == 0.0) need = 0;
+      if (!need) continue;
+      double x = sx[g] * nxb[i], y = sy[g] * nyb[i], z = sz[g] * nzb[i];
+      if (count < maxpts && bnd) {
+        bnd[3 * count + 0] = x;
+        bnd[3 * count + 1] = y;
+        bnd[3 * count + 2] = z;
+        double rr = sqrt(x * x + y * y + z * z);
+        if (cd) {
+          cd[3 * count + 0] = x / rr;
+          cd[3 * count + 1] = y / rr;
+          cd[3 * count + 2] = z / rr;
+        }
+      }
+      ++count;
+    }
+  }
+  free(nxb); free(nyb); free(nzb);
+  return count;
+}
+
+/* ======================================================================
+ * context
+ * ====================================================================== */
+static void* xcalloc(size_t n, size_t sz) { return calloc(n ? n : 1, sz); }
+
+int orc_create(const pib_config* cfg, void** ctx) {
+  if (!cfg || !ctx) return PIB_ERR_ARG;
+  if (cfg->nx < 3 || cfg->my < 4 || cfg->mz < 4 || (cfg->my & 1) || (cfg->mz & 1)) return PIB_ERR_ARG;
+  orc* o = (orc*)calloc(1, sizeof(orc));
+  o->cfg = *cfg;
+  o->nx = cfg->nx; o->my = cfg->my; o->mz = cfg->mz;
+  o->my2 = cfg->my / 2 + 1;
+  o->mx = cfg->nx + 1;
+  o->mxf = o->mx;
+  o->dx = cfg->dx;
+  o->dx2 = cfg->dx * cfg->dx;
+  o->dx3 = pow(cfg->dx, 3.0);
+  o->vis = cfg->vis;
+  /* coef, initialize_flo.F90:861-885 */
+  if (cfg->rk3) {
+    double c[3][4] = {{4.0 / 15.0, 4.0 / 15.0, 8.0 / 15.0, 0.0},
+                      {1.0 / 15.0, 1.0 / 15.0, 5.0 / 12.0, -17.0 / 60.0},
+                      {1.0 / 6.0, 1.0 / 6.0, 3.0 / 4.0, -5.0 / 12.0}};
+    memcpy(o->coef, c, sizeof c);
+    o->itrmax = 3;
+  } else {
+    o->coef[0][0] = 0.5; o->coef[0][1] = 0.5; o->coef[0][2] = 3.0 / 2.0; o->coef[0][3] = -0.5;
+    o->itrmax = 1;
+  }
+  /* wavenumbers, initialize_flo.F90:735-763 (dy = dz = dx) */
+  int my = o->my, mz = o->mz, my2 = o->my2;
+  double twopi = 2.0 * PI;
+  o->wy = (double*)xcalloc((size_t)my2, sizeof(double));
+  o->wz = (double*)xcalloc((size_t)mz, sizeof(double));
+  o->w2 = (double*)xcalloc((size_t)my2 * mz, sizeof(double));
+  for (int i = 1; i <= my2; ++i) o->wy[i - 1] = twopi * (double)(i - 1) / ((double)my * o->dx);
+  for (int i = 1; i <= mz / 2; ++i) {
+    o->wz[i - 1] = twopi * (double)(i - 1) / ((double)mz * o->dx);
+    o->wz[mz + 1 - i - 1] = -twopi * (double)i / ((double)mz * o->dx);
+  }
+  for (int k = 0; k < mz; ++k)
+    for (int j = 0; j < my2; ++j) {
+      /* w2 = -dreal(wy*wy + wz*wz) with wy, wz pure imaginary */
+      double wyy = -(o->wy[j] * o->wy[j]);
+      double wzz = -(o->wz[k] * o->wz[k]);
+      o->w2[(size_t)k * my2 + j] = -(wyy + wzz);
+    }
+  size_t nc = (size_t)(o->nx + 1) * my2 * mz * 3;
+  size_t nr = (size_t)(o->nx + 1) * my * mz * 3;
+  o->u = (cpx*)xcalloc(nc, sizeof(cpx));
+  o->nl = (cpx*)xcalloc(nc, sizeof(cpx));
+  o->onl = (cpx*)xcalloc(nc, sizeof(cpx));
+  o->ff = (cpx*)xcalloc(nc, sizeof(cpx));
+  o->p = (cpx*)xcalloc((size_t)o->nx * my2 * mz, sizeof(cpx));
+  o->ubcp = (double*)xcalloc(nr, sizeof(double));
+  o->nlbc = (double*)xcalloc(nr, sizeof(double));
+  o->onlbc = (double*)xcalloc(nr, sizeof(double));
+  o->diffn = (double*)xcalloc(nr, sizeof(double));
+  o->fr = (double*)xcalloc(nr, sizeof(double));
+  o->omega_copy = (double*)xcalloc(nr, sizeof(double));
+  o->pbcp = (double*)xcalloc((size_t)o->nx * my * mz, sizeof(double));
+  o->ppr = (double*)xcalloc((size_t)o->nx * my * mz * 3, sizeof(double));
+  o->fluid = (unsigned char*)xcalloc((size_t)(o->nx + 2) * my * mz, 1);
+  memset(o->fluid, 1, (size_t)(o->nx + 2) * my * mz);
+  o->ra = (double*)xcalloc((size_t)my * mz, sizeof(double));
+  o->rb = (double*)xcalloc((size_t)my * mz, sizeof(double));
+  o->ca = (cpx*)xcalloc((size_t)my2 * mz, sizeof(cpx));
+  o->cb = (cpx*)xcalloc((size_t)my2 * mz, sizeof(cpx));
+  o->count_fluid = (long long)o->nx * my * mz;
+  *ctx = o;
+  return PIB_OK;
+}
+
+static void free_particles(orc* o) {
+  free(o->xc); free(o->radbdy); free(o->radibdy); free(o->radobdy);
+  free(o->velbdy); free(o->angv); free(o->if_rev); free(o->nrpr_active);
+  free(o->pres); free(o->visc); free(o->torq); free(o->force); free(o->force_chem);
+  o->xc = o->radbdy = o->radibdy = o->radobdy = o->velbdy = o->angv = NULL;
+  o->pres = o->visc = o->torq = o->force = o->force_chem = NULL;
+  o->if_rev = NULL;
+  o->nrpr_active = NULL;
+}
+
+int orc_destroy(void* ctx) {
+  orc* o = (orc*)ctx;
+  if (!o) return PIB_OK;
+  free(o->wy); free(o->wz); free(o->w2);
+  free(o->u); free(o->nl); free(o->onl); free(o->ff); free(o->p);
+  free(o->ubcp); free(o->nlbc); free(o->onlbc); free(o->diffn); free(o->fr); free(o->omega_copy);
+  free(o->pbcp); free(o->ppr); free(o->fluid);
+  free(o->ra); free(o->rb); free(o->ca); free(o->cb);
+  free(o->bnd); free(o->cd);
+  free_particles(o);
+  free(o);
+  return PIB_OK;
+}
+const char* orc_last_error(void* ctx) { return ctx ? ((orc*)ctx)->err : "null context"; }
+
+int orc_set_points(void* ctx, int nbnd, const double* bndpts, const double* cd) {
+  orc* o = (orc*)ctx;
+  if (!o || nbnd <= 0 || !bndpts || !cd) return PIB_ERR_ARG;
+  free(o->bnd); free(o->cd);
+  o->nbnd = nbnd;
+  o->bnd = (double*)malloc(sizeof(double) * 3 * (size_t)nbnd);
+  o->cd = (double*)malloc(sizeof(double) * 3 * (size_t)nbnd);
+  memcpy(o->bnd, bndpts, sizeof(double) * 3 * (size_t)nbnd);
+  memcpy(o->cd, cd, sizeof(double) * 3 * (size_t)nbnd);
+  o->have_points = 1;
+  return PIB_OK;
+}
+
+/* grid_nodes_insphere, dependent_functions.F90:783-963 (serial, xperiodic) */
+static void grid_nodes_insphere(orc* o) {
+  int nx = o->nx, my = o->my, mz = o->mz, mxf = o->mxf;
+  for (int k = 1; k <= mz; ++k)
+    for (int j = 1; j <= my; ++j)
+      for (int i = 0; i <= nx + 1; ++i) o->fluid[IFL(o, i, j, k)] = 1;
+  long long count_solid_tmp = 0, count_fluid_tmp = (long long)nx * my * mz;
+  for (int m = 1; m <= o->nbody; ++m) {
+    double r = o->radbdy[m - 1];
+    int imin = (int)ceil(XC(o, m, 1) - r), imax = (int)floor(XC(o, m, 1) + r);
+    int jmin = (int)ceil(XC(o, m, 2) - r), jmax = (int)floor(XC(o, m, 2) + r);
+    int kmin = (int)ceil(XC(o, m, 3) - r), kmax = (int)floor(XC(o, m, 3) + r);
+    for (int i = imin; i <= imax; ++i) {
+      int ii = i;
+      if (i < 1) ii = mxf + i - 1;
+      if (i > mxf - 1) ii = i - (mxf - 1);
+      for (int j = jmin; j <= jmax; ++j) {
+        int jj = j;
+        if (j < 1) jj = my + j;
+        if (j > my) jj = j - my;
+        for (int k = kmin; k <= kmax; ++k) {
+          int kk = k;
+          if (k < 1) kk = mz + k;
+          if (k > mz) kk = k - mz;
+          double dist = (i - XC(o, m, 1)) * (i - XC(o, m, 1));
+          dist = dist + (j - XC(o, m, 2)) * (j - XC(o, m, 2)) + (k - XC(o, m, 3)) * (k - XC(o, m, 3));
+          dist = sqrt(dist);
+          if ((dist - r) <= SMALL_NUMBER) {
+            o->fluid[IFL(o, ii, jj, kk)] = 0;
+            count_solid_tmp += 1;
+            count_fluid_tmp -= 1;
+          }
+        }
+      }
+    }
+  }
+  o->count_solid = count_solid_tmp;
+  o->count_fluid = count_fluid_tmp;
+  for (int k = 1; k <= mz; ++k)
+    for (int j = 1; j <= my; ++j) {
+      o->fluid[IFL(o, mxf, j, k)] = o->fluid[IFL(o, 1, j, k)];
+      o->fluid[IFL(o, 0, j, k)] = o->fluid[IFL(o, mxf - 1, j, k)];
+    }
+  o->s.maxvolfrac = (double)o->count_solid / (double)((long long)nx * my * mz);
+  o->s.count_solid = (double)o->count_solid;
+  o->s.count_fluid = (double)o->count_fluid;
+}
+
+/* neighbour pairs of the soft-sphere search (collision_mod.F90:807-1062):
+ * a, b neighbours iff some periodic image has 1.2*(r_a+r_b) - |d| > SMALL */
+static int are_neighbours(const orc* o, int a, int b) {
+  double dx = o->dx;
+  double L[3] = {o->nx * dx, o->my * dx, o->mz * dx};
+  double pa[3], pb[3];
+  for (int n = 1; n <= 3; ++n) {
+    pa[n - 1] = (XC(o, a, n) - 1.0) * dx;
+    pb[n - 1] = (XC(o, b, n) - 1.0) * dx;
+  }
+  double rlm = 1.2 * (o->radbdy[a - 1] * dx + o->radbdy[b - 1] * dx);
+  for (int sx = -1; sx <= 1; ++sx)
+    for (int sy = -1; sy <= 1; ++sy)
+      for (int sz = -1; sz <= 1; ++sz) {
+        double d0 = pb[0] + sx * L[0] - pa[0];
+        double d1 = pb[1] + sy * L[1] - pa[1];
+        double d2 = pb[2] + sz * L[2] - pa[2];
+        double dm = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        if (rlm - dm > SMALL_NUMBER) return 1;
+      }
+  return 0;
+}
+
+/* update_nrpr_array + check_external_pt, dependent_functions.F90:966-1051 */
+static void update_nrpr(orc* o) {
+  int nb = o->nbody, nbnd = o->nbnd;
+  unsigned char* nbr = (unsigned char*)xcalloc((size_t)nb * nb, 1);
+  for (int a = 1; a <= nb; ++a)
+    for (int b = a + 1; b <= nb; ++b)
+      if (are_neighbours(o, a, b)) nbr[(size_t)(a - 1) * nb + (b - 1)] = nbr[(size_t)(b - 1) * nb + (a - 1)] = 1;
+  for (int m = 1; m <= nb; ++m) {
+    int active = nbnd;
+    for (int l = 1; l <= nbnd; ++l) {
+      double xlo[3];
+      for (int n = 1; n <= 3; ++n) xlo[n - 1] = XC(o, m, n) + BND(o, n, l) * o->radobdy[m - 1];
+      if (xlo[0] < 1.0) xlo[0] = o->mxf + xlo[0];
+      if (xlo[0] > o->mxf) xlo[0] = xlo[0] - o->mxf;
+      if (xlo[1] < 1.0) xlo[1] = o->my + 1.0 + xlo[1];
+      if (xlo[1] > o->my) xlo[1] = xlo[1] - o->my - 1.0;
+      if (xlo[2] < 1.0) xlo[2] = o->mz + 1.0 + xlo[2];
+      if (xlo[2] > o->mz) xlo[2] = xlo[2] - o->mz - 1.0;
+      int rev = 1;
+      for (int q = 1; q <= nb && rev; ++q) {
+        if (q == m || !nbr[(size_t)(m - 1) * nb + (q - 1)]) continue;
+        double dist = 0.0;
+        for (int n = 1; n <= 3; ++n) dist = dist + (XC(o, q, n) - xlo[n - 1]) * (XC(o, q, n) - xlo[n - 1]);
+        dist = sqrt(dist);
+        if (dist < o->radbdy[q - 1]) rev = 0;
+      }
+      o->if_rev[(size_t)(m - 1) * nbnd + (l - 1)] = (unsigned char)rev;
+      if (!rev) active -= 1;
+    }
+    o->nrpr_active[m - 1] = active;
+  }
+  free(nbr);
+}
+
+int orc_set_particles(void* ctx, int nbody, const double* xc, const double* radbdy,
+                      const double* velbdy, const double* angv, long long* count_solid,
+                      int* nrpr_active) {
+  orc* o = (orc*)ctx;
+  if (!o || nbody < 0 || (nbody > 0 && (!xc || !radbdy))) return PIB_ERR_ARG;
+  if (nbody > 0 && !o->have_points) return fail(o, PIB_ERR_STATE, "set_points first");
+  free_particles(o);
+  o->nbody = nbody;
+  size_t n3 = (size_t)nbody * 3;
+  o->xc = (double*)xcalloc(n3, sizeof(double));
+  o->radbdy = (double*)xcalloc((size_t)nbody, sizeof(double));
+  o->radibdy = (double*)xcalloc((size_t)nbody, sizeof(double));
+  o->radobdy = (double*)xcalloc((size_t)nbody, sizeof(double));
+  o->velbdy = (double*)xcalloc(n3, sizeof(double));
+  o->angv = (double*)xcalloc(n3, sizeof(double));
+  o->pres = (double*)xcalloc(n3, sizeof(double));
+  o->visc = (double*)xcalloc(n3, sizeof(double));
+  o->torq = (double*)xcalloc(n3, sizeof(double));
+  o->force = (double*)xcalloc(n3, sizeof(double));
+  o->force_chem = (double*)xcalloc(n3, sizeof(double));
+  o->if_rev = (unsigned char*)xcalloc((size_t)nbody * (o->nbnd ? o->nbnd : 1), 1);
+  o->nrpr_active = (int*)xcalloc((size_t)nbody, sizeof(int));
+  if (nbody > 0) {
+    memcpy(o->xc, xc, sizeof(double) * n3);
+    memcpy(o->radbdy, radbdy, sizeof(double) * (size_t)nbody);
+    if (velbdy) memcpy(o->velbdy, velbdy, sizeof(double) * n3);
+    if (angv) memcpy(o->angv, angv, sizeof(double) * n3);
+  }
+  for (int m = 0; m < nbody; ++m) {
+    o->radibdy[m] = o->radbdy[m] - o->cfg.dr; /* initialize_flo.F90:501-505 */
+    o->radobdy[m] = o->radbdy[m] + o->cfg.dr;
+  }
+  grid_nodes_insphere(o);
+  if (nbody > 0) update_nrpr(o);
+  o->have_particles = 1;
+  if (count_solid) *count_solid = o->count_solid;
+  if (nrpr_active)
+    for (int m = 0; m < nbody; ++m) nrpr_active[m] = o->nrpr_active[m];
+  return PIB_OK;
+}
+
+int orc_get_if_rev(void* ctx, unsigned char* if_rev) {
+  orc* o = (orc*)ctx;
+  if (!o || !if_rev) return PIB_ERR_ARG;
+  memcpy(if_rev, o->if_rev, (size_t)o->nbody * o->nbnd);
+  return PIB_OK;
+}
+
+/* ---- field transfer ---------------------------------------------------- */
+static int field_info(orc* o, int field, void** ptr, size_t* ndoubles) {
+  size_t nc = (size_t)(o->nx + 1) * o->my2 * o->mz * 3;
+  size_t nr = (size_t)(o->nx + 1) * o->my * o->mz * 3;
+  switch (field) {
+    case PIB_F_U: *ptr = o->u; *ndoubles = 2 * nc; return 0;
+    case PIB_F_P: *ptr = o->p; *ndoubles = 2 * (size_t)o->nx * o->my2 * o->mz; return 0;
+    case PIB_F_NL: *ptr = o->nl; *ndoubles = 2 * nc; return 0;
+    case PIB_F_ONL: *ptr = o->onl; *ndoubles = 2 * nc; return 0;
+    case PIB_F_FF: *ptr = o->ff; *ndoubles = 2 * nc; return 0;
+    case PIB_F_UBCP: *ptr = o->ubcp; *ndoubles = nr; return 0;
+    case PIB_F_PBCP: *ptr = o->pbcp; *ndoubles = (size_t)o->nx * o->my * o->mz; return 0;
+    case PIB_F_NLBCP: *ptr = o->nlbc; *ndoubles = nr; return 0;
+    case PIB_F_ONLBCP: *ptr = o->onlbc; *ndoubles = nr; return 0;
+    case PIB_F_DIFFN: *ptr = o->diffn; *ndoubles = nr; return 0;
+    case PIB_F_PPR: *ptr = o->ppr; *ndoubles = (size_t)o->nx * o->my * o->mz * 3; return 0;
+    case PIB_F_FR: *ptr = o->fr; *ndoubles = nr; return 0;
+    case PIB_F_OMEGA: *ptr = o->omega_copy; *ndoubles = nr; return 0;
+    default: return -1;
+  }
+}
+int orc_set_field(void* ctx, int field, const double* host) {
+  orc* o = (orc*)ctx;
+  void* ptr;
+  size_t n;
+  if (!o || !host || field_info(o, field, &ptr, &n)) return PIB_ERR_ARG;
+  memcpy(ptr, host, n * sizeof(double));
+  return PIB_OK;
+}
+int orc_get_field(void* ctx, int field, double* host) {
+  orc* o = (orc*)ctx;
+  if (!o || !host) return PIB_ERR_ARG;
+  if (field == PIB_F_MASK) {
+    for (int k = 1; k <= o->mz; ++k)
+      for (int j = 1; j <= o->my; ++j)
+        for (int i = 1; i <= o->nx; ++i)
+          host[IPR(o, i, j, k)] = o->fluid[IFL(o, i, j, k)] ? 1.0 : 0.0;
+    return PIB_OK;
+  }
+  void* ptr;
+  size_t n;
+  if (field_info(o, field, &ptr, &n)) return PIB_ERR_ARG;
+  memcpy(host, ptr, n * sizeof(double));
+  return PIB_OK;
+}
+int orc_set_scalars(void* ctx, const pib_scalars* s) {
+  orc* o = (orc*)ctx;
+  if (!o || !s) return PIB_ERR_ARG;
+  double ms = o->s.maxvolfrac, cs = o->s.count_solid, cf = o->s.count_fluid;
+  o->s = *s;
+  if (o->have_particles) { o->s.maxvolfrac = ms; o->s.count_solid = cs; o->s.count_fluid = cf; }
+  return PIB_OK;
+}
+int orc_get_scalars(void* ctx, pib_scalars* s) {
+  orc* o = (orc*)ctx;
+  if (!o || !s) return PIB_ERR_ARG;
+  *s = o->s;
+  return PIB_OK;
+}
+int orc_get_forces(void* ctx, double* force, double* pres, double* visc, double* torq) {
+  orc* o = (orc*)ctx;
+  if (!o) return PIB_ERR_ARG;
+  size_t n = (size_t)o->nbody * 3 * sizeof(double);
+  if (force) memcpy(force, o->force, n);
+  if (pres) memcpy(pres, o->pres, n);
+  if (visc) memcpy(visc, o->visc, n);
+  if (torq) memcpy(torq, o->torq, n);
+  return PIB_OK;
+}
+
+/* ======================================================================
+ * compute_new_timestep + calc_pressure, dependent_functions.F90:1054-1288,
+ * 298-351 (serial, not moving, movingcv = .false.)
+ * ====================================================================== */
+int orc_compute_new_timestep(void* ctx, int rks) {
+  orc* o = (orc*)ctx;
+  if (!o || rks < 1 || rks > o->itrmax) return PIB_ERR_ARG;
+  pib_scalars* s = &o->s;
+  int nx = o->nx, my = o->my, mz = o->mz, my2 = o->my2;
+  double umax_loc = SMALL_NUMBER, vmax_loc = SMALL_NUMBER, wmax_loc = SMALL_NUMBER;
+  double ufl[3] = {0, 0, 0}, usl[3] = {0, 0, 0}, uml[3] = {0, 0, 0};
+  double mesh_vel[3] = {0, 0, 0};
+  for (int i = 1; i <= nx; ++i) {
+    for (int n = 1; n <= 3; ++n) {
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my2; ++j) o->ca[(size_t)(k - 1) * my2 + (j - 1)] = o->u[IU(o, i, j, k, n)];
+      orc_ff2cr(my, mz, (const double*)o->ca, o->ra);
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my; ++j) o->ubcp[IR(o, i, j, k, n)] = o->ra[(size_t)(k - 1) * my + (j - 1)];
+    }
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my; ++j) {
+        for (int n = 1; n <= 3; ++n) o->ubcp[IR(o, i, j, k, n)] = o->ubcp[IR(o, i, j, k, n)] + s->umean[n - 1];
+        if (o->fluid[IFL(o, i, j, k)]) {
+          for (int n = 1; n <= 3; ++n) ufl[n - 1] = ufl[n - 1] + o->ubcp[IR(o, i, j, k, n)];
+        } else {
+          for (int n = 1; n <= 3; ++n) usl[n - 1] = usl[n - 1] + o->ubcp[IR(o, i, j, k, n)];
+        }
+        for (int n = 1; n <= 3; ++n) uml[n - 1] = uml[n - 1] + o->ubcp[IR(o, i, j, k, n)];
+        umax_loc = fmax(umax_loc, fabs(o->ubcp[IR(o, i, j, k, 1)] - mesh_vel[0]));
+        vmax_loc = fmax(vmax_loc, fabs(o->ubcp[IR(o, i, j, k, 2)] - mesh_vel[1]));
+        wmax_loc = fmax(wmax_loc, fabs(o->ubcp[IR(o, i, j, k, 3)] - mesh_vel[2]));
+      }
+  }
+  for (int n = 1; n <= 3; ++n)
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my; ++j) o->ubcp[IR(o, o->mx, j, k, n)] = o->ubcp[IR(o, 1, j, k, n)];
+  double umax_tmp = fmax(fmax(umax_loc, vmax_loc), wmax_loc);
+  double dtnew;
+  if (umax_tmp * s->dt / o->dx > o->cfg.cfl_max)
+    dtnew = o->cfg.cfl_max * o->dx / umax_tmp;
+  else
+    dtnew = s->dt;
+  s->dtnew = dtnew;
+  s->umax_tmp = umax_tmp;
+  for (int n = 0; n < 3; ++n) {
+    s->ufmean_old[n] = s->ufmean[n];
+    s->ufmean[n] = ufl[n] / (double)o->count_fluid;
+    s->usmean_temp[n] = usl[n] / (double)o->count_solid;
+    s->umean_temp[n] = uml[n] / (double)((long long)nx * my * mz);
+  }
+  for (int n = 1; n <= 3; ++n) {
+    double sum = 0.0;
+    for (int m = 1; m <= o->nbody; ++m) sum += PB(o, velbdy, m, n);
+    s->usmean_act[n - 1] = o->nbody > 0 ? sum / (double)o->nbody : 0.0;
+  }
+  double meanslip[3], mixmeanslip[3];
+  for (int n = 0; n < 3; ++n) {
+    s->dufmeandt[n] = -s->cf * (s->ufmean[n] - s->ufmean_old[n]);
+    meanslip[n] = (s->ufmean[n] - s->usmean_des[n]) * (1.0 - s->maxvolfrac);
+  }
+  s->meanslipmod = sqrt(meanslip[0] * meanslip[0] + meanslip[1] * meanslip[1] + meanslip[2] * meanslip[2]);
+  /* single phase: usmean = volfrac*mean_spec_vel/mean_volfrac */
+  for (int n = 0; n < 3; ++n) {
+    double mean_spec_vel = 0.0;
+    if (o->nbody > 0) {
+      double sum = 0.0;
+      for (int m = 1; m <= o->nbody; ++m) sum += PB(o, velbdy, m, n + 1);
+      mean_spec_vel = sum / (double)o->nbody;
+    }
+    double us = 0.0 + s->mean_volfrac * mean_spec_vel;
+    s->usmean[n] = us / s->mean_volfrac;
+    mixmeanslip[n] = (1.0 - s->maxvolfrac) * (s->usmean[n] - s->ufmean[n]);
+  }
+  s->mixmeanslipmod = sqrt(mixmeanslip[0] * mixmeanslip[0] + mixmeanslip[1] * mixmeanslip[1] +
+                           mixmeanslip[2] * mixmeanslip[2]);
+  /* calc_pressure */
+  for (int i = 1; i <= nx; ++i) {
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my2; ++j) o->ca[(size_t)(k - 1) * my2 + (j - 1)] = o->p[IP(o, i, j, k)];
+    orc_ff2cr(my, mz, (const double*)o->ca, o->ra);
+    put_rplane(o, o->pbcp, nx, i, o->ra);
+  }
+  if (rks == o->itrmax && o->cfg.adaptive) s->dt = dtnew;
+  s->t = s->t + s->dt * (o->coef[rks - 1][0] + o->coef[rks - 1][1]);
+  double tempt = s->ramp_frac_time * (double)(float)(o->mx - 1) * o->dx / o->cfg.ucharmod;
+  if (o->cfg.soft_start) {
+    if (s->t < tempt) {
+      s->cf = -1.0 / s->dt;
+      s->cforig = s->cf;
+      s->cf = s->cf * (s->t / tempt);
+    } else {
+      s->cf = -1.0 / s->dt;
+      s->cforig = s->cf;
+    }
+  } else {
+    s->cf = -1.0 / s->dt;
+    s->cforig = s->cf;
+  }
+  return PIB_OK;
+}
+
+/* ======================================================================
+ * form_nl, nl_allflow.F90:71-327 (serial, mesh_vel = 0)
+ * ====================================================================== */
+static void form_nl(orc* o) {
+  int nx = o->nx, my = o->my, mz = o->mz, my2 = o->my2, mxf = o->mxf;
+  double dx = o->dx;
+  int center = o->cfg.discr_scheme == PIB_SCHEME_CENTER;
+  size_t pl = (size_t)my * mz;
+  /* fdderiv(mx, my, mz, 3) */
+  double* fd = (double*)xcalloc((size_t)(nx + 1) * pl * 3, sizeof(double));
+#define FD(i, j, k, n) fd[((((size_t)(n)-1) * mz + ((k)-1)) * my + ((j)-1)) * (nx + 1) + ((i)-1)]
+  double* pr[5];
+  cpx* pf[5];
+  for (int q = 0; q < 5; ++q) {
+    pr[q] = (double*)xcalloc(pl, sizeof(double));
+    pf[q] = (cpx*)xcalloc((size_t)my2 * mz, sizeof(cpx));
+  }
+  for (int i = 1; i <= nx; ++i) {
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my; ++j) {
+        double u1 = o->ubcp[IR(o, i, j, k, 1)], u2 = o->ubcp[IR(o, i, j, k, 2)], u3 = o->ubcp[IR(o, i, j, k, 3)];
+        size_t q = (size_t)(k - 1) * my + (j - 1);
+        pr[0][q] = u2 * u2; /* ur22 */
+        pr[1][q] = u3 * u3; /* ur33 */
+        pr[2][q] = u1 * u2; /* ur12 */
+        pr[3][q] = u1 * u3; /* ur13 */
+        pr[4][q] = u2 * u3; /* ur23 */
+        if (center) {
+          FD(i, j, k, 1) = u1 * u1;
+          FD(i, j, k, 2) = u1 * u2;
+          FD(i, j, k, 3) = u1 * u3;
+        } else {
+          FD(i, j, k, 1) = FD(i, j, k, 2) = FD(i, j, k, 3) = 0.0;
+        }
+      }
+    for (int q = 0; q < 5; ++q) orc_ff2rc(my, mz, pr[q], (double*)pf[q]);
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my2; ++j) {
+        size_t q = (size_t)(k - 1) * my2 + (j - 1);
+        double wy = o->wy[j - 1], wz = o->wz[k - 1];
+        o->nl[IU(o, i, j, k, 1)] = cadd(cik(wy, pf[2][q]), cik(wz, pf[3][q]));
+        o->nl[IU(o, i, j, k, 2)] = cadd(cik(wy, pf[0][q]), cik(wz, pf[4][q]));
+        o->nl[IU(o, i, j, k, 3)] = cadd(cik(wy, pf[4][q]), cik(wz, pf[1][q]));
+      }
+  }
+  if (center) {
+    for (int n = 1; n <= 3; ++n)
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my; ++j) FD(o->mx, j, k, n) = FD(1, j, k, n);
+  } else {
+    /* MUSCL reconstruction + Rusanov flux, nl_allflow.F90:142-294 */
+    double* uplus = (double*)xcalloc((size_t)nx + 1, sizeof(double));
+    double* uminus = (double*)xcalloc((size_t)nx + 1, sizeof(double));
+    double* Uplus = (double*)xcalloc((size_t)nx + 1, sizeof(double));
+    double* Uminus = (double*)xcalloc((size_t)nx + 1, sizeof(double));
+    double* slope = (double*)xcalloc((size_t)nx + 1, sizeof(double));
+    double slope_factor = 1.0;
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my; ++j)
+        for (int idim = 1; idim <= 3; ++idim) {
+          for (int i = 1; i <= nx; ++i) {
+            int im1 = i - 1, ip1 = i + 1;
+            if (im1 < 1) im1 = mxf + im1 - 1;
+            if (ip1 > mxf - 1) ip1 = ip1 - (mxf - 1);
+            double ui = o->ubcp[IR(o, i, j, k, idim)];
+            double uip = o->ubcp[IR(o, ip1, j, k, idim)];
+            double uim = o->ubcp[IR(o, im1, j, k, idim)];
+            double phi_ri = 1.0;
+            slope[i] = (uip - uim) / (2.0 * dx);
+            if (o->cfg.limiter == PIB_LIMITER_MINMOD) {
+              phi_ri = (ui - uim) / (uip - ui + SMALL_NUMBER);
+              if (phi_ri <= 0.0) phi_ri = 0.0;
+              else phi_ri = fmin(1.0, phi_ri);
+            }
+            slope[i] = slope[i] * phi_ri * slope_factor;
+            if (idim == 1) {
+              uplus[im1] = ui - slope[i] * (dx / 2.0);
+              uminus[i] = ui + slope[i] * (dx / 2.0);
+              Uplus[im1] = uplus[im1];
+              Uminus[i] = uminus[i];
+            } else {
+              Uplus[im1] = ui - slope[i] * (dx / 2.0);
+              Uminus[i] = ui + slope[i] * (dx / 2.0);
+            }
+            FD(im1, j, k, idim) = FD(im1, j, k, idim) + 0.5 * (uplus[im1] - 0.0) * (Uplus[im1] - 0.0);
+            FD(i, j, k, idim) = FD(i, j, k, idim) + 0.5 * (uminus[i] - 0.0) * (Uminus[i] - 0.0);
+            if (i != 1) {
+              double max_speed = fmax(fabs(uminus[im1] - 0.0), fabs(uplus[im1] - 0.0));
+              FD(im1, j, k, idim) = FD(im1, j, k, idim) - 0.5 * max_speed * (Uplus[im1] - Uminus[im1]);
+            }
+          }
+          double max_speed = fmax(fabs(uminus[nx] - 0.0), fabs(uplus[nx] - 0.0));
+          FD(nx, j, k, idim) = FD(nx, j, k, idim) - 0.5 * max_speed * (Uplus[nx] - Uminus[nx]);
+          FD(o->mx, j, k, idim) = FD(1, j, k, idim);
+        }
+    free(uplus); free(uminus); free(Uplus); free(Uminus); free(slope);
+  }
+  for (int idim = 1; idim <= 3; ++idim) {
+    for (int i = 1; i <= nx; ++i) {
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my; ++j) {
+          int im1 = i - 1, ip1 = i + 1;
+          if (im1 < 1) im1 = mxf + im1 - 1;
+          if (ip1 > mxf - 1) ip1 = ip1 - (mxf - 1);
+          double v;
+          if (center)
+            v = (FD(ip1, j, k, idim) - FD(im1, j, k, idim)) / (2.0 * dx);
+          else
+            v = (FD(i, j, k, idim) - FD(im1, j, k, idim)) / dx;
+          o->ra[(size_t)(k - 1) * my + (j - 1)] = v;
+        }
+      orc_ff2rc(my, mz, o->ra, (double*)o->cb);
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my2; ++j)
+          o->nl[IU(o, i, j, k, idim)] = cadd(o->nl[IU(o, i, j, k, idim)], o->cb[(size_t)(k - 1) * my2 + (j - 1)]);
+    }
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my2; ++j) o->nl[IU(o, nx + 1, j, k, idim)] = o->nl[IU(o, 1, j, k, idim)];
+  }
+#undef FD
+  for (int q = 0; q < 5; ++q) { free(pr[q]); free(pf[q]); }
+  free(fd);
+}
+
+/* ======================================================================
+ * bcset pieces, boundary_condition.F90
+ * ====================================================================== */
+/* calc_visc :1767-1815 */
+static void calc_visc(orc* o) {
+  int nx = o->nx, my = o->my, mz = o->mz, my2 = o->my2;
+  for (int n = 1; n <= 3; ++n)
+    for (int i = 1; i <= nx + 1; ++i) {
+      int im = (i == 1) ? nx : i - 1;
+      int ip = (i == nx + 1) ? 2 : i + 1;
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my2; ++j) {
+          cpx um = o->u[IU(o, im, j, k, n)], uc = o->u[IU(o, i, j, k, n)], up = o->u[IU(o, ip, j, k, n)];
+          cpx d2 = cadd(csub(um, cscal(2.0, uc)), up);
+          cpx f = cscal(1.0 / o->dx2, d2);
+          f = csub(f, cscal(o->w2[(size_t)(k - 1) * my2 + (j - 1)], uc));
+          f = cscal(o->vis, f);
+          o->ca[(size_t)(k - 1) * my2 + (j - 1)] = f;
+        }
+      orc_ff2cr(my, mz, (const double*)o->ca, o->ra);
+      put_rplane(o, o->diffn + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, o->ra);
+    }
+}
+
+/* compute_omega :1648-1765 (omega stored in fr) */
+static void compute_omega(orc* o) {
+  int nx = o->nx, my = o->my, mz = o->mz, my2 = o->my2, mxf = o->mxf;
+  double dx = o->dx;
+  double* om = o->fr;
+  for (int i = 1; i <= nx + 1; ++i) {
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my2; ++j)
+        o->ca[(size_t)(k - 1) * my2 + (j - 1)] = cik(o->wy[j - 1], o->u[IU(o, i, j, k, 1)]);
+    orc_ff2cr(my, mz, (const double*)o->ca, o->ra); /* dudy */
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my; ++j) {
+        double dudy = o->ra[(size_t)(k - 1) * my + (j - 1)];
+        double v;
+        if (i == 1)
+          v = (o->ubcp[IR(o, i + 1, j, k, 2)] - o->ubcp[IR(o, mxf - 1, j, k, 2)]) / (2.0 * dx) - dudy;
+        else if (i == mxf)
+          v = (o->ubcp[IR(o, 2, j, k, 2)] - o->ubcp[IR(o, mxf - 1, j, k, 2)]) / (2.0 * dx) - dudy;
+        else
+          v = (o->ubcp[IR(o, i + 1, j, k, 2)] - o->ubcp[IR(o, i - 1, j, k, 2)]) / (2.0 * dx) - dudy;
+        om[IR(o, i, j, k, 3)] = v;
+      }
+  }
+  for (int i = 1; i <= nx + 1; ++i) {
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my2; ++j)
+        o->ca[(size_t)(k - 1) * my2 + (j - 1)] = cik(o->wz[k - 1], o->u[IU(o, i, j, k, 1)]);
+    orc_ff2cr(my, mz, (const double*)o->ca, o->ra); /* dudz */
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my; ++j) {
+        double dudz = o->ra[(size_t)(k - 1) * my + (j - 1)];
+        double v;
+        if (i == 1)
+          v = dudz - (o->ubcp[IR(o, i + 1, j, k, 3)] - o->ubcp[IR(o, mxf - 1, j, k, 3)]) / (2.0 * dx);
+        else if (i == mxf)
+          v = dudz - (o->ubcp[IR(o, 2, j, k, 3)] - o->ubcp[IR(o, mxf - 1, j, k, 3)]) / (2.0 * dx);
+        else
+          v = dudz - (o->ubcp[IR(o, i + 1, j, k, 3)] - o->ubcp[IR(o, i - 1, j, k, 3)]) / (2.0 * dx);
+        om[IR(o, i, j, k, 2)] = v;
+      }
+  }
+  for (int i = 1; i <= nx + 1; ++i) {
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my2; ++j) {
+        o->ca[(size_t)(k - 1) * my2 + (j - 1)] = cik(o->wz[k - 1], o->u[IU(o, i, j, k, 2)]); /* dvdz */
+        o->cb[(size_t)(k - 1) * my2 + (j - 1)] = cik(o->wy[j - 1], o->u[IU(o, i, j, k, 3)]); /* dwdy */
+      }
+    orc_ff2cr(my, mz, (const double*)o->ca, o->ra);
+    orc_ff2cr(my, mz, (const double*)o->cb, o->rb);
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my; ++j) {
+        size_t q = (size_t)(k - 1) * my + (j - 1);
+        om[IR(o, i, j, k, 1)] = o->rb[q] - o->ra[q];
+      }
+  }
+  memcpy(o->omega_copy, o->fr, sizeof(double) * (size_t)(nx + 1) * my * mz * 3);
+}
+
+/* calc_pgrad :1817-1896 (xperiodic, nlcons) */
+static void calc_pgrad(orc* o) {
+  int nx = o->nx, my = o->my, mz = o->mz, my2 = o->my2;
+  double dx = o->dx;
+  pib_scalars* s = &o->s;
+  cpx* f3 = (cpx*)xcalloc((size_t)my2 * mz, sizeof(cpx));
+  double* r3 = (double*)xcalloc((size_t)my * mz, sizeof(double));
+  for (int i = 1; i <= nx; ++i) {
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my2; ++j) {
+        size_t q = (size_t)(k - 1) * my2 + (j - 1);
+        cpx f1;
+        if (i == 1) f1 = cdivr(csub(o->p[IP(o, i + 1, j, k)], o->p[IP(o, nx, j, k)]), 2.0 * dx);
+        else if (i == nx) f1 = cdivr(csub(o->p[IP(o, 1, j, k)], o->p[IP(o, i - 1, j, k)]), 2.0 * dx);
+        else f1 = cdivr(csub(o->p[IP(o, i + 1, j, k)], o->p[IP(o, i - 1, j, k)]), 2.0 * dx);
+        o->ca[q] = f1;
+        o->cb[q] = cik(o->wy[j - 1], o->p[IP(o, i, j, k)]);
+        f3[q] = cik(o->wz[k - 1], o->p[IP(o, i, j, k)]);
+      }
+    orc_ff2cr(my, mz, (const double*)o->ca, o->ra);
+    orc_ff2cr(my, mz, (const double*)o->cb, o->rb);
+    orc_ff2cr(my, mz, (const double*)f3, r3);
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my; ++j) {
+        size_t q = (size_t)(k - 1) * my + (j - 1);
+        o->ppr[IPPR(o, i, j, k, 1)] = o->ra[q] + s->mpg[0];
+        o->ppr[IPPR(o, i, j, k, 2)] = o->rb[q] + s->mpg[1];
+        o->ppr[IPPR(o, i, j, k, 3)] = r3[q] + s->mpg[2];
+      }
+  }
+  free(f3);
+  free(r3);
+}
+
+/* ---- trilinear ('lpi' order 2) interpolation -------------------------
+ * set_interpolation_stencil/_pstencil reduce to ib = pc for order 2 with
+ * periodic x, y, z (dependent_functions.F90:124-156, 241-273); weights
+ * w(i,j,k) = xval(i)*yval(j)*zval(k), shape2 (interpolation.F90:740-751). */
+typedef struct { int ib, jb, kb; double w[8]; /* w[(k*2+j)*2+i] */ } stencil;
+
+static void make_weights(stencil* st, const double pos[3]) {
+  double zx = pos[0] - (double)st->ib, zy = pos[1] - (double)st->jb, zz = pos[2] - (double)st->kb;
+  double xv[2] = {1.0 - zx, zx}, yv[2] = {1.0 - zy, zy}, zv[2] = {1.0 - zz, zz};
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i) st->w[(k * 2 + j) * 2 + i] = xv[i] * yv[j] * zv[k];
+}
+static inline int wrapx(const orc* o, int ii) {
+  if (ii < 1) ii = o->mxf + ii - 1;
+  if (ii > o->mxf - 1) ii = ii - o->mxf + 1;
+  return ii;
+}
+static inline int wrapy(const orc* o, int jj) {
+  if (jj < 1) jj = o->my + jj;
+  if (jj > o->my) jj = jj - o->my;
+  return jj;
+}
+static inline int wrapz(const orc* o, int kk) {
+  if (kk < 1) kk = o->mz + kk;
+  if (kk > o->mz) kk = kk - o->mz;
+  return kk;
+}
+static int cellof(double x) { return x < 0.0 ? (int)(x - 1.0) : (int)x; }
+
+/* interp_threed_scalar loop order (k, j, i) */
+static double interp_kji(const stencil* st, const double v[8]) {
+  double s = 0.0;
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i) s = s + v[(k * 2 + j) * 2 + i] * st->w[(k * 2 + j) * 2 + i];
+  return s;
+}
+/* array_dot_product_3d loop order (i, j, k) */
+static double dot_ijk(const stencil* st, const double v[8]) {
+  double s = 0.0;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k) s = s + v[(k * 2 + j) * 2 + i] * st->w[(k * 2 + j) * 2 + i];
+  return s;
+}
+
+/* interpolate_udata, dependent_functions.F90:612-708 */
+static void interpolate_udata(const orc* o, const int pc[3], const double pos[3], stencil* st,
+                              double ul[3], double nll[3], double onll[3], double dfll[3], int flag) {
+  st->ib = pc[0]; st->jb = pc[1]; st->kb = pc[2];
+  make_weights(st, pos);
+  double vs[3][8], ns[3][8], os[3][8], ds[3][8];
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i) {
+        int ii = wrapx(o, st->ib + i), jj = wrapy(o, st->jb + j), kk = wrapz(o, st->kb + k);
+        int q = (k * 2 + j) * 2 + i;
+        for (int n = 1; n <= 3; ++n) {
+          vs[n - 1][q] = o->ubcp[IR(o, ii, jj, kk, n)];
+          if (flag == 1) {
+            ns[n - 1][q] = o->nlbc[IR(o, ii, jj, kk, n)];
+            os[n - 1][q] = o->onlbc[IR(o, ii, jj, kk, n)];
+            ds[n - 1][q] = o->diffn[IR(o, ii, jj, kk, n)];
+          }
+        }
+      }
+  for (int n = 0; n < 3; ++n) ul[n] = interp_kji(st, vs[n]);
+  if (flag == 1)
+    for (int n = 0; n < 3; ++n) {
+      nll[n] = dot_ijk(st, ns[n]);
+      onll[n] = dot_ijk(st, os[n]);
+      dfll[n] = dot_ijk(st, ds[n]);
+    }
+}
+/* interpolate_pdata :710-781 */
+static void interpolate_pdata(const orc* o, const int pc[3], const double pos[3], double ppl[3],
+                              double* pl) {
+  stencil st;
+  st.ib = pc[0]; st.jb = pc[1]; st.kb = pc[2];
+  make_weights(&st, pos);
+  double gs[3][8], ps[8];
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i) {
+        int ii = wrapx(o, st.ib + i), jj = wrapy(o, st.jb + j), kk = wrapz(o, st.kb + k);
+        int q = (k * 2 + j) * 2 + i;
+        for (int n = 1; n <= 3; ++n) gs[n - 1][q] = o->ppr[IPPR(o, ii, jj, kk, n)];
+        ps[q] = o->pbcp[IPR(o, ii, jj, kk)];
+      }
+  for (int n = 0; n < 3; ++n) ppl[n] = interp_kji(&st, gs[n]);
+  *pl = dot_ijk(&st, ps);
+}
+
+/* calc_pres_visc_drag :1481-1645 */
+static void calc_pres_visc_drag(orc* o, int m, double da1) {
+  pib_scalars* s = &o->s;
+  double rb = o->radbdy[m - 1];
+  for (int l = 1; l <= o->nbnd; ++l) {
+    double xl[3], xpb[3];
+    for (int n = 1; n <= 3; ++n) xl[n - 1] = XC(o, m, n) + BND(o, n, l) * rb;
+    xpb[0] = xl[0] - 0.5;
+    xpb[1] = xl[1];
+    xpb[2] = xl[2];
+    int pcellb[3], vcellb[3];
+    for (int n = 0; n < 3; ++n) {
+      pcellb[n] = cellof(xpb[n]);
+      vcellb[n] = cellof(xl[n]);
+    }
+    double ppll[3], pl;
+    interpolate_pdata(o, pcellb, xpb, ppll, &pl);
+    stencil st;
+    double ul[3];
+    interpolate_udata(o, vcellb, xl, &st, ul, NULL, NULL, NULL, 0);
+    double vort[3] = {0, 0, 0};
+    for (int k = 0; k < 2; ++k)
+      for (int j = 0; j < 2; ++j)
+        for (int i = 0; i < 2; ++i) {
+          int ii = wrapx(o, st.ib + i), jj = wrapy(o, st.jb + j), kk = wrapz(o, st.kb + k);
+          for (int n = 1; n <= 3; ++n)
+            vort[n - 1] = vort[n - 1] + st.w[(k * 2 + j) * 2 + i] * o->fr[IR(o, ii, jj, kk, n)];
+        }
+    double df[3];
+    df[0] = vort[2] * CD(o, 2, l);
+    df[0] = df[0] - vort[1] * CD(o, 3, l);
+    df[1] = vort[0] * CD(o, 3, l);
+    df[1] = df[1] - vort[2] * CD(o, 1, l);
+    df[2] = vort[1] * CD(o, 1, l);
+    df[2] = df[2] - vort[0] * CD(o, 2, l);
+    df[0] = -df[0]; df[1] = -df[1]; df[2] = -df[2];
+    double tempforce[3];
+    for (int d = 1; d <= 3; ++d) {
+      PB(o, pres, m, d) = PB(o, pres, m, d) - pl * CD(o, d, l) * da1;
+      s->pres_total[d - 1] = s->pres_total[d - 1] + pl * CD(o, d, l) * da1;
+      PB(o, visc, m, d) = PB(o, visc, m, d) + df[d - 1] * o->vis * da1;
+      s->visc_total[d - 1] = s->visc_total[d - 1] + df[d - 1] * o->vis * da1;
+      tempforce[d - 1] = df[d - 1] * o->vis * da1;
+    }
+    double b1 = BND(o, 1, l), b2 = BND(o, 2, l), b3 = BND(o, 3, l);
+    double cr[3] = {b2 * tempforce[2] - b3 * tempforce[1], b3 * tempforce[0] - b1 * tempforce[2],
+                    b1 * tempforce[1] - b2 * tempforce[0]};
+    for (int n = 1; n <= 3; ++n) PB(o, torq, m, n) = PB(o, torq, m, n) + cr[n - 1] * rb * o->dx;
+  }
+}
+
+/* compute_mpg_at_n :302-464 (fixed particles) */
+static void compute_mpg_at_n(orc* o, int rks, double norm_factor) {
+  pib_scalars* s = &o->s;
+  compute_omega(o);
+  for (int n = 0; n < 3; ++n) {
+    s->visc_total_old[n] = s->visc_total[n];
+    s->pres_total[n] = 0.0;
+    s->visc_total[n] = 0.0;
+  }
+  for (int m = 1; m <= o->nbody; ++m) {
+    if (rks == 1)
+      for (int n = 1; n <= 3; ++n) PB(o, pres, m, n) = PB(o, visc, m, n) = PB(o, torq, m, n) = 0.0;
+    double rdx = o->radbdy[m - 1] * o->dx;
+    double da1 = 4.0 * PI * (rdx * rdx) / (double)o->nbnd;
+    calc_pres_visc_drag(o, m, da1);
+  }
+  double pres_avg[3], visc_avg[3];
+  for (int n = 1; n <= 3; ++n) {
+    double a = 0.0, b = 0.0;
+    for (int m = 1; m <= o->nbody; ++m) { a += PB(o, pres, m, n); b += PB(o, visc, m, n); }
+    pres_avg[n - 1] = a;
+    visc_avg[n - 1] = b;
+  }
+  s->pres_drag = sqrt(pres_avg[0] * pres_avg[0] + pres_avg[1] * pres_avg[1] + pres_avg[2] * pres_avg[2]);
+  s->visc_drag = sqrt(visc_avg[0] * visc_avg[0] + visc_avg[1] * visc_avg[1] + visc_avg[2] * visc_avg[2]);
+  s->pres_drag = s->pres_drag / norm_factor / (1.0 - s->maxvolfrac);
+  s->visc_drag = s->visc_drag / norm_factor / (1.0 - s->maxvolfrac);
+  s->total_drag = s->pres_drag + s->visc_drag;
+  if (!o->cfg.set_mpg) {
+    for (int n = 0; n < 3; ++n) {
+      s->mpg[n] = s->cf * (s->ufmean_des[n] - s->ufmean[n]) +
+                  (s->pres_total[n] / o->cfg.voldom - (s->visc_total[n]) / o->cfg.voldom) / (1.0 - s->maxvolfrac);
+      if (o->cfg.include_frmeanfluid) s->mpg[n] = s->mpg[n] + s->frmeanfluid[n];
+    }
+  }
+  double c12 = o->coef[rks - 1][0] + o->coef[rks - 1][1];
+  for (int m = 1; m <= o->nbody; ++m) {
+    double vol = pow(2.0 * o->radbdy[m - 1] * o->dx, 3.0);
+    for (int n = 1; n <= 3; ++n) {
+      PB(o, force, m, n) = c12 * (PB(o, pres, m, n) + PB(o, visc, m, n)) - c12 * s->mpg[n - 1] * PI * vol / 6.0;
+      PB(o, force_chem, m, n) = c12 * (PB(o, pres, m, n) + PB(o, visc, m, n));
+    }
+  }
+}
+
+/* calc_inner_reversal_forcing :1104-1479 (serial) */
+static void calc_inner_reversal_forcing(orc* o, int m, int rks, double da2) {
+  pib_scalars* s = &o->s;
+  double rb = o->radbdy[m - 1], ri = o->radibdy[m - 1], ro = o->radobdy[m - 1];
+  double drm = o->cfg.dr;
+  double c1 = o->coef[rks - 1][0], c2 = o->coef[rks - 1][1], c3 = o->coef[rks - 1][2], c4 = o->coef[rks - 1][3];
+  for (int l = 1; l <= o->nbnd; ++l) {
+    if (!o->if_rev[(size_t)(m - 1) * o->nbnd + (l - 1)]) continue;
+    double xli[3], xlo[3], rad = 0.0;
+    for (int n = 1; n <= 3; ++n) {
+      xli[n - 1] = XC(o, m, n) + BND(o, n, l) * ri;
+      xlo[n - 1] = XC(o, m, n) + BND(o, n, l) * ro;
+      rad = rad + (BND(o, n, l) * ri) * (BND(o, n, l) * ri);
+    }
+    rad = sqrt(rad);
+    double snorm[3];
+    for (int n = 1; n <= 3; ++n) snorm[n - 1] = (BND(o, n, l) * ri) / rad;
+    double xpi[3] = {xli[0] - 0.5, xli[1], xli[2]};
+    int vcello[3], pcelli[3], vcelli[3];
+    for (int n = 0; n < 3; ++n) {
+      vcello[n] = cellof(xlo[n]);
+      pcelli[n] = cellof(xpi[n]);
+      vcelli[n] = cellof(xli[n]);
+    }
+    stencil st;
+    double ulo[3], uli[3], nll[3], onll[3], dfll[3], ppll[3], pl;
+    interpolate_udata(o, vcello, xlo, &st, ulo, NULL, NULL, NULL, 0);
+    interpolate_pdata(o, pcelli, xpi, ppll, &pl);
+    interpolate_udata(o, vcelli, xli, &st, uli, nll, onll, dfll, 1);
+    double unmag = 0.0, ubnmag = 0.0;
+    double w1 = PB(o, angv, m, 1), w2v = PB(o, angv, m, 2), w3 = PB(o, angv, m, 3);
+    double crossp[3] = {w2v * snorm[2] - w3 * snorm[1], w3 * snorm[0] - w1 * snorm[2], w1 * snorm[1] - w2v * snorm[0]};
+    double linvel[3];
+    for (int n = 1; n <= 3; ++n) linvel[n - 1] = PB(o, velbdy, m, n) + crossp[n - 1] * rb * o->dx;
+    for (int d = 0; d < 3; ++d) {
+      unmag = unmag + snorm[d] * ulo[d];
+      ubnmag = ubnmag + snorm[d] * linvel[d];
+    }
+    double ucar[3];
+    for (int d = 0; d < 3; ++d) {
+      double unorm = snorm[d] * unmag;
+      double utang = ulo[d] - unorm;
+      double ubnorm = snorm[d] * ubnmag;
+      double ubtang = linvel[d] - ubnorm;
+      double utangi = ubtang * (ro - ri) - utang * (rb - ri);
+      utangi = utangi / (ro - rb);
+      double unormi;
+      if (o->cfg.revernorm == 1) {
+        unormi = ubnorm * (ro - ri) - unorm * (rb - ri);
+        unormi = unormi / (ro - rb);
+      } else {
+        unormi = 0.0;
+      }
+      ucar[d] = utangi + unormi;
+    }
+    double ft[3];
+    for (int n = 0; n < 3; ++n) {
+      ft[n] = s->cf * (-ucar[n] + uli[n]);
+      ft[n] = ft[n] - c3 * nll[n] - c4 * onll[n];
+      ft[n] = ft[n] + (c1 + c2) * (ppll[n] - dfll[n]);
+      ft[n] = ft[n] * da2 * drm * o->dx;
+    }
+    /* spreading with redistribution of the fluid-node share, :1379-1462 */
+    int count_fl = 0;
+    double force_fl[3] = {0, 0, 0};
+    for (int k = 0; k < 2; ++k) {
+      int kk = wrapz(o, st.kb + k);
+      for (int j = 0; j < 2; ++j) {
+        int jj = wrapy(o, st.jb + j);
+        for (int i = 0; i < 2; ++i) {
+          int ii = wrapx(o, st.ib + i);
+          if (o->fluid[IFL(o, ii, jj, kk)]) {
+            count_fl = count_fl + 1;
+            for (int n = 0; n < 3; ++n) {
+              double tmppa = st.w[(k * 2 + j) * 2 + i] * ft[n];
+              force_fl[n] = force_fl[n] + tmppa / o->dx3;
+            }
+          }
+        }
+      }
+    }
+    int count_so = 8 - count_fl;
+    double force_dist[3];
+    for (int n = 0; n < 3; ++n) force_dist[n] = force_fl[n] / (double)(float)count_so;
+    for (int k = 0; k < 2; ++k) {
+      int kk = wrapz(o, st.kb + k);
+      for (int j = 0; j < 2; ++j) {
+        int jj = wrapy(o, st.jb + j);
+        for (int i = 0; i < 2; ++i) {
+          int ii = wrapx(o, st.ib + i);
+          for (int n = 1; n <= 3; ++n) {
+            double tmppa = st.w[(k * 2 + j) * 2 + i] * ft[n - 1];
+            if (o->cfg.include_frmeanfluid) {
+              if (o->fluid[IFL(o, ii, jj, kk)]) s->frmeanfluid[n - 1] = s->frmeanfluid[n - 1] + tmppa / o->dx3;
+              o->fr[IR(o, ii, jj, kk, n)] = o->fr[IR(o, ii, jj, kk, n)] + tmppa / o->dx3;
+            } else {
+              if (!o->fluid[IFL(o, ii, jj, kk)])
+                o->fr[IR(o, ii, jj, kk, n)] = o->fr[IR(o, ii, jj, kk, n)] + tmppa / o->dx3 + force_dist[n - 1];
+            }
+            s->frmean[n - 1] = s->frmean[n - 1] + tmppa / o->dx3;
+          }
+        }
+      }
+    }
+  }
+}
+
+/* bcset :467-900 (not moving, serial) */
+static int bcset(orc* o, int rks) {
+  pib_scalars* s = &o->s;
+  int nx = o->nx, my = o->my, mz = o->mz, my2 = o->my2;
+  double norm_factor = (3.0 * PI * o->vis * (s->mixmeanslipmod + SMALL_NUMBER) * o->cfg.char_length) * (double)o->nbody;
+  calc_visc(o);
+  if (rks == 1)
+    for (int m = 1; m <= o->nbody; ++m)
+      for (int n = 1; n <= 3; ++n) PB(o, force, m, n) = 0.0;
+  compute_mpg_at_n(o, rks, norm_factor);
+  calc_pgrad(o);
+  for (int n = 0; n < 3; ++n) { s->frmean[n] = 0.0; s->frmeanfluid[n] = 0.0; }
+  memset(o->fr, 0, sizeof(double) * (size_t)(nx + 1) * my * mz * 3);
+  for (int m = 1; m <= o->nbody; ++m) {
+    double rdx = o->radibdy[m - 1] * o->dx;
+    double da2 = 4.0 * PI * (rdx * rdx) / (double)o->nrpr_active[m - 1];
+    if (o->cfg.dorpr) calc_inner_reversal_forcing(o, m, rks, da2);
+  }
+  for (int n = 0; n < 3; ++n) {
+    s->frmean[n] = s->frmean[n] / (double)((long long)nx * my * mz);
+    s->frmeanfluid[n] = s->frmeanfluid[n] / (double)o->count_fluid;
+  }
+  for (int n = 1; n <= 3; ++n) {
+    for (int i = 1; i <= nx; ++i) {
+      for (int j = 1; j <= my; ++j)
+        for (int k = 1; k <= mz; ++k) o->fr[IR(o, i, j, k, n)] = o->fr[IR(o, i, j, k, n)] - s->frmean[n - 1];
+      get_rplane(o, o->fr + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, o->ra);
+      orc_ff2rc(my, mz, o->ra, (double*)o->ca);
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my2; ++j) o->ff[IU(o, i, j, k, n)] = o->ca[(size_t)(k - 1) * my2 + (j - 1)];
+    }
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my2; ++j) o->ff[IU(o, nx + 1, j, k, n)] = o->ff[IU(o, 1, j, k, n)];
+  }
+  /* diagnostics, :723-884 (single phase) */
+  double avg_force[3], avg_chem[3];
+  for (int n = 1; n <= 3; ++n) {
+    double a = 0.0, b = 0.0;
+    for (int m = 1; m <= o->nbody; ++m) { a += PB(o, force, m, n); b += PB(o, force_chem, m, n); }
+    avg_force[n - 1] = a / o->nbody;
+    avg_chem[n - 1] = b / o->nbody;
+  }
+  /* avg_force = volfrac*avg_force_spec/mean_volfrac with one phase */
+  for (int n = 0; n < 3; ++n) {
+    avg_force[n] = (0.0 + s->mean_volfrac * avg_force[n]) / s->mean_volfrac;
+    avg_chem[n] = (0.0 + s->mean_volfrac * avg_chem[n]) / s->mean_volfrac;
+  }
+  double nd = sqrt(avg_force[0] * avg_force[0] + avg_force[1] * avg_force[1] + avg_force[2] * avg_force[2]);
+  double ndc = sqrt(avg_chem[0] * avg_chem[0] + avg_chem[1] * avg_chem[1] + avg_chem[2] * avg_chem[2]);
+  double mpg_avg = sqrt(s->mpg[0] * s->mpg[0] + s->mpg[1] * s->mpg[1] + s->mpg[2] * s->mpg[2]);
+  s->total_drag_mpg = mpg_avg * o->cfg.voldom / norm_factor;
+  double nf;
+  if (o->cfg.set_mpg)
+    nf = (3.0 * PI * o->vis * (s->mixmeanslipmod + SMALL_NUMBER) * o->cfg.dia);
+  else
+    nf = (3.0 * PI * o->vis * o->cfg.ucharmod * o->cfg.dia);
+  s->norm_drag = nd / nf;
+  s->norm_drag_chem = ndc / nf;
+  if (s->norm_drag > 0.0) s->ferror = fabs(s->norm_drag - s->fold) / s->norm_drag;
+  else s->ferror = 1.0;
+  s->fold = s->norm_drag;
+  if (s->norm_drag > (double)1.0E+06f) return fail(o, PIB_ERR_BLOWUP, "norm_drag > 1e6 (blow-up)");
+  return PIB_OK;
+}
+
+/* nonlinear, nl.F90:63-205 */
+int orc_nonlinear(void* ctx, int rks) {
+  orc* o = (orc*)ctx;
+  if (!o || rks < 1 || rks > o->itrmax) return PIB_ERR_ARG;
+  int nx = o->nx, my = o->my, mz = o->mz, my2 = o->my2;
+  size_t nc = (size_t)(nx + 1) * my2 * mz * 3;
+  for (size_t q = 0; q < nc; ++q) { o->onl[q] = o->nl[q]; o->nl[q] = C(0.0, 0.0); }
+  form_nl(o);
+  for (size_t q = 0; q < nc; ++q) o->nl[q] = C(-o->nl[q].re, -o->nl[q].im);
+  for (int n = 1; n <= 3; ++n)
+    for (int i = 1; i <= nx + 1; ++i) {
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my2; ++j) {
+          o->ca[(size_t)(k - 1) * my2 + (j - 1)] = o->nl[IU(o, i, j, k, n)];
+          o->cb[(size_t)(k - 1) * my2 + (j - 1)] = o->onl[IU(o, i, j, k, n)];
+        }
+      orc_ff2cr(my, mz, (const double*)o->ca, o->ra);
+      orc_ff2cr(my, mz, (const double*)o->cb, o->rb);
+      put_rplane(o, o->nlbc + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, o->ra);
+      put_rplane(o, o->onlbc + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, o->rb);
+    }
+  if (o->nbody > 0) return bcset(o, rks);
+  return PIB_OK;
+}
+
+/* ======================================================================
+ * velstep: pressure / ustep / divcorr, poisson.F90:55-508,
+ * usteptime.F90:45-258, steptheflow.F90:52-240
+ * ====================================================================== */
+static int pressure(orc* o, int rks, int j, int k) {
+  int nx = o->nx;
+  double dx = o->dx, dx2 = o->dx2;
+  double w2 = o->w2[(size_t)(k - 1) * o->my2 + (j - 1)], wy = o->wy[j - 1], wz = o->wz[k - 1];
+  double c1 = o->coef[rks - 1][0], c2 = o->coef[rks - 1][1], c3 = o->coef[rks - 1][2], c4 = o->coef[rks - 1][3];
+  double *a = (double*)malloc(sizeof(double) * nx), *b = (double*)malloc(sizeof(double) * nx),
+         *c = (double*)malloc(sizeof(double) * nx), *dv = (double*)calloc(nx, sizeof(double)),
+         *dvi = (double*)calloc(nx, sizeof(double)), *dvt = (double*)calloc(nx, sizeof(double)),
+         *dvti = (double*)calloc(nx, sizeof(double));
+  for (int i = 0; i < nx; ++i) {
+    a[i] = 1.0 / dx2 - 0.25 * w2;
+    b[i] = -2.0 / dx2 - 0.5 * w2;
+    c[i] = 1.0 / dx2 - 0.25 * w2;
+  }
+  for (int i = 1; i <= nx; ++i) {
+    cpx tmpc = C(0.0, 0.0);
+    cpx nl1 = csub(o->nl[IU(o, i + 1, j, k, 1)], o->nl[IU(o, i, j, k, 1)]);
+    cpx ff1 = csub(o->ff[IU(o, i + 1, j, k, 1)], o->ff[IU(o, i, j, k, 1)]);
+    tmpc = cadd(cadd(tmpc, cdivr(cscal(c3, nl1), dx)), cdivr(ff1, dx));
+    cpx nl2 = cadd(o->nl[IU(o, i + 1, j, k, 2)], o->nl[IU(o, i, j, k, 2)]);
+    cpx ff2 = cadd(o->ff[IU(o, i + 1, j, k, 2)], o->ff[IU(o, i, j, k, 2)]);
+    tmpc = cadd(cadd(tmpc, cdivr(cik(c3 * wy, nl2), 2.0)), cdivr(cik(wy, ff2), 2.0));
+    cpx nl3 = cadd(o->nl[IU(o, i + 1, j, k, 3)], o->nl[IU(o, i, j, k, 3)]);
+    cpx ff3 = cadd(o->ff[IU(o, i + 1, j, k, 3)], o->ff[IU(o, i, j, k, 3)]);
+    tmpc = cadd(cadd(tmpc, cdivr(cik(c3 * wz, nl3), 2.0)), cdivr(cik(wz, ff3), 2.0));
+    cpx on1 = csub(o->onl[IU(o, i + 1, j, k, 1)], o->onl[IU(o, i, j, k, 1)]);
+    tmpc = cadd(tmpc, cdivr(cscal(c4, on1), dx));
+    cpx on2 = cadd(o->onl[IU(o, i + 1, j, k, 2)], o->onl[IU(o, i, j, k, 2)]);
+    tmpc = cadd(tmpc, cdivr(cik(c4 * wy, on2), 2.0));
+    cpx on3 = cadd(o->onl[IU(o, i + 1, j, k, 3)], o->onl[IU(o, i, j, k, 3)]);
+    tmpc = cadd(tmpc, cdivr(cik(c4 * wz, on3), 2.0));
+    tmpc = cscal(0.5 * (1.0 + 1.0), tmpc);
+    dv[i - 1] = tmpc.re / (c1 + c2);
+    dvi[i - 1] = tmpc.im / (c1 + c2);
+  }
+  int st = 0;
+  if (j == 1 && k == 1) {
+    st = orc_tridag(a, b, c, dv, dvt, nx - 1);
+    dvt[nx - 1] = 0.0;
+    for (int i = 0; i < nx; ++i) dvti[i] = 0.0;
+  } else {
+    st = sm_solve(a, b, c, dv, dvi, dvt, dvti, nx);
+  }
+  for (int i = 1; i <= nx; ++i) {
+    cpx tmpp = C(dvt[i - 1], dvti[i - 1]);
+    cpx pold = o->p[IP(o, i, j, k)];
+    o->p[IP(o, i, j, k)] = cadd(pold, cscal(1.0, csub(tmpp, pold)));
+  }
+  free(a); free(b); free(c); free(dv); free(dvi); free(dvt); free(dvti);
+  return st;
+}
+
+static int ustep(orc* o, int rks, int j, int k) {
+  int nx = o->nx;
+  double dx = o->dx, dx2 = o->dx2, dt = o->s.dt, vis = o->vis;
+  double w2 = o->w2[(size_t)(k - 1) * o->my2 + (j - 1)], wy = o->wy[j - 1], wz = o->wz[k - 1];
+  double c1 = o->coef[rks - 1][0], c2 = o->coef[rks - 1][1], c3 = o->coef[rks - 1][2], c4 = o->coef[rks - 1][3];
+  double g = 1.0;
+  double *a = (double*)malloc(sizeof(double) * nx), *b = (double*)malloc(sizeof(double) * nx),
+         *c = (double*)malloc(sizeof(double) * nx), *bb = (double*)malloc(sizeof(double) * nx);
+  double *tr = (double*)malloc(sizeof(double) * nx), *ti = (double*)malloc(sizeof(double) * nx);
+  double *xr = (double*)malloc(sizeof(double) * nx), *xi = (double*)malloc(sizeof(double) * nx);
+  cpx* r = (cpx*)malloc(sizeof(cpx) * nx);
+  int st = 0;
+  for (int i = 0; i < nx; ++i) {
+    a[i] = (-g * dt * vis * c2 / dx2);
+    b[i] = (1.0 + dt * vis * c2 * (g * 2.0 / dx2 + w2));
+    c[i] = (-g * dt * vis * c2 / dx2);
+  }
+  for (int idim = 1; idim <= 3; ++idim) {
+    for (int i = 1; i <= nx; ++i) {
+      cpx uu = o->u[IU(o, i, j, k, idim)];
+      cpx ri = cscal(1.0 - dt * vis * c1 * w2, uu);
+      ri = cadd(cadd(ri, cscal(dt * c3, o->nl[IU(o, i, j, k, idim)])), cscal(dt, o->ff[IU(o, i, j, k, idim)]));
+      ri = cadd(ri, cscal(dt * c4, o->onl[IU(o, i, j, k, idim)]));
+      r[i - 1] = ri;
+    }
+    double c12 = c1 + c2;
+    for (int i = 2; i <= nx; ++i) {
+      cpx pi_ = o->p[IP(o, i, j, k)], pm = o->p[IP(o, i - 1, j, k)];
+      if (idim == 1)
+        r[i - 1] = csub(r[i - 1], cdivr(cscal(dt * c12, csub(pi_, pm)), dx));
+      else if (idim == 2)
+        r[i - 1] = csub(r[i - 1], cik(wy, cscal(dt * c12 * 0.5, cadd(pm, pi_))));
+      else
+        r[i - 1] = csub(r[i - 1], cik(wz, cscal(dt * c12 * 0.5, cadd(pm, pi_))));
+      cpx um = o->u[IU(o, i - 1, j, k, idim)], up = o->u[IU(o, i + 1, j, k, idim)], uc = o->u[IU(o, i, j, k, idim)];
+      cpx lap = csub(cadd(um, up), cscal(2.0, uc));
+      r[i - 1] = cadd(r[i - 1], cdivr(cscal(g * dt * vis * c1, lap), dx2));
+    }
+    {
+      cpx p1 = o->p[IP(o, 1, j, k)], pn = o->p[IP(o, nx, j, k)];
+      if (idim == 1)
+        r[0] = csub(r[0], cdivr(cscal(dt * c12, csub(p1, pn)), dx));
+      else if (idim == 2)
+        r[0] = csub(r[0], cik(wy, cscal(dt * c12 * 0.5, cadd(pn, p1))));
+      else
+        r[0] = csub(r[0], cik(wz, cscal(dt * c12 * 0.5, cadd(pn, p1))));
+      cpx u2 = o->u[IU(o, 2, j, k, idim)], u1 = o->u[IU(o, 1, j, k, idim)], un = o->u[IU(o, nx, j, k, idim)];
+      cpx lap = cadd(csub(u2, cscal(2.0, u1)), un);
+      r[0] = cadd(r[0], cdivr(cscal(g * dt * vis * c1, lap), dx2));
+    }
+    for (int i = 0; i < nx; ++i) { tr[i] = r[i].re; ti[i] = r[i].im; bb[i] = b[i]; }
+    int s2 = sm_solve(a, bb, c, tr, ti, xr, xi, nx);
+    if (s2) st = s2;
+    for (int i = 1; i <= nx; ++i) o->u[IU(o, i, j, k, idim)] = C(xr[i - 1], xi[i - 1]);
+    o->u[IU(o, nx + 1, j, k, idim)] = o->u[IU(o, 1, j, k, idim)];
+  }
+  free(a); free(b); free(c); free(bb); free(tr); free(ti); free(xr); free(xi); free(r);
+  return st;
+}
+
+static int divcorr(orc* o, int rks, int j, int k) {
+  int nx = o->nx, mx1 = o->nx;
+  double dx = o->dx, dx2 = o->dx2, dt = o->s.dt, vis = o->vis;
+  double w2 = o->w2[(size_t)(k - 1) * o->my2 + (j - 1)], wy = o->wy[j - 1], wz = o->wz[k - 1];
+  double c1 = o->coef[rks - 1][0], c2 = o->coef[rks - 1][1];
+  double alp = ((c1 + c2) * dt);
+  double *a = (double*)malloc(sizeof(double) * nx), *b = (double*)malloc(sizeof(double) * nx),
+         *c = (double*)malloc(sizeof(double) * nx), *dv = (double*)calloc(nx, sizeof(double)),
+         *dvi = (double*)calloc(nx, sizeof(double)), *dv1 = (double*)calloc(nx, sizeof(double)),
+         *dv1i = (double*)calloc(nx, sizeof(double));
+  cpx* div = (cpx*)malloc(sizeof(cpx) * (nx + 1));
+  for (int i = 0; i < nx; ++i) {
+    a[i] = 1.0 / dx2 - 0.25 * w2;
+    b[i] = -2.0 / dx2 - 0.5 * w2;
+    c[i] = 1.0 / dx2 - 0.25 * w2;
+  }
+  for (int i = 1; i <= nx; ++i) {
+    cpx t = cdivr(csub(o->u[IU(o, i + 1, j, k, 1)], o->u[IU(o, i, j, k, 1)]), dx);
+    t = cadd(t, cscal(0.5, cik(wy, cadd(o->u[IU(o, i, j, k, 2)], o->u[IU(o, i + 1, j, k, 2)]))));
+    t = cadd(t, cscal(0.5, cik(wz, cadd(o->u[IU(o, i, j, k, 3)], o->u[IU(o, i + 1, j, k, 3)]))));
+    dv[i - 1] = 0.5 * (1.0 + 1.0) * t.re;
+    dvi[i - 1] = 0.5 * (1.0 + 1.0) * t.im;
+    dv[i - 1] = dv[i - 1] / alp;
+    dvi[i - 1] = dvi[i - 1] / alp;
+  }
+  int st;
+  if (j == 1 && k == 1) {
+    st = orc_tridag(a, b, c, dv, dv1, nx - 1);
+    dv1[nx - 1] = 0.0;
+    for (int i = 0; i < nx; ++i) dv1i[i] = 0.0;
+  } else {
+    st = sm_solve(a, b, c, dv, dvi, dv1, dv1i, nx);
+  }
+  for (int i = 1; i <= nx; ++i) div[i] = C(dv1[i - 1], dv1i[i - 1]);
+  /* u(t+dt) = u* - grad(phi) */
+  {
+    cpx d1 = div[1], dn = div[mx1];
+    o->u[IU(o, 1, j, k, 1)] = csub(o->u[IU(o, 1, j, k, 1)], cdivr(cscal(alp, csub(d1, dn)), dx));
+    o->u[IU(o, 1, j, k, 2)] = csub(o->u[IU(o, 1, j, k, 2)], cscal(0.5, cik(alp * wy, cadd(d1, dn))));
+    o->u[IU(o, 1, j, k, 3)] = csub(o->u[IU(o, 1, j, k, 3)], cscal(0.5, cik(alp * wz, cadd(d1, dn))));
+  }
+  for (int i = 2; i <= nx; ++i) {
+    o->u[IU(o, i, j, k, 1)] = csub(o->u[IU(o, i, j, k, 1)], cdivr(cscal(alp, csub(div[i], div[i - 1])), dx));
+    o->u[IU(o, i, j, k, 2)] = csub(o->u[IU(o, i, j, k, 2)], cik(alp * 0.5 * wy, cadd(div[i - 1], div[i])));
+    o->u[IU(o, i, j, k, 3)] = csub(o->u[IU(o, i, j, k, 3)], cik(alp * 0.5 * wz, cadd(div[i - 1], div[i])));
+  }
+  for (int n = 1; n <= 3; ++n) o->u[IU(o, nx + 1, j, k, n)] = o->u[IU(o, 1, j, k, n)];
+  double cv = (c2 * dt) * vis;
+  for (int i = 1; i <= nx; ++i) {
+    cpx dm = (i == 1) ? div[mx1] : div[i - 1];
+    cpx dp = (i == nx) ? div[1] : div[i + 1];
+    cpx lap = cadd(csub(dm, cscal(2.0, div[i])), dp);
+    cpx corr = csub(cdivr(lap, dx2), cscal(w2, div[i]));
+    o->p[IP(o, i, j, k)] = csub(cadd(o->p[IP(o, i, j, k)], div[i]), cscal(cv, corr));
+  }
+  free(a); free(b); free(c); free(dv); free(dvi); free(dv1); free(dv1i); free(div);
+  return st;
+}
+
+int orc_velstep(void* ctx, int rks) {
+  orc* o = (orc*)ctx;
+  if (!o || rks < 1 || rks > o->itrmax) return PIB_ERR_ARG;
+  int st = 0, s2;
+  for (int j = 1; j <= o->my2; ++j)
+    for (int k = 1; k <= o->mz; ++k)
+      if ((s2 = pressure(o, rks, j, k))) st = s2;
+  for (int j = 1; j <= o->my2; ++j)
+    for (int k = 1; k <= o->mz; ++k)
+      if ((s2 = ustep(o, rks, j, k))) st = s2;
+  for (int j = 1; j <= o->my2; ++j)
+    for (int k = 1; k <= o->mz; ++k)
+      if ((s2 = divcorr(o, rks, j, k))) st = s2;
+  /* check_divergence, steptheflow.F90:219-240 */
+  double divmax = 0.0;
+  for (int k = 1; k <= o->mz; ++k)
+    for (int j = 1; j <= o->my2; ++j)
+      for (int i = 1; i <= o->nx; ++i) {
+        cpx t = cdivr(csub(o->u[IU(o, i + 1, j, k, 1)], o->u[IU(o, i, j, k, 1)]), o->dx);
+        t = cadd(t, cscal(0.5, cik(o->wy[j - 1], cadd(o->u[IU(o, i, j, k, 2)], o->u[IU(o, i + 1, j, k, 2)]))));
+        t = cadd(t, cscal(0.5, cik(o->wz[k - 1], cadd(o->u[IU(o, i, j, k, 3)], o->u[IU(o, i + 1, j, k, 3)]))));
+        double m = sqrt(t.re * t.re + t.im * t.im);
+        if (m > divmax) divmax = m;
+      }
+  o->s.divmax = divmax;
+  for (int n = 0; n < 3; ++n)
+    o->s.umean[n] = o->s.umean[n] + (-o->s.mpg[n] + o->s.frmean[n] - o->s.frame_accln[n]) * o->s.dt;
+  if (st) return fail(o, st, "zero pivot in tridiagonal solve");
+  return PIB_OK;
+}
+
+int orc_step(void* ctx, int nsteps) {
+  orc* o = (orc*)ctx;
+  if (!o || nsteps < 0) return PIB_ERR_ARG;
+  for (int n = 0; n < nsteps; ++n)
+    for (int rks = 1; rks <= o->itrmax; ++rks) {
+      int st;
+      if ((st = orc_compute_new_timestep(o, rks))) return st;
+      if ((st = orc_nonlinear(o, rks))) return st;
+      if ((st = orc_velstep(o, rks))) return st;
+    }
+  return PIB_OK;
+}

@@ -1,0 +1,137 @@
+"""Input decks for the fixed-bed workloads, written in the reference's formats.
+
+* ``<RUN>_floparam.in`` -- Fortran namelists FLOANDGRID / PART_PROPT / FLAGS
+  (+ FILES, PSD, GCG) as read by read_nmls (initialize.F90:274-359).
+* ``<RUN>_sphr_center.inp`` -- one header line, then ``x y z r`` per sphere in
+  diameters, read by generate_configuration 'default'
+  (initialize_flo.F90:1485-1605): xc = x*DBYDX.
+
+Random assemblies come from this project's own seeded random sequential
+addition (RSA): centres uniform in [0, LYBYD)^3, accepted when the periodic
+minimum-image distance to every accepted centre is at least ``min_gap``
+diameters.  The reference builds random beds differently (lattice + DEM
+shaking, gener_config_random); RSA is only a source of reproducible inputs.
+"""
+import math
+import os
+
+import numpy as np
+
+
+def rsa_centers(nbody, lybyd, seed, min_gap=1.0, max_tries=2_000_000):
+    """Seeded RSA of ``nbody`` unit-diameter spheres in a periodic cube."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pts = np.zeros((0, 3))
+    tries = 0
+    while len(pts) < nbody:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError(f"RSA jammed after {len(pts)} of {nbody} spheres")
+        c = rng.uniform(0.0, lybyd, size=3)
+        if len(pts):
+            d = pts - c
+            d -= lybyd * np.round(d / lybyd)
+            if np.min(np.einsum("ij,ij->i", d, d)) < min_gap * min_gap:
+                continue
+        pts = np.vstack([pts, c])
+    return pts
+
+
+def _fmt(v):
+    if isinstance(v, bool):
+        return "T" if v else "F"
+    if isinstance(v, str):
+        return "'" + v + "'"
+    if isinstance(v, (list, tuple)):
+        return ", ".join(_fmt(x) for x in v)
+    if isinstance(v, float):
+        return repr(v).replace("e", "d") if "e" in repr(v) else repr(v) + "d0"
+    return str(v)
+
+
+def write_namelists(path, groups):
+    with open(path, "w") as f:
+        for gname, entries in groups.items():
+            f.write(f" &{gname}\n")
+            for k, v in entries.items():
+                f.write(f" {k} = {_fmt(v)},\n")
+            f.write(" /\n")
+
+
+def write_case(directory, run_name, *, my, lybyd, dbydx, centers=None, input_type=None,
+               vol_frac1=None, re=20.0, vis=0.012, cfl=0.1, cfl_max=0.5, adaptive=True,
+               discr_scheme="center", limiter="none", rk3=False, ramp_frac_steps=0,
+               set_mpg=False, mpg=(0.0, 0.0, 0.0), revernorm=1, dr=1.0, f2=1.6,
+               include_frmeanfluid=False, dorpr=True):
+    """Write ``<directory>/<run_name>_floparam.in`` (+ sphr_center file)."""
+    os.makedirs(directory, exist_ok=True)
+    base = os.path.join(directory, run_name)
+    if input_type is None:
+        input_type = "default" if centers is not None else "simple"
+    part = {
+        "NBODY": int(len(centers)) if centers is not None else 1,
+        "DR": float(dr), "F1": 1.6, "F2": float(f2), "LYBYD": float(lybyd),
+        "DBYDX": float(dbydx), "DIA_PHYS": 1.0, "RHOS": 1.0,
+        "INPUT_TYPE": input_type, "DISCR_SCHEME": discr_scheme, "LIMITER": limiter,
+        "COLLISION_TYPE": "none",
+    }
+    if vol_frac1 is not None:
+        part["VOL_FRAC1"] = float(vol_frac1)
+    groups = {
+        "FLOANDGRID": {
+            "MY": int(my), "TEND": 1.0, "NERR_STEPS": 100, "RAMP_FRAC_STEPS": int(ramp_frac_steps),
+            "VIS": float(vis), "RE": float(re), "RHOF": 1.0, "CFL": float(cfl),
+            "CFL_MAX": float(cfl_max), "MPG": [float(x) for x in mpg],
+            "FLO_ANG": [0.0, 90.0, 90.0],
+        },
+        "PART_PROPT": part,
+        "FLAGS": {
+            "IMOVE": 0, "ISCALON": 0, "IRESTART": 0, "ADAPTIVE": bool(adaptive),
+            "RK3": bool(rk3), "DORPR": bool(dorpr), "DOBND": False,
+            "INCLUDE_FRMEANFLUID": bool(include_frmeanfluid), "SET_MPG": bool(set_mpg),
+            "REVERNORM": int(revernorm),
+        },
+        "FILES": {"SAVEITNS": 10, "WRITE_OUTPUT": False},
+        "PSD": {"PSD_TYPE": "discrete", "NPHASES": 1},
+        "GCG": {"COEFF_REST": 1.0},
+    }
+    write_namelists(base + "_floparam.in", groups)
+    if centers is not None:
+        with open(base + "_sphr_center.inp", "w") as f:
+            f.write(" x y z r (diameters)\n")
+            for c in np.asarray(centers):
+                f.write(f" {c[0]:.15f} {c[1]:.15f} {c[2]:.15f} 0.5\n")
+    return base
+
+
+def simple_cubic(directory, run_name, *, my, dbydx, **kw):
+    """One sphere of DBYDX cells centred in a my^3 box ('simple')."""
+    lybyd = my / dbydx
+    vol_frac1 = math.pi / (6.0 * lybyd ** 3)
+    return write_case(directory, run_name, my=my, lybyd=lybyd, dbydx=dbydx,
+                      input_type="simple", vol_frac1=vol_frac1, **kw)
+
+
+def random_bed(directory, run_name, *, my, dbydx, nbody, seed, **kw):
+    """``nbody`` spheres of DBYDX cells placed by seeded RSA ('default')."""
+    lybyd = my / dbydx
+    centers = rsa_centers(nbody, lybyd, seed)
+    return write_case(directory, run_name, my=my, lybyd=lybyd, dbydx=dbydx,
+                      centers=centers, **kw)
+
+
+# Workloads of the benchmark table (box, spheres, Re); C2 is the headline.
+WORKLOADS = {
+    "C1": dict(kind="simple", my=64, dbydx=16, re=20.0),
+    "C2": dict(kind="random", my=128, dbydx=16, nbody=195, seed=20261202, re=50.0),
+    "C4": dict(kind="random", my=128, dbydx=16, nbody=50, seed=20261204, re=20.0),
+}
+
+
+def write_workload(directory, name, **over):
+    w = dict(WORKLOADS[name])
+    w.update(over)
+    kind = w.pop("kind")
+    if kind == "simple":
+        return simple_cubic(directory, name, **w)
+    return random_bed(directory, name, **w)

@@ -1,0 +1,139 @@
+! pib_types.f90 -- Fortran mirrors (bind(C)) of the C-ABI structs declared in
+! include/pureibm_gpu.h, plus the interfaces to the C-ABI entry points.
+! Field order and kinds must stay identical to the C declarations.
+module pib_types
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: PIB_OK = 0, PIB_ERR_ARG = -1, PIB_ERR_UNSUPPORTED = -2, &
+       PIB_ERR_HIP = -3, PIB_ERR_STATE = -4, PIB_ERR_BLOWUP = -5, PIB_ERR_PIVOT = -6
+  integer(c_int), parameter :: PIB_FP64 = 0, PIB_FP32_BULK = 1
+  integer(c_int), parameter :: PIB_SCHEME_CENTER = 0, PIB_SCHEME_UPWIND = 1
+  integer(c_int), parameter :: PIB_LIMITER_NONE = 0, PIB_LIMITER_MINMOD = 1
+  integer(c_int), parameter :: PIB_F_U = 0, PIB_F_P = 1
+
+  type, bind(C) :: pib_config
+     integer(c_int) :: nx, my, mz, precision
+     real(c_double) :: dx, vis, cfl_max
+     integer(c_int) :: rk3, adaptive, discr_scheme, limiter
+     integer(c_int) :: revernorm, dorpr, dobnd, include_frmeanfluid
+     integer(c_int) :: set_mpg, soft_start, reserved_i(2)
+     real(c_double) :: ucharmod, char_length, dia, dr, voldom
+     real(c_double) :: reserved_d(8)
+  end type pib_config
+
+  type, bind(C) :: pib_scalars
+     real(c_double) :: t, dt, cf, cforig
+     real(c_double) :: umean(3), mpg(3), frmean(3), frmeanfluid(3), frame_accln(3)
+     real(c_double) :: ufmean(3), ufmean_old(3), ufmean_des(3)
+     real(c_double) :: usmean(3), usmean_des(3), usmean_act(3)
+     real(c_double) :: dufmeandt(3), umean_temp(3), usmean_temp(3)
+     real(c_double) :: meanslipmod, mixmeanslipmod
+     real(c_double) :: maxvolfrac, mean_volfrac
+     real(c_double) :: pres_total(3), visc_total(3), visc_total_old(3)
+     real(c_double) :: pres_drag, visc_drag, total_drag, norm_drag, norm_drag_chem
+     real(c_double) :: ferror, fold, total_drag_mpg
+     real(c_double) :: dtnew, umax_tmp, divmax
+     real(c_double) :: count_solid, count_fluid
+     real(c_double) :: ramp_frac_time
+     real(c_double) :: reserved(15)
+  end type pib_scalars
+
+  interface
+     function pib_create(cfg, ctx) bind(C, name="pib_create") result(st)
+       import :: c_int, c_ptr, pib_config
+       type(pib_config), intent(in) :: cfg
+       type(c_ptr), intent(out) :: ctx
+       integer(c_int) :: st
+     end function pib_create
+     function pib_destroy(ctx) bind(C, name="pib_destroy") result(st)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int) :: st
+     end function pib_destroy
+     function pib_last_error(ctx) bind(C, name="pib_last_error") result(msg)
+       import :: c_ptr
+       type(c_ptr), value :: ctx
+       type(c_ptr) :: msg
+     end function pib_last_error
+     function pib_set_points(ctx, nbnd, bndpts, cd) bind(C, name="pib_set_points") result(st)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nbnd
+       real(c_double), intent(in) :: bndpts(*), cd(*)
+       integer(c_int) :: st
+     end function pib_set_points
+     function pib_set_particles(ctx, nbody, xc, radbdy, velbdy, angv, count_solid, nrpr_active) &
+          bind(C, name="pib_set_particles") result(st)
+       import :: c_int, c_ptr, c_double, c_long_long
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nbody
+       real(c_double), intent(in) :: xc(*), radbdy(*)
+       type(c_ptr), value :: velbdy, angv
+       integer(c_long_long), intent(out) :: count_solid
+       integer(c_int), intent(out) :: nrpr_active(*)
+       integer(c_int) :: st
+     end function pib_set_particles
+     function pib_set_scalars(ctx, s) bind(C, name="pib_set_scalars") result(st)
+       import :: c_int, c_ptr, pib_scalars
+       type(c_ptr), value :: ctx
+       type(pib_scalars), intent(in) :: s
+       integer(c_int) :: st
+     end function pib_set_scalars
+     function pib_get_scalars(ctx, s) bind(C, name="pib_get_scalars") result(st)
+       import :: c_int, c_ptr, pib_scalars
+       type(c_ptr), value :: ctx
+       type(pib_scalars), intent(out) :: s
+       integer(c_int) :: st
+     end function pib_get_scalars
+     function pib_get_forces(ctx, force, pres, visc, torq) bind(C, name="pib_get_forces") result(st)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: force(*), pres(*), visc(*), torq(*)
+       integer(c_int) :: st
+     end function pib_get_forces
+     function pib_set_field(ctx, field, host) bind(C, name="pib_set_field") result(st)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: field
+       real(c_double), intent(in) :: host(*)
+       integer(c_int) :: st
+     end function pib_set_field
+     function pib_get_field(ctx, field, host) bind(C, name="pib_get_field") result(st)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: field
+       real(c_double), intent(out) :: host(*)
+       integer(c_int) :: st
+     end function pib_get_field
+     function pib_compute_new_timestep(ctx, rks) bind(C, name="pib_compute_new_timestep") result(st)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: rks
+       integer(c_int) :: st
+     end function pib_compute_new_timestep
+     function pib_nonlinear(ctx, rks) bind(C, name="pib_nonlinear") result(st)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: rks
+       integer(c_int) :: st
+     end function pib_nonlinear
+     function pib_velstep(ctx, rks) bind(C, name="pib_velstep") result(st)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: rks
+       integer(c_int) :: st
+     end function pib_velstep
+     function pib_phase_times(ctx, ms4) bind(C, name="pib_phase_times") result(st)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: ms4(4)
+       integer(c_int) :: st
+     end function pib_phase_times
+     function pib_sync(ctx) bind(C, name="pib_sync") result(st)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int) :: st
+     end function pib_sync
+  end interface
+end module pib_types

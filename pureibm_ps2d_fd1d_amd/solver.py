@@ -1,0 +1,169 @@
+"""Host-side mirror of the reference's per-step operator interface.
+
+The reference's time loop (dtibm_main_scal_flo.F90:426-527) calls, for each
+Runge-Kutta stage rks,
+
+    compute_new_timestep(rks)   dependent_functions.F90:1054
+    nonlinear(rks)              nl.F90:63  (form_nl, transforms, bcset)
+    velstep(sflag, rks)         steptheflow.F90:52
+
+``Solver`` exposes the same three calls over the C-ABI of
+include/pureibm_gpu.h, with the state owned by the device.  ``RunSetup`` is
+the reference's initialisation (read_nmls + generate_configuration + initflo)
+done by the Fortran host library from a ``<RUN>_floparam.in`` namelist file.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _abi
+from ._abi import PibConfig, PibScalars, dptr, PIB_OK
+
+
+class PibError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"status {code}: {msg}")
+        self.code = code
+
+
+class RunSetup:
+    """Namelist input + configuration for one run (Fortran host library)."""
+
+    def __init__(self, run_name, precision=_abi.PIB_FP64):
+        host = _abi.load_host()
+        nml = (run_name + "_floparam.in").encode()
+        st = host.pib_host_read(nml, len(nml))
+        if st != PIB_OK:
+            raise PibError(st, _abi.host_error())
+        rn = run_name.encode()
+        st = host.pib_host_configure(rn, len(rn))
+        if st != PIB_OK:
+            raise PibError(st, _abi.host_error())
+        nb, npnt, nx, my, mz = (ctypes.c_int() for _ in range(5))
+        host.pib_host_counts(ctypes.byref(nb), ctypes.byref(npnt), ctypes.byref(nx),
+                             ctypes.byref(my), ctypes.byref(mz))
+        self.nbody, self.nbnd = nb.value, npnt.value
+        self.nx, self.my, self.mz = nx.value, my.value, mz.value
+        self.config = PibConfig()
+        host.pib_host_get_config(ctypes.byref(self.config))
+        self.config.precision = precision
+        self.xc = np.zeros((self.nbody, 3), order="F")
+        self.radbdy = np.zeros(self.nbody)
+        host.pib_host_get_particles(dptr(self.xc), dptr(self.radbdy))
+        self.bndpts = np.zeros((3, self.nbnd), order="F")
+        self.cd = np.zeros((3, self.nbnd), order="F")
+        host.pib_host_get_points(dptr(self.bndpts), dptr(self.cd))
+        self.run_name = run_name
+
+    def finish(self, count_solid):
+        """Mask-dependent initial scalars (dt, cf, umean, ufmean_des ...)."""
+        s = PibScalars()
+        st = _abi.load_host().pib_host_finish(int(count_solid), ctypes.byref(s))
+        if st != PIB_OK:
+            raise PibError(st, _abi.host_error())
+        return s
+
+
+class Solver:
+    """One realization on one device context.
+
+    ``lib``/``prefix`` select the library implementing the C-ABI; the default
+    is the HIP library (``pib_``).  A missing HIP library raises.
+    """
+
+    def __init__(self, setup, lib=None, prefix="pib_"):
+        self.lib = lib if lib is not None else _abi.load_gpu()
+        self.p = prefix
+        self.setup = setup
+        self.cfg = setup.config
+        self.ctx = ctypes.c_void_p()
+        self._call("create", ctypes.byref(self.cfg), ctypes.byref(self.ctx), ctx=False)
+        self._call("set_points", self.ctx, setup.nbnd, dptr(setup.bndpts), dptr(setup.cd))
+        cs = ctypes.c_longlong()
+        self.nrpr_active = np.zeros(setup.nbody, dtype=np.int32)
+        self._call("set_particles", self.ctx, setup.nbody, dptr(setup.xc), dptr(setup.radbdy),
+                   None, None, ctypes.byref(cs),
+                   self.nrpr_active.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        self.count_solid = cs.value
+        s0 = setup.finish(self.count_solid)
+        self._call("set_scalars", self.ctx, ctypes.byref(s0))
+        c = self.cfg
+        self.nx, self.my, self.mz = c.nx, c.my, c.mz
+        self.my2 = c.my // 2 + 1
+        self.itrmax = 3 if c.rk3 else 1
+
+    def _fn(self, name):
+        return getattr(self.lib, self.p + name)
+
+    def _call(self, name, *args, ctx=True):
+        st = self._fn(name)(*args)
+        if st != PIB_OK:
+            msg = self._fn("last_error")(self.ctx if ctx else None)
+            raise PibError(st, (msg or b"").decode(errors="replace"))
+        return st
+
+    # ---- the reference's per-step operators ----
+    def compute_new_timestep(self, rks=1):
+        self._call("compute_new_timestep", self.ctx, rks)
+
+    def nonlinear(self, rks=1):
+        self._call("nonlinear", self.ctx, rks)
+
+    def velstep(self, rks=1):
+        self._call("velstep", self.ctx, rks)
+
+    def step(self, nsteps=1):
+        self._call("step", self.ctx, nsteps)
+
+    # ---- state access in the reference layout ----
+    def _shape(self, name):
+        nx, my, mz, my2 = self.nx, self.my, self.mz, self.my2
+        return {
+            "u": (nx + 1, my2, mz, 3), "nl": (nx + 1, my2, mz, 3), "onl": (nx + 1, my2, mz, 3),
+            "ff": (nx + 1, my2, mz, 3), "p": (nx, my2, mz),
+            "ubcp": (nx + 1, my, mz, 3), "nlbcp": (nx + 1, my, mz, 3),
+            "onlbcp": (nx + 1, my, mz, 3), "diffn": (nx + 1, my, mz, 3),
+            "fr": (nx + 1, my, mz, 3), "omega": (nx + 1, my, mz, 3),
+            "pbcp": (nx, my, mz), "ppr": (nx, my, mz, 3), "mask": (nx, my, mz),
+        }[name]
+
+    def field(self, name):
+        shp = self._shape(name)
+        if name in _abi.COMPLEX_FIELDS:
+            a = np.zeros(shp, dtype=np.complex128, order="F")
+        else:
+            a = np.zeros(shp, dtype=np.float64, order="F")
+        self._call("get_field", self.ctx, _abi.FIELDS[name], a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return a
+
+    def set_field(self, name, arr):
+        shp = self._shape(name)
+        dt = np.complex128 if name in _abi.COMPLEX_FIELDS else np.float64
+        a = np.asfortranarray(np.asarray(arr, dtype=dt).reshape(shp, order="F"))
+        self._call("set_field", self.ctx, _abi.FIELDS[name], a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+
+    def scalars(self):
+        s = PibScalars()
+        self._call("get_scalars", self.ctx, ctypes.byref(s))
+        return s
+
+    def set_scalars(self, s):
+        self._call("set_scalars", self.ctx, ctypes.byref(s))
+
+    def forces(self):
+        nb = self.setup.nbody
+        out = [np.zeros((nb, 3), order="F") for _ in range(4)]
+        self._call("get_forces", self.ctx, *[dptr(a) for a in out])
+        return dict(zip(("force", "pres", "visc", "torq"), out))
+
+    def close(self):
+        if self.ctx:
+            self._fn("destroy")(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
