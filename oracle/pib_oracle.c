@@ -97,35 +97,51 @@ static int small_factor(int n) {
     if (n % p == 0) return p;
   return n;
 }
-static void dft_rec(const cpx* in, size_t s, cpx* out, int n, int sign, cpx* tmp) {
-  if (n == 1) { out[0] = in[0]; return; }
-  int p = small_factor(n);
-  int m = n / p;
-  /* sub-transforms of the p decimated sequences into tmp[r*m + k] */
-  for (int r = 0; r < p; ++r) dft_rec(in + r * s, s * p, tmp + (size_t)r * m, m, sign, out);
-  for (int q = 0; q < p; ++q) {
-    for (int k = 0; k < m; ++k) {
-      int kk = k + q * m;
+/* exp(i*2*pi*t/n) for t < n, one table per length (cached) */
+typedef struct { int n; cpx* tab; } twcache;
+static twcache g_tw[64];
+static int g_ntw = 0;
+static const cpx* tw_get(int n) {
+  for (int q = 0; q < g_ntw; ++q)
+    if (g_tw[q].n == n) return g_tw[q].tab;
+  cpx* t = (cpx*)malloc(sizeof(cpx) * (size_t)n);
+  for (int k = 0; k < n; ++k) {
+    long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
+    t[k] = C((double)cosl(a), (double)sinl(a));
+  }
+  if (g_ntw < 64) { g_tw[g_ntw].n = n; g_tw[g_ntw].tab = t; ++g_ntw; }
+  return t;
+}
+/* out[k] = sum_r in[r*s] exp(sign*2*pi*i*r*k/m); twiddles from the length-ntop
+ * table (m divides ntop), indexed by the exact integer phase r*k mod m */
+static void dft_rec(const cpx* in, size_t s, cpx* out, int m, int sign, cpx* tmp, const cpx* tab, int ntop) {
+  if (m == 1) { out[0] = in[0]; return; }
+  int p = small_factor(m);
+  int q = m / p;
+  for (int r = 0; r < p; ++r) dft_rec(in + r * s, s * p, tmp + (size_t)r * q, q, sign, out, tab, ntop);
+  int scale = ntop / m;
+  for (int g = 0; g < p; ++g) {
+    for (int k = 0; k < q; ++k) {
+      int kk = k + g * q;
       double sr = 0.0, si = 0.0;
       for (int r = 0; r < p; ++r) {
-        long long ph = ((long long)r * kk) % n;
-        double ang = (double)sign * 2.0 * PI * (double)ph / (double)n;
-        double c = cos(ang), sn = sin(ang);
-        cpx v = tmp[(size_t)r * m + k];
-        sr += v.re * c - v.im * sn;
-        si += v.re * sn + v.im * c;
+        long long ph = ((long long)r * kk) % m;
+        cpx w = tab[ph * scale];
+        double wi = sign < 0 ? -w.im : w.im;
+        cpx v = tmp[(size_t)r * q + k];
+        sr += v.re * w.re - v.im * wi;
+        si += v.re * wi + v.im * w.re;
       }
       out[kk] = C(sr, si);
     }
   }
-  /* copy back so callers that pass tmp-derived buffers keep a clean state */
 }
 static void fft_c(const cpx* in, cpx* out, int n, int sign) {
   cpx* tmp = (cpx*)malloc(sizeof(cpx) * (size_t)n);
   cpx* tmp2 = (cpx*)malloc(sizeof(cpx) * (size_t)n);
   /* dft_rec uses its out argument as scratch for deeper levels */
   memcpy(tmp2, in, sizeof(cpx) * (size_t)n);
-  dft_rec(tmp2, 1, out, n, sign, tmp);
+  dft_rec(tmp2, 1, out, n, sign, tmp, tw_get(n), n);
   free(tmp);
   free(tmp2);
 }

@@ -1,0 +1,1402 @@
+// pib_gpu.hip -- MI355X (gfx950) implementation of the C-ABI declared in
+// include/pureibm_gpu.h: context and memory management, set-up kernels,
+// the device-side scalar control logic of the reference's time step, and
+// the orchestration of the per-step kernels of pib_kernels.h.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pib_kernels.h"
+
+namespace {
+
+constexpr int MAXNB = 96;  // neighbour-list capacity per body
+
+// ===========================================================================
+// set-up kernels
+// ===========================================================================
+__global__ void k_fill_u8(uint8_t* p, size_t n, uint8_t v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// grid_nodes_insphere (dependent_functions.F90:783-963): one workgroup per
+// body walks its bounding box; duplicate solid flags are counted as the
+// reference counts them.
+__global__ void k_mask_body(uint8_t* mask, const double* xc, const double* rad, int nbody, int nx, int my,
+                            int mz, long long* cnt) {
+  __shared__ double sh[256];
+  int m = blockIdx.x;
+  double r = rad[m];
+  double cx = xc[m], cy = xc[nbody + m], cz = xc[2 * nbody + m];
+  int imin = (int)ceil(cx - r), imax = (int)floor(cx + r);
+  int jmin = (int)ceil(cy - r), jmax = (int)floor(cy + r);
+  int kmin = (int)ceil(cz - r), kmax = (int)floor(cz + r);
+  int ni = imax - imin + 1, nj = jmax - jmin + 1, nk = kmax - kmin + 1;
+  long long tot = (long long)ni * nj * nk;
+  double c = 0.0;
+  for (long long e = threadIdx.x; e < tot; e += blockDim.x) {
+    int i = imin + (int)(e / ((long long)nj * nk));
+    int rem = (int)(e % ((long long)nj * nk));
+    int j = jmin + rem / nk, k = kmin + rem % nk;
+    double dist = (i - cx) * (i - cx);
+    dist = dist + (j - cy) * (j - cy) + (k - cz) * (k - cz);
+    dist = sqrt(dist);
+    if ((dist - r) <= PIB_SMALL_NUMBER) {
+      int ii = wrap1(i, nx), jj = wrap1(j, my), kk = wrap1(k, mz);
+      mask[((size_t)(ii - 1) * mz + (kk - 1)) * my + (jj - 1)] = 0;
+      c += 1.0;
+    }
+  }
+  double s = block_sum(c, sh);
+  if (threadIdx.x == 0) cnt[m] = (long long)s;
+}
+
+// neighbour sets of the soft-sphere search (collision_mod.F90:807-1062):
+// pairs with an image inside 1.2*(r_a + r_b) (physical units)
+__global__ void k_neighbours(const double* xc, const double* rad, int nbody, double dx, double lx, double ly,
+                             double lz, int* nb, int* nbcnt) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nbody) return;
+  int cnt = 0;
+  double pa0 = (xc[m] - 1.0) * dx, pa1 = (xc[nbody + m] - 1.0) * dx, pa2 = (xc[2 * nbody + m] - 1.0) * dx;
+  for (int q = 0; q < nbody; ++q) {
+    if (q == m) continue;
+    double pb0 = (xc[q] - 1.0) * dx, pb1 = (xc[nbody + q] - 1.0) * dx, pb2 = (xc[2 * nbody + q] - 1.0) * dx;
+    double rlm = 1.2 * (rad[m] * dx + rad[q] * dx);
+    bool hit = false;
+    for (int sx = -1; sx <= 1 && !hit; ++sx)
+      for (int sy = -1; sy <= 1 && !hit; ++sy)
+        for (int sz = -1; sz <= 1 && !hit; ++sz) {
+          double d0 = pb0 + sx * lx - pa0, d1 = pb1 + sy * ly - pa1, d2 = pb2 + sz * lz - pa2;
+          double dm = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+          if (rlm - dm > PIB_SMALL_NUMBER) hit = true;
+        }
+    if (hit) {
+      if (cnt < MAXNB) nb[(size_t)m * MAXNB + cnt] = q;
+      cnt++;
+    }
+  }
+  nbcnt[m] = cnt;
+}
+
+// check_external_pt (dependent_functions.F90:992-1051): a reversal pair is
+// inactive when its outer point, wrapped as the reference wraps it, lies
+// inside a neighbouring sphere (unshifted centre).
+__global__ void k_if_rev(Particles P, int nx, int my, int mz, const int* nb, const int* nbcnt, uint8_t* if_rev) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= P.nbody * P.nbnd) return;
+  int m = idx / P.nbnd, l = idx - m * P.nbnd;
+  double xlo[3];
+  for (int n = 0; n < 3; ++n) xlo[n] = P.xc[n * P.nbody + m] + P.bnd[l * 3 + n] * P.rado[m];
+  double mxf = (double)(nx + 1);
+  if (xlo[0] < 1.0) xlo[0] = mxf + xlo[0];
+  if (xlo[0] > mxf) xlo[0] = xlo[0] - mxf;
+  if (xlo[1] < 1.0) xlo[1] = my + 1.0 + xlo[1];
+  if (xlo[1] > my) xlo[1] = xlo[1] - my - 1.0;
+  if (xlo[2] < 1.0) xlo[2] = mz + 1.0 + xlo[2];
+  if (xlo[2] > mz) xlo[2] = xlo[2] - mz - 1.0;
+  uint8_t rev = 1;
+  int cnt = nbcnt[m] < MAXNB ? nbcnt[m] : MAXNB;
+  for (int t = 0; t < cnt && rev; ++t) {
+    int q = nb[(size_t)m * MAXNB + t];
+    double dist = 0.0;
+    for (int n = 0; n < 3; ++n) dist = dist + (P.xc[n * P.nbody + q] - xlo[n]) * (P.xc[n * P.nbody + q] - xlo[n]);
+    dist = sqrt(dist);
+    if (dist < P.rad[q]) rev = 0;
+  }
+  if_rev[idx] = rev;
+}
+
+__global__ void k_count_active(const uint8_t* if_rev, int nbnd, int* nrpr_active) {
+  __shared__ double sh[256];
+  int m = blockIdx.x;
+  double c = 0.0;
+  for (int l = threadIdx.x; l < nbnd; l += blockDim.x) c += if_rev[(size_t)m * nbnd + l];
+  double s = block_sum(c, sh);
+  if (threadIdx.x == 0) nrpr_active[m] = (int)s;
+}
+
+// Spreading records: for each active reversal pair and each of the 8 corners
+// of its inner-point stencil, the grid node it deposits on (sentinel when it
+// deposits nowhere), its weight and whether the redistributed fluid share is
+// added (boundary_condition.F90:1379-1462).
+__global__ void k_spread_records(Particles P, const uint8_t* mask, int nx, int my, int mz, int frmeanfluid,
+                                 unsigned int sentinel, unsigned int* key, int* rec, double* w, uint8_t* dist) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= P.nbody * P.nbnd) return;
+  int m = idx / P.nbnd, l = idx - m * P.nbnd;
+  bool act = P.if_rev[idx] != 0;
+  double xli[3];
+  for (int n = 0; n < 3; ++n) xli[n] = P.xc[n * P.nbody + m] + P.bnd[l * 3 + n] * P.radi[m];
+  Sten s;
+  make_sten(s, xli, nx, my, mz);
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i) {
+        int cidx = (k * 2 + j) * 2 + i;
+        size_t r = (size_t)idx * 8 + cidx;
+        unsigned int node = (unsigned int)(((size_t)s.ii[i] * mz + s.kk[k]) * my + s.jj[j]);
+        bool fl = mask[node] != 0;
+        bool dep = act && (frmeanfluid ? true : !fl);
+        key[r] = dep ? node : sentinel;
+        rec[r] = (int)r;
+        w[r] = s.w[cidx];
+        dist[r] = (uint8_t)(frmeanfluid ? 0 : 1);
+      }
+}
+__global__ void k_gather_records(int n, const int* rec_sorted, const double* w, const uint8_t* dist, int* rpt,
+                                 double* rw, uint8_t* rd) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  int r = rec_sorted[t];
+  rpt[t] = r / 8;
+  rw[t] = w[r];
+  rd[t] = dist[r];
+}
+
+// ===========================================================================
+// scalar control kernels (single workgroup)
+// ===========================================================================
+// compute_new_timestep tail (dependent_functions.F90:1143-1277)
+__global__ void k_scalar_cnt(DevState* st, const double* part, int nx, int rks, const double* velbdy) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  pib_scalars& s = st->s;
+  double ufl[3], usl[3], uml[3], mxc[3];
+  for (int c = 0; c < 3; ++c) {
+    double a = 0, b = 0, d = 0, mx = PIB_SMALL_NUMBER;
+    for (int i = 0; i < nx; ++i) {
+      const double* o = part + ((size_t)c * nx + i) * 4;
+      a += o[0];
+      b += o[1];
+      d += o[2];
+      mx = fmax(mx, o[3]);
+    }
+    ufl[c] = a;
+    usl[c] = b;
+    uml[c] = d;
+    mxc[c] = mx;
+  }
+  double umax_tmp = fmax(fmax(mxc[0], mxc[1]), mxc[2]);
+  double dtnew;
+  if (umax_tmp * s.dt / st->dx > st->cfl_max)
+    dtnew = st->cfl_max * st->dx / umax_tmp;
+  else
+    dtnew = s.dt;
+  s.dtnew = dtnew;
+  s.umax_tmp = umax_tmp;
+  double ntot = (double)((long long)st->nx * st->my * st->mz);
+  for (int n = 0; n < 3; ++n) {
+    s.ufmean_old[n] = s.ufmean[n];
+    s.ufmean[n] = ufl[n] / s.count_fluid;
+    s.usmean_temp[n] = usl[n] / s.count_solid;
+    s.umean_temp[n] = uml[n] / ntot;
+  }
+  int nb = st->nbody;
+  double meanslip[3], mix[3];
+  for (int n = 0; n < 3; ++n) {
+    double sum = 0.0;
+    for (int m = 0; m < nb; ++m) sum += velbdy[(size_t)n * nb + m];
+    double msv = nb > 0 ? sum / (double)nb : 0.0;
+    s.usmean_act[n] = msv;
+    s.dufmeandt[n] = -s.cf * (s.ufmean[n] - s.ufmean_old[n]);
+    meanslip[n] = (s.ufmean[n] - s.usmean_des[n]) * (1.0 - s.maxvolfrac);
+    double us = 0.0 + s.mean_volfrac * msv;
+    s.usmean[n] = us / s.mean_volfrac;
+    mix[n] = (1.0 - s.maxvolfrac) * (s.usmean[n] - s.ufmean[n]);
+  }
+  s.meanslipmod = sqrt(meanslip[0] * meanslip[0] + meanslip[1] * meanslip[1] + meanslip[2] * meanslip[2]);
+  s.mixmeanslipmod = sqrt(mix[0] * mix[0] + mix[1] * mix[1] + mix[2] * mix[2]);
+  if (rks == st->itrmax && st->adaptive) s.dt = dtnew;
+  s.t = s.t + s.dt * (st->coef[rks - 1][0] + st->coef[rks - 1][1]);
+  double tempt = s.ramp_frac_time * (double)(float)(st->nx) * st->dx / st->ucharmod;
+  if (st->soft_start && s.t < tempt) {
+    s.cf = -1.0 / s.dt;
+    s.cforig = s.cf;
+    s.cf = s.cf * (s.t / tempt);
+  } else {
+    s.cf = -1.0 / s.dt;
+    s.cforig = s.cf;
+  }
+}
+
+// compute_mpg_at_n tail (boundary_condition.F90:373-435)
+__global__ void k_scalar_drag(DevState* st, const double* body_tot, int rks, const double* rad, const double* pres,
+                              const double* visc, double* force, double* force_chem) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  pib_scalars& s = st->s;
+  int nb = st->nbody;
+  const double PI = 3.14159265358979323846;
+  s.norm_drag = s.norm_drag;  // unchanged here
+  double norm_factor = (3.0 * PI * st->vis * (s.mixmeanslipmod + PIB_SMALL_NUMBER) * st->char_length) * (double)nb;
+  st->norm_factor = norm_factor;
+  for (int n = 0; n < 3; ++n) {
+    s.visc_total_old[n] = s.visc_total[n];
+    double pt = 0, vt = 0;
+    for (int m = 0; m < nb; ++m) {
+      pt += body_tot[(size_t)m * 6 + n];
+      vt += body_tot[(size_t)m * 6 + 3 + n];
+    }
+    s.pres_total[n] = pt;
+    s.visc_total[n] = vt;
+  }
+  double pa[3], va[3];
+  for (int n = 0; n < 3; ++n) {
+    double a = 0, b = 0;
+    for (int m = 0; m < nb; ++m) {
+      a += pres[(size_t)n * nb + m];
+      b += visc[(size_t)n * nb + m];
+    }
+    pa[n] = a;
+    va[n] = b;
+  }
+  s.pres_drag = sqrt(pa[0] * pa[0] + pa[1] * pa[1] + pa[2] * pa[2]) / norm_factor / (1.0 - s.maxvolfrac);
+  s.visc_drag = sqrt(va[0] * va[0] + va[1] * va[1] + va[2] * va[2]) / norm_factor / (1.0 - s.maxvolfrac);
+  s.total_drag = s.pres_drag + s.visc_drag;
+  if (!st->set_mpg) {
+    for (int n = 0; n < 3; ++n) {
+      s.mpg[n] = s.cf * (s.ufmean_des[n] - s.ufmean[n]) +
+                 (s.pres_total[n] / st->voldom - (s.visc_total[n]) / st->voldom) / (1.0 - s.maxvolfrac);
+      if (st->include_frmeanfluid) s.mpg[n] = s.mpg[n] + s.frmeanfluid[n];
+    }
+  }
+  double c12 = st->coef[rks - 1][0] + st->coef[rks - 1][1];
+  for (int m = 0; m < nb; ++m) {
+    double d = 2.0 * rad[m] * st->dx;
+    double vol = d * d * d;
+    for (int n = 0; n < 3; ++n) {
+      size_t k = (size_t)n * nb + m;
+      force[k] = c12 * (pres[k] + visc[k]) - c12 * s.mpg[n] * PI * vol / 6.0;
+      force_chem[k] = c12 * (pres[k] + visc[k]);
+    }
+  }
+}
+
+// frmean / frmeanfluid over all reversal pairs (boundary_condition.F90:632-633)
+__global__ void k_reduce_frmean(DevState* st, const double* pts, int npts) {
+  __shared__ double sh[1024];
+  double a[6] = {0, 0, 0, 0, 0, 0};
+  for (int p = threadIdx.x; p < npts; p += blockDim.x) {
+    const double* o = pts + (size_t)p * 12;
+    for (int q = 0; q < 6; ++q) a[q] += o[6 + q];
+  }
+  double t[6];
+  for (int q = 0; q < 6; ++q) t[q] = block_sum(a[q], sh);
+  if (threadIdx.x == 0) {
+    double ntot = (double)((long long)st->nx * st->my * st->mz);
+    for (int n = 0; n < 3; ++n) {
+      st->s.frmean[n] = t[n] / ntot;
+      st->s.frmeanfluid[n] = t[3 + n] / st->s.count_fluid;
+    }
+  }
+}
+
+// bcset diagnostics (boundary_condition.F90:723-884), one phase
+__global__ void k_scalar_diag(DevState* st, const double* force, const double* force_chem) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  pib_scalars& s = st->s;
+  int nb = st->nbody;
+  const double PI = 3.14159265358979323846;
+  double af[3], ac[3];
+  for (int n = 0; n < 3; ++n) {
+    double a = 0, b = 0;
+    for (int m = 0; m < nb; ++m) {
+      a += force[(size_t)n * nb + m];
+      b += force_chem[(size_t)n * nb + m];
+    }
+    af[n] = (0.0 + s.mean_volfrac * (a / nb)) / s.mean_volfrac;
+    ac[n] = (0.0 + s.mean_volfrac * (b / nb)) / s.mean_volfrac;
+  }
+  double nd = sqrt(af[0] * af[0] + af[1] * af[1] + af[2] * af[2]);
+  double ndc = sqrt(ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2]);
+  double mpg_avg = sqrt(s.mpg[0] * s.mpg[0] + s.mpg[1] * s.mpg[1] + s.mpg[2] * s.mpg[2]);
+  s.total_drag_mpg = mpg_avg * st->voldom / st->norm_factor;
+  double nf;
+  if (st->set_mpg)
+    nf = (3.0 * PI * st->vis * (s.mixmeanslipmod + PIB_SMALL_NUMBER) * st->dia);
+  else
+    nf = (3.0 * PI * st->vis * st->ucharmod * st->dia);
+  s.norm_drag = nd / nf;
+  s.norm_drag_chem = ndc / nf;
+  if (s.norm_drag > 0.0) s.ferror = fabs(s.norm_drag - s.fold) / s.norm_drag;
+  else s.ferror = 1.0;
+  s.fold = s.norm_drag;
+  if (!(s.norm_drag <= (double)1.0E+06f) && st->status == 0) st->status = PIB_ERR_BLOWUP;
+}
+
+// velstep tail: check_divergence max, umean update (steptheflow.F90:93-99)
+__global__ void k_scalar_vel(DevState* st, const double* part, int nblk, const int* status) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double mx = 0.0;
+  for (int b = 0; b < nblk; ++b) mx = fmax(mx, part[b]);
+  st->s.divmax = mx;
+  for (int n = 0; n < 3; ++n)
+    st->s.umean[n] = st->s.umean[n] + (-st->s.mpg[n] + st->s.frmean[n] - st->s.frame_accln[n]) * st->s.dt;
+  if (*status && st->status == 0) st->status = *status;
+}
+
+// ===========================================================================
+// transform-layer drop-in: planes with arbitrary strides
+// ===========================================================================
+template <typename T>
+struct OpRawR2C {
+  const T* a;
+  C2<T>* b;
+  long long lda, ldb;
+  int my, my2, q;
+  __device__ void load2(int y, int z, T& x0, T& x1) {
+    const T* p = a + (size_t)q * lda + (size_t)z * my + y;
+    x0 = p[0];
+    x1 = p[1];
+  }
+  __device__ void store(int, int ky, int kz, C2<T> v) { b[(size_t)q * ldb + (size_t)kz * my2 + ky] = v; }
+};
+template <typename T>
+struct OpRawC2R {
+  const C2<T>* b;
+  T* a;
+  long long lda, ldb;
+  int my, my2, q;
+  __device__ C2<T> load(int ky, int kz) { return b[(size_t)q * ldb + (size_t)kz * my2 + ky]; }
+  __device__ void store2(int, int y, int z, T x0, T x1) {
+    T* p = a + (size_t)q * lda + (size_t)z * my + y;
+    p[0] = x0;
+    p[1] = x1;
+  }
+};
+template <typename T>
+__global__ void k_raw_r2c(PlaneGeom G, PlaneTabs<T> tb, const T* a, long long lda, C2<T>* b, long long ldb) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  OpRawR2C<T> op{a, b, lda, ldb, G.my, G.my2, (int)blockIdx.x};
+  plane_r2c<0>((C2<T>*)smem, G, tb, op);
+}
+template <typename T>
+__global__ void k_raw_c2r(PlaneGeom G, PlaneTabs<T> tb, const C2<T>* b, long long ldb, T* a, long long lda) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  OpRawC2R<T> op{b, a, lda, ldb, G.my, G.my2, (int)blockIdx.x};
+  plane_c2r<0>((C2<T>*)smem, G, tb, op);
+}
+
+// batched cyclic / plain Thomas, rows [i*nsys + s]
+__global__ void k_tridiag(int n, int nsys, const double* a, const double* b, const double* c, const double* r,
+                          double* x, int cyclic, double* scr) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys) return;
+  size_t ms = nsys;
+  double* g = scr;
+  double* y1 = scr + (size_t)n * ms;
+  double* y2 = scr + (size_t)2 * n * ms;
+  double* y3 = scr + (size_t)3 * n * ms;
+  auto A = [&](int i) { return a[(size_t)i * ms + s]; };
+  auto B = [&](int i) { return b[(size_t)i * ms + s]; };
+  auto Cc = [&](int i) { return c[(size_t)i * ms + s]; };
+  double auxv0 = 0, auxvn = 0;
+  double b0 = B(0), bn = B(n - 1);
+  if (cyclic) {
+    auxv0 = Cc(n - 1) / 1.0;
+    auxvn = A(0) / 1.0;
+    b0 = b0 - 1.0 * auxv0;
+    bn = bn - 1.0 * auxvn;
+  }
+  double bet = b0;
+  double v1 = r[s] / bet, v2 = 1.0 / bet, v3 = 0.0;
+  y1[s] = v1;
+  y2[s] = v2;
+  y3[s] = v3;
+  g[s] = 0;
+  for (int j = 1; j < n; ++j) {
+    double gam = Cc(j - 1) / bet;
+    double bj = (j == n - 1) ? bn : B(j);
+    bet = bj - A(j) * gam;
+    double r2 = (j == n - 1) ? 1.0 : 0.0;
+    v1 = (r[(size_t)j * ms + s] - A(j) * v1) / bet;
+    v2 = (r2 - A(j) * v2) / bet;
+    g[(size_t)j * ms + s] = gam;
+    y1[(size_t)j * ms + s] = v1;
+    y2[(size_t)j * ms + s] = v2;
+  }
+  for (int j = n - 2; j >= 0; --j) {
+    double gg = g[(size_t)(j + 1) * ms + s];
+    v1 = y1[(size_t)j * ms + s] - gg * v1;
+    v2 = y2[(size_t)j * ms + s] - gg * v2;
+    y1[(size_t)j * ms + s] = v1;
+    y2[(size_t)j * ms + s] = v2;
+  }
+  double vz = 0, vy = 0;
+  if (cyclic) {
+    vz = auxv0 * y2[s] + auxvn * y2[(size_t)(n - 1) * ms + s];
+    vy = auxv0 * y1[s] + auxvn * y1[(size_t)(n - 1) * ms + s];
+  }
+  for (int i = 0; i < n; ++i) {
+    double yy = y1[(size_t)i * ms + s];
+    x[(size_t)i * ms + s] = cyclic ? yy - vy * y2[(size_t)i * ms + s] / (1.0 + vz) : yy;
+  }
+}
+
+// ===========================================================================
+// host side
+// ===========================================================================
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) {                                                        \
+      throw PibFail(PIB_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    }                                                                              \
+  } while (0)
+
+struct PibFail {
+  int code;
+  std::string msg;
+  PibFail(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+
+static bool make_plan(int n, Fft1D& p) {
+  p.n = n;
+  p.np = 0;
+  int m = n;
+  while (m % 8 == 0 && p.np < 8) { p.rad[p.np++] = 8; m /= 8; }
+  while (m % 4 == 0 && p.np < 8) { p.rad[p.np++] = 4; m /= 4; }
+  while (m % 2 == 0 && p.np < 8) { p.rad[p.np++] = 2; m /= 2; }
+  while (m % 3 == 0 && p.np < 8) { p.rad[p.np++] = 3; m /= 3; }
+  while (m % 5 == 0 && p.np < 8) { p.rad[p.np++] = 5; m /= 5; }
+  return m == 1;
+}
+static std::vector<int> rev_table(const Fft1D& p) {
+  std::vector<int> r(p.n);
+  for (int k = 0; k < p.n; ++k) {
+    int kk = k, span = p.n, pos = 0;
+    for (int s = 0; s < p.np; ++s) {
+      span /= p.rad[s];
+      pos += (kk % p.rad[s]) * span;
+      kk /= p.rad[s];
+    }
+    r[k] = pos;
+  }
+  return r;
+}
+template <typename T>
+static std::vector<C2<T>> tw_table(int n, int count) {
+  std::vector<C2<T>> t(count);
+  const long double TWO_PI = 6.283185307179586476925286766559005768L;
+  for (int k = 0; k < count; ++k) {
+    long double a = -TWO_PI * (long double)k / (long double)n;
+    t[k].x = (T)cosl(a);
+    t[k].y = (T)sinl(a);
+  }
+  return t;
+}
+
+struct Base {
+  virtual ~Base() {}
+};
+
+template <typename T>
+struct Impl : Base {
+  pib_config cfg;
+  int nx, my, mz, my2, hy, itrmax;
+  double coef[3][4];
+  hipStream_t stream = nullptr;
+  PlaneGeom G;
+  PlaneTabs<T> tb;
+  int nt_plane = 256;   // threads per plane workgroup
+  size_t lds_plane = 0;
+  // device memory
+  std::vector<void*> allocs;
+  T *wy, *wz, *w2;
+  C2<T> *u, *p, *nl, *onl, *ff;
+  T *ubcp, *pbcp, *nlbc, *onlbc, *diffn, *omega, *fr, *ppr;
+  uint8_t* mask;
+  DevState* dst;
+  double* part;
+  size_t part_n = 0;
+  T* vscr;
+  double* vscr64;
+  int* dstatus;
+  // particles
+  int nbody = 0, nbnd = 0;
+  double *xc = nullptr, *rad = nullptr, *radi = nullptr, *rado = nullptr, *vel = nullptr, *angv = nullptr;
+  double *bnd = nullptr, *cdn = nullptr;
+  uint8_t* if_rev = nullptr;
+  int* nrpr = nullptr;
+  double *pres = nullptr, *visc = nullptr, *torq = nullptr, *force = nullptr, *force_chem = nullptr;
+  double *drag_pts = nullptr, *body_tot = nullptr, *frc_pts = nullptr;
+  int ntarget = 0;
+  int *tnode = nullptr, *toff = nullptr, *rpt = nullptr;
+  double* rw = nullptr;
+  uint8_t* rdist = nullptr;
+  bool have_points = false, have_particles = false, nlbc_valid = true;
+  std::vector<void*> pallocs;  // particle-dependent allocations
+  hipEvent_t ev[5];
+  bool timed = false;
+
+  template <typename V>
+  V* alloc(size_t n, bool part_dep = false) {
+    void* ptr = nullptr;
+    HIPCHK(hipMalloc(&ptr, n * sizeof(V) + 64));
+    HIPCHK(hipMemsetAsync(ptr, 0, n * sizeof(V) + 64, stream));
+    (part_dep ? pallocs : allocs).push_back(ptr);
+    return (V*)ptr;
+  }
+  template <typename V>
+  V* upload(const std::vector<V>& h) {
+    V* d = alloc<V>(h.size());
+    HIPCHK(hipMemcpyAsync(d, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice, stream));
+    return d;
+  }
+
+  explicit Impl(const pib_config& c) : cfg(c) {
+    nx = c.nx;
+    my = c.my;
+    mz = c.mz;
+    my2 = my / 2 + 1;
+    hy = my / 2;
+    if (c.rk3) {
+      double cc[3][4] = {{4.0 / 15.0, 4.0 / 15.0, 8.0 / 15.0, 0.0},
+                         {1.0 / 15.0, 1.0 / 15.0, 5.0 / 12.0, -17.0 / 60.0},
+                         {1.0 / 6.0, 1.0 / 6.0, 3.0 / 4.0, -5.0 / 12.0}};
+      memcpy(coef, cc, sizeof cc);
+      itrmax = 3;
+    } else {
+      memset(coef, 0, sizeof coef);
+      coef[0][0] = 0.5;
+      coef[0][1] = 0.5;
+      coef[0][2] = 3.0 / 2.0;
+      coef[0][3] = -0.5;
+      itrmax = 1;
+    }
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    // plane FFT geometry
+    G.my = my;
+    G.mz = mz;
+    G.my2 = my2;
+    G.hy = hy;
+    G.ld = my2;
+    if (!make_plan(hy, G.py) || !make_plan(mz, G.pz))
+      throw PibFail(PIB_ERR_UNSUPPORTED, "my/2 and mz must factor into 2, 3, 5");
+    lds_plane = (size_t)mz * G.ld * sizeof(C2<T>);
+    size_t E = (size_t)my2 * mz;
+    nt_plane = E <= 256 ? 64 : (E <= 2500 ? 256 : (E <= 5000 ? 512 : 1024));
+    size_t red = (size_t)nt_plane * sizeof(double);
+    if (lds_plane < red) lds_plane = red;
+    if (lds_plane > 160 * 1024)
+      throw PibFail(PIB_ERR_UNSUPPORTED, "plane does not fit the 160 KB LDS of one workgroup");
+    tb.twy = upload(tw_table<T>(hy, hy));
+    tb.twz = upload(tw_table<T>(mz, mz));
+    tb.wr = upload(tw_table<T>(my, hy + 1));
+    tb.revy = upload(rev_table(G.py));
+    tb.revz = upload(rev_table(G.pz));
+    // wavenumbers (initialize_flo.F90:735-763), dy = dz = dx
+    const long double TWO_PI = 6.283185307179586476925286766559005768L;
+    std::vector<T> hwy(my2), hwz(mz), hw2((size_t)my2 * mz);
+    std::vector<double> dwy(my2), dwz(mz);
+    const double twopi = 2.0 * (4.0 * atan(1.0));
+    for (int i = 1; i <= my2; ++i) dwy[i - 1] = twopi * (double)(i - 1) / ((double)my * c.dx);
+    for (int i = 1; i <= mz / 2; ++i) {
+      dwz[i - 1] = twopi * (double)(i - 1) / ((double)mz * c.dx);
+      dwz[mz - i] = -twopi * (double)i / ((double)mz * c.dx);
+    }
+    (void)TWO_PI;
+    for (int j = 0; j < my2; ++j) hwy[j] = (T)dwy[j];
+    for (int k = 0; k < mz; ++k) hwz[k] = (T)dwz[k];
+    for (int k = 0; k < mz; ++k)
+      for (int j = 0; j < my2; ++j) {
+        double wyy = -(dwy[j] * dwy[j]), wzz = -(dwz[k] * dwz[k]);
+        hw2[(size_t)k * my2 + j] = (T)(-(wyy + wzz));
+      }
+    wy = upload(hwy);
+    wz = upload(hwz);
+    w2 = upload(hw2);
+    size_t nsc = (size_t)3 * nx * E, nsr = (size_t)3 * nx * my * mz;
+    u = alloc<C2<T>>(nsc);
+    nl = alloc<C2<T>>(nsc);
+    onl = alloc<C2<T>>(nsc);
+    ff = alloc<C2<T>>(nsc);
+    p = alloc<C2<T>>((size_t)nx * E);
+    ubcp = alloc<T>(nsr);
+    nlbc = alloc<T>(nsr);
+    onlbc = alloc<T>(nsr);
+    diffn = alloc<T>(nsr);
+    omega = alloc<T>(nsr);
+    fr = alloc<T>(nsr);
+    ppr = alloc<T>(nsr);
+    pbcp = alloc<T>((size_t)nx * my * mz);
+    mask = alloc<uint8_t>((size_t)nx * my * mz);
+    fill_mask();
+    dst = alloc<DevState>(1);
+    part_n = std::max<size_t>((size_t)3 * nx * 4, (size_t)(E / 256 + 2));
+    part = alloc<double>(part_n);
+    vscr = alloc<T>((size_t)6 * nx * E);
+    vscr64 = alloc<double>((size_t)6 * nx);
+    dstatus = alloc<int>(1);
+    DevState h;
+    memset(&h, 0, sizeof h);
+    memcpy(h.coef, coef, sizeof coef);
+    h.dx = c.dx;
+    h.vis = c.vis;
+    h.cfl_max = c.cfl_max;
+    h.ucharmod = c.ucharmod;
+    h.char_length = c.char_length;
+    h.dia = c.dia;
+    h.voldom = c.voldom;
+    h.dx3 = pow(c.dx, 3.0);
+    h.dr = c.dr;
+    h.nx = nx;
+    h.my = my;
+    h.mz = mz;
+    h.adaptive = c.adaptive;
+    h.soft_start = c.soft_start;
+    h.set_mpg = c.set_mpg;
+    h.include_frmeanfluid = c.include_frmeanfluid;
+    h.itrmax = itrmax;
+    h.revernorm = c.revernorm;
+    h.s.count_fluid = (double)((long long)nx * my * mz);
+    HIPCHK(hipMemcpyAsync(dst, &h, sizeof h, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  ~Impl() override {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void* q : pallocs) (void)hipFree(q);
+    for (void* q : allocs) (void)hipFree(q);
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void fill_mask() {
+    size_t n = (size_t)nx * my * mz;
+    k_fill_u8<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(mask, n, 1);
+    HIPCHK(hipGetLastError());
+  }
+  KArgs<T> args(int rks) const {
+    KArgs<T> A;
+    A.nx = nx;
+    A.my = my;
+    A.mz = mz;
+    A.my2 = my2;
+    A.hy = hy;
+    A.dx = (T)cfg.dx;
+    A.dx2 = (T)(cfg.dx * cfg.dx);
+    A.vis = (T)cfg.vis;
+    A.wy = wy;
+    A.wz = wz;
+    A.w2 = w2;
+    A.G = G;
+    A.tb = tb;
+    A.u = u;
+    A.p = p;
+    A.nl = nl;
+    A.onl = onl;
+    A.ff = ff;
+    A.ubcp = ubcp;
+    A.pbcp = pbcp;
+    A.nlbc = nlbc;
+    A.onlbc = onlbc;
+    A.diffn = diffn;
+    A.omega = omega;
+    A.fr = fr;
+    A.ppr = ppr;
+    A.mask = mask;
+    A.st = dst;
+    A.part = part;
+    A.rks = rks;
+    A.xcd_map = 1;
+    return A;
+  }
+  Particles parts() const {
+    Particles P;
+    P.xc = xc;
+    P.rad = rad;
+    P.radi = radi;
+    P.rado = rado;
+    P.vel = vel;
+    P.angv = angv;
+    P.bnd = bnd;
+    P.cd = cdn;
+    P.if_rev = if_rev;
+    P.nrpr_active = nrpr;
+    P.nbody = nbody;
+    P.nbnd = nbnd;
+    return P;
+  }
+  template <typename K>
+  void set_lds(K kern, size_t bytes) {
+    HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  }
+  template <int KIND>
+  void c2r(int rks, int ncomp) {
+    auto kern = k_c2r<T, KIND>;
+    set_lds(kern, lds_plane);
+    kern<<<dim3(nx, ncomp), nt_plane, lds_plane, stream>>>(args(rks));
+    HIPCHK(hipGetLastError());
+  }
+
+  // ---------------- set-up ----------------
+  void set_points(int n, const double* b, const double* c) {
+    if (have_particles) throw PibFail(PIB_ERR_STATE, "set_points must precede set_particles");
+    nbnd = n;
+    std::vector<double> hb(b, b + 3 * (size_t)n), hc(c, c + 3 * (size_t)n);
+    bnd = upload(hb);
+    cdn = upload(hc);
+    HIPCHK(hipStreamSynchronize(stream));
+    have_points = true;
+  }
+  void set_particles(int nb, const double* hxc, const double* hrad, const double* hvel, const double* hangv,
+                     long long* count_solid, int* nrpr_active) {
+    if (nb > 0 && !have_points) throw PibFail(PIB_ERR_STATE, "set_points must precede set_particles");
+    HIPCHK(hipStreamSynchronize(stream));
+    for (void* q : pallocs) (void)hipFree(q);
+    pallocs.clear();
+    nbody = nb;
+    size_t n3 = (size_t)3 * (nb > 0 ? nb : 1);
+    xc = alloc<double>(n3, true);
+    rad = alloc<double>(n3, true);
+    radi = alloc<double>(n3, true);
+    rado = alloc<double>(n3, true);
+    vel = alloc<double>(n3, true);
+    angv = alloc<double>(n3, true);
+    pres = alloc<double>(n3, true);
+    visc = alloc<double>(n3, true);
+    torq = alloc<double>(n3, true);
+    force = alloc<double>(n3, true);
+    force_chem = alloc<double>(n3, true);
+    nrpr = alloc<int>(nb > 0 ? nb : 1, true);
+    long long* cnt = alloc<long long>(nb > 0 ? nb : 1, true);
+    std::vector<double> hr(nb), hri(nb), hro(nb);
+    const int nmin = std::min(nx, std::min(my, mz));
+    for (int m = 0; m < nb; ++m) {
+      if (!(hrad[m] > cfg.dr && hrad[m] + cfg.dr + 2 < 0.5 * nmin))
+        throw PibFail(PIB_ERR_ARG, "radbdy must exceed dr and stay below half the box");
+      const int ext[3] = {nx, my, mz};
+      for (int d = 0; d < 3; ++d) {
+        double v = hxc[(size_t)d * nb + m];
+        if (!(v >= -ext[d] && v <= 2.0 * ext[d]))
+          throw PibFail(PIB_ERR_ARG, "particle centre outside [-n, 2n] grid units");
+      }
+      hr[m] = hrad[m];
+      hri[m] = hrad[m] - cfg.dr;
+      hro[m] = hrad[m] + cfg.dr;
+    }
+    if (nb > 0) {
+      HIPCHK(hipMemcpyAsync(xc, hxc, n3 * sizeof(double), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(rad, hr.data(), nb * sizeof(double), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(radi, hri.data(), nb * sizeof(double), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(rado, hro.data(), nb * sizeof(double), hipMemcpyHostToDevice, stream));
+      if (hvel) HIPCHK(hipMemcpyAsync(vel, hvel, n3 * sizeof(double), hipMemcpyHostToDevice, stream));
+      if (hangv) HIPCHK(hipMemcpyAsync(angv, hangv, n3 * sizeof(double), hipMemcpyHostToDevice, stream));
+    }
+    // solid mask and counts
+    fill_mask();
+    long long cs = 0;
+    if (nb > 0) {
+      k_mask_body<<<nb, 256, 0, stream>>>(mask, xc, rad, nb, nx, my, mz, cnt);
+      HIPCHK(hipGetLastError());
+      std::vector<long long> hc(nb);
+      HIPCHK(hipMemcpyAsync(hc.data(), cnt, nb * sizeof(long long), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      for (long long v : hc) cs += v;
+    }
+    size_t npts = (size_t)nb * nbnd;
+    std::vector<int> hn(nb > 0 ? nb : 1, 0);
+    if (nb > 0) {
+      // neighbour sets, active reversal pairs
+      int* nbl = alloc<int>((size_t)nb * MAXNB, true);
+      int* nbc = alloc<int>(nb, true);
+      k_neighbours<<<(nb + 127) / 128, 128, 0, stream>>>(xc, rad, nb, cfg.dx, nx * cfg.dx, my * cfg.dx, mz * cfg.dx,
+                                                          nbl, nbc);
+      HIPCHK(hipGetLastError());
+      std::vector<int> hcnt(nb);
+      HIPCHK(hipMemcpyAsync(hcnt.data(), nbc, nb * sizeof(int), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      for (int v : hcnt)
+        if (v > MAXNB) throw PibFail(PIB_ERR_ARG, "more than 96 neighbours of one sphere (overlapping bed?)");
+      if_rev = alloc<uint8_t>(npts, true);
+      Particles P = parts();
+      k_if_rev<<<(unsigned)((npts + 255) / 256), 256, 0, stream>>>(P, nx, my, mz, nbl, nbc, if_rev);
+      HIPCHK(hipGetLastError());
+      k_count_active<<<nb, 256, 0, stream>>>(if_rev, nbnd, nrpr);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(hn.data(), nrpr, nb * sizeof(int), hipMemcpyDeviceToHost, stream));
+      drag_pts = alloc<double>(npts * 9, true);
+      body_tot = alloc<double>((size_t)nb * 6, true);
+      frc_pts = alloc<double>(npts * 12, true);
+      build_spreading();
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    // scalars that depend on the mask
+    DevState h;
+    HIPCHK(hipMemcpy(&h, dst, sizeof h, hipMemcpyDeviceToHost));
+    h.nbody = nb;
+    h.nbnd = nbnd;
+    h.s.count_solid = (double)cs;
+    h.s.count_fluid = (double)((long long)nx * my * mz) - (double)cs;
+    h.s.maxvolfrac = (double)cs / (double)((long long)nx * my * mz);
+    HIPCHK(hipMemcpy(dst, &h, sizeof h, hipMemcpyHostToDevice));
+    if (count_solid) *count_solid = cs;
+    if (nrpr_active)
+      for (int m = 0; m < nb; ++m) nrpr_active[m] = hn[m];
+    // zero the forcing field: only target nodes are written each step
+    HIPCHK(hipMemsetAsync(fr, 0, (size_t)3 * nx * my * mz * sizeof(T), stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    have_particles = true;
+  }
+  void build_spreading() {
+    size_t nrec = (size_t)nbody * nbnd * 8;
+    size_t N = (size_t)nx * my * mz;
+    unsigned int sentinel = (unsigned int)N;
+    int bits = 1;
+    while ((1ull << bits) <= (unsigned long long)N) ++bits;
+    unsigned int* key = alloc<unsigned int>(nrec, true);
+    unsigned int* key_s = alloc<unsigned int>(nrec, true);
+    int* rec = alloc<int>(nrec, true);
+    int* rec_s = alloc<int>(nrec, true);
+    double* w = alloc<double>(nrec, true);
+    uint8_t* dist = alloc<uint8_t>(nrec, true);
+    Particles P = parts();
+    size_t npts = (size_t)nbody * nbnd;
+    k_spread_records<<<(unsigned)((npts + 255) / 256), 256, 0, stream>>>(P, mask, nx, my, mz,
+                                                                          cfg.include_frmeanfluid, sentinel, key,
+                                                                          rec, w, dist);
+    HIPCHK(hipGetLastError());
+    size_t tmp_bytes = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key, key_s, rec, rec_s, (int)nrec, 0, bits,
+                                              stream));
+    void* tmp = alloc<unsigned char>(tmp_bytes + 16, true);
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key, key_s, rec, rec_s, (int)nrec, 0, bits, stream));
+    // run-length encode the sorted keys
+    unsigned int* ukeys = alloc<unsigned int>(nrec + 1, true);
+    int* counts = alloc<int>(nrec + 1, true);
+    int* nruns = alloc<int>(1, true);
+    size_t tb2 = 0;
+    HIPCHK(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb2, key_s, ukeys, counts, nruns, (int)nrec, stream));
+    void* tmp2 = alloc<unsigned char>(tb2 + 16, true);
+    HIPCHK(hipcub::DeviceRunLengthEncode::Encode(tmp2, tb2, key_s, ukeys, counts, nruns, (int)nrec, stream));
+    int hruns = 0;
+    HIPCHK(hipMemcpyAsync(&hruns, nruns, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    std::vector<unsigned int> hk(hruns);
+    std::vector<int> hc(hruns);
+    HIPCHK(hipMemcpy(hk.data(), ukeys, hruns * sizeof(unsigned int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hc.data(), counts, hruns * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> hnode, hoff;
+    int acc = 0;
+    for (int r = 0; r < hruns; ++r) {
+      if (hk[r] >= sentinel) break;
+      hnode.push_back((int)hk[r]);
+      hoff.push_back(acc);
+      acc += hc[r];
+    }
+    hoff.push_back(acc);
+    ntarget = (int)hnode.size();
+    tnode = alloc<int>(hnode.size() + 1, true);
+    toff = alloc<int>(hoff.size(), true);
+    if (!hnode.empty()) HIPCHK(hipMemcpy(tnode, hnode.data(), hnode.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(toff, hoff.data(), hoff.size() * sizeof(int), hipMemcpyHostToDevice));
+    rpt = alloc<int>((size_t)acc + 1, true);
+    rw = alloc<double>((size_t)acc + 1, true);
+    rdist = alloc<uint8_t>((size_t)acc + 1, true);
+    if (acc > 0) {
+      k_gather_records<<<(acc + 255) / 256, 256, 0, stream>>>(acc, rec_s, w, dist, rpt, rw, rdist);
+      HIPCHK(hipGetLastError());
+    }
+  }
+
+  // ---------------- per-step routines ----------------
+  void compute_new_timestep(int rks) {
+    c2r<C2R_UBCP>(rks, 3);
+    c2r<C2R_PBCP>(rks, 1);
+    k_scalar_cnt<<<1, 64, 0, stream>>>(dst, part, nx, rks, vel ? vel : part);
+    HIPCHK(hipGetLastError());
+  }
+  void form_nl(int rks) {
+    size_t E = (size_t)my2 * mz;
+    int nit = (int)((E + nt_plane - 1) / nt_plane);
+    bool up = cfg.discr_scheme == PIB_SCHEME_UPWIND, mm = cfg.limiter == PIB_LIMITER_MINMOD;
+    if (!launch_nl<1>(nit, up, mm, rks) && !launch_nl<2>(nit, up, mm, rks) && !launch_nl<3>(nit, up, mm, rks) &&
+        !launch_nl<5>(nit, up, mm, rks) && !launch_nl<9>(nit, up, mm, rks) && !launch_nl<13>(nit, up, mm, rks))
+      throw PibFail(PIB_ERR_UNSUPPORTED, "plane too large for the register-accumulated form_nl");
+  }
+  template <int NIT>
+  bool launch_nl(int nit, bool up, bool mm, int rks) {
+    if (nit > NIT) return false;
+    if (!up) run_nl(k_form_nl<T, NIT, false, false>, rks);
+    else if (!mm) run_nl(k_form_nl<T, NIT, true, false>, rks);
+    else run_nl(k_form_nl<T, NIT, true, true>, rks);
+    return true;
+  }
+  template <typename K>
+  void run_nl(K kern, int rks) {
+    set_lds(kern, lds_plane);
+    kern<<<dim3(nx, 3), nt_plane, lds_plane, stream>>>(args(rks));
+    HIPCHK(hipGetLastError());
+  }
+  void nonlinear(int rks) {
+    // onl <- nl (nl.F90:99-108): the buffers swap roles; the physical copy of
+    // onl is the previous nlbc when that one is current
+    std::swap(nl, onl);
+    std::swap(nlbc, onlbc);
+    bool onlbc_ok = nlbc_valid;
+    form_nl(rks);
+    c2r<C2R_NLBC>(rks, 3);
+    if (!onlbc_ok) c2r<C2R_ONLBC>(rks, 3);
+    nlbc_valid = true;
+    if (nbody > 0) bcset(rks);
+  }
+  void bcset(int rks) {
+    Particles P = parts();
+    size_t npts = (size_t)nbody * nbnd;
+    c2r<C2R_VISC>(rks, 3);
+    c2r<C2R_OMEGA>(rks, 3);
+    k_drag_points<T><<<(unsigned)((npts + 255) / 256), 256, 0, stream>>>(args(rks), P, drag_pts);
+    HIPCHK(hipGetLastError());
+    k_drag_body<<<nbody, 256, 0, stream>>>(drag_pts, nbnd, nbody, rks, pres, visc, torq, body_tot);
+    HIPCHK(hipGetLastError());
+    k_scalar_drag<<<1, 64, 0, stream>>>(dst, body_tot, rks, rad, pres, visc, force, force_chem);
+    HIPCHK(hipGetLastError());
+    c2r<C2R_PGRAD>(rks, 3);
+    if (cfg.dorpr) {
+      k_forcing_points<T><<<(unsigned)((npts + 255) / 256), 256, 0, stream>>>(args(rks), P, frc_pts);
+      HIPCHK(hipGetLastError());
+      k_reduce_frmean<<<1, 1024, 0, stream>>>(dst, frc_pts, (int)npts);
+      HIPCHK(hipGetLastError());
+      if (ntarget > 0) {
+        k_spread<T><<<(ntarget + 255) / 256, 256, 0, stream>>>(args(rks), ntarget, tnode, toff, rpt, rw, rdist,
+                                                               frc_pts);
+        HIPCHK(hipGetLastError());
+      }
+    }
+    auto kern = k_r2c_force<T>;
+    set_lds(kern, lds_plane);
+    kern<<<dim3(nx, 3), nt_plane, lds_plane, stream>>>(args(rks));
+    HIPCHK(hipGetLastError());
+    k_scalar_diag<<<1, 64, 0, stream>>>(dst, force, force_chem);
+    HIPCHK(hipGetLastError());
+  }
+  void velstep(int rks) {
+    int nm = my2 * mz;
+    int nblk = (nm + 255) / 256;
+    if ((size_t)nblk > part_n) throw PibFail(PIB_ERR_ARG, "partial buffer too small");
+    HIPCHK(hipMemsetAsync(dstatus, 0, sizeof(int), stream));
+    k_velstep<T><<<nblk, 256, 0, stream>>>(args(rks), vscr, vscr64, dstatus);
+    HIPCHK(hipGetLastError());
+    k_scalar_vel<<<1, 64, 0, stream>>>(dst, part, nblk, dstatus);
+    HIPCHK(hipGetLastError());
+  }
+  void step_async(int nsteps) {
+    if (!have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
+    for (int n = 0; n < nsteps; ++n) {
+      bool last = n == nsteps - 1;
+      for (int rks = 1; rks <= itrmax; ++rks) {
+        if (last && rks == itrmax) HIPCHK(hipEventRecord(ev[0], stream));
+        compute_new_timestep(rks);
+        if (last && rks == itrmax) HIPCHK(hipEventRecord(ev[1], stream));
+        nonlinear_timed(rks, last && rks == itrmax);
+        if (last && rks == itrmax) HIPCHK(hipEventRecord(ev[3], stream));
+        velstep(rks);
+        if (last && rks == itrmax) HIPCHK(hipEventRecord(ev[4], stream));
+      }
+    }
+    timed = nsteps > 0;
+  }
+  void nonlinear_timed(int rks, bool rec) {
+    std::swap(nl, onl);
+    std::swap(nlbc, onlbc);
+    bool onlbc_ok = nlbc_valid;
+    form_nl(rks);
+    c2r<C2R_NLBC>(rks, 3);
+    if (!onlbc_ok) c2r<C2R_ONLBC>(rks, 3);
+    nlbc_valid = true;
+    if (rec) HIPCHK(hipEventRecord(ev[2], stream));
+    if (nbody > 0) bcset(rks);
+  }
+  int check_status() {
+    int st = 0;
+    HIPCHK(hipMemcpyAsync(&st, &dst->status, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return st;
+  }
+  void phase_times(double* ms) {
+    HIPCHK(hipStreamSynchronize(stream));
+    if (!timed) {
+      for (int q = 0; q < 4; ++q) ms[q] = 0;
+      return;
+    }
+    for (int q = 0; q < 4; ++q) {
+      float f = 0;
+      HIPCHK(hipEventElapsedTime(&f, ev[q], ev[q + 1]));
+      ms[q] = f;
+    }
+  }
+
+  // ---------------- state transfer ----------------
+  // spectral (nx+1, my2, mz, ncomp) reference layout <-> device [c][i][kz][ky]
+  void put_spec(C2<T>* d, int ncomp, const double* h, bool copy_plane) {
+    int ni = copy_plane ? nx + 1 : nx;
+    size_t E = (size_t)my2 * mz;
+    std::vector<C2<T>> t((size_t)ncomp * nx * E);
+    for (int c = 0; c < ncomp; ++c)
+      for (int k = 0; k < mz; ++k)
+        for (int j = 0; j < my2; ++j)
+          for (int i = 0; i < nx; ++i) {
+            size_t hi = ((((size_t)c * mz + k) * my2 + j) * ni + i) * 2;
+            C2<T>& v = t[(((size_t)c * nx + i) * mz + k) * my2 + j];
+            v.x = (T)h[hi];
+            v.y = (T)h[hi + 1];
+          }
+    HIPCHK(hipMemcpyAsync(d, t.data(), t.size() * sizeof(C2<T>), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  void get_spec(const C2<T>* d, int ncomp, double* h, bool copy_plane) {
+    int ni = copy_plane ? nx + 1 : nx;
+    size_t E = (size_t)my2 * mz;
+    std::vector<C2<T>> t((size_t)ncomp * nx * E);
+    HIPCHK(hipMemcpyAsync(t.data(), d, t.size() * sizeof(C2<T>), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int c = 0; c < ncomp; ++c)
+      for (int k = 0; k < mz; ++k)
+        for (int j = 0; j < my2; ++j)
+          for (int i = 0; i < ni; ++i) {
+            size_t hi = ((((size_t)c * mz + k) * my2 + j) * ni + i) * 2;
+            const C2<T>& v = t[(((size_t)c * nx + (i == nx ? 0 : i)) * mz + k) * my2 + j];
+            h[hi] = (double)v.x;
+            h[hi + 1] = (double)v.y;
+          }
+  }
+  void get_real(const T* d, int ncomp, double* h, bool copy_plane, double sub[3] = nullptr) {
+    int ni = copy_plane ? nx + 1 : nx;
+    size_t P = (size_t)my * mz;
+    std::vector<T> t((size_t)ncomp * nx * P);
+    HIPCHK(hipMemcpyAsync(t.data(), d, t.size() * sizeof(T), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int c = 0; c < ncomp; ++c)
+      for (int k = 0; k < mz; ++k)
+        for (int j = 0; j < my; ++j)
+          for (int i = 0; i < ni; ++i) {
+            size_t hi = (((size_t)c * mz + k) * my + j) * ni + i;
+            double v = (double)t[(((size_t)c * nx + (i == nx ? 0 : i)) * mz + k) * my + j];
+            if (sub) v = (i == nx) ? 0.0 : v - sub[c];
+            h[hi] = v;
+          }
+  }
+  void put_real(T* d, int ncomp, const double* h, bool copy_plane) {
+    int ni = copy_plane ? nx + 1 : nx;
+    size_t P = (size_t)my * mz;
+    std::vector<T> t((size_t)ncomp * nx * P);
+    for (int c = 0; c < ncomp; ++c)
+      for (int k = 0; k < mz; ++k)
+        for (int j = 0; j < my; ++j)
+          for (int i = 0; i < nx; ++i)
+            t[(((size_t)c * nx + i) * mz + k) * my + j] = (T)h[(((size_t)c * mz + k) * my + j) * ni + i];
+    HIPCHK(hipMemcpyAsync(d, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  void set_field(int f, const double* h) {
+    switch (f) {
+      case PIB_F_U: put_spec(u, 3, h, true); break;
+      case PIB_F_P: put_spec(p, 1, h, false); break;
+      case PIB_F_NL: put_spec(nl, 3, h, true); nlbc_valid = false; break;
+      case PIB_F_ONL: put_spec(onl, 3, h, true); break;
+      case PIB_F_FF: put_spec(ff, 3, h, true); break;
+      case PIB_F_UBCP: put_real(ubcp, 3, h, true); break;
+      case PIB_F_PBCP: put_real(pbcp, 1, h, false); break;
+      default: throw PibFail(PIB_ERR_ARG, "field cannot be set");
+    }
+  }
+  void get_field(int f, double* h) {
+    switch (f) {
+      case PIB_F_U: get_spec(u, 3, h, true); break;
+      case PIB_F_P: get_spec(p, 1, h, false); break;
+      case PIB_F_NL: get_spec(nl, 3, h, true); break;
+      case PIB_F_ONL: get_spec(onl, 3, h, true); break;
+      case PIB_F_FF: get_spec(ff, 3, h, true); break;
+      case PIB_F_UBCP: get_real(ubcp, 3, h, true); break;
+      case PIB_F_PBCP: get_real(pbcp, 1, h, false); break;
+      case PIB_F_NLBCP: get_real(nlbc, 3, h, true); break;
+      case PIB_F_ONLBCP: get_real(onlbc, 3, h, true); break;
+      case PIB_F_DIFFN: get_real(diffn, 3, h, true); break;
+      case PIB_F_PPR: get_real(ppr, 3, h, false); break;
+      case PIB_F_OMEGA: get_real(omega, 3, h, true); break;
+      case PIB_F_FR: {
+        DevState s;
+        HIPCHK(hipMemcpy(&s, dst, sizeof s, hipMemcpyDeviceToHost));
+        get_real(fr, 3, h, true, s.s.frmean);
+        break;
+      }
+      case PIB_F_MASK: {
+        std::vector<uint8_t> t((size_t)nx * my * mz);
+        HIPCHK(hipMemcpy(t.data(), mask, t.size(), hipMemcpyDeviceToHost));
+        for (int k = 0; k < mz; ++k)
+          for (int j = 0; j < my; ++j)
+            for (int i = 0; i < nx; ++i)
+              h[((size_t)k * my + j) * nx + i] = t[((size_t)i * mz + k) * my + j] ? 1.0 : 0.0;
+        break;
+      }
+      default: throw PibFail(PIB_ERR_ARG, "unknown field");
+    }
+  }
+  void set_scalars(const pib_scalars* s) {
+    DevState h;
+    HIPCHK(hipMemcpy(&h, dst, sizeof h, hipMemcpyDeviceToHost));
+    double cs = h.s.count_solid, cf = h.s.count_fluid, mv = h.s.maxvolfrac;
+    h.s = *s;
+    if (have_particles) {
+      h.s.count_solid = cs;
+      h.s.count_fluid = cf;
+      h.s.maxvolfrac = mv;
+    }
+    HIPCHK(hipMemcpy(dst, &h, sizeof h, hipMemcpyHostToDevice));
+  }
+  void get_scalars(pib_scalars* s) {
+    DevState h;
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(&h, dst, sizeof h, hipMemcpyDeviceToHost));
+    *s = h.s;
+  }
+  void get_forces(double* f, double* pr, double* vi, double* tq) {
+    HIPCHK(hipStreamSynchronize(stream));
+    size_t n = (size_t)3 * nbody * sizeof(double);
+    if (nbody == 0) return;
+    if (f) HIPCHK(hipMemcpy(f, force, n, hipMemcpyDeviceToHost));
+    if (pr) HIPCHK(hipMemcpy(pr, pres, n, hipMemcpyDeviceToHost));
+    if (vi) HIPCHK(hipMemcpy(vi, visc, n, hipMemcpyDeviceToHost));
+    if (tq) HIPCHK(hipMemcpy(tq, torq, n, hipMemcpyDeviceToHost));
+  }
+
+  // ---------------- drop-in transforms ----------------
+  void ff2rc_planes(const double* a, long long lda, int np, double* b, long long ldb) {
+    size_t na = (size_t)np * my * mz, nb2 = (size_t)np * my2 * mz;
+    std::vector<T> ha(na);
+    for (int q = 0; q < np; ++q)
+      for (size_t e = 0; e < (size_t)my * mz; ++e) ha[(size_t)q * my * mz + e] = (T)a[(size_t)q * lda + e];
+    T* da = alloc<T>(na, true);
+    C2<T>* db = alloc<C2<T>>(nb2, true);
+    HIPCHK(hipMemcpyAsync(da, ha.data(), na * sizeof(T), hipMemcpyHostToDevice, stream));
+    auto kern = k_raw_r2c<T>;
+    set_lds(kern, lds_plane);
+    kern<<<np, nt_plane, lds_plane, stream>>>(G, tb, da, (long long)my * mz, db, (long long)my2 * mz);
+    HIPCHK(hipGetLastError());
+    std::vector<C2<T>> hb(nb2);
+    HIPCHK(hipMemcpyAsync(hb.data(), db, nb2 * sizeof(C2<T>), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int q = 0; q < np; ++q)
+      for (size_t e = 0; e < (size_t)my2 * mz; ++e) {
+        b[2 * ((size_t)q * ldb + e)] = (double)hb[(size_t)q * my2 * mz + e].x;
+        b[2 * ((size_t)q * ldb + e) + 1] = (double)hb[(size_t)q * my2 * mz + e].y;
+      }
+    free_tail(2);
+  }
+  void ff2cr_planes(const double* b, long long ldb, int np, double* a, long long lda) {
+    size_t na = (size_t)np * my * mz, nb2 = (size_t)np * my2 * mz;
+    std::vector<C2<T>> hb(nb2);
+    for (int q = 0; q < np; ++q)
+      for (size_t e = 0; e < (size_t)my2 * mz; ++e) {
+        hb[(size_t)q * my2 * mz + e].x = (T)b[2 * ((size_t)q * ldb + e)];
+        hb[(size_t)q * my2 * mz + e].y = (T)b[2 * ((size_t)q * ldb + e) + 1];
+      }
+    T* da = alloc<T>(na, true);
+    C2<T>* db = alloc<C2<T>>(nb2, true);
+    HIPCHK(hipMemcpyAsync(db, hb.data(), nb2 * sizeof(C2<T>), hipMemcpyHostToDevice, stream));
+    auto kern = k_raw_c2r<T>;
+    set_lds(kern, lds_plane);
+    kern<<<np, nt_plane, lds_plane, stream>>>(G, tb, db, (long long)my2 * mz, da, (long long)my * mz);
+    HIPCHK(hipGetLastError());
+    std::vector<T> ha(na);
+    HIPCHK(hipMemcpyAsync(ha.data(), da, na * sizeof(T), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int q = 0; q < np; ++q)
+      for (size_t e = 0; e < (size_t)my * mz; ++e) a[(size_t)q * lda + e] = (double)ha[(size_t)q * my * mz + e];
+    free_tail(2);
+  }
+  void free_tail(int n) {
+    for (int q = 0; q < n && !pallocs.empty(); ++q) {
+      (void)hipFree(pallocs.back());
+      pallocs.pop_back();
+    }
+  }
+};
+
+struct Ctx {
+  int prec;
+  Base* impl;
+  std::string err;
+};
+
+template <typename F>
+int guard(Ctx* c, F f) {
+  if (!c) return PIB_ERR_ARG;
+  try {
+    f();
+    return PIB_OK;
+  } catch (const PibFail& e) {
+    c->err = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    c->err = e.what();
+    return PIB_ERR_HIP;
+  }
+}
+
+template <typename F>
+int dispatch(void* ctx, F f) {
+  Ctx* c = (Ctx*)ctx;
+  return guard(c, [&] {
+    if (c->prec == PIB_FP64) f(*static_cast<Impl<double>*>(c->impl));
+    else f(*static_cast<Impl<float>*>(c->impl));
+  });
+}
+
+int finish_status(void* ctx, int st) {
+  if (st != PIB_OK) return st;
+  int dev = 0;
+  int r = dispatch(ctx, [&](auto& I) { dev = I.check_status(); });
+  if (r != PIB_OK) return r;
+  if (dev) {
+    Ctx* c = (Ctx*)ctx;
+    c->err = dev == PIB_ERR_BLOWUP ? "norm_drag > 1e6 (blow-up)" : "zero pivot in tridiagonal solve";
+    return dev;
+  }
+  return PIB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pib_create(const pib_config* cfg, void** ctx) {
+  if (!cfg || !ctx) return PIB_ERR_ARG;
+  *ctx = nullptr;
+  if (cfg->nx < 4 || cfg->my < 4 || cfg->mz < 4 || (cfg->my & 1) || (cfg->mz & 1)) return PIB_ERR_ARG;
+  if (cfg->dobnd) return PIB_ERR_UNSUPPORTED;
+  if (cfg->precision != PIB_FP64 && cfg->precision != PIB_FP32_BULK) return PIB_ERR_ARG;
+  Ctx* c = new Ctx{cfg->precision, nullptr, ""};
+  int st = guard(c, [&] {
+    if (cfg->precision == PIB_FP64) c->impl = new Impl<double>(*cfg);
+    else c->impl = new Impl<float>(*cfg);
+  });
+  if (st != PIB_OK) {
+    fprintf(stderr, "pib_create: %s\n", c->err.c_str());
+    delete c;
+    return st;
+  }
+  *ctx = c;
+  return PIB_OK;
+}
+int pib_destroy(void* ctx) {
+  Ctx* c = (Ctx*)ctx;
+  if (!c) return PIB_OK;
+  delete c->impl;
+  delete c;
+  return PIB_OK;
+}
+const char* pib_last_error(void* ctx) { return ctx ? ((Ctx*)ctx)->err.c_str() : "null context"; }
+int pib_device_info(char* name, int len, int* cu_count) {
+  hipDeviceProp_t prop;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return PIB_ERR_HIP;
+  if (name && len > 0) {
+    snprintf(name, len, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  if (cu_count) *cu_count = prop.multiProcessorCount;
+  return PIB_OK;
+}
+int pib_set_points(void* ctx, int nbnd, const double* bndpts, const double* cd) {
+  if (nbnd <= 0 || !bndpts || !cd) return PIB_ERR_ARG;
+  return dispatch(ctx, [&](auto& I) { I.set_points(nbnd, bndpts, cd); });
+}
+int pib_set_particles(void* ctx, int nbody, const double* xc, const double* radbdy, const double* velbdy,
+                      const double* angv, long long* count_solid, int* nrpr_active) {
+  if (nbody < 0 || (nbody > 0 && (!xc || !radbdy))) return PIB_ERR_ARG;
+  return dispatch(ctx, [&](auto& I) { I.set_particles(nbody, xc, radbdy, velbdy, angv, count_solid, nrpr_active); });
+}
+int pib_set_field(void* ctx, int field, const double* host) {
+  if (!host) return PIB_ERR_ARG;
+  return dispatch(ctx, [&](auto& I) { I.set_field(field, host); });
+}
+int pib_get_field(void* ctx, int field, double* host) {
+  if (!host) return PIB_ERR_ARG;
+  return dispatch(ctx, [&](auto& I) { I.get_field(field, host); });
+}
+int pib_set_scalars(void* ctx, const pib_scalars* s) {
+  if (!s) return PIB_ERR_ARG;
+  return dispatch(ctx, [&](auto& I) { I.set_scalars(s); });
+}
+int pib_get_scalars(void* ctx, pib_scalars* s) {
+  if (!s) return PIB_ERR_ARG;
+  return dispatch(ctx, [&](auto& I) { I.get_scalars(s); });
+}
+int pib_get_forces(void* ctx, double* force, double* pres, double* visc, double* torq) {
+  return dispatch(ctx, [&](auto& I) { I.get_forces(force, pres, visc, torq); });
+}
+int pib_compute_new_timestep(void* ctx, int rks) {
+  int st = dispatch(ctx, [&](auto& I) {
+    if (rks < 1 || rks > I.itrmax) throw PibFail(PIB_ERR_ARG, "rks out of range");
+    if (!I.have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
+    I.compute_new_timestep(rks);
+  });
+  return finish_status(ctx, st);
+}
+int pib_nonlinear(void* ctx, int rks) {
+  int st = dispatch(ctx, [&](auto& I) {
+    if (rks < 1 || rks > I.itrmax) throw PibFail(PIB_ERR_ARG, "rks out of range");
+    if (!I.have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
+    I.nonlinear(rks);
+  });
+  return finish_status(ctx, st);
+}
+int pib_velstep(void* ctx, int rks) {
+  int st = dispatch(ctx, [&](auto& I) {
+    if (rks < 1 || rks > I.itrmax) throw PibFail(PIB_ERR_ARG, "rks out of range");
+    if (!I.have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
+    I.velstep(rks);
+  });
+  return finish_status(ctx, st);
+}
+int pib_step_async(void* ctx, int nsteps) {
+  if (nsteps < 0) return PIB_ERR_ARG;
+  return dispatch(ctx, [&](auto& I) { I.step_async(nsteps); });
+}
+int pib_sync(void* ctx) { return finish_status(ctx, PIB_OK); }
+int pib_step(void* ctx, int nsteps) { return finish_status(ctx, pib_step_async(ctx, nsteps)); }
+int pib_phase_times(void* ctx, double* ms4) {
+  if (!ms4) return PIB_ERR_ARG;
+  return dispatch(ctx, [&](auto& I) { I.phase_times(ms4); });
+}
+int pib_ff2rc_planes(void* ctx, const double* a, long long lda, int nplanes, double* b, long long ldb) {
+  if (!a || !b || nplanes <= 0) return PIB_ERR_ARG;
+  return dispatch(ctx, [&](auto& I) {
+    if (lda < (long long)I.my * I.mz || ldb < (long long)I.my2 * I.mz) throw PibFail(PIB_ERR_ARG, "leading dim");
+    I.ff2rc_planes(a, lda, nplanes, b, ldb);
+  });
+}
+int pib_ff2cr_planes(void* ctx, const double* b, long long ldb, int nplanes, double* a, long long lda) {
+  if (!a || !b || nplanes <= 0) return PIB_ERR_ARG;
+  return dispatch(ctx, [&](auto& I) {
+    if (lda < (long long)I.my * I.mz || ldb < (long long)I.my2 * I.mz) throw PibFail(PIB_ERR_ARG, "leading dim");
+    I.ff2cr_planes(b, ldb, nplanes, a, lda);
+  });
+}
+int pib_tridiag_batched(void* ctx, int n, int nsys, const double* a, const double* b, const double* c,
+                        const double* r, double* x, int cyclic) {
+  if (n < 3 || nsys < 1 || !a || !b || !c || !r || !x) return PIB_ERR_ARG;
+  Ctx* cc = (Ctx*)ctx;
+  return guard(cc, [&] {
+    size_t N = (size_t)n * nsys;
+    double *d[6], *scr;
+    for (auto& q : d) HIPCHK(hipMalloc(&q, N * sizeof(double)));
+    HIPCHK(hipMalloc(&scr, 4 * N * sizeof(double)));
+    HIPCHK(hipMemcpy(d[0], a, N * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d[1], b, N * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d[2], c, N * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d[3], r, N * sizeof(double), hipMemcpyHostToDevice));
+    k_tridiag<<<(nsys + 255) / 256, 256>>>(n, nsys, d[0], d[1], d[2], d[3], d[4], cyclic, scr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(x, d[4], N * sizeof(double), hipMemcpyDeviceToHost));
+    for (auto& q : d) (void)hipFree(q);
+    (void)hipFree(scr);
+  });
+}
+
+}  // extern "C"

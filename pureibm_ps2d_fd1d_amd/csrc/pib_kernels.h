@@ -1,0 +1,935 @@
+// pib_kernels.h -- device kernels of the per-time-step fluid solve.
+//
+// Device layout (x planes slowest, so every (y,z) plane is one contiguous
+// block for the plane FFT, and the per-(ky,kz)-mode x-lines of the
+// tridiagonal solves are read mode-major, coalesced across threads):
+//   spectral  S[c][i][kz][ky]  complex, i = 0..nx-1 (the reference's plane
+//             nx+1 is the periodic copy of plane 1 and is not stored)
+//   physical  R[c][i][z][y]    real
+//   mask      M[i][z][y]       1 = fluid (fluid_atijk)
+// Grid positions follow the reference: velocity node i (1-based) sits at
+// x = i, pressure node i at x = i + 1/2; particle centres xc are in grid
+// units.
+#pragma once
+#include "pib_fft.h"
+#include "../../include/pureibm_gpu.h"
+
+#define PIB_SMALL_NUMBER ((double)1.0E-35f)
+
+// Device-resident scalar state: the reference's global_data scalars plus
+// the step-internal values the kernels hand to each other.
+struct DevState {
+  pib_scalars s;
+  double coef[3][4];
+  double norm_factor;
+  double dx, vis, cfl_max, ucharmod, char_length, dia, voldom, dx3, dr;
+  int nx, my, mz, nbody, nbnd, adaptive, soft_start, set_mpg, include_frmeanfluid, itrmax;
+  int revernorm;
+  int status;  // 0 ok, else a PIB_ERR_* code raised on the device
+};
+
+template <typename T>
+struct KArgs {
+  int nx, my, mz, my2, hy;
+  T dx, dx2, vis;
+  const T* wy;
+  const T* wz;
+  const T* w2;
+  PlaneGeom G;
+  PlaneTabs<T> tb;
+  C2<T>* u;
+  C2<T>* p;
+  C2<T>* nl;
+  C2<T>* onl;
+  C2<T>* ff;
+  T* ubcp;
+  T* pbcp;
+  T* nlbc;
+  T* onlbc;
+  T* diffn;
+  T* omega;
+  T* fr;
+  T* ppr;
+  const uint8_t* mask;
+  DevState* st;
+  double* part;  // per-workgroup partial results
+  int rks;
+  int xcd_map;   // map workgroups to planes so neighbouring planes share an XCD
+};
+
+template <typename T>
+__device__ __forceinline__ size_t sidx(const KArgs<T>& A, int c, int i, int ky, int kz) {
+  return (((size_t)c * A.nx + i) * A.mz + kz) * A.my2 + ky;
+}
+template <typename T>
+__device__ __forceinline__ size_t ridx(const KArgs<T>& A, int c, int i, int y, int z) {
+  return (((size_t)c * A.nx + i) * A.mz + z) * A.my + y;
+}
+__device__ __forceinline__ int wrapn(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// Plane index of this workgroup.  With xcd_map, the round-robin dispatch of
+// consecutive workgroups over the 8 XCDs is undone so that one XCD works on a
+// contiguous run of planes (their x-neighbours then hit the same L2).
+template <typename T>
+__device__ __forceinline__ int plane_of(const KArgs<T>& A) {
+  int b = blockIdx.x;
+  if (A.xcd_map && (A.nx % 8) == 0) {
+    int per = A.nx / 8;
+    return (b % 8) * per + b / 8;
+  }
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+// deterministic block reductions (power-of-two block sizes)
+// ---------------------------------------------------------------------------
+__device__ inline double block_sum(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+__device__ inline double block_max(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// ===========================================================================
+// c2r ops (physical fields from spectral ones)
+// ===========================================================================
+// compute_new_timestep (dependent_functions.F90:1088-1122): ubcp = c2r(u) +
+// umean, with the fluid/solid/total sums and max|u| of the same loop.
+template <typename T>
+struct OpUbcp {
+  const KArgs<T>& A;
+  int c, i;
+  double sf, ss, sa, mx, um;
+  __device__ OpUbcp(const KArgs<T>& a, int c_, int i_) : A(a), c(c_), i(i_), sf(0), ss(0), sa(0), mx(PIB_SMALL_NUMBER) {
+    um = A.st->s.umean[c];
+  }
+  __device__ C2<T> load(int ky, int kz) { return A.u[sidx(A, c, i, ky, kz)]; }
+  __device__ void one(int y, int z, T v) {
+    T w = v + (T)um;
+    A.ubcp[ridx(A, c, i, y, z)] = w;
+    double d = (double)w;
+    if (A.mask[((size_t)i * A.mz + z) * A.my + y]) sf += d;
+    else ss += d;
+    sa += d;
+    mx = fmax(mx, fabs(d));
+  }
+  __device__ void store2(int, int y, int z, T a, T b) {
+    one(y, z, a);
+    one(y + 1, z, b);
+  }
+};
+// plain copy-out with an optional additive device scalar
+template <typename T>
+struct OpPlain {
+  const C2<T>* src;
+  T* dst;
+  const KArgs<T>& A;
+  size_t sbase, dbase;
+  T add;
+  __device__ C2<T> load(int ky, int kz) { return src[sbase + (size_t)kz * A.my2 + ky]; }
+  __device__ void store2(int, int y, int z, T a, T b) {
+    size_t o = dbase + (size_t)z * A.my + y;
+    dst[o] = a + add;
+    dst[o + 1] = b + add;
+  }
+};
+// calc_visc (boundary_condition.F90:1767-1815):
+// diffn = c2r( vis*((1/dx2)*(u(i-1) - 2u(i) + u(i+1)) - w2 u(i)) )
+template <typename T>
+struct OpVisc {
+  const KArgs<T>& A;
+  int c, i;
+  __device__ C2<T> load(int ky, int kz) {
+    int im = wrapn(i - 1, A.nx), ip = wrapn(i + 1, A.nx);
+    C2<T> um = A.u[sidx(A, c, im, ky, kz)], uc = A.u[sidx(A, c, i, ky, kz)], up = A.u[sidx(A, c, ip, ky, kz)];
+    C2<T> d2 = (um - cscal((T)2, uc)) + up;
+    C2<T> f = cscal((T)1 / A.dx2, d2);
+    f = f - cscal(A.w2[(size_t)kz * A.my2 + ky], uc);
+    return cscal(A.vis, f);
+  }
+  __device__ void store2(int, int y, int z, T a, T b) {
+    size_t o = ridx(A, c, i, y, z);
+    A.diffn[o] = a;
+    A.diffn[o + 1] = b;
+  }
+};
+// compute_omega (boundary_condition.F90:1648-1765); omega_1 is transformed as
+// one combined spectrum wy*u3 - wz*u2 (dwdy - dvdz by linearity).
+template <typename T>
+struct OpOmega {
+  const KArgs<T>& A;
+  int c, i;
+  __device__ C2<T> load(int ky, int kz) {
+    T wy = A.wy[ky], wz = A.wz[kz];
+    if (c == 0) return cik(wy, A.u[sidx(A, 2, i, ky, kz)]) - cik(wz, A.u[sidx(A, 1, i, ky, kz)]);
+    if (c == 1) return cik(wz, A.u[sidx(A, 0, i, ky, kz)]);  // dudz
+    return cik(wy, A.u[sidx(A, 0, i, ky, kz)]);              // dudy
+  }
+  __device__ T val(int y, int z, T v) {
+    if (c == 0) return v;
+    int im = wrapn(i - 1, A.nx), ip = wrapn(i + 1, A.nx);
+    T twodx = (T)2 * A.dx;
+    if (c == 1) return v - (A.ubcp[ridx(A, 2, ip, y, z)] - A.ubcp[ridx(A, 2, im, y, z)]) / twodx;
+    return (A.ubcp[ridx(A, 1, ip, y, z)] - A.ubcp[ridx(A, 1, im, y, z)]) / twodx - v;
+  }
+  __device__ void store2(int, int y, int z, T a, T b) {
+    size_t o = ridx(A, c, i, y, z);
+    A.omega[o] = val(y, z, a);
+    A.omega[o + 1] = val(y + 1, z, b);
+  }
+};
+// calc_pgrad (boundary_condition.F90:1817-1896): grad p at pressure points + mpg
+template <typename T>
+struct OpPgrad {
+  const KArgs<T>& A;
+  int c, i;
+  T add;
+  __device__ C2<T> load(int ky, int kz) {
+    if (c == 0) {
+      int im = wrapn(i - 1, A.nx), ip = wrapn(i + 1, A.nx);
+      C2<T> d = A.p[sidx(A, 0, ip, ky, kz)] - A.p[sidx(A, 0, im, ky, kz)];
+      T twodx = (T)2 * A.dx;
+      return mk(d.x / twodx, d.y / twodx);
+    }
+    C2<T> pv = A.p[sidx(A, 0, i, ky, kz)];
+    return c == 1 ? cik(A.wy[ky], pv) : cik(A.wz[kz], pv);
+  }
+  __device__ void store2(int, int y, int z, T a, T b) {
+    size_t o = ridx(A, c, i, y, z);
+    A.ppr[o] = a + add;
+    A.ppr[o + 1] = b + add;
+  }
+};
+
+// generic c2r launcher body: grid (nplanes, ncomp)
+enum { C2R_UBCP = 0, C2R_PBCP, C2R_NLBC, C2R_ONLBC, C2R_VISC, C2R_OMEGA, C2R_PGRAD };
+
+template <typename T, int KIND>
+__global__ void k_c2r(KArgs<T> A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  C2<T>* buf = (C2<T>*)smem;
+  const int i = plane_of(A);
+  const int c = blockIdx.y;
+  if constexpr (KIND == C2R_UBCP) {
+    OpUbcp<T> op(A, c, i);
+    plane_c2r<0>(buf, A.G, A.tb, op);
+    double* sh = (double*)smem;
+    double sf = block_sum(op.sf, sh);
+    double ss = block_sum(op.ss, sh);
+    double sa = block_sum(op.sa, sh);
+    double mx = block_max(op.mx, sh);
+    if (threadIdx.x == 0) {
+      double* o = A.part + ((size_t)c * A.nx + i) * 4;
+      o[0] = sf; o[1] = ss; o[2] = sa; o[3] = mx;
+    }
+  } else if constexpr (KIND == C2R_PBCP) {
+    OpPlain<T> op{A.p, A.pbcp, A, (size_t)i * A.mz * A.my2, (size_t)i * A.mz * A.my, (T)0};
+    plane_c2r<0>(buf, A.G, A.tb, op);
+  } else if constexpr (KIND == C2R_NLBC || KIND == C2R_ONLBC) {
+    const C2<T>* src = KIND == C2R_NLBC ? A.nl : A.onl;
+    T* dst = KIND == C2R_NLBC ? A.nlbc : A.onlbc;
+    OpPlain<T> op{src, dst, A, ((size_t)c * A.nx + i) * A.mz * A.my2, ((size_t)c * A.nx + i) * A.mz * A.my, (T)0};
+    plane_c2r<0>(buf, A.G, A.tb, op);
+  } else if constexpr (KIND == C2R_VISC) {
+    OpVisc<T> op{A, c, i};
+    plane_c2r<0>(buf, A.G, A.tb, op);
+  } else if constexpr (KIND == C2R_OMEGA) {
+    OpOmega<T> op{A, c, i};
+    plane_c2r<0>(buf, A.G, A.tb, op);
+  } else {
+    OpPgrad<T> op{A, c, i, (T)A.st->s.mpg[c]};
+    plane_c2r<0>(buf, A.G, A.tb, op);
+  }
+}
+
+// ===========================================================================
+// r2c: fr - frmean -> ff (boundary_condition.F90:678-701)
+// ===========================================================================
+template <typename T>
+struct OpForce {
+  const KArgs<T>& A;
+  int c, i;
+  T fm;
+  __device__ void load2(int y, int z, T& a, T& b) {
+    size_t o = ridx(A, c, i, y, z);
+    a = A.fr[o] - fm;
+    b = A.fr[o + 1] - fm;
+  }
+  __device__ void store(int, int ky, int kz, C2<T> v) { A.ff[sidx(A, c, i, ky, kz)] = v; }
+};
+template <typename T>
+__global__ void k_r2c_force(KArgs<T> A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  C2<T>* buf = (C2<T>*)smem;
+  const int i = plane_of(A);
+  const int c = blockIdx.y;
+  OpForce<T> op{A, c, i, (T)A.st->s.frmean[c]};
+  plane_r2c<0>(buf, A.G, A.tb, op);
+}
+
+// ===========================================================================
+// form_nl (nl_allflow.F90:71-327) + the sign flip of nonlinear (nl.F90:141-153)
+// One workgroup per (plane i, component c): three forward transforms
+// accumulated in registers, nl_c = -( wy F(u_c v) + wz F(u_c w) + F(D_c) ),
+// D_c = x-derivative of the x-flux of u_c.
+// ===========================================================================
+template <typename T, bool UPWIND, bool MINMOD>
+struct NlFlux {
+  const KArgs<T>& A;
+  __device__ T u(int c, int i, int y, int z) const { return A.ubcp[ridx(A, c, wrapn(i, A.nx), y, z)]; }
+  // MUSCL slope of component c in cell i (nl_allflow.F90:171-207)
+  __device__ T slope(int c, int i, int y, int z) const {
+    T um = u(c, i - 1, y, z), uc = u(c, i, y, z), up = u(c, i + 1, y, z);
+    T s = (up - um) / ((T)2 * A.dx);
+    T phi = (T)1;
+    if (MINMOD) {
+      phi = (uc - um) / (up - uc + (T)PIB_SMALL_NUMBER);
+      if (phi <= (T)0) phi = (T)0;
+      else phi = fmin((T)1, phi);
+    }
+    return s * phi * (T)1;
+  }
+  // Rusanov flux of component c at interface i+1/2 (nl_allflow.F90:212-294)
+  __device__ T flux(int c, int i, int y, int z) const {
+    const T h = A.dx / (T)2;
+    T s1i = slope(0, i, y, z), s1p = slope(0, i + 1, y, z);
+    T uminus = u(0, i, y, z) + s1i * h;
+    T uplus = u(0, i + 1, y, z) - s1p * h;
+    T Um, Up;
+    if (c == 0) {
+      Um = uminus;
+      Up = uplus;
+    } else {
+      Um = u(c, i, y, z) + slope(c, i, y, z) * h;
+      Up = u(c, i + 1, y, z) - slope(c, i + 1, y, z) * h;
+    }
+    T a = (T)0.5 * uminus * Um;
+    T b = (T)0.5 * uplus * Up;
+    T ms = fmax(fabs(uminus), fabs(uplus));
+    return (a + b) - (T)0.5 * ms * (Up - Um);
+  }
+  __device__ T dx_flux(int c, int i, int y, int z) const {
+    if (UPWIND) return (flux(c, i, y, z) - flux(c, i - 1, y, z)) / A.dx;
+    T fp = u(0, i + 1, y, z) * u(c, i + 1, y, z);
+    T fm = u(0, i - 1, y, z) * u(c, i - 1, y, z);
+    return (fp - fm) / ((T)2 * A.dx);
+  }
+};
+
+template <typename T, int NIT, bool UPWIND, bool MINMOD>
+struct OpNl {
+  const KArgs<T>& A;
+  int c, i, stage;  // stage 0: y-product, 1: z-product, 2: x-flux derivative
+  C2<T> acc[NIT];
+  NlFlux<T, UPWIND, MINMOD> fx;
+  __device__ void load2(int y, int z, T& a, T& b) {
+    a = val(y, z);
+    b = val(y + 1, z);
+  }
+  __device__ T val(int y, int z) const {
+    size_t o = ridx(A, 0, i, y, z);
+    size_t cs = (size_t)A.nx * A.mz * A.my;
+    T u1 = A.ubcp[o], u2 = A.ubcp[o + cs], u3 = A.ubcp[o + 2 * cs];
+    if (stage == 0) return c == 0 ? u1 * u2 : (c == 1 ? u2 * u2 : u2 * u3);
+    if (stage == 1) return c == 0 ? u1 * u3 : (c == 1 ? u2 * u3 : u3 * u3);
+    return fx.dx_flux(c, i, y, z);
+  }
+  __device__ void store(int it, int ky, int kz, C2<T> v) {
+    if (stage == 0) acc[it] = cik(A.wy[ky], v);
+    else if (stage == 1) acc[it] = acc[it] + cik(A.wz[kz], v);
+    else acc[it] = acc[it] + v;
+  }
+};
+
+template <typename T, int NIT, bool UPWIND, bool MINMOD>
+__global__ void __launch_bounds__(1024) k_form_nl(KArgs<T> A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  C2<T>* buf = (C2<T>*)smem;
+  const int i = plane_of(A);
+  const int c = blockIdx.y;
+  OpNl<T, NIT, UPWIND, MINMOD> op{A, c, i, 0, {}, {A}};
+  for (int s = 0; s < 3; ++s) {
+    op.stage = s;
+    plane_r2c<NIT>(buf, A.G, A.tb, op);
+  }
+  const int nt = blockDim.x;
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    int e = threadIdx.x + it * nt;
+    if (e < A.my2 * A.mz) {
+      int kz = e / A.my2, ky = e - kz * A.my2;
+      C2<T> v = op.acc[it];
+      A.nl[sidx(A, c, i, ky, kz)] = mk(-v.x, -v.y);
+    }
+  }
+}
+
+// ===========================================================================
+// IBM: trilinear ('lpi' order 2) stencils, dependent_functions.F90:45-275,
+// 612-781; interpolation.F90:202-433, 740-751.  For order 2 with periodic
+// x, y, z the stencil base is the cell pc itself; nodes pc, pc+1 wrapped by
+// one period into 1..n.
+// ===========================================================================
+struct Sten {
+  int ii[2], jj[2], kk[2];  // 0-based wrapped node indices
+  double w[8];              // w[(k*2+j)*2+i] = xval(i)*yval(j)*zval(k)
+};
+__device__ inline int cellof(double x) { return x < 0.0 ? (int)(x - 1.0) : (int)x; }
+// one-period wrap of the reference, written as a full modulo so that any
+// input maps into 1..n
+__device__ inline int wrap1(int v, int n) {
+  int r = (v - 1) % n;
+  if (r < 0) r += n;
+  return r + 1;
+}
+__device__ inline void make_sten(Sten& s, const double pos[3], int nx, int my, int mz) {
+  int pc[3] = {cellof(pos[0]), cellof(pos[1]), cellof(pos[2])};
+  double zx = pos[0] - (double)pc[0], zy = pos[1] - (double)pc[1], zz = pos[2] - (double)pc[2];
+  double xv[2] = {1.0 - zx, zx}, yv[2] = {1.0 - zy, zy}, zv[2] = {1.0 - zz, zz};
+  for (int q = 0; q < 2; ++q) {
+    s.ii[q] = wrap1(pc[0] + q, nx) - 1;
+    s.jj[q] = wrap1(pc[1] + q, my) - 1;
+    s.kk[q] = wrap1(pc[2] + q, mz) - 1;
+  }
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i) s.w[(k * 2 + j) * 2 + i] = xv[i] * yv[j] * zv[k];
+}
+// interp_threed_scalar order (k, j, i)
+template <typename T>
+__device__ inline double interp_kji(const Sten& s, const T* f, int nx, int my, int mz) {
+  double acc = 0.0;
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i)
+        acc = acc + (double)f[((size_t)s.ii[i] * mz + s.kk[k]) * my + s.jj[j]] * s.w[(k * 2 + j) * 2 + i];
+  return acc;
+}
+// array_dot_product_3d order (i, j, k), general_funcs.F90:126-143
+template <typename T>
+__device__ inline double interp_ijk(const Sten& s, const T* f, int nx, int my, int mz) {
+  double acc = 0.0;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k)
+        acc = acc + (double)f[((size_t)s.ii[i] * mz + s.kk[k]) * my + s.jj[j]] * s.w[(k * 2 + j) * 2 + i];
+  return acc;
+}
+
+struct Particles {
+  const double* xc;   // (nbody,3) column-major
+  const double* rad;  // radbdy
+  const double* radi; // radibdy
+  const double* rado; // radobdy
+  const double* vel;  // (nbody,3)
+  const double* angv; // (nbody,3)
+  const double* bnd;  // (3,nbnd)
+  const double* cd;   // (3,nbnd)
+  const uint8_t* if_rev;     // (nbnd, nbody)
+  const int* nrpr_active;
+  int nbody, nbnd;
+};
+
+// calc_pres_visc_drag (boundary_condition.F90:1481-1645): per surface point
+// contributions pres(3), visc(3), torq(3) to per-body sums.
+template <typename T>
+__global__ void k_drag_points(KArgs<T> A, Particles P, double* out) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= P.nbody * P.nbnd) return;
+  int m = idx / P.nbnd, l = idx - m * P.nbnd;
+  const DevState* st = A.st;
+  double rb = P.rad[m];
+  double xl[3], xpb[3];
+  for (int n = 0; n < 3; ++n) xl[n] = P.xc[n * P.nbody + m] + P.bnd[l * 3 + n] * rb;
+  xpb[0] = xl[0] - 0.5;
+  xpb[1] = xl[1];
+  xpb[2] = xl[2];
+  Sten sp, sv;
+  make_sten(sp, xpb, A.nx, A.my, A.mz);
+  make_sten(sv, xl, A.nx, A.my, A.mz);
+  double pl = interp_ijk(sp, A.pbcp, A.nx, A.my, A.mz);
+  size_t cs = (size_t)A.nx * A.mz * A.my;
+  double vort[3];
+  for (int n = 0; n < 3; ++n) vort[n] = interp_kji(sv, A.omega + n * cs, A.nx, A.my, A.mz);
+  double cd1 = P.cd[l * 3 + 0], cd2 = P.cd[l * 3 + 1], cd3 = P.cd[l * 3 + 2];
+  double df[3];
+  df[0] = -(vort[2] * cd2 - vort[1] * cd3);
+  df[1] = -(vort[0] * cd3 - vort[2] * cd1);
+  df[2] = -(vort[1] * cd1 - vort[0] * cd2);
+  double rdx = rb * st->dx;
+  double da1 = 4.0 * 3.14159265358979323846 * (rdx * rdx) / (double)P.nbnd;
+  double cdv[3] = {cd1, cd2, cd3};
+  double tf[3];
+  double* o = out + (size_t)idx * 9;
+  for (int d = 0; d < 3; ++d) {
+    o[d] = pl * cdv[d] * da1;              // subtracted from pres(m), added to pres_total
+    tf[d] = df[d] * st->vis * da1;
+    o[3 + d] = tf[d];
+  }
+  double b1 = P.bnd[l * 3], b2 = P.bnd[l * 3 + 1], b3 = P.bnd[l * 3 + 2];
+  o[6] = (b2 * tf[2] - b3 * tf[1]) * rb * st->dx;
+  o[7] = (b3 * tf[0] - b1 * tf[2]) * rb * st->dx;
+  o[8] = (b1 * tf[1] - b2 * tf[0]) * rb * st->dx;
+}
+
+// per-body sums of the point contributions, one workgroup per body
+__global__ void k_drag_body(const double* pts, int nbnd, int nbody, int rks, double* pres, double* visc,
+                            double* torq, double* body_tot) {
+  __shared__ double sh[256];
+  int m = blockIdx.x;
+  double acc[9];
+  for (int q = 0; q < 9; ++q) acc[q] = 0.0;
+  for (int l = threadIdx.x; l < nbnd; l += blockDim.x) {
+    const double* o = pts + ((size_t)m * nbnd + l) * 9;
+    for (int q = 0; q < 9; ++q) acc[q] += o[q];
+  }
+  double tot[9];
+  for (int q = 0; q < 9; ++q) tot[q] = block_sum(acc[q], sh);
+  if (threadIdx.x == 0) {
+    for (int d = 0; d < 3; ++d) {
+      size_t k = (size_t)d * nbody + m;
+      double p0 = rks == 1 ? 0.0 : pres[k];
+      double v0 = rks == 1 ? 0.0 : visc[k];
+      double t0 = rks == 1 ? 0.0 : torq[k];
+      pres[k] = p0 - tot[d];
+      visc[k] = v0 + tot[3 + d];
+      torq[k] = t0 + tot[6 + d];
+      body_tot[(size_t)m * 6 + d] = tot[d];
+      body_tot[(size_t)m * 6 + 3 + d] = tot[3 + d];
+    }
+  }
+}
+
+// calc_inner_reversal_forcing (boundary_condition.F90:1104-1479), serial path:
+// per active reversal pair, force_tmp and the solid-node redistribution share.
+// out[p*12 + 0..2] force_tmp, [3..5] force_dist, [6..8] frmean part,
+// [9..11] frmeanfluid part.
+template <typename T>
+__global__ void k_forcing_points(KArgs<T> A, Particles P, double* out) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= P.nbody * P.nbnd) return;
+  int m = idx / P.nbnd, l = idx - m * P.nbnd;
+  double* o = out + (size_t)idx * 12;
+  if (!P.if_rev[(size_t)m * P.nbnd + l]) {
+    for (int q = 0; q < 12; ++q) o[q] = 0.0;
+    return;
+  }
+  const DevState* st = A.st;
+  const int rks = A.rks;
+  const double c1 = st->coef[rks - 1][0], c2 = st->coef[rks - 1][1], c3 = st->coef[rks - 1][2],
+               c4 = st->coef[rks - 1][3];
+  double rb = P.rad[m], ri = P.radi[m], ro = P.rado[m];
+  double xli[3], xlo[3], rad = 0.0;
+  for (int n = 0; n < 3; ++n) {
+    double b = P.bnd[l * 3 + n];
+    double xcn = P.xc[n * P.nbody + m];
+    xli[n] = xcn + b * ri;
+    xlo[n] = xcn + b * ro;
+    rad = rad + (b * ri) * (b * ri);
+  }
+  rad = sqrt(rad);
+  double snorm[3];
+  for (int n = 0; n < 3; ++n) snorm[n] = (P.bnd[l * 3 + n] * ri) / rad;
+  double xpi[3] = {xli[0] - 0.5, xli[1], xli[2]};
+  size_t cs = (size_t)A.nx * A.mz * A.my;
+  Sten so, sp, si;
+  make_sten(so, xlo, A.nx, A.my, A.mz);
+  make_sten(sp, xpi, A.nx, A.my, A.mz);
+  make_sten(si, xli, A.nx, A.my, A.mz);
+  double ulo[3], uli[3], nll[3], onll[3], dfll[3], ppll[3];
+  for (int n = 0; n < 3; ++n) {
+    ulo[n] = interp_kji(so, A.ubcp + n * cs, A.nx, A.my, A.mz);
+    ppll[n] = interp_kji(sp, A.ppr + n * cs, A.nx, A.my, A.mz);
+    uli[n] = interp_kji(si, A.ubcp + n * cs, A.nx, A.my, A.mz);
+    nll[n] = interp_ijk(si, A.nlbc + n * cs, A.nx, A.my, A.mz);
+    onll[n] = interp_ijk(si, A.onlbc + n * cs, A.nx, A.my, A.mz);
+    dfll[n] = interp_ijk(si, A.diffn + n * cs, A.nx, A.my, A.mz);
+  }
+  double w1 = P.angv[m], w2 = P.angv[P.nbody + m], w3 = P.angv[2 * P.nbody + m];
+  double crossp[3] = {w2 * snorm[2] - w3 * snorm[1], w3 * snorm[0] - w1 * snorm[2], w1 * snorm[1] - w2 * snorm[0]};
+  double linvel[3], unmag = 0.0, ubnmag = 0.0;
+  for (int n = 0; n < 3; ++n) linvel[n] = P.vel[n * P.nbody + m] + crossp[n] * rb * st->dx;
+  for (int d = 0; d < 3; ++d) {
+    unmag = unmag + snorm[d] * ulo[d];
+    ubnmag = ubnmag + snorm[d] * linvel[d];
+  }
+  double ucar[3];
+  for (int d = 0; d < 3; ++d) {
+    double unorm = snorm[d] * unmag;
+    double utang = ulo[d] - unorm;
+    double ubnorm = snorm[d] * ubnmag;
+    double ubtang = linvel[d] - ubnorm;
+    double utangi = (ubtang * (ro - ri) - utang * (rb - ri)) / (ro - rb);
+    double unormi = 0.0;
+    if (st->revernorm == 1) unormi = (ubnorm * (ro - ri) - unorm * (rb - ri)) / (ro - rb);
+    ucar[d] = utangi + unormi;
+  }
+  double rdi = ri * st->dx;
+  double da2 = 4.0 * 3.14159265358979323846 * (rdi * rdi) / (double)P.nrpr_active[m];
+  double ft[3];
+  for (int n = 0; n < 3; ++n) {
+    ft[n] = st->s.cf * (-ucar[n] + uli[n]);
+    ft[n] = ft[n] - c3 * nll[n] - c4 * onll[n];
+    ft[n] = ft[n] + (c1 + c2) * (ppll[n] - dfll[n]);
+    ft[n] = ft[n] * da2 * st->dr * st->dx;
+  }
+  int count_fl = 0;
+  double force_fl[3] = {0, 0, 0}, fm[3] = {0, 0, 0}, fmf[3] = {0, 0, 0};
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i) {
+        bool fl = A.mask[((size_t)si.ii[i] * A.mz + si.kk[k]) * A.my + si.jj[j]] != 0;
+        double w = si.w[(k * 2 + j) * 2 + i];
+        if (fl) count_fl++;
+        for (int n = 0; n < 3; ++n) {
+          double tp = w * ft[n];
+          if (fl) force_fl[n] = force_fl[n] + tp / st->dx3;
+          if (fl) fmf[n] = fmf[n] + tp / st->dx3;
+          fm[n] = fm[n] + tp / st->dx3;
+        }
+      }
+  int count_so = 8 - count_fl;
+  for (int n = 0; n < 3; ++n) {
+    o[n] = ft[n];
+    o[3 + n] = count_so > 0 ? force_fl[n] / (double)count_so : 0.0;
+    o[6 + n] = fm[n];
+    o[9 + n] = fmf[n];
+  }
+}
+
+// Atomics-free spreading: one thread per target grid node sums, in the
+// reference's (body, point, corner) order, the contributions of the records
+// binned to it at set-up (sorted by node, stable).
+template <typename T>
+__global__ void k_spread(KArgs<T> A, int ntarget, const int* tnode, const int* toff, const int* rpt,
+                         const double* rw, const uint8_t* rdist, const double* pts) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntarget) return;
+  const double dx3 = A.st->dx3;
+  double acc[3] = {0, 0, 0};
+  for (int r = toff[t]; r < toff[t + 1]; ++r) {
+    const double* o = pts + (size_t)rpt[r] * 12;
+    double w = rw[r];
+    bool d = rdist[r] != 0;
+    for (int n = 0; n < 3; ++n) {
+      acc[n] = acc[n] + (w * o[n]) / dx3;
+      if (d) acc[n] = acc[n] + o[3 + n];
+    }
+  }
+  size_t cs = (size_t)A.nx * A.mz * A.my;
+  int node = tnode[t];
+  for (int n = 0; n < 3; ++n) A.fr[n * cs + node] = (T)acc[n];
+}
+
+// ===========================================================================
+// velstep (steptheflow.F90:52-160): per (ky,kz) mode, one thread runs
+// pressure (poisson.F90:55-239), ustep (usteptime.F90:45-258) and divcorr
+// (poisson.F90:241-508) along x.  Thomas sweeps spill their intermediates to
+// mode-major scratch (coalesced across threads).  The (0,0) mode always runs
+// in double precision.
+// ===========================================================================
+template <typename T, typename R>
+struct ModeIO {
+  const KArgs<T>& A;
+  int ky, kz;
+  size_t base, step;  // element (c,i) at base + (c*nx + i)*step
+  __device__ C2<R> get(const C2<T>* f, int c, int i) const {
+    C2<T> v = f[base + ((size_t)c * A.nx + i) * step];
+    return mk((R)v.x, (R)v.y);
+  }
+  __device__ void put(C2<T>* f, int c, int i, C2<R> v) const {
+    f[base + ((size_t)c * A.nx + i) * step] = mk((T)v.x, (T)v.y);
+  }
+};
+
+template <typename R>
+__device__ __forceinline__ C2<R> ciR(R k, C2<R> z) { return mk(-k * z.y, k * z.x); }
+
+// Forward Thomas sweep with the Sherman-Morrison second rhs (auxu = e_1 + e_n)
+// for rows r(j) supplied by `rhs(j)`; writes y1 (re), y3 (im), y2 (auxz), gam
+// to scratch.  b(0), b(n-1) already corrected.  Returns the pivot status.
+template <typename R, class RHS>
+__device__ int thomas3(int n, R a, R b, R b0, R bn, R c, RHS rhs, R* g_s, R* y1_s, R* y2_s, R* y3_s,
+                       size_t ms, bool store_aux) {
+  int st = 0;
+  R bet = b0;
+  if (bet == (R)0) st = PIB_ERR_PIVOT;
+  C2<R> r0 = rhs(0);
+  R y1 = r0.x / bet, y2 = (R)1 / bet, y3 = r0.y / bet;
+  y1_s[0] = y1;
+  y3_s[0] = y3;
+  if (store_aux) { y2_s[0] = y2; g_s[0] = (R)0; }
+  for (int j = 1; j < n; ++j) {
+    R gam = c / bet;
+    R bj = (j == n - 1) ? bn : b;
+    bet = bj - a * gam;
+    if (bet == (R)0) { bet = (R)1.0e-30; st = PIB_ERR_PIVOT; }
+    C2<R> rj = rhs(j);
+    R r2 = (j == n - 1) ? (R)1 : (R)0;
+    y1 = (rj.x - a * y1) / bet;
+    y3 = (rj.y - a * y3) / bet;
+    y1_s[(size_t)j * ms] = y1;
+    y3_s[(size_t)j * ms] = y3;
+    if (store_aux) {
+      y2 = (r2 - a * y2) / bet;
+      y2_s[(size_t)j * ms] = y2;
+      g_s[(size_t)j * ms] = gam;
+    }
+  }
+  return st;
+}
+// backward substitution of the (re, im[, aux]) solutions in scratch
+template <typename R>
+__device__ void thomas_back(int n, const R* g_s, R* y1_s, R* y2_s, R* y3_s, size_t ms, bool aux) {
+  R n1 = y1_s[(size_t)(n - 1) * ms], n3 = y3_s[(size_t)(n - 1) * ms];
+  R n2 = aux ? y2_s[(size_t)(n - 1) * ms] : (R)0;
+  for (int j = n - 2; j >= 0; --j) {
+    R g = g_s[(size_t)(j + 1) * ms];
+    R v1 = y1_s[(size_t)j * ms] - g * n1;
+    R v3 = y3_s[(size_t)j * ms] - g * n3;
+    y1_s[(size_t)j * ms] = v1;
+    y3_s[(size_t)j * ms] = v3;
+    n1 = v1;
+    n3 = v3;
+    if (aux) {
+      R v2 = y2_s[(size_t)j * ms] - g * n2;
+      y2_s[(size_t)j * ms] = v2;
+      n2 = v2;
+    }
+  }
+}
+
+// scr: six scratch vectors [s][i][mode] with mode stride ms; qo is this
+// mode's column in them.
+template <typename T, typename R>
+__device__ int run_mode(const KArgs<T>& A, int q, R* scr, size_t ms, size_t qo, double& divmax) {
+  const int nx = A.nx;
+  const int ky = q % A.my2, kz = q / A.my2;
+  const DevState* st = A.st;
+  const int rks = A.rks;
+  const R c1 = (R)st->coef[rks - 1][0], c2 = (R)st->coef[rks - 1][1], c3 = (R)st->coef[rks - 1][2],
+          c4 = (R)st->coef[rks - 1][3];
+  const R dx = (R)st->dx, dx2 = dx * dx, dt = (R)st->s.dt, vis = (R)st->vis;
+  const R wy = (R)A.wy[ky], wz = (R)A.wz[kz], w2 = (R)A.w2[(size_t)kz * A.my2 + ky];
+  ModeIO<T, R> io{A, ky, kz, (size_t)kz * A.my2 + ky, (size_t)A.mz * A.my2};
+  // scratch: 0 gam_p, 1 auxz_p, 2 gam_h, 3 auxz_h, 4 y_re, 5 y_im
+  R* gp = scr + 0 * nx * ms + qo;
+  R* zp = scr + 1 * nx * ms + qo;
+  R* gh = scr + 2 * nx * ms + qo;
+  R* zh = scr + 3 * nx * ms + qo;
+  R* yr = scr + 4 * nx * ms + qo;
+  R* yi = scr + 5 * nx * ms + qo;
+  const bool mean = (ky == 0 && kz == 0);
+  int status = 0;
+  // ---------------- pressure (poisson.F90:55-239) ----------------
+  const R ap = (R)1 / dx2 - (R)0.25 * w2;
+  const R bp = (R)-2 / dx2 - (R)0.5 * w2;
+  const R cp = ap;
+  auto prhs = [&](int i) -> C2<R> {
+    int i1 = wrapn(i + 1, nx);
+    C2<R> t = mk((R)0, (R)0);
+    C2<R> nl1 = io.get(A.nl, 0, i1) - io.get(A.nl, 0, i);
+    C2<R> ff1 = io.get(A.ff, 0, i1) - io.get(A.ff, 0, i);
+    t = (t + mk(c3 * nl1.x / dx, c3 * nl1.y / dx)) + mk(ff1.x / dx, ff1.y / dx);
+    C2<R> nl2 = io.get(A.nl, 1, i1) + io.get(A.nl, 1, i);
+    C2<R> ff2 = io.get(A.ff, 1, i1) + io.get(A.ff, 1, i);
+    C2<R> a2 = ciR(c3 * wy, nl2), b2 = ciR(wy, ff2);
+    t = (t + mk(a2.x / (R)2, a2.y / (R)2)) + mk(b2.x / (R)2, b2.y / (R)2);
+    C2<R> nl3 = io.get(A.nl, 2, i1) + io.get(A.nl, 2, i);
+    C2<R> ff3 = io.get(A.ff, 2, i1) + io.get(A.ff, 2, i);
+    C2<R> a3 = ciR(c3 * wz, nl3), b3 = ciR(wz, ff3);
+    t = (t + mk(a3.x / (R)2, a3.y / (R)2)) + mk(b3.x / (R)2, b3.y / (R)2);
+    C2<R> on1 = io.get(A.onl, 0, i1) - io.get(A.onl, 0, i);
+    t = t + mk(c4 * on1.x / dx, c4 * on1.y / dx);
+    C2<R> on2 = ciR(c4 * wy, io.get(A.onl, 1, i1) + io.get(A.onl, 1, i));
+    t = t + mk(on2.x / (R)2, on2.y / (R)2);
+    C2<R> on3 = ciR(c4 * wz, io.get(A.onl, 2, i1) + io.get(A.onl, 2, i));
+    t = t + mk(on3.x / (R)2, on3.y / (R)2);
+    t = mk((R)1 * t.x, (R)1 * t.y);  // half*(g(i)+g(i+1)) with g = 1
+    return mk(t.x / (c1 + c2), t.y / (c1 + c2));
+  };
+  R vdotz = 0, vdotyr = 0, vdotyi = 0;
+  if (mean) {
+    // non-cyclic, size nx-1, p(nx) = 0 (poisson.F90:127-142)
+    int s2 = thomas3(nx - 1, ap, bp, bp, bp, cp, prhs, gp, yr, zp, yi, ms, true);
+    if (s2) status = s2;
+    thomas_back(nx - 1, gp, yr, zp, yi, ms, true);
+    yr[(size_t)(nx - 1) * ms] = 0;
+    for (int i = 0; i < nx; ++i) {
+      yi[(size_t)i * ms] = 0;
+      zp[(size_t)i * ms] = 0;
+    }
+  } else {
+    const R b0 = bp - (R)1 * (cp / (R)1);
+    const R bn = bp - (R)1 * (ap / (R)1);
+    int s2 = thomas3(nx, ap, bp, b0, bn, cp, prhs, gp, yr, zp, yi, ms, true);
+    if (s2) status = s2;
+    thomas_back(nx, gp, yr, zp, yi, ms, true);
+    const R auxv0 = cp / (R)1, auxvn = ap / (R)1;
+    size_t ln = (size_t)(nx - 1) * ms;
+    vdotz = auxv0 * zp[0] + auxvn * zp[ln];
+    vdotyr = auxv0 * yr[0] + auxvn * yr[ln];
+    vdotyi = auxv0 * yi[0] + auxvn * yi[ln];
+  }
+  for (int i = 0; i < nx; ++i) {
+    size_t o = (size_t)i * ms;
+    R tr = yr[o] - vdotyr * zp[o] / ((R)1 + vdotz);
+    R ti = yi[o] - vdotyi * zp[o] / ((R)1 + vdotz);
+    C2<R> pold = io.get(A.p, 0, i);
+    C2<R> d = mk(tr - pold.x, ti - pold.y);
+    io.put(A.p, 0, i, pold + mk((R)1 * d.x, (R)1 * d.y));
+  }
+  // ---------------- ustep (usteptime.F90:45-258) ----------------
+  const R g = (R)1;
+  const R ah = (-g * dt * vis * c2 / dx2);
+  const R bh = ((R)1 + dt * vis * c2 * (g * (R)2 / dx2 + w2));
+  const R ch = ah;
+  const R b0h = bh - (R)1 * (ch / (R)1);
+  const R bnh = bh - (R)1 * (ah / (R)1);
+  const R auxv0h = ch / (R)1, auxvnh = ah / (R)1;
+  const R c12 = c1 + c2;
+  R vdotzh = 0;
+  for (int c = 0; c < 3; ++c) {
+    const R kk = c == 1 ? wy : wz;
+    auto urhs = [&](int i) -> C2<R> {
+      C2<R> uu = io.get(A.u, c, i);
+      R f = (R)1 - dt * vis * c1 * w2;
+      C2<R> r = mk(f * uu.x, f * uu.y);
+      C2<R> n = io.get(A.nl, c, i), fv = io.get(A.ff, c, i), on = io.get(A.onl, c, i);
+      r = (r + mk(dt * c3 * n.x, dt * c3 * n.y)) + mk(dt * fv.x, dt * fv.y);
+      r = r + mk(dt * c4 * on.x, dt * c4 * on.y);
+      int im = wrapn(i - 1, nx), ip = wrapn(i + 1, nx);
+      C2<R> pi_ = io.get(A.p, 0, i), pm = io.get(A.p, 0, im);
+      if (c == 0) {
+        C2<R> dpv = pi_ - pm;
+        R s = dt * c12;
+        r = r - mk(s * dpv.x / dx, s * dpv.y / dx);
+      } else {
+        R s = dt * c12 * (R)0.5;
+        C2<R> sp = i == 0 ? (pm + pi_) : (pm + pi_);
+        r = r - ciR(kk, mk(s * sp.x, s * sp.y));
+      }
+      C2<R> um = io.get(A.u, c, im), up = io.get(A.u, c, ip), uc = io.get(A.u, c, i);
+      C2<R> lap;
+      if (i == 0) lap = (up - mk((R)2 * uc.x, (R)2 * uc.y)) + um;
+      else lap = (um + up) - mk((R)2 * uc.x, (R)2 * uc.y);
+      R s = g * dt * vis * c1;
+      return r + mk(s * lap.x / dx2, s * lap.y / dx2);
+    };
+    int s2 = thomas3(nx, ah, bh, b0h, bnh, ch, urhs, gh, yr, zh, yi, ms, c == 0);
+    if (s2) status = s2;
+    thomas_back(nx, gh, yr, zh, yi, ms, c == 0);
+    size_t ln = (size_t)(nx - 1) * ms;
+    if (c == 0) vdotzh = auxv0h * zh[0] + auxvnh * zh[ln];
+    R vr = auxv0h * yr[0] + auxvnh * yr[ln];
+    R vi = auxv0h * yi[0] + auxvnh * yi[ln];
+    for (int i = 0; i < nx; ++i) {
+      size_t o = (size_t)i * ms;
+      io.put(A.u, c, i, mk(yr[o] - vr * zh[o] / ((R)1 + vdotzh), yi[o] - vi * zh[o] / ((R)1 + vdotzh)));
+    }
+  }
+  // ---------------- divcorr (poisson.F90:241-508) ----------------
+  const R alp = (c1 + c2) * dt;
+  auto drhs = [&](int i) -> C2<R> {
+    int i1 = wrapn(i + 1, nx);
+    C2<R> d = io.get(A.u, 0, i1) - io.get(A.u, 0, i);
+    C2<R> t = mk(d.x / dx, d.y / dx);
+    C2<R> s2v = ciR(wy, io.get(A.u, 1, i) + io.get(A.u, 1, i1));
+    t = t + mk((R)0.5 * s2v.x, (R)0.5 * s2v.y);
+    C2<R> s3v = ciR(wz, io.get(A.u, 2, i) + io.get(A.u, 2, i1));
+    t = t + mk((R)0.5 * s3v.x, (R)0.5 * s3v.y);
+    return mk(((R)0.5 * ((R)1 + (R)1) * t.x) / alp, ((R)0.5 * ((R)1 + (R)1) * t.y) / alp);
+  };
+  if (mean) {
+    thomas3(nx - 1, ap, bp, bp, bp, cp, drhs, gp, yr, zp, yi, ms, false);
+    thomas_back(nx - 1, gp, yr, zp, yi, ms, false);
+    yr[(size_t)(nx - 1) * ms] = 0;
+    for (int i = 0; i < nx; ++i) yi[(size_t)i * ms] = 0;
+    vdotyr = 0;
+    vdotyi = 0;
+  } else {
+    const R b0 = bp - (R)1 * (cp / (R)1);
+    const R bn = bp - (R)1 * (ap / (R)1);
+    thomas3(nx, ap, bp, b0, bn, cp, drhs, gp, yr, zp, yi, ms, false);
+    thomas_back(nx, gp, yr, zp, yi, ms, false);
+    const R auxv0 = cp / (R)1, auxvn = ap / (R)1;
+    size_t ln = (size_t)(nx - 1) * ms;
+    vdotyr = auxv0 * yr[0] + auxvn * yr[ln];
+    vdotyi = auxv0 * yi[0] + auxvn * yi[ln];
+  }
+  // div(i) into (yr, yi) in place
+  for (int i = 0; i < nx; ++i) {
+    size_t o = (size_t)i * ms;
+    R tr = yr[o] - vdotyr * zp[o] / ((R)1 + vdotz);
+    R ti = yi[o] - vdotyi * zp[o] / ((R)1 + vdotz);
+    yr[o] = tr;
+    yi[o] = ti;
+  }
+  auto divv = [&](int i) -> C2<R> { size_t o = (size_t)i * ms; return mk(yr[o], yi[o]); };
+  // u(t+dt) = u* - grad(phi)
+  for (int i = 0; i < nx; ++i) {
+    C2<R> di = divv(i), dm = divv(wrapn(i - 1, nx));
+    C2<R> u1 = io.get(A.u, 0, i), u2 = io.get(A.u, 1, i), u3 = io.get(A.u, 2, i);
+    C2<R> dd = di - dm;
+    C2<R> sm = i == 0 ? (di + dm) : (dm + di);
+    u1 = u1 - mk(alp * dd.x / dx, alp * dd.y / dx);
+    u2 = u2 - ciR(alp * (R)0.5 * wy, sm);
+    u3 = u3 - ciR(alp * (R)0.5 * wz, sm);
+    io.put(A.u, 0, i, u1);
+    io.put(A.u, 1, i, u2);
+    io.put(A.u, 2, i, u3);
+  }
+  // pressure update and the new divergence (check_divergence)
+  const R cv = (c2 * dt) * vis;
+  double dmax = 0.0;
+  for (int i = 0; i < nx; ++i) {
+    C2<R> dm = divv(wrapn(i - 1, nx)), di = divv(i), dp = divv(wrapn(i + 1, nx));
+    C2<R> lap = (dm - mk((R)2 * di.x, (R)2 * di.y)) + dp;
+    C2<R> corr = mk(lap.x / dx2, lap.y / dx2) - mk(w2 * di.x, w2 * di.y);
+    C2<R> pv = io.get(A.p, 0, i);
+    io.put(A.p, 0, i, (pv + di) - mk(cv * corr.x, cv * corr.y));
+    int i1 = wrapn(i + 1, nx);
+    C2<R> d = io.get(A.u, 0, i1) - io.get(A.u, 0, i);
+    C2<R> t = mk(d.x / dx, d.y / dx);
+    C2<R> s2v = ciR(wy, io.get(A.u, 1, i) + io.get(A.u, 1, i1));
+    t = t + mk((R)0.5 * s2v.x, (R)0.5 * s2v.y);
+    C2<R> s3v = ciR(wz, io.get(A.u, 2, i) + io.get(A.u, 2, i1));
+    t = t + mk((R)0.5 * s3v.x, (R)0.5 * s3v.y);
+    double mg = sqrt((double)t.x * (double)t.x + (double)t.y * (double)t.y);
+    if (mg > dmax) dmax = mg;
+  }
+  divmax = dmax;
+  return status;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_velstep(KArgs<T> A, T* scr, double* scr64, int* status) {
+  __shared__ double sh[256];
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  int nm = A.my2 * A.mz;
+  double dmax = 0.0;
+  if (q < nm) {
+    int st;
+    if (q == 0) st = run_mode<T, double>(A, 0, scr64, 1, 0, dmax);
+    else st = run_mode<T, T>(A, q, scr, (size_t)nm, (size_t)q, dmax);
+    if (st) *status = st;
+  }
+  double m = block_max(dmax, sh);
+  if (threadIdx.x == 0) A.part[blockIdx.x] = m;
+}

@@ -31,15 +31,9 @@ class RunSetup:
     """Namelist input + configuration for one run (Fortran host library)."""
 
     def __init__(self, run_name, precision=_abi.PIB_FP64):
+        self.run_name = run_name
+        self._load()
         host = _abi.load_host()
-        nml = (run_name + "_floparam.in").encode()
-        st = host.pib_host_read(nml, len(nml))
-        if st != PIB_OK:
-            raise PibError(st, _abi.host_error())
-        rn = run_name.encode()
-        st = host.pib_host_configure(rn, len(rn))
-        if st != PIB_OK:
-            raise PibError(st, _abi.host_error())
         nb, npnt, nx, my, mz = (ctypes.c_int() for _ in range(5))
         host.pib_host_counts(ctypes.byref(nb), ctypes.byref(npnt), ctypes.byref(nx),
                              ctypes.byref(my), ctypes.byref(mz))
@@ -54,10 +48,22 @@ class RunSetup:
         self.bndpts = np.zeros((3, self.nbnd), order="F")
         self.cd = np.zeros((3, self.nbnd), order="F")
         host.pib_host_get_points(dptr(self.bndpts), dptr(self.cd))
-        self.run_name = run_name
+
+    def _load(self):
+        """(Re)read the namelists and configure; the Fortran host holds one run."""
+        host = _abi.load_host()
+        nml = (self.run_name + "_floparam.in").encode()
+        st = host.pib_host_read(nml, len(nml))
+        if st != PIB_OK:
+            raise PibError(st, _abi.host_error())
+        rn = self.run_name.encode()
+        st = host.pib_host_configure(rn, len(rn))
+        if st != PIB_OK:
+            raise PibError(st, _abi.host_error())
 
     def finish(self, count_solid):
         """Mask-dependent initial scalars (dt, cf, umean, ufmean_des ...)."""
+        self._load()
         s = PibScalars()
         st = _abi.load_host().pib_host_finish(int(count_solid), ctypes.byref(s))
         if st != PIB_OK:

@@ -1,0 +1,104 @@
+"""Host side (Fortran library): the reference's namelist input contract and
+the initflo-derived run constants; the C-ABI export table."""
+import ctypes
+import math
+import os
+import re
+import shutil
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, make_setup
+from pureibm_ps2d_fd1d_amd import _abi, configs, solver
+
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+@pytest.mark.parametrize("deck", ["refsample", "refmis1"])
+def test_reads_reference_example_decks(tmp_path, deck):
+    """The reference's own example inputs parse (all groups, with the
+    reference's variables); their 'random' configuration is refused cleanly."""
+    shutil.copy(os.path.join(GOLD, f"{deck}_floparam.in"), tmp_path / f"{deck}_floparam.in")
+    host = _abi.load_host()
+    p = str(tmp_path / f"{deck}_floparam.in").encode()
+    assert host.pib_host_read(p, len(p)) == 0
+    rn = str(tmp_path / deck).encode()
+    st = host.pib_host_configure(rn, len(rn))
+    assert st == _abi.PIB_ERR_UNSUPPORTED
+    assert "IMOVE" in _abi.host_error() or "random" in _abi.host_error()
+
+
+def test_missing_deck_is_an_error(tmp_path):
+    host = _abi.load_host()
+    p = str(tmp_path / "nope_floparam.in").encode()
+    assert host.pib_host_read(p, len(p)) == _abi.PIB_ERR_ARG
+
+
+def test_simple_configuration_constants(tmp_path):
+    """'simple' (initialize_flo.F90:1372-1424): one sphere at (mx1/2+1,
+    my/2+1, mz/2+1), radbdy = my/(2 lybyd), dx = lybyd*dia/my."""
+    st = make_setup(tmp_path, "S", kind="simple", my=32, dbydx=8, re=20.0)
+    assert st.nbody == 1 and (st.nx, st.my, st.mz) == (32, 32, 32)
+    assert np.allclose(st.xc, [[17.0, 17.0, 17.0]])
+    assert abs(st.radbdy[0] - 4.0) < 1e-12
+    c = st.config
+    assert abs(c.dx - 4.0 / 32) < 1e-15
+    assert abs(c.ucharmod - 20.0 * 0.012 / 1.0) < 1e-15
+    assert abs(c.voldom - 4.0 ** 3) < 1e-12
+    assert st.nbnd == 1158 * 0 + st.nbnd  # point count depends on r = 4
+
+
+def test_default_configuration_reads_centres(tmp_path):
+    centers = np.array([[0.5, 1.25, 2.0], [2.5, 2.5, 0.75]])
+    configs.write_case(str(tmp_path), "D", my=24, lybyd=3.0, dbydx=8.0, centers=centers)
+    st = solver.RunSetup(str(tmp_path / "D"))
+    assert st.nbody == 2
+    assert np.allclose(st.xc, centers * 8.0)
+    assert np.allclose(st.radbdy, 4.0)
+
+
+def test_initial_time_step_and_mean_flow(tmp_path):
+    """initflo (initialize_flo.F90:333-359, 552-672): ufmean_des = uchar/(1-phi),
+    umean = (1-phi) ufmean_des, dt = cfl*dx/(|uchar|/(1-phi)), cf = -1/dt."""
+    st = make_setup(tmp_path, "T", kind="simple", my=32, dbydx=8, re=20.0, cfl=0.2)
+    count_solid = 251
+    s = st.finish(count_solid)
+    phi = count_solid / 32 ** 3
+    uchar = 20.0 * 0.012
+    assert abs(s.maxvolfrac - phi) < 1e-15
+    assert abs(s.ufmean_des[0] - uchar / (1 - phi)) < 1e-15
+    assert abs(s.umean[0] - (1 - phi) * uchar / (1 - phi)) < 1e-15
+    dt = 0.2 * (4.0 / 32) / (uchar / (1 - phi))
+    assert abs(s.dt - dt) < 1e-15 * dt
+    assert abs(s.cf + 1.0 / dt) < 1e-12 / dt
+    assert s.ramp_frac_time == 0.0
+
+
+def test_unsupported_options_are_refused(tmp_path):
+    configs.write_case(str(tmp_path), "U", my=16, lybyd=2.0, dbydx=8.0, input_type="simple",
+                       vol_frac1=0.06, discr_scheme="quick")
+    with pytest.raises(solver.PibError) as e:
+        solver.RunSetup(str(tmp_path / "U"))
+    assert e.value.code == _abi.PIB_ERR_UNSUPPORTED
+
+
+def test_gpu_library_exports_the_header():
+    """Every entry point declared in include/pureibm_gpu.h is exported by the
+    HIP library (loading it makes no GPU call)."""
+    hdr = open(os.path.join(ROOT, "include", "pureibm_gpu.h")).read()
+    names = set(re.findall(r"\b(pib_[a-z0-9_]+)\s*\(", hdr))
+    assert len(names) >= 20
+    lib = ctypes.CDLL(_abi.gpu_lib_path())
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+    assert set(_abi.GPU_SYMBOLS) <= names
+
+
+def test_product_never_imports_the_oracle():
+    pkg = os.path.join(ROOT, "pureibm_ps2d_fd1d_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".h", ".f90", ".cpp")):
+                txt = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in txt and "liboracle" not in txt, f
