@@ -1,0 +1,179 @@
+"""Benchmark of the per-time-step fluid solve on one MI355X.
+
+Workload (default): the fixed random assembly -- 128^3 periodic box, 195
+spheres of 16 cells (solid fraction 0.2) from seeded RSA, Re_m = 50 -- run
+as FP32-bulk (the headline) beside plain FP64.  A step is one full time
+step (compute_new_timestep, nonlinear + bcset, velstep).
+
+Prints one JSON line with Mgridpt-steps/s, the roofline of the dominant
+kernel (algorithmic bytes per launch / average launch time from HIP events
+recorded on the solver's stream during the timed region) and a bounded
+single-core sample of the CPU oracle as the CPU baseline.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from pureibm_ps2d_fd1d_amd import _abi, configs, solver  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+# Algorithmic field passes per launch (reads + writes of N-word fields,
+# spectral and physical fields both counted as N words).
+PASSES = {
+    "c2r_ubcp": 6, "c2r_pbcp": 2, "form_nl": 6, "c2r_nlbc": 6, "c2r_onlbc": 6,
+    "c2r_visc": 6, "c2r_omega": 8, "c2r_pgrad": 4, "r2c_force": 6, "velstep": 17,
+}
+
+
+def shell_fraction(st):
+    """Fraction s of grid nodes touched by the 2x2x2 IBM stencils (surface,
+    pressure, inner and outer reversal points), exact for this geometry."""
+    nx, my, mz = st.nx, st.my, st.mz
+    b = st.bndpts
+    nodes = set()
+    dr = st.config.dr
+    for m in range(st.nbody):
+        c = st.xc[m][:, None]
+        r = st.radbdy[m]
+        for pos in (c + b * r, c + b * r - np.array([[0.5], [0], [0]]), c + b * (r - dr),
+                    c + b * (r - dr) - np.array([[0.5], [0], [0]]), c + b * (r + dr)):
+            base = np.floor(pos).astype(np.int64)
+            for dx in (0, 1):
+                for dy in (0, 1):
+                    for dz in (0, 1):
+                        i = (base[0] + dx - 1) % nx
+                        j = (base[1] + dy - 1) % my
+                        k = (base[2] + dz - 1) % mz
+                        nodes.update((i * my + j) * mz + k)
+    return len(nodes) / float(nx * my * mz)
+
+
+def run_gpu(setup, steps, warmup):
+    g = solver.Solver(setup)
+    lib = g.lib
+    g.step(warmup)
+    lib.pib_sync(g.ctx)
+    lib.pib_set_timing(g.ctx, 1)
+    t0 = time.perf_counter()
+    st = lib.pib_step(g.ctx, steps)
+    t1 = time.perf_counter()
+    if st != 0:
+        raise RuntimeError(lib.pib_last_error(g.ctx).decode())
+    maxn = 64
+    names = ctypes.create_string_buffer(32 * maxn)
+    tot = np.zeros(maxn)
+    cnt = np.zeros(maxn, dtype=np.int32)
+    n = ctypes.c_int()
+    lib.pib_kernel_times(g.ctx, maxn, names, _abi.dptr(tot),
+                         cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.byref(n))
+    lib.pib_set_timing(g.ctx, 0)
+    kern = {}
+    for q in range(n.value):
+        nm = names.raw[32 * q:32 * q + 32].split(b"\0")[0].decode()
+        kern[nm] = (float(tot[q]), int(cnt[q]))
+    res = dict(wall_s=t1 - t0, kernels=kern, scalars=g.scalars(), forces=g.forces()["force"])
+    g.close()
+    return res
+
+
+def cpu_baseline(setup, nsteps):
+    """Single-core oracle on the same workload (bounded sample)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    oracle.build()
+    o = solver.Solver(setup, lib=oracle.load(), prefix="orc_")
+    t0 = time.perf_counter()
+    o.step(nsteps)
+    dt = time.perf_counter() - t0
+    o.close()
+    return dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="C2")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["both", "fp32", "fp64"], default="both")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; the x-slab sharding across GPUs is "
+                                   "described in DESIGN.md and not launched here", "n_gpus": a.gpus}))
+        sys.exit(2)
+
+    work = tempfile.mkdtemp(prefix="pib_bench_")
+    configs.write_workload(work, a.workload)
+    run = os.path.join(work, a.workload)
+    st64 = solver.RunSetup(run, precision=_abi.PIB_FP64)
+    st32 = solver.RunSetup(run, precision=_abi.PIB_FP32_BULK)
+    N = st64.nx * st64.my * st64.mz
+    s = shell_fraction(st64)
+
+    r32 = run_gpu(st32, a.steps, a.warmup) if a.only in ("both", "fp32") else None
+    r64 = run_gpu(st64, a.steps, a.warmup) if a.only in ("both", "fp64") else None
+    head = r32 if r32 is not None else r64
+    w = 4 if r32 is not None else 8
+    ms_step = head["wall_s"] / a.steps * 1e3
+    value = N * a.steps / head["wall_s"] / 1e6
+
+    # dominant kernel and its roofline
+    kern = head["kernels"]
+    dom = max(kern, key=lambda k: kern[k][0])
+    tot_ms, launches = kern[dom]
+    avg_s = tot_ms / launches / 1e3
+    passes = PASSES.get(dom)
+    achieved = (passes * w * N / avg_s / 1e9) if passes else None
+    roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+            "algorithmic_bytes_per_launch": passes * w * N if passes else None,
+            "avg_launch_us": avg_s * 1e6, "launches": launches}
+    step_bytes = w * N * (70 + 22 * s)
+    step_roof = {"bytes_per_step": step_bytes, "achieved_GBs": step_bytes / (ms_step / 1e3) / 1e9,
+                 "frac": step_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, "shell_fraction": s}
+
+    out = {
+        "metric": "Mgridpt-steps/s (FP32-bulk fixed-bed step, 128^3, 195 spheres)",
+        "value": value, "unit": "Mgridpt-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if r32 is not None else "f64", "data": "synthetic (seeded RSA bed, flow started from rest)",
+        "config": {"workload": "C2: 128^3 periodic box, 195 spheres d=16 cells (phi=0.1994), Re_m=50, "
+                               "fixed bed, AB2/CN step, central x-flux",
+                   "grid_points": N, "nbody": st64.nbody, "precision": "fp32-bulk" if r32 else "fp64"},
+        "roofline": roof,
+        "step_roofline": step_roof,
+        "kernel_ms_per_step": {k: v[0] / a.steps for k, v in sorted(kern.items(), key=lambda kv: -kv[1][0])},
+    }
+    if r64 is not None and r32 is not None:
+        v64 = N * a.steps / r64["wall_s"] / 1e6
+        d64, d32 = r64["scalars"].norm_drag, r32["scalars"].norm_drag
+        f64, f32 = r64["forces"], r32["forces"]
+        out["fp64"] = {"value": v64, "ms_per_step": r64["wall_s"] / a.steps * 1e3,
+                       "speedup_fp32_bulk": value / v64,
+                       "mean_drag_rel_err_fp32_vs_fp64": abs(d32 - d64) / abs(d64),
+                       "max_particle_drag_rel_err": float(np.max(np.abs(f32[:, 0] - f64[:, 0])) /
+                                                          np.max(np.abs(f64[:, 0]))),
+                       "norm_drag_fp64": d64, "norm_drag_fp32": d32}
+    if not a.no_cpu:
+        ncpu = max(1, a.cpu_steps)
+        tcpu = cpu_baseline(st64, ncpu)
+        out["cpu_baseline"] = {"value": N * ncpu / tcpu / 1e6, "unit": "Mgridpt-steps/s", "cores": 1,
+                               "kind": "port", "sample": f"{ncpu} FP64 step(s) of the same 128^3 bed "
+                                                         f"on the CPU oracle, {tcpu:.1f} s"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -188,6 +188,12 @@ int pib_tridiag_batched(void* ctx, int n, int nsys, const double* a, const doubl
  * HIP events on the context stream: [0] compute_new_timestep, [1] nonlinear
  * (form_nl + transforms), [2] bcset, [3] velstep. */
 int pib_phase_times(void* ctx, double* ms4);
+/* Per-launch timing: with on != 0, every kernel launch of the context is
+ * bracketed by HIP events on the context stream.  pib_kernel_times syncs,
+ * aggregates by kernel name (names: maxn slots of 32 chars), returns the
+ * number of distinct kernels in *count and clears the records. */
+int pib_set_timing(void* ctx, int on);
+int pib_kernel_times(void* ctx, int maxn, char* names, double* total_ms, int* launches, int* count);
 
 #ifdef __cplusplus
 }

@@ -127,6 +127,8 @@ _GPU_ONLY_SIGS = {
     "ff2cr_planes": ([_VP, _PD, ctypes.c_longlong, _I, _PD, ctypes.c_longlong], _I),
     "tridiag_batched": ([_VP, _I, _I, _PD, _PD, _PD, _PD, _PD, _I], _I),
     "phase_times": ([_VP, _PD], _I),
+    "set_timing": ([_VP, _I], _I),
+    "kernel_times": ([_VP, _I, ctypes.c_char_p, _PD, _PI, _PI], _I),
 }
 
 GPU_SYMBOLS = ["pib_" + k for k in list(_STEP_SIGS) + list(_GPU_ONLY_SIGS)]

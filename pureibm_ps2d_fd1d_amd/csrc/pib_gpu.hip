@@ -533,6 +533,33 @@ struct Impl : Base {
   std::vector<void*> pallocs;  // particle-dependent allocations
   hipEvent_t ev[5];
   bool timed = false;
+  // optional per-launch timing: events recorded around every launch on `stream`
+  bool ktime = false;
+  std::vector<hipEvent_t> evpool;
+  size_t evused = 0;
+  std::vector<std::pair<const char*, std::pair<hipEvent_t, hipEvent_t>>> krec;
+  const char* kname = nullptr;
+  hipEvent_t kbeg = nullptr;
+  hipEvent_t get_ev() {
+    if (evused == evpool.size()) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      evpool.push_back(e);
+    }
+    return evpool[evused++];
+  }
+  void kb(const char* name) {
+    if (!ktime) return;
+    kname = name;
+    kbeg = get_ev();
+    HIPCHK(hipEventRecord(kbeg, stream));
+  }
+  void ke() {
+    if (!ktime) return;
+    hipEvent_t e = get_ev();
+    HIPCHK(hipEventRecord(e, stream));
+    krec.push_back({kname, {kbeg, e}});
+  }
 
   template <typename V>
   V* alloc(size_t n, bool part_dep = false) {
@@ -664,6 +691,7 @@ struct Impl : Base {
     for (void* q : pallocs) (void)hipFree(q);
     for (void* q : allocs) (void)hipFree(q);
     for (auto& e : ev) (void)hipEventDestroy(e);
+    for (auto& e : evpool) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
   void fill_mask() {
@@ -728,10 +756,13 @@ struct Impl : Base {
   }
   template <int KIND>
   void c2r(int rks, int ncomp) {
+    static const char* names[] = {"c2r_ubcp", "c2r_pbcp", "c2r_nlbc", "c2r_onlbc", "c2r_visc", "c2r_omega", "c2r_pgrad"};
     auto kern = k_c2r<T, KIND>;
     set_lds(kern, lds_plane);
+    kb(names[KIND]);
     kern<<<dim3(nx, ncomp), nt_plane, lds_plane, stream>>>(args(rks));
     HIPCHK(hipGetLastError());
+    ke();
   }
 
   // ---------------- set-up ----------------
@@ -908,8 +939,10 @@ struct Impl : Base {
   void compute_new_timestep(int rks) {
     c2r<C2R_UBCP>(rks, 3);
     c2r<C2R_PBCP>(rks, 1);
+    kb("scalar_cnt");
     k_scalar_cnt<<<1, 64, 0, stream>>>(dst, part, nx, rks, vel ? vel : part);
     HIPCHK(hipGetLastError());
+    ke();
   }
   void form_nl(int rks) {
     size_t E = (size_t)my2 * mz;
@@ -930,8 +963,10 @@ struct Impl : Base {
   template <typename K>
   void run_nl(K kern, int rks) {
     set_lds(kern, lds_plane);
+    kb("form_nl");
     kern<<<dim3(nx, 3), nt_plane, lds_plane, stream>>>(args(rks));
     HIPCHK(hipGetLastError());
+    ke();
   }
   void nonlinear(int rks) {
     // onl <- nl (nl.F90:99-108): the buffers swap roles; the physical copy of
@@ -950,40 +985,60 @@ struct Impl : Base {
     size_t npts = (size_t)nbody * nbnd;
     c2r<C2R_VISC>(rks, 3);
     c2r<C2R_OMEGA>(rks, 3);
+    kb("drag_points");
     k_drag_points<T><<<(unsigned)((npts + 255) / 256), 256, 0, stream>>>(args(rks), P, drag_pts);
     HIPCHK(hipGetLastError());
+    ke();
+    kb("drag_body");
     k_drag_body<<<nbody, 256, 0, stream>>>(drag_pts, nbnd, nbody, rks, pres, visc, torq, body_tot);
     HIPCHK(hipGetLastError());
+    ke();
+    kb("scalar_drag");
     k_scalar_drag<<<1, 64, 0, stream>>>(dst, body_tot, rks, rad, pres, visc, force, force_chem);
     HIPCHK(hipGetLastError());
+    ke();
     c2r<C2R_PGRAD>(rks, 3);
     if (cfg.dorpr) {
+      kb("forcing_points");
       k_forcing_points<T><<<(unsigned)((npts + 255) / 256), 256, 0, stream>>>(args(rks), P, frc_pts);
       HIPCHK(hipGetLastError());
+      ke();
+      kb("reduce_frmean");
       k_reduce_frmean<<<1, 1024, 0, stream>>>(dst, frc_pts, (int)npts);
       HIPCHK(hipGetLastError());
+      ke();
       if (ntarget > 0) {
+        kb("spread");
         k_spread<T><<<(ntarget + 255) / 256, 256, 0, stream>>>(args(rks), ntarget, tnode, toff, rpt, rw, rdist,
                                                                frc_pts);
         HIPCHK(hipGetLastError());
+        ke();
       }
     }
     auto kern = k_r2c_force<T>;
     set_lds(kern, lds_plane);
+    kb("r2c_force");
     kern<<<dim3(nx, 3), nt_plane, lds_plane, stream>>>(args(rks));
     HIPCHK(hipGetLastError());
+    ke();
+    kb("scalar_diag");
     k_scalar_diag<<<1, 64, 0, stream>>>(dst, force, force_chem);
     HIPCHK(hipGetLastError());
+    ke();
   }
   void velstep(int rks) {
     int nm = my2 * mz;
     int nblk = (nm + 255) / 256;
     if ((size_t)nblk > part_n) throw PibFail(PIB_ERR_ARG, "partial buffer too small");
     HIPCHK(hipMemsetAsync(dstatus, 0, sizeof(int), stream));
+    kb("velstep");
     k_velstep<T><<<nblk, 256, 0, stream>>>(args(rks), vscr, vscr64, dstatus);
     HIPCHK(hipGetLastError());
+    ke();
+    kb("scalar_vel");
     k_scalar_vel<<<1, 64, 0, stream>>>(dst, part, nblk, dstatus);
     HIPCHK(hipGetLastError());
+    ke();
   }
   void step_async(int nsteps) {
     if (!have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
@@ -1011,6 +1066,40 @@ struct Impl : Base {
     nlbc_valid = true;
     if (rec) HIPCHK(hipEventRecord(ev[2], stream));
     if (nbody > 0) bcset(rks);
+  }
+  void set_timing(int on) {
+    HIPCHK(hipStreamSynchronize(stream));
+    ktime = on != 0;
+    krec.clear();
+    evused = 0;
+  }
+  int kernel_times(int maxn, char* names, double* total_ms, int* launches) {
+    HIPCHK(hipStreamSynchronize(stream));
+    std::vector<std::string> nm;
+    std::vector<double> tot;
+    std::vector<int> cnt;
+    for (auto& r : krec) {
+      float f = 0;
+      HIPCHK(hipEventElapsedTime(&f, r.second.first, r.second.second));
+      size_t q = 0;
+      while (q < nm.size() && nm[q] != r.first) ++q;
+      if (q == nm.size()) {
+        nm.push_back(r.first);
+        tot.push_back(0);
+        cnt.push_back(0);
+      }
+      tot[q] += f;
+      cnt[q] += 1;
+    }
+    int n = (int)nm.size();
+    for (int q = 0; q < n && q < maxn; ++q) {
+      snprintf(names + 32 * q, 32, "%s", nm[q].c_str());
+      total_ms[q] = tot[q];
+      launches[q] = cnt[q];
+    }
+    krec.clear();
+    evused = 0;
+    return n;
   }
   int check_status() {
     int st = 0;
@@ -1360,6 +1449,13 @@ int pib_step_async(void* ctx, int nsteps) {
 }
 int pib_sync(void* ctx) { return finish_status(ctx, PIB_OK); }
 int pib_step(void* ctx, int nsteps) { return finish_status(ctx, pib_step_async(ctx, nsteps)); }
+int pib_set_timing(void* ctx, int on) {
+  return dispatch(ctx, [&](auto& I) { I.set_timing(on); });
+}
+int pib_kernel_times(void* ctx, int maxn, char* names, double* total_ms, int* launches, int* count) {
+  if (!names || !total_ms || !launches || !count || maxn < 0) return PIB_ERR_ARG;
+  return dispatch(ctx, [&](auto& I) { *count = I.kernel_times(maxn, names, total_ms, launches); });
+}
 int pib_phase_times(void* ctx, double* ms4) {
   if (!ms4) return PIB_ERR_ARG;
   return dispatch(ctx, [&](auto& I) { I.phase_times(ms4); });
