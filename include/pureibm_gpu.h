@@ -195,6 +195,18 @@ int pib_phase_times(void* ctx, double* ms4);
 int pib_set_timing(void* ctx, int on);
 int pib_kernel_times(void* ctx, int maxn, char* names, double* total_ms, int* launches, int* count);
 
+/* ---- kernel variants ---------------------------------------------------- */
+/* Selects among equivalent kernel variants of one context (no reference
+ * counterpart: the reference has one implementation per routine).
+ *   PIB_TUNE_VELSTEP: 0 = one lane per (ky,kz) mode running the reference's
+ *     Thomas + Sherman-Morrison sweeps; 1 = tiled partition solve with 8
+ *     (FP32-bulk) / 4 (FP64) x rows per lane (default); 2 = the same with
+ *     4 / 2 rows per lane.  Shapes a tiled form does not take fall back to
+ *     the next form down.
+ * Returns PIB_ERR_ARG for an unknown key or value. */
+#define PIB_TUNE_VELSTEP 0
+int pib_set_tuning(void* ctx, int key, int value);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,6 +22,8 @@ PIB_ERR_STATE = -4
 PIB_ERR_BLOWUP = -5
 PIB_ERR_PIVOT = -6
 
+PIB_TUNE_VELSTEP = 0
+
 PIB_FP64 = 0
 PIB_FP32_BULK = 1
 
@@ -129,6 +131,7 @@ _GPU_ONLY_SIGS = {
     "phase_times": ([_VP, _PD], _I),
     "set_timing": ([_VP, _I], _I),
     "kernel_times": ([_VP, _I, ctypes.c_char_p, _PD, _PI, _PI], _I),
+    "set_tuning": ([_VP, _I, _I], _I),
 }
 
 GPU_SYMBOLS = ["pib_" + k for k in list(_STEP_SIGS) + list(_GPU_ONLY_SIGS)]

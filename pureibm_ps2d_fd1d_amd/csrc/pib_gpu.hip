@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "pib_kernels.h"
+#include "pib_velstep.h"
 
 namespace {
 
@@ -330,7 +331,7 @@ __global__ void k_scalar_diag(DevState* st, const double* force, const double* f
 }
 
 // velstep tail: check_divergence max, umean update (steptheflow.F90:93-99)
-__global__ void k_scalar_vel(DevState* st, const double* part, int nblk, const int* status) {
+__global__ void k_scalar_vel(DevState* st, const double* part, int nblk, int* status) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double mx = 0.0;
   for (int b = 0; b < nblk; ++b) mx = fmax(mx, part[b]);
@@ -338,6 +339,7 @@ __global__ void k_scalar_vel(DevState* st, const double* part, int nblk, const i
   for (int n = 0; n < 3; ++n)
     st->s.umean[n] = st->s.umean[n] + (-st->s.mpg[n] + st->s.frmean[n] - st->s.frame_accln[n]) * st->s.dt;
   if (*status && st->status == 0) st->status = *status;
+  *status = 0;  // the velstep kernels only ever raise it
 }
 
 // ===========================================================================
@@ -535,6 +537,7 @@ struct Impl : Base {
   bool timed = false;
   // optional per-launch timing: events recorded around every launch on `stream`
   bool ktime = false;
+  int vel_variant = 1;  // 0 thread-per-mode, 1 tiled (L=8/4), 2 tiled (L=4/2)
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
   std::vector<std::pair<const char*, std::pair<hipEvent_t, hipEvent_t>>> krec;
@@ -656,7 +659,7 @@ struct Impl : Base {
     mask = alloc<uint8_t>((size_t)nx * my * mz);
     fill_mask();
     dst = alloc<DevState>(1);
-    part_n = std::max<size_t>((size_t)3 * nx * 4, (size_t)(E / 256 + 2));
+    part_n = std::max<size_t>((size_t)3 * nx * 4, (size_t)(E / 8 + 2));
     part = alloc<double>(part_n);
     vscr = alloc<T>((size_t)6 * nx * E);
     vscr64 = alloc<double>((size_t)6 * nx);
@@ -1026,19 +1029,50 @@ struct Impl : Base {
     HIPCHK(hipGetLastError());
     ke();
   }
-  void velstep(int rks) {
+  // tiled velstep: M modes per workgroup, L x rows per lane; false when the
+  // shape does not fit (nx % L, workgroup size, the (0,0)-mode workgroup)
+  template <int L, int M>
+  bool velstep_tiled(int rks) {
+    if (nx % L != 0 || nx % 2 != 0) return false;
+    int bd = M * (nx / L);
+    if (bd > 512 || nx / 2 > bd) return false;
     int nm = my2 * mz;
-    int nblk = (nm + 255) / 256;
-    if ((size_t)nblk > part_n) throw PibFail(PIB_ERR_ARG, "partial buffer too small");
-    HIPCHK(hipMemsetAsync(dstatus, 0, sizeof(int), stream));
+    int ntiles = (nm + M - 1) / M;
+    if ((size_t)ntiles + 1 > part_n) return false;
+    size_t lds = std::max(vel_lds_bytes(bd), (size_t)bd * sizeof(double));
+    auto kern = k_velstep_tiled<T, L, M>;
+    set_lds(kern, lds);
     kb("velstep");
-    k_velstep<T><<<nblk, 256, 0, stream>>>(args(rks), vscr, vscr64, dstatus);
+    kern<<<ntiles + 1, bd, lds, stream>>>(args(rks), ntiles, dstatus);
     HIPCHK(hipGetLastError());
     ke();
+    vel_tail(ntiles + 1);
+    return true;
+  }
+  void vel_tail(int nblk) {
     kb("scalar_vel");
     k_scalar_vel<<<1, 64, 0, stream>>>(dst, part, nblk, dstatus);
     HIPCHK(hipGetLastError());
     ke();
+  }
+  void velstep(int rks) {
+    if (vel_variant > 0) {
+      bool done;
+      if constexpr (sizeof(T) == 4)
+        done = (vel_variant == 1 && velstep_tiled<8, 16>(rks)) || velstep_tiled<4, 16>(rks);
+      else
+        done = (vel_variant == 1 && velstep_tiled<4, 8>(rks)) || velstep_tiled<2, 8>(rks);
+      if (done) return;
+    }
+    // thread-per-mode path for shapes the tiled kernel does not take
+    int nm = my2 * mz;
+    int nblk = (nm + 255) / 256;
+    if ((size_t)nblk > part_n) throw PibFail(PIB_ERR_ARG, "partial buffer too small");
+    kb("velstep");
+    k_velstep<T><<<nblk, 256, 0, stream>>>(args(rks), vscr, vscr64, dstatus);
+    HIPCHK(hipGetLastError());
+    ke();
+    vel_tail(nblk);
   }
   void step_async(int nsteps) {
     if (!have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
@@ -1449,6 +1483,10 @@ int pib_step_async(void* ctx, int nsteps) {
 }
 int pib_sync(void* ctx) { return finish_status(ctx, PIB_OK); }
 int pib_step(void* ctx, int nsteps) { return finish_status(ctx, pib_step_async(ctx, nsteps)); }
+int pib_set_tuning(void* ctx, int key, int value) {
+  if (key != PIB_TUNE_VELSTEP || value < 0 || value > 2) return PIB_ERR_ARG;
+  return dispatch(ctx, [&](auto& I) { I.vel_variant = value; });
+}
 int pib_set_timing(void* ctx, int on) {
   return dispatch(ctx, [&](auto& I) { I.set_timing(on); });
 }
