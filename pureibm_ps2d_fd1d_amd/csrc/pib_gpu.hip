@@ -162,27 +162,36 @@ __global__ void k_gather_records(int n, const int* rec_sorted, const double* w, 
 }
 
 // ===========================================================================
-// scalar control kernels (single workgroup)
+// scalar control kernels: one workgroup of SCAL_T lanes; the sums are
+// deterministic block trees, the scalar logic runs on lane 0
 // ===========================================================================
-// compute_new_timestep tail (dependent_functions.F90:1143-1277)
-__global__ void k_scalar_cnt(DevState* st, const double* part, int nx, int rks, const double* velbdy) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  pib_scalars& s = st->s;
-  double ufl[3], usl[3], uml[3], mxc[3];
-  for (int c = 0; c < 3; ++c) {
-    double a = 0, b = 0, d = 0, mx = PIB_SMALL_NUMBER;
-    for (int i = 0; i < nx; ++i) {
+#define SCAL_T 256
+
+// compute_new_timestep tail (dependent_functions.F90:1143-1277); part holds
+// per-(component, plane) fluid/solid/total sums and max|u| from c2r_ubcp
+__global__ void __launch_bounds__(SCAL_T) k_scalar_cnt(DevState* st, const double* part, int nx, int rks,
+                                                       const double* velbdy) {
+  __shared__ double sh[12 * SCAL_T];
+  const int t = threadIdx.x;
+  double v[12];
+  for (int q = 0; q < 12; ++q) v[q] = 0.0;
+  double mx[3] = {PIB_SMALL_NUMBER, PIB_SMALL_NUMBER, PIB_SMALL_NUMBER};
+  for (int c = 0; c < 3; ++c)
+    for (int i = t; i < nx; i += SCAL_T) {
       const double* o = part + ((size_t)c * nx + i) * 4;
-      a += o[0];
-      b += o[1];
-      d += o[2];
-      mx = fmax(mx, o[3]);
+      v[c * 3 + 0] += o[0];
+      v[c * 3 + 1] += o[1];
+      v[c * 3 + 2] += o[2];
+      mx[c] = fmax(mx[c], o[3]);
     }
-    ufl[c] = a;
-    usl[c] = b;
-    uml[c] = d;
-    mxc[c] = mx;
-  }
+  const int nb = st->nbody;
+  for (int m = t; m < nb; m += SCAL_T)
+    for (int n = 0; n < 3; ++n) v[9 + n] += velbdy[(size_t)n * nb + m];
+  block_sums<12>(v, sh);
+  double mxc[3];
+  for (int c = 0; c < 3; ++c) mxc[c] = block_max_any(mx[c], sh);
+  if (t != 0) return;
+  pib_scalars& s = st->s;
   double umax_tmp = fmax(fmax(mxc[0], mxc[1]), mxc[2]);
   double dtnew;
   if (umax_tmp * s.dt / st->dx > st->cfl_max)
@@ -194,16 +203,13 @@ __global__ void k_scalar_cnt(DevState* st, const double* part, int nx, int rks, 
   double ntot = (double)((long long)st->nx * st->my * st->mz);
   for (int n = 0; n < 3; ++n) {
     s.ufmean_old[n] = s.ufmean[n];
-    s.ufmean[n] = ufl[n] / s.count_fluid;
-    s.usmean_temp[n] = usl[n] / s.count_solid;
-    s.umean_temp[n] = uml[n] / ntot;
+    s.ufmean[n] = v[n * 3 + 0] / s.count_fluid;
+    s.usmean_temp[n] = v[n * 3 + 1] / s.count_solid;
+    s.umean_temp[n] = v[n * 3 + 2] / ntot;
   }
-  int nb = st->nbody;
   double meanslip[3], mix[3];
   for (int n = 0; n < 3; ++n) {
-    double sum = 0.0;
-    for (int m = 0; m < nb; ++m) sum += velbdy[(size_t)n * nb + m];
-    double msv = nb > 0 ? sum / (double)nb : 0.0;
+    double msv = nb > 0 ? v[9 + n] / (double)nb : 0.0;
     s.usmean_act[n] = msv;
     s.dufmeandt[n] = -s.cf * (s.ufmean[n] - s.ufmean_old[n]);
     meanslip[n] = (s.ufmean[n] - s.usmean_des[n]) * (1.0 - s.maxvolfrac);
@@ -226,92 +232,112 @@ __global__ void k_scalar_cnt(DevState* st, const double* part, int nx, int rks, 
   }
 }
 
-// compute_mpg_at_n tail (boundary_condition.F90:373-435)
-__global__ void k_scalar_drag(DevState* st, const double* body_tot, int rks, const double* rad, const double* pres,
-                              const double* visc, double* force, double* force_chem) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// calc_pres_visc_drag per-body totals (boundary_condition.F90:1631-1645) from
+// the (body, chunk) partials of k_drag_points, then the compute_mpg_at_n
+// tail (boundary_condition.F90:373-435) and the per-body forces.
+__global__ void __launch_bounds__(SCAL_T) k_scalar_drag(DevState* st, const double* dpart, int G, int rks,
+                                                        const double* rad, double* pres, double* visc,
+                                                        double* torq, double* force, double* force_chem) {
+  __shared__ double sh[12 * SCAL_T];
+  __shared__ double mpg_s[3];
+  const int t = threadIdx.x;
+  const int nb = st->nbody;
+  double v[12];
+  for (int q = 0; q < 12; ++q) v[q] = 0.0;
+  for (int m = t; m < nb; m += SCAL_T) {
+    double tot[9];
+    for (int q = 0; q < 9; ++q) tot[q] = 0.0;
+    for (int g = 0; g < G; ++g)
+      for (int q = 0; q < 9; ++q) tot[q] += dpart[((size_t)m * G + g) * 9 + q];
+    for (int d = 0; d < 3; ++d) {
+      size_t k = (size_t)d * nb + m;
+      double p0 = rks == 1 ? 0.0 : pres[k];
+      double v0 = rks == 1 ? 0.0 : visc[k];
+      double t0 = rks == 1 ? 0.0 : torq[k];
+      double pn = p0 - tot[d], vn = v0 + tot[3 + d];
+      pres[k] = pn;
+      visc[k] = vn;
+      torq[k] = t0 + tot[6 + d];
+      v[d] += tot[d];          // pres_total
+      v[3 + d] += tot[3 + d];  // visc_total
+      v[6 + d] += pn;          // sum of pres(m)
+      v[9 + d] += vn;          // sum of visc(m)
+    }
+  }
+  block_sums<12>(v, sh);
   pib_scalars& s = st->s;
-  int nb = st->nbody;
   const double PI = 3.14159265358979323846;
-  s.norm_drag = s.norm_drag;  // unchanged here
-  double norm_factor = (3.0 * PI * st->vis * (s.mixmeanslipmod + PIB_SMALL_NUMBER) * st->char_length) * (double)nb;
-  st->norm_factor = norm_factor;
-  for (int n = 0; n < 3; ++n) {
-    s.visc_total_old[n] = s.visc_total[n];
-    double pt = 0, vt = 0;
-    for (int m = 0; m < nb; ++m) {
-      pt += body_tot[(size_t)m * 6 + n];
-      vt += body_tot[(size_t)m * 6 + 3 + n];
-    }
-    s.pres_total[n] = pt;
-    s.visc_total[n] = vt;
-  }
-  double pa[3], va[3];
-  for (int n = 0; n < 3; ++n) {
-    double a = 0, b = 0;
-    for (int m = 0; m < nb; ++m) {
-      a += pres[(size_t)n * nb + m];
-      b += visc[(size_t)n * nb + m];
-    }
-    pa[n] = a;
-    va[n] = b;
-  }
-  s.pres_drag = sqrt(pa[0] * pa[0] + pa[1] * pa[1] + pa[2] * pa[2]) / norm_factor / (1.0 - s.maxvolfrac);
-  s.visc_drag = sqrt(va[0] * va[0] + va[1] * va[1] + va[2] * va[2]) / norm_factor / (1.0 - s.maxvolfrac);
-  s.total_drag = s.pres_drag + s.visc_drag;
-  if (!st->set_mpg) {
+  if (t == 0) {
+    double norm_factor = (3.0 * PI * st->vis * (s.mixmeanslipmod + PIB_SMALL_NUMBER) * st->char_length) * (double)nb;
+    st->norm_factor = norm_factor;
     for (int n = 0; n < 3; ++n) {
-      s.mpg[n] = s.cf * (s.ufmean_des[n] - s.ufmean[n]) +
-                 (s.pres_total[n] / st->voldom - (s.visc_total[n]) / st->voldom) / (1.0 - s.maxvolfrac);
-      if (st->include_frmeanfluid) s.mpg[n] = s.mpg[n] + s.frmeanfluid[n];
+      s.visc_total_old[n] = s.visc_total[n];
+      s.pres_total[n] = v[n];
+      s.visc_total[n] = v[3 + n];
     }
+    const double* pa = v + 6;
+    const double* va = v + 9;
+    s.pres_drag = sqrt(pa[0] * pa[0] + pa[1] * pa[1] + pa[2] * pa[2]) / norm_factor / (1.0 - s.maxvolfrac);
+    s.visc_drag = sqrt(va[0] * va[0] + va[1] * va[1] + va[2] * va[2]) / norm_factor / (1.0 - s.maxvolfrac);
+    s.total_drag = s.pres_drag + s.visc_drag;
+    if (!st->set_mpg) {
+      for (int n = 0; n < 3; ++n) {
+        s.mpg[n] = s.cf * (s.ufmean_des[n] - s.ufmean[n]) +
+                   (s.pres_total[n] / st->voldom - (s.visc_total[n]) / st->voldom) / (1.0 - s.maxvolfrac);
+        if (st->include_frmeanfluid) s.mpg[n] = s.mpg[n] + s.frmeanfluid[n];
+      }
+    }
+    for (int n = 0; n < 3; ++n) mpg_s[n] = s.mpg[n];
   }
+  __syncthreads();
   double c12 = st->coef[rks - 1][0] + st->coef[rks - 1][1];
-  for (int m = 0; m < nb; ++m) {
+  for (int m = t; m < nb; m += SCAL_T) {
     double d = 2.0 * rad[m] * st->dx;
     double vol = d * d * d;
     for (int n = 0; n < 3; ++n) {
       size_t k = (size_t)n * nb + m;
-      force[k] = c12 * (pres[k] + visc[k]) - c12 * s.mpg[n] * PI * vol / 6.0;
+      force[k] = c12 * (pres[k] + visc[k]) - c12 * mpg_s[n] * PI * vol / 6.0;
       force_chem[k] = c12 * (pres[k] + visc[k]);
     }
   }
 }
 
 // frmean / frmeanfluid over all reversal pairs (boundary_condition.F90:632-633)
-__global__ void k_reduce_frmean(DevState* st, const double* pts, int npts) {
-  __shared__ double sh[1024];
+// from the (body, chunk) partials of k_forcing_points
+__global__ void __launch_bounds__(SCAL_T) k_reduce_frmean(DevState* st, const double* fpart, int nrec) {
+  __shared__ double sh[6 * SCAL_T];
   double a[6] = {0, 0, 0, 0, 0, 0};
-  for (int p = threadIdx.x; p < npts; p += blockDim.x) {
-    const double* o = pts + (size_t)p * 12;
-    for (int q = 0; q < 6; ++q) a[q] += o[6 + q];
-  }
-  double t[6];
-  for (int q = 0; q < 6; ++q) t[q] = block_sum(a[q], sh);
+  for (int p = threadIdx.x; p < nrec; p += SCAL_T)
+    for (int q = 0; q < 6; ++q) a[q] += fpart[(size_t)p * 6 + q];
+  block_sums<6>(a, sh);
   if (threadIdx.x == 0) {
     double ntot = (double)((long long)st->nx * st->my * st->mz);
     for (int n = 0; n < 3; ++n) {
-      st->s.frmean[n] = t[n] / ntot;
-      st->s.frmeanfluid[n] = t[3 + n] / st->s.count_fluid;
+      st->s.frmean[n] = a[n] / ntot;
+      st->s.frmeanfluid[n] = a[3 + n] / st->s.count_fluid;
     }
   }
 }
 
 // bcset diagnostics (boundary_condition.F90:723-884), one phase
-__global__ void k_scalar_diag(DevState* st, const double* force, const double* force_chem) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void __launch_bounds__(SCAL_T) k_scalar_diag(DevState* st, const double* force,
+                                                        const double* force_chem) {
+  __shared__ double sh[6 * SCAL_T];
+  const int nb = st->nbody;
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (int m = threadIdx.x; m < nb; m += SCAL_T)
+    for (int n = 0; n < 3; ++n) {
+      v[n] += force[(size_t)n * nb + m];
+      v[3 + n] += force_chem[(size_t)n * nb + m];
+    }
+  block_sums<6>(v, sh);
+  if (threadIdx.x != 0) return;
   pib_scalars& s = st->s;
-  int nb = st->nbody;
   const double PI = 3.14159265358979323846;
   double af[3], ac[3];
   for (int n = 0; n < 3; ++n) {
-    double a = 0, b = 0;
-    for (int m = 0; m < nb; ++m) {
-      a += force[(size_t)n * nb + m];
-      b += force_chem[(size_t)n * nb + m];
-    }
-    af[n] = (0.0 + s.mean_volfrac * (a / nb)) / s.mean_volfrac;
-    ac[n] = (0.0 + s.mean_volfrac * (b / nb)) / s.mean_volfrac;
+    af[n] = (0.0 + s.mean_volfrac * (v[n] / nb)) / s.mean_volfrac;
+    ac[n] = (0.0 + s.mean_volfrac * (v[3 + n] / nb)) / s.mean_volfrac;
   }
   double nd = sqrt(af[0] * af[0] + af[1] * af[1] + af[2] * af[2]);
   double ndc = sqrt(ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2]);
@@ -331,10 +357,12 @@ __global__ void k_scalar_diag(DevState* st, const double* force, const double* f
 }
 
 // velstep tail: check_divergence max, umean update (steptheflow.F90:93-99)
-__global__ void k_scalar_vel(DevState* st, const double* part, int nblk, int* status) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void __launch_bounds__(SCAL_T) k_scalar_vel(DevState* st, const double* part, int nblk, int* status) {
+  __shared__ double sh[SCAL_T];
   double mx = 0.0;
-  for (int b = 0; b < nblk; ++b) mx = fmax(mx, part[b]);
+  for (int b = threadIdx.x; b < nblk; b += SCAL_T) mx = fmax(mx, part[b]);
+  mx = block_max_any(mx, sh);
+  if (threadIdx.x != 0) return;
   st->s.divmax = mx;
   for (int n = 0; n < 3; ++n)
     st->s.umean[n] = st->s.umean[n] + (-st->s.mpg[n] + st->s.frmean[n] - st->s.frame_accln[n]) * st->s.dt;
@@ -526,7 +554,7 @@ struct Impl : Base {
   uint8_t* if_rev = nullptr;
   int* nrpr = nullptr;
   double *pres = nullptr, *visc = nullptr, *torq = nullptr, *force = nullptr, *force_chem = nullptr;
-  double *drag_pts = nullptr, *body_tot = nullptr, *frc_pts = nullptr;
+  double *drag_part = nullptr, *frc_part = nullptr, *frc_pts = nullptr;
   int ntarget = 0;
   int *tnode = nullptr, *toff = nullptr, *rpt = nullptr;
   double* rw = nullptr;
@@ -854,9 +882,10 @@ struct Impl : Base {
       k_count_active<<<nb, 256, 0, stream>>>(if_rev, nbnd, nrpr);
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(hn.data(), nrpr, nb * sizeof(int), hipMemcpyDeviceToHost, stream));
-      drag_pts = alloc<double>(npts * 9, true);
-      body_tot = alloc<double>((size_t)nb * 6, true);
-      frc_pts = alloc<double>(npts * 12, true);
+      const size_t G = (nbnd + 255) / 256;
+      drag_part = alloc<double>((size_t)nb * G * 9, true);
+      frc_part = alloc<double>((size_t)nb * G * 6, true);
+      frc_pts = alloc<double>(npts * 6, true);
       build_spreading();
     }
     HIPCHK(hipStreamSynchronize(stream));
@@ -943,7 +972,7 @@ struct Impl : Base {
     c2r<C2R_UBCP>(rks, 3);
     c2r<C2R_PBCP>(rks, 1);
     kb("scalar_cnt");
-    k_scalar_cnt<<<1, 64, 0, stream>>>(dst, part, nx, rks, vel ? vel : part);
+    k_scalar_cnt<<<1, SCAL_T, 0, stream>>>(dst, part, nx, rks, vel ? vel : part);
     HIPCHK(hipGetLastError());
     ke();
   }
@@ -988,26 +1017,23 @@ struct Impl : Base {
     size_t npts = (size_t)nbody * nbnd;
     c2r<C2R_VISC>(rks, 3);
     c2r<C2R_OMEGA>(rks, 3);
+    const int G = (nbnd + 255) / 256;
     kb("drag_points");
-    k_drag_points<T><<<(unsigned)((npts + 255) / 256), 256, 0, stream>>>(args(rks), P, drag_pts);
-    HIPCHK(hipGetLastError());
-    ke();
-    kb("drag_body");
-    k_drag_body<<<nbody, 256, 0, stream>>>(drag_pts, nbnd, nbody, rks, pres, visc, torq, body_tot);
+    k_drag_points<T><<<dim3(G, nbody), 256, 0, stream>>>(args(rks), P, drag_part);
     HIPCHK(hipGetLastError());
     ke();
     kb("scalar_drag");
-    k_scalar_drag<<<1, 64, 0, stream>>>(dst, body_tot, rks, rad, pres, visc, force, force_chem);
+    k_scalar_drag<<<1, SCAL_T, 0, stream>>>(dst, drag_part, G, rks, rad, pres, visc, torq, force, force_chem);
     HIPCHK(hipGetLastError());
     ke();
     c2r<C2R_PGRAD>(rks, 3);
     if (cfg.dorpr) {
       kb("forcing_points");
-      k_forcing_points<T><<<(unsigned)((npts + 255) / 256), 256, 0, stream>>>(args(rks), P, frc_pts);
+      k_forcing_points<T><<<dim3(G, nbody), 256, 0, stream>>>(args(rks), P, frc_pts, frc_part);
       HIPCHK(hipGetLastError());
       ke();
       kb("reduce_frmean");
-      k_reduce_frmean<<<1, 1024, 0, stream>>>(dst, frc_pts, (int)npts);
+      k_reduce_frmean<<<1, SCAL_T, 0, stream>>>(dst, frc_part, G * nbody);
       HIPCHK(hipGetLastError());
       ke();
       if (ntarget > 0) {
@@ -1025,7 +1051,7 @@ struct Impl : Base {
     HIPCHK(hipGetLastError());
     ke();
     kb("scalar_diag");
-    k_scalar_diag<<<1, 64, 0, stream>>>(dst, force, force_chem);
+    k_scalar_diag<<<1, SCAL_T, 0, stream>>>(dst, force, force_chem);
     HIPCHK(hipGetLastError());
     ke();
   }
@@ -1051,7 +1077,7 @@ struct Impl : Base {
   }
   void vel_tail(int nblk) {
     kb("scalar_vel");
-    k_scalar_vel<<<1, 64, 0, stream>>>(dst, part, nblk, dstatus);
+    k_scalar_vel<<<1, SCAL_T, 0, stream>>>(dst, part, nblk, dstatus);
     HIPCHK(hipGetLastError());
     ke();
   }

@@ -106,6 +106,39 @@ __device__ inline double block_max(double v, double* sh) {
   return r;
 }
 
+// Block-wide sums of K values per lane in one deterministic tree; any block
+// size; sh holds K * blockDim.x doubles.
+template <int K>
+__device__ inline void block_sums(double* v, double* sh) {
+  const int bd = blockDim.x, t = threadIdx.x;
+  int n2 = 1;
+  while (n2 < bd) n2 <<= 1;
+  for (int q = 0; q < K; ++q) sh[q * bd + t] = v[q];
+  __syncthreads();
+  for (int s = n2 / 2; s > 0; s >>= 1) {
+    if (t < s && t + s < bd)
+      for (int q = 0; q < K; ++q) sh[q * bd + t] += sh[q * bd + t + s];
+    __syncthreads();
+  }
+  for (int q = 0; q < K; ++q) v[q] = sh[q * bd];
+  __syncthreads();
+}
+// deterministic max over any block size
+__device__ inline double block_max_any(double v, double* sh) {
+  const int bd = blockDim.x, t = threadIdx.x;
+  int n2 = 1;
+  while (n2 < bd) n2 <<= 1;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = n2 / 2; s > 0; s >>= 1) {
+    if (t < s && t + s < bd) sh[t] = fmax(sh[t], sh[t + s]);
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
 // ===========================================================================
 // c2r ops (physical fields from spectral ones)
 // ===========================================================================
@@ -446,90 +479,84 @@ struct Particles {
   int nbody, nbnd;
 };
 
-// calc_pres_visc_drag (boundary_condition.F90:1481-1645): per surface point
-// contributions pres(3), visc(3), torq(3) to per-body sums.
+// calc_pres_visc_drag (boundary_condition.F90:1481-1645).  Grid (G, nbody):
+// workgroup (g, m) takes surface points g*blockDim.x + t of body m and writes
+// the workgroup sums of their contributions, dpart[(m*G + g)*9 + q]:
+// q = 0..2 pressure part (subtracted from pres(m)), 3..5 viscous part,
+// 6..8 torque.
 template <typename T>
-__global__ void k_drag_points(KArgs<T> A, Particles P, double* out) {
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= P.nbody * P.nbnd) return;
-  int m = idx / P.nbnd, l = idx - m * P.nbnd;
-  const DevState* st = A.st;
-  double rb = P.rad[m];
-  double xl[3], xpb[3];
-  for (int n = 0; n < 3; ++n) xl[n] = P.xc[n * P.nbody + m] + P.bnd[l * 3 + n] * rb;
-  xpb[0] = xl[0] - 0.5;
-  xpb[1] = xl[1];
-  xpb[2] = xl[2];
-  Sten sp, sv;
-  make_sten(sp, xpb, A.nx, A.my, A.mz);
-  make_sten(sv, xl, A.nx, A.my, A.mz);
-  double pl = interp_ijk(sp, A.pbcp, A.nx, A.my, A.mz);
-  size_t cs = (size_t)A.nx * A.mz * A.my;
-  double vort[3];
-  for (int n = 0; n < 3; ++n) vort[n] = interp_kji(sv, A.omega + n * cs, A.nx, A.my, A.mz);
-  double cd1 = P.cd[l * 3 + 0], cd2 = P.cd[l * 3 + 1], cd3 = P.cd[l * 3 + 2];
-  double df[3];
-  df[0] = -(vort[2] * cd2 - vort[1] * cd3);
-  df[1] = -(vort[0] * cd3 - vort[2] * cd1);
-  df[2] = -(vort[1] * cd1 - vort[0] * cd2);
-  double rdx = rb * st->dx;
-  double da1 = 4.0 * 3.14159265358979323846 * (rdx * rdx) / (double)P.nbnd;
-  double cdv[3] = {cd1, cd2, cd3};
-  double tf[3];
-  double* o = out + (size_t)idx * 9;
-  for (int d = 0; d < 3; ++d) {
-    o[d] = pl * cdv[d] * da1;              // subtracted from pres(m), added to pres_total
-    tf[d] = df[d] * st->vis * da1;
-    o[3 + d] = tf[d];
-  }
-  double b1 = P.bnd[l * 3], b2 = P.bnd[l * 3 + 1], b3 = P.bnd[l * 3 + 2];
-  o[6] = (b2 * tf[2] - b3 * tf[1]) * rb * st->dx;
-  o[7] = (b3 * tf[0] - b1 * tf[2]) * rb * st->dx;
-  o[8] = (b1 * tf[1] - b2 * tf[0]) * rb * st->dx;
-}
-
-// per-body sums of the point contributions, one workgroup per body
-__global__ void k_drag_body(const double* pts, int nbnd, int nbody, int rks, double* pres, double* visc,
-                            double* torq, double* body_tot) {
-  __shared__ double sh[256];
-  int m = blockIdx.x;
-  double acc[9];
-  for (int q = 0; q < 9; ++q) acc[q] = 0.0;
-  for (int l = threadIdx.x; l < nbnd; l += blockDim.x) {
-    const double* o = pts + ((size_t)m * nbnd + l) * 9;
-    for (int q = 0; q < 9; ++q) acc[q] += o[q];
-  }
-  double tot[9];
-  for (int q = 0; q < 9; ++q) tot[q] = block_sum(acc[q], sh);
-  if (threadIdx.x == 0) {
+__global__ void __launch_bounds__(256) k_drag_points(KArgs<T> A, Particles P, double* dpart) {
+  __shared__ double sh[9 * 256];
+  const int m = blockIdx.y, G = gridDim.x;
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  double o[9];
+  for (int q = 0; q < 9; ++q) o[q] = 0.0;
+  if (l < P.nbnd) {
+    const DevState* st = A.st;
+    double rb = P.rad[m];
+    double xl[3], xpb[3];
+    for (int n = 0; n < 3; ++n) xl[n] = P.xc[n * P.nbody + m] + P.bnd[l * 3 + n] * rb;
+    xpb[0] = xl[0] - 0.5;
+    xpb[1] = xl[1];
+    xpb[2] = xl[2];
+    Sten sp, sv;
+    make_sten(sp, xpb, A.nx, A.my, A.mz);
+    make_sten(sv, xl, A.nx, A.my, A.mz);
+    double pl = interp_ijk(sp, A.pbcp, A.nx, A.my, A.mz);
+    size_t cs = (size_t)A.nx * A.mz * A.my;
+    double vort[3];
+    for (int n = 0; n < 3; ++n) vort[n] = interp_kji(sv, A.omega + n * cs, A.nx, A.my, A.mz);
+    double cd1 = P.cd[l * 3 + 0], cd2 = P.cd[l * 3 + 1], cd3 = P.cd[l * 3 + 2];
+    double df[3];
+    df[0] = -(vort[2] * cd2 - vort[1] * cd3);
+    df[1] = -(vort[0] * cd3 - vort[2] * cd1);
+    df[2] = -(vort[1] * cd1 - vort[0] * cd2);
+    double rdx = rb * st->dx;
+    double da1 = 4.0 * 3.14159265358979323846 * (rdx * rdx) / (double)P.nbnd;
+    double cdv[3] = {cd1, cd2, cd3};
+    double tf[3];
     for (int d = 0; d < 3; ++d) {
-      size_t k = (size_t)d * nbody + m;
-      double p0 = rks == 1 ? 0.0 : pres[k];
-      double v0 = rks == 1 ? 0.0 : visc[k];
-      double t0 = rks == 1 ? 0.0 : torq[k];
-      pres[k] = p0 - tot[d];
-      visc[k] = v0 + tot[3 + d];
-      torq[k] = t0 + tot[6 + d];
-      body_tot[(size_t)m * 6 + d] = tot[d];
-      body_tot[(size_t)m * 6 + 3 + d] = tot[3 + d];
+      o[d] = pl * cdv[d] * da1;
+      tf[d] = df[d] * st->vis * da1;
+      o[3 + d] = tf[d];
     }
+    double b1 = P.bnd[l * 3], b2 = P.bnd[l * 3 + 1], b3 = P.bnd[l * 3 + 2];
+    o[6] = (b2 * tf[2] - b3 * tf[1]) * rb * st->dx;
+    o[7] = (b3 * tf[0] - b1 * tf[2]) * rb * st->dx;
+    o[8] = (b1 * tf[1] - b2 * tf[0]) * rb * st->dx;
   }
+  block_sums<9>(o, sh);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 9; ++q) dpart[((size_t)m * G + blockIdx.x) * 9 + q] = o[q];
 }
 
 // calc_inner_reversal_forcing (boundary_condition.F90:1104-1479), serial path:
 // per active reversal pair, force_tmp and the solid-node redistribution share.
-// out[p*12 + 0..2] force_tmp, [3..5] force_dist, [6..8] frmean part,
-// [9..11] frmeanfluid part.
+// Grid (G, nbody): workgroup (g, m) takes points l = g*blockDim.x + t of body
+// m; out[(m*nbnd + l)*6 + 0..2] force_tmp, [3..5] force_dist; the workgroup
+// sums of the frmean / frmeanfluid parts go to fpart[(m*G + g)*6 + 0..5].
 template <typename T>
-__global__ void k_forcing_points(KArgs<T> A, Particles P, double* out) {
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= P.nbody * P.nbnd) return;
-  int m = idx / P.nbnd, l = idx - m * P.nbnd;
-  double* o = out + (size_t)idx * 12;
-  if (!P.if_rev[(size_t)m * P.nbnd + l]) {
-    for (int q = 0; q < 12; ++q) o[q] = 0.0;
-    return;
+__device__ void forcing_point(const KArgs<T>& A, const Particles& P, int m, int l, double* o, double* fsum);
+template <typename T>
+__global__ void __launch_bounds__(256) k_forcing_points(KArgs<T> A, Particles P, double* out, double* fpart) {
+  __shared__ double sh[6 * 256];
+  const int m = blockIdx.y, G = gridDim.x;
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  double fs[6] = {0, 0, 0, 0, 0, 0};
+  if (l < P.nbnd) {
+    double* o = out + ((size_t)m * P.nbnd + l) * 6;
+    if (!P.if_rev[(size_t)m * P.nbnd + l]) {
+      for (int q = 0; q < 6; ++q) o[q] = 0.0;
+    } else {
+      forcing_point(A, P, m, l, o, fs);
+    }
   }
+  block_sums<6>(fs, sh);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 6; ++q) fpart[((size_t)m * G + blockIdx.x) * 6 + q] = fs[q];
+}
+template <typename T>
+__device__ void forcing_point(const KArgs<T>& A, const Particles& P, int m, int l, double* o, double* fsum) {
   const DevState* st = A.st;
   const int rks = A.rks;
   const double c1 = st->coef[rks - 1][0], c2 = st->coef[rks - 1][1], c3 = st->coef[rks - 1][2],
@@ -608,8 +635,8 @@ __global__ void k_forcing_points(KArgs<T> A, Particles P, double* out) {
   for (int n = 0; n < 3; ++n) {
     o[n] = ft[n];
     o[3 + n] = count_so > 0 ? force_fl[n] / (double)count_so : 0.0;
-    o[6 + n] = fm[n];
-    o[9 + n] = fmf[n];
+    fsum[n] = fm[n];
+    fsum[3 + n] = fmf[n];
   }
 }
 
@@ -624,7 +651,7 @@ __global__ void k_spread(KArgs<T> A, int ntarget, const int* tnode, const int* t
   const double dx3 = A.st->dx3;
   double acc[3] = {0, 0, 0};
   for (int r = toff[t]; r < toff[t + 1]; ++r) {
-    const double* o = pts + (size_t)rpt[r] * 12;
+    const double* o = pts + (size_t)rpt[r] * 6;
     double w = rw[r];
     bool d = rdist[r] != 0;
     for (int n = 0; n < 3; ++n) {

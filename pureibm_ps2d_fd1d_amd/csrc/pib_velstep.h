@@ -462,22 +462,6 @@ __device__ void vel_tile(const KArgs<T>& A, int q0, bool mean, unsigned char* ra
   if (V.sst) status = V.sst;
 }
 
-// deterministic max over any block size
-__device__ inline double block_max_any(double v, double* sh) {
-  const int bd = blockDim.x, t = threadIdx.x;
-  int n2 = 1;
-  while (n2 < bd) n2 <<= 1;
-  sh[t] = v;
-  __syncthreads();
-  for (int s = n2 / 2; s > 0; s >>= 1) {
-    if (t < s && t + s < bd) sh[t] = fmax(sh[t], sh[t + s]);
-    __syncthreads();
-  }
-  double r = sh[0];
-  __syncthreads();
-  return r;
-}
-
 // grid: ntiles tile workgroups + 1 workgroup for the (0,0) mode;
 // block = M * nx / L lanes
 template <typename T, int L, int M>
