@@ -9,6 +9,11 @@
  * column-major layout with 1-based index helpers.  Build with
  * -ffp-contract=off so that every operation rounds as written.
  *
+ * Parity unpinned (no independent check beyond these restatements): the
+ * mpg controller of compute_mpg_at_n, the drag normalisation and ferror of
+ * bcset, and the reflected-velocity formula of calc_inner_reversal_forcing.
+ * Everything else is pinned by tests/test_oracle.py.
+ *
  * The only product-independent choices are (a) the FFT, a recursive
  * mixed-radix DIT DFT with directly evaluated twiddles standing in for FFTW
  * (whose 2-D c2r uses the row-column convention: inverse c2c along z, then

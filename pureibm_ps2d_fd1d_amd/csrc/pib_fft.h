@@ -73,6 +73,7 @@ struct Fft1D {
 
 struct PlaneGeom {
   int my, mz, my2, hy, ld;
+  int tab_off;  // byte offset from the plane buffer to the LDS copy of the tables
   Fft1D py, pz;
 };
 
@@ -84,6 +85,43 @@ struct PlaneTabs {
   const int* revy;   // slot of natural index k in a length-hy DIF output
   const int* revz;   // slot of natural index k in a length-mz DIF output
 };
+
+// bytes of LDS that stage_tabs needs after tab_off
+template <typename T>
+__host__ __device__ inline size_t tab_lds_bytes(int hy, int mz) {
+  return (size_t)(2 * hy + mz + 1) * sizeof(C2<T>) + (size_t)(hy + mz) * sizeof(int);
+}
+// Copies the twiddle and slot tables into LDS (they are read in every pass);
+// visible after the next barrier, which every transform's load phase ends
+// with.
+template <typename T>
+__device__ inline PlaneTabs<T> stage_tabs(const PlaneGeom& G, const PlaneTabs<T>& g, unsigned char* lds) {
+  C2<T>* twy = (C2<T>*)lds;
+  C2<T>* twz = twy + G.hy;
+  C2<T>* wr = twz + G.mz;
+  int* revy = (int*)(wr + G.hy + 1);
+  int* revz = revy + G.hy;
+  for (int e = threadIdx.x; e < G.hy; e += blockDim.x) {
+    twy[e] = g.twy[e];
+    revy[e] = g.revy[e];
+  }
+  for (int e = threadIdx.x; e < G.mz; e += blockDim.x) {
+    twz[e] = g.twz[e];
+    revz[e] = g.revz[e];
+  }
+  for (int e = threadIdx.x; e <= G.hy; e += blockDim.x) wr[e] = g.wr[e];
+  PlaneTabs<T> t;
+  t.twy = twy;
+  t.twz = twz;
+  t.wr = wr;
+  t.revy = revy;
+  t.revz = revz;
+  return t;
+}
+
+// loads of a plane are issued in chunks of PIB_LD_CHUNK per lane so that
+// that many global reads are in flight before the LDS writes
+#define PIB_LD_CHUNK 8
 
 // ---- radix-R DFT kernels, y_q = sum_r v_r w^{r q}, w = e^{-/+ 2 pi i/R} ----
 template <bool INV, typename T>
@@ -221,14 +259,28 @@ __device__ void fft_lines(C2<T>* buf, int L, int ls, int es, const Fft1D& pl,
 // ---- plane transforms ----------------------------------------------------
 // Forward: real plane -> (my/2+1, mz) half spectrum scaled by 1/(my*mz).
 template <int NIT, typename T, class Op>
-__device__ void plane_r2c(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& tb, Op& op) {
+__device__ void plane_r2c(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& gtb, Op& op) {
   const int hy = G.hy, ld = G.ld, mz = G.mz, my2 = G.my2;
   const int nt = blockDim.x;
-  for (int e = threadIdx.x; e < mz * hy; e += nt) {
-    int z = e / hy, m = e - z * hy;
-    T a, b;
-    op.load2(2 * m, z, a, b);
-    buf[z * ld + m] = mk(a, b);
+  const PlaneTabs<T> tb = stage_tabs(G, gtb, (unsigned char*)buf + G.tab_off);
+  for (int e0 = threadIdx.x; e0 < mz * hy; e0 += PIB_LD_CHUNK * nt) {
+    C2<T> v[PIB_LD_CHUNK];
+#pragma unroll
+    for (int u = 0; u < PIB_LD_CHUNK; ++u) {
+      int e = e0 + u * nt;
+      if (e < mz * hy) {
+        int z = e / hy, m = e - z * hy;
+        op.load2(2 * m, z, v[u].x, v[u].y);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PIB_LD_CHUNK; ++u) {
+      int e = e0 + u * nt;
+      if (e < mz * hy) {
+        int z = e / hy, m = e - z * hy;
+        buf[z * ld + m] = v[u];
+      }
+    }
   }
   __syncthreads();
   fft_lines<false>(buf, mz, ld, 1, G.py, tb.twy, false);
@@ -278,12 +330,28 @@ __device__ void plane_r2c(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& tb
 
 // Inverse: (my/2+1, mz) half spectrum -> real plane, unscaled.
 template <int NIT, typename T, class Op>
-__device__ void plane_c2r(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& tb, Op& op) {
+__device__ void plane_c2r(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& gtb, Op& op) {
   const int hy = G.hy, ld = G.ld, mz = G.mz, my2 = G.my2;
   const int nt = blockDim.x;
-  for (int e = threadIdx.x; e < my2 * mz; e += nt) {
-    int kz = e / my2, ky = e - kz * my2;
-    buf[kz * ld + ky] = op.load(ky, kz);
+  const PlaneTabs<T> tb = stage_tabs(G, gtb, (unsigned char*)buf + G.tab_off);
+  for (int e0 = threadIdx.x; e0 < my2 * mz; e0 += PIB_LD_CHUNK * nt) {
+    C2<T> v[PIB_LD_CHUNK];
+#pragma unroll
+    for (int u = 0; u < PIB_LD_CHUNK; ++u) {
+      int e = e0 + u * nt;
+      if (e < my2 * mz) {
+        int kz = e / my2, ky = e - kz * my2;
+        v[u] = op.load(ky, kz);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PIB_LD_CHUNK; ++u) {
+      int e = e0 + u * nt;
+      if (e < my2 * mz) {
+        int kz = e / my2, ky = e - kz * my2;
+        buf[kz * ld + ky] = v[u];
+      }
+    }
   }
   __syncthreads();
   fft_lines<true>(buf, my2, 1, ld, G.pz, tb.twz, true);

@@ -642,6 +642,9 @@ struct Impl : Base {
     nt_plane = E <= 256 ? 64 : (E <= 2500 ? 256 : (E <= 5000 ? 512 : 1024));
     size_t red = (size_t)nt_plane * sizeof(double);
     if (lds_plane < red) lds_plane = red;
+    lds_plane = (lds_plane + 15) / 16 * 16;
+    G.tab_off = (int)lds_plane;
+    lds_plane += tab_lds_bytes<T>(hy, mz);
     if (lds_plane > 160 * 1024)
       throw PibFail(PIB_ERR_UNSUPPORTED, "plane does not fit the 160 KB LDS of one workgroup");
     tb.twy = upload(tw_table<T>(hy, hy));

@@ -93,13 +93,15 @@ module pib_host
   integer :: nbnd = 0
   real(prcn) :: dx, doml(3), char_length, uchar(3), ucharmod, quad_r
   type(pib_config), save :: cfg
+  ! time scales of the drag output (initialize_flo.F90:556-574, 627)
+  real(prcn) :: ts_conv = 1.d0, ts_vis = 1.d0, ts_diff = large_number
   character(len=512) :: last_err = ''
   logical :: configured = .false.
 
   public :: pib_host_read, pib_host_configure, pib_host_finish, pib_host_counts, &
        pib_host_get_config, pib_host_get_particles, pib_host_get_points, &
        pib_host_last_error, pib_host_quadgener, pib_host_get_nsteps_hint, &
-       pib_host_set_option
+       pib_host_set_option, pib_host_time_scales
 contains
 
   subroutine calculate_constants
@@ -537,7 +539,21 @@ contains
     s%count_solid = real(count_solid, prcn)
     s%count_fluid = real((mx-1)*my*mz, prcn) - real(count_solid, prcn)
     s%ramp_frac_time = ramp_frac_steps*ucharmod*dt/(float(mx-1)*dx)
+    ts_conv = t_conv
+    ts_vis = (char_length*char_length)/vis
+    ts_diff = large_number
   end function pib_host_finish
+
+  ! t_conv, t_vis, t_diff of the _norm_drag.dat time columns
+  ! (boundary_condition.F90:861); valid after pib_host_finish
+  function pib_host_time_scales(tc, tv, td) bind(C, name="pib_host_time_scales") result(st)
+    real(c_double), intent(out) :: tc, tv, td
+    integer(c_int) :: st
+    tc = ts_conv
+    tv = ts_vis
+    td = ts_diff
+    st = PIB_OK
+  end function pib_host_time_scales
 
   ! Number of steps a run of TEND flow-through times would take at the initial
   ! dt (initflo prints this as the minimum number of steps).

@@ -1,0 +1,120 @@
+! pib_main.f90 -- RUN_NAME-driven fixed-bed run on the GPU step.
+!
+!   pureibm_gpu RUN_NAME [NSTEPS [fp64|fp32]]
+!
+! Reads RUN_NAME_floparam.in (and RUN_NAME_sphr_center.inp for 'default'
+! input) like the reference's main program (dtibm_main_scal_flo.F90:98-392),
+! advances NSTEPS time steps through the C-ABI and writes, once per step
+! (rks = itrmax), the reference's drag records:
+!   RUN_NAME_norm_drag.dat        boundary_condition.F90:861-862
+!   RUN_NAME_drag_components.dat  boundary_condition.F90:865
+! A blown-up run writes RUN_NAME_BLOWUP.dat and stops, as bcset does
+! (boundary_condition.F90:874-884).
+program pureibm_gpu
+  use iso_c_binding
+  use pib_types
+  use pib_host
+  implicit none
+  character(len=512) :: run_name, arg
+  integer :: nsteps, prec, n, unorm, ucomp, ios
+  integer(c_int) :: st, nb, np, nx, my, mz
+  type(pib_config) :: cfg
+  type(pib_scalars) :: sc
+  type(c_ptr) :: ctx
+  real(c_double), allocatable :: xc(:,:), rad(:), bnd(:,:), cd(:,:)
+  integer(c_int), allocatable :: nrpr(:)
+  integer(c_long_long) :: nsolid
+  real(c_double) :: tc, tv, td, ms4(4)
+  character(len=600) :: fname
+
+  call get_command_argument(1, run_name)
+  if (len_trim(run_name) == 0) then
+     write(*,'(A)') 'usage: pureibm_gpu RUN_NAME [NSTEPS [fp64|fp32]]'
+     stop 2
+  end if
+  nsteps = 10
+  call get_command_argument(2, arg)
+  if (len_trim(arg) > 0) read(arg, *, iostat=ios) nsteps
+  prec = PIB_FP64
+  call get_command_argument(3, arg)
+  if (trim(arg) == 'fp32') prec = PIB_FP32_BULK
+
+  fname = trim(run_name)//'_floparam.in'
+  call check(pib_host_read(fname, len_trim(fname)), 'reading '//trim(fname))
+  call check(pib_host_configure(run_name, len_trim(run_name)), 'configuration')
+  call check(pib_host_counts(nb, np, nx, my, mz), 'counts')
+  allocate(xc(nb,3), rad(nb), bnd(3,np), cd(3,np), nrpr(max(nb,1)))
+  call check(pib_host_get_config(cfg), 'config')
+  cfg%precision = prec
+  call check(pib_host_get_particles(xc, rad), 'particles')
+  call check(pib_host_get_points(bnd, cd), 'points')
+
+  call gpu(pib_create(cfg, ctx), 'pib_create')
+  call gpu(pib_set_points(ctx, np, bnd, cd), 'pib_set_points')
+  call gpu(pib_set_particles(ctx, nb, xc, rad, c_null_ptr, c_null_ptr, nsolid, nrpr), 'pib_set_particles')
+  call check(pib_host_finish(nsolid, sc), 'initial scalars')
+  call gpu(pib_set_scalars(ctx, sc), 'pib_set_scalars')
+  call check(pib_host_time_scales(tc, tv, td), 'time scales')
+  write(*,'(A,3I6,A,I6,A,I8)') 'GRID ', nx, my, mz, '  NBODY ', nb, '  SOLID NODES ', nsolid
+
+  open(newunit=unorm, file=trim(run_name)//'_norm_drag.dat', form='formatted', status='replace')
+  open(newunit=ucomp, file=trim(run_name)//'_drag_components.dat', form='formatted', status='replace')
+  do n = 1, nsteps
+     st = pib_step(ctx, 1)
+     if (st == PIB_ERR_BLOWUP) then
+        call gpu(pib_get_scalars(ctx, sc), 'pib_get_scalars')
+        write(*,'(A,2x,g17.8,2x,A)') 'NORM DRAG', sc%norm_drag, ' is greater than 1E+06'
+        write(*,'(A)') 'STOPPING THIS CASE AFTER WRITING THE BLOW UP INDICATOR FILE'
+        open(2000, file=trim(run_name)//'_BLOWUP.dat', form='formatted')
+        close(2000, status='keep')
+        stop
+     end if
+     call gpu(st, 'pib_step')
+     call gpu(pib_get_scalars(ctx, sc), 'pib_get_scalars')
+     write(*,'(A25,4(2x,g17.8))') 'NORM DRAGS, FERROR', sc%norm_drag, sc%ferror
+     write(unorm,'(500(2x, e20.12))') sc%t/tc/(1-sc%maxvolfrac), sc%t/tv, sc%t/td, sc%norm_drag, &
+          sc%norm_drag, sc%usmean_act(1:3), sc%ufmean(1:3)/cfg%ucharmod
+     write(ucomp,'(20(2x, e20.12))') sc%pres_drag, sc%visc_drag, sc%total_drag, sc%total_drag_mpg, &
+          sc%norm_drag, abs(sc%total_drag - sc%total_drag_mpg)/(sc%total_drag + 1.0e-35)
+  end do
+  close(unorm)
+  close(ucomp)
+  call gpu(pib_phase_times(ctx, ms4), 'pib_phase_times')
+  write(*,'(A,4F10.4)') 'LAST STEP MS (HYDRO NL BC VP) ', ms4
+  st = pib_destroy(ctx)
+
+contains
+
+  subroutine check(code, what)
+    integer(c_int), intent(in) :: code
+    character(len=*), intent(in) :: what
+    character(kind=c_char) :: buf(600)
+    character(len=600) :: msg
+    integer :: i, l
+    if (code == PIB_OK) return
+    l = pib_host_last_error(buf, 600)
+    msg = ''
+    do i = 1, l
+       msg(i:i) = buf(i)
+    end do
+    write(*,'(A,A,A,I4,A,A)') 'ERROR in ', what, ' (status ', code, '): ', trim(msg)
+    stop 1
+  end subroutine check
+
+  subroutine gpu(code, what)
+    integer(c_int), intent(in) :: code
+    character(len=*), intent(in) :: what
+    character(kind=c_char), pointer :: p(:)
+    character(len=600) :: msg
+    integer :: i
+    if (code == PIB_OK) return
+    msg = ''
+    call c_f_pointer(pib_last_error(ctx), p, [600])
+    do i = 1, 600
+       if (p(i) == c_null_char) exit
+       msg(i:i) = p(i)
+    end do
+    write(*,'(A,A,A,I4,A,A)') 'ERROR in ', what, ' (status ', code, '): ', trim(msg)
+    stop 1
+  end subroutine gpu
+end program pureibm_gpu

@@ -135,5 +135,11 @@ module pib_types
        type(c_ptr), value :: ctx
        integer(c_int) :: st
      end function pib_sync
+     function pib_step(ctx, nsteps) bind(C, name="pib_step") result(st)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nsteps
+       integer(c_int) :: st
+     end function pib_step
   end interface
 end module pib_types

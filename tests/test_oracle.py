@@ -141,11 +141,12 @@ def test_mask_brute_force(tmp_path, orc):
 
 
 # ------------------------------------------------------------- operators
-def _bare_oracle(orc, nx, my, mz, dx=0.1, vis=0.05, dt=0.01):
+def _bare_oracle(orc, nx, my, mz, dx=0.1, vis=0.05, dt=0.01, scheme=0, limiter=0):
     lib = orc.load()
     c = _abi.PibConfig()
     c.nx, c.my, c.mz = nx, my, mz
     c.dx, c.vis, c.cfl_max = dx, vis, 0.5
+    c.discr_scheme, c.limiter = scheme, limiter
     c.revernorm, c.dorpr, c.soft_start = 1, 1, 1
     c.ucharmod, c.char_length, c.dia, c.dr, c.voldom = 1.0, 1.0, 1.0, 1.0, 1.0
     ctx = ctypes.c_void_p()
@@ -264,3 +265,127 @@ def test_forcing_redistribution_conserves(tmp_path, orc):
     ff00 = o.field("ff")[: st.nx, 0, 0, :]
     assert np.max(np.abs(ff00.sum(0))) < 1e-12 * np.max(np.abs(fr))
     assert abs(s.frmean[0]) > 0
+
+
+@pytest.mark.parametrize("limiter", [0, 1])
+def test_upwind_flux_is_fromm_for_constant_advection(orc, limiter):
+    """u = U > 0 constant, v = A cos(2 pi m x/Lx), w = 0.  The Rusanov flux of
+    the MUSCL states reduces to pure upwinding, F(i+1/2) = U (v_i + s_i dx/2):
+    without a limiter s_i is the central slope (Fromm's scheme); with minmod
+    it is the central slope times phi(r) = max(0, min(1, r)),
+    r = (v_i - v_{i-1})/(v_{i+1} - v_i).  nl_2 = -(F(i+1/2) - F(i-1/2))/dx,
+    and nl_1 = 0 (u is constant, v does not depend on y)."""
+    nx, my, mz, dx = 16, 8, 8, 0.25
+    lib, ctx = _bare_oracle(orc, nx, my, mz, dx=dx, scheme=1, limiter=limiter)
+    U, A, m = 0.8, 0.3, 1
+    x = np.arange(nx) * dx
+    v = A * np.cos(2 * np.pi * m * x / (nx * dx))
+    uspec = np.zeros((nx + 1, my // 2 + 1, mz, 3), dtype=np.complex128)
+    uspec[:nx, 0, 0, 1] = v
+    uspec[nx] = uspec[0]
+    _put(lib, ctx, 0, uspec)
+    s = _abi.PibScalars()
+    lib.orc_get_scalars(ctx, ctypes.byref(s))
+    s.umean[0] = U
+    lib.orc_set_scalars(ctx, ctypes.byref(s))
+    assert lib.orc_compute_new_timestep(ctx, 1) == 0
+    assert lib.orc_nonlinear(ctx, 1) == 0
+    nlb = _get(lib, ctx, 7, (nx + 1, my, mz, 3), cplx=False)
+    vm, vp = np.roll(v, 1), np.roll(v, -1)
+    slope = (vp - vm) / (2 * dx)
+    if limiter:
+        with np.errstate(divide="ignore", invalid="ignore"):
+            r = (v - vm) / (vp - v)
+        slope = slope * np.clip(r, 0.0, 1.0)
+    F = U * (v + slope * dx / 2)  # at i+1/2
+    ref = -(F - np.roll(F, 1)) / dx
+    for j in range(my):
+        for k in range(mz):
+            assert rel_err(nlb[:nx, j, k, 1], ref) < 1e-12
+    assert np.max(np.abs(nlb[:nx, :, :, 0])) < 1e-14 * U * A / dx
+    lib.orc_destroy(ctx)
+
+
+def _one_sphere(tmp_path, orc, name):
+    st = make_setup(tmp_path, name, kind="simple", my=32, dbydx=8, re=20.0)
+    o = solver.Solver(st, lib=orc.load(), prefix="orc_")
+    m = 0
+    r = st.radbdy[m]
+    xl = st.xc[m][:, None] + st.bndpts * r  # (3, nbnd) grid units, node i at i
+    da1 = 4 * np.pi * (r * st.config.dx) ** 2 / st.nbnd
+    return st, o, xl, da1
+
+
+def test_pressure_drag_of_a_linear_pressure_field(tmp_path, orc):
+    """With u = 0 and pbcp = G * x on the pressure grid (node i at x = (i+1/2) dx)
+    the trilinear interpolation is exact, so pres = -sum_l G x_l cd_l dA and,
+    as the point set refines, -G * volume in x."""
+    st, o, xl, da1 = _one_sphere(tmp_path, orc, "PD")
+    G, dx = 0.7, st.config.dx
+    o.compute_new_timestep(1)
+    i = np.arange(1, st.nx + 1)
+    pb = np.broadcast_to((G * (i + 0.5) * dx)[:, None, None], (st.nx, st.my, st.mz))
+    o.set_field("pbcp", pb)
+    o.nonlinear(1)
+    pres = o.forces()["pres"][0]
+    ref = -np.sum(G * xl[0] * dx * st.cd * da1, axis=1)
+    assert rel_err(pres, ref) < 1e-12
+    vol = 4.0 / 3.0 * np.pi * (st.radbdy[0] * dx) ** 3
+    assert abs(pres[0] + G * vol) < 0.05 * G * vol
+    assert np.max(np.abs(o.forces()["visc"])) < 1e-14 * G * vol
+
+
+def test_viscous_drag_of_a_shear_mode(tmp_path, orc):
+    """u_x = A cos(k y) gives omega = (0, 0, A k sin(k y)) exactly at the nodes
+    (spectral y-derivative of one mode); the viscous traction on a no-slip
+    surface is nu * (omega x n) dA (n the outward normal), with omega
+    interpolated trilinearly (boundary_condition.F90:1579-1613)."""
+    st, o, xl, da1 = _one_sphere(tmp_path, orc, "VD")
+    A, dx, my, mz = 0.2, st.config.dx, st.my, st.mz
+    k = 2 * np.pi / (my * dx)
+    yj = np.arange(my) * dx
+    plane = np.broadcast_to((A * np.cos(k * yj))[:, None], (my, mz))
+    b = np.fft.rfft2(plane.T).T / (my * mz)
+    u = np.zeros((st.nx + 1, my // 2 + 1, mz, 3), dtype=np.complex128)
+    u[:, :, :, 0] = b[None]
+    o.set_field("u", u)
+    o.compute_new_timestep(1)
+    o.nonlinear(1)
+    visc = o.forces()["visc"][0]
+    # trilinear along y (omega is constant in x and z): node j (1-based) at y = j
+    y = xl[1]
+    jb = np.floor(y).astype(int)
+    eta = y - jb
+    wz_node = lambda j: A * k * np.sin(k * ((j - 1) % my) * dx)
+    wz = (1 - eta) * wz_node(jb) + eta * wz_node(jb + 1)
+    cd = st.cd
+    df = np.stack([-wz * cd[1], wz * cd[0], np.zeros_like(wz)])  # omega x n
+    ref = np.sum(st.config.vis * df * da1, axis=1)
+    assert abs(ref[0]) > 1e-3 * st.config.vis * A * k * np.sum(da1 * np.abs(cd[1]))
+    assert rel_err(visc, ref) < 1e-12
+    assert np.max(np.abs(o.forces()["pres"])) < 1e-14 * np.max(np.abs(ref))
+
+
+def test_compute_new_timestep_masked_means(tmp_path, orc):
+    """Brute-force fluid/solid/box means and max|u| of ubcp over the mask."""
+    st = make_setup(tmp_path, "CN", kind="random", my=24, dbydx=6, nbody=4, seed=12, re=20.0)
+    o = solver.Solver(st, lib=orc.load(), prefix="orc_")
+    rng = np.random.default_rng(3)
+    nx, my, mz = st.nx, st.my, st.mz
+    u = np.zeros((nx + 1, my // 2 + 1, mz, 3), dtype=np.complex128)
+    for i in range(nx):
+        for c in range(3):
+            u[i, :, :, c] = np.fft.rfft2(rng.standard_normal((mz, my))).T / (my * mz)
+    u[nx] = u[0]
+    o.set_field("u", u)
+    s0 = o.scalars()
+    o.compute_new_timestep(1)
+    s = o.scalars()
+    ub = o.field("ubcp")[:nx]
+    fl = o.field("mask") != 0
+    for c in range(3):
+        assert abs(s.ufmean[c] - ub[..., c][fl].sum() / fl.sum()) < 1e-13
+        assert abs(s.usmean_temp[c] - ub[..., c][~fl].sum() / (~fl).sum()) < 1e-13
+        assert abs(s.umean_temp[c] - ub[..., c].mean()) < 1e-13
+    assert abs(s.umax_tmp - np.abs(ub).max()) < 1e-14
+    assert s.ufmean_old[0] == s0.ufmean[0]
