@@ -555,7 +555,8 @@ struct Impl : Base {
   int* nrpr = nullptr;
   double *pres = nullptr, *visc = nullptr, *torq = nullptr, *force = nullptr, *force_chem = nullptr;
   double *drag_part = nullptr, *frc_part = nullptr, *frc_pts = nullptr;
-  int ntarget = 0;
+  int ntarget = 0, nspread_blk = 0;
+  int* spread_bt = nullptr;  // target range of each spreading workgroup
   int *tnode = nullptr, *toff = nullptr, *rpt = nullptr;
   double* rw = nullptr;
   uint8_t* rdist = nullptr;
@@ -957,6 +958,17 @@ struct Impl : Base {
     }
     hoff.push_back(acc);
     ntarget = (int)hnode.size();
+    // cut the targets into workgroup ranges of at most SPREAD_CAP records
+    std::vector<int> hbt(1, 0);
+    for (int t = 0; t < ntarget; ++t) {
+      int cnt = hoff[t + 1] - hoff[t];
+      if (cnt > SPREAD_CAP) throw PibFail(PIB_ERR_UNSUPPORTED, "more reversal records on one node than a workgroup holds");
+      if (hoff[t + 1] - hoff[hbt.back()] > SPREAD_CAP) hbt.push_back(t);
+    }
+    if (ntarget > 0) hbt.push_back(ntarget);
+    nspread_blk = (int)hbt.size() - 1;
+    spread_bt = alloc<int>(hbt.size(), true);
+    HIPCHK(hipMemcpy(spread_bt, hbt.data(), hbt.size() * sizeof(int), hipMemcpyHostToDevice));
     tnode = alloc<int>(hnode.size() + 1, true);
     toff = alloc<int>(hoff.size(), true);
     if (!hnode.empty()) HIPCHK(hipMemcpy(tnode, hnode.data(), hnode.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -979,12 +991,16 @@ struct Impl : Base {
     HIPCHK(hipGetLastError());
     ke();
   }
+  // form_nl keeps NIT spectral accumulators per lane across its three
+  // transforms; at most 512 lanes per workgroup leave each lane 256 VGPRs
+  int nt_nl() const { return nt_plane < 512 ? nt_plane : 512; }
   void form_nl(int rks) {
     size_t E = (size_t)my2 * mz;
-    int nit = (int)((E + nt_plane - 1) / nt_plane);
+    int nit = (int)((E + nt_nl() - 1) / nt_nl());
     bool up = cfg.discr_scheme == PIB_SCHEME_UPWIND, mm = cfg.limiter == PIB_LIMITER_MINMOD;
     if (!launch_nl<1>(nit, up, mm, rks) && !launch_nl<2>(nit, up, mm, rks) && !launch_nl<3>(nit, up, mm, rks) &&
-        !launch_nl<5>(nit, up, mm, rks) && !launch_nl<9>(nit, up, mm, rks) && !launch_nl<13>(nit, up, mm, rks))
+        !launch_nl<5>(nit, up, mm, rks) && !launch_nl<9>(nit, up, mm, rks) && !launch_nl<13>(nit, up, mm, rks) &&
+        !launch_nl<17>(nit, up, mm, rks) && !launch_nl<26>(nit, up, mm, rks))
       throw PibFail(PIB_ERR_UNSUPPORTED, "plane too large for the register-accumulated form_nl");
   }
   template <int NIT>
@@ -999,7 +1015,7 @@ struct Impl : Base {
   void run_nl(K kern, int rks) {
     set_lds(kern, lds_plane);
     kb("form_nl");
-    kern<<<dim3(nx, 3), nt_plane, lds_plane, stream>>>(args(rks));
+    kern<<<dim3(nx, 3), nt_nl(), lds_plane, stream>>>(args(rks));
     HIPCHK(hipGetLastError());
     ke();
   }
@@ -1041,7 +1057,7 @@ struct Impl : Base {
       ke();
       if (ntarget > 0) {
         kb("spread");
-        k_spread<T><<<(ntarget + 255) / 256, 256, 0, stream>>>(args(rks), ntarget, tnode, toff, rpt, rw, rdist,
+        k_spread<T><<<nspread_blk, 256, 0, stream>>>(args(rks), spread_bt, tnode, toff, rpt, rw, rdist,
                                                                frc_pts);
         HIPCHK(hipGetLastError());
         ke();

@@ -391,7 +391,7 @@ struct OpNl {
 };
 
 template <typename T, int NIT, bool UPWIND, bool MINMOD>
-__global__ void __launch_bounds__(1024) k_form_nl(KArgs<T> A) {
+__global__ void __launch_bounds__(512) k_form_nl(KArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
   C2<T>* buf = (C2<T>*)smem;
   const int i = plane_of(A);
@@ -640,28 +640,50 @@ __device__ void forcing_point(const KArgs<T>& A, const Particles& P, int m, int 
   }
 }
 
-// Atomics-free spreading: one thread per target grid node sums, in the
-// reference's (body, point, corner) order, the contributions of the records
-// binned to it at set-up (sorted by node, stable).
+// Atomics-free spreading.  The records (point, corner) that deposit on a
+// grid node were binned to that node at set-up (stable sort: the reference's
+// (body, point, corner) order within a node) and the target nodes were cut
+// into workgroup ranges of at most SPREAD_CAP records (bt).  A workgroup
+// first computes the contributions of its records with coalesced, batched
+// loads into LDS -- the weighted share (w F)/dx^3 and the redistributed
+// fluid share -- then one lane per target node adds its records in order:
+// acc = (acc + share) + dist, as the reference accumulates fr.
+#define SPREAD_CAP 1024
 template <typename T>
-__global__ void k_spread(KArgs<T> A, int ntarget, const int* tnode, const int* toff, const int* rpt,
-                         const double* rw, const uint8_t* rdist, const double* pts) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntarget) return;
+__global__ void __launch_bounds__(256) k_spread(KArgs<T> A, const int* bt, const int* tnode, const int* toff,
+                                                const int* rpt, const double* rw, const uint8_t* rdist,
+                                                const double* pts) {
+  __shared__ double cw[SPREAD_CAP * 3];
+  __shared__ double cdist[SPREAD_CAP * 3];
+  const int t0 = bt[blockIdx.x], t1 = bt[blockIdx.x + 1];
+  const int r0 = toff[t0], r1 = toff[t1];
   const double dx3 = A.st->dx3;
-  double acc[3] = {0, 0, 0};
-  for (int r = toff[t]; r < toff[t + 1]; ++r) {
-    const double* o = pts + (size_t)rpt[r] * 6;
-    double w = rw[r];
-    bool d = rdist[r] != 0;
-    for (int n = 0; n < 3; ++n) {
-      acc[n] = acc[n] + (w * o[n]) / dx3;
-      if (d) acc[n] = acc[n] + o[3 + n];
+  constexpr int CH = SPREAD_CAP / 256;
+#pragma unroll
+  for (int k = 0; k < CH; ++k) {
+    int r = r0 + (int)threadIdx.x + k * 256;
+    if (r < r1) {
+      const double* o = pts + (size_t)rpt[r] * 6;
+      double w = rw[r];
+      bool d = rdist[r] != 0;
+      int q = r - r0;
+      for (int n = 0; n < 3; ++n) {
+        cw[q * 3 + n] = (w * o[n]) / dx3;
+        cdist[q * 3 + n] = d ? o[3 + n] : 0.0;
+      }
     }
   }
-  size_t cs = (size_t)A.nx * A.mz * A.my;
-  int node = tnode[t];
-  for (int n = 0; n < 3; ++n) A.fr[n * cs + node] = (T)acc[n];
+  __syncthreads();
+  const size_t cs = (size_t)A.nx * A.mz * A.my;
+  for (int t = t0 + (int)threadIdx.x; t < t1; t += 256) {
+    double acc[3] = {0, 0, 0};
+    for (int r = toff[t]; r < toff[t + 1]; ++r) {
+      int q = r - r0;
+      for (int n = 0; n < 3; ++n) acc[n] = (acc[n] + cw[q * 3 + n]) + cdist[q * 3 + n];
+    }
+    int node = tnode[t];
+    for (int n = 0; n < 3; ++n) A.fr[n * cs + node] = (T)acc[n];
+  }
 }
 
 // ===========================================================================
