@@ -203,8 +203,12 @@ int pib_kernel_times(void* ctx, int maxn, char* names, double* total_ms, int* la
  *     (FP32-bulk) / 4 (FP64) x rows per lane (default); 2 = the same with
  *     4 / 2 rows per lane.  Shapes a tiled form does not take fall back to
  *     the next form down.
+ *   PIB_TUNE_GRAPH: 1 = pib_step replays whole steps from HIP graphs (default;
+ *     the last step of each call and steps under pib_set_timing run eagerly);
+ *     0 = every step is launched kernel by kernel.
  * Returns PIB_ERR_ARG for an unknown key or value. */
 #define PIB_TUNE_VELSTEP 0
+#define PIB_TUNE_GRAPH 1
 int pib_set_tuning(void* ctx, int key, int value);
 
 #ifdef __cplusplus

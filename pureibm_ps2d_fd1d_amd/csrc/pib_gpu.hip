@@ -723,6 +723,7 @@ struct Impl : Base {
   }
   ~Impl() override {
     if (stream) (void)hipStreamSynchronize(stream);
+    drop_graphs();
     for (void* q : pallocs) (void)hipFree(q);
     for (void* q : allocs) (void)hipFree(q);
     for (auto& e : ev) (void)hipEventDestroy(e);
@@ -785,8 +786,10 @@ struct Impl : Base {
     P.nbnd = nbnd;
     return P;
   }
+  bool capturing = false;
   template <typename K>
   void set_lds(K kern, size_t bytes) {
+    if (capturing) return;  // set by the eager steps that precede any capture
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
   }
   template <int KIND>
@@ -814,6 +817,7 @@ struct Impl : Base {
                      long long* count_solid, int* nrpr_active) {
     if (nb > 0 && !have_points) throw PibFail(PIB_ERR_STATE, "set_points must precede set_particles");
     HIPCHK(hipStreamSynchronize(stream));
+    drop_graphs();
     for (void* q : pallocs) (void)hipFree(q);
     pallocs.clear();
     nbody = nb;
@@ -968,11 +972,13 @@ struct Impl : Base {
     if (ntarget > 0) hbt.push_back(ntarget);
     nspread_blk = (int)hbt.size() - 1;
     spread_bt = alloc<int>(hbt.size(), true);
-    HIPCHK(hipMemcpy(spread_bt, hbt.data(), hbt.size() * sizeof(int), hipMemcpyHostToDevice));
+    // uploads go on the context stream, after the zero fill that alloc queued there
+    HIPCHK(hipMemcpyAsync(spread_bt, hbt.data(), hbt.size() * sizeof(int), hipMemcpyHostToDevice, stream));
     tnode = alloc<int>(hnode.size() + 1, true);
     toff = alloc<int>(hoff.size(), true);
-    if (!hnode.empty()) HIPCHK(hipMemcpy(tnode, hnode.data(), hnode.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(toff, hoff.data(), hoff.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!hnode.empty())
+      HIPCHK(hipMemcpyAsync(tnode, hnode.data(), hnode.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(toff, hoff.data(), hoff.size() * sizeof(int), hipMemcpyHostToDevice, stream));
     rpt = alloc<int>((size_t)acc + 1, true);
     rw = alloc<double>((size_t)acc + 1, true);
     rdist = alloc<uint8_t>((size_t)acc + 1, true);
@@ -980,6 +986,7 @@ struct Impl : Base {
       k_gather_records<<<(acc + 255) / 256, 256, 0, stream>>>(acc, rec_s, w, dist, rpt, rw, rdist);
       HIPCHK(hipGetLastError());
     }
+    HIPCHK(hipStreamSynchronize(stream));  // host vectors of the uploads end here
   }
 
   // ---------------- per-step routines ----------------
@@ -1119,19 +1126,82 @@ struct Impl : Base {
     ke();
     vel_tail(nblk);
   }
+  // ---- whole steps replayed from HIP graphs ----
+  // A step swaps the nl/onl buffers, so it is captured twice, once per buffer
+  // assignment; a step that is timed per launch or per phase runs eagerly.
+  // Graphs are captured only after eager steps have run, so every kernel
+  // attribute is already set, and are dropped when the particle set or a
+  // kernel form changes.
+  bool use_graph = true;
+  int eager_steps = 0;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  C2<T>* nl_a = nullptr;  // the nl buffer of assignment 0
+  void drop_graphs() {
+    for (auto& g : gexec)
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+  }
+  void eager_step(bool rec) {
+    for (int rks = 1; rks <= itrmax; ++rks) {
+      bool r = rec && rks == itrmax;
+      if (r) HIPCHK(hipEventRecord(ev[0], stream));
+      compute_new_timestep(rks);
+      if (r) HIPCHK(hipEventRecord(ev[1], stream));
+      nonlinear_timed(rks, r);
+      if (r) HIPCHK(hipEventRecord(ev[3], stream));
+      velstep(rks);
+      if (r) HIPCHK(hipEventRecord(ev[4], stream));
+    }
+  }
+  void graph_step() {
+    if (!nl_a) nl_a = nl;
+    int par = nl == nl_a ? 0 : 1;
+    if (!gexec[par]) {
+      C2<T>*nl0 = nl, *onl0 = onl;
+      T *nb0 = nlbc, *onb0 = onlbc;
+      bool v0 = nlbc_valid;
+      hipGraph_t g;
+      HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      capturing = true;
+      try {
+        eager_step(false);
+      } catch (...) {
+        capturing = false;
+        (void)hipStreamEndCapture(stream, &g);
+        nl = nl0;
+        onl = onl0;
+        nlbc = nb0;
+        onlbc = onb0;
+        nlbc_valid = v0;
+        throw;
+      }
+      capturing = false;
+      HIPCHK(hipStreamEndCapture(stream, &g));
+      HIPCHK(hipGraphInstantiate(&gexec[par], g, nullptr, nullptr, 0));
+      HIPCHK(hipGraphDestroy(g));
+      nl = nl0;
+      onl = onl0;
+      nlbc = nb0;
+      onlbc = onb0;
+      nlbc_valid = v0;
+    }
+    HIPCHK(hipGraphLaunch(gexec[par], stream));
+    std::swap(nl, onl);
+    std::swap(nlbc, onlbc);
+    nlbc_valid = true;
+  }
   void step_async(int nsteps) {
     if (!have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
     for (int n = 0; n < nsteps; ++n) {
       bool last = n == nsteps - 1;
-      for (int rks = 1; rks <= itrmax; ++rks) {
-        if (last && rks == itrmax) HIPCHK(hipEventRecord(ev[0], stream));
-        compute_new_timestep(rks);
-        if (last && rks == itrmax) HIPCHK(hipEventRecord(ev[1], stream));
-        nonlinear_timed(rks, last && rks == itrmax);
-        if (last && rks == itrmax) HIPCHK(hipEventRecord(ev[3], stream));
-        velstep(rks);
-        if (last && rks == itrmax) HIPCHK(hipEventRecord(ev[4], stream));
+      if (use_graph && !ktime && !last && eager_steps >= 2 && nlbc_valid) {
+        graph_step();
+        continue;
       }
+      eager_step(last);
+      ++eager_steps;
     }
     timed = nsteps > 0;
   }
@@ -1529,8 +1599,23 @@ int pib_step_async(void* ctx, int nsteps) {
 int pib_sync(void* ctx) { return finish_status(ctx, PIB_OK); }
 int pib_step(void* ctx, int nsteps) { return finish_status(ctx, pib_step_async(ctx, nsteps)); }
 int pib_set_tuning(void* ctx, int key, int value) {
-  if (key != PIB_TUNE_VELSTEP || value < 0 || value > 2) return PIB_ERR_ARG;
-  return dispatch(ctx, [&](auto& I) { I.vel_variant = value; });
+  if (key == PIB_TUNE_VELSTEP) {
+    if (value < 0 || value > 2) return PIB_ERR_ARG;
+    return dispatch(ctx, [&](auto& I) {
+      HIPCHK(hipStreamSynchronize(I.stream));
+      I.drop_graphs();
+      I.vel_variant = value;
+    });
+  }
+  if (key == PIB_TUNE_GRAPH) {
+    if (value < 0 || value > 1) return PIB_ERR_ARG;
+    return dispatch(ctx, [&](auto& I) {
+      HIPCHK(hipStreamSynchronize(I.stream));
+      I.drop_graphs();
+      I.use_graph = value == 1;
+    });
+  }
+  return PIB_ERR_ARG;
 }
 int pib_set_timing(void* ctx, int on) {
   return dispatch(ctx, [&](auto& I) { I.set_timing(on); });

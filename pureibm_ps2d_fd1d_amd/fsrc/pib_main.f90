@@ -53,6 +53,7 @@ program pureibm_gpu
   call gpu(pib_set_points(ctx, np, bnd, cd), 'pib_set_points')
   call gpu(pib_set_particles(ctx, nb, xc, rad, c_null_ptr, c_null_ptr, nsolid, nrpr), 'pib_set_particles')
   call check(pib_host_finish(nsolid, sc), 'initial scalars')
+  write(*,'(A25,4(2x,g17.8))') 'DT, UMEAN, UFMEAN_DES', sc%dt, sc%umean(1), sc%ufmean_des(1)
   call gpu(pib_set_scalars(ctx, sc), 'pib_set_scalars')
   call check(pib_host_time_scales(tc, tv, td), 'time scales')
   write(*,'(A,3I6,A,I6,A,I8)') 'GRID ', nx, my, mz, '  NBODY ', nb, '  SOLID NODES ', nsolid
@@ -72,6 +73,7 @@ program pureibm_gpu
      call gpu(st, 'pib_step')
      call gpu(pib_get_scalars(ctx, sc), 'pib_get_scalars')
      write(*,'(A25,4(2x,g17.8))') 'NORM DRAGS, FERROR', sc%norm_drag, sc%ferror
+     write(*,'(A25,4(2x,g17.8))') 'T, DT, MPG, UFMEAN', sc%t, sc%dt, sc%mpg(1), sc%ufmean(1)
      write(unorm,'(500(2x, e20.12))') sc%t/tc/(1-sc%maxvolfrac), sc%t/tv, sc%t/td, sc%norm_drag, &
           sc%norm_drag, sc%usmean_act(1:3), sc%ufmean(1:3)/cfg%ucharmod
      write(ucomp,'(20(2x, e20.12))') sc%pres_drag, sc%visc_drag, sc%total_drag, sc%total_drag_mpg, &

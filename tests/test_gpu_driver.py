@@ -22,6 +22,14 @@ def test_driver_writes_reference_drag_records(tmp_path, orc):
     st = make_setup(tmp_path, "DRV", **kw)
     o = solver.Solver(st, lib=orc.load(), prefix="orc_")
     nsteps = 5
+    # the same single-step calls from Python, for comparison with the driver
+    g = solver.Solver(st)
+    for n in range(nsteps):
+        g.step(1)
+        sg = g.scalars()
+        print(f"  python step {n + 1}: norm_drag {sg.norm_drag:.10e} t {sg.t:.6e} mpg {sg.mpg[0]:.6e} "
+              f"ufmean {sg.ufmean[0]:.6e} ufmean_des {sg.ufmean_des[0]:.6e}", flush=True)
+    g.close()
     run = os.path.join(str(tmp_path), "DRV")
     r = subprocess.run([DRIVER, run, str(nsteps), "fp64"], capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
