@@ -203,12 +203,16 @@ int pib_kernel_times(void* ctx, int maxn, char* names, double* total_ms, int* la
  *     (FP32-bulk) / 4 (FP64) x rows per lane (default); 2 = the same with
  *     4 / 2 rows per lane.  Shapes a tiled form does not take fall back to
  *     the next form down.
+ *   PIB_TUNE_FFT: 1 = plane transforms of a compiled shape (my/2, mz) in
+ *     {(64,128), (32,64), (16,32)} use the two-stage kernels (default); 0 =
+ *     always the generic mixed-radix transform.
  *   PIB_TUNE_GRAPH: 1 = pib_step replays whole steps from HIP graphs (default;
  *     the last step of each call and steps under pib_set_timing run eagerly);
  *     0 = every step is launched kernel by kernel.
  * Returns PIB_ERR_ARG for an unknown key or value. */
 #define PIB_TUNE_VELSTEP 0
 #define PIB_TUNE_GRAPH 1
+#define PIB_TUNE_FFT 2
 int pib_set_tuning(void* ctx, int key, int value);
 
 #ifdef __cplusplus

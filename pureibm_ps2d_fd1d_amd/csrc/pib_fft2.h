@@ -1,0 +1,375 @@
+// pib_fft2.h -- plane FFT for power-of-two plane sizes, two stages per axis.
+//
+// Same transforms and the same Op interface as pib_fft.h (ff2rc / ff2cr of
+// fftw_interface.F90:256-338), for my/2 = HY and mz = MZ known at compile
+// time.  Each 1-D transform of length N = R1*R2 is a four-step FFT whose two
+// stages run in registers (radix-4/8/16), with the stage that stays in LDS
+// arranged to work in place:
+//   decimation in time  n = n1 + R1 n2, k = k2 + R2 k1:
+//     A: per n1, DFT_R2 over n2, times W_N^(n1 k2), to slot n1 R2 + k2
+//     B: per k2, DFT_R1 over n1 of slots n1 R2 + k2, to slots k2 + R2 k1
+//        (the same slots: in place)
+//   decimation in frequency  n = R2 a + b, k = c + R1 d:
+//     A: per b, DFT_R1 over a of slots R2 a + b, times W_N^(b c), to slots
+//        R2 c + b (the same slots: in place)
+//     B: per c, DFT_R2 over b of slots R2 c + b, giving X[c + R1 d]
+// so that the first stage of a transform can read straight from HBM (DIT)
+// and the last can write straight to HBM (DIF):
+//   c2r: z DIT-A (HBM -> LDS) | z DIT-B | merge | y DIF-A | y DIF-B (-> HBM)
+//   r2c: y DIT-A (HBM -> LDS) | y DIT-B | split | z DIF-A | z DIF-B (-> HBM)
+// four barriers and eight plane-sized LDS sweeps per transform.  Results are
+// in natural order, so no slot tables are needed.
+//
+// Every phase walks its items as e = threadIdx.x + j*NT, j < ceil(items/NT).
+// `it` of op.store / op.store2 is j*R + r (walk round j, output r of the
+// item): below r2c_nout / c2r_nout.
+#pragma once
+#include "pib_fft.h"
+
+template <int N>
+struct Split2;
+template <>
+struct Split2<8> {
+  static constexpr int A = 2, B = 4;
+};
+template <>
+struct Split2<16> {
+  static constexpr int A = 4, B = 4;
+};
+template <>
+struct Split2<32> {
+  static constexpr int A = 4, B = 8;
+};
+template <>
+struct Split2<64> {
+  static constexpr int A = 8, B = 8;
+};
+template <>
+struct Split2<128> {
+  static constexpr int A = 8, B = 16;
+};
+template <>
+struct Split2<256> {
+  static constexpr int A = 16, B = 16;
+};
+
+constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// e^{-2 pi i j/16} for the products j = n1*k2 of a 4 x 4 split
+template <typename T>
+__device__ __forceinline__ C2<T> w16(int j) {
+  const T c1 = (T)0.92387953251128675612818318939678828682;  // cos(pi/8)
+  const T s1 = (T)0.38268343236508977172845998403039886676;  // sin(pi/8)
+  const T h = (T)0.70710678118654752440084436210484903928;   // cos(pi/4)
+  switch (j) {
+    case 1: return mk(c1, -s1);
+    case 2: return mk(h, -h);
+    case 3: return mk(s1, -c1);
+    case 4: return mk((T)0, (T)-1);
+    case 6: return mk(-h, -h);
+    case 9: return mk(-c1, s1);
+    default: return mk((T)1, (T)0);
+  }
+}
+
+template <bool INV, typename T>
+__device__ __forceinline__ void dft16(C2<T>* v) {
+  C2<T> a[4][4];
+#pragma unroll
+  for (int n1 = 0; n1 < 4; ++n1) {
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) a[n1][n2] = v[n1 + 4 * n2];
+    dft4<INV>(a[n1]);
+#pragma unroll
+    for (int k2 = 1; k2 < 4; ++k2)
+      if (n1 > 0) {
+        C2<T> w = w16<T>(n1 * k2);
+        if (INV) w.y = -w.y;
+        a[n1][k2] = cmul(a[n1][k2], w);
+      }
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < 4; ++k2) {
+    C2<T> b[4] = {a[0][k2], a[1][k2], a[2][k2], a[3][k2]};
+    dft4<INV>(b);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) v[k2 + 4 * k1] = b[k1];
+  }
+}
+template <bool INV, int R, typename T>
+__device__ __forceinline__ void dft2n(C2<T>* v) {
+  if constexpr (R == 2) dft2<INV>(v);
+  else if constexpr (R == 4) dft4<INV>(v);
+  else if constexpr (R == 8) dft8<INV>(v);
+  else dft16<INV>(v);
+}
+// v * W^j (forward) or v * conj(W^j) (inverse), W^j from a table in LDS
+template <bool INV, typename T>
+__device__ __forceinline__ C2<T> twmul(C2<T> v, const C2<T>* tw, int j) {
+  if (j == 0) return v;
+  C2<T> w = tw[j];
+  if (INV) w.y = -w.y;
+  return cmul(v, w);
+}
+
+template <int HY, int MZ, int NT>
+struct Plane2 {
+  static constexpr int MY2 = HY + 1, LD = HY + 1;
+  static constexpr int ZA = Split2<MZ>::A, ZB = Split2<MZ>::B;
+  static constexpr int YA = Split2<HY>::A, YB = Split2<HY>::B;
+  // outputs per lane of the last phase (the Op's `it` bound)
+  static constexpr int c2r_nout = cdiv(MZ * YA, NT) * YB;  // y DIF-B items (z, c), YB outputs each
+  // z DIF-B items (ky < HY, c), ZB outputs each; the ky = HY column's items
+  // go to lanes c < ZA with it = -1
+  static constexpr int r2c_nout = cdiv(HY * ZA, NT) * ZB;
+};
+
+// ---------------------------------------------------------------------------
+// Inverse: (HY+1, MZ) half spectrum -> real plane, unscaled; Im of the ky = 0
+// and ky = HY entries is ignored after the z transform.
+//   z: DIT with R1 = ZA, R2 = ZB;  y: DIF with R1 = YA, R2 = YB.
+template <int HY, int MZ, int NT, typename T, class Op>
+__device__ void plane_c2r2(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& gtb, Op& op) {
+  using P = Plane2<HY, MZ, NT>;
+  constexpr int MY2 = P::MY2, LD = P::LD, ZA = P::ZA, ZB = P::ZB, YA = P::YA, YB = P::YB;
+  const int t = threadIdx.x;
+  const PlaneTabs<T> tb = stage_tabs(G, gtb, (unsigned char*)buf + G.tab_off);
+  __syncthreads();  // the first stage already multiplies by twiddles
+  // z DIT-A: item (ky, n1), ky fastest; reads HBM
+#pragma unroll
+  for (int j = 0; j < cdiv(MY2 * ZA, NT); ++j) {
+    const int e = t + j * NT;
+    if (e < MY2 * ZA) {
+      const int ky = e % MY2, n1 = e / MY2;
+      C2<T> v[ZB];
+#pragma unroll
+      for (int n2 = 0; n2 < ZB; ++n2) v[n2] = op.load(ky, n1 + ZA * n2);
+      dft2n<true, ZB>(v);
+#pragma unroll
+      for (int k2 = 0; k2 < ZB; ++k2) buf[(n1 * ZB + k2) * LD + ky] = twmul<true>(v[k2], tb.twz, n1 * k2);
+    }
+  }
+  __syncthreads();
+  // z DIT-B: item (ky, k2), in place
+#pragma unroll
+  for (int j = 0; j < cdiv(MY2 * ZB, NT); ++j) {
+    const int e = t + j * NT;
+    if (e < MY2 * ZB) {
+      const int ky = e % MY2, k2 = e / MY2;
+      C2<T> v[ZA];
+#pragma unroll
+      for (int n1 = 0; n1 < ZA; ++n1) v[n1] = buf[(n1 * ZB + k2) * LD + ky];
+      dft2n<true, ZA>(v);
+#pragma unroll
+      for (int k1 = 0; k1 < ZA; ++k1) buf[(k2 + ZB * k1) * LD + ky] = v[k1];
+    }
+  }
+  __syncthreads();
+  // merge: Z_k = A_k + i B_k, A_k = X_k + conj X_{HY-k},
+  // B_k = (X_k - conj X_{HY-k}) e^{+2 pi i k/my}; pairs (k, HY-k) in place
+  constexpr int NP = HY / 2 + 1;
+#pragma unroll
+  for (int j = 0; j < cdiv(MZ * NP, NT); ++j) {
+    const int e = t + j * NT;
+    if (e < MZ * NP) {
+      const int k = e % NP, z = e / NP;
+      C2<T>* row = buf + z * LD;
+      if (k == 0) {
+        T x0 = row[0].x, xh = row[HY].x;
+        row[0] = mk(x0 + xh, x0 - xh);
+      } else if (2 * k == HY) {
+        C2<T> a = row[k];
+        row[k] = mk(a.x + a.x, -(a.y + a.y));
+      } else {
+        C2<T> a = row[k], b = row[HY - k];
+        C2<T> A = mk(a.x + b.x, a.y - b.y);
+        C2<T> D = mk(a.x - b.x, a.y + b.y);
+        C2<T> B = cmul(D, cconj(tb.wr[k]));
+        row[k] = mk(A.x - B.y, A.y + B.x);
+        row[HY - k] = mk(A.x + B.y, B.x - A.y);
+      }
+    }
+  }
+  __syncthreads();
+  // y DIF-A: item (z, b), b fastest; slots R2 a + b, in place
+#pragma unroll
+  for (int j = 0; j < cdiv(MZ * YB, NT); ++j) {
+    const int e = t + j * NT;
+    if (e < MZ * YB) {
+      const int b = e % YB, z = e / YB;
+      C2<T>* row = buf + z * LD;
+      C2<T> v[YA];
+#pragma unroll
+      for (int a = 0; a < YA; ++a) v[a] = row[YB * a + b];
+      dft2n<true, YA>(v);
+#pragma unroll
+      for (int c = 0; c < YA; ++c) row[YB * c + b] = twmul<true>(v[c], tb.twy, b * c);
+    }
+  }
+  __syncthreads();
+  // y DIF-B: item (z, c), writes y = 2m, 2m+1 for m = c + YA d
+#pragma unroll
+  for (int j = 0; j < cdiv(MZ * YA, NT); ++j) {
+    const int e = t + j * NT;
+    if (e < MZ * YA) {
+      const int c = e % YA, z = e / YA;
+      const C2<T>* row = buf + z * LD;
+      C2<T> v[YB];
+#pragma unroll
+      for (int b = 0; b < YB; ++b) v[b] = row[YB * c + b];
+      dft2n<true, YB>(v);
+#pragma unroll
+      for (int d = 0; d < YB; ++d) op.store2(j * YB + d, 2 * (c + YA * d), z, v[d].x, v[d].y);
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Forward: real plane -> (HY+1, MZ) half spectrum scaled by 1/(my*mz).
+//   y: DIT with R1 = YA, R2 = YB;  z: DIF with R1 = ZA, R2 = ZB.
+template <int HY, int MZ, int NT, typename T, class Op>
+__device__ void plane_r2c2(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& gtb, Op& op) {
+  using P = Plane2<HY, MZ, NT>;
+  constexpr int MY2 = P::MY2, LD = P::LD, ZA = P::ZA, ZB = P::ZB, YA = P::YA, YB = P::YB;
+  const int t = threadIdx.x;
+  const PlaneTabs<T> tb = stage_tabs(G, gtb, (unsigned char*)buf + G.tab_off);
+  __syncthreads();  // the first stage already multiplies by twiddles
+  // y DIT-A: item (z, n1), n1 fastest; pairs (y, y+1) from HBM as one complex
+#pragma unroll
+  for (int j = 0; j < cdiv(MZ * YA, NT); ++j) {
+    const int e = t + j * NT;
+    if (e < MZ * YA) {
+      const int n1 = e % YA, z = e / YA;
+      C2<T> v[YB];
+#pragma unroll
+      for (int n2 = 0; n2 < YB; ++n2) op.load2(2 * (n1 + YA * n2), z, v[n2].x, v[n2].y);
+      dft2n<false, YB>(v);
+      C2<T>* row = buf + z * LD;
+#pragma unroll
+      for (int k2 = 0; k2 < YB; ++k2) row[n1 * YB + k2] = twmul<false>(v[k2], tb.twy, n1 * k2);
+    }
+  }
+  __syncthreads();
+  // y DIT-B: item (z, k2), in place
+#pragma unroll
+  for (int j = 0; j < cdiv(MZ * YB, NT); ++j) {
+    const int e = t + j * NT;
+    if (e < MZ * YB) {
+      const int k2 = e % YB, z = e / YB;
+      C2<T>* row = buf + z * LD;
+      C2<T> v[YA];
+#pragma unroll
+      for (int n1 = 0; n1 < YA; ++n1) v[n1] = row[n1 * YB + k2];
+      dft2n<false, YA>(v);
+#pragma unroll
+      for (int k1 = 0; k1 < YA; ++k1) row[k2 + YB * k1] = v[k1];
+    }
+  }
+  __syncthreads();
+  // split: X_k = E_k + w^k O_k, X_{hy-k} = conj(E_k - w^k O_k); pairs in place
+  constexpr int NP = HY / 2 + 1;
+#pragma unroll
+  for (int j = 0; j < cdiv(MZ * NP, NT); ++j) {
+    const int e = t + j * NT;
+    if (e < MZ * NP) {
+      const int k = e % NP, z = e / NP;
+      C2<T>* row = buf + z * LD;
+      if (k == 0) {
+        C2<T> z0 = row[0];
+        row[0] = mk(z0.x + z0.y, (T)0);
+        row[HY] = mk(z0.x - z0.y, (T)0);
+      } else if (2 * k == HY) {
+        row[k] = cconj(row[k]);
+      } else {
+        C2<T> a = row[k], b = row[HY - k];
+        C2<T> E = mk((a.x + b.x) * (T)0.5, (a.y - b.y) * (T)0.5);
+        C2<T> O = mk((a.y + b.y) * (T)0.5, (b.x - a.x) * (T)0.5);
+        C2<T> wO = cmul(tb.wr[k], O);
+        row[k] = E + wO;
+        row[HY - k] = cconj(E - wO);
+      }
+    }
+  }
+  __syncthreads();
+  // z DIF-A: item (ky, b), ky fastest; rows R2 a + b, in place
+#pragma unroll
+  for (int j = 0; j < cdiv(MY2 * ZB, NT); ++j) {
+    const int e = t + j * NT;
+    if (e < MY2 * ZB) {
+      const int ky = e % MY2, b = e / MY2;
+      C2<T> v[ZA];
+#pragma unroll
+      for (int a = 0; a < ZA; ++a) v[a] = buf[(ZB * a + b) * LD + ky];
+      dft2n<false, ZA>(v);
+#pragma unroll
+      for (int c = 0; c < ZA; ++c) buf[(ZB * c + b) * LD + ky] = twmul<false>(v[c], tb.twz, b * c);
+    }
+  }
+  __syncthreads();
+  // z DIF-B: item (ky, c), outputs kz = c + ZA d; columns ky < HY first,
+  // then the ky = HY column on lanes c < ZA (it = -1)
+  const T scale = (T)((double)(2 * HY) * (double)MZ);
+#pragma unroll
+  for (int j = 0; j < cdiv(HY * ZA, NT); ++j) {
+    const int e = t + j * NT;
+    if (e < HY * ZA) {
+      const int ky = e % HY, c = e / HY;
+      C2<T> v[ZB];
+#pragma unroll
+      for (int b = 0; b < ZB; ++b) v[b] = buf[(ZB * c + b) * LD + ky];
+      dft2n<false, ZB>(v);
+#pragma unroll
+      for (int d = 0; d < ZB; ++d) op.store(j * ZB + d, ky, c + ZA * d, mk(v[d].x / scale, v[d].y / scale));
+    }
+  }
+  if (t < ZA) {
+    const int c = t;
+    C2<T> v[ZB];
+#pragma unroll
+    for (int b = 0; b < ZB; ++b) v[b] = buf[(ZB * c + b) * LD + HY];
+    dft2n<false, ZB>(v);
+#pragma unroll
+    for (int d = 0; d < ZB; ++d) op.store(-1, HY, c + ZA * d, mk(v[d].x / scale, v[d].y / scale));
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Compiled plane shapes: SH selects (HY, MZ) and the workgroup size; SH = 0
+// is the generic mixed-radix transform of pib_fft.h (any 2,3,5-smooth size).
+template <int SH>
+struct Shape {
+  static constexpr int HY = 0, MZ = 0, NT = 0;
+};
+template <>
+struct Shape<1> {
+  static constexpr int HY = 64, MZ = 128, NT = 1024;
+};
+template <>
+struct Shape<2> {
+  static constexpr int HY = 32, MZ = 64, NT = 256;
+};
+template <>
+struct Shape<3> {
+  static constexpr int HY = 16, MZ = 32, NT = 128;
+};
+static inline int shape_of(int hy, int mz) {
+  if (hy == 64 && mz == 128) return 1;
+  if (hy == 32 && mz == 64) return 2;
+  if (hy == 16 && mz == 32) return 3;
+  return 0;
+}
+static inline int shape_nt(int sh) {
+  return sh == 1 ? Shape<1>::NT : (sh == 2 ? Shape<2>::NT : (sh == 3 ? Shape<3>::NT : 0));
+}
+template <int SH, int NIT, typename T, class Op>
+__device__ __forceinline__ void plane_c2r_sh(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& tb, Op& op) {
+  if constexpr (SH == 0) plane_c2r<NIT>(buf, G, tb, op);
+  else plane_c2r2<Shape<SH>::HY, Shape<SH>::MZ, Shape<SH>::NT>(buf, G, tb, op);
+}
+template <int SH, int NIT, typename T, class Op>
+__device__ __forceinline__ void plane_r2c_sh(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& tb, Op& op) {
+  if constexpr (SH == 0) plane_r2c<NIT>(buf, G, tb, op);
+  else plane_r2c2<Shape<SH>::HY, Shape<SH>::MZ, Shape<SH>::NT>(buf, G, tb, op);
+}

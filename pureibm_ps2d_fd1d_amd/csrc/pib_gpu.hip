@@ -641,11 +641,13 @@ struct Impl : Base {
     lds_plane = (size_t)mz * G.ld * sizeof(C2<T>);
     size_t E = (size_t)my2 * mz;
     nt_plane = E <= 256 ? 64 : (E <= 2500 ? 256 : (E <= 5000 ? 512 : 1024));
-    size_t red = (size_t)nt_plane * sizeof(double);
+    sh = shape_of(hy, mz);
+    size_t red = (size_t)std::max(nt_plane, shape_nt(sh)) * sizeof(double);
     if (lds_plane < red) lds_plane = red;
     lds_plane = (lds_plane + 15) / 16 * 16;
     G.tab_off = (int)lds_plane;
     lds_plane += tab_lds_bytes<T>(hy, mz);
+    lds_plane += (size_t)mz * sizeof(C2<T>);  // form_nl: ky = my/2 column accumulators
     if (lds_plane > 160 * 1024)
       throw PibFail(PIB_ERR_UNSUPPORTED, "plane does not fit the 160 KB LDS of one workgroup");
     tb.twy = upload(tw_table<T>(hy, hy));
@@ -792,15 +794,30 @@ struct Impl : Base {
     if (capturing) return;  // set by the eager steps that precede any capture
     HIPCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
   }
-  template <int KIND>
-  void c2r(int rks, int ncomp) {
+  // plane transforms: the compiled shape sh (pib_fft2.h) when the plane has
+  // one and fft_compiled is set, else the generic mixed-radix transform
+  int sh = 0;
+  bool fft_compiled = true;
+  int cur_sh() const { return fft_compiled ? sh : 0; }
+  int plane_nt(int s) const { return s ? shape_nt(s) : nt_plane; }
+  template <int KIND, int SH>
+  void launch_c2r(int rks, int ncomp) {
     static const char* names[] = {"c2r_ubcp", "c2r_pbcp", "c2r_nlbc", "c2r_onlbc", "c2r_visc", "c2r_omega", "c2r_pgrad"};
-    auto kern = k_c2r<T, KIND>;
+    auto kern = k_c2r<T, KIND, SH>;
     set_lds(kern, lds_plane);
     kb(names[KIND]);
-    kern<<<dim3(nx, ncomp), nt_plane, lds_plane, stream>>>(args(rks));
+    kern<<<dim3(nx, ncomp), plane_nt(SH), lds_plane, stream>>>(args(rks));
     HIPCHK(hipGetLastError());
     ke();
+  }
+  template <int KIND>
+  void c2r(int rks, int ncomp) {
+    switch (cur_sh()) {
+      case 1: launch_c2r<KIND, 1>(rks, ncomp); break;
+      case 2: launch_c2r<KIND, 2>(rks, ncomp); break;
+      case 3: launch_c2r<KIND, 3>(rks, ncomp); break;
+      default: launch_c2r<KIND, 0>(rks, ncomp);
+    }
   }
 
   // ---------------- set-up ----------------
@@ -1002,6 +1019,13 @@ struct Impl : Base {
   // transforms; at most 512 lanes per workgroup leave each lane 256 VGPRs
   int nt_nl() const { return nt_plane < 512 ? nt_plane : 512; }
   void form_nl(int rks) {
+    bool up0 = cfg.discr_scheme == PIB_SCHEME_UPWIND, mm0 = cfg.limiter == PIB_LIMITER_MINMOD;
+    switch (cur_sh()) {
+      case 1: return launch_nl_sh<1>(up0, mm0, rks);
+      case 2: return launch_nl_sh<2>(up0, mm0, rks);
+      case 3: return launch_nl_sh<3>(up0, mm0, rks);
+      default: break;
+    }
     size_t E = (size_t)my2 * mz;
     int nit = (int)((E + nt_nl() - 1) / nt_nl());
     bool up = cfg.discr_scheme == PIB_SCHEME_UPWIND, mm = cfg.limiter == PIB_LIMITER_MINMOD;
@@ -1013,16 +1037,23 @@ struct Impl : Base {
   template <int NIT>
   bool launch_nl(int nit, bool up, bool mm, int rks) {
     if (nit > NIT) return false;
-    if (!up) run_nl(k_form_nl<T, NIT, false, false>, rks);
-    else if (!mm) run_nl(k_form_nl<T, NIT, true, false>, rks);
-    else run_nl(k_form_nl<T, NIT, true, true>, rks);
+    if (!up) run_nl(k_form_nl<T, NIT, false, false, 0>, rks, nt_nl());
+    else if (!mm) run_nl(k_form_nl<T, NIT, true, false, 0>, rks, nt_nl());
+    else run_nl(k_form_nl<T, NIT, true, true, 0>, rks, nt_nl());
     return true;
   }
+  template <int SH>
+  void launch_nl_sh(bool up, bool mm, int rks) {
+    constexpr int NIT = NlShape<SH>::NIT, NT = NlShape<SH>::NT;
+    if (!up) run_nl(k_form_nl<T, NIT, false, false, SH>, rks, NT);
+    else if (!mm) run_nl(k_form_nl<T, NIT, true, false, SH>, rks, NT);
+    else run_nl(k_form_nl<T, NIT, true, true, SH>, rks, NT);
+  }
   template <typename K>
-  void run_nl(K kern, int rks) {
+  void run_nl(K kern, int rks, int nt) {
     set_lds(kern, lds_plane);
     kb("form_nl");
-    kern<<<dim3(nx, 3), nt_nl(), lds_plane, stream>>>(args(rks));
+    kern<<<dim3(nx, 3), nt, lds_plane, stream>>>(args(rks));
     HIPCHK(hipGetLastError());
     ke();
   }
@@ -1070,12 +1101,12 @@ struct Impl : Base {
         ke();
       }
     }
-    auto kern = k_r2c_force<T>;
-    set_lds(kern, lds_plane);
-    kb("r2c_force");
-    kern<<<dim3(nx, 3), nt_plane, lds_plane, stream>>>(args(rks));
-    HIPCHK(hipGetLastError());
-    ke();
+    switch (cur_sh()) {
+      case 1: r2c_force<1>(rks); break;
+      case 2: r2c_force<2>(rks); break;
+      case 3: r2c_force<3>(rks); break;
+      default: r2c_force<0>(rks);
+    }
     kb("scalar_diag");
     k_scalar_diag<<<1, SCAL_T, 0, stream>>>(dst, force, force_chem);
     HIPCHK(hipGetLastError());
@@ -1104,6 +1135,15 @@ struct Impl : Base {
   void vel_tail(int nblk) {
     kb("scalar_vel");
     k_scalar_vel<<<1, SCAL_T, 0, stream>>>(dst, part, nblk, dstatus);
+    HIPCHK(hipGetLastError());
+    ke();
+  }
+  template <int SH>
+  void r2c_force(int rks) {
+    auto kern = k_r2c_force<T, SH>;
+    set_lds(kern, lds_plane);
+    kb("r2c_force");
+    kern<<<dim3(nx, 3), plane_nt(SH), lds_plane, stream>>>(args(rks));
     HIPCHK(hipGetLastError());
     ke();
   }
@@ -1605,6 +1645,14 @@ int pib_set_tuning(void* ctx, int key, int value) {
       HIPCHK(hipStreamSynchronize(I.stream));
       I.drop_graphs();
       I.vel_variant = value;
+    });
+  }
+  if (key == PIB_TUNE_FFT) {
+    if (value < 0 || value > 1) return PIB_ERR_ARG;
+    return dispatch(ctx, [&](auto& I) {
+      HIPCHK(hipStreamSynchronize(I.stream));
+      I.drop_graphs();
+      I.fft_compiled = value == 1;
     });
   }
   if (key == PIB_TUNE_GRAPH) {

@@ -11,7 +11,7 @@
 // x = i, pressure node i at x = i + 1/2; particle centres xc are in grid
 // units.
 #pragma once
-#include "pib_fft.h"
+#include "pib_fft2.h"
 #include "../../include/pureibm_gpu.h"
 
 #define PIB_SMALL_NUMBER ((double)1.0E-35f)
@@ -253,7 +253,7 @@ struct OpPgrad {
 // generic c2r launcher body: grid (nplanes, ncomp)
 enum { C2R_UBCP = 0, C2R_PBCP, C2R_NLBC, C2R_ONLBC, C2R_VISC, C2R_OMEGA, C2R_PGRAD };
 
-template <typename T, int KIND>
+template <typename T, int KIND, int SH>
 __global__ void k_c2r(KArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
   C2<T>* buf = (C2<T>*)smem;
@@ -261,7 +261,7 @@ __global__ void k_c2r(KArgs<T> A) {
   const int c = blockIdx.y;
   if constexpr (KIND == C2R_UBCP) {
     OpUbcp<T> op(A, c, i);
-    plane_c2r<0>(buf, A.G, A.tb, op);
+    plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op);
     double* sh = (double*)smem;
     double sf = block_sum(op.sf, sh);
     double ss = block_sum(op.ss, sh);
@@ -273,21 +273,21 @@ __global__ void k_c2r(KArgs<T> A) {
     }
   } else if constexpr (KIND == C2R_PBCP) {
     OpPlain<T> op{A.p, A.pbcp, A, (size_t)i * A.mz * A.my2, (size_t)i * A.mz * A.my, (T)0};
-    plane_c2r<0>(buf, A.G, A.tb, op);
+    plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op);
   } else if constexpr (KIND == C2R_NLBC || KIND == C2R_ONLBC) {
     const C2<T>* src = KIND == C2R_NLBC ? A.nl : A.onl;
     T* dst = KIND == C2R_NLBC ? A.nlbc : A.onlbc;
     OpPlain<T> op{src, dst, A, ((size_t)c * A.nx + i) * A.mz * A.my2, ((size_t)c * A.nx + i) * A.mz * A.my, (T)0};
-    plane_c2r<0>(buf, A.G, A.tb, op);
+    plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op);
   } else if constexpr (KIND == C2R_VISC) {
     OpVisc<T> op{A, c, i};
-    plane_c2r<0>(buf, A.G, A.tb, op);
+    plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op);
   } else if constexpr (KIND == C2R_OMEGA) {
     OpOmega<T> op{A, c, i};
-    plane_c2r<0>(buf, A.G, A.tb, op);
+    plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op);
   } else {
     OpPgrad<T> op{A, c, i, (T)A.st->s.mpg[c]};
-    plane_c2r<0>(buf, A.G, A.tb, op);
+    plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op);
   }
 }
 
@@ -306,14 +306,14 @@ struct OpForce {
   }
   __device__ void store(int, int ky, int kz, C2<T> v) { A.ff[sidx(A, c, i, ky, kz)] = v; }
 };
-template <typename T>
+template <typename T, int SH>
 __global__ void k_r2c_force(KArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
   C2<T>* buf = (C2<T>*)smem;
   const int i = plane_of(A);
   const int c = blockIdx.y;
   OpForce<T> op{A, c, i, (T)A.st->s.frmean[c]};
-  plane_r2c<0>(buf, A.G, A.tb, op);
+  plane_r2c_sh<SH, 0>(buf, A.G, A.tb, op);
 }
 
 // ===========================================================================
@@ -371,6 +371,7 @@ struct OpNl {
   int c, i, stage;  // stage 0: y-product, 1: z-product, 2: x-flux derivative
   C2<T> acc[NIT];
   NlFlux<T, UPWIND, MINMOD> fx;
+  C2<T>* nyq;  // LDS accumulators of the ky = my/2 column (outputs with it = -1)
   __device__ void load2(int y, int z, T& a, T& b) {
     a = val(y, z);
     b = val(y + 1, z);
@@ -384,31 +385,72 @@ struct OpNl {
     return fx.dx_flux(c, i, y, z);
   }
   __device__ void store(int it, int ky, int kz, C2<T> v) {
-    if (stage == 0) acc[it] = cik(A.wy[ky], v);
-    else if (stage == 1) acc[it] = acc[it] + cik(A.wz[kz], v);
-    else acc[it] = acc[it] + v;
+    C2<T>& a = it < 0 ? nyq[kz] : acc[it];
+    if (stage == 0) a = cik(A.wy[ky], v);
+    else if (stage == 1) a = a + cik(A.wz[kz], v);
+    else a = a + v;
   }
 };
 
-template <typename T, int NIT, bool UPWIND, bool MINMOD>
+// Lanes per form_nl workgroup for a compiled shape: at most 512, so that each
+// lane keeps up to 256 VGPRs for its accumulators.
+template <int SH>
+struct NlShape {
+  static constexpr int NT = Shape<SH>::NT < 512 ? Shape<SH>::NT : 512;
+  static constexpr int NIT = SH == 0 ? 0 : Plane2<Shape<SH>::HY, Shape<SH>::MZ, NT>::r2c_nout;
+};
+
+template <typename T, int NIT, bool UPWIND, bool MINMOD, int SH>
 __global__ void __launch_bounds__(512) k_form_nl(KArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
   C2<T>* buf = (C2<T>*)smem;
   const int i = plane_of(A);
   const int c = blockIdx.y;
-  OpNl<T, NIT, UPWIND, MINMOD> op{A, c, i, 0, {}, {A}};
-  for (int s = 0; s < 3; ++s) {
-    op.stage = s;
-    plane_r2c<NIT>(buf, A.G, A.tb, op);
-  }
-  const int nt = blockDim.x;
+  OpNl<T, NIT, UPWIND, MINMOD> op{A, c, i, 0, {}, {A}, nullptr};
+  if constexpr (SH == 0) {
+    for (int s = 0; s < 3; ++s) {
+      op.stage = s;
+      plane_r2c<NIT>(buf, A.G, A.tb, op);
+    }
+    const int nt = blockDim.x;
 #pragma unroll
-  for (int it = 0; it < NIT; ++it) {
-    int e = threadIdx.x + it * nt;
-    if (e < A.my2 * A.mz) {
-      int kz = e / A.my2, ky = e - kz * A.my2;
-      C2<T> v = op.acc[it];
-      A.nl[sidx(A, c, i, ky, kz)] = mk(-v.x, -v.y);
+    for (int it = 0; it < NIT; ++it) {
+      int e = threadIdx.x + it * nt;
+      if (e < A.my2 * A.mz) {
+        int kz = e / A.my2, ky = e - kz * A.my2;
+        C2<T> v = op.acc[it];
+        A.nl[sidx(A, c, i, ky, kz)] = mk(-v.x, -v.y);
+      }
+    }
+  } else {
+    constexpr int HY = Shape<SH>::HY, MZ = Shape<SH>::MZ, NT = NlShape<SH>::NT;
+    using P = Plane2<HY, MZ, NT>;
+    op.nyq = (C2<T>*)(smem + A.G.tab_off + tab_lds_bytes<T>(HY, MZ));
+    for (int s = 0; s < 3; ++s) {
+      op.stage = s;
+      plane_r2c2<HY, MZ, NT>(buf, A.G, A.tb, op);
+    }
+    // the lane's outputs: items (ky, c) of the last z stage, kz = c + ZA d
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < cdiv(HY * P::ZA, NT); ++j) {
+      const int e = t + j * NT;
+      if (e < HY * P::ZA) {
+        const int ky = e % HY, cc = e / HY;
+#pragma unroll
+        for (int d = 0; d < P::ZB; ++d) {
+          C2<T> v = op.acc[j * P::ZB + d];
+          A.nl[sidx(A, c, i, ky, cc + P::ZA * d)] = mk(-v.x, -v.y);
+        }
+      }
+    }
+    if (t < P::ZA) {
+#pragma unroll
+      for (int d = 0; d < P::ZB; ++d) {
+        const int kz = t + P::ZA * d;
+        C2<T> v = op.nyq[kz];
+        A.nl[sidx(A, c, i, HY, kz)] = mk(-v.x, -v.y);
+      }
     }
   }
 }

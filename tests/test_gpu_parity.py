@@ -227,3 +227,27 @@ def test_blowup_detected_like_the_reference(tmp_path, orc):
     with pytest.raises(solver.PibError) as eg:
         g.step(1)
     assert eo.value.code == _abi.PIB_ERR_BLOWUP == eg.value.code
+
+
+@pytest.mark.parametrize("my,dbydx,nbody", [(32, 8, 6), (64, 16, 3), (128, 16, 8)])
+def test_compiled_plane_shapes_match_generic_and_oracle(tmp_path, orc, my, dbydx, nbody):
+    """The two-stage plane transforms of the compiled shapes (my/2, mz) =
+    (16,32), (32,64), (64,128) against the generic mixed-radix transforms
+    (PIB_TUNE_FFT = 0) and against the oracle, over whole steps."""
+    print(f"compiled shapes my={my}", flush=True)
+    kw = dict(kind="random", my=my, dbydx=dbydx, nbody=nbody, seed=31, re=30.0)
+    st = make_setup(tmp_path, "SH", **kw)
+    o = solver.Solver(st, lib=orc.load(), prefix="orc_")
+    g1 = solver.Solver(st)
+    g0 = solver.Solver(st)
+    assert g0.lib.pib_set_tuning(g0.ctx, _abi.PIB_TUNE_FFT, 0) == 0
+    nsteps = 2 if my == 128 else 4
+    o.step(nsteps)
+    g1.step(nsteps)
+    g0.step(nsteps)
+    uo = o.field("u")
+    e1, e0 = rel_err(g1.field("u"), uo), rel_err(g0.field("u"), uo)
+    f1 = rel_err(g1.forces()["force"], o.forces()["force"])
+    print(f"  u err compiled {e1:.2e} generic {e0:.2e} force {f1:.2e}", flush=True)
+    assert e1 < 1e-10 and e0 < 1e-10 and f1 < 1e-9
+    assert rel_err(g1.field("nl"), g0.field("nl")) < 1e-12
