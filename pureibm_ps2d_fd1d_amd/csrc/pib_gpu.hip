@@ -171,7 +171,7 @@ __global__ void k_gather_records(int n, const int* rec_sorted, const double* w, 
 // per-(component, plane) fluid/solid/total sums and max|u| from c2r_ubcp
 __global__ void __launch_bounds__(SCAL_T) k_scalar_cnt(DevState* st, const double* part, int nx, int rks,
                                                        const double* velbdy) {
-  __shared__ double sh[12 * SCAL_T];
+  __shared__ double sh[12 * 16];
   const int t = threadIdx.x;
   double v[12];
   for (int q = 0; q < 12; ++q) v[q] = 0.0;
@@ -187,9 +187,9 @@ __global__ void __launch_bounds__(SCAL_T) k_scalar_cnt(DevState* st, const doubl
   const int nb = st->nbody;
   for (int m = t; m < nb; m += SCAL_T)
     for (int n = 0; n < 3; ++n) v[9 + n] += velbdy[(size_t)n * nb + m];
-  block_sums<12>(v, sh);
-  double mxc[3];
-  for (int c = 0; c < 3; ++c) mxc[c] = block_max_any(mx[c], sh);
+  block_sums_w<12>(v, sh);
+  double mxc[3] = {mx[0], mx[1], mx[2]};
+  block_maxes_w<3>(mxc, sh);
   if (t != 0) return;
   pib_scalars& s = st->s;
   double umax_tmp = fmax(fmax(mxc[0], mxc[1]), mxc[2]);
@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(SCAL_T) k_scalar_cnt(DevState* st, const doubl
 __global__ void __launch_bounds__(SCAL_T) k_scalar_drag(DevState* st, const double* dpart, int G, int rks,
                                                         const double* rad, double* pres, double* visc,
                                                         double* torq, double* force, double* force_chem) {
-  __shared__ double sh[12 * SCAL_T];
+  __shared__ double sh[12 * 16];
   __shared__ double mpg_s[3];
   const int t = threadIdx.x;
   const int nb = st->nbody;
@@ -247,7 +247,9 @@ __global__ void __launch_bounds__(SCAL_T) k_scalar_drag(DevState* st, const doub
   for (int m = t; m < nb; m += SCAL_T) {
     double tot[9];
     for (int q = 0; q < 9; ++q) tot[q] = 0.0;
+#pragma unroll 3
     for (int g = 0; g < G; ++g)
+#pragma unroll
       for (int q = 0; q < 9; ++q) tot[q] += dpart[((size_t)m * G + g) * 9 + q];
     for (int d = 0; d < 3; ++d) {
       size_t k = (size_t)d * nb + m;
@@ -264,7 +266,7 @@ __global__ void __launch_bounds__(SCAL_T) k_scalar_drag(DevState* st, const doub
       v[9 + d] += vn;          // sum of visc(m)
     }
   }
-  block_sums<12>(v, sh);
+  block_sums_w<12>(v, sh);
   pib_scalars& s = st->s;
   const double PI = 3.14159265358979323846;
   if (t == 0) {
@@ -305,11 +307,11 @@ __global__ void __launch_bounds__(SCAL_T) k_scalar_drag(DevState* st, const doub
 // frmean / frmeanfluid over all reversal pairs (boundary_condition.F90:632-633)
 // from the (body, chunk) partials of k_forcing_points
 __global__ void __launch_bounds__(SCAL_T) k_reduce_frmean(DevState* st, const double* fpart, int nrec) {
-  __shared__ double sh[6 * SCAL_T];
+  __shared__ double sh[6 * 16];
   double a[6] = {0, 0, 0, 0, 0, 0};
   for (int p = threadIdx.x; p < nrec; p += SCAL_T)
     for (int q = 0; q < 6; ++q) a[q] += fpart[(size_t)p * 6 + q];
-  block_sums<6>(a, sh);
+  block_sums_w<6>(a, sh);
   if (threadIdx.x == 0) {
     double ntot = (double)((long long)st->nx * st->my * st->mz);
     for (int n = 0; n < 3; ++n) {
@@ -322,7 +324,7 @@ __global__ void __launch_bounds__(SCAL_T) k_reduce_frmean(DevState* st, const do
 // bcset diagnostics (boundary_condition.F90:723-884), one phase
 __global__ void __launch_bounds__(SCAL_T) k_scalar_diag(DevState* st, const double* force,
                                                         const double* force_chem) {
-  __shared__ double sh[6 * SCAL_T];
+  __shared__ double sh[6 * 16];
   const int nb = st->nbody;
   double v[6] = {0, 0, 0, 0, 0, 0};
   for (int m = threadIdx.x; m < nb; m += SCAL_T)
@@ -330,7 +332,7 @@ __global__ void __launch_bounds__(SCAL_T) k_scalar_diag(DevState* st, const doub
       v[n] += force[(size_t)n * nb + m];
       v[3 + n] += force_chem[(size_t)n * nb + m];
     }
-  block_sums<6>(v, sh);
+  block_sums_w<6>(v, sh);
   if (threadIdx.x != 0) return;
   pib_scalars& s = st->s;
   const double PI = 3.14159265358979323846;
@@ -358,12 +360,12 @@ __global__ void __launch_bounds__(SCAL_T) k_scalar_diag(DevState* st, const doub
 
 // velstep tail: check_divergence max, umean update (steptheflow.F90:93-99)
 __global__ void __launch_bounds__(SCAL_T) k_scalar_vel(DevState* st, const double* part, int nblk, int* status) {
-  __shared__ double sh[SCAL_T];
-  double mx = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += SCAL_T) mx = fmax(mx, part[b]);
-  mx = block_max_any(mx, sh);
+  __shared__ double sh[16];
+  double mx[1] = {0.0};
+  for (int b = threadIdx.x; b < nblk; b += SCAL_T) mx[0] = fmax(mx[0], part[b]);
+  block_maxes_w<1>(mx, sh);
   if (threadIdx.x != 0) return;
-  st->s.divmax = mx;
+  st->s.divmax = mx[0];
   for (int n = 0; n < 3; ++n)
     st->s.umean[n] = st->s.umean[n] + (-st->s.mpg[n] + st->s.frmean[n] - st->s.frame_accln[n]) * st->s.dt;
   if (*status && st->status == 0) st->status = *status;
@@ -810,6 +812,24 @@ struct Impl : Base {
     HIPCHK(hipGetLastError());
     ke();
   }
+  template <int KA, int NA, int KB, int NB, int SH>
+  void launch_c2r_pair(int rks, const char* name) {
+    auto kern = k_c2r_pair<T, KA, NA, KB, SH>;
+    set_lds(kern, lds_plane);
+    kb(name);
+    kern<<<dim3(nx, NA + NB), plane_nt(SH), lds_plane, stream>>>(args(rks));
+    HIPCHK(hipGetLastError());
+    ke();
+  }
+  template <int KA, int NA, int KB, int NB>
+  void c2r_pair(int rks, const char* name) {
+    switch (cur_sh()) {
+      case 1: launch_c2r_pair<KA, NA, KB, NB, 1>(rks, name); break;
+      case 2: launch_c2r_pair<KA, NA, KB, NB, 2>(rks, name); break;
+      case 3: launch_c2r_pair<KA, NA, KB, NB, 3>(rks, name); break;
+      default: launch_c2r_pair<KA, NA, KB, NB, 0>(rks, name);
+    }
+  }
   template <int KIND>
   void c2r(int rks, int ncomp) {
     switch (cur_sh()) {
@@ -1008,8 +1028,7 @@ struct Impl : Base {
 
   // ---------------- per-step routines ----------------
   void compute_new_timestep(int rks) {
-    c2r<C2R_UBCP>(rks, 3);
-    c2r<C2R_PBCP>(rks, 1);
+    c2r_pair<C2R_UBCP, 3, C2R_PBCP, 1>(rks, "c2r_ubcp_pbcp");
     kb("scalar_cnt");
     k_scalar_cnt<<<1, SCAL_T, 0, stream>>>(dst, part, nx, rks, vel ? vel : part);
     HIPCHK(hipGetLastError());
@@ -1072,8 +1091,7 @@ struct Impl : Base {
   void bcset(int rks) {
     Particles P = parts();
     size_t npts = (size_t)nbody * nbnd;
-    c2r<C2R_VISC>(rks, 3);
-    c2r<C2R_OMEGA>(rks, 3);
+    c2r_pair<C2R_VISC, 3, C2R_OMEGA, 3>(rks, "c2r_visc_omega");
     const int G = (nbnd + 255) / 256;
     kb("drag_points");
     k_drag_points<T><<<dim3(G, nbody), 256, 0, stream>>>(args(rks), P, drag_part);

@@ -123,6 +123,52 @@ __device__ inline void block_sums(double* v, double* sh) {
   for (int q = 0; q < K; ++q) v[q] = sh[q * bd];
   __syncthreads();
 }
+// Block sums for blocks whose size is a multiple of 64 (at most 1024):
+// xor-butterfly within each wavefront (every lane ends with the same,
+// order-independent-of-lane result), then the wave totals in wave order.
+// sh holds K * 16 doubles.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+template <int K>
+__device__ inline void block_sums_w(double* v, double* sh) {
+  const int t = threadIdx.x, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int q = 0; q < K; ++q) v[q] = wave_sum(v[q]);
+  if ((t & 63) == 0)
+    for (int q = 0; q < K; ++q) sh[q * 16 + (t >> 6)] = v[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    double a = 0.0;
+    for (int w = 0; w < nw; ++w) a += sh[q * 16 + w];
+    v[q] = a;
+  }
+  __syncthreads();
+}
+template <int K>
+__device__ inline void block_maxes_w(double* v, double* sh) {
+  const int t = threadIdx.x, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int q = 0; q < K; ++q) v[q] = wave_max(v[q]);
+  if ((t & 63) == 0)
+    for (int q = 0; q < K; ++q) sh[q * 16 + (t >> 6)] = v[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    double a = sh[q * 16];
+    for (int w = 1; w < nw; ++w) a = fmax(a, sh[q * 16 + w]);
+    v[q] = a;
+  }
+  __syncthreads();
+}
 // deterministic max over any block size
 __device__ inline double block_max_any(double v, double* sh) {
   const int bd = blockDim.x, t = threadIdx.x;
@@ -254,11 +300,8 @@ struct OpPgrad {
 enum { C2R_UBCP = 0, C2R_PBCP, C2R_NLBC, C2R_ONLBC, C2R_VISC, C2R_OMEGA, C2R_PGRAD };
 
 template <typename T, int KIND, int SH>
-__global__ void k_c2r(KArgs<T> A) {
-  extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void c2r_plane(const KArgs<T>& A, unsigned char* smem, int c, int i) {
   C2<T>* buf = (C2<T>*)smem;
-  const int i = plane_of(A);
-  const int c = blockIdx.y;
   if constexpr (KIND == C2R_UBCP) {
     OpUbcp<T> op(A, c, i);
     plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op);
@@ -289,6 +332,21 @@ __global__ void k_c2r(KArgs<T> A) {
     OpPgrad<T> op{A, c, i, (T)A.st->s.mpg[c]};
     plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op);
   }
+}
+// grid (planes, components of KIND)
+template <typename T, int KIND, int SH>
+__global__ void k_c2r(KArgs<T> A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  c2r_plane<T, KIND, SH>(A, smem, blockIdx.y, plane_of(A));
+}
+// two independent transforms in one launch: grid (planes, NA + NB), the
+// first NA rows of workgroups do KA, the rest KB
+template <typename T, int KA, int NA, int KB, int SH>
+__global__ void k_c2r_pair(KArgs<T> A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int y = blockIdx.y;
+  if (y < NA) c2r_plane<T, KA, SH>(A, smem, y, plane_of(A));
+  else c2r_plane<T, KB, SH>(A, smem, y - NA, plane_of(A));
 }
 
 // ===========================================================================
@@ -528,7 +586,7 @@ struct Particles {
 // 6..8 torque.
 template <typename T>
 __global__ void __launch_bounds__(256) k_drag_points(KArgs<T> A, Particles P, double* dpart) {
-  __shared__ double sh[9 * 256];
+  __shared__ double sh[9 * 16];
   const int m = blockIdx.y, G = gridDim.x;
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
   double o[9];
@@ -567,7 +625,7 @@ __global__ void __launch_bounds__(256) k_drag_points(KArgs<T> A, Particles P, do
     o[7] = (b3 * tf[0] - b1 * tf[2]) * rb * st->dx;
     o[8] = (b1 * tf[1] - b2 * tf[0]) * rb * st->dx;
   }
-  block_sums<9>(o, sh);
+  block_sums_w<9>(o, sh);
   if (threadIdx.x == 0)
     for (int q = 0; q < 9; ++q) dpart[((size_t)m * G + blockIdx.x) * 9 + q] = o[q];
 }
@@ -581,7 +639,7 @@ template <typename T>
 __device__ void forcing_point(const KArgs<T>& A, const Particles& P, int m, int l, double* o, double* fsum);
 template <typename T>
 __global__ void __launch_bounds__(256) k_forcing_points(KArgs<T> A, Particles P, double* out, double* fpart) {
-  __shared__ double sh[6 * 256];
+  __shared__ double sh[6 * 16];
   const int m = blockIdx.y, G = gridDim.x;
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
   double fs[6] = {0, 0, 0, 0, 0, 0};
@@ -593,7 +651,7 @@ __global__ void __launch_bounds__(256) k_forcing_points(KArgs<T> A, Particles P,
       forcing_point(A, P, m, l, o, fs);
     }
   }
-  block_sums<6>(fs, sh);
+  block_sums_w<6>(fs, sh);
   if (threadIdx.x == 0)
     for (int q = 0; q < 6; ++q) fpart[((size_t)m * G + blockIdx.x) * 6 + q] = fs[q];
 }
