@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 # Algorithmic field passes per launch (reads + writes of N-word fields,
 # spectral and physical fields both counted as N words).
 PASSES = {
-    "c2r_ubcp_pbcp": 8, "form_nl": 6, "c2r_nlbc": 6, "c2r_onlbc": 6,
+    "c2r_ubcp_pbcp": 8, "form_nl": 6, "form_nl_yz": 6, "form_nl_x": 9, "c2r_nlbc": 6, "c2r_onlbc": 6,
     "c2r_visc_omega": 14, "c2r_pgrad": 4, "r2c_force": 6, "velstep": 17,
 }
 

@@ -800,6 +800,7 @@ struct Impl : Base {
   // one and fft_compiled is set, else the generic mixed-radix transform
   int sh = 0;
   bool fft_compiled = true;
+  int nl_form = 2;  // form_nl of a compiled shape: 0 accumulators, 1 two launches, 2 in place
   int cur_sh() const { return fft_compiled ? sh : 0; }
   int plane_nt(int s) const { return s ? shape_nt(s) : nt_plane; }
   template <int KIND, int SH>
@@ -1063,15 +1064,30 @@ struct Impl : Base {
   }
   template <int SH>
   void launch_nl_sh(bool up, bool mm, int rks) {
+    if (nl_form == 2) {
+      constexpr int NT = Shape<SH>::NT;
+      if (!up) run_nl(k_form_nl<T, 1, false, false, SH, 3>, rks, NT);
+      else if (!mm) run_nl(k_form_nl<T, 1, true, false, SH, 3>, rks, NT);
+      else run_nl(k_form_nl<T, 1, true, true, SH, 3>, rks, NT);
+      return;
+    }
+    if (nl_form == 1) {
+      constexpr int NIT = NlShape<SH>::NIT2, NT = Shape<SH>::NT;
+      run_nl(k_form_nl<T, NIT, false, false, SH, 1>, rks, NT, "form_nl_yz");
+      if (!up) run_nl(k_form_nl<T, 1, false, false, SH, 2>, rks, NT, "form_nl_x");
+      else if (!mm) run_nl(k_form_nl<T, 1, true, false, SH, 2>, rks, NT, "form_nl_x");
+      else run_nl(k_form_nl<T, 1, true, true, SH, 2>, rks, NT, "form_nl_x");
+      return;
+    }
     constexpr int NIT = NlShape<SH>::NIT, NT = NlShape<SH>::NT;
     if (!up) run_nl(k_form_nl<T, NIT, false, false, SH>, rks, NT);
     else if (!mm) run_nl(k_form_nl<T, NIT, true, false, SH>, rks, NT);
     else run_nl(k_form_nl<T, NIT, true, true, SH>, rks, NT);
   }
   template <typename K>
-  void run_nl(K kern, int rks, int nt) {
+  void run_nl(K kern, int rks, int nt, const char* name = "form_nl") {
     set_lds(kern, lds_plane);
-    kb("form_nl");
+    kb(name);
     kern<<<dim3(nx, 3), nt, lds_plane, stream>>>(args(rks));
     HIPCHK(hipGetLastError());
     ke();
@@ -1671,6 +1687,14 @@ int pib_set_tuning(void* ctx, int key, int value) {
       HIPCHK(hipStreamSynchronize(I.stream));
       I.drop_graphs();
       I.fft_compiled = value == 1;
+    });
+  }
+  if (key == PIB_TUNE_NL_FORM) {
+    if (value < 0 || value > 2) return PIB_ERR_ARG;
+    return dispatch(ctx, [&](auto& I) {
+      HIPCHK(hipStreamSynchronize(I.stream));
+      I.drop_graphs();
+      I.nl_form = value;
     });
   }
   if (key == PIB_TUNE_GRAPH) {

@@ -435,19 +435,67 @@ struct OpNl {
     b = val(y + 1, z);
   }
   __device__ T val(int y, int z) const {
+    if (stage == 2) return fx.dx_flux(c, i, y, z);
+    // stage 0: u_c * u_2, stage 1: u_c * u_3 (products as the reference forms
+    // them: uv, vv, vw / uw, vw, ww)
     size_t o = ridx(A, 0, i, y, z);
     size_t cs = (size_t)A.nx * A.mz * A.my;
-    T u1 = A.ubcp[o], u2 = A.ubcp[o + cs], u3 = A.ubcp[o + 2 * cs];
-    if (stage == 0) return c == 0 ? u1 * u2 : (c == 1 ? u2 * u2 : u2 * u3);
-    if (stage == 1) return c == 0 ? u1 * u3 : (c == 1 ? u2 * u3 : u3 * u3);
-    return fx.dx_flux(c, i, y, z);
+    T uc = A.ubcp[o + (size_t)c * cs], us = A.ubcp[o + (size_t)(stage + 1) * cs];
+    return uc * us;
   }
+  bool rmw = false;     // stage 2 adds to the partial sum kept in nl and writes -(sum)
+  bool direct = false;  // every stage updates nl in place (no accumulators)
   __device__ void store(int it, int ky, int kz, C2<T> v) {
+    if (rmw || direct) {
+      C2<T>* p = A.nl + sidx(A, c, i, ky, kz);
+      if (stage == 0) {
+        *p = cik(A.wy[ky], v);
+      } else if (stage == 1) {
+        *p = *p + cik(A.wz[kz], v);
+      } else {
+        C2<T> a = *p + v;
+        *p = mk(-a.x, -a.y);
+      }
+      return;
+    }
     C2<T>& a = it < 0 ? nyq[kz] : acc[it];
     if (stage == 0) a = cik(A.wy[ky], v);
     else if (stage == 1) a = a + cik(A.wz[kz], v);
     else a = a + v;
   }
+  // the same with the stage fixed at compile time (in-place form)
+  template <int ST>
+  __device__ T val_at(int y, int z) const {
+    if constexpr (ST == 2) {
+      return fx.dx_flux(c, i, y, z);
+    } else {
+      size_t o = ridx(A, 0, i, y, z);
+      size_t cs = (size_t)A.nx * A.mz * A.my;
+      return A.ubcp[o + (size_t)c * cs] * A.ubcp[o + (size_t)(ST + 1) * cs];
+    }
+  }
+  template <int ST>
+  __device__ void store_at(int ky, int kz, C2<T> v) {
+    C2<T>* p = A.nl + sidx(A, c, i, ky, kz);
+    if constexpr (ST == 0) {
+      *p = cik(A.wy[ky], v);
+    } else if constexpr (ST == 1) {
+      *p = *p + cik(A.wz[kz], v);
+    } else {
+      C2<T> a = *p + v;
+      *p = mk(-a.x, -a.y);
+    }
+  }
+};
+// one stage of the in-place form as an Op of its own
+template <int ST, class O, typename T>
+struct OpNlStage {
+  O& o;
+  __device__ void load2(int y, int z, T& a, T& b) {
+    a = o.template val_at<ST>(y, z);
+    b = o.template val_at<ST>(y + 1, z);
+  }
+  __device__ void store(int, int ky, int kz, C2<T> v) { o.template store_at<ST>(ky, kz, v); }
 };
 
 // Lanes per form_nl workgroup for a compiled shape: at most 512, so that each
@@ -456,10 +504,18 @@ template <int SH>
 struct NlShape {
   static constexpr int NT = Shape<SH>::NT < 512 ? Shape<SH>::NT : 512;
   static constexpr int NIT = SH == 0 ? 0 : Plane2<Shape<SH>::HY, Shape<SH>::MZ, NT>::r2c_nout;
+  // the split form runs both parts at the shape's own workgroup size
+  static constexpr int NIT2 = SH == 0 ? 0 : Plane2<Shape<SH>::HY, Shape<SH>::MZ, Shape<SH>::NT>::r2c_nout;
 };
 
-template <typename T, int NIT, bool UPWIND, bool MINMOD, int SH>
-__global__ void __launch_bounds__(512) k_form_nl(KArgs<T> A) {
+// PART 0: the three transforms in one workgroup with register accumulators;
+// for a compiled shape, PART 1 writes the partial sum wy F(u_c v) + wz F(u_c w)
+// to nl and PART 2 adds F(D_c) to it and stores the negated total; PART 3
+// runs the three transforms in one workgroup, each updating nl in place
+// (the lane that writes an output element is the lane that updates it).
+// All forms add in the order ((wy F0 + wz F1) + F2).
+template <typename T, int NIT, bool UPWIND, bool MINMOD, int SH, int PART = 0>
+__global__ void __launch_bounds__(PART == 0 ? 512 : 1024) k_form_nl(KArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
   C2<T>* buf = (C2<T>*)smem;
   const int i = plane_of(A);
@@ -481,13 +537,31 @@ __global__ void __launch_bounds__(512) k_form_nl(KArgs<T> A) {
       }
     }
   } else {
-    constexpr int HY = Shape<SH>::HY, MZ = Shape<SH>::MZ, NT = NlShape<SH>::NT;
+    constexpr int HY = Shape<SH>::HY, MZ = Shape<SH>::MZ;
+    constexpr int NT = PART == 0 ? NlShape<SH>::NT : Shape<SH>::NT;
     using P = Plane2<HY, MZ, NT>;
     op.nyq = (C2<T>*)(smem + A.G.tab_off + tab_lds_bytes<T>(HY, MZ));
-    for (int s = 0; s < 3; ++s) {
+    if constexpr (PART == 2) {
+      op.stage = 2;
+      op.rmw = true;
+      plane_r2c2<HY, MZ, NT>(buf, A.G, A.tb, op);
+      return;
+    }
+    if constexpr (PART == 3) {
+      using O = OpNl<T, NIT, UPWIND, MINMOD>;
+      OpNlStage<0, O, T> o0{op};
+      plane_r2c2<HY, MZ, NT>(buf, A.G, A.tb, o0);
+      OpNlStage<1, O, T> o1{op};
+      plane_r2c2<HY, MZ, NT>(buf, A.G, A.tb, o1);
+      OpNlStage<2, O, T> o2{op};
+      plane_r2c2<HY, MZ, NT>(buf, A.G, A.tb, o2);
+      return;
+    }
+    for (int s = 0; s < (PART == 1 ? 2 : 3); ++s) {
       op.stage = s;
       plane_r2c2<HY, MZ, NT>(buf, A.G, A.tb, op);
     }
+    const T sg = PART == 1 ? (T)1 : (T)-1;
     // the lane's outputs: items (ky, c) of the last z stage, kz = c + ZA d
     const int t = threadIdx.x;
 #pragma unroll
@@ -498,7 +572,7 @@ __global__ void __launch_bounds__(512) k_form_nl(KArgs<T> A) {
 #pragma unroll
         for (int d = 0; d < P::ZB; ++d) {
           C2<T> v = op.acc[j * P::ZB + d];
-          A.nl[sidx(A, c, i, ky, cc + P::ZA * d)] = mk(-v.x, -v.y);
+          A.nl[sidx(A, c, i, ky, cc + P::ZA * d)] = mk(sg * v.x, sg * v.y);
         }
       }
     }
@@ -507,7 +581,7 @@ __global__ void __launch_bounds__(512) k_form_nl(KArgs<T> A) {
       for (int d = 0; d < P::ZB; ++d) {
         const int kz = t + P::ZA * d;
         C2<T> v = op.nyq[kz];
-        A.nl[sidx(A, c, i, HY, kz)] = mk(-v.x, -v.y);
+        A.nl[sidx(A, c, i, HY, kz)] = mk(sg * v.x, sg * v.y);
       }
     }
   }
