@@ -117,7 +117,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["both", "fp32", "fp64"], default="both")
     ap.add_argument("--kernel-steps", type=int, default=20, help="steps timed per launch for the roofline")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (scale for FETCH_SIZE/WRITE_SIZE)")
     a = ap.parse_args()
+    if a.calib_copy:
+        import torch
+        n = a.calib_copy * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 4, "copies": 3}))
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the x-slab sharding across GPUs is "
                                    "described in DESIGN.md and not launched here", "n_gpus": a.gpus}))

@@ -1110,7 +1110,8 @@ struct Impl : Base {
     c2r_pair<C2R_VISC, 3, C2R_OMEGA, 3>(rks, "c2r_visc_omega");
     const int G = (nbnd + 255) / 256;
     kb("drag_points");
-    k_drag_points<T><<<dim3(G, nbody), 256, 0, stream>>>(args(rks), P, drag_part);
+    const unsigned nbg = (unsigned)((nbody * G + 7) / 8 * 8);
+    k_drag_points<T><<<nbg, 256, 0, stream>>>(args(rks), P, drag_part, G);
     HIPCHK(hipGetLastError());
     ke();
     kb("scalar_drag");
@@ -1120,7 +1121,7 @@ struct Impl : Base {
     c2r<C2R_PGRAD>(rks, 3);
     if (cfg.dorpr) {
       kb("forcing_points");
-      k_forcing_points<T><<<dim3(G, nbody), 256, 0, stream>>>(args(rks), P, frc_pts, frc_part);
+      k_forcing_points<T><<<nbg, 256, 0, stream>>>(args(rks), P, frc_pts, frc_part, G);
       HIPCHK(hipGetLastError());
       ke();
       kb("reduce_frmean");
@@ -1129,7 +1130,8 @@ struct Impl : Base {
       ke();
       if (ntarget > 0) {
         kb("spread");
-        k_spread<T><<<nspread_blk, 256, 0, stream>>>(args(rks), spread_bt, tnode, toff, rpt, rw, rdist,
+        k_spread<T><<<(nspread_blk + 7) / 8 * 8, 256, 0, stream>>>(args(rks), spread_bt, nspread_blk, tnode, toff,
+                                                                  rpt, rw, rdist,
                                                                frc_pts);
         HIPCHK(hipGetLastError());
         ke();

@@ -80,6 +80,15 @@ __device__ __forceinline__ int plane_of(const KArgs<T>& A) {
   return b;
 }
 
+// Work index of a workgroup in a 1-D grid of nblk workgroups (nblk a multiple
+// of 8).  The hardware deals consecutive workgroups round-robin to the 8
+// XCDs; this mapping gives each XCD a contiguous run of work indices, so the
+// workgroups that gather around one sphere share one L2.
+__device__ __forceinline__ int xcd_linear(int b, int nblk) {
+  const int per = nblk >> 3;
+  return (b & 7) * per + (b >> 3);
+}
+
 // ---------------------------------------------------------------------------
 // deterministic block reductions (power-of-two block sizes)
 // ---------------------------------------------------------------------------
@@ -653,16 +662,19 @@ struct Particles {
   int nbody, nbnd;
 };
 
-// calc_pres_visc_drag (boundary_condition.F90:1481-1645).  Grid (G, nbody):
-// workgroup (g, m) takes surface points g*blockDim.x + t of body m and writes
+// calc_pres_visc_drag (boundary_condition.F90:1481-1645).  Work item (m, g)
+// (xcd_linear over nbody*G workgroups) takes surface points g*blockDim.x + t
+// of body m and writes
 // the workgroup sums of their contributions, dpart[(m*G + g)*9 + q]:
 // q = 0..2 pressure part (subtracted from pres(m)), 3..5 viscous part,
 // 6..8 torque.
 template <typename T>
-__global__ void __launch_bounds__(256) k_drag_points(KArgs<T> A, Particles P, double* dpart) {
+__global__ void __launch_bounds__(256) k_drag_points(KArgs<T> A, Particles P, double* dpart, int G) {
   __shared__ double sh[9 * 16];
-  const int m = blockIdx.y, G = gridDim.x;
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = xcd_linear(blockIdx.x, gridDim.x);
+  const int m = w / G, g = w - m * G;
+  if (m >= P.nbody) return;  // padding of the grid to a multiple of 8
+  const int l = g * blockDim.x + threadIdx.x;
   double o[9];
   for (int q = 0; q < 9; ++q) o[q] = 0.0;
   if (l < P.nbnd) {
@@ -701,21 +713,25 @@ __global__ void __launch_bounds__(256) k_drag_points(KArgs<T> A, Particles P, do
   }
   block_sums_w<9>(o, sh);
   if (threadIdx.x == 0)
-    for (int q = 0; q < 9; ++q) dpart[((size_t)m * G + blockIdx.x) * 9 + q] = o[q];
+    for (int q = 0; q < 9; ++q) dpart[((size_t)m * G + g) * 9 + q] = o[q];
 }
 
 // calc_inner_reversal_forcing (boundary_condition.F90:1104-1479), serial path:
 // per active reversal pair, force_tmp and the solid-node redistribution share.
-// Grid (G, nbody): workgroup (g, m) takes points l = g*blockDim.x + t of body
-// m; out[(m*nbnd + l)*6 + 0..2] force_tmp, [3..5] force_dist; the workgroup
+// Work item (m, g) (xcd_linear over nbody*G workgroups) takes points
+// l = g*blockDim.x + t of body m; out[(m*nbnd + l)*6 + 0..2] force_tmp,
+// [3..5] force_dist; the workgroup
 // sums of the frmean / frmeanfluid parts go to fpart[(m*G + g)*6 + 0..5].
 template <typename T>
 __device__ void forcing_point(const KArgs<T>& A, const Particles& P, int m, int l, double* o, double* fsum);
 template <typename T>
-__global__ void __launch_bounds__(256) k_forcing_points(KArgs<T> A, Particles P, double* out, double* fpart) {
+__global__ void __launch_bounds__(256) k_forcing_points(KArgs<T> A, Particles P, double* out, double* fpart,
+                                                        int G) {
   __shared__ double sh[6 * 16];
-  const int m = blockIdx.y, G = gridDim.x;
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = xcd_linear(blockIdx.x, gridDim.x);
+  const int m = w / G, g = w - m * G;
+  if (m >= P.nbody) return;  // padding of the grid to a multiple of 8
+  const int l = g * blockDim.x + threadIdx.x;
   double fs[6] = {0, 0, 0, 0, 0, 0};
   if (l < P.nbnd) {
     double* o = out + ((size_t)m * P.nbnd + l) * 6;
@@ -727,7 +743,7 @@ __global__ void __launch_bounds__(256) k_forcing_points(KArgs<T> A, Particles P,
   }
   block_sums_w<6>(fs, sh);
   if (threadIdx.x == 0)
-    for (int q = 0; q < 6; ++q) fpart[((size_t)m * G + blockIdx.x) * 6 + q] = fs[q];
+    for (int q = 0; q < 6; ++q) fpart[((size_t)m * G + g) * 6 + q] = fs[q];
 }
 template <typename T>
 __device__ void forcing_point(const KArgs<T>& A, const Particles& P, int m, int l, double* o, double* fsum) {
@@ -824,12 +840,14 @@ __device__ void forcing_point(const KArgs<T>& A, const Particles& P, int m, int 
 // acc = (acc + share) + dist, as the reference accumulates fr.
 #define SPREAD_CAP 1024
 template <typename T>
-__global__ void __launch_bounds__(256) k_spread(KArgs<T> A, const int* bt, const int* tnode, const int* toff,
-                                                const int* rpt, const double* rw, const uint8_t* rdist,
-                                                const double* pts) {
+__global__ void __launch_bounds__(256) k_spread(KArgs<T> A, const int* bt, int nrange, const int* tnode,
+                                                const int* toff, const int* rpt, const double* rw,
+                                                const uint8_t* rdist, const double* pts) {
   __shared__ double cw[SPREAD_CAP * 3];
   __shared__ double cdist[SPREAD_CAP * 3];
-  const int t0 = bt[blockIdx.x], t1 = bt[blockIdx.x + 1];
+  const int w = xcd_linear(blockIdx.x, gridDim.x);  // neighbouring ranges share an L2
+  if (w >= nrange) return;
+  const int t0 = bt[w], t1 = bt[w + 1];
   const int r0 = toff[t0], r1 = toff[t1];
   const double dx3 = A.st->dx3;
   constexpr int CH = SPREAD_CAP / 256;

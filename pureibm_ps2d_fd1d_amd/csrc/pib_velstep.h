@@ -327,16 +327,18 @@ __device__ void vel_tile(const KArgs<T>& A, int q0, bool mean, unsigned char* ra
   // ---- h_c = c3 nl_c + ff_c + c4 onl_c (rows i0..i0+L-1) ----
   C2<R> h[3][L];
 #pragma unroll
-  for (int c = 0; c < 3; ++c)
+  for (int c = 0; c < 3; ++c) {
 #pragma unroll
     for (int k = 0; k < L; ++k)
       h[c][k] = (cscal(c3, V.ld(A.nl, c, i0 + k)) + V.ld(A.ff, c, i0 + k)) + cscal(c4, V.ld(A.onl, c, i0 + k));
-  // ---- pressure (poisson.F90:55-239) ----
-  Toep<R, L> tp;
-  {
-    const R ap = idx2 - (R)0.25 * w2, bp = (R)-2 * idx2 - (R)0.5 * w2;
-    toep_init(tp, ap, bp, V.st);
+    // one component's 3L loads in flight at a time: keeps the load phase
+    // within the register budget of three workgroups per CU
+    __asm__ __volatile__("" ::: "memory");
   }
+  // ---- pressure (poisson.F90:55-239) ----
+  // the pressure/divcorr matrix is set up again for divcorr rather than kept
+  // live through ustep (registers bound the workgroups per CU)
+  const R ap = idx2 - (R)0.25 * w2, bp = (R)-2 * idx2 - (R)0.5 * w2;
   C2<R> hn[3];
   V.template next_first<3>(h, hn);
   C2<R> p[1][L];
@@ -351,18 +353,16 @@ __device__ void vel_tile(const KArgs<T>& A, int q0, bool mean, unsigned char* ra
       p[0][k] = cscal(ic, t);
     }
   }
-  V.template stage<1>(tp);
-  V.template solve<1>(tp, p, mean);
+  {
+    Toep<R, L> tp;
+    toep_init(tp, ap, bp, V.st);
+    V.template stage<1>(tp);
+    V.template solve<1>(tp, p, mean);
+  }
   if (mean)  // the (0,0) mode keeps only the real part (poisson.F90:139)
 #pragma unroll
     for (int k = 0; k < L; ++k) p[0][k].y = (R)0;
   // ---- ustep (usteptime.F90:45-258) ----
-  Toep<R, L> th;
-  {
-    const R ah = -dt * vis * c2 * idx2;
-    const R bh = (R)1 + dt * vis * c2 * ((R)2 * idx2 + w2);
-    toep_init(th, ah, bh, V.st);
-  }
   C2<R> pl;
   {
     __syncthreads();
@@ -390,8 +390,14 @@ __device__ void vel_tile(const KArgs<T>& A, int q0, bool mean, unsigned char* ra
       }
     }
   }
-  V.template stage<3>(th);
-  V.template solve<3>(th, h, false);
+  {
+    Toep<R, L> th;
+    const R ah = -dt * vis * c2 * idx2;
+    const R bh = (R)1 + dt * vis * c2 * ((R)2 * idx2 + w2);
+    toep_init(th, ah, bh, V.st);
+    V.template stage<3>(th);
+    V.template solve<3>(th, h, false);
+  }
   // ---- divcorr (poisson.F90:241-508) ----
   const R alp = c12 * dt;
   C2<R> phi[1][L];
@@ -407,8 +413,12 @@ __device__ void vel_tile(const KArgs<T>& A, int q0, bool mean, unsigned char* ra
       phi[0][k] = cscal(ia, t);
     }
   }
-  V.template stage<1>(tp);
-  V.template solve<1>(tp, phi, mean);
+  {
+    Toep<R, L> tp;
+    toep_init(tp, ap, bp, V.st);
+    V.template stage<1>(tp);
+    V.template solve<1>(tp, phi, mean);
+  }
   if (mean)  // poisson.F90:320
 #pragma unroll
     for (int k = 0; k < L; ++k) phi[0][k].y = (R)0;
