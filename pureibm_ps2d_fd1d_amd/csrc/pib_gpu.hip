@@ -142,12 +142,12 @@ __global__ void k_spread_records(Particles P, const uint8_t* mask, int nx, int m
       for (int i = 0; i < 2; ++i) {
         int cidx = (k * 2 + j) * 2 + i;
         size_t r = (size_t)idx * 8 + cidx;
-        unsigned int node = (unsigned int)(((size_t)s.ii[i] * mz + s.kk[k]) * my + s.jj[j]);
+        unsigned int node = s.node(i, j, k);
         bool fl = mask[node] != 0;
         bool dep = act && (frmeanfluid ? true : !fl);
         key[r] = dep ? node : sentinel;
         rec[r] = (int)r;
-        w[r] = s.w[cidx];
+        w[r] = s.w(i, j, k);
         dist[r] = (uint8_t)(frmeanfluid ? 0 : 1);
       }
 }
@@ -684,15 +684,17 @@ struct Impl : Base {
     onl = alloc<C2<T>>(nsc);
     ff = alloc<C2<T>>(nsc);
     p = alloc<C2<T>>((size_t)nx * E);
-    ubcp = alloc<T>(nsr);
-    nlbc = alloc<T>(nsr);
-    onlbc = alloc<T>(nsr);
-    diffn = alloc<T>(nsr);
-    omega = alloc<T>(nsr);
+    // fields the IBM stencils gather from carry GATHER_PAD elements past
+    // their end (read, never used, by a y-pair load at the last node)
+    ubcp = alloc<T>(nsr + GATHER_PAD);
+    nlbc = alloc<T>(nsr + GATHER_PAD);
+    onlbc = alloc<T>(nsr + GATHER_PAD);
+    diffn = alloc<T>(nsr + GATHER_PAD);
+    omega = alloc<T>(nsr + GATHER_PAD);
     fr = alloc<T>(nsr);
-    ppr = alloc<T>(nsr);
-    pbcp = alloc<T>((size_t)nx * my * mz);
-    mask = alloc<uint8_t>((size_t)nx * my * mz);
+    ppr = alloc<T>(nsr + GATHER_PAD);
+    pbcp = alloc<T>((size_t)nx * my * mz + GATHER_PAD);
+    mask = alloc<uint8_t>((size_t)nx * my * mz + GATHER_PAD);
     fill_mask();
     dst = alloc<DevState>(1);
     part_n = std::max<size_t>((size_t)3 * nx * 4, (size_t)(E / 8 + 2));
@@ -1159,7 +1161,7 @@ struct Impl : Base {
     int ntiles = (nm + M - 1) / M;
     if ((size_t)ntiles + 1 > part_n) return false;
     size_t lds = std::max(vel_lds_bytes(bd), (size_t)bd * sizeof(double));
-    auto kern = k_velstep_tiled<T, L, M>;
+    auto kern = bd <= 256 ? k_velstep_tiled<T, L, M, 256> : k_velstep_tiled<T, L, M, 512>;
     set_lds(kern, lds);
     kb("velstep");
     kern<<<ntiles + 1, bd, lds, stream>>>(args(rks), ntiles, dstatus);

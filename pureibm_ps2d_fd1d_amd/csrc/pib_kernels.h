@@ -603,8 +603,14 @@ __global__ void __launch_bounds__(PART == 0 ? 512 : 1024) k_form_nl(KArgs<T> A) 
 // one period into 1..n.
 // ===========================================================================
 struct Sten {
-  int ii[2], jj[2], kk[2];  // 0-based wrapped node indices
-  double w[8];              // w[(k*2+j)*2+i] = xval(i)*yval(j)*zval(k)
+  unsigned oi[2], ok[2], oj[2];  // per-axis parts of the 0-based wrapped node index
+  double z[3];                   // position within the cell
+  // node (i, j, k) of the stencil: (ii*mz + kk)*my + jj
+  __device__ unsigned node(int i, int j, int k) const { return oi[i] + ok[k] + oj[j]; }
+  // xval(i)*yval(j)*zval(k), formed where it is used
+  __device__ double w(int i, int j, int k) const {
+    return (i ? z[0] : 1.0 - z[0]) * (j ? z[1] : 1.0 - z[1]) * (k ? z[2] : 1.0 - z[2]);
+  }
 };
 __device__ inline int cellof(double x) { return x < 0.0 ? (int)(x - 1.0) : (int)x; }
 // one-period wrap of the reference, written as a full modulo so that any
@@ -616,36 +622,65 @@ __device__ inline int wrap1(int v, int n) {
 }
 __device__ inline void make_sten(Sten& s, const double pos[3], int nx, int my, int mz) {
   int pc[3] = {cellof(pos[0]), cellof(pos[1]), cellof(pos[2])};
-  double zx = pos[0] - (double)pc[0], zy = pos[1] - (double)pc[1], zz = pos[2] - (double)pc[2];
-  double xv[2] = {1.0 - zx, zx}, yv[2] = {1.0 - zy, zy}, zv[2] = {1.0 - zz, zz};
+  for (int d = 0; d < 3; ++d) s.z[d] = pos[d] - (double)pc[d];
   for (int q = 0; q < 2; ++q) {
-    s.ii[q] = wrap1(pc[0] + q, nx) - 1;
-    s.jj[q] = wrap1(pc[1] + q, my) - 1;
-    s.kk[q] = wrap1(pc[2] + q, mz) - 1;
+    s.oi[q] = (unsigned)(wrap1(pc[0] + q, nx) - 1) * (unsigned)mz * (unsigned)my;
+    s.oj[q] = (unsigned)(wrap1(pc[1] + q, my) - 1);
+    s.ok[q] = (unsigned)(wrap1(pc[2] + q, mz) - 1) * (unsigned)my;
   }
+}
+// The two y corners (j = 0, 1) of a stencil row are adjacent in memory
+// unless the stencil wraps in y, so one 2-element load (4-byte aligned for
+// float, which the hardware's unaligned mode takes) at corner j = 0 fetches
+// both; a wrapping lane then loads its j = 1 corner on its own.  The pair
+// load at the last node of a field reads one element past it: gathered
+// fields are allocated GATHER_PAD elements longer.  v[k][i][j].
+#define GATHER_PAD 64
+template <typename T>
+__device__ inline void gather8(const Sten& s, const T* f, T (&v)[2][2][2]) {
+  typedef T Two __attribute__((ext_vector_type(2)));
+  const bool wrap = s.oj[1] != s.oj[0] + 1u;
+#pragma unroll
   for (int k = 0; k < 2; ++k)
-    for (int j = 0; j < 2; ++j)
-      for (int i = 0; i < 2; ++i) s.w[(k * 2 + j) * 2 + i] = xv[i] * yv[j] * zv[k];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const unsigned n = s.oi[i] + s.ok[k];
+      Two t;
+      __builtin_memcpy(&t, f + n + s.oj[0], sizeof t);
+      v[k][i][0] = t.x;
+      v[k][i][1] = t.y;
+      if (wrap) v[k][i][1] = f[n + s.oj[1]];
+    }
 }
 // interp_threed_scalar order (k, j, i)
 template <typename T>
-__device__ inline double interp_kji(const Sten& s, const T* f, int nx, int my, int mz) {
+__device__ inline double interp_kji(const Sten& s, const T* f) {
+  T v[2][2][2];
+  gather8(s, f, v);
   double acc = 0.0;
   for (int k = 0; k < 2; ++k)
     for (int j = 0; j < 2; ++j)
-      for (int i = 0; i < 2; ++i)
-        acc = acc + (double)f[((size_t)s.ii[i] * mz + s.kk[k]) * my + s.jj[j]] * s.w[(k * 2 + j) * 2 + i];
+      for (int i = 0; i < 2; ++i) acc = acc + (double)v[k][i][j] * s.w(i, j, k);
   return acc;
 }
 // array_dot_product_3d order (i, j, k), general_funcs.F90:126-143
 template <typename T>
-__device__ inline double interp_ijk(const Sten& s, const T* f, int nx, int my, int mz) {
+__device__ inline double interp_ijk(const Sten& s, const T* f) {
+  T v[2][2][2];
+  gather8(s, f, v);
   double acc = 0.0;
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j)
-      for (int k = 0; k < 2; ++k)
-        acc = acc + (double)f[((size_t)s.ii[i] * mz + s.kk[k]) * my + s.jj[j]] * s.w[(k * 2 + j) * 2 + i];
+      for (int k = 0; k < 2; ++k) acc = acc + (double)v[k][i][j] * s.w(i, j, k);
   return acc;
+}
+// The value is complete at this point of the program: the loads and
+// arithmetic it depends on are not deferred past here.  Ends a group of
+// gathers, so that the compiler keeps one group's loads in flight at a time
+// instead of hoisting every gather of a point (registers, occupancy).
+template <typename X>
+__device__ __forceinline__ void pin_reg(X& x) {
+  __asm__ __volatile__("" : "+v"(x));
 }
 
 struct Particles {
@@ -688,10 +723,10 @@ __global__ void __launch_bounds__(256) k_drag_points(KArgs<T> A, Particles P, do
     Sten sp, sv;
     make_sten(sp, xpb, A.nx, A.my, A.mz);
     make_sten(sv, xl, A.nx, A.my, A.mz);
-    double pl = interp_ijk(sp, A.pbcp, A.nx, A.my, A.mz);
+    double pl = interp_ijk(sp, A.pbcp);
     size_t cs = (size_t)A.nx * A.mz * A.my;
     double vort[3];
-    for (int n = 0; n < 3; ++n) vort[n] = interp_kji(sv, A.omega + n * cs, A.nx, A.my, A.mz);
+    for (int n = 0; n < 3; ++n) vort[n] = interp_kji(sv, A.omega + n * cs);
     double cd1 = P.cd[l * 3 + 0], cd2 = P.cd[l * 3 + 1], cd3 = P.cd[l * 3 + 2];
     double df[3];
     df[0] = -(vort[2] * cd2 - vort[1] * cd3);
@@ -769,15 +804,11 @@ __device__ void forcing_point(const KArgs<T>& A, const Particles& P, int m, int 
   make_sten(so, xlo, A.nx, A.my, A.mz);
   make_sten(sp, xpi, A.nx, A.my, A.mz);
   make_sten(si, xli, A.nx, A.my, A.mz);
-  double ulo[3], uli[3], nll[3], onll[3], dfll[3], ppll[3];
-  for (int n = 0; n < 3; ++n) {
-    ulo[n] = interp_kji(so, A.ubcp + n * cs, A.nx, A.my, A.mz);
-    ppll[n] = interp_kji(sp, A.ppr + n * cs, A.nx, A.my, A.mz);
-    uli[n] = interp_kji(si, A.ubcp + n * cs, A.nx, A.my, A.mz);
-    nll[n] = interp_ijk(si, A.nlbc + n * cs, A.nx, A.my, A.mz);
-    onll[n] = interp_ijk(si, A.onlbc + n * cs, A.nx, A.my, A.mz);
-    dfll[n] = interp_ijk(si, A.diffn + n * cs, A.nx, A.my, A.mz);
-  }
+  // gathers in groups of three interpolations; ft accumulates in the
+  // reference's order of terms
+  double ulo[3];
+  for (int n = 0; n < 3; ++n) ulo[n] = interp_kji(so, A.ubcp + n * cs);
+  for (int n = 0; n < 3; ++n) pin_reg(ulo[n]);
   double w1 = P.angv[m], w2 = P.angv[P.nbody + m], w3 = P.angv[2 * P.nbody + m];
   double crossp[3] = {w2 * snorm[2] - w3 * snorm[1], w3 * snorm[0] - w1 * snorm[2], w1 * snorm[1] - w2 * snorm[0]};
   double linvel[3], unmag = 0.0, ubnmag = 0.0;
@@ -786,7 +817,7 @@ __device__ void forcing_point(const KArgs<T>& A, const Particles& P, int m, int 
     unmag = unmag + snorm[d] * ulo[d];
     ubnmag = ubnmag + snorm[d] * linvel[d];
   }
-  double ucar[3];
+  double ft[3];
   for (int d = 0; d < 3; ++d) {
     double unorm = snorm[d] * unmag;
     double utang = ulo[d] - unorm;
@@ -795,24 +826,31 @@ __device__ void forcing_point(const KArgs<T>& A, const Particles& P, int m, int 
     double utangi = (ubtang * (ro - ri) - utang * (rb - ri)) / (ro - rb);
     double unormi = 0.0;
     if (st->revernorm == 1) unormi = (ubnorm * (ro - ri) - unorm * (rb - ri)) / (ro - rb);
-    ucar[d] = utangi + unormi;
+    ft[d] = -(utangi + unormi);  // -ucar
   }
+  for (int n = 0; n < 3; ++n) ft[n] = st->s.cf * (ft[n] + interp_kji(si, A.ubcp + n * cs));
+  for (int n = 0; n < 3; ++n) pin_reg(ft[n]);
+  for (int n = 0; n < 3; ++n) ft[n] = ft[n] - c3 * interp_ijk(si, A.nlbc + n * cs);
+  for (int n = 0; n < 3; ++n) pin_reg(ft[n]);
+  for (int n = 0; n < 3; ++n) ft[n] = ft[n] - c4 * interp_ijk(si, A.onlbc + n * cs);
+  for (int n = 0; n < 3; ++n) pin_reg(ft[n]);
+  double ppll[3];
+  for (int n = 0; n < 3; ++n) ppll[n] = interp_kji(sp, A.ppr + n * cs);
+  for (int n = 0; n < 3; ++n) pin_reg(ppll[n]);
+  for (int n = 0; n < 3; ++n) ft[n] = ft[n] + (c1 + c2) * (ppll[n] - interp_ijk(si, A.diffn + n * cs));
+  for (int n = 0; n < 3; ++n) pin_reg(ft[n]);
   double rdi = ri * st->dx;
   double da2 = 4.0 * 3.14159265358979323846 * (rdi * rdi) / (double)P.nrpr_active[m];
-  double ft[3];
-  for (int n = 0; n < 3; ++n) {
-    ft[n] = st->s.cf * (-ucar[n] + uli[n]);
-    ft[n] = ft[n] - c3 * nll[n] - c4 * onll[n];
-    ft[n] = ft[n] + (c1 + c2) * (ppll[n] - dfll[n]);
-    ft[n] = ft[n] * da2 * st->dr * st->dx;
-  }
+  for (int n = 0; n < 3; ++n) ft[n] = ft[n] * da2 * st->dr * st->dx;
   int count_fl = 0;
   double force_fl[3] = {0, 0, 0}, fm[3] = {0, 0, 0}, fmf[3] = {0, 0, 0};
+  uint8_t mk8[2][2][2];
+  gather8(si, A.mask, mk8);
   for (int k = 0; k < 2; ++k)
     for (int j = 0; j < 2; ++j)
       for (int i = 0; i < 2; ++i) {
-        bool fl = A.mask[((size_t)si.ii[i] * A.mz + si.kk[k]) * A.my + si.jj[j]] != 0;
-        double w = si.w[(k * 2 + j) * 2 + i];
+        bool fl = mk8[k][i][j] != 0;
+        double w = si.w(i, j, k);
         if (fl) count_fl++;
         for (int n = 0; n < 3; ++n) {
           double tp = w * ft[n];

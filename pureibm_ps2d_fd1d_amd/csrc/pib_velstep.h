@@ -20,9 +20,11 @@
 // lane then back-substitutes its interior rows.  The arithmetic is that of
 // an exact direct solve, rounded differently from the reference's.
 //
-// The pressure solution replaces p (the reference's p = pold + (x - pold)),
-// so p is only written here.  The (0,0) mode is processed in double
-// precision by the last workgroup, with L = 2 and one mode.
+// The pressure solution replaces p (the reference's p = pold + (x - pold));
+// it is stored right after the pressure solve and updated in place by
+// divcorr.  The (0,0) mode is processed in double precision by the last
+// workgroup, with L = 2 and one mode (its pressure is rounded to the bulk
+// type when stored between the two phases).
 #pragma once
 #include "pib_kernels.h"
 
@@ -202,6 +204,12 @@ static inline size_t vel_lds_bytes(int bd) {
   return (size_t)bd * (6 * sizeof(C2<double>) + 3 * sizeof(C2<double>) + 6 * sizeof(double));
 }
 
+// The value is complete at this point of the program: its loads and
+// arithmetic are not deferred past here.  Bounds how many loads are in
+// flight (and so the registers) of the per-component load phases.
+template <typename R>
+__device__ __forceinline__ void pin(C2<R>& v) { __asm__ __volatile__("" : "+v"(v.x), "+v"(v.y)); }
+
 template <typename T, typename R, int L, int M>
 struct VTile {
   const KArgs<T>& A;
@@ -331,9 +339,9 @@ __device__ void vel_tile(const KArgs<T>& A, int q0, bool mean, unsigned char* ra
 #pragma unroll
     for (int k = 0; k < L; ++k)
       h[c][k] = (cscal(c3, V.ld(A.nl, c, i0 + k)) + V.ld(A.ff, c, i0 + k)) + cscal(c4, V.ld(A.onl, c, i0 + k));
-    // one component's 3L loads in flight at a time: keeps the load phase
-    // within the register budget of three workgroups per CU
-    __asm__ __volatile__("" ::: "memory");
+    // one component's 3L loads in flight at a time
+#pragma unroll
+    for (int k = 0; k < L; ++k) pin(h[c][k]);
   }
   // ---- pressure (poisson.F90:55-239) ----
   // the pressure/divcorr matrix is set up again for divcorr rather than kept
@@ -362,6 +370,10 @@ __device__ void vel_tile(const KArgs<T>& A, int q0, bool mean, unsigned char* ra
   if (mean)  // the (0,0) mode keeps only the real part (poisson.F90:139)
 #pragma unroll
     for (int k = 0; k < L; ++k) p[0][k].y = (R)0;
+  // the pressure goes to p now and is read back for the divcorr update, so
+  // it holds no registers through the ustep and divcorr solves
+#pragma unroll
+  for (int k = 0; k < L; ++k) V.stv(A.p, 0, i0 + k, p[0][k]);
   // ---- ustep (usteptime.F90:45-258) ----
   C2<R> pl;
   {
@@ -388,6 +400,8 @@ __device__ void vel_tile(const KArgs<T>& A, int q0, bool mean, unsigned char* ra
         C2<R> lap = (uu[k] + uu[k + 2]) - cscal((R)2, uu[k + 1]);
         h[c][k] = r + cscal(sl, lap);
       }
+#pragma unroll
+      for (int k = 0; k < L; ++k) pin(h[c][k]);
     }
   }
   {
@@ -445,14 +459,13 @@ __device__ void vel_tile(const KArgs<T>& A, int q0, bool mean, unsigned char* ra
       h[2][k] = h[2][k] - cik(gh * wz, sm);
       C2<R> lap = (dm - cscal((R)2, di)) + dp;
       C2<R> corr = cscal(idx2, lap) - cscal(w2, di);
-      p[0][k] = (p[0][k] + di) - cscal(cv, corr);
+      V.stv(A.p, 0, i0 + k, (V.ld(A.p, 0, i0 + k) + di) - cscal(cv, corr));
     }
 #pragma unroll
     for (int k = 0; k < L; ++k) {
       V.stv(A.u, 0, i0 + k, h[0][k]);
       V.stv(A.u, 1, i0 + k, h[1][k]);
       V.stv(A.u, 2, i0 + k, h[2][k]);
-      V.stv(A.p, 0, i0 + k, p[0][k]);
     }
   }
   // ---- check_divergence of the new velocity ----
@@ -473,9 +486,11 @@ __device__ void vel_tile(const KArgs<T>& A, int q0, bool mean, unsigned char* ra
 }
 
 // grid: ntiles tile workgroups + 1 workgroup for the (0,0) mode;
-// block = M * nx / L lanes
-template <typename T, int L, int M>
-__global__ void __launch_bounds__(512) k_velstep_tiled(KArgs<T> A, int ntiles, int* status) {
+// block = M * nx / L lanes, at most NT.  Workgroups of up to 256 lanes are
+// held to three per CU (168 registers), so that the 128^3 tile counts fill
+// the chip in one (FP32) or two (FP64) rounds of workgroups.
+template <typename T, int L, int M, int NT>
+__global__ void __launch_bounds__(NT, NT <= 256 ? 3 : 1) k_velstep_tiled(KArgs<T> A, int ntiles, int* status) {
   extern __shared__ __align__(16) unsigned char vlds[];
   double dmax = 0.0;
   int st = 0;
