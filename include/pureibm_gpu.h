@@ -206,11 +206,6 @@ int pib_kernel_times(void* ctx, int maxn, char* names, double* total_ms, int* la
  *   PIB_TUNE_FFT: 1 = plane transforms of a compiled shape (my/2, mz) in
  *     {(64,128), (32,64), (16,32)} use the two-stage kernels (default); 0 =
  *     always the generic mixed-radix transform.
- *   PIB_TUNE_NL_FORM, for form_nl of a compiled shape: 0 = one launch that
- *     holds the three transforms' sums in registers; 1 = two launches, the
- *     y/z product transforms, then the x-flux transform added in place; 2 =
- *     one launch whose three transforms each update nl in place (default).
- *     All forms add in the same order.
  *   PIB_TUNE_GRAPH: 1 = pib_step replays whole steps from HIP graphs (default;
  *     the last step of each call and steps under pib_set_timing run eagerly);
  *     0 = every step is launched kernel by kernel.
@@ -218,7 +213,6 @@ int pib_kernel_times(void* ctx, int maxn, char* names, double* total_ms, int* la
 #define PIB_TUNE_VELSTEP 0
 #define PIB_TUNE_GRAPH 1
 #define PIB_TUNE_FFT 2
-#define PIB_TUNE_NL_FORM 3
 int pib_set_tuning(void* ctx, int key, int value);
 
 #ifdef __cplusplus

@@ -25,7 +25,6 @@ PIB_ERR_PIVOT = -6
 PIB_TUNE_VELSTEP = 0
 PIB_TUNE_GRAPH = 1
 PIB_TUNE_FFT = 2
-PIB_TUNE_NL_FORM = 3
 
 PIB_FP64 = 0
 PIB_FP32_BULK = 1

@@ -166,106 +166,171 @@ __global__ void k_gather_records(int n, const int* rec_sorted, const double* w, 
 // deterministic block trees, the scalar logic runs on lane 0
 // ===========================================================================
 #define SCAL_T 256
+#define DRAG_T 1024
+
+// The serial logic of a scalar kernel runs on lane 0 against a copy of the
+// device state in LDS: one coalesced load of the whole state and one store
+// back, instead of a global-memory round trip for each field it touches.
+static_assert(sizeof(DevState) % sizeof(unsigned) == 0, "state copied in words");
+constexpr int STATE_WORDS = (int)(sizeof(DevState) / sizeof(unsigned));
+constexpr int STATE_PER_LANE = (STATE_WORDS + SCAL_T - 1) / SCAL_T;
+// issue: the lane's words of the state (loads in flight until state_put)
+__device__ inline void state_get(unsigned (&w)[STATE_PER_LANE], const DevState* st) {
+#pragma unroll
+  for (int r = 0; r < STATE_PER_LANE; ++r) {
+    int q = threadIdx.x + r * blockDim.x;
+    w[r] = q < STATE_WORDS ? ((const unsigned*)st)[q] : 0u;
+  }
+}
+__device__ inline void state_put(DevState* L, const unsigned (&w)[STATE_PER_LANE]) {
+#pragma unroll
+  for (int r = 0; r < STATE_PER_LANE; ++r) {
+    int q = threadIdx.x + r * blockDim.x;
+    if (q < STATE_WORDS) ((unsigned*)L)[q] = w[r];
+  }
+  __syncthreads();
+}
+__device__ inline void state_store(DevState* st, const DevState* L) {
+  __syncthreads();
+  for (int q = threadIdx.x; q < STATE_WORDS; q += blockDim.x) ((unsigned*)st)[q] = ((const unsigned*)L)[q];
+}
 
 // compute_new_timestep tail (dependent_functions.F90:1143-1277); part holds
 // per-(component, plane) fluid/solid/total sums and max|u| from c2r_ubcp
-__global__ void __launch_bounds__(SCAL_T) k_scalar_cnt(DevState* st, const double* part, int nx, int rks,
-                                                       const double* velbdy) {
+__global__ void __launch_bounds__(SCAL_T) k_scalar_cnt(DevState* stg, const double* part, int nx, int rks,
+                                                       const double* velbdy, int nb) {
+  __shared__ DevState L;
   __shared__ double sh[12 * 16];
   const int t = threadIdx.x;
+  // every global load of the kernel is in flight before the first use
+  unsigned sw[STATE_PER_LANE];
+  state_get(sw, stg);
   double v[12];
   for (int q = 0; q < 12; ++q) v[q] = 0.0;
   double mx[3] = {PIB_SMALL_NUMBER, PIB_SMALL_NUMBER, PIB_SMALL_NUMBER};
-  for (int c = 0; c < 3; ++c)
-    for (int i = t; i < nx; i += SCAL_T) {
+  for (int i = t; i < nx; i += SCAL_T)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
       const double* o = part + ((size_t)c * nx + i) * 4;
       v[c * 3 + 0] += o[0];
       v[c * 3 + 1] += o[1];
       v[c * 3 + 2] += o[2];
       mx[c] = fmax(mx[c], o[3]);
     }
-  const int nb = st->nbody;
   for (int m = t; m < nb; m += SCAL_T)
+#pragma unroll
     for (int n = 0; n < 3; ++n) v[9 + n] += velbdy[(size_t)n * nb + m];
+  state_put(&L, sw);
+  DevState* st = &L;
   block_sums_w<12>(v, sh);
   double mxc[3] = {mx[0], mx[1], mx[2]};
   block_maxes_w<3>(mxc, sh);
-  if (t != 0) return;
-  pib_scalars& s = st->s;
-  double umax_tmp = fmax(fmax(mxc[0], mxc[1]), mxc[2]);
-  double dtnew;
-  if (umax_tmp * s.dt / st->dx > st->cfl_max)
-    dtnew = st->cfl_max * st->dx / umax_tmp;
-  else
-    dtnew = s.dt;
-  s.dtnew = dtnew;
-  s.umax_tmp = umax_tmp;
-  double ntot = (double)((long long)st->nx * st->my * st->mz);
-  for (int n = 0; n < 3; ++n) {
-    s.ufmean_old[n] = s.ufmean[n];
-    s.ufmean[n] = v[n * 3 + 0] / s.count_fluid;
-    s.usmean_temp[n] = v[n * 3 + 1] / s.count_solid;
-    s.umean_temp[n] = v[n * 3 + 2] / ntot;
+  if (t == 0) {
+    pib_scalars& s = st->s;
+    double umax_tmp = fmax(fmax(mxc[0], mxc[1]), mxc[2]);
+    double dtnew;
+    if (umax_tmp * s.dt / st->dx > st->cfl_max)
+      dtnew = st->cfl_max * st->dx / umax_tmp;
+    else
+      dtnew = s.dt;
+    s.dtnew = dtnew;
+    s.umax_tmp = umax_tmp;
+    double ntot = (double)((long long)st->nx * st->my * st->mz);
+    for (int n = 0; n < 3; ++n) {
+      s.ufmean_old[n] = s.ufmean[n];
+      s.ufmean[n] = v[n * 3 + 0] / s.count_fluid;
+      s.usmean_temp[n] = v[n * 3 + 1] / s.count_solid;
+      s.umean_temp[n] = v[n * 3 + 2] / ntot;
+    }
+    double meanslip[3], mix[3];
+    for (int n = 0; n < 3; ++n) {
+      double msv = nb > 0 ? v[9 + n] / (double)nb : 0.0;
+      s.usmean_act[n] = msv;
+      s.dufmeandt[n] = -s.cf * (s.ufmean[n] - s.ufmean_old[n]);
+      meanslip[n] = (s.ufmean[n] - s.usmean_des[n]) * (1.0 - s.maxvolfrac);
+      double us = 0.0 + s.mean_volfrac * msv;
+      s.usmean[n] = us / s.mean_volfrac;
+      mix[n] = (1.0 - s.maxvolfrac) * (s.usmean[n] - s.ufmean[n]);
+    }
+    s.meanslipmod = sqrt(meanslip[0] * meanslip[0] + meanslip[1] * meanslip[1] + meanslip[2] * meanslip[2]);
+    s.mixmeanslipmod = sqrt(mix[0] * mix[0] + mix[1] * mix[1] + mix[2] * mix[2]);
+    if (rks == st->itrmax && st->adaptive) s.dt = dtnew;
+    s.t = s.t + s.dt * (st->coef[rks - 1][0] + st->coef[rks - 1][1]);
+    double tempt = s.ramp_frac_time * (double)(float)(st->nx) * st->dx / st->ucharmod;
+    if (st->soft_start && s.t < tempt) {
+      s.cf = -1.0 / s.dt;
+      s.cforig = s.cf;
+      s.cf = s.cf * (s.t / tempt);
+    } else {
+      s.cf = -1.0 / s.dt;
+      s.cforig = s.cf;
+    }
   }
-  double meanslip[3], mix[3];
-  for (int n = 0; n < 3; ++n) {
-    double msv = nb > 0 ? v[9 + n] / (double)nb : 0.0;
-    s.usmean_act[n] = msv;
-    s.dufmeandt[n] = -s.cf * (s.ufmean[n] - s.ufmean_old[n]);
-    meanslip[n] = (s.ufmean[n] - s.usmean_des[n]) * (1.0 - s.maxvolfrac);
-    double us = 0.0 + s.mean_volfrac * msv;
-    s.usmean[n] = us / s.mean_volfrac;
-    mix[n] = (1.0 - s.maxvolfrac) * (s.usmean[n] - s.ufmean[n]);
-  }
-  s.meanslipmod = sqrt(meanslip[0] * meanslip[0] + meanslip[1] * meanslip[1] + meanslip[2] * meanslip[2]);
-  s.mixmeanslipmod = sqrt(mix[0] * mix[0] + mix[1] * mix[1] + mix[2] * mix[2]);
-  if (rks == st->itrmax && st->adaptive) s.dt = dtnew;
-  s.t = s.t + s.dt * (st->coef[rks - 1][0] + st->coef[rks - 1][1]);
-  double tempt = s.ramp_frac_time * (double)(float)(st->nx) * st->dx / st->ucharmod;
-  if (st->soft_start && s.t < tempt) {
-    s.cf = -1.0 / s.dt;
-    s.cforig = s.cf;
-    s.cf = s.cf * (s.t / tempt);
-  } else {
-    s.cf = -1.0 / s.dt;
-    s.cforig = s.cf;
-  }
+  state_store(stg, &L);
 }
 
 // calc_pres_visc_drag per-body totals (boundary_condition.F90:1631-1645) from
-// the (body, chunk) partials of k_drag_points, then the compute_mpg_at_n
-// tail (boundary_condition.F90:373-435) and the per-body forces.
-__global__ void __launch_bounds__(SCAL_T) k_scalar_drag(DevState* st, const double* dpart, int G, int rks,
-                                                        const double* rad, double* pres, double* visc,
-                                                        double* torq, double* force, double* force_chem) {
+// the (body, chunk) partials of k_drag_points, the compute_mpg_at_n tail
+// (boundary_condition.F90:373-435), the per-body forces, then the bcset
+// diagnostics of the phase (boundary_condition.F90:723-884).  One workgroup
+// of DRAG_T lanes; lane items are (body, quantity) pairs.
+__global__ void __launch_bounds__(DRAG_T) k_scalar_drag(DevState* stg, const double* __restrict__ dpart, int G,
+                                                        int nb, int rks, const double* __restrict__ rad,
+                                                        double* __restrict__ pres, double* __restrict__ visc,
+                                                        double* __restrict__ torq, double* __restrict__ force,
+                                                        double* __restrict__ force_chem) {
+  __shared__ DevState L;
   __shared__ double sh[12 * 16];
   __shared__ double mpg_s[3];
   const int t = threadIdx.x;
-  const int nb = st->nbody;
+  unsigned sw[STATE_PER_LANE];
+  state_get(sw, stg);
   double v[12];
   for (int q = 0; q < 12; ++q) v[q] = 0.0;
-  for (int m = t; m < nb; m += SCAL_T) {
-    double tot[9];
-    for (int q = 0; q < 9; ++q) tot[q] = 0.0;
-#pragma unroll 3
-    for (int g = 0; g < G; ++g)
+  // items (body m, quantity q); two items per lane per round, their partial
+  // loads (the first 16 chunks) and previous totals in flight together
+  const int nitem = nb * 9;
+  for (int j0 = t; j0 < nitem; j0 += 2 * DRAG_T) {
+    double x[2][16], old[2];
 #pragma unroll
-      for (int q = 0; q < 9; ++q) tot[q] += dpart[((size_t)m * G + g) * 9 + q];
-    for (int d = 0; d < 3; ++d) {
-      size_t k = (size_t)d * nb + m;
-      double p0 = rks == 1 ? 0.0 : pres[k];
-      double v0 = rks == 1 ? 0.0 : visc[k];
-      double t0 = rks == 1 ? 0.0 : torq[k];
-      double pn = p0 - tot[d], vn = v0 + tot[3 + d];
-      pres[k] = pn;
-      visc[k] = vn;
-      torq[k] = t0 + tot[6 + d];
-      v[d] += tot[d];          // pres_total
-      v[3 + d] += tot[3 + d];  // visc_total
-      v[6 + d] += pn;          // sum of pres(m)
-      v[9 + d] += vn;          // sum of visc(m)
+    for (int u = 0; u < 2; ++u) {
+      const int j = j0 + u * DRAG_T;
+      const bool ok = j < nitem;
+      const int m = ok ? j / 9 : 0, q = ok ? j - m * 9 : 0;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) x[u][r] = ok && r < G ? dpart[((size_t)m * G + r) * 9 + q] : 0.0;
+      const double* acc = q < 3 ? pres : (q < 6 ? visc : torq);
+      old[u] = ok && rks != 1 ? acc[(size_t)(q % 3) * nb + m] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int j = j0 + u * DRAG_T;
+      if (j >= nitem) continue;
+      const int m = j / 9, q = j - m * 9, d = q % 3;
+      const size_t k = (size_t)d * nb + m;
+      // the chunk sums in chunk order
+      double tot = 0.0;
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+        if (r < G) tot += x[u][r];
+      for (int g = 16; g < G; ++g) tot += dpart[((size_t)m * G + g) * 9 + q];
+      if (q < 3) {
+        double pn = old[u] - tot;
+        pres[k] = pn;
+        v[d] += tot;     // pres_total
+        v[6 + d] += pn;  // sum of pres(m)
+      } else if (q < 6) {
+        double vn = old[u] + tot;
+        visc[k] = vn;
+        v[3 + d] += tot;  // visc_total
+        v[9 + d] += vn;   // sum of visc(m)
+      } else {
+        torq[k] = old[u] + tot;
+      }
     }
   }
+  state_put(&L, sw);
+  DevState* st = &L;
   block_sums_w<12>(v, sh);
   pib_scalars& s = st->s;
   const double PI = 3.14159265358979323846;
@@ -291,17 +356,45 @@ __global__ void __launch_bounds__(SCAL_T) k_scalar_drag(DevState* st, const doub
     }
     for (int n = 0; n < 3; ++n) mpg_s[n] = s.mpg[n];
   }
-  __syncthreads();
-  double c12 = st->coef[rks - 1][0] + st->coef[rks - 1][1];
-  for (int m = t; m < nb; m += SCAL_T) {
+  __syncthreads();  // pres/visc of every body and mpg_s are in place
+  const double c12 = st->coef[rks - 1][0] + st->coef[rks - 1][1];
+  double f6[6] = {0, 0, 0, 0, 0, 0};
+  for (int j = t; j < nb * 3; j += DRAG_T) {
+    const int m = j / 3, n = j - m * 3;
+    const size_t k = (size_t)n * nb + m;
     double d = 2.0 * rad[m] * st->dx;
     double vol = d * d * d;
-    for (int n = 0; n < 3; ++n) {
-      size_t k = (size_t)n * nb + m;
-      force[k] = c12 * (pres[k] + visc[k]) - c12 * mpg_s[n] * PI * vol / 6.0;
-      force_chem[k] = c12 * (pres[k] + visc[k]);
-    }
+    double fc = c12 * (pres[k] + visc[k]);
+    double fo = fc - c12 * mpg_s[n] * PI * vol / 6.0;
+    force[k] = fo;
+    force_chem[k] = fc;
+    f6[n] += fo;
+    f6[3 + n] += fc;
   }
+  block_sums_w<6>(f6, sh);
+  if (t == 0) {
+    double af[3], ac[3];
+    for (int n = 0; n < 3; ++n) {
+      af[n] = (0.0 + s.mean_volfrac * (f6[n] / nb)) / s.mean_volfrac;
+      ac[n] = (0.0 + s.mean_volfrac * (f6[3 + n] / nb)) / s.mean_volfrac;
+    }
+    double nd = sqrt(af[0] * af[0] + af[1] * af[1] + af[2] * af[2]);
+    double ndc = sqrt(ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2]);
+    double mpg_avg = sqrt(s.mpg[0] * s.mpg[0] + s.mpg[1] * s.mpg[1] + s.mpg[2] * s.mpg[2]);
+    s.total_drag_mpg = mpg_avg * st->voldom / st->norm_factor;
+    double nf;
+    if (st->set_mpg)
+      nf = (3.0 * PI * st->vis * (s.mixmeanslipmod + PIB_SMALL_NUMBER) * st->dia);
+    else
+      nf = (3.0 * PI * st->vis * st->ucharmod * st->dia);
+    s.norm_drag = nd / nf;
+    s.norm_drag_chem = ndc / nf;
+    if (s.norm_drag > 0.0) s.ferror = fabs(s.norm_drag - s.fold) / s.norm_drag;
+    else s.ferror = 1.0;
+    s.fold = s.norm_drag;
+    if (!(s.norm_drag <= (double)1.0E+06f) && st->status == 0) st->status = PIB_ERR_BLOWUP;
+  }
+  state_store(stg, &L);
 }
 
 // frmean / frmeanfluid over all reversal pairs (boundary_condition.F90:632-633)
@@ -314,62 +407,45 @@ __global__ void __launch_bounds__(SCAL_T) k_reduce_frmean(DevState* st, const do
   block_sums_w<6>(a, sh);
   if (threadIdx.x == 0) {
     double ntot = (double)((long long)st->nx * st->my * st->mz);
+    double cfl = st->s.count_fluid;
+    double r[6];
     for (int n = 0; n < 3; ++n) {
-      st->s.frmean[n] = a[n] / ntot;
-      st->s.frmeanfluid[n] = a[3 + n] / st->s.count_fluid;
+      r[n] = a[n] / ntot;
+      r[3 + n] = a[3 + n] / cfl;
+    }
+    for (int n = 0; n < 3; ++n) {
+      st->s.frmean[n] = r[n];
+      st->s.frmeanfluid[n] = r[3 + n];
     }
   }
-}
-
-// bcset diagnostics (boundary_condition.F90:723-884), one phase
-__global__ void __launch_bounds__(SCAL_T) k_scalar_diag(DevState* st, const double* force,
-                                                        const double* force_chem) {
-  __shared__ double sh[6 * 16];
-  const int nb = st->nbody;
-  double v[6] = {0, 0, 0, 0, 0, 0};
-  for (int m = threadIdx.x; m < nb; m += SCAL_T)
-    for (int n = 0; n < 3; ++n) {
-      v[n] += force[(size_t)n * nb + m];
-      v[3 + n] += force_chem[(size_t)n * nb + m];
-    }
-  block_sums_w<6>(v, sh);
-  if (threadIdx.x != 0) return;
-  pib_scalars& s = st->s;
-  const double PI = 3.14159265358979323846;
-  double af[3], ac[3];
-  for (int n = 0; n < 3; ++n) {
-    af[n] = (0.0 + s.mean_volfrac * (v[n] / nb)) / s.mean_volfrac;
-    ac[n] = (0.0 + s.mean_volfrac * (v[3 + n] / nb)) / s.mean_volfrac;
-  }
-  double nd = sqrt(af[0] * af[0] + af[1] * af[1] + af[2] * af[2]);
-  double ndc = sqrt(ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2]);
-  double mpg_avg = sqrt(s.mpg[0] * s.mpg[0] + s.mpg[1] * s.mpg[1] + s.mpg[2] * s.mpg[2]);
-  s.total_drag_mpg = mpg_avg * st->voldom / st->norm_factor;
-  double nf;
-  if (st->set_mpg)
-    nf = (3.0 * PI * st->vis * (s.mixmeanslipmod + PIB_SMALL_NUMBER) * st->dia);
-  else
-    nf = (3.0 * PI * st->vis * st->ucharmod * st->dia);
-  s.norm_drag = nd / nf;
-  s.norm_drag_chem = ndc / nf;
-  if (s.norm_drag > 0.0) s.ferror = fabs(s.norm_drag - s.fold) / s.norm_drag;
-  else s.ferror = 1.0;
-  s.fold = s.norm_drag;
-  if (!(s.norm_drag <= (double)1.0E+06f) && st->status == 0) st->status = PIB_ERR_BLOWUP;
 }
 
 // velstep tail: check_divergence max, umean update (steptheflow.F90:93-99)
-__global__ void __launch_bounds__(SCAL_T) k_scalar_vel(DevState* st, const double* part, int nblk, int* status) {
+__global__ void __launch_bounds__(SCAL_T) k_scalar_vel(DevState* stg, const double* part, int nblk, int* status) {
+  __shared__ DevState L;
   __shared__ double sh[16];
+  unsigned sw[STATE_PER_LANE];
+  state_get(sw, stg);
   double mx[1] = {0.0};
-  for (int b = threadIdx.x; b < nblk; b += SCAL_T) mx[0] = fmax(mx[0], part[b]);
+  for (int b = threadIdx.x; b < nblk; b += 4 * SCAL_T) {
+    double a[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) a[r] = b + r * SCAL_T < nblk ? part[b + r * SCAL_T] : 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) mx[0] = fmax(mx[0], a[r]);
+  }
+  state_put(&L, sw);
   block_maxes_w<1>(mx, sh);
-  if (threadIdx.x != 0) return;
-  st->s.divmax = mx[0];
-  for (int n = 0; n < 3; ++n)
-    st->s.umean[n] = st->s.umean[n] + (-st->s.mpg[n] + st->s.frmean[n] - st->s.frame_accln[n]) * st->s.dt;
-  if (*status && st->status == 0) st->status = *status;
-  *status = 0;  // the velstep kernels only ever raise it
+  if (threadIdx.x == 0) {
+    DevState* st = &L;
+    st->s.divmax = mx[0];
+    for (int n = 0; n < 3; ++n)
+      st->s.umean[n] = st->s.umean[n] + (-st->s.mpg[n] + st->s.frmean[n] - st->s.frame_accln[n]) * st->s.dt;
+    int e = *status;
+    if (e && st->status == 0) st->status = e;
+    *status = 0;  // the velstep kernels only ever raise it
+  }
+  state_store(stg, &L);
 }
 
 // ===========================================================================
@@ -802,7 +878,6 @@ struct Impl : Base {
   // one and fft_compiled is set, else the generic mixed-radix transform
   int sh = 0;
   bool fft_compiled = true;
-  int nl_form = 2;  // form_nl of a compiled shape: 0 accumulators, 1 two launches, 2 in place
   int cur_sh() const { return fft_compiled ? sh : 0; }
   int plane_nt(int s) const { return s ? shape_nt(s) : nt_plane; }
   template <int KIND, int SH>
@@ -1033,7 +1108,7 @@ struct Impl : Base {
   void compute_new_timestep(int rks) {
     c2r_pair<C2R_UBCP, 3, C2R_PBCP, 1>(rks, "c2r_ubcp_pbcp");
     kb("scalar_cnt");
-    k_scalar_cnt<<<1, SCAL_T, 0, stream>>>(dst, part, nx, rks, vel ? vel : part);
+    k_scalar_cnt<<<1, SCAL_T, 0, stream>>>(dst, part, nx, rks, vel ? vel : part, vel ? nbody : 0);
     HIPCHK(hipGetLastError());
     ke();
   }
@@ -1066,25 +1141,10 @@ struct Impl : Base {
   }
   template <int SH>
   void launch_nl_sh(bool up, bool mm, int rks) {
-    if (nl_form == 2) {
-      constexpr int NT = Shape<SH>::NT;
-      if (!up) run_nl(k_form_nl<T, 1, false, false, SH, 3>, rks, NT);
-      else if (!mm) run_nl(k_form_nl<T, 1, true, false, SH, 3>, rks, NT);
-      else run_nl(k_form_nl<T, 1, true, true, SH, 3>, rks, NT);
-      return;
-    }
-    if (nl_form == 1) {
-      constexpr int NIT = NlShape<SH>::NIT2, NT = Shape<SH>::NT;
-      run_nl(k_form_nl<T, NIT, false, false, SH, 1>, rks, NT, "form_nl_yz");
-      if (!up) run_nl(k_form_nl<T, 1, false, false, SH, 2>, rks, NT, "form_nl_x");
-      else if (!mm) run_nl(k_form_nl<T, 1, true, false, SH, 2>, rks, NT, "form_nl_x");
-      else run_nl(k_form_nl<T, 1, true, true, SH, 2>, rks, NT, "form_nl_x");
-      return;
-    }
-    constexpr int NIT = NlShape<SH>::NIT, NT = NlShape<SH>::NT;
-    if (!up) run_nl(k_form_nl<T, NIT, false, false, SH>, rks, NT);
-    else if (!mm) run_nl(k_form_nl<T, NIT, true, false, SH>, rks, NT);
-    else run_nl(k_form_nl<T, NIT, true, true, SH>, rks, NT);
+    constexpr int NT = Shape<SH>::NT;
+    if (!up) run_nl(k_form_nl<T, 1, false, false, SH>, rks, NT);
+    else if (!mm) run_nl(k_form_nl<T, 1, true, false, SH>, rks, NT);
+    else run_nl(k_form_nl<T, 1, true, true, SH>, rks, NT);
   }
   template <typename K>
   void run_nl(K kern, int rks, int nt, const char* name = "form_nl") {
@@ -1117,7 +1177,7 @@ struct Impl : Base {
     HIPCHK(hipGetLastError());
     ke();
     kb("scalar_drag");
-    k_scalar_drag<<<1, SCAL_T, 0, stream>>>(dst, drag_part, G, rks, rad, pres, visc, torq, force, force_chem);
+    k_scalar_drag<<<1, DRAG_T, 0, stream>>>(dst, drag_part, G, nbody, rks, rad, pres, visc, torq, force, force_chem);
     HIPCHK(hipGetLastError());
     ke();
     c2r<C2R_PGRAD>(rks, 3);
@@ -1145,10 +1205,6 @@ struct Impl : Base {
       case 3: r2c_force<3>(rks); break;
       default: r2c_force<0>(rks);
     }
-    kb("scalar_diag");
-    k_scalar_diag<<<1, SCAL_T, 0, stream>>>(dst, force, force_chem);
-    HIPCHK(hipGetLastError());
-    ke();
   }
   // tiled velstep: M modes per workgroup, L x rows per lane; false when the
   // shape does not fit (nx % L, workgroup size, the (0,0)-mode workgroup)
@@ -1691,14 +1747,6 @@ int pib_set_tuning(void* ctx, int key, int value) {
       HIPCHK(hipStreamSynchronize(I.stream));
       I.drop_graphs();
       I.fft_compiled = value == 1;
-    });
-  }
-  if (key == PIB_TUNE_NL_FORM) {
-    if (value < 0 || value > 2) return PIB_ERR_ARG;
-    return dispatch(ctx, [&](auto& I) {
-      HIPCHK(hipStreamSynchronize(I.stream));
-      I.drop_graphs();
-      I.nl_form = value;
     });
   }
   if (key == PIB_TUNE_GRAPH) {

@@ -132,51 +132,90 @@ __device__ inline void block_sums(double* v, double* sh) {
   for (int q = 0; q < K; ++q) v[q] = sh[q * bd];
   __syncthreads();
 }
-// Block sums for blocks whose size is a multiple of 64 (at most 1024):
-// xor-butterfly within each wavefront (every lane ends with the same,
-// order-independent-of-lane result), then the wave totals in wave order.
-// sh holds K * 16 doubles.
+// Block sums for blocks whose size is a multiple of 64 (at most 1024).  Each
+// wavefront reduces its 64 values with DPP lane moves (VALU, no LDS
+// traffic): within each quad (xor 1, xor 2), then across the quads of each
+// 16-lane row (rotate 4, rotate 8); the four row totals are read from lanes
+// 0, 16, 32, 48 and added as (r0 + r1) + (r2 + r3).  The (at most 16) wave
+// totals of value q then sit in one 16-lane row and are added the same way.
+// Every association is fixed, so results repeat bit for bit.  sh holds
+// K * 16 doubles.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+// DPP controls: quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_ror:4, row_ror:8
+#define PIB_DPP_QXOR1 0xB1
+#define PIB_DPP_QXOR2 0x4E
+#define PIB_DPP_ROR4 0x124
+#define PIB_DPP_ROR8 0x128
+// sum / max over each 16-lane row; every lane of the row gets a total, lane 0
+// of the row the one with the documented association
+__device__ __forceinline__ double row_sum(double v) {
+  v = v + dpp_move<PIB_DPP_QXOR1>(v);
+  v = v + dpp_move<PIB_DPP_QXOR2>(v);
+  v = v + dpp_move<PIB_DPP_ROR4>(v);
+  return v + dpp_move<PIB_DPP_ROR8>(v);
+}
+__device__ __forceinline__ double row_max(double v) {
+  v = fmax(v, dpp_move<PIB_DPP_QXOR1>(v));
+  v = fmax(v, dpp_move<PIB_DPP_QXOR2>(v));
+  v = fmax(v, dpp_move<PIB_DPP_ROR4>(v));
+  return fmax(v, dpp_move<PIB_DPP_ROR8>(v));
+}
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v = row_sum(v);
+  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
 }
 __device__ inline double wave_max(double v) {
+  v = row_max(v);
+  return fmax(fmax(lane_value(v, 0), lane_value(v, 16)), fmax(lane_value(v, 32), lane_value(v, 48)));
+}
+template <int K, bool MAX>
+__device__ inline void block_reduce_w(double* v, double* sh) {
+  const int t = threadIdx.x, nw = blockDim.x >> 6;
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
+  for (int q = 0; q < K; ++q) v[q] = MAX ? wave_max(v[q]) : wave_sum(v[q]);
+  if (nw == 1) return;
+  if ((t & 63) == 0)
+#pragma unroll
+    for (int q = 0; q < K; ++q) sh[q * 16 + (t >> 6)] = v[q];
+  __syncthreads();
+  // slot idx = q*16 + w: row q of 16 lanes holds the wave totals of value q
+  constexpr int NCH = (K * 16 + 63) / 64;
+  double r[NCH];
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+    const int idx = ch * (int)blockDim.x + t;
+    double x = MAX ? -__builtin_huge_val() : 0.0;
+    if (idx < K * 16 && (idx & 15) < nw) x = sh[idx];
+    r[ch] = MAX ? row_max(x) : row_sum(x);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+    const int idx = ch * (int)blockDim.x + t;
+    if (idx < K * 16 && (idx & 15) == 0) sh[idx >> 4] = r[ch];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < K; ++q) v[q] = sh[q];
+  __syncthreads();
 }
 template <int K>
 __device__ inline void block_sums_w(double* v, double* sh) {
-  const int t = threadIdx.x, nw = blockDim.x >> 6;
-#pragma unroll
-  for (int q = 0; q < K; ++q) v[q] = wave_sum(v[q]);
-  if ((t & 63) == 0)
-    for (int q = 0; q < K; ++q) sh[q * 16 + (t >> 6)] = v[q];
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < K; ++q) {
-    double a = 0.0;
-    for (int w = 0; w < nw; ++w) a += sh[q * 16 + w];
-    v[q] = a;
-  }
-  __syncthreads();
+  block_reduce_w<K, false>(v, sh);
 }
 template <int K>
 __device__ inline void block_maxes_w(double* v, double* sh) {
-  const int t = threadIdx.x, nw = blockDim.x >> 6;
-#pragma unroll
-  for (int q = 0; q < K; ++q) v[q] = wave_max(v[q]);
-  if ((t & 63) == 0)
-    for (int q = 0; q < K; ++q) sh[q * 16 + (t >> 6)] = v[q];
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < K; ++q) {
-    double a = sh[q * 16];
-    for (int w = 1; w < nw; ++w) a = fmax(a, sh[q * 16 + w]);
-    v[q] = a;
-  }
-  __syncthreads();
+  block_reduce_w<K, true>(v, sh);
 }
 // deterministic max over any block size
 __device__ inline double block_max_any(double v, double* sh) {
@@ -315,13 +354,12 @@ __device__ __forceinline__ void c2r_plane(const KArgs<T>& A, unsigned char* smem
     OpUbcp<T> op(A, c, i);
     plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op);
     double* sh = (double*)smem;
-    double sf = block_sum(op.sf, sh);
-    double ss = block_sum(op.ss, sh);
-    double sa = block_sum(op.sa, sh);
-    double mx = block_max(op.mx, sh);
+    double sm[3] = {op.sf, op.ss, op.sa}, mx[1] = {op.mx};
+    block_sums_w<3>(sm, sh);
+    block_maxes_w<1>(mx, sh);
     if (threadIdx.x == 0) {
       double* o = A.part + ((size_t)c * A.nx + i) * 4;
-      o[0] = sf; o[1] = ss; o[2] = sa; o[3] = mx;
+      o[0] = sm[0]; o[1] = sm[1]; o[2] = sm[2]; o[3] = mx[0];
     }
   } else if constexpr (KIND == C2R_PBCP) {
     OpPlain<T> op{A.p, A.pbcp, A, (size_t)i * A.mz * A.my2, (size_t)i * A.mz * A.my, (T)0};
@@ -507,24 +545,13 @@ struct OpNlStage {
   __device__ void store(int, int ky, int kz, C2<T> v) { o.template store_at<ST>(ky, kz, v); }
 };
 
-// Lanes per form_nl workgroup for a compiled shape: at most 512, so that each
-// lane keeps up to 256 VGPRs for its accumulators.
-template <int SH>
-struct NlShape {
-  static constexpr int NT = Shape<SH>::NT < 512 ? Shape<SH>::NT : 512;
-  static constexpr int NIT = SH == 0 ? 0 : Plane2<Shape<SH>::HY, Shape<SH>::MZ, NT>::r2c_nout;
-  // the split form runs both parts at the shape's own workgroup size
-  static constexpr int NIT2 = SH == 0 ? 0 : Plane2<Shape<SH>::HY, Shape<SH>::MZ, Shape<SH>::NT>::r2c_nout;
-};
-
-// PART 0: the three transforms in one workgroup with register accumulators;
-// for a compiled shape, PART 1 writes the partial sum wy F(u_c v) + wz F(u_c w)
-// to nl and PART 2 adds F(D_c) to it and stores the negated total; PART 3
-// runs the three transforms in one workgroup, each updating nl in place
-// (the lane that writes an output element is the lane that updates it).
-// All forms add in the order ((wy F0 + wz F1) + F2).
-template <typename T, int NIT, bool UPWIND, bool MINMOD, int SH, int PART = 0>
-__global__ void __launch_bounds__(PART == 0 ? 512 : 1024) k_form_nl(KArgs<T> A) {
+// SH = 0 (generic plane): the three transforms in one workgroup with
+// register accumulators, stored negated at the end.  Compiled shape: the
+// three transforms in one workgroup, each updating nl in place (the lane
+// that writes an output element is the lane that updates it).  Both add in
+// the order ((wy F0 + wz F1) + F2).
+template <typename T, int NIT, bool UPWIND, bool MINMOD, int SH>
+__global__ void __launch_bounds__(SH == 0 ? 512 : 1024) k_form_nl(KArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
   C2<T>* buf = (C2<T>*)smem;
   const int i = plane_of(A);
@@ -546,53 +573,15 @@ __global__ void __launch_bounds__(PART == 0 ? 512 : 1024) k_form_nl(KArgs<T> A) 
       }
     }
   } else {
-    constexpr int HY = Shape<SH>::HY, MZ = Shape<SH>::MZ;
-    constexpr int NT = PART == 0 ? NlShape<SH>::NT : Shape<SH>::NT;
-    using P = Plane2<HY, MZ, NT>;
+    constexpr int HY = Shape<SH>::HY, MZ = Shape<SH>::MZ, NT = Shape<SH>::NT;
     op.nyq = (C2<T>*)(smem + A.G.tab_off + tab_lds_bytes<T>(HY, MZ));
-    if constexpr (PART == 2) {
-      op.stage = 2;
-      op.rmw = true;
-      plane_r2c2<HY, MZ, NT>(buf, A.G, A.tb, op);
-      return;
-    }
-    if constexpr (PART == 3) {
-      using O = OpNl<T, NIT, UPWIND, MINMOD>;
-      OpNlStage<0, O, T> o0{op};
-      plane_r2c2<HY, MZ, NT>(buf, A.G, A.tb, o0);
-      OpNlStage<1, O, T> o1{op};
-      plane_r2c2<HY, MZ, NT>(buf, A.G, A.tb, o1);
-      OpNlStage<2, O, T> o2{op};
-      plane_r2c2<HY, MZ, NT>(buf, A.G, A.tb, o2);
-      return;
-    }
-    for (int s = 0; s < (PART == 1 ? 2 : 3); ++s) {
-      op.stage = s;
-      plane_r2c2<HY, MZ, NT>(buf, A.G, A.tb, op);
-    }
-    const T sg = PART == 1 ? (T)1 : (T)-1;
-    // the lane's outputs: items (ky, c) of the last z stage, kz = c + ZA d
-    const int t = threadIdx.x;
-#pragma unroll
-    for (int j = 0; j < cdiv(HY * P::ZA, NT); ++j) {
-      const int e = t + j * NT;
-      if (e < HY * P::ZA) {
-        const int ky = e % HY, cc = e / HY;
-#pragma unroll
-        for (int d = 0; d < P::ZB; ++d) {
-          C2<T> v = op.acc[j * P::ZB + d];
-          A.nl[sidx(A, c, i, ky, cc + P::ZA * d)] = mk(sg * v.x, sg * v.y);
-        }
-      }
-    }
-    if (t < P::ZA) {
-#pragma unroll
-      for (int d = 0; d < P::ZB; ++d) {
-        const int kz = t + P::ZA * d;
-        C2<T> v = op.nyq[kz];
-        A.nl[sidx(A, c, i, HY, kz)] = mk(sg * v.x, sg * v.y);
-      }
-    }
+    using O = OpNl<T, NIT, UPWIND, MINMOD>;
+    OpNlStage<0, O, T> o0{op};
+    plane_r2c2<HY, MZ, NT>(buf, A.G, A.tb, o0);
+    OpNlStage<1, O, T> o1{op};
+    plane_r2c2<HY, MZ, NT>(buf, A.G, A.tb, o1);
+    OpNlStage<2, O, T> o2{op};
+    plane_r2c2<HY, MZ, NT>(buf, A.G, A.tb, o2);
   }
 }
 

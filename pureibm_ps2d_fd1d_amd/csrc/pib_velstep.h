@@ -498,6 +498,8 @@ __global__ void __launch_bounds__(NT, NT <= 256 ? 3 : 1) k_velstep_tiled(KArgs<T
   else vel_tile<T, T, L, M>(A, (int)blockIdx.x * M, false, vlds, dmax, st);
   if (st) *status = st;
   __syncthreads();
-  double mx = block_max_any(dmax, (double*)vlds);
-  if (threadIdx.x == 0) A.part[blockIdx.x] = mx;
+  double mx[1] = {dmax};
+  if (blockDim.x % 64 == 0) block_maxes_w<1>(mx, (double*)vlds);
+  else mx[0] = block_max_any(dmax, (double*)vlds);
+  if (threadIdx.x == 0) A.part[blockIdx.x] = mx[0];
 }
