@@ -206,6 +206,10 @@ int pib_kernel_times(void* ctx, int maxn, char* names, double* total_ms, int* la
  *   PIB_TUNE_FFT: 1 = plane transforms of a compiled shape (my/2, mz) in
  *     {(64,128), (32,64), (16,32)} use the two-stage kernels (default); 0 =
  *     always the generic mixed-radix transform.
+ *   PIB_TUNE_STREAMS: 1 = with particles, the form_nl branch of a stage
+ *     (form_nl and its transforms) runs on a second stream beside the bcset
+ *     branch up to the pressure gradient, joined before the forcing
+ *     (default); 0 = one stream.  Results are the same either way.
  *   PIB_TUNE_GRAPH: 1 = pib_step replays whole steps from HIP graphs (default;
  *     the last step of each call and steps under pib_set_timing run eagerly);
  *     0 = every step is launched kernel by kernel.
@@ -213,6 +217,7 @@ int pib_kernel_times(void* ctx, int maxn, char* names, double* total_ms, int* la
 #define PIB_TUNE_VELSTEP 0
 #define PIB_TUNE_GRAPH 1
 #define PIB_TUNE_FFT 2
+#define PIB_TUNE_STREAMS 4
 int pib_set_tuning(void* ctx, int key, int value);
 
 #ifdef __cplusplus

@@ -24,6 +24,7 @@ PIB_ERR_PIVOT = -6
 
 PIB_TUNE_VELSTEP = 0
 PIB_TUNE_GRAPH = 1
+PIB_TUNE_STREAMS = 4
 PIB_TUNE_FFT = 2
 
 PIB_FP64 = 0

@@ -609,6 +609,12 @@ struct Impl : Base {
   int nx, my, mz, my2, hy, itrmax;
   double coef[3][4];
   hipStream_t stream = nullptr;
+  // second stream for the form_nl branch of a stage, forked from and joined
+  // back into `stream` with the two events
+  hipStream_t stream2 = nullptr;
+  hipEvent_t evfork = nullptr, evjoin = nullptr;
+  bool use_fork = true;
+  bool forked = false;
   PlaneGeom G;
   PlaneTabs<T> tb;
   int nt_plane = 256;   // threads per plane workgroup
@@ -707,6 +713,9 @@ struct Impl : Base {
       itrmax = 1;
     }
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&evfork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&evjoin, hipEventDisableTiming));
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     // plane FFT geometry
     G.my = my;
@@ -810,6 +819,10 @@ struct Impl : Base {
     for (void* q : allocs) (void)hipFree(q);
     for (auto& e : ev) (void)hipEventDestroy(e);
     for (auto& e : evpool) (void)hipEventDestroy(e);
+    if (stream2) (void)hipStreamSynchronize(stream2);
+    if (evfork) (void)hipEventDestroy(evfork);
+    if (evjoin) (void)hipEventDestroy(evjoin);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
   }
   void fill_mask() {
@@ -1154,18 +1167,7 @@ struct Impl : Base {
     HIPCHK(hipGetLastError());
     ke();
   }
-  void nonlinear(int rks) {
-    // onl <- nl (nl.F90:99-108): the buffers swap roles; the physical copy of
-    // onl is the previous nlbc when that one is current
-    std::swap(nl, onl);
-    std::swap(nlbc, onlbc);
-    bool onlbc_ok = nlbc_valid;
-    form_nl(rks);
-    c2r<C2R_NLBC>(rks, 3);
-    if (!onlbc_ok) c2r<C2R_ONLBC>(rks, 3);
-    nlbc_valid = true;
-    if (nbody > 0) bcset(rks);
-  }
+  void nonlinear(int rks) { nonlinear_timed(rks, false); }
   void bcset(int rks) {
     Particles P = parts();
     size_t npts = (size_t)nbody * nbnd;
@@ -1181,6 +1183,7 @@ struct Impl : Base {
     HIPCHK(hipGetLastError());
     ke();
     c2r<C2R_PGRAD>(rks, 3);
+    join_branches();
     if (cfg.dorpr) {
       kb("forcing_points");
       k_forcing_points<T><<<nbg, 256, 0, stream>>>(args(rks), P, frc_pts, frc_part, G);
@@ -1339,16 +1342,38 @@ struct Impl : Base {
     }
     timed = nsteps > 0;
   }
+  // nonlinear + bcset of one stage.  onl <- nl (nl.F90:99-108): the buffers
+  // swap roles; the physical copy of onl is the previous nlbc when that one
+  // is current.  With particles, the form_nl branch (form_nl and the nlbc
+  // transforms) runs on stream2 beside the bcset branch up to the pressure
+  // gradient (visc/omega transforms, drag, drag scalars, pgrad transform):
+  // the two touch disjoint buffers and only the latter reads or writes the
+  // device state.  The branches join before the forcing.
   void nonlinear_timed(int rks, bool rec) {
     std::swap(nl, onl);
     std::swap(nlbc, onlbc);
     bool onlbc_ok = nlbc_valid;
+    forked = use_fork && nbody > 0;
+    if (forked) {
+      HIPCHK(hipEventRecord(evfork, stream));
+      HIPCHK(hipStreamWaitEvent(stream2, evfork, 0));
+      std::swap(stream, stream2);
+    }
     form_nl(rks);
     c2r<C2R_NLBC>(rks, 3);
     if (!onlbc_ok) c2r<C2R_ONLBC>(rks, 3);
+    if (forked) {
+      std::swap(stream, stream2);
+      HIPCHK(hipEventRecord(evjoin, stream2));
+    }
     nlbc_valid = true;
     if (rec) HIPCHK(hipEventRecord(ev[2], stream));
     if (nbody > 0) bcset(rks);
+  }
+  void join_branches() {
+    if (!forked) return;
+    HIPCHK(hipStreamWaitEvent(stream, evjoin, 0));
+    forked = false;
   }
   void set_timing(int on) {
     HIPCHK(hipStreamSynchronize(stream));
@@ -1747,6 +1772,14 @@ int pib_set_tuning(void* ctx, int key, int value) {
       HIPCHK(hipStreamSynchronize(I.stream));
       I.drop_graphs();
       I.fft_compiled = value == 1;
+    });
+  }
+  if (key == PIB_TUNE_STREAMS) {
+    if (value < 0 || value > 1) return PIB_ERR_ARG;
+    return dispatch(ctx, [&](auto& I) {
+      HIPCHK(hipStreamSynchronize(I.stream));
+      I.drop_graphs();
+      I.use_fork = value == 1;
     });
   }
   if (key == PIB_TUNE_GRAPH) {

@@ -35,3 +35,24 @@ def test_graph_replays_equal_eager_steps(tmp_path, prec, rk3):
         assert list(se.mpg) == list(sg.mpg)
     ge.close()
     gg.close()
+
+
+@pytest.mark.parametrize("prec", [_abi.PIB_FP64, _abi.PIB_FP32_BULK])
+def test_forked_branches_equal_one_stream(tmp_path, prec):
+    """The form_nl branch on its own stream gives the same bits as one stream,
+    launched eagerly and replayed from graphs."""
+    print(f"forked branches prec={prec}", flush=True)
+    kw = dict(kind="random", my=24, dbydx=6, nbody=3, seed=11, re=30.0)
+    st = make_setup(tmp_path, "FK", precision=prec, **kw)
+    g1 = solver.Solver(st)
+    g2 = solver.Solver(st)
+    assert g1.lib.pib_set_tuning(g1.ctx, _abi.PIB_TUNE_STREAMS, 0) == 0
+    assert g2.lib.pib_set_tuning(g2.ctx, _abi.PIB_TUNE_STREAMS, 1) == 0
+    for batch in (1, 4, 3):
+        g1.step(batch)
+        g2.step(batch)
+        for f in ("u", "p", "nl", "nlbcp", "fr", "ppr"):
+            assert np.array_equal(g1.field(f), g2.field(f)), (batch, f)
+        assert g1.scalars().norm_drag == g2.scalars().norm_drag
+    g1.close()
+    g2.close()
