@@ -92,6 +92,96 @@ def run_gpu(setup, steps, warmup, ksteps):
     return res
 
 
+# bench kernel names -> the FP32 kernel symbols rocprofv3 reports for them
+KERNEL_SYMBOLS = {
+    "form_nl": "k_form_nl<float", "velstep": "k_velstep_tiled<float", "forcing_points": "k_forcing_points<float",
+    "spread": "k_spread<float", "c2r_visc_omega": "k_c2r_pair<float, 4,", "c2r_ubcp_pbcp": "k_c2r_pair<float, 0,",
+    "c2r_pgrad": "k_c2r<float, 6,", "c2r_nlbc": "k_c2r<float, 2,", "r2c_force": "k_r2c_force<float",
+    "drag_points": "k_drag_points<float",
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_fluid_step_128cube_fp32.md")
+COPY_MIB = 256
+
+
+def _pmc_pass(counter, args, outdir):
+    """One rocprofv3 counter pass over a child bench process; returns
+    {kernel symbol: [values per dispatch]}."""
+    import csv
+    import subprocess
+    env = dict(os.environ, TMPDIR="/tmp")
+    cmd = ["timeout", "-s", "KILL", "90", "rocprofv3", "--pmc", counter, "-f", "csv", "-d", outdir, "-o", "p",
+           "--", sys.executable, os.path.abspath(__file__)] + args
+    print("+", " ".join(cmd), file=sys.stderr, flush=True)
+    subprocess.run(cmd, cwd="/tmp", env=env, check=True, stdout=subprocess.DEVNULL)
+    vals = {}
+    for root, _, files in os.walk(outdir):
+        for f in files:
+            if f.endswith("counter_collection.csv"):
+                for r in csv.DictReader(open(os.path.join(root, f))):
+                    if r["Counter_Name"] == counter:
+                        vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    return vals
+
+
+def collect_traffic():
+    """HBM bytes per dispatch of the FP32 step kernels: FETCH_SIZE and
+    WRITE_SIZE passes (KiB per dispatch) over a short FP32 run, each scaled by
+    what the counter reads for plain device-to-device copies of known size.
+    Writes TRAFFIC_FILE.  Runs the GPU only in child processes."""
+    work = tempfile.mkdtemp(prefix="pib_pmc_")
+    run_args = ["--only", "fp32", "--steps", "3", "--warmup", "2", "--kernel-steps", "1", "--no-cpu"]
+    copy_args = ["--calib-copy", str(COPY_MIB)]
+    fetch = _pmc_pass("FETCH_SIZE", run_args, os.path.join(work, "f"))
+    write = _pmc_pass("WRITE_SIZE", run_args, os.path.join(work, "w"))
+    cf = _pmc_pass("FETCH_SIZE", copy_args, os.path.join(work, "cf"))
+    cw = _pmc_pass("WRITE_SIZE", copy_args, os.path.join(work, "cw"))
+    copy_kib = COPY_MIB * 1024.0
+
+    def copy_scale(v):
+        # the copy kernels are the only dispatches of that run above 1 MiB
+        big = [x for xs in v.values() for x in xs if x > 1024.0]
+        return copy_kib / (sum(big) / len(big))
+
+    ff, wf = copy_scale(cf), copy_scale(cw)
+    table = {}
+    for name, sym in KERNEL_SYMBOLS.items():
+        fk = [x for k, xs in fetch.items() if sym in k for x in xs]
+        wk = [x for k, xs in write.items() if sym in k for x in xs]
+        if fk and wk:
+            f_kib, w_kib = sum(fk) / len(fk), sum(wk) / len(wk)
+            table[name] = {"fetch_kib": f_kib, "write_kib": w_kib,
+                           "hbm_bytes": f_kib * 1024 * ff + w_kib * 1024 * wf, "dispatches": len(fk)}
+    cmd = "cd /tmp && export TMPDIR=/tmp && python3 /root/repo/bench.py --collect-traffic"
+    lines = ["# HBM traffic per dispatch -- FP32 fluid step kernels, 128^3 bed, 195 spheres", "",
+             "Command (one MI355X, gfx950):", "", "```", cmd, "```", "",
+             "It runs, as child processes, `rocprofv3 --pmc FETCH_SIZE` and then `--pmc WRITE_SIZE` over",
+             "`bench.py " + " ".join(run_args) + "`, and the same two counters over `bench.py --calib-copy "
+             + str(COPY_MIB) + "` (three device-to-device copies of " + str(COPY_MIB) + " MiB) to scale them:",
+             f"FETCH_SIZE factor {ff:.3f}, WRITE_SIZE factor {wf:.3f} (copy bytes / counter bytes).", "",
+             "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes per dispatch (scaled) |",
+             "|---|---:|---:|---:|---:|"]
+    for name, r in sorted(table.items(), key=lambda kv: -kv[1]["hbm_bytes"]):
+        lines.append(f"| {name} | {r['dispatches']} | {r['fetch_kib']:.0f} | {r['write_kib']:.0f} | "
+                     f"{r['hbm_bytes']:.4g} |")
+    lines += ["", "Machine-readable copy (read by bench.py for `roofline.traffic`):", "", "```json",
+              json.dumps({"fetch_factor": ff, "write_factor": wf, "kernels": table}), "```", ""]
+    os.makedirs(os.path.dirname(TRAFFIC_FILE), exist_ok=True)
+    with open(TRAFFIC_FILE, "w") as f:
+        f.write("\n".join(lines))
+    print(json.dumps({"traffic_file": TRAFFIC_FILE, "fetch_factor": ff, "write_factor": wf,
+                      "kernels": {k: v["hbm_bytes"] for k, v in table.items()}}))
+
+
+def traffic_of(kernel):
+    """HBM bytes per dispatch of an FP32 kernel from TRAFFIC_FILE, or None."""
+    try:
+        txt = open(TRAFFIC_FILE).read()
+        blob = txt.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(blob)["kernels"][kernel]["hbm_bytes"]
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(setup, nsteps):
     """Single-core oracle on the same workload (bounded sample: one untimed
     first step, then nsteps timed steps)."""
@@ -119,7 +209,12 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=20, help="steps timed per launch for the roofline")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (scale for FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--collect-traffic", action="store_true",
+                    help="measure HBM bytes per dispatch with rocprofv3 counter passes (child processes)")
     a = ap.parse_args()
+    if a.collect_traffic:
+        collect_traffic()
+        return
     if a.calib_copy:
         import torch
         n = a.calib_copy * (1 << 20) // 4
@@ -158,7 +253,8 @@ def main():
     passes = PASSES.get(dom)
     achieved = (passes * w * N / avg_s / 1e9) if passes else None
     roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+            "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+            "traffic": traffic_of(dom) if w == 4 else None,
             "algorithmic_bytes_per_launch": passes * w * N if passes else None,
             "avg_launch_us": avg_s * 1e6, "launches": launches}
     step_bytes = w * N * (70 + 22 * s)

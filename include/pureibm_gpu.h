@@ -208,8 +208,9 @@ int pib_kernel_times(void* ctx, int maxn, char* names, double* total_ms, int* la
  *     always the generic mixed-radix transform.
  *   PIB_TUNE_STREAMS: 1 = with particles, the form_nl branch of a stage
  *     (form_nl and its transforms) runs on a second stream beside the bcset
- *     branch up to the pressure gradient, joined before the forcing
- *     (default); 0 = one stream.  Results are the same either way.
+ *     branch up to the pressure gradient, joined before the forcing (default
+ *     for FP64); 0 = one stream (default for FP32-bulk).  Steps under
+ *     pib_set_timing use one stream.  Results are the same either way.
  *   PIB_TUNE_GRAPH: 1 = pib_step replays whole steps from HIP graphs (default;
  *     the last step of each call and steps under pib_set_timing run eagerly);
  *     0 = every step is launched kernel by kernel.

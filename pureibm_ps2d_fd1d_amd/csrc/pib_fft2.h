@@ -112,16 +112,37 @@ __device__ __forceinline__ C2<T> twmul(C2<T> v, const C2<T>* tw, int j) {
   return cmul(v, w);
 }
 
+// LDS layout of the plane: row z holds the HY + 1 entries of ky (or y pair)
+// at slots ys(y).  For HY = 64 the rows are padded: one empty slot after
+// every 8 entries and the ky = HY entry last (slot 71), with rows 72 slots
+// apart.  Then the strided y-stage accesses (stride 8 within a row, two rows
+// per 16 lanes) fall in distinct banks.  The z stages walk the rows by slot
+// (W slots per row, empty slots idle) so that their 16-lane groups stay on
+// consecutive slots too.
 template <int HY, int MZ, int NT>
 struct Plane2 {
-  static constexpr int MY2 = HY + 1, LD = HY + 1;
+  static constexpr bool PAD = HY == 64;
+  static constexpr int MY2 = HY + 1, LD = PAD ? HY + HY / 8 : HY + 1, W = LD;
+  __device__ static __forceinline__ int ys(int y) {
+    if constexpr (PAD) return y < HY ? y + (y >> 3) : HY + HY / 8 - 1;
+    else return y;
+  }
+  // the ky of slot s, or -1 for an empty slot
+  __device__ static __forceinline__ int yof(int s) {
+    if constexpr (PAD) {
+      if (s == HY + HY / 8 - 1) return HY;
+      return s % 9 == 8 || s >= HY + HY / 8 - 1 ? -1 : s - s / 9;
+    } else {
+      return s;
+    }
+  }
   static constexpr int ZA = Split2<MZ>::A, ZB = Split2<MZ>::B;
   static constexpr int YA = Split2<HY>::A, YB = Split2<HY>::B;
   // outputs per lane of the last phase (the Op's `it` bound)
   static constexpr int c2r_nout = cdiv(MZ * YA, NT) * YB;  // y DIF-B items (z, c), YB outputs each
-  // z DIF-B items (ky < HY, c), ZB outputs each; the ky = HY column's items
-  // go to lanes c < ZA with it = -1
-  static constexpr int r2c_nout = cdiv(HY * ZA, NT) * ZB;
+  // z DIF-B items (slot of ky < HY, c), ZB outputs each; the ky = HY
+  // column's items go to lanes c < ZA with it = -1
+  static constexpr int r2c_nout = cdiv(W * ZA, NT) * ZB;
 };
 
 // ---------------------------------------------------------------------------
@@ -131,37 +152,37 @@ struct Plane2 {
 template <int HY, int MZ, int NT, typename T, class Op>
 __device__ void plane_c2r2(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& gtb, Op& op) {
   using P = Plane2<HY, MZ, NT>;
-  constexpr int MY2 = P::MY2, LD = P::LD, ZA = P::ZA, ZB = P::ZB, YA = P::YA, YB = P::YB;
+  constexpr int LD = P::LD, W = P::W, ZA = P::ZA, ZB = P::ZB, YA = P::YA, YB = P::YB;
   const int t = threadIdx.x;
   const PlaneTabs<T> tb = stage_tabs(G, gtb, (unsigned char*)buf + G.tab_off);
   __syncthreads();  // the first stage already multiplies by twiddles
-  // z DIT-A: item (ky, n1), ky fastest; reads HBM
+  // z DIT-A: item (slot s, n1), s fastest; reads HBM
 #pragma unroll
-  for (int j = 0; j < cdiv(MY2 * ZA, NT); ++j) {
+  for (int j = 0; j < cdiv(W * ZA, NT); ++j) {
     const int e = t + j * NT;
-    if (e < MY2 * ZA) {
-      const int ky = e % MY2, n1 = e / MY2;
+    const int s = e % W, n1 = e / W, ky = P::yof(s);
+    if (e < W * ZA && ky >= 0) {
       C2<T> v[ZB];
 #pragma unroll
       for (int n2 = 0; n2 < ZB; ++n2) v[n2] = op.load(ky, n1 + ZA * n2);
       dft2n<true, ZB>(v);
 #pragma unroll
-      for (int k2 = 0; k2 < ZB; ++k2) buf[(n1 * ZB + k2) * LD + ky] = twmul<true>(v[k2], tb.twz, n1 * k2);
+      for (int k2 = 0; k2 < ZB; ++k2) buf[(n1 * ZB + k2) * LD + s] = twmul<true>(v[k2], tb.twz, n1 * k2);
     }
   }
   __syncthreads();
-  // z DIT-B: item (ky, k2), in place
+  // z DIT-B: item (slot s, k2), in place
 #pragma unroll
-  for (int j = 0; j < cdiv(MY2 * ZB, NT); ++j) {
+  for (int j = 0; j < cdiv(W * ZB, NT); ++j) {
     const int e = t + j * NT;
-    if (e < MY2 * ZB) {
-      const int ky = e % MY2, k2 = e / MY2;
+    const int s = e % W, k2 = e / W;
+    if (e < W * ZB && P::yof(s) >= 0) {
       C2<T> v[ZA];
 #pragma unroll
-      for (int n1 = 0; n1 < ZA; ++n1) v[n1] = buf[(n1 * ZB + k2) * LD + ky];
+      for (int n1 = 0; n1 < ZA; ++n1) v[n1] = buf[(n1 * ZB + k2) * LD + s];
       dft2n<true, ZA>(v);
 #pragma unroll
-      for (int k1 = 0; k1 < ZA; ++k1) buf[(k2 + ZB * k1) * LD + ky] = v[k1];
+      for (int k1 = 0; k1 < ZA; ++k1) buf[(k2 + ZB * k1) * LD + s] = v[k1];
     }
   }
   __syncthreads();
@@ -175,18 +196,18 @@ __device__ void plane_c2r2(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& g
       const int k = e % NP, z = e / NP;
       C2<T>* row = buf + z * LD;
       if (k == 0) {
-        T x0 = row[0].x, xh = row[HY].x;
+        T x0 = row[0].x, xh = row[P::ys(HY)].x;
         row[0] = mk(x0 + xh, x0 - xh);
       } else if (2 * k == HY) {
-        C2<T> a = row[k];
-        row[k] = mk(a.x + a.x, -(a.y + a.y));
+        C2<T> a = row[P::ys(k)];
+        row[P::ys(k)] = mk(a.x + a.x, -(a.y + a.y));
       } else {
-        C2<T> a = row[k], b = row[HY - k];
+        C2<T> a = row[P::ys(k)], b = row[P::ys(HY - k)];
         C2<T> A = mk(a.x + b.x, a.y - b.y);
         C2<T> D = mk(a.x - b.x, a.y + b.y);
         C2<T> B = cmul(D, cconj(tb.wr[k]));
-        row[k] = mk(A.x - B.y, A.y + B.x);
-        row[HY - k] = mk(A.x + B.y, B.x - A.y);
+        row[P::ys(k)] = mk(A.x - B.y, A.y + B.x);
+        row[P::ys(HY - k)] = mk(A.x + B.y, B.x - A.y);
       }
     }
   }
@@ -200,10 +221,10 @@ __device__ void plane_c2r2(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& g
       C2<T>* row = buf + z * LD;
       C2<T> v[YA];
 #pragma unroll
-      for (int a = 0; a < YA; ++a) v[a] = row[YB * a + b];
+      for (int a = 0; a < YA; ++a) v[a] = row[P::ys(YB * a + b)];
       dft2n<true, YA>(v);
 #pragma unroll
-      for (int c = 0; c < YA; ++c) row[YB * c + b] = twmul<true>(v[c], tb.twy, b * c);
+      for (int c = 0; c < YA; ++c) row[P::ys(YB * c + b)] = twmul<true>(v[c], tb.twy, b * c);
     }
   }
   __syncthreads();
@@ -216,7 +237,7 @@ __device__ void plane_c2r2(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& g
       const C2<T>* row = buf + z * LD;
       C2<T> v[YB];
 #pragma unroll
-      for (int b = 0; b < YB; ++b) v[b] = row[YB * c + b];
+      for (int b = 0; b < YB; ++b) v[b] = row[P::ys(YB * c + b)];
       dft2n<true, YB>(v);
 #pragma unroll
       for (int d = 0; d < YB; ++d) op.store2(j * YB + d, 2 * (c + YA * d), z, v[d].x, v[d].y);
@@ -231,7 +252,7 @@ __device__ void plane_c2r2(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& g
 template <int HY, int MZ, int NT, typename T, class Op>
 __device__ void plane_r2c2(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& gtb, Op& op) {
   using P = Plane2<HY, MZ, NT>;
-  constexpr int MY2 = P::MY2, LD = P::LD, ZA = P::ZA, ZB = P::ZB, YA = P::YA, YB = P::YB;
+  constexpr int LD = P::LD, W = P::W, ZA = P::ZA, ZB = P::ZB, YA = P::YA, YB = P::YB;
   const int t = threadIdx.x;
   const PlaneTabs<T> tb = stage_tabs(G, gtb, (unsigned char*)buf + G.tab_off);
   __syncthreads();  // the first stage already multiplies by twiddles
@@ -247,7 +268,7 @@ __device__ void plane_r2c2(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& g
       dft2n<false, YB>(v);
       C2<T>* row = buf + z * LD;
 #pragma unroll
-      for (int k2 = 0; k2 < YB; ++k2) row[n1 * YB + k2] = twmul<false>(v[k2], tb.twy, n1 * k2);
+      for (int k2 = 0; k2 < YB; ++k2) row[P::ys(n1 * YB + k2)] = twmul<false>(v[k2], tb.twy, n1 * k2);
     }
   }
   __syncthreads();
@@ -260,10 +281,10 @@ __device__ void plane_r2c2(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& g
       C2<T>* row = buf + z * LD;
       C2<T> v[YA];
 #pragma unroll
-      for (int n1 = 0; n1 < YA; ++n1) v[n1] = row[n1 * YB + k2];
+      for (int n1 = 0; n1 < YA; ++n1) v[n1] = row[P::ys(n1 * YB + k2)];
       dft2n<false, YA>(v);
 #pragma unroll
-      for (int k1 = 0; k1 < YA; ++k1) row[k2 + YB * k1] = v[k1];
+      for (int k1 = 0; k1 < YA; ++k1) row[P::ys(k2 + YB * k1)] = v[k1];
     }
   }
   __syncthreads();
@@ -278,46 +299,46 @@ __device__ void plane_r2c2(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& g
       if (k == 0) {
         C2<T> z0 = row[0];
         row[0] = mk(z0.x + z0.y, (T)0);
-        row[HY] = mk(z0.x - z0.y, (T)0);
+        row[P::ys(HY)] = mk(z0.x - z0.y, (T)0);
       } else if (2 * k == HY) {
-        row[k] = cconj(row[k]);
+        row[P::ys(k)] = cconj(row[P::ys(k)]);
       } else {
-        C2<T> a = row[k], b = row[HY - k];
+        C2<T> a = row[P::ys(k)], b = row[P::ys(HY - k)];
         C2<T> E = mk((a.x + b.x) * (T)0.5, (a.y - b.y) * (T)0.5);
         C2<T> O = mk((a.y + b.y) * (T)0.5, (b.x - a.x) * (T)0.5);
         C2<T> wO = cmul(tb.wr[k], O);
-        row[k] = E + wO;
-        row[HY - k] = cconj(E - wO);
+        row[P::ys(k)] = E + wO;
+        row[P::ys(HY - k)] = cconj(E - wO);
       }
     }
   }
   __syncthreads();
-  // z DIF-A: item (ky, b), ky fastest; rows R2 a + b, in place
+  // z DIF-A: item (slot s, b), s fastest; rows R2 a + b, in place
 #pragma unroll
-  for (int j = 0; j < cdiv(MY2 * ZB, NT); ++j) {
+  for (int j = 0; j < cdiv(W * ZB, NT); ++j) {
     const int e = t + j * NT;
-    if (e < MY2 * ZB) {
-      const int ky = e % MY2, b = e / MY2;
+    const int s = e % W, b = e / W;
+    if (e < W * ZB && P::yof(s) >= 0) {
       C2<T> v[ZA];
 #pragma unroll
-      for (int a = 0; a < ZA; ++a) v[a] = buf[(ZB * a + b) * LD + ky];
+      for (int a = 0; a < ZA; ++a) v[a] = buf[(ZB * a + b) * LD + s];
       dft2n<false, ZA>(v);
 #pragma unroll
-      for (int c = 0; c < ZA; ++c) buf[(ZB * c + b) * LD + ky] = twmul<false>(v[c], tb.twz, b * c);
+      for (int c = 0; c < ZA; ++c) buf[(ZB * c + b) * LD + s] = twmul<false>(v[c], tb.twz, b * c);
     }
   }
   __syncthreads();
-  // z DIF-B: item (ky, c), outputs kz = c + ZA d; columns ky < HY first,
-  // then the ky = HY column on lanes c < ZA (it = -1)
+  // z DIF-B: item (slot s, c), outputs kz = c + ZA d; columns ky < HY
+  // first, then the ky = HY column on lanes c < ZA (it = -1)
   const T scale = (T)((double)(2 * HY) * (double)MZ);
 #pragma unroll
-  for (int j = 0; j < cdiv(HY * ZA, NT); ++j) {
+  for (int j = 0; j < cdiv(W * ZA, NT); ++j) {
     const int e = t + j * NT;
-    if (e < HY * ZA) {
-      const int ky = e % HY, c = e / HY;
+    const int s = e % W, c = e / W, ky = P::yof(s);
+    if (e < W * ZA && ky >= 0 && ky < HY) {
       C2<T> v[ZB];
 #pragma unroll
-      for (int b = 0; b < ZB; ++b) v[b] = buf[(ZB * c + b) * LD + ky];
+      for (int b = 0; b < ZB; ++b) v[b] = buf[(ZB * c + b) * LD + s];
       dft2n<false, ZB>(v);
 #pragma unroll
       for (int d = 0; d < ZB; ++d) op.store(j * ZB + d, ky, c + ZA * d, mk(v[d].x / scale, v[d].y / scale));
@@ -327,7 +348,7 @@ __device__ void plane_r2c2(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& g
     const int c = t;
     C2<T> v[ZB];
 #pragma unroll
-    for (int b = 0; b < ZB; ++b) v[b] = buf[(ZB * c + b) * LD + HY];
+    for (int b = 0; b < ZB; ++b) v[b] = buf[(ZB * c + b) * LD + P::ys(HY)];
     dft2n<false, ZB>(v);
 #pragma unroll
     for (int d = 0; d < ZB; ++d) op.store(-1, HY, c + ZA * d, mk(v[d].x / scale, v[d].y / scale));
@@ -359,6 +380,10 @@ static inline int shape_of(int hy, int mz) {
   if (hy == 32 && mz == 64) return 2;
   if (hy == 16 && mz == 32) return 3;
   return 0;
+}
+// LDS row stride (in complex entries) of a compiled shape's plane buffer
+static inline int shape_ld(int sh) {
+  return sh == 1 ? Plane2<64, 128, 1024>::LD : (sh == 2 ? Plane2<32, 64, 256>::LD : (sh == 3 ? Plane2<16, 32, 128>::LD : 0));
 }
 static inline int shape_nt(int sh) {
   return sh == 1 ? Shape<1>::NT : (sh == 2 ? Shape<2>::NT : (sh == 3 ? Shape<3>::NT : 0));

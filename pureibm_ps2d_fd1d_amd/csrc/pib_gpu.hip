@@ -613,7 +613,10 @@ struct Impl : Base {
   // back into `stream` with the two events
   hipStream_t stream2 = nullptr;
   hipEvent_t evfork = nullptr, evjoin = nullptr;
-  bool use_fork = true;
+  // forked by default where the transforms hold one workgroup per CU (FP64
+  // planes fill the LDS), so that the two branches share the CUs; per-launch
+  // timing always runs one stream so that each kernel's time is its own
+  bool use_fork = sizeof(T) == 8;
   bool forked = false;
   PlaneGeom G;
   PlaneTabs<T> tb;
@@ -725,10 +728,11 @@ struct Impl : Base {
     G.ld = my2;
     if (!make_plan(hy, G.py) || !make_plan(mz, G.pz))
       throw PibFail(PIB_ERR_UNSUPPORTED, "my/2 and mz must factor into 2, 3, 5");
-    lds_plane = (size_t)mz * G.ld * sizeof(C2<T>);
     size_t E = (size_t)my2 * mz;
     nt_plane = E <= 256 ? 64 : (E <= 2500 ? 256 : (E <= 5000 ? 512 : 1024));
     sh = shape_of(hy, mz);
+    // rows of the generic transform, or of the compiled shape's (padded) layout
+    lds_plane = (size_t)mz * std::max(G.ld, shape_ld(sh)) * sizeof(C2<T>);
     size_t red = (size_t)std::max(nt_plane, shape_nt(sh)) * sizeof(double);
     if (lds_plane < red) lds_plane = red;
     lds_plane = (lds_plane + 15) / 16 * 16;
@@ -1154,7 +1158,7 @@ struct Impl : Base {
   }
   template <int SH>
   void launch_nl_sh(bool up, bool mm, int rks) {
-    constexpr int NT = Shape<SH>::NT;
+    constexpr int NT = NlOcc<T, SH>::NT;
     if (!up) run_nl(k_form_nl<T, 1, false, false, SH>, rks, NT);
     else if (!mm) run_nl(k_form_nl<T, 1, true, false, SH>, rks, NT);
     else run_nl(k_form_nl<T, 1, true, true, SH>, rks, NT);
@@ -1353,7 +1357,7 @@ struct Impl : Base {
     std::swap(nl, onl);
     std::swap(nlbc, onlbc);
     bool onlbc_ok = nlbc_valid;
-    forked = use_fork && nbody > 0;
+    forked = use_fork && !ktime && nbody > 0;
     if (forked) {
       HIPCHK(hipEventRecord(evfork, stream));
       HIPCHK(hipStreamWaitEvent(stream2, evfork, 0));

@@ -380,16 +380,23 @@ __device__ __forceinline__ void c2r_plane(const KArgs<T>& A, unsigned char* smem
     plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op);
   }
 }
+// Waves per SIMD the FP32 transforms of the 128^2 plane are compiled for:
+// 8 (64 VGPRs), so that two 1024-lane workgroups share a CU -- the padded
+// plane (74 KB) leaves room for two in the 160 KB LDS.  Others: no bound.
+template <typename T, int SH>
+struct PlaneOcc {
+  static constexpr int WPE = (SH == 1 && sizeof(T) == 4) ? 8 : 1;
+};
 // grid (planes, components of KIND)
 template <typename T, int KIND, int SH>
-__global__ void k_c2r(KArgs<T> A) {
+__global__ void __launch_bounds__(1024, (PlaneOcc<T, SH>::WPE)) k_c2r(KArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
   c2r_plane<T, KIND, SH>(A, smem, blockIdx.y, plane_of(A));
 }
 // two independent transforms in one launch: grid (planes, NA + NB), the
 // first NA rows of workgroups do KA, the rest KB
 template <typename T, int KA, int NA, int KB, int SH>
-__global__ void k_c2r_pair(KArgs<T> A) {
+__global__ void __launch_bounds__(1024, (PlaneOcc<T, SH>::WPE)) k_c2r_pair(KArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int y = blockIdx.y;
   if (y < NA) c2r_plane<T, KA, SH>(A, smem, y, plane_of(A));
@@ -412,7 +419,7 @@ struct OpForce {
   __device__ void store(int, int ky, int kz, C2<T> v) { A.ff[sidx(A, c, i, ky, kz)] = v; }
 };
 template <typename T, int SH>
-__global__ void k_r2c_force(KArgs<T> A) {
+__global__ void __launch_bounds__(1024, (PlaneOcc<T, SH>::WPE)) k_r2c_force(KArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
   C2<T>* buf = (C2<T>*)smem;
   const int i = plane_of(A);
@@ -550,8 +557,16 @@ struct OpNlStage {
 // three transforms in one workgroup, each updating nl in place (the lane
 // that writes an output element is the lane that updates it).  Both add in
 // the order ((wy F0 + wz F1) + F2).
+// Workgroup size and waves per SIMD of a compiled-shape form_nl: the FP32
+// 128^2 plane runs 512 lanes at 4 waves per SIMD (128 VGPRs), so that two
+// workgroups share a CU; others use the shape's lanes with no bound.
+template <typename T, int SH>
+struct NlOcc {
+  static constexpr int NT = (SH == 1 && sizeof(T) == 4) ? 512 : (SH == 0 ? 512 : Shape<SH>::NT);
+  static constexpr int WPE = (SH == 1 && sizeof(T) == 4) ? 4 : 1;
+};
 template <typename T, int NIT, bool UPWIND, bool MINMOD, int SH>
-__global__ void __launch_bounds__(SH == 0 ? 512 : 1024) k_form_nl(KArgs<T> A) {
+__global__ void __launch_bounds__((NlOcc<T, SH>::NT), (NlOcc<T, SH>::WPE)) k_form_nl(KArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
   C2<T>* buf = (C2<T>*)smem;
   const int i = plane_of(A);
@@ -573,7 +588,7 @@ __global__ void __launch_bounds__(SH == 0 ? 512 : 1024) k_form_nl(KArgs<T> A) {
       }
     }
   } else {
-    constexpr int HY = Shape<SH>::HY, MZ = Shape<SH>::MZ, NT = Shape<SH>::NT;
+    constexpr int HY = Shape<SH>::HY, MZ = Shape<SH>::MZ, NT = NlOcc<T, SH>::NT;
     op.nyq = (C2<T>*)(smem + A.G.tab_off + tab_lds_bytes<T>(HY, MZ));
     using O = OpNl<T, NIT, UPWIND, MINMOD>;
     OpNlStage<0, O, T> o0{op};
