@@ -85,9 +85,11 @@ __device__ __forceinline__ void toep_interior(const Toep<R, L>& t, C2<R>* d) {
 
 // Cyclic symmetric tridiagonal (al, be, al) of size P in place on r; z and g
 // are P-long work vectors.  Thomas on the Sherman-Morrison-modified matrix
-// with gamma = -be, then x = y - (v.y)/(1 + v.z) z.
+// with gamma = -be, then x = y - (v.y)/(1 + v.z) z.  Entry k of each vector
+// is at k*ld: the systems of a workgroup are interleaved, so that the lanes
+// solving neighbouring systems touch neighbouring LDS words at every step.
 template <typename R>
-__device__ void reduced_cyclic(C2<R>* r, R* z, R* g, int P, R al, R be, int& st) {
+__device__ void reduced_cyclic(C2<R>* r, R* z, R* g, int P, int ld, R al, R be, int& st) {
   if (P == 1) {
     R d = be + (R)2 * al;
     if (d == (R)0) {
@@ -103,10 +105,10 @@ __device__ void reduced_cyclic(C2<R>* r, R* z, R* g, int P, R al, R be, int& st)
       det = (R)1.0e-30;
       st = PIB_ERR_PIVOT;
     }
-    C2<R> r0 = r[0], r1 = r[1];
+    C2<R> r0 = r[0], r1 = r[ld];
     R id = (R)1 / det;
     r[0] = cscal(id, cscal(be, r0) - cscal((R)2 * al, r1));
-    r[1] = cscal(id, cscal(be, r1) - cscal((R)2 * al, r0));
+    r[ld] = cscal(id, cscal(be, r1) - cscal((R)2 * al, r0));
     return;
   }
   const R gm = -be;
@@ -130,28 +132,29 @@ __device__ void reduced_cyclic(C2<R>* r, R* z, R* g, int P, R al, R be, int& st)
       st = PIB_ERR_PIVOT;
     }
     ib = (R)1 / bet;
-    g[k] = gk;
-    y = cscal(ib, r[k] - cscal(al, y));
-    r[k] = y;
+    g[k * ld] = gk;
+    y = cscal(ib, r[k * ld] - cscal(al, y));
+    r[k * ld] = y;
     R uk = (k == P - 1) ? al : (R)0;
     zz = (uk - al * zz) * ib;
-    z[k] = zz;
+    z[k * ld] = zz;
   }
   for (int k = P - 2; k >= 0; --k) {
-    R gk1 = g[k + 1];
-    y = r[k] - cscal(gk1, y);
-    r[k] = y;
-    zz = z[k] - gk1 * zz;
-    z[k] = zz;
+    R gk1 = g[(k + 1) * ld];
+    y = r[k * ld] - cscal(gk1, y);
+    r[k * ld] = y;
+    zz = z[k * ld] - gk1 * zz;
+    z[k * ld] = zz;
   }
-  R den = (R)1 + z[0] + al * z[P - 1] / gm;
-  C2<R> f = cscal((R)1 / den, r[0] + cscal(al / gm, r[P - 1]));
-  for (int k = 0; k < P; ++k) r[k] = r[k] - cscal(z[k], f);
+  R den = (R)1 + z[0] + al * z[(P - 1) * ld] / gm;
+  C2<R> f = cscal((R)1 / den, r[0] + cscal(al / gm, r[(P - 1) * ld]));
+  for (int k = 0; k < P; ++k) r[k * ld] = r[k * ld] - cscal(z[k * ld], f);
 }
 
-// Non-cyclic (al, be, al) on separators 0..P-2 with X_{P-1} = 0.
+// Non-cyclic (al, be, al) on separators 0..P-2 with X_{P-1} = 0 (entry k at
+// k*ld as above).
 template <typename R>
-__device__ void reduced_pinned(C2<R>* r, R* g, int P, R al, R be, int& st) {
+__device__ void reduced_pinned(C2<R>* r, R* g, int P, int ld, R al, R be, int& st) {
   const int n = P - 1;
   if (n > 0) {
     R bet = be;
@@ -163,27 +166,28 @@ __device__ void reduced_pinned(C2<R>* r, R* g, int P, R al, R be, int& st) {
     r[0] = y;
     for (int k = 1; k < n; ++k) {
       R gk = al / bet;
-      g[k] = gk;
+      g[k * ld] = gk;
       bet = be - al * gk;
       if (bet == (R)0) {
         bet = (R)1.0e-30;
         st = PIB_ERR_PIVOT;
       }
-      y = cscal((R)1 / bet, r[k] - cscal(al, y));
-      r[k] = y;
+      y = cscal((R)1 / bet, r[k * ld] - cscal(al, y));
+      r[k * ld] = y;
     }
     for (int k = n - 2; k >= 0; --k) {
-      y = r[k] - cscal(g[k + 1], y);
-      r[k] = y;
+      y = r[k * ld] - cscal(g[(k + 1) * ld], y);
+      r[k * ld] = y;
     }
   }
-  r[P - 1] = mk((R)0, (R)0);
+  r[(P - 1) * ld] = mk((R)0, (R)0);
 }
 
 // LDS carve-up of one tile workgroup (bd = blockDim.x lanes):
 //   ef, el  [3][bd] complex      first / last row exchange between segments
-//   rr      [3][M*P] complex     reduced right-hand sides -> separators
-//   rz, rg  [3][M*P] real        reduced-solve work vectors
+//   rr      [P][NR*M] complex    reduced right-hand sides -> separators
+//                                (separator k of system j = r*M + m)
+//   rz, rg  [P][NR*M] real       reduced-solve work vectors
 template <typename R>
 struct VLds {
   C2<R>* ef;
@@ -275,7 +279,7 @@ struct VTile {
       for (int r = 0; r < NR; ++r) {
         C2<R> yn = S.ef[r * bd + nbl(s + 1)];
         C2<R> rhs = (x[r][L - 1] - cscal(t.a, x[r][L - 2])) - cscal(t.a, yn);
-        S.rr[(r * M + m) * P + s] = rhs;
+        S.rr[s * (NR * M) + r * M + m] = rhs;
       }
     }
     __syncthreads();
@@ -284,11 +288,10 @@ struct VTile {
       // mode's reduced coefficients, staged by its s = 0 lane (a workgroup
       // of fewer than NR*M lanes takes several systems per lane)
       int r = j / M, mm = j % M;
-      size_t o = (size_t)(r * M + mm) * P;
       int e = 0;
-      R al = S.rg[o], be = S.rz[o];
-      if (pinned) reduced_pinned(S.rr + o, S.rg + o, P, al, be, e);
-      else reduced_cyclic(S.rr + o, S.rz + o, S.rg + o, P, al, be, e);
+      R al = S.rg[j], be = S.rz[j];
+      if (pinned) reduced_pinned(S.rr + j, S.rg + j, P, NR * M, al, be, e);
+      else reduced_cyclic(S.rr + j, S.rz + j, S.rg + j, P, NR * M, al, be, e);
       int qm = q0 + mm;
       if (e && qm < nm && (mean || qm != 0)) sst = e;
     }
@@ -296,8 +299,8 @@ struct VTile {
     if (lane) {
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
-        size_t o = (size_t)(r * M + m) * P;
-        C2<R> Xl = S.rr[o + (s == 0 ? P - 1 : s - 1)], Xs = S.rr[o + s];
+        const int o = r * M + m;
+        C2<R> Xl = S.rr[(s == 0 ? P - 1 : s - 1) * (NR * M) + o], Xs = S.rr[s * (NR * M) + o];
 #pragma unroll
         for (int k = 0; k < L - 1; ++k) x[r][k] = (x[r][k] - cscal(t.v[k], Xl)) - cscal(t.v[L - 2 - k], Xs);
         x[r][L - 1] = Xs;
@@ -312,9 +315,8 @@ struct VTile {
     if (lane && s == 0) {
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
-        size_t o = (size_t)(r * M + m) * P;
-        S.rg[o] = t.al;
-        S.rz[o] = t.be;
+        S.rg[r * M + m] = t.al;
+        S.rz[r * M + m] = t.be;
       }
     }
   }
