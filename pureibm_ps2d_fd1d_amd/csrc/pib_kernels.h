@@ -876,35 +876,39 @@ __device__ void forcing_point(const KArgs<T>& A, const Particles& P, int m, int 
 // grid node were binned to that node at set-up (stable sort: the reference's
 // (body, point, corner) order within a node) and the target nodes were cut
 // into workgroup ranges of at most SPREAD_CAP records (bt).  A workgroup
-// first computes the contributions of its records with coalesced, batched
-// loads into LDS -- the weighted share (w F)/dx^3 and the redistributed
-// fluid share -- then one lane per target node adds its records in order:
-// acc = (acc + share) + dist, as the reference accumulates fr.
+// first computes the contribution of each of its records with coalesced,
+// batched loads -- the weighted share (w F)/dx^3 plus the redistributed
+// fluid share -- into LDS (component-major, 24 KB), then one lane per target
+// node adds its records in the reference's order.  The reference adds the
+// two shares to fr one after the other; here they are added together first
+// (a rounding-level difference).
 #define SPREAD_CAP 1024
 template <typename T>
 __global__ void __launch_bounds__(256) k_spread(KArgs<T> A, const int* bt, int nrange, const int* tnode,
                                                 const int* toff, const int* rpt, const double* rw,
                                                 const uint8_t* rdist, const double* pts) {
-  __shared__ double cw[SPREAD_CAP * 3];
-  __shared__ double cdist[SPREAD_CAP * 3];
+  __shared__ double cv[3 * SPREAD_CAP];
   const int w = xcd_linear(blockIdx.x, gridDim.x);  // neighbouring ranges share an L2
   if (w >= nrange) return;
   const int t0 = bt[w], t1 = bt[w + 1];
   const int r0 = toff[t0], r1 = toff[t1];
   const double dx3 = A.st->dx3;
   constexpr int CH = SPREAD_CAP / 256;
+  int rp[CH];
 #pragma unroll
   for (int k = 0; k < CH; ++k) {
     int r = r0 + (int)threadIdx.x + k * 256;
-    if (r < r1) {
-      const double* o = pts + (size_t)rpt[r] * 6;
-      double w = rw[r];
+    rp[k] = r < r1 ? rpt[r] : -1;
+  }
+#pragma unroll
+  for (int k = 0; k < CH; ++k) {
+    int r = r0 + (int)threadIdx.x + k * 256;
+    if (rp[k] >= 0) {
+      const double* o = pts + (size_t)rp[k] * 6;
+      double wt = rw[r];
       bool d = rdist[r] != 0;
       int q = r - r0;
-      for (int n = 0; n < 3; ++n) {
-        cw[q * 3 + n] = (w * o[n]) / dx3;
-        cdist[q * 3 + n] = d ? o[3 + n] : 0.0;
-      }
+      for (int n = 0; n < 3; ++n) cv[n * SPREAD_CAP + q] = (wt * o[n]) / dx3 + (d ? o[3 + n] : 0.0);
     }
   }
   __syncthreads();
@@ -913,7 +917,7 @@ __global__ void __launch_bounds__(256) k_spread(KArgs<T> A, const int* bt, int n
     double acc[3] = {0, 0, 0};
     for (int r = toff[t]; r < toff[t + 1]; ++r) {
       int q = r - r0;
-      for (int n = 0; n < 3; ++n) acc[n] = (acc[n] + cw[q * 3 + n]) + cdist[q * 3 + n];
+      for (int n = 0; n < 3; ++n) acc[n] = acc[n] + cv[n * SPREAD_CAP + q];
     }
     int node = tnode[t];
     for (int n = 0; n < 3; ++n) A.fr[n * cs + node] = (T)acc[n];
