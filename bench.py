@@ -182,6 +182,39 @@ def traffic_of(kernel):
         return None
 
 
+def run_batched(nreal, steps, warmup):
+    """Batched realizations (FP32-bulk): nreal independent contexts of the C3
+    mix (96^3/128^3/160^3 beds), each on its own stream.  Times the contexts
+    one after the other and then all launched together (pib_step_async on
+    every context, then pib_sync on every context)."""
+    work = tempfile.mkdtemp(prefix="pib_c3_")
+    sts = [solver.RunSetup(configs.c3_realization(work, k), precision=_abi.PIB_FP32_BULK) for k in range(nreal)]
+    gs = [solver.Solver(st) for st in sts]
+    for g in gs:
+        g.step(warmup)
+    alone = 0.0
+    for g in gs:
+        t0 = time.perf_counter()
+        if g.lib.pib_step(g.ctx, steps) != 0:
+            raise RuntimeError(g.lib.pib_last_error(g.ctx).decode())
+        alone += time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for g in gs:
+        if g.lib.pib_step_async(g.ctx, steps) != 0:
+            raise RuntimeError(g.lib.pib_last_error(g.ctx).decode())
+    for g in gs:
+        if g.lib.pib_sync(g.ctx) != 0:
+            raise RuntimeError(g.lib.pib_last_error(g.ctx).decode())
+    together = time.perf_counter() - t0
+    pts = sum(st.nx * st.my * st.mz for st in sts)
+    for g in gs:
+        g.close()
+    return {"realizations": nreal, "steps": steps, "grid_points": pts,
+            "value": pts * steps / together / 1e6, "unit": "Mgridpt-steps/s",
+            "one_after_another": pts * steps / alone / 1e6,
+            "sizes": [f"{st.nx}^3/{st.nbody}" for st in sts]}
+
+
 def cpu_baseline(setup, nsteps):
     """Single-core oracle on the same workload (bounded sample: one untimed
     first step, then nsteps timed steps)."""
@@ -209,6 +242,8 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=20, help="steps timed per launch for the roofline")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (scale for FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-realization figure")
+    ap.add_argument("--batched-steps", type=int, default=100)
     ap.add_argument("--collect-traffic", action="store_true",
                     help="measure HBM bytes per dispatch with rocprofv3 counter passes (child processes)")
     a = ap.parse_args()
@@ -283,6 +318,11 @@ def main():
                        "max_particle_drag_rel_err": float(np.max(np.abs(f32[:, 0] - f64[:, 0])) /
                                                           np.max(np.abs(f64[:, 0]))),
                        "norm_drag_fp64": d64, "norm_drag_fp32": d32}
+    if not a.no_batched and a.only in ("both", "fp32"):
+        try:
+            out["batched_c3"] = run_batched(12, a.batched_steps, 5)
+        except Exception as e:  # reported, never fatal for the headline figure
+            out["batched_c3"] = {"error": str(e)}
     if not a.no_cpu:
         ncpu = max(1, a.cpu_steps)
         tcpu = cpu_baseline(st64, ncpu)

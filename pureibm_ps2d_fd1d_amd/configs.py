@@ -120,6 +120,22 @@ def random_bed(directory, run_name, *, my, dbydx, nbody, seed, **kw):
                       centers=centers, **kw)
 
 
+def c3_realization(directory, k):
+    """Realization k (0..11) of the batched workload: grids 96^3/128^3/160^3 in
+    turn, sphere diameter d ~ U{12..20} cells, solid fraction ~ U[0.1, 0.3]
+    (RSA fills that quickly), Re_m ~ U[10, 50], all drawn from seed 1000 + k.
+    Re_m stays at or below 50 because the central x-flux scheme at these
+    resolutions diverges above Re_m of about 60 (FP64 and FP32 alike)."""
+    rng = np.random.Generator(np.random.PCG64(1000 + k))
+    my = (96, 128, 160)[k % 3]
+    d = int(rng.integers(12, 21))
+    phi = float(rng.uniform(0.1, 0.3))
+    re = float(rng.uniform(10.0, 50.0))
+    lybyd = my / d
+    nbody = max(1, int(round(phi * lybyd ** 3 / (math.pi / 6.0))))
+    return random_bed(directory, f"C3_{k}", my=my, dbydx=d, nbody=nbody, seed=k + 1, re=re)
+
+
 # Workloads of the benchmark table (box, spheres, Re); C2 is the headline.
 WORKLOADS = {
     "C1": dict(kind="simple", my=64, dbydx=16, re=20.0),
