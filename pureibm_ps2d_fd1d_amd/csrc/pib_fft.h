@@ -258,11 +258,14 @@ __device__ void fft_lines(C2<T>* buf, int L, int ls, int es, const Fft1D& pl,
 
 // ---- plane transforms ----------------------------------------------------
 // Forward: real plane -> (my/2+1, mz) half spectrum scaled by 1/(my*mz).
+// buf is the plane buffer: the workgroup's LDS, or a slot in HBM for planes
+// that do not fit it; the tables go to lds + tab_off (lds = buf when null).
 template <int NIT, typename T, class Op>
-__device__ void plane_r2c(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& gtb, Op& op) {
+__device__ void plane_r2c(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& gtb, Op& op,
+                          unsigned char* lds = nullptr) {
   const int hy = G.hy, ld = G.ld, mz = G.mz, my2 = G.my2;
   const int nt = blockDim.x;
-  const PlaneTabs<T> tb = stage_tabs(G, gtb, (unsigned char*)buf + G.tab_off);
+  const PlaneTabs<T> tb = stage_tabs(G, gtb, (lds ? lds : (unsigned char*)buf) + G.tab_off);
   for (int e0 = threadIdx.x; e0 < mz * hy; e0 += PIB_LD_CHUNK * nt) {
     C2<T> v[PIB_LD_CHUNK];
 #pragma unroll
@@ -328,12 +331,14 @@ __device__ void plane_r2c(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& gt
   __syncthreads();
 }
 
-// Inverse: (my/2+1, mz) half spectrum -> real plane, unscaled.
+// Inverse: (my/2+1, mz) half spectrum -> real plane, unscaled (buf and lds as
+// for plane_r2c).
 template <int NIT, typename T, class Op>
-__device__ void plane_c2r(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& gtb, Op& op) {
+__device__ void plane_c2r(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& gtb, Op& op,
+                          unsigned char* lds = nullptr) {
   const int hy = G.hy, ld = G.ld, mz = G.mz, my2 = G.my2;
   const int nt = blockDim.x;
-  const PlaneTabs<T> tb = stage_tabs(G, gtb, (unsigned char*)buf + G.tab_off);
+  const PlaneTabs<T> tb = stage_tabs(G, gtb, (lds ? lds : (unsigned char*)buf) + G.tab_off);
   for (int e0 = threadIdx.x; e0 < my2 * mz; e0 += PIB_LD_CHUNK * nt) {
     C2<T> v[PIB_LD_CHUNK];
 #pragma unroll

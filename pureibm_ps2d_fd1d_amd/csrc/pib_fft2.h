@@ -388,13 +388,28 @@ static inline int shape_ld(int sh) {
 static inline int shape_nt(int sh) {
   return sh == 1 ? Shape<1>::NT : (sh == 2 ? Shape<2>::NT : (sh == 3 ? Shape<3>::NT : 0));
 }
+// lds: the workgroup's LDS, which holds the plane (and the tables); hbm: the
+// workgroup's plane slot in HBM, or null.  The generic transform (SH = 0)
+// works in the slot when there is one; the two calls are separate so that
+// each is compiled for its own address space.  A compiled shape always
+// works in LDS.
 template <int SH, int NIT, typename T, class Op>
-__device__ __forceinline__ void plane_c2r_sh(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& tb, Op& op) {
-  if constexpr (SH == 0) plane_c2r<NIT>(buf, G, tb, op);
-  else plane_c2r2<Shape<SH>::HY, Shape<SH>::MZ, Shape<SH>::NT>(buf, G, tb, op);
+__device__ __forceinline__ void plane_c2r_sh(C2<T>* hbm, const PlaneGeom& G, const PlaneTabs<T>& tb, Op& op,
+                                             unsigned char* lds) {
+  if constexpr (SH == 0) {
+    if (hbm) plane_c2r<NIT>(hbm, G, tb, op, lds);
+    else plane_c2r<NIT>((C2<T>*)lds, G, tb, op, lds);
+  } else {
+    plane_c2r2<Shape<SH>::HY, Shape<SH>::MZ, Shape<SH>::NT>((C2<T>*)lds, G, tb, op);
+  }
 }
 template <int SH, int NIT, typename T, class Op>
-__device__ __forceinline__ void plane_r2c_sh(C2<T>* buf, const PlaneGeom& G, const PlaneTabs<T>& tb, Op& op) {
-  if constexpr (SH == 0) plane_r2c<NIT>(buf, G, tb, op);
-  else plane_r2c2<Shape<SH>::HY, Shape<SH>::MZ, Shape<SH>::NT>(buf, G, tb, op);
+__device__ __forceinline__ void plane_r2c_sh(C2<T>* hbm, const PlaneGeom& G, const PlaneTabs<T>& tb, Op& op,
+                                             unsigned char* lds) {
+  if constexpr (SH == 0) {
+    if (hbm) plane_r2c<NIT>(hbm, G, tb, op, lds);
+    else plane_r2c<NIT>((C2<T>*)lds, G, tb, op, lds);
+  } else {
+    plane_r2c2<Shape<SH>::HY, Shape<SH>::MZ, Shape<SH>::NT>((C2<T>*)lds, G, tb, op);
+  }
 }

@@ -478,16 +478,18 @@ struct OpRawC2R {
   }
 };
 template <typename T>
-__global__ void k_raw_r2c(PlaneGeom G, PlaneTabs<T> tb, const T* a, long long lda, C2<T>* b, long long ldb) {
+__global__ void k_raw_r2c(PlaneGeom G, PlaneTabs<T> tb, const T* a, long long lda, C2<T>* b, long long ldb,
+                          C2<T>* scr, size_t pslot) {
   extern __shared__ __align__(16) unsigned char smem[];
   OpRawR2C<T> op{a, b, lda, ldb, G.my, G.my2, (int)blockIdx.x};
-  plane_r2c<0>((C2<T>*)smem, G, tb, op);
+  plane_r2c_sh<0, 0>(scr ? scr + blockIdx.x * pslot : nullptr, G, tb, op, smem);
 }
 template <typename T>
-__global__ void k_raw_c2r(PlaneGeom G, PlaneTabs<T> tb, const C2<T>* b, long long ldb, T* a, long long lda) {
+__global__ void k_raw_c2r(PlaneGeom G, PlaneTabs<T> tb, const C2<T>* b, long long ldb, T* a, long long lda,
+                          C2<T>* scr, size_t pslot) {
   extern __shared__ __align__(16) unsigned char smem[];
   OpRawC2R<T> op{b, a, lda, ldb, G.my, G.my2, (int)blockIdx.x};
-  plane_c2r<0>((C2<T>*)smem, G, tb, op);
+  plane_c2r_sh<0, 0>(scr ? scr + blockIdx.x * pslot : nullptr, G, tb, op, smem);
 }
 
 // batched cyclic / plain Thomas, rows [i*nsys + s]
@@ -611,7 +613,7 @@ struct Impl : Base {
   hipStream_t stream = nullptr;
   // second stream for the form_nl branch of a stage, forked from and joined
   // back into `stream` with the two events
-  hipStream_t stream2 = nullptr;
+  hipStream_t stream2 = nullptr, stream_main = nullptr;
   hipEvent_t evfork = nullptr, evjoin = nullptr;
   // forked by default where the transforms hold one workgroup per CU (FP64
   // planes fill the LDS), so that the two branches share the CUs; per-launch
@@ -717,6 +719,7 @@ struct Impl : Base {
     }
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+    stream_main = stream;
     HIPCHK(hipEventCreateWithFlags(&evfork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&evjoin, hipEventDisableTiming));
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
@@ -739,8 +742,12 @@ struct Impl : Base {
     G.tab_off = (int)lds_plane;
     lds_plane += tab_lds_bytes<T>(hy, mz);
     lds_plane += (size_t)mz * sizeof(C2<T>);  // form_nl: ky = my/2 column accumulators
-    if (lds_plane > 160 * 1024)
-      throw PibFail(PIB_ERR_UNSUPPORTED, "plane does not fit the 160 KB LDS of one workgroup");
+    // a plane that does not fit the 160 KB LDS of a workgroup is transformed
+    // in a slot of HBM scratch (the generic transform; LDS holds the tables)
+    hbm_needed = lds_plane > 160 * 1024;
+    pslot = (size_t)mz * G.ld;
+    lds_hbm = (tab_lds_bytes<T>(hy, mz) + 15) / 16 * 16 + 4096;
+    if (hbm_needed) set_plane_hbm(true);
     tb.twy = upload(tw_table<T>(hy, hy));
     tb.twz = upload(tw_table<T>(mz, mz));
     tb.wr = upload(tw_table<T>(my, hy + 1));
@@ -834,6 +841,17 @@ struct Impl : Base {
     k_fill_u8<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(mask, n, 1);
     HIPCHK(hipGetLastError());
   }
+  // planes in HBM: one slot of pslot entries per workgroup of a plane launch
+  // (at most 6 transforms x nx planes), and 3 x nx more for the form_nl
+  // branch, which may run beside the others on stream2
+  bool hbm_needed = false, plane_hbm = false;
+  C2<T>* pscr = nullptr;
+  size_t pslot = 0, lds_hbm = 0;
+  void set_plane_hbm(bool on) {
+    if (on && !pscr) pscr = alloc<C2<T>>((size_t)9 * nx * pslot);
+    plane_hbm = on;
+  }
+  size_t plane_lds() const { return plane_hbm ? lds_hbm : lds_plane; }
   KArgs<T> args(int rks) const {
     KArgs<T> A;
     A.nx = nx;
@@ -848,6 +866,9 @@ struct Impl : Base {
     A.wz = wz;
     A.w2 = w2;
     A.G = G;
+    A.pscr = plane_hbm ? pscr + (stream == stream_main ? 0 : (size_t)6 * nx * pslot) : nullptr;
+    A.pslot = pslot;
+    if (plane_hbm) A.G.tab_off = 0;
     A.tb = tb;
     A.u = u;
     A.p = p;
@@ -895,24 +916,24 @@ struct Impl : Base {
   // one and fft_compiled is set, else the generic mixed-radix transform
   int sh = 0;
   bool fft_compiled = true;
-  int cur_sh() const { return fft_compiled ? sh : 0; }
+  int cur_sh() const { return fft_compiled && !plane_hbm ? sh : 0; }
   int plane_nt(int s) const { return s ? shape_nt(s) : nt_plane; }
   template <int KIND, int SH>
   void launch_c2r(int rks, int ncomp) {
     static const char* names[] = {"c2r_ubcp", "c2r_pbcp", "c2r_nlbc", "c2r_onlbc", "c2r_visc", "c2r_omega", "c2r_pgrad"};
     auto kern = k_c2r<T, KIND, SH>;
-    set_lds(kern, lds_plane);
+    set_lds(kern, plane_lds());
     kb(names[KIND]);
-    kern<<<dim3(nx, ncomp), plane_nt(SH), lds_plane, stream>>>(args(rks));
+    kern<<<dim3(nx, ncomp), plane_nt(SH), plane_lds(), stream>>>(args(rks));
     HIPCHK(hipGetLastError());
     ke();
   }
   template <int KA, int NA, int KB, int NB, int SH>
   void launch_c2r_pair(int rks, const char* name) {
     auto kern = k_c2r_pair<T, KA, NA, KB, SH>;
-    set_lds(kern, lds_plane);
+    set_lds(kern, plane_lds());
     kb(name);
-    kern<<<dim3(nx, NA + NB), plane_nt(SH), lds_plane, stream>>>(args(rks));
+    kern<<<dim3(nx, NA + NB), plane_nt(SH), plane_lds(), stream>>>(args(rks));
     HIPCHK(hipGetLastError());
     ke();
   }
@@ -1143,6 +1164,13 @@ struct Impl : Base {
     size_t E = (size_t)my2 * mz;
     int nit = (int)((E + nt_nl() - 1) / nt_nl());
     bool up = cfg.discr_scheme == PIB_SCHEME_UPWIND, mm = cfg.limiter == PIB_LIMITER_MINMOD;
+    if (plane_hbm || nit > 26) {
+      // the in-place form: no accumulators, the plane wherever it lives
+      if (!up) run_nl(k_form_nl<T, 1, false, false, -1>, rks, nt_plane);
+      else if (!mm) run_nl(k_form_nl<T, 1, true, false, -1>, rks, nt_plane);
+      else run_nl(k_form_nl<T, 1, true, true, -1>, rks, nt_plane);
+      return;
+    }
     if (!launch_nl<1>(nit, up, mm, rks) && !launch_nl<2>(nit, up, mm, rks) && !launch_nl<3>(nit, up, mm, rks) &&
         !launch_nl<5>(nit, up, mm, rks) && !launch_nl<9>(nit, up, mm, rks) && !launch_nl<13>(nit, up, mm, rks) &&
         !launch_nl<17>(nit, up, mm, rks) && !launch_nl<26>(nit, up, mm, rks))
@@ -1165,9 +1193,9 @@ struct Impl : Base {
   }
   template <typename K>
   void run_nl(K kern, int rks, int nt, const char* name = "form_nl") {
-    set_lds(kern, lds_plane);
+    set_lds(kern, plane_lds());
     kb(name);
-    kern<<<dim3(nx, 3), nt, lds_plane, stream>>>(args(rks));
+    kern<<<dim3(nx, 3), nt, plane_lds(), stream>>>(args(rks));
     HIPCHK(hipGetLastError());
     ke();
   }
@@ -1242,9 +1270,9 @@ struct Impl : Base {
   template <int SH>
   void r2c_force(int rks) {
     auto kern = k_r2c_force<T, SH>;
-    set_lds(kern, lds_plane);
+    set_lds(kern, plane_lds());
     kb("r2c_force");
-    kern<<<dim3(nx, 3), plane_nt(SH), lds_plane, stream>>>(args(rks));
+    kern<<<dim3(nx, 3), plane_nt(SH), plane_lds(), stream>>>(args(rks));
     HIPCHK(hipGetLastError());
     ke();
   }
@@ -1576,8 +1604,14 @@ struct Impl : Base {
     C2<T>* db = alloc<C2<T>>(nb2, true);
     HIPCHK(hipMemcpyAsync(da, ha.data(), na * sizeof(T), hipMemcpyHostToDevice, stream));
     auto kern = k_raw_r2c<T>;
-    set_lds(kern, lds_plane);
-    kern<<<np, nt_plane, lds_plane, stream>>>(G, tb, da, (long long)my * mz, db, (long long)my2 * mz);
+    PlaneGeom g = G;
+    C2<T>* scr = nullptr;
+    if (plane_hbm) {
+      g.tab_off = 0;
+      scr = alloc<C2<T>>((size_t)np * pslot, true);
+    }
+    set_lds(kern, plane_lds());
+    kern<<<np, nt_plane, plane_lds(), stream>>>(g, tb, da, (long long)my * mz, db, (long long)my2 * mz, scr, pslot);
     HIPCHK(hipGetLastError());
     std::vector<C2<T>> hb(nb2);
     HIPCHK(hipMemcpyAsync(hb.data(), db, nb2 * sizeof(C2<T>), hipMemcpyDeviceToHost, stream));
@@ -1587,7 +1621,7 @@ struct Impl : Base {
         b[2 * ((size_t)q * ldb + e)] = (double)hb[(size_t)q * my2 * mz + e].x;
         b[2 * ((size_t)q * ldb + e) + 1] = (double)hb[(size_t)q * my2 * mz + e].y;
       }
-    free_tail(2);
+    free_tail(scr ? 3 : 2);
   }
   void ff2cr_planes(const double* b, long long ldb, int np, double* a, long long lda) {
     size_t na = (size_t)np * my * mz, nb2 = (size_t)np * my2 * mz;
@@ -1601,15 +1635,21 @@ struct Impl : Base {
     C2<T>* db = alloc<C2<T>>(nb2, true);
     HIPCHK(hipMemcpyAsync(db, hb.data(), nb2 * sizeof(C2<T>), hipMemcpyHostToDevice, stream));
     auto kern = k_raw_c2r<T>;
-    set_lds(kern, lds_plane);
-    kern<<<np, nt_plane, lds_plane, stream>>>(G, tb, db, (long long)my2 * mz, da, (long long)my * mz);
+    PlaneGeom g = G;
+    C2<T>* scr = nullptr;
+    if (plane_hbm) {
+      g.tab_off = 0;
+      scr = alloc<C2<T>>((size_t)np * pslot, true);
+    }
+    set_lds(kern, plane_lds());
+    kern<<<np, nt_plane, plane_lds(), stream>>>(g, tb, db, (long long)my2 * mz, da, (long long)my * mz, scr, pslot);
     HIPCHK(hipGetLastError());
     std::vector<T> ha(na);
     HIPCHK(hipMemcpyAsync(ha.data(), da, na * sizeof(T), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     for (int q = 0; q < np; ++q)
       for (size_t e = 0; e < (size_t)my * mz; ++e) a[(size_t)q * lda + e] = (double)ha[(size_t)q * my * mz + e];
-    free_tail(2);
+    free_tail(scr ? 3 : 2);
   }
   void free_tail(int n) {
     for (int q = 0; q < n && !pallocs.empty(); ++q) {
@@ -1771,11 +1811,12 @@ int pib_set_tuning(void* ctx, int key, int value) {
     });
   }
   if (key == PIB_TUNE_FFT) {
-    if (value < 0 || value > 1) return PIB_ERR_ARG;
+    if (value < 0 || value > 2) return PIB_ERR_ARG;
     return dispatch(ctx, [&](auto& I) {
       HIPCHK(hipStreamSynchronize(I.stream));
       I.drop_graphs();
       I.fft_compiled = value == 1;
+      I.set_plane_hbm(value == 2 || I.hbm_needed);
     });
   }
   if (key == PIB_TUNE_STREAMS) {

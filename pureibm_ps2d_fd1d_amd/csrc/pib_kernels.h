@@ -53,10 +53,20 @@ struct KArgs {
   const uint8_t* mask;
   DevState* st;
   double* part;  // per-workgroup partial results
+  // planes that do not fit the LDS of a workgroup live in HBM: one slot of
+  // pslot entries per workgroup of a launch (null when planes fit the LDS)
+  C2<T>* pscr;
+  size_t pslot;
   int rks;
   int xcd_map;   // map workgroups to planes so neighbouring planes share an XCD
 };
 
+// This workgroup's plane slot in HBM, or null when planes live in LDS.
+template <typename T>
+__device__ __forceinline__ C2<T>* plane_slot(const KArgs<T>& A) {
+  if (!A.pscr) return nullptr;
+  return A.pscr + (size_t)(blockIdx.y * gridDim.x + blockIdx.x) * A.pslot;
+}
 template <typename T>
 __device__ __forceinline__ size_t sidx(const KArgs<T>& A, int c, int i, int ky, int kz) {
   return (((size_t)c * A.nx + i) * A.mz + kz) * A.my2 + ky;
@@ -349,10 +359,10 @@ enum { C2R_UBCP = 0, C2R_PBCP, C2R_NLBC, C2R_ONLBC, C2R_VISC, C2R_OMEGA, C2R_PGR
 
 template <typename T, int KIND, int SH>
 __device__ __forceinline__ void c2r_plane(const KArgs<T>& A, unsigned char* smem, int c, int i) {
-  C2<T>* buf = (C2<T>*)smem;
+  C2<T>* buf = SH == 0 ? plane_slot(A) : nullptr;
   if constexpr (KIND == C2R_UBCP) {
     OpUbcp<T> op(A, c, i);
-    plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op);
+    plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op, smem);
     double* sh = (double*)smem;
     double sm[3] = {op.sf, op.ss, op.sa}, mx[1] = {op.mx};
     block_sums_w<3>(sm, sh);
@@ -363,21 +373,21 @@ __device__ __forceinline__ void c2r_plane(const KArgs<T>& A, unsigned char* smem
     }
   } else if constexpr (KIND == C2R_PBCP) {
     OpPlain<T> op{A.p, A.pbcp, A, (size_t)i * A.mz * A.my2, (size_t)i * A.mz * A.my, (T)0};
-    plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op);
+    plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op, smem);
   } else if constexpr (KIND == C2R_NLBC || KIND == C2R_ONLBC) {
     const C2<T>* src = KIND == C2R_NLBC ? A.nl : A.onl;
     T* dst = KIND == C2R_NLBC ? A.nlbc : A.onlbc;
     OpPlain<T> op{src, dst, A, ((size_t)c * A.nx + i) * A.mz * A.my2, ((size_t)c * A.nx + i) * A.mz * A.my, (T)0};
-    plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op);
+    plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op, smem);
   } else if constexpr (KIND == C2R_VISC) {
     OpVisc<T> op{A, c, i};
-    plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op);
+    plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op, smem);
   } else if constexpr (KIND == C2R_OMEGA) {
     OpOmega<T> op{A, c, i};
-    plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op);
+    plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op, smem);
   } else {
     OpPgrad<T> op{A, c, i, (T)A.st->s.mpg[c]};
-    plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op);
+    plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op, smem);
   }
 }
 // Waves per SIMD the FP32 transforms of the 128^2 plane are compiled for:
@@ -421,11 +431,11 @@ struct OpForce {
 template <typename T, int SH>
 __global__ void __launch_bounds__(1024, (PlaneOcc<T, SH>::WPE)) k_r2c_force(KArgs<T> A) {
   extern __shared__ __align__(16) unsigned char smem[];
-  C2<T>* buf = (C2<T>*)smem;
+  C2<T>* buf = SH == 0 ? plane_slot(A) : nullptr;
   const int i = plane_of(A);
   const int c = blockIdx.y;
   OpForce<T> op{A, c, i, (T)A.st->s.frmean[c]};
-  plane_r2c_sh<SH, 0>(buf, A.G, A.tb, op);
+  plane_r2c_sh<SH, 0>(buf, A.G, A.tb, op, smem);
 }
 
 // ===========================================================================
@@ -553,16 +563,19 @@ struct OpNlStage {
 };
 
 // SH = 0 (generic plane): the three transforms in one workgroup with
-// register accumulators, stored negated at the end.  Compiled shape: the
-// three transforms in one workgroup, each updating nl in place (the lane
-// that writes an output element is the lane that updates it).  Both add in
-// the order ((wy F0 + wz F1) + F2).
-// Workgroup size and waves per SIMD of a compiled-shape form_nl: the FP32
-// 128^2 plane runs 512 lanes at 4 waves per SIMD (128 VGPRs), so that two
-// workgroups share a CU; others use the shape's lanes with no bound.
+// register accumulators, stored negated at the end.  Compiled shape, and
+// SH = -1 (generic plane, for planes whose accumulators do not fit the
+// registers or whose buffer is in HBM): the three transforms in one
+// workgroup, each updating nl in place (the lane that writes an output
+// element is the lane that updates it).  All add in the order
+// ((wy F0 + wz F1) + F2).
+// Workgroup size and waves per SIMD of form_nl: the FP32 128^2 plane runs
+// 512 lanes at 4 waves per SIMD (128 VGPRs), so that two workgroups share a
+// CU; the generic in-place form (SH = -1) takes up to 1024 lanes; others use
+// the shape's lanes with no bound.
 template <typename T, int SH>
 struct NlOcc {
-  static constexpr int NT = (SH == 1 && sizeof(T) == 4) ? 512 : (SH == 0 ? 512 : Shape<SH>::NT);
+  static constexpr int NT = (SH == 1 && sizeof(T) == 4) ? 512 : (SH == 0 ? 512 : (SH < 0 ? 1024 : Shape<SH>::NT));
   static constexpr int WPE = (SH == 1 && sizeof(T) == 4) ? 4 : 1;
 };
 template <typename T, int NIT, bool UPWIND, bool MINMOD, int SH>
@@ -587,6 +600,15 @@ __global__ void __launch_bounds__((NlOcc<T, SH>::NT), (NlOcc<T, SH>::WPE)) k_for
         A.nl[sidx(A, c, i, ky, kz)] = mk(-v.x, -v.y);
       }
     }
+  } else if constexpr (SH < 0) {
+    C2<T>* pb = plane_slot(A);
+    using O = OpNl<T, NIT, UPWIND, MINMOD>;
+    OpNlStage<0, O, T> o0{op};
+    plane_r2c_sh<0, 0>(pb, A.G, A.tb, o0, smem);
+    OpNlStage<1, O, T> o1{op};
+    plane_r2c_sh<0, 0>(pb, A.G, A.tb, o1, smem);
+    OpNlStage<2, O, T> o2{op};
+    plane_r2c_sh<0, 0>(pb, A.G, A.tb, o2, smem);
   } else {
     constexpr int HY = Shape<SH>::HY, MZ = Shape<SH>::MZ, NT = NlOcc<T, SH>::NT;
     op.nyq = (C2<T>*)(smem + A.G.tab_off + tab_lds_bytes<T>(HY, MZ));

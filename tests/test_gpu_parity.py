@@ -31,7 +31,8 @@ def _ctx(cfg):
 
 
 @pytest.mark.parametrize("my,mz,prec", [(8, 8, 0), (16, 12, 0), (24, 20, 0), (64, 64, 0), (96, 96, 0),
-                                         (128, 128, 0), (128, 128, 1), (160, 160, 1)])
+                                         (128, 128, 0), (128, 128, 1), (160, 160, 1), (160, 160, 0),
+                                         (256, 256, 1)])
 def test_transform_layer(orc, my, mz, prec):
     """pib_ff2rc_planes / pib_ff2cr_planes against the oracle's ff2rc/ff2cr,
     with non-Hermitian DC/Nyquist input for c2r."""
@@ -57,8 +58,10 @@ def test_transform_layer(orc, my, mz, prec):
     lib.pib_destroy(ctx)
 
 
-def test_transform_too_large_is_an_error():
-    lib, ctx, st = _ctx(_cfg(160, 160, prec=0))
+def test_transform_size_not_235_smooth_is_an_error():
+    """my/2 = 7 has no 2,3,5 factorisation: refused at create (planes too
+    large for the LDS are not an error: they go to HBM slots)."""
+    lib, ctx, st = _ctx(_cfg(14, 16, prec=0))
     assert st == _abi.PIB_ERR_UNSUPPORTED
 
 
@@ -251,3 +254,32 @@ def test_compiled_plane_shapes_match_generic_and_oracle(tmp_path, orc, my, dbydx
     print(f"  u err compiled {e1:.2e} generic {e0:.2e} force {f1:.2e}", flush=True)
     assert e1 < 1e-10 and e0 < 1e-10 and f1 < 1e-9
     assert rel_err(g1.field("nl"), g0.field("nl")) < 1e-12
+
+
+@pytest.mark.parametrize("my,dbydx,nbody,scheme", [(24, 6, 3, "center"), (40, 10, 2, "upwind")])
+def test_planes_in_hbm_match_lds_and_oracle(tmp_path, orc, my, dbydx, nbody, scheme):
+    """Plane transforms whose buffer is a slot of HBM scratch (the path of planes
+    too large for the LDS of a workgroup, forced here with PIB_TUNE_FFT = 2,
+    including the in-place generic form_nl) against the LDS-resident generic
+    transforms and against the oracle, over whole steps."""
+    print(f"planes in HBM my={my} scheme={scheme}", flush=True)
+    kw = dict(kind="random", my=my, dbydx=dbydx, nbody=nbody, seed=41, re=30.0, discr_scheme=scheme)
+    st = make_setup(tmp_path, "HB", **kw)
+    o = solver.Solver(st, lib=orc.load(), prefix="orc_")
+    g2 = solver.Solver(st)
+    g0 = solver.Solver(st)
+    assert g2.lib.pib_set_tuning(g2.ctx, _abi.PIB_TUNE_FFT, 2) == 0
+    assert g0.lib.pib_set_tuning(g0.ctx, _abi.PIB_TUNE_FFT, 0) == 0
+    o.step(4)
+    g2.step(4)
+    g0.step(4)
+    uo = o.field("u")
+    e2, e0 = rel_err(g2.field("u"), uo), rel_err(g0.field("u"), uo)
+    f2 = rel_err(g2.forces()["force"], o.forces()["force"])
+    print(f"  u err hbm {e2:.2e} lds {e0:.2e} force {f2:.2e}", flush=True)
+    assert e2 < 1e-10 and e0 < 1e-10 and f2 < 1e-9
+    assert rel_err(g2.field("nl"), g0.field("nl")) < 1e-12
+    assert rel_err(g2.field("ubcp"), g0.field("ubcp")) < 1e-12
+    o.close()
+    g2.close()
+    g0.close()
