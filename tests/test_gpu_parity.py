@@ -283,3 +283,24 @@ def test_planes_in_hbm_match_lds_and_oracle(tmp_path, orc, my, dbydx, nbody, sch
     o.close()
     g2.close()
     g0.close()
+
+
+def test_c5_size_bed_runs_and_precisions_agree(tmp_path):
+    """The 256^3 box of the largest workload (1 200 spheres of 20 cells, fixed
+    bed): planes too large for the LDS run in HBM slots.  No oracle run at this
+    size; the size-independent checks are the discrete divergence after the
+    projection (round-off) and FP32-bulk against FP64."""
+    print("c5-size bed 256^3", flush=True)
+    d = str(tmp_path)
+    configs.random_bed(d, "C5F", my=256, dbydx=20, nbody=1200, seed=5, re=50.0)
+    runs = {}
+    for prec in (_abi.PIB_FP64, _abi.PIB_FP32_BULK):
+        st = solver.RunSetup(d + "/C5F", precision=prec)
+        g = solver.Solver(st)
+        g.step(3)
+        runs[prec] = g.scalars()
+        g.close()
+    s64, s32 = runs[_abi.PIB_FP64], runs[_abi.PIB_FP32_BULK]
+    print(f"  norm_drag fp64 {s64.norm_drag:.8e} fp32 {s32.norm_drag:.8e} divmax {s64.divmax:.2e}", flush=True)
+    assert s64.divmax < 1e-10
+    assert abs(s32.norm_drag - s64.norm_drag) < 1e-4 * abs(s64.norm_drag)
