@@ -12,6 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "liboracle.so")
 SRC = os.path.join(HERE, "pib_oracle.c")
+SRCS = [SRC, os.path.join(HERE, "pib_dem_oracle.c")]
 
 _lib = None
 
@@ -19,9 +20,9 @@ _lib = None
 def build(force=False):
     """Compile the oracle with gcc (FP contraction off, so each op rounds as written)."""
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
-    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
+    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < max(map(os.path.getmtime, SRCS)):
         subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared",
-                               "-o", LIB, SRC, "-lm"])
+                               "-o", LIB] + SRCS + ["-lm"])
     return LIB
 
 
@@ -35,6 +36,9 @@ def load():
         from pureibm_ps2d_fd1d_amd import _abi
         lib = ctypes.CDLL(LIB)
         _abi.bind(lib, "orc_", _abi._STEP_SIGS)
+        _abi.bind(lib, "orc_", _abi._DEM_SIGS)
+        lib.orc_dem_neighbours.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        lib.orc_dem_neighbours.restype = ctypes.c_int
         PD = ctypes.POINTER(ctypes.c_double)
         lib.orc_ff2rc.argtypes = [ctypes.c_int, ctypes.c_int, PD, PD]
         lib.orc_ff2rc.restype = None
@@ -109,4 +113,18 @@ def if_rev(solver):
     nb, nbnd = solver.setup.nbody, solver.setup.nbnd
     out = np.zeros((nbnd, nb), dtype=np.uint8, order="F")
     load().orc_get_if_rev(solver.ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
+    return out
+
+
+def dem_neighbours(dem):
+    """NEIGHBOURS(l, 1..) of an oracle DEM context as a list of sorted sets of
+    0-based partner indices (the particle itself left out)."""
+    n = dem.nbody
+    buf = np.zeros(n * 64, dtype=np.int32)
+    w = load().orc_dem_neighbours(dem.ctx, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    out = []
+    for l in range(n):
+        row = buf[l * w:(l + 1) * w]
+        cnt = row[0]
+        out.append(sorted(int(v) - 1 for v in row[1:cnt + 1] if v - 1 != l))
     return out

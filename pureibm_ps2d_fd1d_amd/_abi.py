@@ -21,6 +21,8 @@ PIB_ERR_HIP = -3
 PIB_ERR_STATE = -4
 PIB_ERR_BLOWUP = -5
 PIB_ERR_PIVOT = -6
+PIB_ERR_MOTION = -7
+PIB_ERR_CAPACITY = -8
 
 PIB_TUNE_VELSTEP = 0
 PIB_TUNE_GRAPH = 1
@@ -137,7 +139,41 @@ _GPU_ONLY_SIGS = {
     "set_tuning": ([_VP, _I, _I], _I),
 }
 
-GPU_SYMBOLS = ["pib_" + k for k in list(_STEP_SIGS) + list(_GPU_ONLY_SIGS)]
+
+
+class PibDemConfig(ctypes.Structure):
+    """pib_dem_config (include/pureibm_dem.h)."""
+    _fields_ = [
+        ("nbody", ctypes.c_int), ("only_dem", ctypes.c_int), ("reserved_i", ctypes.c_int * 2),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+        ("xlength", ctypes.c_double), ("ylength", ctypes.c_double), ("zlength", ctypes.c_double),
+        ("foffset", ctypes.c_double), ("rhos", ctypes.c_double), ("rhof", ctypes.c_double),
+        ("dia_phys", ctypes.c_double), ("coeff_rest", ctypes.c_double), ("kn", ctypes.c_double),
+        ("reserved_d", ctypes.c_double * 6),
+    ]
+
+
+class PibDemInfo(ctypes.Structure):
+    """pib_dem_info (include/pureibm_dem.h)."""
+    _fields_ = [
+        ("dtsolid", ctypes.c_double), ("tcoll", ctypes.c_double), ("dtsolid_used", ctypes.c_double),
+        ("s_time", ctypes.c_double), ("overlap_max", ctypes.c_double), ("box", ctypes.c_double * 3),
+        ("factor", ctypes.c_int), ("ncontacts", ctypes.c_int), ("cells", ctypes.c_int * 3),
+        ("substeps", ctypes.c_int), ("rebuilds", ctypes.c_int), ("reserved", ctypes.c_int * 7),
+    ]
+
+
+# DEM entry points shared by the GPU library (pib_dem_) and the oracle (orc_dem_)
+_DEM_SIGS = {
+    "dem_create": ([ctypes.POINTER(PibDemConfig), _PD, _PD, _PD, ctypes.POINTER(_VP)], _I),
+    "dem_destroy": ([_VP], _I),
+    "dem_last_error": ([_VP], ctypes.c_char_p),
+    "dem_march": ([_VP, _D, _PD, _PD, _PD, _PD, _PD, _PD, _PD], _I),
+    "dem_get_state": ([_VP, _PD, _PD, _PD, _PD, ctypes.POINTER(PibDemInfo)], _I),
+    "dem_set_state": ([_VP, _PD, _PD], _I),
+}
+
+GPU_SYMBOLS = ["pib_" + k for k in list(_STEP_SIGS) + list(_GPU_ONLY_SIGS) + list(_DEM_SIGS)]
 
 
 def bind(lib, prefix, sigs):
@@ -171,6 +207,7 @@ def load_gpu():
         lib = ctypes.CDLL(path)
         bind(lib, "pib_", _STEP_SIGS)
         bind(lib, "pib_", _GPU_ONLY_SIGS)
+        bind(lib, "pib_", _DEM_SIGS)
         _gpu = lib
     return _gpu
 

@@ -48,7 +48,8 @@ def build_gpu():
     os.makedirs(BUILD, exist_ok=True)
     out = os.path.join(BUILD, "libpureibm_gpu.so")
     _run([hipcc()] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"),
-                                  os.path.join(CSRC, "pib_gpu.hip"), "-o", out + ".tmp"])
+                                  os.path.join(CSRC, "pib_gpu.hip"), os.path.join(CSRC, "pib_dem.hip"),
+                                  "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out
 

@@ -173,3 +173,97 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+
+class Dem:
+    """Soft-sphere DEM of a freely moving bed, one context per realization.
+
+    Mirrors des_time_march (collision_mod.F90:49-344) over the C-ABI of
+    include/pureibm_dem.h: the constructor is des_time_march(.TRUE.) (coarse
+    grid, DES_INIT_ARRAYS, CFASSIGN, first neighbour search) and ``march`` is
+    des_time_march(.FALSE.) followed by IBMUPDATE, returning the new XC,
+    VELBDY and ANGV.  Per-particle arrays are (nbody, 3); xc and radbdy are in
+    grid units as the fluid solver holds them.  ``lib``/``prefix`` select the
+    implementation (HIP library by default; a missing library raises).
+    """
+
+    def __init__(self, xc, radbdy, *, dx, xlength, ylength, zlength, velbdy=None, dy=None, dz=None,
+                 foffset=0.0, rhos=1.0, rhof=1.0, dia_phys=1.0, coeff_rest=0.9, kn=0.0,
+                 only_dem=False, lib=None, prefix="pib_"):
+        self.lib = lib if lib is not None else _abi.load_gpu()
+        self.p = prefix
+        xc = np.asfortranarray(xc, dtype=np.float64)
+        rad = np.ascontiguousarray(radbdy, dtype=np.float64)
+        self.nbody = n = xc.shape[0]
+        if xc.shape != (n, 3) or rad.shape != (n,):
+            raise ValueError("xc must be (nbody, 3) and radbdy (nbody,)")
+        c = _abi.PibDemConfig()
+        c.nbody = n
+        c.only_dem = 1 if only_dem else 0
+        c.dx = dx
+        c.dy = dx if dy is None else dy
+        c.dz = dx if dz is None else dz
+        c.xlength, c.ylength, c.zlength = xlength, ylength, zlength
+        c.foffset, c.rhos, c.rhof = foffset, rhos, rhof
+        c.dia_phys, c.coeff_rest, c.kn = dia_phys, coeff_rest, kn
+        self.cfg = c
+        self.only_dem = bool(only_dem)
+        vel = None if velbdy is None else np.asfortranarray(velbdy, dtype=np.float64)
+        if vel is not None and vel.shape != (n, 3):
+            raise ValueError("velbdy must be (nbody, 3)")
+        self.ctx = ctypes.c_void_p()
+        st = self._fn("create")(ctypes.byref(c), dptr(xc), dptr(rad), dptr(vel) if vel is not None else None,
+                                ctypes.byref(self.ctx))
+        if st != PIB_OK:
+            self._raise(st)
+
+    def _fn(self, name):
+        return getattr(self.lib, self.p + "dem_" + name)
+
+    def _raise(self, st):
+        msg = self._fn("last_error")(self.ctx) if self.ctx else b"create failed"
+        raise PibError(st, (msg or b"").decode(errors="replace"))
+
+    def _call(self, name, *args):
+        st = self._fn(name)(*args)
+        if st != PIB_OK:
+            self._raise(st)
+
+    def march(self, dt, pres=None, visc=None, mpg=None, frame_accln=None):
+        """des_time_march(.FALSE.) + IBMUPDATE -> (xc, velbdy, angv), each (nbody, 3)."""
+        n = self.nbody
+
+        def arr(a, shape):
+            return None if a is None else np.asfortranarray(np.broadcast_to(a, shape), dtype=np.float64)
+        pres, visc = arr(pres, (n, 3)), arr(visc, (n, 3))
+        mpg, fa = arr(mpg, (3,)), arr(frame_accln, (3,))
+        xc = np.zeros((n, 3), order="F")
+        vel = np.zeros((n, 3), order="F")
+        angv = np.zeros((n, 3), order="F")
+        self._call("march", self.ctx, float(dt), *(dptr(a) if a is not None else None for a in (pres, visc, mpg, fa)),
+                   dptr(xc), dptr(vel), dptr(angv))
+        return xc, vel, angv
+
+    def state(self):
+        """(pos, vel, omega, contact_force, info) in physical units."""
+        n = self.nbody
+        out = [np.zeros((n, 3), order="F") for _ in range(4)]
+        info = _abi.PibDemInfo()
+        self._call("get_state", self.ctx, *(dptr(a) for a in out), ctypes.byref(info))
+        return (*out, info)
+
+    def set_state(self, pos, vel):
+        pos = np.asfortranarray(pos, dtype=np.float64)
+        vel = np.asfortranarray(vel, dtype=np.float64)
+        self._call("set_state", self.ctx, dptr(pos), dptr(vel))
+
+    def close(self):
+        if self.ctx:
+            self._fn("destroy")(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,188 @@
+"""CPU checks that pin the DEM oracle (oracle/pib_dem_oracle.c) to what the
+mathematics fixes: the closed-form contact time and restitution of the
+linear spring-dashpot (collision_mod.F90:680-693), the pairwise momentum
+balance of CFFCTOW, free flight with the hydrodynamic terms of CFNEWVALUES
+(:1374), the periodic wrap (:1431-1459), IBMUPDATE (:1464-1474) and the
+neighbour sets of GRID_BASED_NEIGHBOR_SEARCH (:807-1062) against brute force
+over periodic images.  No GPU."""
+import math
+
+import numpy as np
+import pytest
+
+from pureibm_ps2d_fd1d_amd import _abi, solver
+from dem_beds import random_bed
+
+DX = 1.0 / 16
+KN = 800000.0
+SMALL = float(np.float32(1.0e-35))
+
+
+def odem(orc, xc, rad, **kw):
+    kw.setdefault("dx", DX)
+    return solver.Dem(xc, rad, lib=orc.load(), prefix="orc_", **kw)
+
+
+def head_on(orc, e, gap=0.01, speed=1.0, box=5.0):
+    """Two unit-diameter spheres on the x axis approaching at 2*speed."""
+    xc = np.array([[1.5 / DX + 1, 2.5 / DX + 1, 2.5 / DX + 1],
+                   [(2.5 + gap) / DX + 1, 2.5 / DX + 1, 2.5 / DX + 1]])
+    vel = np.array([[speed, 0.0, 0.0], [-speed, 0.0, 0.0]])
+    return odem(orc, xc, np.array([8.0, 8.0]), xlength=box, ylength=box, zlength=box,
+                velbdy=vel, coeff_rest=e, only_dem=True)
+
+
+def collide(d, dts):
+    """March in sub-steps of dts until the pair separates; -> (e, contact time)."""
+    pos, v, _, _, info = d.state()
+    vrel0 = v[0, 0] - v[1, 0]
+    t = 0.0
+    tin = None
+    for _ in range(100000):
+        d.march(dts)
+        t += dts
+        pos, v, _, _, info = d.state()
+        if info.ncontacts and tin is None:
+            tin = t
+        if tin is not None and not info.ncontacts:
+            return (v[1, 0] - v[0, 0]) / vrel0, t - tin
+    raise AssertionError("no separation")
+
+
+@pytest.mark.parametrize("e", [0.3, 0.7, 0.9])
+def test_contact_time_is_the_damped_half_period(orc, e):
+    d = head_on(orc, e)
+    info = d.state()[4]
+    m = math.pi / 6.0                      # unit diameter, unit density
+    meff = m * m / (m + m)
+    wd = math.sqrt(KN / meff) * math.pi / math.sqrt(math.pi ** 2 + math.log(e) ** 2)
+    assert abs(info.tcoll - math.pi / wd) <= 1e-13 * info.tcoll
+    assert info.dtsolid == info.tcoll / 50.0
+
+
+@pytest.mark.parametrize("e", [0.5, 0.9])
+def test_head_on_restitution_converges_to_the_input(orc, e):
+    errs = []
+    for refine in (1, 10):
+        d = head_on(orc, e)
+        info = d.state()[4]
+        dts = info.dtsolid / refine
+        e_num, tc = collide(d, dts)
+        errs.append(abs(e_num - e))
+        assert abs(tc - info.tcoll) <= 2 * dts
+        pos, v, _, _, _ = d.state()
+        assert v[0, 0] + v[1, 0] == 0.0        # equal masses: the pair forces cancel exactly
+    assert errs[1] < 1e-3 and errs[1] < errs[0] / 5   # first order in the sub-step
+
+
+def test_momentum_balance_in_a_polydisperse_bed(orc):
+    xc, rad, vel = random_bed(250, 8.0, rad_range=(6.0, 10.0), overlap=0.01, vel=1.0, seed=3)
+    d = odem(orc, xc, rad, xlength=8.0, ylength=8.0, zlength=8.0, velbdy=vel, coeff_rest=0.7,
+             only_dem=True)
+    r = rad * DX
+    m = (4.0 / 3.0) * math.pi * r ** 3
+    pos, v, _, _, info = d.state()
+    p0 = (m[:, None] * v).sum(0)
+    scale = (m[:, None] * np.abs(v)).sum()
+    contacts = 0
+    for _ in range(100):
+        d.march(info.dtsolid)
+        pos, v, _, _, info = d.state()
+        contacts += info.ncontacts
+    assert contacts > 100
+    assert np.max(np.abs((m[:, None] * v).sum(0) - p0)) < 1e-12 * scale
+    assert info.substeps == 100
+
+
+def test_free_flight_with_hydrodynamic_terms_and_wrap(orc):
+    box = 6.0
+    xc = np.array([[(box - 1e-4) / DX + 1, 5e-5 / DX + 1, 3.0 / DX + 1]])
+    v0 = np.array([[1.0, -0.5, 0.0]])
+    rhof, rhos = 1.5, 2.0
+    pres, visc = np.array([[0.3, -0.2, 0.1]]), np.array([[0.05, 0.1, -0.07]])
+    mpg, fa = np.array([0.4, 0.0, 0.1]), np.array([0.2, 0.1, 0.0])
+    d = odem(orc, xc, np.array([8.0]), xlength=box, ylength=box, zlength=box, velbdy=v0,
+             rhof=rhof, rhos=rhos)
+    pos0, _, _, _, info = d.state()
+    dt = 7.5 * info.dtsolid
+    xcn, vn, angv = d.march(dt, pres, visc, mpg, fa)
+    pos, v, _, _, info = d.state()
+    F, dts = 7, info.dtsolid
+    assert info.factor == F and info.dtsolid_used == dts
+    r = 8.0 * DX
+    vol = (4.0 / 3.0) * math.pi * r ** 3
+    acc = rhof * (pres + visc)[0] / (vol * rhos) - fa - mpg / rhos
+    assert np.allclose(v[0], v0[0] + F * dts * acc, rtol=0, atol=1e-13)
+    x = pos0[0] + F * dts * v0[0] + dts * dts * acc * F * (F + 1) / 2
+    x = np.mod(x, box)
+    assert np.all((pos[0] >= 0) & (pos[0] <= box))
+    assert np.allclose(pos[0], x, rtol=0, atol=1e-12)
+    assert pos[0, 0] < 0.01 and pos[0, 1] > box - 0.01      # both wrapped
+    # IBMUPDATE
+    assert np.array_equal(xcn[0], pos[0] / DX + 1.0)
+    assert np.array_equal(vn, v) and not angv.any()
+
+
+def test_ibmupdate_applies_foffset(orc):
+    xc, rad, vel = random_bed(20, 6.0, seed=4)
+    d = odem(orc, xc, rad, xlength=6.0, ylength=6.0, zlength=6.0, velbdy=vel, foffset=0.25,
+             only_dem=True)
+    xcn, _, _ = d.march(3 * d.state()[4].dtsolid)
+    pos = d.state()[0]
+    assert np.array_equal(xcn[:, 0], pos[:, 0] / DX + 1.0 - 0.25)
+    assert np.array_equal(xcn[:, 1:], pos[:, 1:] / DX + 1.0)
+
+
+@pytest.mark.parametrize("box,n", [(8.0, 250), (4.6, 30), (6.0, 80)])  # 4, 2 and 3 cells
+def test_neighbour_sets_match_brute_force_over_images(orc, box, n):
+    xc, rad, vel = random_bed(n, box, rad_range=(6.0, 9.0), overlap=0.0, seed=5)
+    d = odem(orc, xc, rad, xlength=box, ylength=box, zlength=box, velbdy=vel, only_dem=True)
+    import oracle
+    for it in range(3):
+        pos, _, _, _, info = d.state()
+        L = np.array(info.box)
+        r = rad * DX
+        got = oracle.dem_neighbours(d)
+        shifts = np.array(list(np.ndindex(3, 3, 3))) - 1.0
+        for l in range(n):
+            dist = np.sqrt(((pos[None, :, :] + shifts[:, None, :] * L - pos[l]) ** 2).sum(2)).min(0)
+            hit = 1.2 * (r[l] + r) - dist > SMALL
+            hit[l] = False
+            assert got[l] == list(np.nonzero(hit)[0]), l
+        d.march(40 * info.dtsolid)
+
+
+def test_sub_step_count(orc):
+    xc, rad, vel = random_bed(10, 6.0, seed=6)
+    d = odem(orc, xc, rad, xlength=6.0, ylength=6.0, zlength=6.0, velbdy=vel, only_dem=True)
+    info = d.state()[4]
+    dts = info.dtsolid
+    d.march(3.7 * dts)
+    i2 = d.state()[4]
+    assert i2.factor == 3 and i2.dtsolid_used == dts and i2.substeps == 3
+    d.march(0.4 * dts)
+    i3 = d.state()[4]
+    assert i3.factor == 1 and i3.dtsolid_used == 0.4 * dts and i3.substeps == 4
+    assert abs(i3.s_time - (3 * dts + 0.4 * dts)) < 1e-18
+
+
+def test_error_paths(orc):
+    xc, rad, vel = random_bed(10, 6.0, seed=7)
+    d = odem(orc, xc, rad, xlength=6.0, ylength=6.0, zlength=6.0, velbdy=vel, only_dem=False)
+    with pytest.raises(solver.PibError) as ei:
+        d.march(1e-4)                                  # hydrodynamic terms missing
+    assert ei.value.code == _abi.PIB_ERR_ARG
+    fast = vel * 0
+    fast[0, 0] = 1e5
+    d = odem(orc, xc, rad, xlength=6.0, ylength=6.0, zlength=6.0, velbdy=fast, only_dem=True)
+    with pytest.raises(solver.PibError) as ei:
+        d.march(d.state()[4].dtsolid)
+    assert ei.value.code == _abi.PIB_ERR_MOTION
+    with pytest.raises(solver.PibError) as ei:
+        odem(orc, xc, rad, xlength=3.9, ylength=6.0, zlength=6.0, only_dem=True)
+    assert ei.value.code == _abi.PIB_ERR_UNSUPPORTED
+    bad = xc.copy()
+    bad[0, 1] = 6.5 / DX + 1
+    with pytest.raises(solver.PibError) as ei:
+        odem(orc, bad, rad, xlength=6.0, ylength=6.0, zlength=6.0, only_dem=True)
+    assert ei.value.code == _abi.PIB_ERR_ARG

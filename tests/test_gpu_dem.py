@@ -1,0 +1,121 @@
+"""GPU parity of the soft-sphere DEM (include/pureibm_dem.h) against the oracle
+restatement of des_time_march (oracle/pib_dem_oracle.c), on the same seeded
+beds: positions, velocities, IBMUPDATE outputs, contact forces and the
+contact diagnostics after each march.  The GPU path sums every particle's
+contact force in the reference's order with FP contraction off and the
+device's correctly rounded sqrt and division, so positions and velocities
+must agree bit for bit; the outputs derived from them are held to 1e-13
+relative to the box."""
+import numpy as np
+import pytest
+
+from pureibm_ps2d_fd1d_amd import _abi, solver
+from dem_beds import random_bed
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-13
+
+
+def pair(orc, xc, rad, vel, **kw):
+    g = solver.Dem(xc, rad, velbdy=vel, **kw)
+    o = solver.Dem(xc, rad, velbdy=vel, lib=orc.load(), prefix="orc_", **kw)
+    return g, o
+
+
+def compare(g, o, out_g, out_o, box):
+    for a, b in zip(out_g, out_o):
+        assert np.max(np.abs(a - b)) <= TOL * box
+    sg, so = g.state(), o.state()
+    for a, b in zip(sg[:4], so[:4]):
+        assert np.max(np.abs(a - b)) <= TOL * max(1.0, np.max(np.abs(b)))
+    ig, io = sg[4], so[4]
+    assert ig.ncontacts == io.ncontacts and ig.factor == io.factor and ig.substeps == io.substeps
+    assert abs(ig.overlap_max - io.overlap_max) <= 1e-9 * max(io.overlap_max, 1.0)
+    assert ig.dtsolid == io.dtsolid and ig.tcoll == io.tcoll
+    assert list(ig.cells) == list(io.cells) and list(ig.box) == list(io.box)
+    return all(np.array_equal(a, b) for a, b in zip(sg[:2], so[:2]))
+
+
+CASES = [
+    # n, box, dx, radii (grid units), only_dem, seed
+    (300, 8.0, 1.0 / 16, (8.0, 8.0), True, 11),
+    (250, 8.0, 1.0 / 16, (6.0, 10.0), False, 12),
+    (40, 4.6, 1.0 / 16, (6.0, 9.0), True, 13),        # two cells a direction: repeated images
+    (1500, 14.0, 1.0 / 16, (7.0, 9.0), False, 14),    # three particles a thread
+    (1200, 12.8, 1.0 / 20, (10.0, 10.0), False, 15),  # 256^3 bed of d = 20 cells
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"n{c[0]}_box{c[1]}")
+def test_dem_matches_oracle(orc, case):
+    n, box, dx, rr, only_dem, seed = case
+    print(f"DEM parity n={n} box={box} seed={seed}", flush=True)
+    xc, rad, vel = random_bed(n, box, dx=dx, rad_range=rr, overlap=0.01, vel=1.0, seed=seed)
+    kw = dict(dx=dx, xlength=box, ylength=box, zlength=box, coeff_rest=0.8, only_dem=only_dem,
+              rhof=1.2, rhos=2.5, foffset=0.0)
+    g, o = pair(orc, xc, rad, vel, **kw)
+    rng = np.random.default_rng(seed)
+    dts = g.state()[4].dtsolid
+    exact = True
+    for k, mult in enumerate([20, 20, 0.5, 37, 20, 60]):
+        hyd = {}
+        if not only_dem:
+            hyd = dict(pres=rng.normal(0, 0.05, (n, 3)), visc=rng.normal(0, 0.05, (n, 3)),
+                       mpg=rng.normal(0, 0.1, 3), frame_accln=rng.normal(0, 0.1, 3))
+        out_g = g.march(mult * dts, **hyd)
+        out_o = o.march(mult * dts, **hyd)
+        exact &= compare(g, o, out_g, out_o, box)
+    print(f"bitwise equal: {exact}", flush=True)
+    assert exact
+    g.close()
+    o.close()
+
+
+def test_gpu_head_on_restitution(orc):
+    dx = 1.0 / 16
+    xc = np.array([[1.5 / dx + 1, 2.5 / dx + 1, 2.5 / dx + 1], [2.51 / dx + 1, 2.5 / dx + 1, 2.5 / dx + 1]])
+    vel = np.array([[1.0, 0.0, 0.0], [-1.0, 0.0, 0.0]])
+    e = 0.6
+    g = solver.Dem(xc, np.array([8.0, 8.0]), dx=dx, xlength=5.0, ylength=5.0, zlength=5.0, velbdy=vel,
+                   coeff_rest=e, only_dem=True)
+    info = g.state()[4]
+    # the whole collision inside one march: separation 0.01 at closing speed 2
+    g.march(0.005 + 2 * info.tcoll)
+    v = g.state()[1]
+    assert abs((v[1, 0] - v[0, 0]) / 2.0 - e) < 0.01
+    assert v[0, 0] + v[1, 0] == 0.0
+    g.close()
+
+
+def test_gpu_motion_error_is_reported():
+    dx = 1.0 / 16
+    xc, rad, vel = random_bed(10, 6.0, seed=7)
+    fast = vel * 0
+    fast[0, 0] = 1e5
+    g = solver.Dem(xc, rad, dx=dx, xlength=6.0, ylength=6.0, zlength=6.0, velbdy=fast, only_dem=True)
+    with pytest.raises(solver.PibError) as ei:
+        g.march(g.state()[4].dtsolid)
+    assert ei.value.code == _abi.PIB_ERR_MOTION
+    g.close()
+
+
+def test_gpu_set_state_restarts_the_sub_cycle(orc):
+    """A march after set_state equals a fresh context created at that state."""
+    dx = 1.0 / 16
+    xc, rad, vel = random_bed(200, 8.0, seed=21, vel=1.0)
+    kw = dict(dx=dx, xlength=8.0, ylength=8.0, zlength=8.0, only_dem=True)
+    g = solver.Dem(xc, rad, velbdy=vel, **kw)
+    dts = g.state()[4].dtsolid
+    g.march(30 * dts)
+    pos, v = g.state()[:2]
+    h = solver.Dem(xc, rad, velbdy=vel, **kw)
+    o = solver.Dem(xc, rad, velbdy=vel, lib=orc.load(), prefix="orc_", **kw)
+    for d in (g, h, o):
+        d.set_state(pos, v)
+    a, b, c = g.march(25 * dts), h.march(25 * dts), o.march(25 * dts)
+    for x, y, z in zip(a, b, c):
+        assert np.array_equal(x, y)
+        assert np.max(np.abs(x - z)) <= TOL * 8.0
+    for d in (g, h, o):
+        d.close()
