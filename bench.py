@@ -215,6 +215,50 @@ def run_batched(nreal, steps, warmup):
             "sizes": [f"{st.nx}^3/{st.nbody}" for st in sts]}
 
 
+def run_dem(marches, per_march, cpu_substeps):
+    """Soft-sphere DEM sub-cycle at the 256^3 bed size: 1200 spheres of 20
+    cells (dx = 1/20, box 12.8, solid fraction 0.30), coeff_rest 0.8, random
+    velocities in [-0.5, 0.5] and fixed random hydrodynamic terms.  Each march
+    is des_time_march with dt = per_march*DTSOLID; the GPU figure is wall time
+    over the timed marches (host copies of XC/VELBDY included), the CPU figure
+    the oracle on the same bed for cpu_substeps sub-steps."""
+    n, box, dx = 1200, 12.8, 1.0 / 20
+    xc, rad, vel = configs.moving_bed(n, box, dx, (10.0, 10.0), overlap=0.0, vel=0.5, seed=2024)
+    rng = np.random.default_rng(7)
+    hyd = dict(pres=rng.normal(0, 0.05, (n, 3)), visc=rng.normal(0, 0.05, (n, 3)),
+               mpg=np.array([0.1, 0.0, 0.0]), frame_accln=np.zeros(3))
+    kw = dict(dx=dx, xlength=box, ylength=box, zlength=box, coeff_rest=0.8, rhos=2.0, rhof=1.0)
+    g = solver.Dem(xc, rad, velbdy=vel, **kw)
+    dts = g.state()[4].dtsolid
+    g.march(per_march * dts, **hyd)
+    r0 = g.state()[4].rebuilds
+    t0 = time.perf_counter()
+    for _ in range(marches):
+        g.march(per_march * dts, **hyd)
+    wall = time.perf_counter() - t0
+    info = g.state()[4]
+    g.close()
+    nsub = marches * per_march
+    out = {"workload": "1200 spheres d=20 cells in a 12.8^3 periodic box (256^3 grid), phi=0.30",
+           "substeps": nsub, "us_per_substep": wall / nsub * 1e6,
+           "value": n * nsub / wall / 1e6, "unit": "M particle-substeps/s",
+           "contacts_last_substep": info.ncontacts, "candidate_rebuilds": info.rebuilds - r0,
+           "dtsolid": dts}
+    if cpu_substeps:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        oracle.build()
+        o = solver.Dem(xc, rad, velbdy=vel, lib=oracle.load(), prefix="orc_", **kw)
+        t0 = time.perf_counter()
+        o.march(cpu_substeps * dts, **hyd)
+        tc = time.perf_counter() - t0
+        o.close()
+        out["cpu_baseline"] = {"value": n * cpu_substeps / tc / 1e6, "unit": "M particle-substeps/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"{cpu_substeps} sub-steps of the same bed on the CPU oracle, {tc:.2f} s"}
+    return out
+
+
 def cpu_baseline(setup, nsteps):
     """Single-core oracle on the same workload (bounded sample: one untimed
     first step, then nsteps timed steps)."""
@@ -244,6 +288,8 @@ def main():
                     help="only run 3 device-to-device copies of MIB MiB (scale for FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-realization figure")
     ap.add_argument("--batched-steps", type=int, default=100)
+    ap.add_argument("--no-dem", action="store_true", help="skip the DEM sub-cycle figure")
+    ap.add_argument("--dem-marches", type=int, default=10)
     ap.add_argument("--collect-traffic", action="store_true",
                     help="measure HBM bytes per dispatch with rocprofv3 counter passes (child processes)")
     a = ap.parse_args()
@@ -323,6 +369,11 @@ def main():
             out["batched_c3"] = run_batched(12, a.batched_steps, 5)
         except Exception as e:  # reported, never fatal for the headline figure
             out["batched_c3"] = {"error": str(e)}
+    if not a.no_dem:
+        try:
+            out["dem"] = run_dem(a.dem_marches, 500, 0 if a.no_cpu else 2000)
+        except Exception as e:  # reported, never fatal for the headline figure
+            out["dem"] = {"error": str(e)}
     if not a.no_cpu:
         ncpu = max(1, a.cpu_steps)
         tcpu = cpu_baseline(st64, ncpu)

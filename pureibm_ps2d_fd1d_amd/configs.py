@@ -151,3 +151,28 @@ def write_workload(directory, name, **over):
     if kind == "simple":
         return simple_cubic(directory, name, **w)
     return random_bed(directory, name, **w)
+
+
+def moving_bed(n, box, dx, rad_range, overlap=0.01, vel=0.5, seed=0):
+    """Freely moving bed for the soft-sphere DEM: n spheres placed by random
+    sequential addition in a periodic cube of side ``box`` (physical units),
+    pairs allowed to overlap by ``overlap`` of r_a + r_b so that contacts exist
+    from the start, velocities uniform in [-vel, vel].  Returns xc (n,3) and
+    radbdy (n,) in grid units of spacing dx, and velbdy (n,3)."""
+    rng = np.random.default_rng(seed)
+    rad = rng.uniform(rad_range[0], rad_range[1], n)
+    pos = np.zeros((n, 3))
+    k = tries = 0
+    while k < n:
+        tries += 1
+        if tries > 200 * n:
+            raise RuntimeError("bed too dense for random sequential addition")
+        p = rng.uniform(0.0, box, 3)
+        if k:
+            d = pos[:k] - p
+            d -= box * np.round(d / box)
+            if np.any(np.sqrt((d * d).sum(1)) < (1.0 - overlap) * (rad[:k] + rad[k]) * dx):
+                continue
+        pos[k] = p
+        k += 1
+    return np.asfortranarray(pos / dx + 1.0), rad, np.asfortranarray(rng.uniform(-vel, vel, (n, 3)))

@@ -7,11 +7,14 @@
 // and then moves every particle (CFNEWVALUES :1335-1462).
 //
 // Execution: one workgroup of 512 threads runs a whole chunk of sub-steps
-// with the particle positions, velocities, radii and collision masses
-// resident in LDS (68 bytes a particle, up to 2048 particles), two barriers a
-// sub-step.  Per-particle work is a few dozen dependent FP64 operations, so a
-// single CU with no round trip to the host between sub-steps beats any
-// multi-workgroup split that would need a grid-wide barrier per sub-step.
+// with the particle positions, velocities and radii resident in LDS (56 bytes
+// a particle, up to 2048 particles) beside 16-bit candidate lists in the room
+// left, and each particle's contact-free acceleration in registers; two
+// barriers a sub-step.  Per-particle work is a few dozen dependent FP64
+// operations, so a single CU with no round trip to the host between sub-steps
+// beats any multi-workgroup split that would need a grid-wide barrier per
+// sub-step.  Global memory is touched only for contacts (collision mass,
+// hydrodynamic terms) and at the start and end of a launch.
 //
 // Exactness.  The reference sums FC(p) in a fixed order: first -FN(l,p) for
 // the contacting l < p in increasing l (the outer particle loop of
@@ -46,7 +49,8 @@ namespace {
 constexpr int DEM_NT = 512;      // threads of the one workgroup (256 VGPRs a lane)
 constexpr int DEM_NMAX = 2048;   // particles held in LDS
 constexpr int DEM_PPT = DEM_NMAX / DEM_NT;  // particles a thread
-constexpr int DEM_VCAP = 48;     // candidate partners per particle
+constexpr int DEM_VCAP = 32;     // candidate partners per particle, at most
+constexpr int DEM_LDS = 159 * 1024;  // LDS a workgroup may take on gfx950 (160 KB), less a margin
 constexpr int DEM_HCAP = 16;     // contacts with higher-numbered partners per particle
 constexpr int DEM_CHUNK = 4096;  // sub-steps per launch
 constexpr double SMALL_NUMBER = (double)1.0E-35f;  // constants.F90:92
@@ -56,7 +60,7 @@ enum { ERR_MOTION = 1, ERR_VCAP = 2, ERR_HCAP = 4 };
 
 struct DemArgs {
   int n, nsub, only_dem, rebuild;
-  int imax1, jmax1, kmax1, pad_;
+  int imax1, jmax1, kmax1, cap;  // cap: candidate slots a particle (LDS lists)
   double dxc, dyc, dzc;   // coarse cell sizes
   double lx, ly, lz;      // XE(IMAX1)-XE(1) etc., the shift of the cell scan
   double bx, by, bz;      // XLENGTH etc. (EX2-WX1), the shift of CALC_FORCE_DES and CFNEWVALUES
@@ -71,11 +75,12 @@ struct DemArgs {
   const double* pmass;    // PMASS
   const double* t1;       // (3,n) RHOF*(PRES+VISC)/PMASS
   const double* t2;       // (3,n) mpg*PVOL/PMASS
+  const double* a0;       // (3,n) acceleration without contact: ((+0 + t1) - frame_accln) - t2
   double* pos;            // (3,n) DES_POS_NEW
   double* vel;            // (3,n) DES_VEL_NEW
   double* disp;           // (3,n) displacement since the last candidate rebuild
-  int* vl;                // (VCAP, n) candidate partners, ascending
-  int* vcnt;              // n
+  unsigned short* vl;     // (cap, n) candidate partners, ascending, kept between launches
+  unsigned short* vcnt;   // n
   double* cforce;         // (3,n) FC of the last sub-step
   double* xc_out;         // (n,3) IBMUPDATE XC
   int* status;            // [0] error bits, [1] contacts, [2] rebuilds
@@ -104,15 +109,15 @@ __device__ inline int wrap_cell(int ii, int imax1, double& f) {
 }
 
 struct Lds {
-  double *px, *py, *pz, *vx, *vy, *vz, *rad, *cm;
-  int* cell;
+  double *px, *py, *pz, *vx, *vy, *vz, *rad;
+  unsigned short *cl, *cc;  // candidate lists (cap, n) and counts
 };
 
 // Position of b in a's 27-cell scan (0..26) with the first image that passes
 // the neighbour test 1.2*(r_a+r_b) - |d| > SMALL_NUMBER (:955-1003), or -1
 // when b is not a neighbour of a.
 __device__ int scan_pos(const DemArgs& A, const Lds& S, int a, int b) {
-  const int ca = S.cell[a], cb = S.cell[b];
+  const int ca = cell_of(A, S.px[a], S.py[a], S.pz[a]), cb = cell_of(A, S.px[b], S.py[b], S.pz[b]);
   const int ai = ca & 1023, aj = (ca >> 10) & 1023, ak = ca >> 20;
   const int bi = cb & 1023, bj = (cb >> 10) & 1023, bk = cb >> 20;
   int o = 0;
@@ -150,14 +155,17 @@ __device__ inline bool pair_force(const DemArgs& A, const Lds& S, int a, int b, 
   if (fabs(paz - pbz) > A.four_rmax) pbz = pbz > paz ? pbz - A.bz : pbz + A.bz;
   const double rlm = S.rad[a] + S.rad[b];
   const double d0 = pbx - pax, d1 = pby - pay, d2 = pbz - paz;
-  const double dm = sqrt(((0.0 + d0 * d0) + d1 * d1) + d2 * d2);
+  const double dsq = ((0.0 + d0 * d0) + d1 * d1) + d2 * d2;
+  // |d| > r_a + r_b for certain: no contact, and no sqrt needed to say so
+  if (dsq > rlm * rlm * (1.0 + 1.0e-9)) return false;
+  const double dm = sqrt(dsq);
   if (!(rlm - dm > SMALL_NUMBER)) return false;
   ovpct = ((rlm - dm) / rlm) * 100.0;
   const double n0 = d0 / dm, n1 = d1 / dm, n2 = d2 / dm;
   const double vr0 = S.vx[a] - S.vx[b], vr1 = S.vy[a] - S.vy[b], vr2 = S.vz[a] - S.vz[b];
   const double vrn = ((0.0 + vr0 * n0) + vr1 * n1) + vr2 * n2;
   const double ov = rlm - dm;
-  const double ma = S.cm[a], mb = S.cm[b];
+  const double ma = A.cmass[a], mb = A.cmass[b];
   const double meff = (ma * mb) / (ma + mb);
   double eta = 2.0 * sqrt(A.kn * meff) * A.alog;
   eta = eta / A.den;
@@ -181,12 +189,12 @@ __device__ int build_candidates(const DemArgs& A, const Lds& S, int p) {
     d2 = d2 > hz ? d2 - A.bz : (d2 < -hz ? d2 + A.bz : d2);
     const double r = S.rad[p] + S.rad[q] + A.skin;
     if (d0 * d0 + d1 * d1 + d2 * d2 < r * r) {
-      if (cnt < DEM_VCAP) A.vl[(size_t)cnt * n + p] = q;
+      if (cnt < A.cap) S.cl[cnt * n + p] = (unsigned short)q;
       ++cnt;
     }
   }
-  A.vcnt[p] = cnt < DEM_VCAP ? cnt : DEM_VCAP;
-  return cnt > DEM_VCAP ? ERR_VCAP : 0;
+  S.cc[p] = (unsigned short)(cnt < A.cap ? cnt : A.cap);
+  return cnt > A.cap ? ERR_VCAP : 0;
 }
 
 // Insert key into an ascending register array padded with INT_MAX.
@@ -200,22 +208,25 @@ __device__ inline void insert_key(int (&hk)[DEM_HCAP], int key) {
 }
 
 // FC(p) summed in the reference's order; counts the contacts p heads.
-__device__ void contact_sum(const DemArgs& A, const Lds& S, int p, double& f0, double& f1, double& f2, double& ovmax,
+// Returns whether any contact force reached FC(p).
+__device__ bool contact_sum(const DemArgs& A, const Lds& S, int p, double& f0, double& f1, double& f2, double& ovmax,
                             int& ncont, int& err) {
   const int n = A.n;
-  const int cnt = A.vcnt[p];
+  const int cnt = S.cc[p];
   int hk[DEM_HCAP];
 #pragma unroll
   for (int j = 0; j < DEM_HCAP; ++j) hk[j] = INT_MAX;
   int nh = 0;
+  bool touched = false;
   f0 = 0.0; f1 = 0.0; f2 = 0.0;
   for (int t = 0; t < cnt; ++t) {
-    const int q = A.vl[(size_t)t * n + p];
+    const int q = S.cl[t * n + p];
     const int a = q < p ? q : p, b = q < p ? p : q;
     double g0, g1, g2, ov;
     if (!pair_force(A, S, a, b, g0, g1, g2, ov)) continue;
     const int o = scan_pos(A, S, a, b);
     if (o < 0) continue;
+    touched = true;
     if (q < p) {  // FC(I) = FC(I) - FN(L) during particle q's turn (CFFCTOW :873-904)
       f0 = f0 - g0; f1 = f1 - g1; f2 = f2 - g2;
     } else {
@@ -228,7 +239,7 @@ __device__ void contact_sum(const DemArgs& A, const Lds& S, int p, double& f0, d
   }
   // pop the keys in ascending order from the front of the register array
   while (hk[0] != INT_MAX) {
-    const int q = A.vl[(size_t)(hk[0] & 63) * n + p];
+    const int q = S.cl[(hk[0] & 63) * n + p];
 #pragma unroll
     for (int j = 0; j + 1 < DEM_HCAP; ++j) hk[j] = hk[j + 1];
     hk[DEM_HCAP - 1] = INT_MAX;
@@ -236,6 +247,7 @@ __device__ void contact_sum(const DemArgs& A, const Lds& S, int p, double& f0, d
     pair_force(A, S, p, q, g0, g1, g2, ov);
     f0 = f0 + g0; f1 = f1 + g1; f2 = f2 + g2;
   }
+  return touched;
 }
 
 __global__ __launch_bounds__(DEM_NT) void k_dem_march(DemArgs A) {
@@ -244,34 +256,40 @@ __global__ __launch_bounds__(DEM_NT) void k_dem_march(DemArgs A) {
   Lds S;
   S.px = lds_d; S.py = S.px + n; S.pz = S.py + n;
   S.vx = S.pz + n; S.vy = S.vx + n; S.vz = S.vy + n;
-  S.rad = S.vz + n; S.cm = S.rad + n;
-  S.cell = (int*)(S.cm + n);
-  unsigned long long* s_ovmax = (unsigned long long*)(S.cell + n + (n & 1));
+  S.rad = S.vz + n;
+  unsigned long long* s_ovmax = (unsigned long long*)(S.rad + n);
+  S.cc = (unsigned short*)(s_ovmax + 2);
+  S.cl = S.cc + n;
   int* s_red = (int*)(s_ovmax + 1);  // [0] contacts, [1] rebuilds
   if (tid == 0) { *s_ovmax = 0ull; s_red[0] = 0; s_red[1] = 0; }
   for (int p = tid; p < n; p += DEM_NT) {
     S.px[p] = A.pos[p]; S.py[p] = A.pos[n + p]; S.pz[p] = A.pos[2 * n + p];
     S.vx[p] = A.vel[p]; S.vy[p] = A.vel[n + p]; S.vz[p] = A.vel[2 * n + p];
-    S.rad[p] = A.rad[p]; S.cm[p] = A.cmass[p];
-    S.cell[p] = cell_of(A, S.px[p], S.py[p], S.pz[p]);
+    S.rad[p] = A.rad[p];
+    if (!A.rebuild) {
+      S.cc[p] = A.vcnt[p];
+      for (int t = 0; t < A.cap; ++t) S.cl[t * n + p] = A.vl[(size_t)t * n + p];
+    }
   }
   __syncthreads();
   int err = 0, nreb = 0;
-  double dsp[DEM_PPT][3];
+  double dsp[DEM_PPT][3], a0r[DEM_PPT][3];
 #pragma unroll
   for (int k = 0; k < DEM_PPT; ++k) {
     const int p = tid + k * DEM_NT;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) dsp[k][c] = p < n ? A.disp[c * n + p] : 0.0;
+    for (int c = 0; c < 3; ++c) {
+      dsp[k][c] = p < n ? A.disp[c * n + p] : 0.0;
+      a0r[k][c] = p < n ? A.a0[c * n + p] : 0.0;
+    }
   }
   if (A.rebuild) {
+#pragma unroll 1
+    for (int p = tid; p < n; p += DEM_NT) err |= build_candidates(A, S, p);
 #pragma unroll
-    for (int k = 0; k < DEM_PPT; ++k) {
-      const int p = tid + k * DEM_NT;
-      if (p < n) err |= build_candidates(A, S, p);
+    for (int k = 0; k < DEM_PPT; ++k)
 #pragma unroll
       for (int c = 0; c < 3; ++c) dsp[k][c] = 0.0;
-    }
     ++nreb;
     __syncthreads();
   }
@@ -283,12 +301,13 @@ __global__ __launch_bounds__(DEM_NT) void k_dem_march(DemArgs A) {
     for (int k = 0; k < DEM_PPT; ++k) f[k][0] = f[k][1] = f[k][2] = 0.0;
     ovmax = 0.0;
     ncont = 0;
+    int touched = 0;  // bit k: a contact force reached particle k of this thread
 #pragma unroll 1
     for (int k = 0; k < DEM_PPT; ++k) {
       const int p = tid + k * DEM_NT;
       if (p >= n) break;
       double g0, g1, g2;
-      contact_sum(A, S, p, g0, g1, g2, ovmax, ncont, err);
+      if (contact_sum(A, S, p, g0, g1, g2, ovmax, ncont, err)) touched |= 1 << k;
 #pragma unroll
       for (int kk = 0; kk < DEM_PPT; ++kk)
         if (kk == k) { f[kk][0] = g0; f[kk][1] = g1; f[kk][2] = g2; }
@@ -300,21 +319,26 @@ __global__ __launch_bounds__(DEM_NT) void k_dem_march(DemArgs A) {
     for (int k = 0; k < DEM_PPT; ++k) {
       const int p = tid + k * DEM_NT;
       if (p >= n) break;
-      double g[3], ds[3];
+      double g[3], ds[3], a0[3];
 #pragma unroll
       for (int kk = 0; kk < DEM_PPT; ++kk)
         if (kk == k)
-          for (int c = 0; c < 3; ++c) { g[c] = f[kk][c]; ds[c] = dsp[kk][c]; }
+          for (int c = 0; c < 3; ++c) { g[c] = f[kk][c]; ds[c] = dsp[kk][c]; a0[c] = a0r[kk][c]; }
       // CFNEWVALUES (collision_mod.F90:1361-1459)
-      const double pm = A.pmass[p];
+      const bool hit = (touched >> k) & 1;
+      const double pm = hit ? A.pmass[p] : 1.0;
       double* sp[3] = {S.px, S.py, S.pz};
       double* sv[3] = {S.vx, S.vy, S.vz};
       const double box[3] = {A.bx, A.by, A.bz};
       double dd[3], pn[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        double acc = g[c] / pm;
-        if (!A.only_dem) acc = acc + A.t1[c * n + p] - A.fa[c] - A.t2[c * n + p];
+        // without a contact FC is +0, +0/PMASS is +0 and the sum is a0
+        double acc = a0[c];
+        if (hit) {
+          acc = g[c] / pm;
+          if (!A.only_dem) acc = acc + A.t1[c * n + p] - A.fa[c] - A.t2[c * n + p];
+        }
         const double vn = sv[c][p] + acc * A.dts;
         const double po = sp[c][p];
         pn[c] = po + vn * A.dts;
@@ -322,7 +346,9 @@ __global__ __launch_bounds__(DEM_NT) void k_dem_march(DemArgs A) {
         sv[c][p] = vn;
         if (last) A.cforce[c * n + p] = g[c];
       }
-      if (sqrt(((0.0 + dd[0] * dd[0]) + dd[1] * dd[1]) + dd[2] * dd[2]) >= S.rad[p]) err |= ERR_MOTION;
+      const double dsq = ((0.0 + dd[0] * dd[0]) + dd[1] * dd[1]) + dd[2] * dd[2];
+      const double rp = S.rad[p];
+      if (dsq >= rp * rp * (1.0 - 1.0e-9) && sqrt(dsq) >= rp) err |= ERR_MOTION;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         if (pn[c] > box[c]) pn[c] = pn[c] - box[c];
@@ -330,7 +356,6 @@ __global__ __launch_bounds__(DEM_NT) void k_dem_march(DemArgs A) {
         sp[c][p] = pn[c];
         ds[c] = ds[c] + dd[c];
       }
-      S.cell[p] = cell_of(A, pn[0], pn[1], pn[2]);
       need |= ds[0] * ds[0] + ds[1] * ds[1] + ds[2] * ds[2] > A.half_skin2;
 #pragma unroll
       for (int kk = 0; kk < DEM_PPT; ++kk)
@@ -338,13 +363,12 @@ __global__ __launch_bounds__(DEM_NT) void k_dem_march(DemArgs A) {
           for (int c = 0; c < 3; ++c) dsp[kk][c] = ds[c];
     }
     if (__syncthreads_or(need)) {
+#pragma unroll 1
+      for (int p = tid; p < n; p += DEM_NT) err |= build_candidates(A, S, p);
 #pragma unroll
-      for (int k = 0; k < DEM_PPT; ++k) {
-        const int p = tid + k * DEM_NT;
-        if (p < n) err |= build_candidates(A, S, p);
+      for (int k = 0; k < DEM_PPT; ++k)
 #pragma unroll
         for (int c = 0; c < 3; ++c) dsp[k][c] = 0.0;
-      }
       ++nreb;
       __syncthreads();
     }
@@ -358,6 +382,8 @@ __global__ __launch_bounds__(DEM_NT) void k_dem_march(DemArgs A) {
     A.vel[p] = S.vx[p]; A.vel[n + p] = S.vy[p]; A.vel[2 * n + p] = S.vz[p];
 #pragma unroll
     for (int c = 0; c < 3; ++c) A.disp[c * n + p] = dsp[k][c];
+    A.vcnt[p] = S.cc[p];
+    for (int t = 0; t < A.cap; ++t) A.vl[(size_t)t * n + p] = S.cl[t * n + p];
     A.xc_out[p] = S.px[p] / A.dx + 1.0 - A.foffset;
     A.xc_out[n + p] = S.py[p] / A.dy + 1.0;
     A.xc_out[2 * n + p] = S.pz[p] / A.dz + 1.0;
@@ -390,7 +416,10 @@ struct DemCtx {
   std::vector<double> pmass_h, pvol_h;
   double *rad = nullptr, *cmass = nullptr, *pmass = nullptr, *t1 = nullptr, *t2 = nullptr;
   double *pos = nullptr, *vel = nullptr, *disp = nullptr, *cforce = nullptr, *xco = nullptr, *diag = nullptr;
-  int *vl = nullptr, *vcnt = nullptr, *status = nullptr;
+  double* a0 = nullptr;
+  unsigned short *vl = nullptr, *vcnt = nullptr;
+  int* status = nullptr;
+  int cap = 0;
 };
 
 struct DemFail {
@@ -416,7 +445,14 @@ struct DemFail {
 
 int nint_(double x) { return (int)(x >= 0 ? std::floor(x + 0.5) : -std::floor(-x + 0.5)); }
 
-size_t lds_bytes(int n) { return (size_t)n * (8 * sizeof(double) + sizeof(int)) + 8 + 16 + 16; }
+// candidate slots that fit in LDS beside 7 doubles a particle
+int cand_cap(int n) {
+  long long room = (long long)DEM_LDS - 7LL * 8 * n - 2LL * n - 64;
+  long long c = room / (2LL * n);
+  return (int)(c < DEM_VCAP ? c : DEM_VCAP);
+}
+
+size_t lds_bytes(int n, int cap) { return (size_t)n * 7 * sizeof(double) + 16 + 2 * (size_t)n * (1 + cap) + 64; }
 
 template <typename T>
 T* dalloc(size_t count) {
@@ -428,7 +464,7 @@ T* dalloc(size_t count) {
 
 void dem_free(DemCtx* D) {
   void* ps[] = {D->rad, D->cmass, D->pmass, D->t1, D->t2, D->pos, D->vel, D->disp, D->cforce, D->xco,
-                D->diag, D->vl, D->vcnt, D->status};
+                D->diag, D->vl, D->vcnt, D->status, D->a0};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (D->stream) (void)hipStreamDestroy(D->stream);
@@ -524,8 +560,11 @@ void do_create(DemCtx* D, void* va) {
   D->cforce = dalloc<double>(3 * (size_t)n);
   D->xco = dalloc<double>(3 * (size_t)n);
   D->diag = dalloc<double>(4);
-  D->vl = dalloc<int>((size_t)DEM_VCAP * n);
-  D->vcnt = dalloc<int>(n);
+  D->cap = cand_cap(n);
+  if (D->cap < 8) dem_fail(PIB_ERR_UNSUPPORTED, "too many particles for the LDS candidate lists");
+  D->vl = dalloc<unsigned short>((size_t)D->cap * n);
+  D->vcnt = dalloc<unsigned short>(n);
+  D->a0 = dalloc<double>(3 * (size_t)n);
   D->status = dalloc<int>(4);
   DEMCHK(hipMemcpy(D->rad, rad.data(), n * sizeof(double), hipMemcpyHostToDevice));
   DEMCHK(hipMemcpy(D->cmass, cm.data(), n * sizeof(double), hipMemcpyHostToDevice));
@@ -533,7 +572,7 @@ void do_create(DemCtx* D, void* va) {
   DEMCHK(hipMemcpy(D->pos, pos.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
   DEMCHK(hipMemcpy(D->vel, vel.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
   DEMCHK(hipFuncSetAttribute((const void*)k_dem_march, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)lds_bytes(DEM_NMAX)));
+                             (int)DEM_LDS));
   D->need_rebuild = true;
 }
 
@@ -574,28 +613,32 @@ void do_march(DemCtx* D, void* va) {
   A.dx = c.dx; A.dy = c.dy; A.dz = c.dz; A.foffset = c.foffset;
   A.rad = D->rad; A.cmass = D->cmass; A.pmass = D->pmass; A.t1 = D->t1; A.t2 = D->t2;
   A.pos = D->pos; A.vel = D->vel; A.disp = D->disp; A.vl = D->vl; A.vcnt = D->vcnt;
+  A.cap = D->cap; A.a0 = D->a0;
   A.cforce = D->cforce; A.xc_out = D->xco; A.status = D->status; A.diag = D->diag;
+  // CFNEWVALUES :1374, the terms that stay fixed over the sub-cycle, and
+  // the whole acceleration of a particle without contact
+  std::vector<double> t1(3 * (size_t)n, 0.0), t2(3 * (size_t)n, 0.0), a0(3 * (size_t)n, 0.0);
   if (!c.only_dem) {
-    // CFNEWVALUES :1374, the terms that stay fixed over the sub-cycle
-    std::vector<double> t1(3 * (size_t)n), t2(3 * (size_t)n);
     for (int d = 0; d < 3; ++d) {
       A.fa[d] = a->fa[d];
       for (int l = 0; l < n; ++l) {
         const size_t i = (size_t)d * n + l;
         t1[i] = c.rhof * (a->pres[i] + a->visc[i]) / D->pmass_h[l];
         t2[i] = a->mpg[d] * D->pvol_h[l] / D->pmass_h[l];
+        a0[i] = 0.0 + t1[i] - a->fa[d] - t2[i];
       }
     }
     DEMCHK(hipMemcpyAsync(D->t1, t1.data(), t1.size() * sizeof(double), hipMemcpyHostToDevice, D->stream));
     DEMCHK(hipMemcpyAsync(D->t2, t2.data(), t2.size() * sizeof(double), hipMemcpyHostToDevice, D->stream));
   }
+  DEMCHK(hipMemcpyAsync(D->a0, a0.data(), a0.size() * sizeof(double), hipMemcpyHostToDevice, D->stream));
   int zero[4] = {0, 0, 0, 0};
   DEMCHK(hipMemcpyAsync(D->status, zero, sizeof zero, hipMemcpyHostToDevice, D->stream));
   for (int done = 0; done < D->factor; done += DEM_CHUNK) {
     A.nsub = std::min(DEM_CHUNK, D->factor - done);
     A.rebuild = D->need_rebuild ? 1 : 0;
     D->need_rebuild = false;
-    hipLaunchKernelGGL(k_dem_march, dim3(1), dim3(DEM_NT), lds_bytes(n), D->stream, A);
+    hipLaunchKernelGGL(k_dem_march, dim3(1), dim3(DEM_NT), lds_bytes(n, D->cap), D->stream, A);
     DEMCHK(hipGetLastError());
   }
   int st[4];
@@ -613,7 +656,7 @@ void do_march(DemCtx* D, void* va) {
   D->overlap_max = dg[0];
   if (st[0] & ERR_MOTION)
     dem_fail(PIB_ERR_MOTION, "MOVEMENT UNDESIRED: a particle moved by its radius in one sub-step");
-  if (st[0] & ERR_VCAP) dem_fail(PIB_ERR_CAPACITY, "more than 48 candidate partners of one particle");
+  if (st[0] & ERR_VCAP) dem_fail(PIB_ERR_CAPACITY, "more candidate partners of one particle than the LDS lists hold");
   if (st[0] & ERR_HCAP) dem_fail(PIB_ERR_CAPACITY, "more than 16 contacts of one particle with higher-numbered ones");
 }
 
