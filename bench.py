@@ -259,6 +259,24 @@ def run_dem(marches, per_march, cpu_substeps):
     return out
 
 
+def time_ibm_update(setup, reps=5):
+    """Per-step cost of moving the bed in the fluid solver: the solid mask,
+    neighbour sets, active reversal pairs and spreading records re-derived on
+    the device for shifted centres (grid_nodes_insphere + update_nrpr_array,
+    as bcset runs them after des_time_march)."""
+    g = solver.Solver(setup)
+    rng = np.random.default_rng(3)
+    ts = []
+    for _ in range(reps):
+        xc = setup.xc + rng.uniform(-0.2, 0.2, setup.xc.shape)
+        t0 = time.perf_counter()
+        g.update_particles(xc)
+        ts.append(time.perf_counter() - t0)
+    g.close()
+    return {"bed": f"{setup.nx}^3, {setup.nbody} spheres, {setup.nbnd} surface points each",
+            "ms": 1e3 * float(np.median(ts))}
+
+
 def cpu_baseline(setup, nsteps):
     """Single-core oracle on the same workload (bounded sample: one untimed
     first step, then nsteps timed steps)."""
@@ -372,6 +390,7 @@ def main():
     if not a.no_dem:
         try:
             out["dem"] = run_dem(a.dem_marches, 500, 0 if a.no_cpu else 2000)
+            out["dem"]["ibm_update"] = time_ibm_update(st32)
         except Exception as e:  # reported, never fatal for the headline figure
             out["dem"] = {"error": str(e)}
     if not a.no_cpu:

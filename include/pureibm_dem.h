@@ -78,7 +78,8 @@ typedef struct pib_dem_info {
  * (collision_mod.F90:68-96): initialize_coarse_grid, DES_INIT_ARRAYS
  * (:1203-1316, particle state from XC, RADBDY, VELBDY), CFASSIGN, and the
  * first cell assignment and neighbour search.  xc, radbdy in grid units;
- * velbdy may be NULL (zero).  Each box length must hold 8 of the largest
+ * velbdy may be NULL (zero).  Centres lie within one coarse cell of the
+ * box (they are wrapped once they move).  Each box length must hold 8 of the largest
  * radius (the periodic image rule of CALC_FORCE_DES, soft_sphere_coll.F90:
  * 309-349, assumes it).  Up to 2048 particles. */
 int pib_dem_create(const pib_dem_config* cfg, const double* xc, const double* radbdy,

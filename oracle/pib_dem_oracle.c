@@ -312,11 +312,14 @@ int orc_dem_create(const pib_dem_config* cfg, const double* xc, const double* ra
     if (D->rad[l] > D->max_radius) D->max_radius = D->rad[l];
   }
   const double box[3] = {cfg->xlength, cfg->ylength, cfg->zlength};
+  const double cell[3] = {D->dxc, D->dyc, D->dzc};
   for (int d = 0; d < 3; ++d) {
     if (!(box[d] >= 8.0 * D->max_radius)) return fail(D, PIB_ERR_UNSUPPORTED, "box shorter than 8 particle radii");
+    /* centres are wrapped once they move (CFNEWVALUES :1431-1459); one coarse
+     * cell outside keeps find_cell_index inside the grid */
     for (int l = 0; l < n; ++l)
-      if (!(P3(D->pos_new, l, d) >= 0.0 && P3(D->pos_new, l, d) <= box[d]))
-        return fail(D, PIB_ERR_ARG, "particle centre outside the periodic box");
+      if (!(P3(D->pos_new, l, d) > -cell[d] && P3(D->pos_new, l, d) < box[d] + cell[d]))
+        return fail(D, PIB_ERR_ARG, "particle centre more than a coarse cell outside the periodic box");
   }
   {
     double mi = (PI * pow(cfg->dia_phys, 3.0) * cfg->rhos) / 6.0;

@@ -526,11 +526,13 @@ void do_create(DemCtx* D, void* va) {
     if (rad[l] > D->max_radius) D->max_radius = rad[l];
   }
   const double box[3] = {c.xlength, c.ylength, c.zlength};
+  const double cell[3] = {D->dxc, D->dyc, D->dzc};
   for (int d = 0; d < 3; ++d) {
     if (!(box[d] >= 8.0 * D->max_radius)) dem_fail(PIB_ERR_UNSUPPORTED, "box shorter than 8 particle radii");
+    // the reference wraps a centre only once it moves (CFNEWVALUES :1431-1459)
     for (int l = 0; l < n; ++l)
-      if (!(pos[(size_t)d * n + l] >= 0.0 && pos[(size_t)d * n + l] <= box[d]))
-        dem_fail(PIB_ERR_ARG, "particle centre outside the periodic box");
+      if (!(pos[(size_t)d * n + l] > -cell[d] && pos[(size_t)d * n + l] < box[d] + cell[d]))
+        dem_fail(PIB_ERR_ARG, "particle centre more than a coarse cell outside the periodic box");
   }
   {
     const double mi = (PI * std::pow(c.dia_phys, 3.0) * c.rhos) / 6.0;
@@ -736,10 +738,11 @@ static void do_set(DemCtx* D, void* va) {
   if (p[0]) {
     const pib_dem_config& c = D->cfg;
     const double box[3] = {c.xlength, c.ylength, c.zlength};
+    const double cell[3] = {D->dxc, D->dyc, D->dzc};
     for (int d = 0; d < 3; ++d)
       for (int l = 0; l < D->n; ++l)
-        if (!(p[0][(size_t)d * D->n + l] >= 0.0 && p[0][(size_t)d * D->n + l] <= box[d]))
-          dem_fail(PIB_ERR_ARG, "particle centre outside the periodic box");
+        if (!(p[0][(size_t)d * D->n + l] > -cell[d] && p[0][(size_t)d * D->n + l] < box[d] + cell[d]))
+          dem_fail(PIB_ERR_ARG, "particle centre more than a coarse cell outside the periodic box");
     DEMCHK(hipMemcpy(D->pos, p[0], b, hipMemcpyHostToDevice));
   }
   if (p[1]) DEMCHK(hipMemcpy(D->vel, p[1], b, hipMemcpyHostToDevice));

@@ -122,6 +122,26 @@ class Solver:
     def step(self, nsteps=1):
         self._call("step", self.ctx, nsteps)
 
+    def update_particles(self, xc, velbdy=None, angv=None):
+        """Moved particles: what bcset does after des_time_march when particles
+        move (boundary_condition.F90:539-554) -- the solid mask of
+        grid_nodes_insphere (dependent_functions.F90:783-963) and the active
+        reversal pairs of update_nrpr_array (:966-990) for the new centres, with
+        the new surface velocities.  xc, velbdy, angv: (nbody, 3).  Returns
+        (count_solid, nrpr_active)."""
+        st = self.setup
+        xc = np.asfortranarray(xc, dtype=np.float64)
+        vel = None if velbdy is None else np.asfortranarray(velbdy, dtype=np.float64)
+        av = None if angv is None else np.asfortranarray(angv, dtype=np.float64)
+        cs = ctypes.c_longlong()
+        nr = np.zeros(st.nbody, dtype=np.int32)
+        self._call("set_particles", self.ctx, st.nbody, dptr(xc), dptr(st.radbdy),
+                   dptr(vel) if vel is not None else None, dptr(av) if av is not None else None,
+                   ctypes.byref(cs), nr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        self.count_solid = cs.value
+        self.nrpr_active = nr
+        return cs.value, nr
+
     # ---- state access in the reference layout ----
     def _shape(self, name):
         nx, my, mz, my2 = self.nx, self.my, self.mz, self.my2

@@ -182,7 +182,7 @@ def test_error_paths(orc):
         odem(orc, xc, rad, xlength=3.9, ylength=6.0, zlength=6.0, only_dem=True)
     assert ei.value.code == _abi.PIB_ERR_UNSUPPORTED
     bad = xc.copy()
-    bad[0, 1] = 6.5 / DX + 1
+    bad[0, 1] = 8.5 / DX + 1
     with pytest.raises(solver.PibError) as ei:
         odem(orc, bad, rad, xlength=6.0, ylength=6.0, zlength=6.0, only_dem=True)
     assert ei.value.code == _abi.PIB_ERR_ARG

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from pureibm_ps2d_fd1d_amd import _abi, solver
+from conftest import make_setup, rel_err
 from dem_beds import random_bed
 
 pytestmark = pytest.mark.gpu
@@ -119,3 +120,42 @@ def test_gpu_set_state_restarts_the_sub_cycle(orc):
         assert np.max(np.abs(x - z)) <= TOL * 8.0
     for d in (g, h, o):
         d.close()
+
+
+def test_dem_moves_the_bed_of_a_fluid_step(orc, tmp_path):
+    """The chain bcset runs for moving particles: DEM march with the drag of
+    the step, IBMUPDATE, the new solid mask and reversal pairs, then further
+    fluid steps with moving surfaces -- GPU against the oracle (FP64)."""
+    st = make_setup(tmp_path, "MV", kind="random", my=32, dbydx=8, nbody=6, seed=3)
+    g = solver.Solver(st)
+    o = solver.Solver(st, lib=orc.load(), prefix="orc_")
+    for s_ in (g, o):
+        s_.step(2)
+    dx = st.config.dx
+    L = (st.config.nx * dx, st.config.my * dx, st.config.mz * dx)
+    rng = np.random.default_rng(5)
+    vel0 = rng.uniform(-0.3, 0.3, (st.nbody, 3))
+    kw = dict(dx=dx, xlength=L[0], ylength=L[1], zlength=L[2], coeff_rest=0.8, rhos=2.0, rhof=1.0,
+              dia_phys=st.config.char_length)
+    dg = solver.Dem(st.xc, st.radbdy, velbdy=vel0, **kw)
+    do = solver.Dem(st.xc, st.radbdy, velbdy=vel0, lib=orc.load(), prefix="orc_", **kw)
+    for it in range(2):
+        fg, fo = g.forces(), o.forces()
+        sg, so = g.scalars(), o.scalars()
+        dt = sg.dt               # des_time_march(.FALSE.) advances the particles over the fluid dt
+        outs = []
+        for d, f, s_ in ((dg, fg, sg), (do, fo, so)):
+            outs.append(d.march(dt, pres=f["pres"], visc=f["visc"], mpg=np.array(s_.mpg),
+                                frame_accln=np.array(s_.frame_accln)))
+        (xg, vg, ag), (xo, vo, ao) = outs
+        assert rel_err(xg, xo) < 1e-12 and rel_err(vg, vo) < 1e-10
+        cg, ng = g.update_particles(xg, vg, ag)
+        co, no = o.update_particles(xg, vg, ag)
+        assert cg == co and np.array_equal(ng, no)
+        assert np.array_equal(g.field("mask"), o.field("mask"))
+        for s_ in (g, o):
+            s_.step(1)
+        assert rel_err(g.field("u"), o.field("u")) < 1e-10
+        assert abs(g.scalars().norm_drag - o.scalars().norm_drag) <= 1e-9 * abs(o.scalars().norm_drag)
+    for x in (g, o, dg, do):
+        x.close()
