@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "pib_kernels.h"
@@ -59,14 +60,17 @@ __global__ void k_mask_body(uint8_t* mask, const double* xc, const double* rad, 
 }
 
 // neighbour sets of the soft-sphere search (collision_mod.F90:807-1062):
-// pairs with an image inside 1.2*(r_a + r_b) (physical units)
+// pairs with an image inside 1.2*(r_a + r_b) (physical units).  One
+// workgroup a sphere; the set is used only as a set (check_external_pt ORs
+// over it), so members are appended in any order.
 __global__ void k_neighbours(const double* xc, const double* rad, int nbody, double dx, double lx, double ly,
                              double lz, int* nb, int* nbcnt) {
-  int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= nbody) return;
-  int cnt = 0;
+  __shared__ int scnt;
+  const int m = blockIdx.x;
+  if (threadIdx.x == 0) scnt = 0;
+  __syncthreads();
   double pa0 = (xc[m] - 1.0) * dx, pa1 = (xc[nbody + m] - 1.0) * dx, pa2 = (xc[2 * nbody + m] - 1.0) * dx;
-  for (int q = 0; q < nbody; ++q) {
+  for (int q = threadIdx.x; q < nbody; q += blockDim.x) {
     if (q == m) continue;
     double pb0 = (xc[q] - 1.0) * dx, pb1 = (xc[nbody + q] - 1.0) * dx, pb2 = (xc[2 * nbody + q] - 1.0) * dx;
     double rlm = 1.2 * (rad[m] * dx + rad[q] * dx);
@@ -79,11 +83,12 @@ __global__ void k_neighbours(const double* xc, const double* rad, int nbody, dou
           if (rlm - dm > PIB_SMALL_NUMBER) hit = true;
         }
     if (hit) {
-      if (cnt < MAXNB) nb[(size_t)m * MAXNB + cnt] = q;
-      cnt++;
+      const int slot = atomicAdd(&scnt, 1);
+      if (slot < MAXNB) nb[(size_t)m * MAXNB + slot] = q;
     }
   }
-  nbcnt[m] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) nbcnt[m] = scnt;
 }
 
 // check_external_pt (dependent_functions.F90:992-1051): a reversal pair is
@@ -650,7 +655,8 @@ struct Impl : Base {
   double* rw = nullptr;
   uint8_t* rdist = nullptr;
   bool have_points = false, have_particles = false, nlbc_valid = true;
-  std::vector<void*> pallocs;  // particle-dependent allocations
+  std::vector<std::pair<void*, size_t>> pallocs;  // particle-dependent allocations and their bytes
+  std::multimap<size_t, void*> ppool;  // those of the previous particle set, reused by size
   hipEvent_t ev[5];
   bool timed = false;
   // optional per-launch timing: events recorded around every launch on `stream`
@@ -684,11 +690,26 @@ struct Impl : Base {
 
   template <typename V>
   V* alloc(size_t n, bool part_dep = false) {
+    const size_t bytes = n * sizeof(V) + 64;
     void* ptr = nullptr;
-    HIPCHK(hipMalloc(&ptr, n * sizeof(V) + 64));
-    HIPCHK(hipMemsetAsync(ptr, 0, n * sizeof(V) + 64, stream));
-    (part_dep ? pallocs : allocs).push_back(ptr);
+    if (part_dep) {
+      auto it = ppool.find(bytes);
+      if (it != ppool.end()) {
+        ptr = it->second;
+        ppool.erase(it);
+      }
+    }
+    if (!ptr) HIPCHK(hipMalloc(&ptr, bytes));
+    HIPCHK(hipMemsetAsync(ptr, 0, bytes, stream));
+    if (part_dep)
+      pallocs.push_back({ptr, bytes});
+    else
+      allocs.push_back(ptr);
     return (V*)ptr;
+  }
+  void drain_pool() {
+    for (auto& e : ppool) (void)hipFree(e.second);
+    ppool.clear();
   }
   template <typename V>
   V* upload(const std::vector<V>& h) {
@@ -826,7 +847,8 @@ struct Impl : Base {
   ~Impl() override {
     if (stream) (void)hipStreamSynchronize(stream);
     drop_graphs();
-    for (void* q : pallocs) (void)hipFree(q);
+    for (auto& q : pallocs) (void)hipFree(q.first);
+    drain_pool();
     for (void* q : allocs) (void)hipFree(q);
     for (auto& e : ev) (void)hipEventDestroy(e);
     for (auto& e : evpool) (void)hipEventDestroy(e);
@@ -971,7 +993,8 @@ struct Impl : Base {
     if (nb > 0 && !have_points) throw PibFail(PIB_ERR_STATE, "set_points must precede set_particles");
     HIPCHK(hipStreamSynchronize(stream));
     drop_graphs();
-    for (void* q : pallocs) (void)hipFree(q);
+    // a moved bed of the same size reuses the previous set's buffers
+    for (auto& q : pallocs) ppool.insert({q.second, q.first});
     pallocs.clear();
     nbody = nb;
     size_t n3 = (size_t)3 * (nb > 0 ? nb : 1);
@@ -1028,8 +1051,7 @@ struct Impl : Base {
       // neighbour sets, active reversal pairs
       int* nbl = alloc<int>((size_t)nb * MAXNB, true);
       int* nbc = alloc<int>(nb, true);
-      k_neighbours<<<(nb + 127) / 128, 128, 0, stream>>>(xc, rad, nb, cfg.dx, nx * cfg.dx, my * cfg.dx, mz * cfg.dx,
-                                                          nbl, nbc);
+      k_neighbours<<<nb, 256, 0, stream>>>(xc, rad, nb, cfg.dx, nx * cfg.dx, my * cfg.dx, mz * cfg.dx, nbl, nbc);
       HIPCHK(hipGetLastError());
       std::vector<int> hcnt(nb);
       HIPCHK(hipMemcpyAsync(hcnt.data(), nbc, nb * sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -1065,6 +1087,7 @@ struct Impl : Base {
     // zero the forcing field: only target nodes are written each step
     HIPCHK(hipMemsetAsync(fr, 0, (size_t)3 * nx * my * mz * sizeof(T), stream));
     HIPCHK(hipStreamSynchronize(stream));
+    drain_pool();
     have_particles = true;
   }
   void build_spreading() {
@@ -1653,7 +1676,7 @@ struct Impl : Base {
   }
   void free_tail(int n) {
     for (int q = 0; q < n && !pallocs.empty(); ++q) {
-      (void)hipFree(pallocs.back());
+      (void)hipFree(pallocs.back().first);
       pallocs.pop_back();
     }
   }
