@@ -223,6 +223,7 @@ _HOST_SIGS = {
     "last_error": ([ctypes.c_char_p, _I], _I),
     "quadgener": ([_D, _D, _I, _PD, _PD], _I),
     "set_option": ([_I, _D], _I),
+    "struct_sizes": ([_PI], _I),
 }
 
 

@@ -562,6 +562,22 @@ contains
     ns = nerr_steps
   end function pib_host_get_nsteps_hint
 
+  ! Byte sizes of the bind(C) mirrors in pib_types, so that callers can check
+  ! them against the C declarations.
+  function pib_host_struct_sizes(sizes) bind(C, name="pib_host_struct_sizes") result(st)
+    integer(c_int), intent(out) :: sizes(4)
+    integer(c_int) :: st
+    type(pib_config) :: a
+    type(pib_scalars) :: b
+    type(pib_dem_config) :: c
+    type(pib_dem_info) :: d
+    sizes(1) = int(c_sizeof(a), c_int)
+    sizes(2) = int(c_sizeof(b), c_int)
+    sizes(3) = int(c_sizeof(c), c_int)
+    sizes(4) = int(c_sizeof(d), c_int)
+    st = PIB_OK
+  end function pib_host_struct_sizes
+
   function pib_host_last_error(buf, n) bind(C, name="pib_host_last_error") result(st)
     character(kind=c_char), intent(out) :: buf(*)
     integer(c_int), value :: n

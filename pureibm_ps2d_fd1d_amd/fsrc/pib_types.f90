@@ -1,12 +1,14 @@
 ! pib_types.f90 -- Fortran mirrors (bind(C)) of the C-ABI structs declared in
-! include/pureibm_gpu.h, plus the interfaces to the C-ABI entry points.
+! include/pureibm_gpu.h and include/pureibm_dem.h, plus the interfaces to the
+! C-ABI entry points.
 ! Field order and kinds must stay identical to the C declarations.
 module pib_types
   use iso_c_binding
   implicit none
 
   integer(c_int), parameter :: PIB_OK = 0, PIB_ERR_ARG = -1, PIB_ERR_UNSUPPORTED = -2, &
-       PIB_ERR_HIP = -3, PIB_ERR_STATE = -4, PIB_ERR_BLOWUP = -5, PIB_ERR_PIVOT = -6
+       PIB_ERR_HIP = -3, PIB_ERR_STATE = -4, PIB_ERR_BLOWUP = -5, PIB_ERR_PIVOT = -6, &
+       PIB_ERR_MOTION = -7, PIB_ERR_CAPACITY = -8
   integer(c_int), parameter :: PIB_FP64 = 0, PIB_FP32_BULK = 1
   integer(c_int), parameter :: PIB_SCHEME_CENTER = 0, PIB_SCHEME_UPWIND = 1
   integer(c_int), parameter :: PIB_LIMITER_NONE = 0, PIB_LIMITER_MINMOD = 1
@@ -39,7 +41,58 @@ module pib_types
      real(c_double) :: reserved(15)
   end type pib_scalars
 
+  ! soft-sphere DEM (include/pureibm_dem.h)
+  type, bind(C) :: pib_dem_config
+     integer(c_int) :: nbody, only_dem, reserved_i(2)
+     real(c_double) :: dx, dy, dz, xlength, ylength, zlength, foffset
+     real(c_double) :: rhos, rhof, dia_phys, coeff_rest, kn
+     real(c_double) :: reserved_d(6)
+  end type pib_dem_config
+
+  type, bind(C) :: pib_dem_info
+     real(c_double) :: dtsolid, tcoll, dtsolid_used, s_time, overlap_max, box(3)
+     integer(c_int) :: factor, ncontacts, cells(3), substeps, rebuilds, reserved(7)
+  end type pib_dem_info
+
   interface
+     ! des_time_march(.TRUE.) (collision_mod.F90:68-96)
+     function pib_dem_create(cfg, xc, radbdy, velbdy, dem) bind(C, name="pib_dem_create") result(st)
+       import :: c_int, c_ptr, c_double, pib_dem_config
+       type(pib_dem_config), intent(in) :: cfg
+       real(c_double), intent(in) :: xc(*), radbdy(*)
+       type(c_ptr), value :: velbdy
+       type(c_ptr), intent(out) :: dem
+       integer(c_int) :: st
+     end function pib_dem_create
+     function pib_dem_destroy(dem) bind(C, name="pib_dem_destroy") result(st)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: dem
+       integer(c_int) :: st
+     end function pib_dem_destroy
+     ! des_time_march(.FALSE.) + IBMUPDATE (collision_mod.F90:272-297, 1464-1474)
+     function pib_dem_march(dem, dt, pres, visc, mpg, frame_accln, xc, velbdy, angv) &
+          bind(C, name="pib_dem_march") result(st)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: dem
+       real(c_double), value :: dt
+       real(c_double), intent(in) :: pres(*), visc(*), mpg(3), frame_accln(3)
+       real(c_double), intent(out) :: xc(*), velbdy(*), angv(*)
+       integer(c_int) :: st
+     end function pib_dem_march
+     function pib_dem_get_state(dem, pos, vel, omega, contact_force, info) &
+          bind(C, name="pib_dem_get_state") result(st)
+       import :: c_int, c_ptr, c_double, pib_dem_info
+       type(c_ptr), value :: dem
+       real(c_double), intent(out) :: pos(*), vel(*), omega(*), contact_force(*)
+       type(pib_dem_info), intent(out) :: info
+       integer(c_int) :: st
+     end function pib_dem_get_state
+     function pib_dem_set_state(dem, pos, vel) bind(C, name="pib_dem_set_state") result(st)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: dem
+       real(c_double), intent(in) :: pos(*), vel(*)
+       integer(c_int) :: st
+     end function pib_dem_set_state
      function pib_create(cfg, ctx) bind(C, name="pib_create") result(st)
        import :: c_int, c_ptr, pib_config
        type(pib_config), intent(in) :: cfg

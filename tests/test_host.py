@@ -97,6 +97,15 @@ def test_gpu_library_exports_the_header():
     assert set(_abi.GPU_SYMBOLS) <= names
 
 
+def test_fortran_mirrors_match_the_c_structs():
+    """The bind(C) types of fsrc/pib_types.f90 have the C structs' sizes."""
+    sizes = (ctypes.c_int * 4)()
+    assert _abi.load_host().pib_host_struct_sizes(sizes) == 0
+    want = [ctypes.sizeof(_abi.PibConfig), ctypes.sizeof(_abi.PibScalars),
+            ctypes.sizeof(_abi.PibDemConfig), ctypes.sizeof(_abi.PibDemInfo)]
+    assert list(sizes) == want
+
+
 def test_product_never_imports_the_oracle():
     pkg = os.path.join(ROOT, "pureibm_ps2d_fd1d_amd")
     for dirpath, _, files in os.walk(pkg):
