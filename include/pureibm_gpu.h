@@ -66,13 +66,18 @@ typedef struct pib_config {
   int include_frmeanfluid;
   int set_mpg;           /* SET_MPG: mpg held fixed */
   int soft_start;        /* initflo sets .true. (initialize_flo.F90:126) */
-  int reserved_i[2];
+  int move_particles;    /* MOVE_PARTICLES: the moving-particle branch of compute_mpg_at_n
+                            (boundary_condition.F90:389-399) and the frame update of velstep
+                            (steptheflow.F90:101-104); the particles themselves are moved by the
+                            DEM (pureibm_dem.h) between pib_nonlinear_begin and _finish */
+  int reserved_i[1];
   double ucharmod;       /* |uchar| (initialize_flo.F90:278-280) */
   double char_length;    /* dia_phys */
   double dia;            /* phase diameter 2*radbdy(1)*dx (initialize_flo.F90:186) */
   double dr;             /* DR: reversal-point offset in cells */
   double voldom;         /* doml(1)*doml(2)*doml(3) */
-  double reserved_d[8];
+  double rhos, rhof;     /* RHOS, RHOF: used when move_particles is set */
+  double reserved_d[6];
 } pib_config;
 
 /* Scalars the reference keeps in global_data between routine calls. */
@@ -93,7 +98,8 @@ typedef struct pib_scalars {
   double dtnew, umax_tmp, divmax;
   double count_solid, count_fluid;
   double ramp_frac_time; /* initialize_flo.F90:700, used by the cf soft start */
-  double reserved[15];
+  double frame_vel[3], frame_pos[3]; /* moving frame (steptheflow.F90:101-104) */
+  double reserved[9];
 } pib_scalars;
 
 /* field ids for pib_get_field / pib_set_field */
@@ -153,6 +159,14 @@ int pib_compute_new_timestep(void* ctx, int rks);
 /* nonlinear (nl.F90:63-205): form_nl, sign, nl/onl -> physical, then bcset
  * (boundary_condition.F90:467-900) when nbody > 0 */
 int pib_nonlinear(void* ctx, int rks);
+/* pib_nonlinear in two halves, split where bcset calls des_time_march for
+ * moving particles (boundary_condition.F90:539-554): _begin runs form_nl, the
+ * transforms, calc_visc and compute_mpg_at_n (drag, mpg, frame_accln, the
+ * bcset diagnostics); _finish runs calc_pgrad and the reversal forcing.  In
+ * between, a caller moves the bed: pib_get_forces, pib_dem_march, then
+ * pib_set_particles with the new centres and velocities. */
+int pib_nonlinear_begin(void* ctx, int rks);
+int pib_nonlinear_finish(void* ctx, int rks);
 /* velstep (steptheflow.F90:52-160): pressure, ustep, divcorr per (ky,kz) mode,
  * check_divergence, umean update */
 int pib_velstep(void* ctx, int rks);

@@ -48,6 +48,7 @@ static inline cpx cmul(cpx a, cpx b) {
 
 typedef struct orc {
   pib_config cfg;
+  int nl_pending; /* rks of an orc_nonlinear_begin not yet finished */
   int nx, my, mz, my2, mx, mxf;
   double dx, dx2, dx3, vis;
   double coef[3][4];
@@ -600,6 +601,18 @@ int orc_set_particles(void* ctx, int nbody, const double* xc, const double* radb
   orc* o = (orc*)ctx;
   if (!o || nbody < 0 || (nbody > 0 && (!xc || !radbdy))) return PIB_ERR_ARG;
   if (nbody > 0 && !o->have_points) return fail(o, PIB_ERR_STATE, "set_points first");
+  /* a moved bed keeps the step's per-particle force sums (grid_nodes_insphere
+   * leaves them alone) */
+  double* keep = NULL;
+  if (o->have_particles && nbody == o->nbody && nbody > 0) {
+    size_t n3k = (size_t)nbody * 3;
+    keep = (double*)xcalloc(5 * n3k, sizeof(double));
+    memcpy(keep, o->pres, n3k * sizeof(double));
+    memcpy(keep + n3k, o->visc, n3k * sizeof(double));
+    memcpy(keep + 2 * n3k, o->torq, n3k * sizeof(double));
+    memcpy(keep + 3 * n3k, o->force, n3k * sizeof(double));
+    memcpy(keep + 4 * n3k, o->force_chem, n3k * sizeof(double));
+  }
   free_particles(o);
   o->nbody = nbody;
   size_t n3 = (size_t)nbody * 3;
@@ -621,6 +634,14 @@ int orc_set_particles(void* ctx, int nbody, const double* xc, const double* radb
     memcpy(o->radbdy, radbdy, sizeof(double) * (size_t)nbody);
     if (velbdy) memcpy(o->velbdy, velbdy, sizeof(double) * n3);
     if (angv) memcpy(o->angv, angv, sizeof(double) * n3);
+  }
+  if (keep) {
+    memcpy(o->pres, keep, n3 * sizeof(double));
+    memcpy(o->visc, keep + n3, n3 * sizeof(double));
+    memcpy(o->torq, keep + 2 * n3, n3 * sizeof(double));
+    memcpy(o->force, keep + 3 * n3, n3 * sizeof(double));
+    memcpy(o->force_chem, keep + 4 * n3, n3 * sizeof(double));
+    free(keep);
   }
   for (int m = 0; m < nbody; ++m) {
     o->radibdy[m] = o->radbdy[m] - o->cfg.dr; /* initialize_flo.F90:501-505 */
@@ -1227,12 +1248,26 @@ static void compute_mpg_at_n(orc* o, int rks, double norm_factor) {
   s->pres_drag = s->pres_drag / norm_factor / (1.0 - s->maxvolfrac);
   s->visc_drag = s->visc_drag / norm_factor / (1.0 - s->maxvolfrac);
   s->total_drag = s->pres_drag + s->visc_drag;
+  const double rhos = o->cfg.rhos, rhof = o->cfg.rhof, mv = s->mean_volfrac, vd = o->cfg.voldom;
   if (!o->cfg.set_mpg) {
     for (int n = 0; n < 3; ++n) {
-      s->mpg[n] = s->cf * (s->ufmean_des[n] - s->ufmean[n]) +
-                  (s->pres_total[n] / o->cfg.voldom - (s->visc_total[n]) / o->cfg.voldom) / (1.0 - s->maxvolfrac);
+      if (o->cfg.move_particles) { /* :389-397 */
+        s->mpg[n] = s->cf * (s->ufmean_des[n] - s->ufmean[n]) + s->cf * (s->usmean[n] - s->usmean_des[n]);
+        s->mpg[n] = s->mpg[n] + (s->pres_total[n] - (s->visc_total[n])) *
+                                    (1.0 / (rhof * (1.0 - mv)) + 1.0 / (mv * rhos)) / vd;
+        s->mpg[n] = s->mpg[n] / (1.0 / rhof - 1.0 / rhos);
+        s->frame_accln[n] = -s->mpg[n] / rhos - (s->pres_total[n] - s->visc_total[n]) / (mv * rhos * vd) +
+                            s->cf * (s->usmean_des[n] - s->usmean[n]);
+      } else {
+        s->mpg[n] = s->cf * (s->ufmean_des[n] - s->ufmean[n]) +
+                    (s->pres_total[n] / o->cfg.voldom - (s->visc_total[n]) / o->cfg.voldom) / (1.0 - s->maxvolfrac);
+      }
       if (o->cfg.include_frmeanfluid) s->mpg[n] = s->mpg[n] + s->frmeanfluid[n];
     }
+  } else if (o->cfg.move_particles) { /* :414-417 */
+    for (int n = 0; n < 3; ++n)
+      s->frame_accln[n] = -s->mpg[n] / rhos - (s->pres_total[n] - s->visc_total[n]) / (mv * rhos * vd) +
+                          s->cf * (s->usmean_des[n] - s->usmean[n]);
   }
   double c12 = o->coef[rks - 1][0] + o->coef[rks - 1][1];
   for (int m = 1; m <= o->nbody; ++m) {
@@ -1351,40 +1386,18 @@ static void calc_inner_reversal_forcing(orc* o, int m, int rks, double da2) {
   }
 }
 
-/* bcset :467-900 (not moving, serial) */
-static int bcset(orc* o, int rks) {
+/* bcset :467-900, serial.  The first half (:517-538 with the diagnostics of
+ * :723-884, which read only force, force_chem and mpg) runs before the point
+ * where the reference moves particles (:539-554); the second half is
+ * calc_pgrad and the reversal forcing. */
+static int bcset_drag(orc* o, int rks) {
   pib_scalars* s = &o->s;
-  int nx = o->nx, my = o->my, mz = o->mz, my2 = o->my2;
   double norm_factor = (3.0 * PI * o->vis * (s->mixmeanslipmod + SMALL_NUMBER) * o->cfg.char_length) * (double)o->nbody;
   calc_visc(o);
   if (rks == 1)
     for (int m = 1; m <= o->nbody; ++m)
       for (int n = 1; n <= 3; ++n) PB(o, force, m, n) = 0.0;
   compute_mpg_at_n(o, rks, norm_factor);
-  calc_pgrad(o);
-  for (int n = 0; n < 3; ++n) { s->frmean[n] = 0.0; s->frmeanfluid[n] = 0.0; }
-  memset(o->fr, 0, sizeof(double) * (size_t)(nx + 1) * my * mz * 3);
-  for (int m = 1; m <= o->nbody; ++m) {
-    double rdx = o->radibdy[m - 1] * o->dx;
-    double da2 = 4.0 * PI * (rdx * rdx) / (double)o->nrpr_active[m - 1];
-    if (o->cfg.dorpr) calc_inner_reversal_forcing(o, m, rks, da2);
-  }
-  for (int n = 0; n < 3; ++n) {
-    s->frmean[n] = s->frmean[n] / (double)((long long)nx * my * mz);
-    s->frmeanfluid[n] = s->frmeanfluid[n] / (double)o->count_fluid;
-  }
-  for (int n = 1; n <= 3; ++n) {
-    for (int i = 1; i <= nx; ++i) {
-      for (int j = 1; j <= my; ++j)
-        for (int k = 1; k <= mz; ++k) o->fr[IR(o, i, j, k, n)] = o->fr[IR(o, i, j, k, n)] - s->frmean[n - 1];
-      get_rplane(o, o->fr + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, o->ra);
-      orc_ff2rc(my, mz, o->ra, (double*)o->ca);
-      for (int k = 1; k <= mz; ++k)
-        for (int j = 1; j <= my2; ++j) o->ff[IU(o, i, j, k, n)] = o->ca[(size_t)(k - 1) * my2 + (j - 1)];
-    }
-    for (int k = 1; k <= mz; ++k)
-      for (int j = 1; j <= my2; ++j) o->ff[IU(o, nx + 1, j, k, n)] = o->ff[IU(o, 1, j, k, n)];
-  }
   /* diagnostics, :723-884 (single phase) */
   double avg_force[3], avg_chem[3];
   for (int n = 1; n <= 3; ++n) {
@@ -1416,10 +1429,45 @@ static int bcset(orc* o, int rks) {
   return PIB_OK;
 }
 
-/* nonlinear, nl.F90:63-205 */
-int orc_nonlinear(void* ctx, int rks) {
-  orc* o = (orc*)ctx;
-  if (!o || rks < 1 || rks > o->itrmax) return PIB_ERR_ARG;
+static int bcset_forcing(orc* o, int rks) {
+  pib_scalars* s = &o->s;
+  int nx = o->nx, my = o->my, mz = o->mz, my2 = o->my2;
+  calc_pgrad(o);
+  for (int n = 0; n < 3; ++n) { s->frmean[n] = 0.0; s->frmeanfluid[n] = 0.0; }
+  memset(o->fr, 0, sizeof(double) * (size_t)(nx + 1) * my * mz * 3);
+  for (int m = 1; m <= o->nbody; ++m) {
+    double rdx = o->radibdy[m - 1] * o->dx;
+    double da2 = 4.0 * PI * (rdx * rdx) / (double)o->nrpr_active[m - 1];
+    if (o->cfg.dorpr) calc_inner_reversal_forcing(o, m, rks, da2);
+  }
+  for (int n = 0; n < 3; ++n) {
+    s->frmean[n] = s->frmean[n] / (double)((long long)nx * my * mz);
+    s->frmeanfluid[n] = s->frmeanfluid[n] / (double)o->count_fluid;
+  }
+  for (int n = 1; n <= 3; ++n) {
+    for (int i = 1; i <= nx; ++i) {
+      for (int j = 1; j <= my; ++j)
+        for (int k = 1; k <= mz; ++k) o->fr[IR(o, i, j, k, n)] = o->fr[IR(o, i, j, k, n)] - s->frmean[n - 1];
+      get_rplane(o, o->fr + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, o->ra);
+      orc_ff2rc(my, mz, o->ra, (double*)o->ca);
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my2; ++j) o->ff[IU(o, i, j, k, n)] = o->ca[(size_t)(k - 1) * my2 + (j - 1)];
+    }
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my2; ++j) o->ff[IU(o, nx + 1, j, k, n)] = o->ff[IU(o, 1, j, k, n)];
+  }
+  return PIB_OK;
+}
+
+static int bcset(orc* o, int rks) {
+  int st = bcset_drag(o, rks);
+  if (st) return st;
+  return bcset_forcing(o, rks);
+}
+
+/* nonlinear, nl.F90:63-205; part 0 whole, 1 up to the drag, 2 the forcing */
+static int nonlinear_part(orc* o, int rks, int part) {
+  if (part == 2) return o->nbody > 0 ? bcset_forcing(o, rks) : PIB_OK;
   int nx = o->nx, my = o->my, mz = o->mz, my2 = o->my2;
   size_t nc = (size_t)(nx + 1) * my2 * mz * 3;
   for (size_t q = 0; q < nc; ++q) { o->onl[q] = o->nl[q]; o->nl[q] = C(0.0, 0.0); }
@@ -1437,8 +1485,30 @@ int orc_nonlinear(void* ctx, int rks) {
       put_rplane(o, o->nlbc + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, o->ra);
       put_rplane(o, o->onlbc + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, o->rb);
     }
-  if (o->nbody > 0) return bcset(o, rks);
+  if (o->nbody > 0) return part == 1 ? bcset_drag(o, rks) : bcset(o, rks);
   return PIB_OK;
+}
+
+int orc_nonlinear(void* ctx, int rks) {
+  orc* o = (orc*)ctx;
+  if (!o || rks < 1 || rks > o->itrmax) return PIB_ERR_ARG;
+  return nonlinear_part(o, rks, 0);
+}
+
+int orc_nonlinear_begin(void* ctx, int rks) {
+  orc* o = (orc*)ctx;
+  if (!o || rks < 1 || rks > o->itrmax) return PIB_ERR_ARG;
+  int st = nonlinear_part(o, rks, 1);
+  o->nl_pending = st ? 0 : rks;
+  return st;
+}
+
+int orc_nonlinear_finish(void* ctx, int rks) {
+  orc* o = (orc*)ctx;
+  if (!o) return PIB_ERR_ARG;
+  if (o->nl_pending != rks) return fail(o, PIB_ERR_STATE, "pib_nonlinear_begin with this rks first");
+  o->nl_pending = 0;
+  return nonlinear_part(o, rks, 2);
 }
 
 /* ======================================================================
@@ -1643,6 +1713,11 @@ int orc_velstep(void* ctx, int rks) {
   o->s.divmax = divmax;
   for (int n = 0; n < 3; ++n)
     o->s.umean[n] = o->s.umean[n] + (-o->s.mpg[n] + o->s.frmean[n] - o->s.frame_accln[n]) * o->s.dt;
+  if (o->cfg.move_particles) /* steptheflow.F90:101-104 */
+    for (int n = 0; n < 3; ++n) {
+      o->s.frame_vel[n] = o->s.frame_vel[n] + o->s.frame_accln[n] * o->s.dt;
+      o->s.frame_pos[n] = o->s.frame_pos[n] + o->s.frame_vel[n] * o->s.dt;
+    }
   if (st) return fail(o, st, "zero pivot in tridiagonal solve");
   return PIB_OK;
 }

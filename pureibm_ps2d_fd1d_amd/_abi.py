@@ -54,10 +54,11 @@ class PibConfig(ctypes.Structure):
         ("limiter", ctypes.c_int), ("revernorm", ctypes.c_int), ("dorpr", ctypes.c_int),
         ("dobnd", ctypes.c_int), ("include_frmeanfluid", ctypes.c_int),
         ("set_mpg", ctypes.c_int), ("soft_start", ctypes.c_int),
-        ("reserved_i", ctypes.c_int * 2),
+        ("move_particles", ctypes.c_int), ("reserved_i", ctypes.c_int * 1),
         ("ucharmod", ctypes.c_double), ("char_length", ctypes.c_double),
         ("dia", ctypes.c_double), ("dr", ctypes.c_double), ("voldom", ctypes.c_double),
-        ("reserved_d", ctypes.c_double * 8),
+        ("rhos", ctypes.c_double), ("rhof", ctypes.c_double),
+        ("reserved_d", ctypes.c_double * 6),
     ]
 
 
@@ -84,7 +85,8 @@ class PibScalars(ctypes.Structure):
         ("dtnew", ctypes.c_double), ("umax_tmp", ctypes.c_double), ("divmax", ctypes.c_double),
         ("count_solid", ctypes.c_double), ("count_fluid", ctypes.c_double),
         ("ramp_frac_time", ctypes.c_double),
-        ("reserved", ctypes.c_double * 15),
+        ("frame_vel", ctypes.c_double * 3), ("frame_pos", ctypes.c_double * 3),
+        ("reserved", ctypes.c_double * 9),
     ]
 
     def as_dict(self):
@@ -122,6 +124,8 @@ _STEP_SIGS = {
     "get_forces": ([_VP, _PD, _PD, _PD, _PD], _I),
     "compute_new_timestep": ([_VP, _I], _I),
     "nonlinear": ([_VP, _I], _I),
+    "nonlinear_begin": ([_VP, _I], _I),
+    "nonlinear_finish": ([_VP, _I], _I),
     "velstep": ([_VP, _I], _I),
     "step": ([_VP, _I], _I),
 }

@@ -354,10 +354,25 @@ __global__ void __launch_bounds__(DRAG_T) k_scalar_drag(DevState* stg, const dou
     s.total_drag = s.pres_drag + s.visc_drag;
     if (!st->set_mpg) {
       for (int n = 0; n < 3; ++n) {
-        s.mpg[n] = s.cf * (s.ufmean_des[n] - s.ufmean[n]) +
-                   (s.pres_total[n] / st->voldom - (s.visc_total[n]) / st->voldom) / (1.0 - s.maxvolfrac);
+        if (st->move_particles) {  // boundary_condition.F90:390-397
+          const double rs = st->rhos, rf = st->rhof, mv = s.mean_volfrac;
+          s.mpg[n] = s.cf * (s.ufmean_des[n] - s.ufmean[n]) + s.cf * (s.usmean[n] - s.usmean_des[n]);
+          s.mpg[n] = s.mpg[n] + (s.pres_total[n] - (s.visc_total[n])) * (1.0 / (rf * (1.0 - mv)) + 1.0 / (mv * rs)) /
+                                    st->voldom;
+          s.mpg[n] = s.mpg[n] / (1.0 / rf - 1.0 / rs);
+          s.frame_accln[n] = -s.mpg[n] / rs - (s.pres_total[n] - s.visc_total[n]) / (mv * rs * st->voldom) +
+                             s.cf * (s.usmean_des[n] - s.usmean[n]);
+        } else {
+          s.mpg[n] = s.cf * (s.ufmean_des[n] - s.ufmean[n]) +
+                     (s.pres_total[n] / st->voldom - (s.visc_total[n]) / st->voldom) / (1.0 - s.maxvolfrac);
+        }
         if (st->include_frmeanfluid) s.mpg[n] = s.mpg[n] + s.frmeanfluid[n];
       }
+    } else if (st->move_particles) {  // :414-417
+      for (int n = 0; n < 3; ++n)
+        s.frame_accln[n] = -s.mpg[n] / st->rhos -
+                           (s.pres_total[n] - s.visc_total[n]) / (s.mean_volfrac * st->rhos * st->voldom) +
+                           s.cf * (s.usmean_des[n] - s.usmean[n]);
     }
     for (int n = 0; n < 3; ++n) mpg_s[n] = s.mpg[n];
   }
@@ -446,6 +461,11 @@ __global__ void __launch_bounds__(SCAL_T) k_scalar_vel(DevState* stg, const doub
     st->s.divmax = mx[0];
     for (int n = 0; n < 3; ++n)
       st->s.umean[n] = st->s.umean[n] + (-st->s.mpg[n] + st->s.frmean[n] - st->s.frame_accln[n]) * st->s.dt;
+    if (st->move_particles)  // steptheflow.F90:101-104
+      for (int n = 0; n < 3; ++n) {
+        st->s.frame_vel[n] = st->s.frame_vel[n] + st->s.frame_accln[n] * st->s.dt;
+        st->s.frame_pos[n] = st->s.frame_pos[n] + st->s.frame_vel[n] * st->s.dt;
+      }
     int e = *status;
     if (e && st->status == 0) st->status = e;
     *status = 0;  // the velstep kernels only ever raise it
@@ -655,6 +675,7 @@ struct Impl : Base {
   double* rw = nullptr;
   uint8_t* rdist = nullptr;
   bool have_points = false, have_particles = false, nlbc_valid = true;
+  int nl_pending = 0;  // rks of a pib_nonlinear_begin not yet finished
   std::vector<std::pair<void*, size_t>> pallocs;  // particle-dependent allocations and their bytes
   std::multimap<size_t, void*> ppool;  // those of the previous particle set, reused by size
   hipEvent_t ev[5];
@@ -840,6 +861,9 @@ struct Impl : Base {
     h.include_frmeanfluid = c.include_frmeanfluid;
     h.itrmax = itrmax;
     h.revernorm = c.revernorm;
+    h.move_particles = c.move_particles;
+    h.rhos = c.rhos;
+    h.rhof = c.rhof;
     h.s.count_fluid = (double)((long long)nx * my * mz);
     HIPCHK(hipMemcpyAsync(dst, &h, sizeof h, hipMemcpyHostToDevice, stream));
     HIPCHK(hipStreamSynchronize(stream));
@@ -993,6 +1017,15 @@ struct Impl : Base {
     if (nb > 0 && !have_points) throw PibFail(PIB_ERR_STATE, "set_points must precede set_particles");
     HIPCHK(hipStreamSynchronize(stream));
     drop_graphs();
+    // the per-particle force sums of the step live on when the same bed
+    // moves: the reference's grid_nodes_insphere leaves them alone
+    std::vector<double> keep;
+    if (have_particles && nb == nbody && nb > 0) {
+      keep.resize(15 * (size_t)nb);
+      double* src[5] = {pres, visc, torq, force, force_chem};
+      for (int a = 0; a < 5; ++a)
+        HIPCHK(hipMemcpy(keep.data() + (size_t)a * 3 * nb, src[a], 3 * nb * sizeof(double), hipMemcpyDeviceToHost));
+    }
     // a moved bed of the same size reuses the previous set's buffers
     for (auto& q : pallocs) ppool.insert({q.second, q.first});
     pallocs.clear();
@@ -1033,6 +1066,12 @@ struct Impl : Base {
       HIPCHK(hipMemcpyAsync(rado, hro.data(), nb * sizeof(double), hipMemcpyHostToDevice, stream));
       if (hvel) HIPCHK(hipMemcpyAsync(vel, hvel, n3 * sizeof(double), hipMemcpyHostToDevice, stream));
       if (hangv) HIPCHK(hipMemcpyAsync(angv, hangv, n3 * sizeof(double), hipMemcpyHostToDevice, stream));
+      if (!keep.empty()) {
+        double* dstp[5] = {pres, visc, torq, force, force_chem};
+        for (int a = 0; a < 5; ++a)
+          HIPCHK(hipMemcpyAsync(dstp[a], keep.data() + (size_t)a * 3 * nb, 3 * nb * sizeof(double),
+                                hipMemcpyHostToDevice, stream));
+      }
     }
     // solid mask and counts
     fill_mask();
@@ -1224,8 +1263,13 @@ struct Impl : Base {
   }
   void nonlinear(int rks) { nonlinear_timed(rks, false); }
   void bcset(int rks) {
+    bcset_drag(rks);
+    bcset_forcing(rks);
+  }
+  // bcset up to where the reference moves particles (boundary_condition.F90:
+  // 517-538): calc_visc, compute_omega, compute_mpg_at_n and the diagnostics
+  void bcset_drag(int rks) {
     Particles P = parts();
-    size_t npts = (size_t)nbody * nbnd;
     c2r_pair<C2R_VISC, 3, C2R_OMEGA, 3>(rks, "c2r_visc_omega");
     const int G = (nbnd + 255) / 256;
     kb("drag_points");
@@ -1237,6 +1281,12 @@ struct Impl : Base {
     k_scalar_drag<<<1, DRAG_T, 0, stream>>>(dst, drag_part, G, nbody, rks, rad, pres, visc, torq, force, force_chem);
     HIPCHK(hipGetLastError());
     ke();
+  }
+  // the rest of bcset (:557-900): calc_pgrad and the reversal forcing
+  void bcset_forcing(int rks) {
+    Particles P = parts();
+    const int G = (nbnd + 255) / 256;
+    const unsigned nbg = (unsigned)((nbody * G + 7) / 8 * 8);
     c2r<C2R_PGRAD>(rks, 3);
     join_branches();
     if (cfg.dorpr) {
@@ -1404,11 +1454,17 @@ struct Impl : Base {
   // gradient (visc/omega transforms, drag, drag scalars, pgrad transform):
   // the two touch disjoint buffers and only the latter reads or writes the
   // device state.  The branches join before the forcing.
-  void nonlinear_timed(int rks, bool rec) {
+  // part 0: the whole of nonlinear; 1: up to the drag (nonlinear_begin);
+  // 2: the forcing (nonlinear_finish)
+  void nonlinear_timed(int rks, bool rec, int part = 0) {
+    if (part == 2) {
+      if (nbody > 0) bcset_forcing(rks);
+      return;
+    }
     std::swap(nl, onl);
     std::swap(nlbc, onlbc);
     bool onlbc_ok = nlbc_valid;
-    forked = use_fork && !ktime && nbody > 0;
+    forked = use_fork && !ktime && nbody > 0 && part == 0;
     if (forked) {
       HIPCHK(hipEventRecord(evfork, stream));
       HIPCHK(hipStreamWaitEvent(stream2, evfork, 0));
@@ -1423,7 +1479,12 @@ struct Impl : Base {
     }
     nlbc_valid = true;
     if (rec) HIPCHK(hipEventRecord(ev[2], stream));
-    if (nbody > 0) bcset(rks);
+    if (nbody > 0) {
+      if (part == 1)
+        bcset_drag(rks);
+      else
+        bcset(rks);
+    }
   }
   void join_branches() {
     if (!forked) return;
@@ -1807,6 +1868,24 @@ int pib_nonlinear(void* ctx, int rks) {
     if (rks < 1 || rks > I.itrmax) throw PibFail(PIB_ERR_ARG, "rks out of range");
     if (!I.have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
     I.nonlinear(rks);
+  });
+  return finish_status(ctx, st);
+}
+int pib_nonlinear_begin(void* ctx, int rks) {
+  int st = dispatch(ctx, [&](auto& I) {
+    if (rks < 1 || rks > I.itrmax) throw PibFail(PIB_ERR_ARG, "rks out of range");
+    if (!I.have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
+    I.nonlinear_timed(rks, false, 1);
+    I.nl_pending = rks;
+  });
+  return finish_status(ctx, st);
+}
+int pib_nonlinear_finish(void* ctx, int rks) {
+  int st = dispatch(ctx, [&](auto& I) {
+    if (I.nl_pending != rks) throw PibFail(PIB_ERR_STATE, "pib_nonlinear_begin with this rks first");
+    if (!I.have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
+    I.nonlinear_timed(rks, false, 2);
+    I.nl_pending = 0;
   });
   return finish_status(ctx, st);
 }

@@ -26,6 +26,8 @@ struct DevState {
   int nx, my, mz, nbody, nbnd, adaptive, soft_start, set_mpg, include_frmeanfluid, itrmax;
   int revernorm;
   int status;  // 0 ok, else a PIB_ERR_* code raised on the device
+  int move_particles, pad_;
+  double rhos, rhof;
 };
 
 template <typename T>

@@ -19,9 +19,9 @@ module pib_types
      real(c_double) :: dx, vis, cfl_max
      integer(c_int) :: rk3, adaptive, discr_scheme, limiter
      integer(c_int) :: revernorm, dorpr, dobnd, include_frmeanfluid
-     integer(c_int) :: set_mpg, soft_start, reserved_i(2)
-     real(c_double) :: ucharmod, char_length, dia, dr, voldom
-     real(c_double) :: reserved_d(8)
+     integer(c_int) :: set_mpg, soft_start, move_particles, reserved_i(1)
+     real(c_double) :: ucharmod, char_length, dia, dr, voldom, rhos, rhof
+     real(c_double) :: reserved_d(6)
   end type pib_config
 
   type, bind(C) :: pib_scalars
@@ -38,7 +38,8 @@ module pib_types
      real(c_double) :: dtnew, umax_tmp, divmax
      real(c_double) :: count_solid, count_fluid
      real(c_double) :: ramp_frac_time
-     real(c_double) :: reserved(15)
+     real(c_double) :: frame_vel(3), frame_pos(3)
+     real(c_double) :: reserved(9)
   end type pib_scalars
 
   ! soft-sphere DEM (include/pureibm_dem.h)
@@ -171,6 +172,19 @@ module pib_types
        integer(c_int), value :: rks
        integer(c_int) :: st
      end function pib_nonlinear
+     ! pib_nonlinear split where bcset calls des_time_march (boundary_condition.F90:543)
+     function pib_nonlinear_begin(ctx, rks) bind(C, name="pib_nonlinear_begin") result(st)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: rks
+       integer(c_int) :: st
+     end function pib_nonlinear_begin
+     function pib_nonlinear_finish(ctx, rks) bind(C, name="pib_nonlinear_finish") result(st)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: rks
+       integer(c_int) :: st
+     end function pib_nonlinear_finish
      function pib_velstep(ctx, rks) bind(C, name="pib_velstep") result(st)
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx

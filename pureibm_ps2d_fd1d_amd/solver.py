@@ -122,6 +122,35 @@ class Solver:
     def step(self, nsteps=1):
         self._call("step", self.ctx, nsteps)
 
+    def nonlinear_begin(self, rks=1):
+        """nonlinear(rks) up to where bcset moves particles (form_nl, the
+        transforms, calc_visc, compute_mpg_at_n)."""
+        self._call("nonlinear_begin", self.ctx, rks)
+
+    def nonlinear_finish(self, rks=1):
+        """The rest of bcset: calc_pgrad and the reversal forcing."""
+        self._call("nonlinear_finish", self.ctx, rks)
+
+    def step_moving(self, dem, nsteps=1):
+        """Time steps of a freely moving bed: per stage rks, compute_new_timestep,
+        nonlinear up to the drag, des_time_march over dt with the step's
+        PRES/VISC/mpg/frame_accln (``dem``, a ``Dem``), grid_nodes_insphere and
+        update_nrpr_array for the moved centres, the forcing, velstep
+        (dtibm_main_scal_flo.F90:426-527 with boundary_condition.F90:539-554).
+        The context must have been created with move_particles set."""
+        if not self.cfg.move_particles:
+            raise ValueError("the run set-up must set move_particles")
+        for _ in range(nsteps):
+            for rks in range(1, self.itrmax + 1):
+                self.compute_new_timestep(rks)
+                self.nonlinear_begin(rks)
+                f, s = self.forces(), self.scalars()
+                xc, vel, angv = dem.march(s.dt, pres=f["pres"], visc=f["visc"], mpg=np.array(s.mpg),
+                                          frame_accln=np.array(s.frame_accln))
+                self.update_particles(xc, vel, angv)
+                self.nonlinear_finish(rks)
+                self.velstep(rks)
+
     def update_particles(self, xc, velbdy=None, angv=None):
         """Moved particles: what bcset does after des_time_march when particles
         move (boundary_condition.F90:539-554) -- the solid mask of

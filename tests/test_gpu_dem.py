@@ -159,3 +159,41 @@ def test_dem_moves_the_bed_of_a_fluid_step(orc, tmp_path):
         assert abs(g.scalars().norm_drag - o.scalars().norm_drag) <= 1e-9 * abs(o.scalars().norm_drag)
     for x in (g, o, dg, do):
         x.close()
+
+
+def test_moving_bed_steps_match_the_oracle(orc, tmp_path):
+    """Freely moving bed, FP64: each step runs nonlinear to the drag, the DEM
+    over dt with the moving-particle mpg and frame_accln, the new mask and
+    reversal pairs, the forcing and velstep -- the GPU chain (GPU DEM) against
+    the oracle chain (oracle DEM)."""
+    st = make_setup(tmp_path, "MB", kind="random", my=32, dbydx=8, nbody=6, seed=3)
+    st.config.move_particles = 1
+    st.config.rhos, st.config.rhof = 2.0, 1.0
+    g = solver.Solver(st)
+    o = solver.Solver(st, lib=orc.load(), prefix="orc_")
+    dx = st.config.dx
+    L = (st.config.nx * dx, st.config.my * dx, st.config.mz * dx)
+    vel0 = np.random.default_rng(5).uniform(-0.3, 0.3, (st.nbody, 3))
+    kw = dict(dx=dx, xlength=L[0], ylength=L[1], zlength=L[2], coeff_rest=0.8, rhos=2.0, rhof=1.0,
+              dia_phys=st.config.char_length)
+    dg = solver.Dem(st.xc, st.radbdy, velbdy=vel0, **kw)
+    do = solver.Dem(st.xc, st.radbdy, velbdy=vel0, lib=orc.load(), prefix="orc_", **kw)
+    for s_ in (g, o):
+        s_.update_particles(st.xc, vel0)
+    solids = set()
+    for k in range(5):
+        g.step_moving(dg, 1)
+        o.step_moving(do, 1)
+        solids.add(g.count_solid)
+        assert g.count_solid == o.count_solid and np.array_equal(g.nrpr_active, o.nrpr_active)
+        assert rel_err(g.field("u"), o.field("u")) < 1e-10
+        sg, so = g.scalars(), o.scalars()
+        for name in ("mpg", "frame_accln", "frame_vel", "umean"):
+            assert rel_err(np.array(getattr(sg, name)), np.array(getattr(so, name))) < 1e-9, name
+        assert abs(sg.norm_drag - so.norm_drag) <= 1e-9 * abs(so.norm_drag)
+        pg, po = dg.state()[0], do.state()[0]
+        assert np.max(np.abs(pg - po)) < 1e-10 * L[0]
+    assert any(np.array(sg.frame_accln) != 0.0)
+    assert len(solids) > 1          # the mask changed as the bed moved
+    for x in (g, o, dg, do):
+        x.close()
