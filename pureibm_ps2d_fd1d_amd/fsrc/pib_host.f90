@@ -101,7 +101,7 @@ module pib_host
   public :: pib_host_read, pib_host_configure, pib_host_finish, pib_host_counts, &
        pib_host_get_config, pib_host_get_particles, pib_host_get_points, &
        pib_host_last_error, pib_host_quadgener, pib_host_get_nsteps_hint, &
-       pib_host_set_option, pib_host_time_scales
+       pib_host_set_option, pib_host_time_scales, pib_host_save_restart, pib_host_read_restart
 contains
 
   subroutine calculate_constants
@@ -561,6 +561,257 @@ contains
     integer(c_int) :: ns
     ns = nerr_steps
   end function pib_host_get_nsteps_hint
+
+  ! ---- restart files (restart_funcs.F90) ------------------------------------
+  ! save_part_restart + save_unformatted (:69-128, 188-362) for the flow of a
+  ! one-phase run: the count alternates 1, 2 through <RUN>_RESTART; the
+  ! records of <RUN>_sphere_config.rst (and _sphere_config_<c>.rst when
+  ! IMOVE = 1), <RUN>_u_<c>.rst, <RUN>_nl_<c>.rst and <RUN>_p_<c>.rst are
+  ! written in the reference's order and kinds with FORM = RSTSAVE
+  ! ("unformatted": sequential records); the other count's files are deleted.
+  ! The granular-temperature history of moving runs is not tracked here and
+  ! is written as its initial value (error 1, old 0, history 1).
+  function pib_host_save_restart(run, n, s, nb, xcin, radin, velin, nx_, my2_, mz_, u, nl, p, &
+       ferr, count_out) bind(C, name="pib_host_save_restart") result(st)
+    character(kind=c_char), intent(in) :: run(*)
+    integer(c_int), value :: n, nb, nx_, my2_, mz_
+    type(pib_scalars), intent(in) :: s
+    real(c_double), intent(in) :: xcin(nb, 3), radin(nb), velin(nb, 3)
+    complex(c_double_complex), intent(in) :: u(nx_ + 1, my2_, mz_, 3), nl(nx_ + 1, my2_, mz_, 3)
+    complex(c_double_complex), intent(in) :: p(nx_, my2_, mz_)
+    real(c_double), intent(in) :: ferr(*)
+    integer(c_int), intent(out) :: count_out
+    integer(c_int) :: st
+    character(len=1024) :: run_name
+    character(len=1100) :: fname
+    character(len=10) :: c1
+    integer :: ru, ios, count_routine, count_tmp, iphs
+    logical :: ex
+    real(prcn), allocatable :: ones(:)
+    st = PIB_OK
+    count_out = 0
+    call cstr(run, n, run_name)
+    if (nb /= nbody) then
+       last_err = 'save_restart: nbody differs from the configured run'; st = PIB_ERR_ARG; return
+    end if
+    count_routine = 0
+    inquire(file=trim(run_name)//'_RESTART', exist=ex)
+    if (ex) then
+       open(newunit=ru, file=trim(run_name)//'_RESTART', form='formatted', status='old', iostat=ios)
+       if (ios == 0) read(ru, *, iostat=ios) count_routine
+       close(ru)
+       if (ios /= 0) count_routine = 0
+    end if
+    count_routine = count_routine + 1
+    if (count_routine >= 3) count_routine = 1
+    write(c1, '(I1)') count_routine
+    allocate(ones(max(nerr_steps, 1)))
+    ones = 1.d0
+    ! sphere configuration
+    open(newunit=ru, file=trim(run_name)//'_sphere_config.rst', form=trim(rstsave), status='unknown', &
+         iostat=ios)
+    if (ios /= 0) then
+       last_err = 'save_restart: cannot open the sphere configuration file'; st = PIB_ERR_ARG; return
+    end if
+    call write_config(ru, .false.)
+    close(ru, status='keep')
+    if (imove == 1) then
+       open(newunit=ru, file=trim(run_name)//'_sphere_config_'//trim(c1)//'.rst', form=trim(rstsave), &
+            status='unknown')
+       call write_config(ru, .true.)
+       close(ru, status='keep')
+    end if
+    ! flow
+    open(newunit=ru, file=trim(run_name)//'_u_'//trim(c1)//'.rst', form=trim(rstsave), status='replace')
+    write(ru) s%t, s%dt
+    write(ru) s%umean(1), s%umean(2), s%umean(3)
+    write(ru) s%usmean_des(1), s%usmean_des(2), s%usmean_des(3)
+    write(ru) s%ufmean_des(1), s%ufmean_des(2), s%ufmean_des(3)
+    write(ru) s%cf
+    write(ru) nerr_steps
+    write(ru) s%ferror, s%fold
+    write(ru) ferr(1:nerr_steps)
+    do iphs = 1, nphases
+       write(ru) s%ferror, s%fold
+       if (imove == 1) write(ru) 1.d0, 0.d0
+    end do
+    do iphs = 1, nphases
+       write(ru) ferr(1:nerr_steps)
+       if (imove == 1) write(ru) ones(1:nerr_steps)
+    end do
+    write(ru) u
+    close(ru, status='keep')
+    open(newunit=ru, file=trim(run_name)//'_nl_'//trim(c1)//'.rst', form=trim(rstsave), status='replace')
+    write(ru) nl
+    close(ru, status='keep')
+    open(newunit=ru, file=trim(run_name)//'_p_'//trim(c1)//'.rst', form=trim(rstsave), status='replace')
+    write(ru) s%mpg(1), s%mpg(2), s%mpg(3)
+    write(ru) p
+    close(ru, status='keep')
+    open(newunit=ru, file=trim(run_name)//'_RESTART', form='formatted', status='replace')
+    write(ru, *) c1
+    close(ru, status='keep')
+    ! delete_restart_files of the other count (:364-395)
+    count_tmp = 3 - count_routine
+    write(c1, '(I1)') count_tmp
+    call remove_file(trim(run_name)//'_u_'//trim(c1)//'.rst')
+    call remove_file(trim(run_name)//'_nl_'//trim(c1)//'.rst')
+    call remove_file(trim(run_name)//'_p_'//trim(c1)//'.rst')
+    if (imove == 1) call remove_file(trim(run_name)//'_sphere_config_'//trim(c1)//'.rst')
+    count_out = count_routine
+  contains
+    subroutine write_config(unit, moving)
+      integer, intent(in) :: unit
+      logical, intent(in) :: moving
+      integer :: k
+      write(unit) nbody
+      if (trim(input_type) == 'random' .or. trim(input_type) == 'default') then
+         write(unit) nphases
+         do k = 1, nphases
+            write(unit) nbody
+         end do
+         do k = 1, nphases
+            write(unit) cfg%dia
+         end do
+         write(unit) doml(1:3)
+         if (moving) write(unit) (cfg%move_particles /= 0)
+      end if
+      write(unit) char_length
+      write(unit) xcin(1:nbody, 1:3)
+      write(unit) radin(1:nbody)
+      write(unit) velin(1:nbody, 1:3)
+      if (moving) then
+         write(unit) s%frame_vel(1:3)
+         write(unit) s%frame_pos(1:3)
+      end if
+    end subroutine write_config
+  end function pib_host_save_restart
+
+  subroutine remove_file(name)
+    character(len=*), intent(in) :: name
+    integer :: ru, ios
+    logical :: ex
+    inquire(file=name, exist=ex)
+    if (.not. ex) return
+    open(newunit=ru, file=name, status='old', iostat=ios)
+    if (ios == 0) close(ru, status='delete')
+  end subroutine remove_file
+
+  ! read_unformatted (restart_funcs.F90:431-616) for the flow of a one-phase
+  ! run, the count taken from <RUN>_RESTART; also reads the sphere
+  ! configuration of <RUN>_sphere_config.rst.  Scalars read: t (TSTART), dt,
+  ! umean, usmean_des, ufmean_des, cf (and cforig), ferror, fold, mpg; the
+  ! ferror history keeps the first min(nerr_steps of the file, NERR_STEPS)
+  ! entries and 1 after them, as the reference does.
+  function pib_host_read_restart(run, n, s, nb, xco, rado, velo, nx_, my2_, mz_, u, nl, p, ferr, &
+       count_out) bind(C, name="pib_host_read_restart") result(st)
+    character(kind=c_char), intent(in) :: run(*)
+    integer(c_int), value :: n, nb, nx_, my2_, mz_
+    type(pib_scalars), intent(inout) :: s
+    real(c_double), intent(out) :: xco(nb, 3), rado(nb), velo(nb, 3)
+    complex(c_double_complex), intent(out) :: u(nx_ + 1, my2_, mz_, 3), nl(nx_ + 1, my2_, mz_, 3)
+    complex(c_double_complex), intent(out) :: p(nx_, my2_, mz_)
+    real(c_double), intent(out) :: ferr(*)
+    integer(c_int), intent(out) :: count_out
+    integer(c_int) :: st
+    character(len=1024) :: run_name
+    character(len=10) :: c1
+    integer :: ru, ios, count_routine, nerr_tmp, iphs, nb_file, nph, k, itmp
+    real(prcn) :: dtmp, d3(3), e1, e2
+    real(prcn), allocatable :: skip(:)
+    st = PIB_OK
+    count_out = 0
+    call cstr(run, n, run_name)
+    open(newunit=ru, file=trim(run_name)//'_RESTART', form='formatted', status='old', iostat=ios)
+    if (ios /= 0) then
+       last_err = 'read_restart: no '//trim(run_name)//'_RESTART file'; st = PIB_ERR_ARG; return
+    end if
+    read(ru, *, iostat=ios) count_routine
+    close(ru)
+    if (ios /= 0 .or. count_routine < 1 .or. count_routine > 2) then
+       last_err = 'read_restart: bad count in the _RESTART file'; st = PIB_ERR_ARG; return
+    end if
+    write(c1, '(I1)') count_routine
+    ! sphere configuration
+    open(newunit=ru, file=trim(run_name)//'_sphere_config.rst', form=trim(rstsave), status='old', iostat=ios)
+    if (ios /= 0) then
+       last_err = 'read_restart: no sphere configuration file'; st = PIB_ERR_ARG; return
+    end if
+    read(ru) nb_file
+    if (nb_file /= nb) then
+       close(ru)
+       last_err = 'read_restart: nbody of the file differs'; st = PIB_ERR_ARG; return
+    end if
+    if (trim(input_type) == 'random' .or. trim(input_type) == 'default') then
+       read(ru) nph
+       do k = 1, nph
+          read(ru) itmp
+       end do
+       do k = 1, nph
+          read(ru) dtmp
+       end do
+       read(ru) d3
+    end if
+    read(ru) dtmp
+    read(ru) xco
+    read(ru) rado
+    read(ru) velo
+    close(ru)
+    ! flow
+    open(newunit=ru, file=trim(run_name)//'_u_'//trim(c1)//'.rst', form=trim(rstsave), status='old', iostat=ios)
+    if (ios /= 0) then
+       last_err = 'read_restart: no u file for the count'; st = PIB_ERR_ARG; return
+    end if
+    read(ru) s%t, s%dt
+    read(ru) s%umean(1), s%umean(2), s%umean(3)
+    read(ru) s%usmean_des(1), s%usmean_des(2), s%usmean_des(3)
+    read(ru) s%ufmean_des(1), s%ufmean_des(2), s%ufmean_des(3)
+    read(ru) s%cf
+    s%cforig = s%cf
+    read(ru) nerr_tmp
+    read(ru) s%ferror, s%fold
+    allocate(skip(max(nerr_tmp, nerr_steps, 1)))
+    ferr(1:nerr_steps) = 1.d0
+    if (nerr_tmp > nerr_steps) then
+       read(ru) ferr(1:nerr_steps)
+    else
+       read(ru) ferr(1:nerr_tmp)
+    end if
+    do iphs = 1, nphases
+       read(ru) e1, e2
+       if (imove == 1 .and. .not. moving_from_fixed_bed) read(ru) e1, e2
+    end do
+    do iphs = 1, nphases
+       if (nerr_tmp > nerr_steps) then
+          read(ru) skip(1:nerr_steps)
+       else
+          read(ru) skip(1:nerr_tmp)
+       end if
+       if (imove == 1 .and. .not. moving_from_fixed_bed) then
+          if (nerr_tmp > nerr_steps) then
+             read(ru) skip(1:nerr_steps)
+          else
+             read(ru) skip(1:nerr_tmp)
+          end if
+       end if
+    end do
+    read(ru) u
+    close(ru)
+    open(newunit=ru, file=trim(run_name)//'_nl_'//trim(c1)//'.rst', form=trim(rstsave), status='old', iostat=ios)
+    if (ios /= 0) then
+       last_err = 'read_restart: no nl file for the count'; st = PIB_ERR_ARG; return
+    end if
+    read(ru) nl
+    close(ru)
+    open(newunit=ru, file=trim(run_name)//'_p_'//trim(c1)//'.rst', form=trim(rstsave), status='old', iostat=ios)
+    if (ios /= 0) then
+       last_err = 'read_restart: no p file for the count'; st = PIB_ERR_ARG; return
+    end if
+    read(ru) s%mpg(1), s%mpg(2), s%mpg(3)
+    read(ru) p
+    close(ru)
+    count_out = count_routine
+  end function pib_host_read_restart
 
   ! Byte sizes of the bind(C) mirrors in pib_types, so that callers can check
   ! them against the C declarations.

@@ -171,6 +171,67 @@ class Solver:
         self.nrpr_active = nr
         return cs.value, nr
 
+    # ---- restart files (restart_funcs.F90) ----
+    def save_restart(self, xc=None, velbdy=None, ferror_array=None):
+        """save_part_restart (restart_funcs.F90:69-128): writes the reference's
+        restart files <RUN>_u_<c>.rst, _nl_<c>.rst, _p_<c>.rst,
+        _sphere_config.rst and <RUN>_RESTART for this context's state, the
+        count c alternating 1, 2.  xc/velbdy default to the set-up's centres
+        at rest (pass the current ones for a moving bed); ferror_array is the
+        NERR_STEPS history of the caller's convergence check (ones when not
+        kept).  Returns c."""
+        st = self.setup
+        st._load()
+        host = _abi.load_host()
+        nerr = max(1, host.pib_host_get_nsteps_hint())
+        u, nl, p = (np.asfortranarray(self.field(k)) for k in ("u", "nl", "p"))
+        s = self.scalars()
+        xc = np.asfortranarray(st.xc if xc is None else xc, dtype=np.float64)
+        vel = np.asfortranarray(np.zeros((st.nbody, 3)) if velbdy is None else velbdy, dtype=np.float64)
+        ferr = np.ones(nerr) if ferror_array is None else np.ascontiguousarray(ferror_array, dtype=np.float64)
+        if ferr.size < nerr:
+            raise ValueError(f"ferror_array needs NERR_STEPS = {nerr} entries")
+        cnt = ctypes.c_int()
+        rn = st.run_name.encode()
+        rc = host.pib_host_save_restart(rn, len(rn), ctypes.byref(s), st.nbody, dptr(xc), dptr(st.radbdy),
+                                        dptr(vel), self.nx, self.my2, self.mz, dptr(u), dptr(nl), dptr(p),
+                                        dptr(ferr), ctypes.byref(cnt))
+        if rc != PIB_OK:
+            raise PibError(rc, _abi.host_error())
+        return cnt.value
+
+    def load_restart(self):
+        """read_restart (restart_funcs.F90:411-616): reads the files of the
+        count named in <RUN>_RESTART into this context (u, nl, p and the
+        restart scalars; the centres too when they differ from the set-up's).
+        Returns (count, ferror_array, xc, velbdy)."""
+        st = self.setup
+        st._load()
+        host = _abi.load_host()
+        nerr = max(1, host.pib_host_get_nsteps_hint())
+        nx, my2, mz = self.nx, self.my2, self.mz
+        u = np.zeros((nx + 1, my2, mz, 3), dtype=np.complex128, order="F")
+        nl = np.zeros_like(u)
+        p = np.zeros((nx, my2, mz), dtype=np.complex128, order="F")
+        xc = np.zeros((st.nbody, 3), order="F")
+        rad = np.zeros(st.nbody)
+        vel = np.zeros((st.nbody, 3), order="F")
+        ferr = np.zeros(nerr)
+        s = self.scalars()
+        cnt = ctypes.c_int()
+        rn = st.run_name.encode()
+        rc = host.pib_host_read_restart(rn, len(rn), ctypes.byref(s), st.nbody, dptr(xc), dptr(rad), dptr(vel),
+                                        nx, my2, mz, dptr(u), dptr(nl), dptr(p), dptr(ferr), ctypes.byref(cnt))
+        if rc != PIB_OK:
+            raise PibError(rc, _abi.host_error())
+        self.set_field("u", u)
+        self.set_field("nl", nl)
+        self.set_field("p", p)
+        self.set_scalars(s)
+        if not np.array_equal(xc, st.xc) or vel.any():
+            self.update_particles(xc, vel)
+        return cnt.value, ferr, xc, vel
+
     # ---- state access in the reference layout ----
     def _shape(self, name):
         nx, my, mz, my2 = self.nx, self.my, self.mz, self.my2
