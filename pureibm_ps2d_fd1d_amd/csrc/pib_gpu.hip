@@ -119,6 +119,17 @@ __global__ void k_if_rev(Particles P, int nx, int my, int mz, const int* nb, con
   if_rev[idx] = rev;
 }
 
+// First target of each spreading range: range k holds the targets whose
+// first record lies in [k*B, (k+1)*B); bt[nblk] = ntarget.  Target offsets
+// are non-decreasing, so every range start has exactly one writer.
+__global__ void k_bucket_starts(const int* toff, int ntarget, int B, int nblk, int* bt) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > ntarget) return;
+  const int b = t < ntarget ? toff[t] / B : nblk;
+  const int bp = t > 0 ? toff[t - 1] / B : -1;
+  for (int k = bp + 1; k <= b; ++k) bt[k] = t;
+}
+
 __global__ void k_count_active(const uint8_t* if_rev, int nbnd, int* nrpr_active) {
   __shared__ double sh[256];
   int m = blockIdx.x;
@@ -1163,45 +1174,47 @@ struct Impl : Base {
     int hruns = 0;
     HIPCHK(hipMemcpyAsync(&hruns, nruns, sizeof(int), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
-    std::vector<unsigned int> hk(hruns);
-    std::vector<int> hc(hruns);
-    HIPCHK(hipMemcpy(hk.data(), ukeys, hruns * sizeof(unsigned int), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(hc.data(), counts, hruns * sizeof(int), hipMemcpyDeviceToHost));
-    std::vector<int> hnode, hoff;
-    int acc = 0;
-    for (int r = 0; r < hruns; ++r) {
-      if (hk[r] >= sentinel) break;
-      hnode.push_back((int)hk[r]);
-      hoff.push_back(acc);
-      acc += hc[r];
+    // the sentinel key (records that deposit nowhere) sorts last
+    unsigned int lastkey = 0;
+    if (hruns > 0) HIPCHK(hipMemcpy(&lastkey, ukeys + hruns - 1, sizeof(unsigned int), hipMemcpyDeviceToHost));
+    ntarget = hruns - (hruns > 0 && lastkey >= sentinel ? 1 : 0);
+    // target nodes are the run keys; record offsets their exclusive prefix sum
+    tnode = (int*)ukeys;
+    toff = alloc<int>(nrec + 2, true);
+    int* dmax = alloc<int>(1, true);
+    size_t tb3 = 0, tb4 = 0;
+    // temporary storage sized for the record bound, so that it is reused too
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, counts, toff, (int)nrec + 1, stream));
+    HIPCHK(hipcub::DeviceReduce::Max(nullptr, tb4, counts, dmax, (int)nrec, stream));
+    void* tmp3 = alloc<unsigned char>(std::max(tb3, tb4) + 16, true);
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(tmp3, tb3, counts, toff, ntarget + 1, stream));
+    int acc = 0, maxcnt = 0;
+    if (ntarget > 0) {
+      HIPCHK(hipcub::DeviceReduce::Max(tmp3, tb4, counts, dmax, ntarget, stream));
+      HIPCHK(hipMemcpyAsync(&maxcnt, dmax, sizeof(int), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipMemcpyAsync(&acc, toff + ntarget, sizeof(int), hipMemcpyDeviceToHost, stream));
     }
-    hoff.push_back(acc);
-    ntarget = (int)hnode.size();
-    // cut the targets into workgroup ranges of at most SPREAD_CAP records
-    std::vector<int> hbt(1, 0);
-    for (int t = 0; t < ntarget; ++t) {
-      int cnt = hoff[t + 1] - hoff[t];
-      if (cnt > SPREAD_CAP) throw PibFail(PIB_ERR_UNSUPPORTED, "more reversal records on one node than a workgroup holds");
-      if (hoff[t + 1] - hoff[hbt.back()] > SPREAD_CAP) hbt.push_back(t);
-    }
-    if (ntarget > 0) hbt.push_back(ntarget);
-    nspread_blk = (int)hbt.size() - 1;
-    spread_bt = alloc<int>(hbt.size(), true);
-    // uploads go on the context stream, after the zero fill that alloc queued there
-    HIPCHK(hipMemcpyAsync(spread_bt, hbt.data(), hbt.size() * sizeof(int), hipMemcpyHostToDevice, stream));
-    tnode = alloc<int>(hnode.size() + 1, true);
-    toff = alloc<int>(hoff.size(), true);
-    if (!hnode.empty())
-      HIPCHK(hipMemcpyAsync(tnode, hnode.data(), hnode.size() * sizeof(int), hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(toff, hoff.data(), hoff.size() * sizeof(int), hipMemcpyHostToDevice, stream));
-    rpt = alloc<int>((size_t)acc + 1, true);
-    rw = alloc<double>((size_t)acc + 1, true);
-    rdist = alloc<uint8_t>((size_t)acc + 1, true);
+    HIPCHK(hipStreamSynchronize(stream));
+    if (maxcnt > SPREAD_CAP)
+      throw PibFail(PIB_ERR_UNSUPPORTED, "more reversal records on one node than a workgroup holds");
+    // workgroup ranges: the targets whose first record falls in one bucket of
+    // B = SPREAD_CAP - maxcnt + 1 records, so that a range holds at most
+    // SPREAD_CAP records (buffers sized by the record bound, so that a moved
+    // bed of the same size finds them in the reuse pool)
+    const int B = SPREAD_CAP - maxcnt + 1;
+    nspread_blk = ntarget > 0 ? (acc + B - 1) / B : 0;
+    spread_bt = alloc<int>(nrec + 2, true);
+    k_bucket_starts<<<(unsigned)((ntarget + 1 + 255) / 256), 256, 0, stream>>>(toff, ntarget, B, nspread_blk,
+                                                                                 spread_bt);
+    HIPCHK(hipGetLastError());
+    rpt = alloc<int>(nrec + 1, true);
+    rw = alloc<double>(nrec + 1, true);
+    rdist = alloc<uint8_t>(nrec + 1, true);
     if (acc > 0) {
       k_gather_records<<<(acc + 255) / 256, 256, 0, stream>>>(acc, rec_s, w, dist, rpt, rw, rdist);
       HIPCHK(hipGetLastError());
     }
-    HIPCHK(hipStreamSynchronize(stream));  // host vectors of the uploads end here
+    HIPCHK(hipStreamSynchronize(stream));
   }
 
   // ---------------- per-step routines ----------------
