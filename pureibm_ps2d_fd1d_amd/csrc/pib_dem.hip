@@ -111,7 +111,40 @@ __device__ inline int wrap_cell(int ii, int imax1, double& f) {
 struct Lds {
   double *px, *py, *pz, *vx, *vy, *vz, *rad;
   unsigned short *cl, *cc;  // candidate lists (cap, n) and counts
+  unsigned short* perm;     // particles by descending candidate count: lane j of a sub-step
+                            // handles perm[j], so the lanes of a wave loop alike
 };
+
+// Order the particles by descending candidate count (counting sort; the
+// order within a count does not matter, since no particle's arithmetic
+// depends on the lane that runs it).
+__device__ void make_perm(const DemArgs& A, const Lds& S, int* hist) {
+  const int n = A.n, tid = threadIdx.x, nb = A.cap + 1;
+  for (int i = tid; i < nb; i += DEM_NT) hist[i] = 0;
+  __syncthreads();
+  for (int p = tid; p < n; p += DEM_NT) atomicAdd(&hist[A.cap - S.cc[p]], 1);
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int i = 0; i < nb; ++i) {
+      const int c = hist[i];
+      hist[i] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+  for (int p = tid; p < n; p += DEM_NT) {
+    const int pos = atomicAdd(&hist[A.cap - S.cc[p]], 1);
+    S.perm[pos] = (unsigned short)p;
+  }
+  __syncthreads();
+}
+
+// the particle of slot k of this lane, or n when there is none
+__device__ inline int owned(const Lds& S, int n, int k) {
+  const int j = threadIdx.x + k * DEM_NT;
+  return j < n ? S.perm[j] : n;
+}
 
 // Position of b in a's 27-cell scan (0..26) with the first image that passes
 // the neighbour test 1.2*(r_a+r_b) - |d| > SMALL_NUMBER (:955-1003), or -1
@@ -260,6 +293,8 @@ __global__ __launch_bounds__(DEM_NT) void k_dem_march(DemArgs A) {
   unsigned long long* s_ovmax = (unsigned long long*)(S.rad + n);
   S.cc = (unsigned short*)(s_ovmax + 2);
   S.cl = S.cc + n;
+  S.perm = S.cl + (size_t)A.cap * n;
+  __shared__ int s_hist[DEM_VCAP + 2];
   int* s_red = (int*)(s_ovmax + 1);  // [0] contacts, [1] rebuilds
   if (tid == 0) { *s_ovmax = 0ull; s_red[0] = 0; s_red[1] = 0; }
   for (int p = tid; p < n; p += DEM_NT) {
@@ -273,25 +308,22 @@ __global__ __launch_bounds__(DEM_NT) void k_dem_march(DemArgs A) {
   }
   __syncthreads();
   int err = 0, nreb = 0;
-  double dsp[DEM_PPT][3], a0r[DEM_PPT][3];
-#pragma unroll
-  for (int k = 0; k < DEM_PPT; ++k) {
-    const int p = tid + k * DEM_NT;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      dsp[k][c] = p < n ? A.disp[c * n + p] : 0.0;
-      a0r[k][c] = p < n ? A.a0[c * n + p] : 0.0;
-    }
-  }
   if (A.rebuild) {
 #pragma unroll 1
     for (int p = tid; p < n; p += DEM_NT) err |= build_candidates(A, S, p);
-#pragma unroll
-    for (int k = 0; k < DEM_PPT; ++k)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) dsp[k][c] = 0.0;
     ++nreb;
     __syncthreads();
+  }
+  make_perm(A, S, s_hist);
+  double dsp[DEM_PPT][3], a0r[DEM_PPT][3];
+#pragma unroll
+  for (int k = 0; k < DEM_PPT; ++k) {
+    const int p = owned(S, n, k);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      dsp[k][c] = p < n && !A.rebuild ? A.disp[c * n + p] : 0.0;
+      a0r[k][c] = p < n ? A.a0[c * n + p] : 0.0;
+    }
   }
   double ovmax = 0.0;
   int ncont = 0;
@@ -304,7 +336,7 @@ __global__ __launch_bounds__(DEM_NT) void k_dem_march(DemArgs A) {
     int touched = 0;  // bit k: a contact force reached particle k of this thread
 #pragma unroll 1
     for (int k = 0; k < DEM_PPT; ++k) {
-      const int p = tid + k * DEM_NT;
+      const int p = owned(S, n, k);
       if (p >= n) break;
       double g0, g1, g2;
       if (contact_sum(A, S, p, g0, g1, g2, ovmax, ncont, err)) touched |= 1 << k;
@@ -317,7 +349,7 @@ __global__ __launch_bounds__(DEM_NT) void k_dem_march(DemArgs A) {
     const bool last = s + 1 == A.nsub;
 #pragma unroll 1
     for (int k = 0; k < DEM_PPT; ++k) {
-      const int p = tid + k * DEM_NT;
+      const int p = owned(S, n, k);
       if (p >= n) break;
       double g[3], ds[3], a0[3];
 #pragma unroll
@@ -365,18 +397,24 @@ __global__ __launch_bounds__(DEM_NT) void k_dem_march(DemArgs A) {
     if (__syncthreads_or(need)) {
 #pragma unroll 1
       for (int p = tid; p < n; p += DEM_NT) err |= build_candidates(A, S, p);
-#pragma unroll
-      for (int k = 0; k < DEM_PPT; ++k)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) dsp[k][c] = 0.0;
       ++nreb;
       __syncthreads();
+      make_perm(A, S, s_hist);
+#pragma unroll
+      for (int k = 0; k < DEM_PPT; ++k) {
+        const int p = owned(S, n, k);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          dsp[k][c] = 0.0;
+          a0r[k][c] = p < n ? A.a0[c * n + p] : 0.0;
+        }
+      }
     }
   }
   // state back to HBM, IBMUPDATE (collision_mod.F90:1464-1474)
 #pragma unroll
   for (int k = 0; k < DEM_PPT; ++k) {
-    const int p = tid + k * DEM_NT;
+    const int p = owned(S, n, k);
     if (p >= n) continue;
     A.pos[p] = S.px[p]; A.pos[n + p] = S.py[p]; A.pos[2 * n + p] = S.pz[p];
     A.vel[p] = S.vx[p]; A.vel[n + p] = S.vy[p]; A.vel[2 * n + p] = S.vz[p];
@@ -445,14 +483,15 @@ struct DemFail {
 
 int nint_(double x) { return (int)(x >= 0 ? std::floor(x + 0.5) : -std::floor(-x + 0.5)); }
 
-// candidate slots that fit in LDS beside 7 doubles a particle
+// candidate slots that fit in LDS beside 7 doubles, a count and a slot of
+// the order a particle
 int cand_cap(int n) {
-  long long room = (long long)DEM_LDS - 7LL * 8 * n - 2LL * n - 64;
+  long long room = (long long)DEM_LDS - 7LL * 8 * n - 4LL * n - 64;
   long long c = room / (2LL * n);
   return (int)(c < DEM_VCAP ? c : DEM_VCAP);
 }
 
-size_t lds_bytes(int n, int cap) { return (size_t)n * 7 * sizeof(double) + 16 + 2 * (size_t)n * (1 + cap) + 64; }
+size_t lds_bytes(int n, int cap) { return (size_t)n * 7 * sizeof(double) + 16 + 2 * (size_t)n * (2 + cap) + 64; }
 
 template <typename T>
 T* dalloc(size_t count) {
