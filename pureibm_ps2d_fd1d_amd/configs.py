@@ -62,8 +62,10 @@ def write_case(directory, run_name, *, my, lybyd, dbydx, centers=None, input_typ
                vol_frac1=None, re=20.0, vis=0.012, cfl=0.1, cfl_max=0.5, adaptive=True,
                discr_scheme="center", limiter="none", rk3=False, ramp_frac_steps=0,
                set_mpg=False, mpg=(0.0, 0.0, 0.0), revernorm=1, dr=1.0, f2=1.6,
-               include_frmeanfluid=False, dorpr=True):
-    """Write ``<directory>/<run_name>_floparam.in`` (+ sphr_center file)."""
+               include_frmeanfluid=False, dorpr=True, imove=0, rhos=1.0, coeff_rest=1.0):
+    """Write ``<directory>/<run_name>_floparam.in`` (+ sphr_center file).
+    imove=1 makes a freely moving bed (soft-sphere collisions, particle
+    density rhos, restitution coeff_rest)."""
     os.makedirs(directory, exist_ok=True)
     base = os.path.join(directory, run_name)
     if input_type is None:
@@ -71,9 +73,9 @@ def write_case(directory, run_name, *, my, lybyd, dbydx, centers=None, input_typ
     part = {
         "NBODY": int(len(centers)) if centers is not None else 1,
         "DR": float(dr), "F1": 1.6, "F2": float(f2), "LYBYD": float(lybyd),
-        "DBYDX": float(dbydx), "DIA_PHYS": 1.0, "RHOS": 1.0,
+        "DBYDX": float(dbydx), "DIA_PHYS": 1.0, "RHOS": float(rhos),
         "INPUT_TYPE": input_type, "DISCR_SCHEME": discr_scheme, "LIMITER": limiter,
-        "COLLISION_TYPE": "none",
+        "COLLISION_TYPE": "softsphere" if imove else "none",
     }
     if vol_frac1 is not None:
         part["VOL_FRAC1"] = float(vol_frac1)
@@ -86,14 +88,14 @@ def write_case(directory, run_name, *, my, lybyd, dbydx, centers=None, input_typ
         },
         "PART_PROPT": part,
         "FLAGS": {
-            "IMOVE": 0, "ISCALON": 0, "IRESTART": 0, "ADAPTIVE": bool(adaptive),
+            "IMOVE": int(imove), "ISCALON": 0, "IRESTART": 0, "ADAPTIVE": bool(adaptive),
             "RK3": bool(rk3), "DORPR": bool(dorpr), "DOBND": False,
             "INCLUDE_FRMEANFLUID": bool(include_frmeanfluid), "SET_MPG": bool(set_mpg),
             "REVERNORM": int(revernorm),
         },
         "FILES": {"SAVEITNS": 10, "WRITE_OUTPUT": False},
         "PSD": {"PSD_TYPE": "discrete", "NPHASES": 1},
-        "GCG": {"COEFF_REST": 1.0},
+        "GCG": {"COEFF_REST": float(coeff_rest)},
     }
     write_namelists(base + "_floparam.in", groups)
     if centers is not None:

@@ -322,8 +322,12 @@ contains
     if (aliasflag /= 0) then
        last_err = 'ALIASFLAG = 1 is not supported'; st = PIB_ERR_UNSUPPORTED; return
     end if
-    if (imove /= 0 .or. iscalon /= 0) then
-       last_err = 'IMOVE/ISCALON: moving particles and scalar transport are not built'
+    if (iscalon /= 0) then
+       last_err = 'ISCALON: scalar transport is not built'
+       st = PIB_ERR_UNSUPPORTED; return
+    end if
+    if (imove == 1 .and. (cage_simul .or. only_dem)) then
+       last_err = 'IMOVE with CAGE_SIMUL or ONLY_DEM: run the DEM alone through pib_dem_* instead'
        st = PIB_ERR_UNSUPPORTED; return
     end if
     if (dobnd) then
@@ -439,6 +443,11 @@ contains
     cfg%set_mpg = merge(1, 0, set_mpg); cfg%soft_start = 1
     cfg%ucharmod = ucharmod; cfg%char_length = char_length; cfg%dia = dia
     cfg%dr = dr; cfg%voldom = doml(1)*doml(2)*doml(3)
+    ! IMOVE = 1 moves the particles from the first step (dtibm_main_scal_flo.F90:
+    ! 426-431); with FIXED_MOVING the reference waits for the fixed-bed flow
+    ! to converge, which the caller decides here by setting the flag later
+    cfg%move_particles = merge(1, 0, imove == 1 .and. .not. fixed_moving)
+    cfg%rhos = rhos; cfg%rhof = rhof
     configured = .true.
   contains
     subroutine set_my_from_lybyd
@@ -699,8 +708,10 @@ contains
 
   ! read_unformatted (restart_funcs.F90:431-616) for the flow of a one-phase
   ! run, the count taken from <RUN>_RESTART; also reads the sphere
-  ! configuration of <RUN>_sphere_config.rst.  Scalars read: t (TSTART), dt,
-  ! umean, usmean_des, ufmean_des, cf (and cforig), ferror, fold, mpg; the
+  ! configuration of <RUN>_sphere_config.rst (of <RUN>_sphere_config_<c>.rst,
+  ! with the frame velocity and position, when IMOVE = 1).  Scalars read:
+  ! t (TSTART), dt, umean, usmean_des, ufmean_des, cf (and cforig), ferror,
+  ! fold, mpg; the
   ! ferror history keeps the first min(nerr_steps of the file, NERR_STEPS)
   ! entries and 1 after them, as the reference does.
   function pib_host_read_restart(run, n, s, nb, xco, rado, velo, nx_, my2_, mz_, u, nl, p, ferr, &
@@ -718,6 +729,7 @@ contains
     character(len=10) :: c1
     integer :: ru, ios, count_routine, nerr_tmp, iphs, nb_file, nph, k, itmp
     real(prcn) :: dtmp, d3(3), e1, e2
+    logical :: lmove
     real(prcn), allocatable :: skip(:)
     st = PIB_OK
     count_out = 0
@@ -757,6 +769,33 @@ contains
     read(ru) rado
     read(ru) velo
     close(ru)
+    ! a moving run keeps the centres, velocities and frame of the count in
+    ! <RUN>_sphere_config_<c>.rst (save_unformatted :229-251)
+    if (imove == 1) then
+       open(newunit=ru, file=trim(run_name)//'_sphere_config_'//trim(c1)//'.rst', form=trim(rstsave), &
+            status='old', iostat=ios)
+       if (ios == 0) then
+          read(ru) nb_file
+          if (trim(input_type) == 'random' .or. trim(input_type) == 'default') then
+             read(ru) nph
+             do k = 1, nph
+                read(ru) itmp
+             end do
+             do k = 1, nph
+                read(ru) dtmp
+             end do
+             read(ru) d3
+             read(ru) lmove
+          end if
+          read(ru) dtmp
+          read(ru) xco
+          read(ru) rado
+          read(ru) velo
+          read(ru) s%frame_vel(1:3)
+          read(ru) s%frame_pos(1:3)
+          close(ru)
+       end if
+    end if
     ! flow
     open(newunit=ru, file=trim(run_name)//'_u_'//trim(c1)//'.rst', form=trim(rstsave), status='old', iostat=ios)
     if (ios /= 0) then

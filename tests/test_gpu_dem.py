@@ -197,3 +197,73 @@ def test_moving_bed_steps_match_the_oracle(orc, tmp_path):
     assert len(solids) > 1          # the mask changed as the bed moved
     for x in (g, o, dg, do):
         x.close()
+
+
+def _moving(tmp_path, name, prec=_abi.PIB_FP64, rk3=False, lib=None, prefix="pib_", vel_seed=5):
+    st = make_setup(tmp_path, name, kind="random", precision=prec, my=32, dbydx=8, nbody=6, seed=3,
+                    imove=1, rhos=2.0, coeff_rest=0.8, rk3=rk3)
+    assert st.config.move_particles == 1 and st.config.rhos == 2.0
+    s = solver.Solver(st, lib=lib, prefix=prefix)
+    dx = st.config.dx
+    L = (st.config.nx * dx, st.config.my * dx, st.config.mz * dx)
+    vel0 = np.random.default_rng(vel_seed).uniform(-0.3, 0.3, (st.nbody, 3))
+    d = solver.Dem(st.xc, st.radbdy, velbdy=vel0, dx=dx, xlength=L[0], ylength=L[1], zlength=L[2],
+                   coeff_rest=0.8, rhos=2.0, rhof=1.0, dia_phys=st.config.char_length, lib=lib,
+                   prefix=prefix)
+    s.update_particles(st.xc, vel0)
+    return st, s, d
+
+
+def test_moving_bed_rk3_matches_the_oracle(orc, tmp_path):
+    """RK3: the DEM runs in every stage, as bcset calls it every stage."""
+    st, g, dg = _moving(tmp_path, "MR", rk3=True)
+    _, o, do = _moving(tmp_path, "MR", rk3=True, lib=orc.load(), prefix="orc_")
+    for _ in range(2):
+        g.step_moving(dg, 1)
+        o.step_moving(do, 1)
+        assert g.count_solid == o.count_solid
+        assert rel_err(g.field("u"), o.field("u")) < 1e-10
+        assert abs(g.scalars().norm_drag - o.scalars().norm_drag) <= 1e-9 * abs(o.scalars().norm_drag)
+    assert dg.state()[4].substeps == do.state()[4].substeps > 0
+    for x in (g, o, dg, do):
+        x.close()
+
+
+def test_moving_bed_fp32_bulk_drag_within_tolerance(tmp_path):
+    """FP32-bulk fluid with the FP64 DEM: mean drag within 1e-4 of FP64."""
+    _, g64, d64 = _moving(tmp_path, "M64")
+    _, g32, d32 = _moving(tmp_path, "M32", prec=_abi.PIB_FP32_BULK)
+    for _ in range(3):
+        g64.step_moving(d64, 1)
+        g32.step_moving(d32, 1)
+    n64, n32 = g64.scalars().norm_drag, g32.scalars().norm_drag
+    assert abs(n32 - n64) <= 1e-4 * abs(n64), (n32, n64)
+    for x in (g64, g32, d64, d32):
+        x.close()
+
+
+def test_moving_bed_restart_continues(tmp_path):
+    """A moving run saved and read back continues with the frame state and
+    the centres of the save: the DEM starts again from XC/VELBDY, as the
+    reference's des_time_march(.TRUE.) does after a restart, so the fields
+    follow the uninterrupted run to round-off, not bit for bit."""
+    st, a, da = _moving(tmp_path, "MS")
+    a.step_moving(da, 2)
+    pos, vel = da.state()[:2]
+    xc = np.asfortranarray(pos / st.config.dx + 1.0)  # IBMUPDATE of the current state
+    a.save_restart(xc=xc, velbdy=vel)
+    sa = a.scalars()
+    b = solver.Solver(st)
+    cnt, _, xr, vr = b.load_restart()
+    assert np.array_equal(xr, xc) and np.array_equal(vr, vel)
+    sb = b.scalars()
+    assert list(sb.frame_vel) == list(sa.frame_vel) and list(sb.frame_pos) == list(sa.frame_pos)
+    dx = st.config.dx
+    L = (st.config.nx * dx, st.config.my * dx, st.config.mz * dx)
+    db = solver.Dem(xr, st.radbdy, velbdy=vr, dx=dx, xlength=L[0], ylength=L[1], zlength=L[2], coeff_rest=0.8,
+                    rhos=2.0, rhof=1.0, dia_phys=st.config.char_length)
+    a.step_moving(da, 2)
+    b.step_moving(db, 2)
+    assert rel_err(b.field("u"), a.field("u")) < 1e-9
+    for x in (a, b, da, db):
+        x.close()

@@ -18,7 +18,8 @@ GOLD = os.path.join(ROOT, "tests", "golden")
 @pytest.mark.parametrize("deck", ["refsample", "refmis1"])
 def test_reads_reference_example_decks(tmp_path, deck):
     """The reference's own example inputs parse (all groups, with the
-    reference's variables); their 'random' configuration is refused cleanly."""
+    reference's variables); what they ask for beyond the GPU path (the
+    'random' configuration generator, scalar transport) is refused cleanly."""
     shutil.copy(os.path.join(GOLD, f"{deck}_floparam.in"), tmp_path / f"{deck}_floparam.in")
     host = _abi.load_host()
     p = str(tmp_path / f"{deck}_floparam.in").encode()
@@ -26,7 +27,7 @@ def test_reads_reference_example_decks(tmp_path, deck):
     rn = str(tmp_path / deck).encode()
     st = host.pib_host_configure(rn, len(rn))
     assert st == _abi.PIB_ERR_UNSUPPORTED
-    assert "IMOVE" in _abi.host_error() or "random" in _abi.host_error()
+    assert "ISCALON" in _abi.host_error() or "random" in _abi.host_error()
 
 
 def test_missing_deck_is_an_error(tmp_path):
