@@ -228,6 +228,7 @@ _HOST_SIGS = {
     "quadgener": ([_D, _D, _I, _PD, _PD], _I),
     "set_option": ([_I, _D], _I),
     "struct_sizes": ([_PI], _I),
+    "get_dem_config": ([ctypes.POINTER(PibDemConfig)], _I),
     "get_nsteps_hint": ([], _I),
     "save_restart": ([ctypes.c_char_p, _I, ctypes.POINTER(PibScalars), _I, _PD, _PD, _PD, _I, _I, _I,
                       _PD, _PD, _PD, _PD, _PI], _I),

@@ -101,7 +101,8 @@ module pib_host
   public :: pib_host_read, pib_host_configure, pib_host_finish, pib_host_counts, &
        pib_host_get_config, pib_host_get_particles, pib_host_get_points, &
        pib_host_last_error, pib_host_quadgener, pib_host_get_nsteps_hint, &
-       pib_host_set_option, pib_host_time_scales, pib_host_save_restart, pib_host_read_restart
+       pib_host_set_option, pib_host_time_scales, pib_host_save_restart, pib_host_read_restart, &
+       pib_host_get_dem_config
 contains
 
   subroutine calculate_constants
@@ -851,6 +852,29 @@ contains
     close(ru)
     count_out = count_routine
   end function pib_host_read_restart
+
+  ! The DEM constants of this run, as des_time_march(.TRUE.) takes them from
+  ! the run (collision_mod.F90:353-390, 1203-1316; initialize_flo.F90:630-635):
+  ! DOML, FOFFSET, RHOS, RHOF, DIA_PHYS, COEFF_REST; KN left at the reference's.
+  function pib_host_get_dem_config(dc) bind(C, name="pib_host_get_dem_config") result(st)
+    type(pib_dem_config), intent(out) :: dc
+    integer(c_int) :: st
+    st = PIB_OK
+    if (.not. configured) then
+       last_err = 'configure the run first'; st = PIB_ERR_STATE; return
+    end if
+    dc%nbody = nbody
+    dc%only_dem = merge(1, 0, only_dem)
+    dc%reserved_i = 0
+    dc%dx = dx; dc%dy = dx; dc%dz = dx
+    dc%xlength = doml(1); dc%ylength = doml(2); dc%zlength = doml(3)
+    dc%foffset = real(foffset, prcn)
+    dc%rhos = rhos; dc%rhof = rhof
+    dc%dia_phys = dia_phys
+    dc%coeff_rest = coeff_rest
+    dc%kn = zero
+    dc%reserved_d = zero
+  end function pib_host_get_dem_config
 
   ! Byte sizes of the bind(C) mirrors in pib_types, so that callers can check
   ! them against the C declarations.

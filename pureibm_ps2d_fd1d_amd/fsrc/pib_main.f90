@@ -9,7 +9,10 @@
 !   RUN_NAME_norm_drag.dat        boundary_condition.F90:861-862
 !   RUN_NAME_drag_components.dat  boundary_condition.F90:865
 ! A blown-up run writes RUN_NAME_BLOWUP.dat and stops, as bcset does
-! (boundary_condition.F90:874-884).
+! (boundary_condition.F90:874-884).  With IMOVE = 1 the particles move: each
+! stage runs nonlinear to the drag, des_time_march over dt (the GPU DEM),
+! grid_nodes_insphere/update_nrpr_array for the new centres, the forcing and
+! velstep (boundary_condition.F90:539-554).
 program pureibm_gpu
   use iso_c_binding
   use pib_types
@@ -20,8 +23,11 @@ program pureibm_gpu
   integer(c_int) :: st, nb, np, nx, my, mz
   type(pib_config) :: cfg
   type(pib_scalars) :: sc
-  type(c_ptr) :: ctx
+  type(c_ptr) :: ctx, dem
+  type(pib_dem_config) :: dc
   real(c_double), allocatable :: xc(:,:), rad(:), bnd(:,:), cd(:,:)
+  real(c_double), allocatable, target :: vel(:,:), angv(:,:), frc(:,:), pres(:,:), visc(:,:), torq(:,:)
+  integer :: rks, itrmax
   integer(c_int), allocatable :: nrpr(:)
   integer(c_long_long) :: nsolid
   real(c_double) :: tc, tv, td, ms4(4)
@@ -44,6 +50,8 @@ program pureibm_gpu
   call check(pib_host_configure(run_name, len_trim(run_name)), 'configuration')
   call check(pib_host_counts(nb, np, nx, my, mz), 'counts')
   allocate(xc(nb,3), rad(nb), bnd(3,np), cd(3,np), nrpr(max(nb,1)))
+  allocate(vel(nb,3), angv(nb,3), frc(nb,3), pres(nb,3), visc(nb,3), torq(nb,3))
+  vel = 0; angv = 0
   call check(pib_host_get_config(cfg), 'config')
   cfg%precision = prec
   call check(pib_host_get_particles(xc, rad), 'particles')
@@ -58,10 +66,42 @@ program pureibm_gpu
   call check(pib_host_time_scales(tc, tv, td), 'time scales')
   write(*,'(A,3I6,A,I6,A,I8)') 'GRID ', nx, my, mz, '  NBODY ', nb, '  SOLID NODES ', nsolid
 
+  itrmax = merge(3, 1, cfg%rk3 /= 0)
+  if (cfg%move_particles /= 0) then
+     ! des_time_march(.TRUE.) (initialize_flo.F90:630-637)
+     call check(pib_host_get_dem_config(dc), 'DEM constants')
+     st = pib_dem_create(dc, xc, rad, c_null_ptr, dem)
+     if (st /= PIB_OK) then
+        write(*,'(A,I4)') 'ERROR in pib_dem_create, status ', st
+        stop 1
+     end if
+     write(*,'(A)') 'MOVING PARTICLES: soft-sphere DEM on the GPU'
+  end if
+
   open(newunit=unorm, file=trim(run_name)//'_norm_drag.dat', form='formatted', status='replace')
   open(newunit=ucomp, file=trim(run_name)//'_drag_components.dat', form='formatted', status='replace')
   do n = 1, nsteps
-     st = pib_step(ctx, 1)
+     if (cfg%move_particles /= 0) then
+        st = PIB_OK
+        do rks = 1, itrmax
+           call gpu(pib_compute_new_timestep(ctx, rks), 'pib_compute_new_timestep')
+           st = pib_nonlinear_begin(ctx, rks)
+           if (st /= PIB_OK) exit
+           call gpu(pib_get_forces(ctx, frc, pres, visc, torq), 'pib_get_forces')
+           call gpu(pib_get_scalars(ctx, sc), 'pib_get_scalars')
+           st = pib_dem_march(dem, sc%dt, pres, visc, sc%mpg, sc%frame_accln, xc, vel, angv)
+           if (st /= PIB_OK) then
+              write(*,'(A,I4)') 'ERROR in pib_dem_march (MOVEMENT UNDESIRED if -7), status ', st
+              stop 1
+           end if
+           call gpu(pib_set_particles(ctx, nb, xc, rad, c_loc(vel), c_loc(angv), nsolid, nrpr), &
+                'pib_set_particles')
+           call gpu(pib_nonlinear_finish(ctx, rks), 'pib_nonlinear_finish')
+           call gpu(pib_velstep(ctx, rks), 'pib_velstep')
+        end do
+     else
+        st = pib_step(ctx, 1)
+     end if
      if (st == PIB_ERR_BLOWUP) then
         call gpu(pib_get_scalars(ctx, sc), 'pib_get_scalars')
         write(*,'(A,2x,g17.8,2x,A)') 'NORM DRAG', sc%norm_drag, ' is greater than 1E+06'
@@ -70,7 +110,7 @@ program pureibm_gpu
         close(2000, status='keep')
         stop
      end if
-     call gpu(st, 'pib_step')
+     call gpu(st, 'the step')
      call gpu(pib_get_scalars(ctx, sc), 'pib_get_scalars')
      write(*,'(A25,4(2x,g17.8))') 'NORM DRAGS, FERROR', sc%norm_drag, sc%ferror
      write(*,'(A25,4(2x,g17.8))') 'T, DT, MPG, UFMEAN', sc%t, sc%dt, sc%mpg(1), sc%ufmean(1)
@@ -83,6 +123,7 @@ program pureibm_gpu
   close(ucomp)
   call gpu(pib_phase_times(ctx, ms4), 'pib_phase_times')
   write(*,'(A,4F10.4)') 'LAST STEP MS (HYDRO NL BC VP) ', ms4
+  if (cfg%move_particles /= 0) st = pib_dem_destroy(dem)
   st = pib_destroy(ctx)
 
 contains
