@@ -84,6 +84,8 @@ typedef struct pib_dem_info {
  * 309-349, assumes it).  Up to 2048 particles. */
 int pib_dem_create(const pib_dem_config* cfg, const double* xc, const double* radbdy,
                    const double* velbdy, void** dem);
+/* A failed create still returns a context in *dem (NULL only for NULL
+ * arguments) so that pib_dem_last_error can explain; destroy it. */
 int pib_dem_destroy(void* dem);
 const char* pib_dem_last_error(void* dem);
 

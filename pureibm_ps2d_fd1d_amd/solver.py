@@ -325,7 +325,9 @@ class Dem:
         st = self._fn("create")(ctypes.byref(c), dptr(xc), dptr(rad), dptr(vel) if vel is not None else None,
                                 ctypes.byref(self.ctx))
         if st != PIB_OK:
-            self._raise(st)
+            msg = self._fn("last_error")(self.ctx) if self.ctx else b"create failed"
+            self.close()  # a failed create still hands back a context to destroy
+            raise PibError(st, (msg or b"").decode(errors="replace"))
 
     def _fn(self, name):
         return getattr(self.lib, self.p + "dem_" + name)
