@@ -117,6 +117,11 @@ typedef struct pib_scalars {
 #define PIB_F_FR 11    /* real (nx+1,my,mz,3): IBM forcing before frmean removal */
 #define PIB_F_OMEGA 12 /* real (nx+1,my,mz,3): vorticity from compute_omega */
 #define PIB_F_MASK 13  /* real (nx,my,mz): 1.0 fluid, 0.0 solid (fluid_atijk) */
+/* scalar transport, one species (scalar_init.F90:215-228, serial layout) */
+#define PIB_F_PHI 14    /* complex (nx+1,my2,mz): phif */
+#define PIB_F_NLPHI 15  /* complex (nx+1,my2,mz): nlphif */
+#define PIB_F_ONLPHI 16 /* complex (nx+1,my2,mz): onlphif */
+#define PIB_F_FFPHI 17  /* complex (nx+1,my2,mz): ffphi, the scalar IBM forcing */
 
 /* ---- lifecycle -------------------------------------------------------- */
 /* Replaces alloc_mem + the grid/wavenumber/coef part of initflo
@@ -176,6 +181,23 @@ int pib_step(void* ctx, int nsteps);
 /* Same work enqueued on the context stream without the final drain. */
 int pib_step_async(void* ctx, int nsteps);
 int pib_sync(void* ctx);
+
+/* ---- scalar transport, one species (SCALAR/stepthescalar.F90) -------- */
+/* GAMMA (diffusivity) and the x-wrap ratios of the species: the solve
+ * imposes phif(0) = heat_ratio_new*phif(nx), phif(nx+1) = phif(1)/heat_ratio_new
+ * on the implicit side and uses heat_ratio on the explicit side
+ * (stepthescalar.F90:115-140).  The first call allocates phif, nlphif,
+ * onlphif, ffphi (PIB_F_PHI..PIB_F_FFPHI, zero) and sets heat_ratio_old.
+ * All three must be > 0. */
+int pib_scalar_set(void* ctx, double gamma, double heat_ratio, double heat_ratio_new);
+/* out[4] = heat_ratio, heat_ratio_new, heat_ratio_old, gamma */
+int pib_scalar_get(void* ctx, double* out);
+/* scalstep's mode loop (stepthescalar.F90:67-91): phistep (:97-241) for
+ * every (ky,kz) mode from phif, nlphif, onlphif and ffphi as they stand
+ * (nlphi_graduphi, which forms them, is not part of this interface), then
+ * heat_ratio_old = heat_ratio, heat_ratio = heat_ratio_new.  No source term;
+ * serial x-periodic branch.  FP64 in both precisions. */
+int pib_scalstep_modes(void* ctx, int rks);
 
 /* ---- transform layer drop-in (fftw_interface.F90:256-338) ------------ */
 /* ff2rc over nplanes planes: a(my,mz) real -> b(my2,mz) complex, forward

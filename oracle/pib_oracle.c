@@ -55,6 +55,9 @@ typedef struct orc {
   int itrmax;
   double *wy, *wz, *w2; /* wy, wz hold the imaginary parts; w2(my2,mz) */
   cpx *u, *nl, *onl, *ff; /* (nx+1, my2, mz, 3) */
+  /* scalar transport, one species: phif, nlphif, onlphif, ffphi (nx+1, my2, mz) */
+  cpx *phif, *nlphif, *onlphif, *ffphi;
+  double sc_gamma, heat_ratio, heat_ratio_new, heat_ratio_old;
   cpx *p;                 /* (nx, my2, mz) */
   double *ubcp, *nlbc, *onlbc, *diffn, *fr; /* (nx+1, my, mz, 3); fr doubles as omega */
   double *omega_copy;     /* last omega, kept for parity dumps */
@@ -467,6 +470,7 @@ int orc_destroy(void* ctx) {
   if (!o) return PIB_OK;
   free(o->wy); free(o->wz); free(o->w2);
   free(o->u); free(o->nl); free(o->onl); free(o->ff); free(o->p);
+  free(o->phif); free(o->nlphif); free(o->onlphif); free(o->ffphi);
   free(o->ubcp); free(o->nlbc); free(o->onlbc); free(o->diffn); free(o->fr); free(o->omega_copy);
   free(o->pbcp); free(o->ppr); free(o->fluid);
   free(o->ra); free(o->rb); free(o->ca); free(o->cb);
@@ -681,6 +685,10 @@ static int field_info(orc* o, int field, void** ptr, size_t* ndoubles) {
     case PIB_F_PPR: *ptr = o->ppr; *ndoubles = (size_t)o->nx * o->my * o->mz * 3; return 0;
     case PIB_F_FR: *ptr = o->fr; *ndoubles = nr; return 0;
     case PIB_F_OMEGA: *ptr = o->omega_copy; *ndoubles = nr; return 0;
+    case PIB_F_PHI: *ptr = o->phif; *ndoubles = 2 * nc / 3; return o->phif ? 0 : -1;
+    case PIB_F_NLPHI: *ptr = o->nlphif; *ndoubles = 2 * nc / 3; return o->phif ? 0 : -1;
+    case PIB_F_ONLPHI: *ptr = o->onlphif; *ndoubles = 2 * nc / 3; return o->phif ? 0 : -1;
+    case PIB_F_FFPHI: *ptr = o->ffphi; *ndoubles = 2 * nc / 3; return o->phif ? 0 : -1;
     default: return -1;
   }
 }
@@ -1732,5 +1740,112 @@ int orc_step(void* ctx, int nsteps) {
       if ((st = orc_nonlinear(o, rks))) return st;
       if ((st = orc_velstep(o, rks))) return st;
     }
+  return PIB_OK;
+}
+
+/* ======================================================================
+ * scalar transport, one species: the per-mode solve of scalstep
+ * (stepthescalar.F90:51-241) without nlphi_graduphi, which supplies
+ * nlphif, onlphif and ffphi
+ * ====================================================================== */
+int orc_scalar_set(void* ctx, double gamma, double heat_ratio, double heat_ratio_new) {
+  orc* o = (orc*)ctx;
+  if (!o || !(gamma > 0) || !(heat_ratio > 0) || !(heat_ratio_new > 0)) return PIB_ERR_ARG;
+  if (!o->phif) {
+    size_t n = (size_t)(o->nx + 1) * o->my2 * o->mz;
+    o->phif = (cpx*)calloc(n, sizeof(cpx));
+    o->nlphif = (cpx*)calloc(n, sizeof(cpx));
+    o->onlphif = (cpx*)calloc(n, sizeof(cpx));
+    o->ffphi = (cpx*)calloc(n, sizeof(cpx));
+    o->heat_ratio_old = heat_ratio;
+  }
+  o->sc_gamma = gamma;
+  o->heat_ratio = heat_ratio;
+  o->heat_ratio_new = heat_ratio_new;
+  return PIB_OK;
+}
+
+int orc_scalar_get(void* ctx, double* out) {
+  orc* o = (orc*)ctx;
+  if (!o || !out || !o->phif) return PIB_ERR_ARG;
+  out[0] = o->heat_ratio;
+  out[1] = o->heat_ratio_new;
+  out[2] = o->heat_ratio_old;
+  out[3] = o->sc_gamma;
+  return PIB_OK;
+}
+
+#define IS(o, i, j, k) ((size_t)((i) - 1) + (size_t)((o)->nx + 1) * ((size_t)((j) - 1) + (size_t)(o)->my2 * ((k) - 1)))
+
+/* phistep (stepthescalar.F90:97-241), serial, x periodic, no source */
+static int phistep(orc* o, int rks, int j, int k) {
+  int nx = o->nx;
+  double dt = o->s.dt, g = o->sc_gamma, dx2 = o->dx2;
+  double w2 = o->w2[(size_t)(k - 1) * o->my2 + (j - 1)];
+  double c1 = o->coef[rks - 1][0], c2 = o->coef[rks - 1][1], c3 = o->coef[rks - 1][2], c4 = o->coef[rks - 1][3];
+  double hr = o->heat_ratio, hrn = o->heat_ratio_new;
+  double alpha = hrn, beta = 1.0 / hrn;
+  double *a = (double*)calloc(nx, sizeof(double)), *b = (double*)calloc(nx, sizeof(double)),
+         *c = (double*)calloc(nx, sizeof(double)), *auxu = (double*)calloc(nx, sizeof(double)),
+         *auxv = (double*)calloc(nx, sizeof(double)), *auxz = (double*)calloc(nx, sizeof(double)),
+         *tmp = (double*)calloc(nx, sizeof(double)), *tmpi = (double*)calloc(nx, sizeof(double)),
+         *ut = (double*)calloc(nx, sizeof(double)), *uti = (double*)calloc(nx, sizeof(double));
+  cpx* r1 = (cpx*)calloc(nx, sizeof(cpx));
+  cpx* phif = o->phif;
+  for (int i = 1; i <= nx; ++i) {
+    a[i - 1] = -(dt * g * c2 / dx2);
+    b[i - 1] = 1.0 + dt * g * c2 * (2.0 / dx2 + w2);
+    c[i - 1] = -(dt * g * c2 / dx2);
+    r1[i - 1] = cscal(1.0 - dt * c1 * g * w2, phif[IS(o, i, j, k)]);
+    r1[i - 1] = cadd(cadd(r1[i - 1], cscal(dt * c3, o->nlphif[IS(o, i, j, k)])), cscal(dt, o->ffphi[IS(o, i, j, k)]));
+    r1[i - 1] = cadd(r1[i - 1], cscal(dt * c4, o->onlphif[IS(o, i, j, k)]));
+  }
+  a[0] = a[0] * hrn;
+  c[nx - 1] = c[nx - 1] / hrn;
+  for (int i = 2; i <= nx; ++i) {
+    cpx lap = csub(cadd(phif[IS(o, i - 1, j, k)], phif[IS(o, i + 1, j, k)]), cscal(2.0, phif[IS(o, i, j, k)]));
+    r1[i - 1] = cadd(r1[i - 1], cdivr(cscal(dt * g * c1, lap), dx2));
+  }
+  {
+    cpx lap = cadd(csub(phif[IS(o, 2, j, k)], cscal(2.0, phif[IS(o, 1, j, k)])), cscal(hr, phif[IS(o, nx, j, k)]));
+    r1[0] = cadd(r1[0], cdivr(cscal(dt * g * c1, lap), dx2));
+  }
+  for (int i = 0; i < nx; ++i) {
+    tmp[i] = r1[i].re;
+    tmpi[i] = r1[i].im;
+  }
+  auxu[0] = alpha;
+  auxu[nx - 1] = beta;
+  auxv[0] = c[nx - 1] / auxu[nx - 1];
+  auxv[nx - 1] = a[0] / auxu[0];
+  b[0] = b[0] - auxu[0] * auxv[0];
+  b[nx - 1] = b[nx - 1] - auxu[nx - 1] * auxv[nx - 1];
+  int st = tridag3(a, b, c, tmp, auxu, tmpi, ut, auxz, uti, nx);
+  double vdotz = auxv[0] * auxz[0] + auxv[nx - 1] * auxz[nx - 1];
+  double vdotyr = auxv[0] * ut[0] + auxv[nx - 1] * ut[nx - 1];
+  double vdotyi = auxv[0] * uti[0] + auxv[nx - 1] * uti[nx - 1];
+  for (int i = 1; i <= nx; ++i)
+    phif[IS(o, i, j, k)] = C(ut[i - 1] - vdotyr * auxz[i - 1] / (1.0 + vdotz),
+                             uti[i - 1] - vdotyi * auxz[i - 1] / (1.0 + vdotz));
+  phif[IS(o, nx + 1, j, k)] = cdivr(phif[IS(o, 1, j, k)], hrn);
+  free(a); free(b); free(c); free(auxu); free(auxv); free(auxz); free(tmp); free(tmpi); free(ut); free(uti);
+  free(r1);
+  return st;
+}
+
+/* scalstep's mode loop and heat_ratio update (stepthescalar.F90:67-91) */
+int orc_scalstep_modes(void* ctx, int rks) {
+  orc* o = (orc*)ctx;
+  if (!o || rks < 1 || rks > o->itrmax) return PIB_ERR_ARG;
+  if (!o->phif) return fail(o, PIB_ERR_STATE, "pib_scalar_set first");
+  int st = 0;
+  for (int j = 1; j <= o->my2; ++j)
+    for (int k = 1; k <= o->mz; ++k) {
+      int r = phistep(o, rks, j, k);
+      if (r) st = r;
+    }
+  o->heat_ratio_old = o->heat_ratio;
+  o->heat_ratio = o->heat_ratio_new;
+  if (st) return fail(o, st, "zero pivot in the scalar solve");
   return PIB_OK;
 }

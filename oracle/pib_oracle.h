@@ -29,6 +29,9 @@ int orc_get_forces(void* ctx, double* force, double* pres, double* visc, double*
 int orc_get_if_rev(void* ctx, unsigned char* if_rev /* (nbnd, nbody) */);
 int orc_compute_new_timestep(void* ctx, int rks);
 int orc_nonlinear(void* ctx, int rks);
+int orc_scalar_set(void* ctx, double gamma, double heat_ratio, double heat_ratio_new);
+int orc_scalar_get(void* ctx, double* out);
+int orc_scalstep_modes(void* ctx, int rks);
 int orc_nonlinear_begin(void* ctx, int rks);
 int orc_nonlinear_finish(void* ctx, int rks);
 int orc_velstep(void* ctx, int rks);

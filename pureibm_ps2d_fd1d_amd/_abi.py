@@ -40,9 +40,9 @@ PIB_LIMITER_MINMOD = 1
 FIELDS = {
     "u": 0, "p": 1, "nl": 2, "onl": 3, "ff": 4, "ubcp": 5, "pbcp": 6,
     "nlbcp": 7, "onlbcp": 8, "diffn": 9, "ppr": 10, "fr": 11, "omega": 12,
-    "mask": 13,
+    "mask": 13, "phi": 14, "nlphi": 15, "onlphi": 16, "ffphi": 17,
 }
-COMPLEX_FIELDS = {"u", "p", "nl", "onl", "ff"}
+COMPLEX_FIELDS = {"u", "p", "nl", "onl", "ff", "phi", "nlphi", "onlphi", "ffphi"}
 
 
 class PibConfig(ctypes.Structure):
@@ -128,6 +128,9 @@ _STEP_SIGS = {
     "nonlinear_finish": ([_VP, _I], _I),
     "velstep": ([_VP, _I], _I),
     "step": ([_VP, _I], _I),
+    "scalar_set": ([_VP, ctypes.c_double, ctypes.c_double, ctypes.c_double], _I),
+    "scalar_get": ([_VP, _PD], _I),
+    "scalstep_modes": ([_VP, _I], _I),
 }
 
 _GPU_ONLY_SIGS = {

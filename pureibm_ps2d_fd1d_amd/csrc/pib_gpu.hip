@@ -14,6 +14,7 @@
 
 #include "pib_kernels.h"
 #include "pib_velstep.h"
+#include "pib_scalar.h"
 
 namespace {
 
@@ -672,6 +673,11 @@ struct Impl : Base {
   T* vscr;
   double* vscr64;
   int* dstatus;
+  // scalar transport, one species (allocated by the first scalar_set)
+  C2<double>* phi[4] = {nullptr, nullptr, nullptr, nullptr};  // phif, nlphif, onlphif, ffphi
+  double* sscr = nullptr;
+  int* sstatus = nullptr;
+  double sc_gamma = 0, heat_ratio = 1, heat_ratio_new = 1, heat_ratio_old = 1;
   // particles
   int nbody = 0, nbnd = 0;
   double *xc = nullptr, *rad = nullptr, *radi = nullptr, *rado = nullptr, *vel = nullptr, *angv = nullptr;
@@ -1619,7 +1625,74 @@ struct Impl : Base {
     HIPCHK(hipMemcpyAsync(d, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice, stream));
     HIPCHK(hipStreamSynchronize(stream));
   }
+  // scalar fields: (nx+1, my2, mz) reference layout <-> device [i][kz][ky], all nx+1 planes
+  C2<double>* scalar_field(int f) {
+    if (!phi[0]) throw PibFail(PIB_ERR_STATE, "pib_scalar_set first");
+    return phi[f - PIB_F_PHI];
+  }
+  void put_phi(C2<double>* d, const double* h) {
+    size_t E = (size_t)my2 * mz;
+    std::vector<C2<double>> t((size_t)(nx + 1) * E);
+    for (int k = 0; k < mz; ++k)
+      for (int j = 0; j < my2; ++j)
+        for (int i = 0; i <= nx; ++i) {
+          size_t hi = (((size_t)k * my2 + j) * (nx + 1) + i) * 2;
+          t[(size_t)i * E + (size_t)k * my2 + j] = {h[hi], h[hi + 1]};
+        }
+    HIPCHK(hipMemcpyAsync(d, t.data(), t.size() * sizeof(C2<double>), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  void get_phi(const C2<double>* d, double* h) {
+    size_t E = (size_t)my2 * mz;
+    std::vector<C2<double>> t((size_t)(nx + 1) * E);
+    HIPCHK(hipMemcpyAsync(t.data(), d, t.size() * sizeof(C2<double>), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int k = 0; k < mz; ++k)
+      for (int j = 0; j < my2; ++j)
+        for (int i = 0; i <= nx; ++i) {
+          size_t hi = (((size_t)k * my2 + j) * (nx + 1) + i) * 2;
+          const C2<double>& v = t[(size_t)i * E + (size_t)k * my2 + j];
+          h[hi] = v.x;
+          h[hi + 1] = v.y;
+        }
+  }
+  void scalar_set(double gamma, double hr, double hrn) {
+    if (!(gamma > 0) || !(hr > 0) || !(hrn > 0)) throw PibFail(PIB_ERR_ARG, "gamma and heat ratios must be > 0");
+    if (!phi[0]) {
+      size_t n = (size_t)(nx + 1) * my2 * mz;
+      for (auto& f : phi) f = alloc<C2<double>>(n);
+      sscr = alloc<double>((size_t)4 * nx * my2 * mz);
+      sstatus = alloc<int>(1);
+      heat_ratio_old = hr;
+    }
+    sc_gamma = gamma;
+    heat_ratio = hr;
+    heat_ratio_new = hrn;
+  }
+  void scalar_get(double* out) {
+    if (!phi[0]) throw PibFail(PIB_ERR_STATE, "pib_scalar_set first");
+    out[0] = heat_ratio;
+    out[1] = heat_ratio_new;
+    out[2] = heat_ratio_old;
+    out[3] = sc_gamma;
+  }
+  void scalstep_modes(int rks) {
+    if (rks < 1 || rks > itrmax) throw PibFail(PIB_ERR_ARG, "rks out of range");
+    if (!phi[0]) throw PibFail(PIB_ERR_STATE, "pib_scalar_set first");
+    const int E = my2 * mz;
+    ScalArgs a{nx, my, mz, my2, rks, sc_gamma, heat_ratio, heat_ratio_new, dst, phi[0], phi[1], phi[2], phi[3], sscr};
+    HIPCHK(hipMemsetAsync(sstatus, 0, sizeof(int), stream));
+    k_phistep<<<(E + 255) / 256, 256, 0, stream>>>(a, sstatus);
+    HIPCHK(hipGetLastError());
+    int st = 0;
+    HIPCHK(hipMemcpyAsync(&st, sstatus, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    heat_ratio_old = heat_ratio;
+    heat_ratio = heat_ratio_new;
+    if (st) throw PibFail(st, "zero pivot in the scalar solve");
+  }
   void set_field(int f, const double* h) {
+    if (f >= PIB_F_PHI && f <= PIB_F_FFPHI) return put_phi(scalar_field(f), h);
     switch (f) {
       case PIB_F_U: put_spec(u, 3, h, true); break;
       case PIB_F_P: put_spec(p, 1, h, false); break;
@@ -1632,6 +1705,7 @@ struct Impl : Base {
     }
   }
   void get_field(int f, double* h) {
+    if (f >= PIB_F_PHI && f <= PIB_F_FFPHI) return get_phi(scalar_field(f), h);
     switch (f) {
       case PIB_F_U: get_spec(u, 3, h, true); break;
       case PIB_F_P: get_spec(p, 1, h, false); break;
@@ -1856,6 +1930,16 @@ int pib_set_field(void* ctx, int field, const double* host) {
 int pib_get_field(void* ctx, int field, double* host) {
   if (!host) return PIB_ERR_ARG;
   return dispatch(ctx, [&](auto& I) { I.get_field(field, host); });
+}
+int pib_scalar_set(void* ctx, double gamma, double heat_ratio, double heat_ratio_new) {
+  return dispatch(ctx, [&](auto& I) { I.scalar_set(gamma, heat_ratio, heat_ratio_new); });
+}
+int pib_scalar_get(void* ctx, double* out) {
+  if (!out) return PIB_ERR_ARG;
+  return dispatch(ctx, [&](auto& I) { I.scalar_get(out); });
+}
+int pib_scalstep_modes(void* ctx, int rks) {
+  return dispatch(ctx, [&](auto& I) { I.scalstep_modes(rks); });
 }
 int pib_set_scalars(void* ctx, const pib_scalars* s) {
   if (!s) return PIB_ERR_ARG;

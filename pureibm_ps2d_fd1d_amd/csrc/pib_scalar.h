@@ -1,0 +1,130 @@
+// pib_scalar.h -- scalar transport, one species: the per-mode implicit x
+// solve of scalstep (stepthescalar.F90:67-241) on the device, FP64.
+//
+// Each (ky, kz) Fourier mode owns one lane.  The lane walks x three times:
+// a forward Thomas sweep that forms the right-hand side of phistep on the
+// fly from three rolling phif planes (:115-140), the back substitution, and
+// the Sherman-Morrison correction of the x wrap with the heat-ratio twist
+// phif(0) = heat_ratio * phif(nx), phif(nx+1) = phif(1) / heat_ratio_new
+// (:142-196).  Fields are plane-major [i][kz][ky] with nx+1 planes, so the
+// lanes of a wavefront touch consecutive modes of one plane (coalesced).
+#pragma once
+
+struct ScalArgs {
+  int nx, my, mz, my2, rks;
+  double gamma, hr, hrn;
+  const DevState* st;
+  C2<double>* phi;            // [i][kz][ky], i = 0..nx
+  const C2<double>* nlphi;    // nlphif, onlphif, ffphi: same layout, i < nx read
+  const C2<double>* onlphi;
+  const C2<double>* ffphi;
+  double* scr;                // four vectors [s][i][mode]: gam, y_re, y_aux, y_im
+};
+
+__global__ void __launch_bounds__(256) k_phistep(ScalArgs A, int* status) {
+  const int E = A.my2 * A.mz;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= E) return;
+  const int nx = A.nx, ky = q % A.my2, kz = q / A.my2;
+  const DevState* st = A.st;
+  const int r = A.rks - 1;
+  const double c1 = st->coef[r][0], c2 = st->coef[r][1], c3 = st->coef[r][2], c4 = st->coef[r][3];
+  const double dx = st->dx, dx2 = dx * dx, dt = st->s.dt, g = A.gamma;
+  // w2 = wy^2 + wz^2 of this mode (initialize_flo.F90 wavenumbers)
+  const double twopi = 2.0 * 3.14159265358979323846264338327950288;
+  const double wy = twopi * (double)ky / ((double)A.my * dx);
+  const double wz = kz < A.mz / 2 ? twopi * (double)kz / ((double)A.mz * dx)
+                                  : -(twopi * (double)(A.mz - kz) / ((double)A.mz * dx));
+  const double w2 = wy * wy + wz * wz;
+  // tridiagonal rows (:118-124, 133-134) and Sherman-Morrison vectors (:178-183)
+  const double a = -(dt * g * c2 / dx2), c = a;
+  const double b = 1.0 + dt * g * c2 * (2.0 / dx2 + w2);
+  const double alpha = A.hrn, beta = 1.0 / A.hrn;
+  const double a1 = a * A.hrn, cn = c / A.hrn;
+  const double v1 = cn / beta, vn = a1 / alpha;
+  const double b0 = b - alpha * v1, bn = b - beta * vn;
+  const double f0 = 1.0 - dt * c1 * g * w2, dc3 = dt * c3, dc4 = dt * c4, dg1 = dt * g * c1;
+  const size_t ms = (size_t)E;
+  const C2<double>* ph = A.phi + q;
+  const C2<double>* nl = A.nlphi + q;
+  const C2<double>* on = A.onlphi + q;
+  const C2<double>* fp = A.ffphi + q;
+  double* gs = A.scr + q;
+  double* y1s = gs + (size_t)nx * ms;
+  double* y2s = y1s + (size_t)nx * ms;
+  double* y3s = y2s + (size_t)nx * ms;
+  auto rhs = [&](int i, C2<double> pm, C2<double> p0, C2<double> pp) {
+    const size_t o = (size_t)i * ms;
+    C2<double> n = nl[o], w = on[o], f = fp[o];
+    double rx = f0 * p0.x, ry = f0 * p0.y;
+    rx = (rx + dc3 * n.x) + dt * f.x;
+    ry = (ry + dc3 * n.y) + dt * f.y;
+    rx = rx + dc4 * w.x;
+    ry = ry + dc4 * w.y;
+    double lx, ly;
+    if (i == 0) {                       // x-periodic first row (:136-140)
+      lx = (pp.x - 2.0 * p0.x) + A.hr * pm.x;
+      ly = (pp.y - 2.0 * p0.y) + A.hr * pm.y;
+    } else {
+      lx = (pm.x + pp.x) - 2.0 * p0.x;
+      ly = (pm.y + pp.y) - 2.0 * p0.y;
+    }
+    return mk(rx + dg1 * lx / dx2, ry + dg1 * ly / dx2);
+  };
+  int status_ = 0;
+  // forward sweep (tridag3 of the rhs pair and auxu)
+  C2<double> pm = ph[(size_t)(nx - 1) * ms], p0 = ph[0], pp = ph[ms];
+  double bet = b0;
+  if (bet == 0.0) status_ = PIB_ERR_PIVOT;
+  C2<double> rv = rhs(0, pm, p0, pp);
+  double y1 = rv.x / bet, y2 = alpha / bet, y3 = rv.y / bet;
+  y1s[0] = y1;
+  y2s[0] = y2;
+  y3s[0] = y3;
+  gs[0] = 0.0;
+  for (int i = 1; i < nx; ++i) {
+    pm = p0;
+    p0 = pp;
+    pp = ph[(size_t)(i + 1) * ms];
+    const double gam = c / bet;
+    bet = (i == nx - 1 ? bn : b) - a * gam;
+    if (bet == 0.0) { bet = 1.0e-30; status_ = PIB_ERR_PIVOT; }
+    rv = rhs(i, pm, p0, pp);
+    const double u2 = i == nx - 1 ? beta : 0.0;
+    y1 = (rv.x - a * y1) / bet;
+    y2 = (u2 - a * y2) / bet;
+    y3 = (rv.y - a * y3) / bet;
+    const size_t o = (size_t)i * ms;
+    gs[o] = gam;
+    y1s[o] = y1;
+    y2s[o] = y2;
+    y3s[o] = y3;
+  }
+  // back substitution; y1..y3 hold row nx-1 already
+  const double z_n = y2, yr_n = y1, yi_n = y3;
+  for (int i = nx - 2; i >= 0; --i) {
+    const size_t o = (size_t)i * ms;
+    const double gm = gs[o + ms];
+    y1 = y1s[o] - gm * y1;
+    y2 = y2s[o] - gm * y2;
+    y3 = y3s[o] - gm * y3;
+    y1s[o] = y1;
+    y2s[o] = y2;
+    y3s[o] = y3;
+  }
+  // Sherman-Morrison correction and the twisted wrap plane (:185-196)
+  const double vdotz = v1 * y2 + vn * z_n;
+  const double vdotyr = v1 * y1 + vn * yr_n;
+  const double vdotyi = v1 * y3 + vn * yi_n;
+  const double den = 1.0 + vdotz;
+  C2<double>* pw = A.phi + q;
+  C2<double> first = mk(y1 - vdotyr * y2 / den, y3 - vdotyi * y2 / den);
+  pw[0] = first;
+  for (int i = 1; i < nx; ++i) {
+    const size_t o = (size_t)i * ms;
+    const double z = y2s[o];
+    pw[o] = mk(y1s[o] - vdotyr * z / den, y3s[o] - vdotyi * z / den);
+  }
+  pw[(size_t)nx * ms] = mk(first.x / A.hrn, first.y / A.hrn);
+  if (status_) *status = status_;
+}

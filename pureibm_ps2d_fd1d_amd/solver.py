@@ -122,6 +122,24 @@ class Solver:
     def step(self, nsteps=1):
         self._call("step", self.ctx, nsteps)
 
+    def scalar_set(self, gamma, heat_ratio=1.0, heat_ratio_new=None):
+        """Diffusivity GAMMA and the x-wrap ratios of the one scalar species
+        (scalar_data.F90; heat_ratio_new is the value the solve uses at the
+        x wrap, heat_ratio the one of the previous solve).  Allocates the
+        scalar fields, zero, on first use."""
+        hrn = heat_ratio if heat_ratio_new is None else heat_ratio_new
+        self._call("scalar_set", self.ctx, gamma, heat_ratio, hrn)
+
+    def scalar_get(self):
+        """-> (heat_ratio, heat_ratio_new, heat_ratio_old, gamma)."""
+        out = (ctypes.c_double * 4)()
+        self._call("scalar_get", self.ctx, out)
+        return tuple(out)
+
+    def scalstep_modes(self, rks=1):
+        """The per-mode implicit solve of scalstep (stepthescalar.F90:67-91)."""
+        self._call("scalstep_modes", self.ctx, rks)
+
     def nonlinear_begin(self, rks=1):
         """nonlinear(rks) up to where bcset moves particles (form_nl, the
         transforms, calc_visc, compute_mpg_at_n)."""
@@ -242,6 +260,8 @@ class Solver:
             "onlbcp": (nx + 1, my, mz, 3), "diffn": (nx + 1, my, mz, 3),
             "fr": (nx + 1, my, mz, 3), "omega": (nx + 1, my, mz, 3),
             "pbcp": (nx, my, mz), "ppr": (nx, my, mz, 3), "mask": (nx, my, mz),
+            "phi": (nx + 1, my2, mz), "nlphi": (nx + 1, my2, mz), "onlphi": (nx + 1, my2, mz),
+            "ffphi": (nx + 1, my2, mz),
         }[name]
 
     def field(self, name):
