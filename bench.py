@@ -277,6 +277,28 @@ def time_ibm_update(setup, reps=5):
             "ms": 1e3 * float(np.median(ts))}
 
 
+def time_scalar_solve(setup, reps=20):
+    """One species' per-mode implicit solve (pib_scalstep_modes, FP64) on the
+    workload's grid from random phif, nlphif, onlphif, ffphi.  Wall time a
+    call, its status read-back included; the algorithmic traffic is 4 complex
+    reads and 1 complex write a grid point."""
+    g = solver.Solver(setup)
+    g.scalar_set(0.02, 1.05, 0.97)
+    rng = np.random.default_rng(5)
+    shp = (setup.nx + 1, setup.my // 2 + 1, setup.mz)
+    for f in ("phi", "nlphi", "onlphi", "ffphi"):
+        g.set_field(f, rng.standard_normal(shp) + 1j * rng.standard_normal(shp))
+    g.scalstep_modes(1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.scalstep_modes(1)
+    t = (time.perf_counter() - t0) / reps
+    g.close()
+    pts = shp[0] * shp[1] * shp[2]
+    return {"grid": f"{setup.nx} x {setup.my // 2 + 1} x {setup.mz} modes", "ms": 1e3 * t,
+            "algorithmic_GB_s": 80.0 * pts / t / 1e9}
+
+
 def time_moving_bed(workload, steps=5, warmup=2):
     """Freely moving bed on the default workload's 128^3/195-sphere bed,
     FP32-bulk fluid with the FP64 DEM: per step nonlinear to the drag, the DEM
@@ -422,6 +444,10 @@ def main():
             out["dem"]["moving_bed"] = time_moving_bed(a.workload)
         except Exception as e:  # reported, never fatal for the headline figure
             out["dem"] = {"error": str(e)}
+        try:
+            out["scalar_solve"] = time_scalar_solve(st32)
+        except Exception as e:  # reported, never fatal for the headline figure
+            out["scalar_solve"] = {"error": str(e)}
     if not a.no_cpu:
         ncpu = max(1, a.cpu_steps)
         tcpu = cpu_baseline(st64, ncpu)

@@ -1682,7 +1682,7 @@ struct Impl : Base {
     const int E = my2 * mz;
     ScalArgs a{nx, my, mz, my2, rks, sc_gamma, heat_ratio, heat_ratio_new, dst, phi[0], phi[1], phi[2], phi[3], sscr};
     HIPCHK(hipMemsetAsync(sstatus, 0, sizeof(int), stream));
-    k_phistep<<<(E + 255) / 256, 256, 0, stream>>>(a, sstatus);
+    k_phistep<<<(E + PHI_T - 1) / PHI_T, PHI_T, 0, stream>>>(a, sstatus);
     HIPCHK(hipGetLastError());
     int st = 0;
     HIPCHK(hipMemcpyAsync(&st, sstatus, sizeof(int), hipMemcpyDeviceToHost, stream));

@@ -8,7 +8,14 @@
 // phif(0) = heat_ratio * phif(nx), phif(nx+1) = phif(1) / heat_ratio_new
 // (:142-196).  Fields are plane-major [i][kz][ky] with nx+1 planes, so the
 // lanes of a wavefront touch consecutive modes of one plane (coalesced).
+// A workgroup is one wavefront: a 128^3 grid has only 8320 modes, and
+// one-wave workgroups spread them over 130 CUs instead of 33.  Each sweep
+// loads PHI_K planes into registers before it runs their PHI_K steps of the
+// recurrence, so a lane keeps PHI_K plane loads in flight.
 #pragma once
+
+#define PHI_T 64
+#define PHI_K 16
 
 struct ScalArgs {
   int nx, my, mz, my2, rks;
@@ -21,7 +28,7 @@ struct ScalArgs {
   double* scr;                // four vectors [s][i][mode]: gam, y_re, y_aux, y_im
 };
 
-__global__ void __launch_bounds__(256) k_phistep(ScalArgs A, int* status) {
+__global__ void __launch_bounds__(PHI_T) k_phistep(ScalArgs A, int* status) {
   const int E = A.my2 * A.mz;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= E) return;
@@ -46,16 +53,15 @@ __global__ void __launch_bounds__(256) k_phistep(ScalArgs A, int* status) {
   const double f0 = 1.0 - dt * c1 * g * w2, dc3 = dt * c3, dc4 = dt * c4, dg1 = dt * g * c1;
   const size_t ms = (size_t)E;
   const C2<double>* ph = A.phi + q;
-  const C2<double>* nl = A.nlphi + q;
-  const C2<double>* on = A.onlphi + q;
-  const C2<double>* fp = A.ffphi + q;
-  double* gs = A.scr + q;
-  double* y1s = gs + (size_t)nx * ms;
-  double* y2s = y1s + (size_t)nx * ms;
-  double* y3s = y2s + (size_t)nx * ms;
-  auto rhs = [&](int i, C2<double> pm, C2<double> p0, C2<double> pp) {
-    const size_t o = (size_t)i * ms;
-    C2<double> n = nl[o], w = on[o], f = fp[o];
+  const C2<double>* __restrict__ nl = A.nlphi + q;
+  const C2<double>* __restrict__ on = A.onlphi + q;
+  const C2<double>* __restrict__ fp = A.ffphi + q;
+  double* __restrict__ gs = A.scr + q;
+  double* __restrict__ y1s = A.scr + (size_t)nx * ms + q;
+  double* __restrict__ y2s = A.scr + (size_t)2 * nx * ms + q;
+  double* __restrict__ y3s = A.scr + (size_t)3 * nx * ms + q;
+  auto rhs = [&](int i, C2<double> pm, C2<double> p0, C2<double> pp, C2<double> n, C2<double> w,
+                 C2<double> f) {
     double rx = f0 * p0.x, ry = f0 * p0.y;
     rx = (rx + dc3 * n.x) + dt * f.x;
     ry = (ry + dc3 * n.y) + dt * f.y;
@@ -76,41 +82,76 @@ __global__ void __launch_bounds__(256) k_phistep(ScalArgs A, int* status) {
   C2<double> pm = ph[(size_t)(nx - 1) * ms], p0 = ph[0], pp = ph[ms];
   double bet = b0;
   if (bet == 0.0) status_ = PIB_ERR_PIVOT;
-  C2<double> rv = rhs(0, pm, p0, pp);
+  C2<double> rv = rhs(0, pm, p0, pp, nl[0], on[0], fp[0]);
   double y1 = rv.x / bet, y2 = alpha / bet, y3 = rv.y / bet;
   y1s[0] = y1;
   y2s[0] = y2;
   y3s[0] = y3;
   gs[0] = 0.0;
-  for (int i = 1; i < nx; ++i) {
-    pm = p0;
-    p0 = pp;
-    pp = ph[(size_t)(i + 1) * ms];
-    const double gam = c / bet;
-    bet = (i == nx - 1 ? bn : b) - a * gam;
-    if (bet == 0.0) { bet = 1.0e-30; status_ = PIB_ERR_PIVOT; }
-    rv = rhs(i, pm, p0, pp);
-    const double u2 = i == nx - 1 ? beta : 0.0;
-    y1 = (rv.x - a * y1) / bet;
-    y2 = (u2 - a * y2) / bet;
-    y3 = (rv.y - a * y3) / bet;
-    const size_t o = (size_t)i * ms;
-    gs[o] = gam;
-    y1s[o] = y1;
-    y2s[o] = y2;
-    y3s[o] = y3;
+  for (int i0 = 1; i0 < nx; i0 += PHI_K) {
+    C2<double> P[PHI_K], N[PHI_K], W[PHI_K], F[PHI_K];
+#pragma unroll
+    for (int u = 0; u < PHI_K; ++u) {
+      const int i = i0 + u;
+      if (i < nx) {
+        const size_t o = (size_t)i * ms;
+        P[u] = ph[o + ms];
+        N[u] = nl[o];
+        W[u] = on[o];
+        F[u] = fp[o];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PHI_K; ++u) {
+      const int i = i0 + u;
+      if (i < nx) {
+        pm = p0;
+        p0 = pp;
+        pp = P[u];
+        const double gam = c / bet;
+        bet = (i == nx - 1 ? bn : b) - a * gam;
+        if (bet == 0.0) { bet = 1.0e-30; status_ = PIB_ERR_PIVOT; }
+        rv = rhs(i, pm, p0, pp, N[u], W[u], F[u]);
+        const double u2 = i == nx - 1 ? beta : 0.0;
+        y1 = (rv.x - a * y1) / bet;
+        y2 = (u2 - a * y2) / bet;
+        y3 = (rv.y - a * y3) / bet;
+        const size_t o = (size_t)i * ms;
+        gs[o] = gam;
+        y1s[o] = y1;
+        y2s[o] = y2;
+        y3s[o] = y3;
+      }
+    }
   }
   // back substitution; y1..y3 hold row nx-1 already
   const double z_n = y2, yr_n = y1, yi_n = y3;
-  for (int i = nx - 2; i >= 0; --i) {
-    const size_t o = (size_t)i * ms;
-    const double gm = gs[o + ms];
-    y1 = y1s[o] - gm * y1;
-    y2 = y2s[o] - gm * y2;
-    y3 = y3s[o] - gm * y3;
-    y1s[o] = y1;
-    y2s[o] = y2;
-    y3s[o] = y3;
+  for (int i0 = nx - 2; i0 >= 0; i0 -= PHI_K) {
+    double G[PHI_K], Y1[PHI_K], Y2[PHI_K], Y3[PHI_K];
+#pragma unroll
+    for (int u = 0; u < PHI_K; ++u) {
+      const int i = i0 - u;
+      if (i >= 0) {
+        const size_t o = (size_t)i * ms;
+        G[u] = gs[o + ms];
+        Y1[u] = y1s[o];
+        Y2[u] = y2s[o];
+        Y3[u] = y3s[o];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PHI_K; ++u) {
+      const int i = i0 - u;
+      if (i >= 0) {
+        const size_t o = (size_t)i * ms;
+        y1 = Y1[u] - G[u] * y1;
+        y2 = Y2[u] - G[u] * y2;
+        y3 = Y3[u] - G[u] * y3;
+        y1s[o] = y1;
+        y2s[o] = y2;
+        y3s[o] = y3;
+      }
+    }
   }
   // Sherman-Morrison correction and the twisted wrap plane (:185-196)
   const double vdotz = v1 * y2 + vn * z_n;
@@ -120,10 +161,21 @@ __global__ void __launch_bounds__(256) k_phistep(ScalArgs A, int* status) {
   C2<double>* pw = A.phi + q;
   C2<double> first = mk(y1 - vdotyr * y2 / den, y3 - vdotyi * y2 / den);
   pw[0] = first;
-  for (int i = 1; i < nx; ++i) {
-    const size_t o = (size_t)i * ms;
-    const double z = y2s[o];
-    pw[o] = mk(y1s[o] - vdotyr * z / den, y3s[o] - vdotyi * z / den);
+  for (int i0 = 1; i0 < nx; i0 += PHI_K) {
+    double Z[PHI_K], Y1[PHI_K], Y3[PHI_K];
+#pragma unroll
+    for (int u = 0; u < PHI_K; ++u) {
+      const size_t o = (size_t)(i0 + u) * ms;
+      if (i0 + u < nx) {
+        Z[u] = y2s[o];
+        Y1[u] = y1s[o];
+        Y3[u] = y3s[o];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PHI_K; ++u)
+      if (i0 + u < nx)
+        pw[(size_t)(i0 + u) * ms] = mk(Y1[u] - vdotyr * Z[u] / den, Y3[u] - vdotyi * Z[u] / den);
   }
   pw[(size_t)nx * ms] = mk(first.x / A.hrn, first.y / A.hrn);
   if (status_) *status = status_;
