@@ -13,6 +13,7 @@ module pib_types
   integer(c_int), parameter :: PIB_SCHEME_CENTER = 0, PIB_SCHEME_UPWIND = 1
   integer(c_int), parameter :: PIB_LIMITER_NONE = 0, PIB_LIMITER_MINMOD = 1
   integer(c_int), parameter :: PIB_F_U = 0, PIB_F_P = 1
+  integer(c_int), parameter :: PIB_F_PHI = 14, PIB_F_NLPHI = 15, PIB_F_ONLPHI = 16, PIB_F_FFPHI = 17
 
   type, bind(C) :: pib_config
      integer(c_int) :: nx, my, mz, precision
@@ -191,6 +192,25 @@ module pib_types
        integer(c_int), value :: rks
        integer(c_int) :: st
      end function pib_velstep
+     ! scalar transport, one species (stepthescalar.F90:67-241)
+     function pib_scalar_set(ctx, gamma, heat_ratio, heat_ratio_new) bind(C, name="pib_scalar_set") result(st)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: gamma, heat_ratio, heat_ratio_new
+       integer(c_int) :: st
+     end function pib_scalar_set
+     function pib_scalar_get(ctx, out4) bind(C, name="pib_scalar_get") result(st)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: out4(4)
+       integer(c_int) :: st
+     end function pib_scalar_get
+     function pib_scalstep_modes(ctx, rks) bind(C, name="pib_scalstep_modes") result(st)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: rks
+       integer(c_int) :: st
+     end function pib_scalstep_modes
      function pib_phase_times(ctx, ms4) bind(C, name="pib_phase_times") result(st)
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx
