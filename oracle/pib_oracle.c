@@ -685,10 +685,10 @@ static int field_info(orc* o, int field, void** ptr, size_t* ndoubles) {
     case PIB_F_PPR: *ptr = o->ppr; *ndoubles = (size_t)o->nx * o->my * o->mz * 3; return 0;
     case PIB_F_FR: *ptr = o->fr; *ndoubles = nr; return 0;
     case PIB_F_OMEGA: *ptr = o->omega_copy; *ndoubles = nr; return 0;
-    case PIB_F_PHI: *ptr = o->phif; *ndoubles = 2 * nc / 3; return o->phif ? 0 : -1;
-    case PIB_F_NLPHI: *ptr = o->nlphif; *ndoubles = 2 * nc / 3; return o->phif ? 0 : -1;
-    case PIB_F_ONLPHI: *ptr = o->onlphif; *ndoubles = 2 * nc / 3; return o->phif ? 0 : -1;
-    case PIB_F_FFPHI: *ptr = o->ffphi; *ndoubles = 2 * nc / 3; return o->phif ? 0 : -1;
+    case PIB_F_PHI: *ptr = o->phif; *ndoubles = 2 * nc / 3; return o->phif ? 0 : 1;
+    case PIB_F_NLPHI: *ptr = o->nlphif; *ndoubles = 2 * nc / 3; return o->phif ? 0 : 1;
+    case PIB_F_ONLPHI: *ptr = o->onlphif; *ndoubles = 2 * nc / 3; return o->phif ? 0 : 1;
+    case PIB_F_FFPHI: *ptr = o->ffphi; *ndoubles = 2 * nc / 3; return o->phif ? 0 : 1;
     default: return -1;
   }
 }
@@ -696,7 +696,9 @@ int orc_set_field(void* ctx, int field, const double* host) {
   orc* o = (orc*)ctx;
   void* ptr;
   size_t n;
-  if (!o || !host || field_info(o, field, &ptr, &n)) return PIB_ERR_ARG;
+  if (!o || !host) return PIB_ERR_ARG;
+  int r = field_info(o, field, &ptr, &n); /* 1: a scalar field before orc_scalar_set */
+  if (r) return r > 0 ? fail(o, PIB_ERR_STATE, "pib_scalar_set first") : PIB_ERR_ARG;
   memcpy(ptr, host, n * sizeof(double));
   return PIB_OK;
 }
@@ -712,7 +714,8 @@ int orc_get_field(void* ctx, int field, double* host) {
   }
   void* ptr;
   size_t n;
-  if (field_info(o, field, &ptr, &n)) return PIB_ERR_ARG;
+  int r = field_info(o, field, &ptr, &n);
+  if (r) return r > 0 ? fail(o, PIB_ERR_STATE, "pib_scalar_set first") : PIB_ERR_ARG;
   memcpy(host, ptr, n * sizeof(double));
   return PIB_OK;
 }
@@ -1767,7 +1770,8 @@ int orc_scalar_set(void* ctx, double gamma, double heat_ratio, double heat_ratio
 
 int orc_scalar_get(void* ctx, double* out) {
   orc* o = (orc*)ctx;
-  if (!o || !out || !o->phif) return PIB_ERR_ARG;
+  if (!o || !out) return PIB_ERR_ARG;
+  if (!o->phif) return fail(o, PIB_ERR_STATE, "pib_scalar_set first");
   out[0] = o->heat_ratio;
   out[1] = o->heat_ratio_new;
   out[2] = o->heat_ratio_old;

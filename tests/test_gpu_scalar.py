@@ -103,6 +103,8 @@ def test_scalar_state_errors():
     a = np.zeros((9, 5, 8), dtype=np.complex128, order="F")
     ptr = a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     assert g.fn("get_field")(g.ctx, _abi.FIELDS["phi"], ptr) == _abi.PIB_ERR_STATE
+    assert g.fn("set_field")(g.ctx, _abi.FIELDS["nlphi"], ptr) == _abi.PIB_ERR_STATE
+    assert g.fn("scalar_get")(g.ctx, (ctypes.c_double * 4)()) == _abi.PIB_ERR_STATE
     assert g.fn("scalar_set")(g.ctx, -1.0, 1.0, 1.0) == _abi.PIB_ERR_ARG
     assert g.fn("scalar_set")(g.ctx, 0.5, 1.0, 1.0) == 0
     assert g.fn("scalstep_modes")(g.ctx, 2) == _abi.PIB_ERR_ARG

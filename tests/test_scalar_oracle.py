@@ -143,6 +143,12 @@ def test_random_fields_match_a_dense_cyclic_solve(orc):
 def test_scalar_errors(orc):
     lib, ctx = _bare_oracle(orc, NX, MY, MZ, DX, 0.05, DT)
     assert lib.orc_scalstep_modes(ctx, 1) == _abi.PIB_ERR_STATE
+    a = np.zeros(SHP, dtype=np.complex128, order="F")
+    ptr = a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    assert lib.orc_get_field(ctx, F_PHI, ptr) == _abi.PIB_ERR_STATE
+    assert lib.orc_set_field(ctx, F_NL, ptr) == _abi.PIB_ERR_STATE
+    assert lib.orc_scalar_get(ctx, (ctypes.c_double * 4)()) == _abi.PIB_ERR_STATE
+    assert lib.orc_get_field(ctx, 99, ptr) == _abi.PIB_ERR_ARG
     assert lib.orc_scalar_set(ctx, 0.0, 1.0, 1.0) == _abi.PIB_ERR_ARG
     assert lib.orc_scalar_set(ctx, 1.0, 1.0, 1.0) == 0
     assert lib.orc_scalstep_modes(ctx, 2) == _abi.PIB_ERR_ARG    # AB2 has one stage
