@@ -375,8 +375,8 @@ def main():
         print(json.dumps({"copy_bytes": n * 4, "copies": 3}))
         return
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the x-slab sharding across GPUs is "
-                                   "described in DESIGN.md and not launched here", "n_gpus": a.gpus}))
+        print(json.dumps({"error": "bench.py measures one GPU; sharding the x-slabs across GPUs is "
+                                   "designed but not launched", "n_gpus": a.gpus}))
         sys.exit(2)
 
     work = tempfile.mkdtemp(prefix="pib_bench_")
