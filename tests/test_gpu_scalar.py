@@ -3,7 +3,8 @@ scalstep, stepthescalar.F90:67-241) against the oracle on random spectral
 state, over shapes with ragged mode counts, odd nx and long lines, both
 context precisions (the scalar solve is FP64 in both), AB2 and RK3 stages,
 and heat_ratio != heat_ratio_new with the ratios rolled across two calls.
-At 128^3 the Crank-Nicolson factor of cos modes is checked in closed form.
+On the 128^3 and 256^3 grids the Crank-Nicolson factor of cos modes is
+checked in closed form.
 
 Tolerance: relative 1e-12 to the field maximum (both sides FP64 Thomas +
 Sherman-Morrison; they differ only by fused multiply-adds).
@@ -72,8 +73,10 @@ def test_scalstep_random_state(orc, nx, my, mz, rk3, prec):
     g.close()
 
 
-def test_scalstep_cos_modes_at_128_cubed():
-    nx = my = mz = 128
+@pytest.mark.parametrize("n", [128, 256])
+def test_scalstep_cos_modes_full_size(n):
+    print(f"scalstep cos modes {n}^3", flush=True)
+    nx = my = mz = n
     cfg = _cfg(nx, my, mz, 0, 0)
     g = Bare(_abi.load_gpu(), "pib_", cfg)
     my2 = my // 2 + 1
