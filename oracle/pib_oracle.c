@@ -1756,10 +1756,16 @@ int orc_scalar_set(void* ctx, double gamma, double heat_ratio, double heat_ratio
   if (!o || !(gamma > 0) || !(heat_ratio > 0) || !(heat_ratio_new > 0)) return PIB_ERR_ARG;
   if (!o->phif) {
     size_t n = (size_t)(o->nx + 1) * o->my2 * o->mz;
-    o->phif = (cpx*)calloc(n, sizeof(cpx));
-    o->nlphif = (cpx*)calloc(n, sizeof(cpx));
-    o->onlphif = (cpx*)calloc(n, sizeof(cpx));
-    o->ffphi = (cpx*)calloc(n, sizeof(cpx));
+    cpx* f[4];
+    for (int q = 0; q < 4; ++q) f[q] = (cpx*)calloc(n, sizeof(cpx));
+    if (!f[0] || !f[1] || !f[2] || !f[3]) {
+      for (int q = 0; q < 4; ++q) free(f[q]);
+      return fail(o, PIB_ERR_ARG, "no host memory for the scalar fields");
+    }
+    o->nlphif = f[1];
+    o->onlphif = f[2];
+    o->ffphi = f[3];
+    o->phif = f[0]; /* marks the fields as present */
     o->heat_ratio_old = heat_ratio;
   }
   o->sc_gamma = gamma;

@@ -1659,10 +1659,17 @@ struct Impl : Base {
   void scalar_set(double gamma, double hr, double hrn) {
     if (!(gamma > 0) || !(hr > 0) || !(hrn > 0)) throw PibFail(PIB_ERR_ARG, "gamma and heat ratios must be > 0");
     if (!phi[0]) {
+      // phi[0] marks the fields as present, so it is set only once every
+      // buffer exists (a failed alloc leaves the context without them; the
+      // buffers already made are freed with the context)
       size_t n = (size_t)(nx + 1) * my2 * mz;
-      for (auto& f : phi) f = alloc<C2<double>>(n);
-      sscr = alloc<double>((size_t)4 * nx * my2 * mz);
-      sstatus = alloc<int>(1);
+      C2<double>* f[4];
+      for (auto& p : f) p = alloc<C2<double>>(n);
+      double* scr = alloc<double>((size_t)4 * nx * my2 * mz);
+      int* stat = alloc<int>(1);
+      sscr = scr;
+      sstatus = stat;
+      for (int q = 3; q >= 0; --q) phi[q] = f[q];
       heat_ratio_old = hr;
     }
     sc_gamma = gamma;
