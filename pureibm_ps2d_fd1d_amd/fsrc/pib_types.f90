@@ -147,18 +147,20 @@ module pib_types
        real(c_double), intent(out) :: force(*), pres(*), visc(*), torq(*)
        integer(c_int) :: st
      end function pib_get_forces
+     ! host: the field in the reference layout, real(8) or complex(8)
+     ! (assumed type, so both kinds pass their address)
      function pib_set_field(ctx, field, host) bind(C, name="pib_set_field") result(st)
-       import :: c_int, c_ptr, c_double
+       import :: c_int, c_ptr
        type(c_ptr), value :: ctx
        integer(c_int), value :: field
-       real(c_double), intent(in) :: host(*)
+       type(*), intent(in) :: host(*)
        integer(c_int) :: st
      end function pib_set_field
      function pib_get_field(ctx, field, host) bind(C, name="pib_get_field") result(st)
-       import :: c_int, c_ptr, c_double
+       import :: c_int, c_ptr
        type(c_ptr), value :: ctx
        integer(c_int), value :: field
-       real(c_double), intent(out) :: host(*)
+       type(*), intent(inout) :: host(*)
        integer(c_int) :: st
      end function pib_get_field
      function pib_compute_new_timestep(ctx, rks) bind(C, name="pib_compute_new_timestep") result(st)
