@@ -194,6 +194,39 @@ module pib_types
        integer(c_int), value :: rks
        integer(c_int) :: st
      end function pib_velstep
+     ! transform layer drop-in (fftw_interface.F90:256-338): a real (my,mz)
+     ! planes, b complex (my2,mz) planes, strides in doubles of a / of b's
+     ! complex elements as in the C header
+     function pib_ff2rc_planes(ctx, a, lda, nplanes, b, ldb) bind(C, name="pib_ff2rc_planes") result(st)
+       import :: c_int, c_ptr, c_long_long
+       type(c_ptr), value :: ctx
+       type(*), intent(in) :: a(*)
+       integer(c_long_long), value :: lda
+       integer(c_int), value :: nplanes
+       type(*), intent(inout) :: b(*)
+       integer(c_long_long), value :: ldb
+       integer(c_int) :: st
+     end function pib_ff2rc_planes
+     function pib_ff2cr_planes(ctx, b, ldb, nplanes, a, lda) bind(C, name="pib_ff2cr_planes") result(st)
+       import :: c_int, c_ptr, c_long_long
+       type(c_ptr), value :: ctx
+       type(*), intent(in) :: b(*)
+       integer(c_long_long), value :: ldb
+       integer(c_int), value :: nplanes
+       type(*), intent(inout) :: a(*)
+       integer(c_long_long), value :: lda
+       integer(c_int) :: st
+     end function pib_ff2cr_planes
+     ! cyclic tridiagonal drop-in (tridiagonal.F90:47-150), real systems
+     function pib_tridiag_batched(ctx, n, nsys, a, b, c, r, x, cyclic) bind(C, name="pib_tridiag_batched") &
+          result(st)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n, nsys, cyclic
+       real(c_double), intent(in) :: a(*), b(*), c(*), r(*)
+       real(c_double), intent(out) :: x(*)
+       integer(c_int) :: st
+     end function pib_tridiag_batched
      ! scalar transport, one species (stepthescalar.F90:67-241)
      function pib_scalar_set(ctx, gamma, heat_ratio, heat_ratio_new) bind(C, name="pib_scalar_set") result(st)
        import :: c_int, c_ptr, c_double

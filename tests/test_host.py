@@ -107,6 +107,46 @@ def test_fortran_mirrors_match_the_c_structs():
     assert list(sizes) == want
 
 
+FORTRAN_CALLER = """
+program caller
+  use pib_types
+  use iso_c_binding
+  implicit none
+  type(c_ptr) :: ctx
+  complex(c_double_complex) :: u(9, 5, 8, 3), phif(9, 5, 8, 1), bc(5, 8)
+  real(c_double) :: ubcp(9, 8, 8, 3), ar(8, 8), aa(16), x(16), out4(4)
+  integer(c_int) :: st
+  ctx = c_null_ptr
+  st = pib_set_field(ctx, PIB_F_U, u)
+  st = pib_get_field(ctx, 5, ubcp)
+  st = pib_scalar_set(ctx, 0.1d0, 1.0d0, 1.0d0)
+  st = pib_set_field(ctx, PIB_F_PHI, phif(:, :, :, 1))
+  st = pib_scalstep_modes(ctx, 1)
+  st = pib_get_field(ctx, PIB_F_PHI, phif(:, :, :, 1))
+  st = pib_scalar_get(ctx, out4)
+  st = pib_ff2rc_planes(ctx, ar, 64_c_long_long, 1, bc, 40_c_long_long)
+  st = pib_ff2cr_planes(ctx, bc, 40_c_long_long, 1, ar, 64_c_long_long)
+  st = pib_tridiag_batched(ctx, 16, 1, aa, aa, aa, aa, x, 1)
+end program caller
+"""
+
+
+def test_fortran_bindings_take_the_reference_arrays(tmp_path):
+    """A Fortran caller passes the reference's complex(8) and real(8) arrays
+    to the field, scalar, transform and tridiagonal entry points through the
+    interfaces of fsrc/pib_types.f90 (compile only, no call)."""
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    mods = os.path.join(ROOT, "pureibm_ps2d_fd1d_amd", "_build")
+    if not os.path.exists(fc) or not os.path.exists(os.path.join(mods, "pib_types.mod")):
+        pytest.skip("amdflang or the built pib_types module is missing")
+    src = tmp_path / "caller.f90"
+    src.write_text(FORTRAN_CALLER)
+    import subprocess
+    r = subprocess.run([fc, "-c", "-J", str(tmp_path), "-I", mods, str(src), "-o", str(tmp_path / "caller.o")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 def test_product_never_imports_the_oracle():
     pkg = os.path.join(ROOT, "pureibm_ps2d_fd1d_amd")
     for dirpath, _, files in os.walk(pkg):
