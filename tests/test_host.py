@@ -130,21 +130,76 @@ program caller
 end program caller
 """
 
+# the fixed-bed and moving-bed call sequences of the Fortran binding example
+FORTRAN_STEP = """
+program moving_step
+  use iso_c_binding
+  use pib_types
+  implicit none
+  integer, parameter :: nbody = 2, nbnd = 10, mx = 9, my = 8, mz = 8, my2 = 5
+  type(c_ptr)        :: ctx, dem
+  type(pib_config)   :: cfg
+  type(pib_scalars)  :: sc
+  type(pib_dem_config) :: dc
+  type(pib_dem_info)   :: di
+  integer(c_int)     :: st, rks, itrmax
+  integer(c_long_long) :: nsolid
+  integer(c_int), allocatable :: nrpr_active(:)
+  real(c_double) :: bndpts(3, nbnd), cd(nbnd), radbdy(nbody), dt
+  real(c_double), target :: xc(nbody, 3), velbdy(nbody, 3), angv(nbody, 3)
+  real(c_double) :: force(nbody, 3), pres(nbody, 3), visc(nbody, 3), torq(nbody, 3)
+  complex(c_double_complex) :: u(mx, my2, mz, 3)
+  ctx = c_null_ptr
+  cfg%nx = mx - 1;  cfg%my = my;  cfg%mz = mz;  cfg%dx = 0.1d0;  cfg%vis = 0.01d0
+  cfg%precision = PIB_FP64
+  st = pib_create(cfg, ctx)
+  st = pib_set_points(ctx, nbnd, bndpts, cd)
+  allocate(nrpr_active(nbody))
+  st = pib_set_particles(ctx, nbody, xc, radbdy, c_loc(velbdy), c_loc(angv), nsolid, nrpr_active)
+  st = pib_set_field(ctx, PIB_F_U, u)
+  st = pib_set_scalars(ctx, sc)
+  rks = 1
+  st = pib_compute_new_timestep(ctx, rks)
+  st = pib_nonlinear(ctx, rks)
+  st = pib_velstep(ctx, rks)
+  if (st == PIB_ERR_BLOWUP) stop
+  st = pib_get_scalars(ctx, sc)
+  st = pib_step(ctx, 3)
+  dc%nbody = nbody;  dc%kn = 0
+  st = pib_dem_create(dc, xc, radbdy, c_loc(velbdy), dem)
+  st = pib_get_forces(ctx, force, pres, visc, torq)
+  st = pib_dem_march(dem, dt, pres, visc, sc%mpg, sc%frame_accln, xc, velbdy, angv)
+  if (st == PIB_ERR_MOTION) stop 'MOVEMENT UNDESIRED'
+  itrmax = 1
+  do rks = 1, itrmax
+     st = pib_compute_new_timestep(ctx, rks)
+     st = pib_nonlinear_begin(ctx, rks)
+     st = pib_dem_march(dem, sc%dt, pres, visc, sc%mpg, sc%frame_accln, xc, velbdy, angv)
+     st = pib_set_particles(ctx, nbody, xc, radbdy, c_loc(velbdy), c_loc(angv), nsolid, nrpr_active)
+     st = pib_nonlinear_finish(ctx, rks)
+     st = pib_velstep(ctx, rks)
+  end do
+  st = pib_dem_get_state(dem, xc, velbdy, angv, force, di)
+end program moving_step
+"""
+
 
 def test_fortran_bindings_take_the_reference_arrays(tmp_path):
     """A Fortran caller passes the reference's complex(8) and real(8) arrays
-    to the field, scalar, transform and tridiagonal entry points through the
-    interfaces of fsrc/pib_types.f90 (compile only, no call)."""
+    to the field, scalar, transform and tridiagonal entry points, and runs the
+    fixed-bed and moving-bed call sequences, through the interfaces of
+    fsrc/pib_types.f90 (compile only, no call)."""
     fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
     mods = os.path.join(ROOT, "pureibm_ps2d_fd1d_amd", "_build")
     if not os.path.exists(fc) or not os.path.exists(os.path.join(mods, "pib_types.mod")):
         pytest.skip("amdflang or the built pib_types module is missing")
-    src = tmp_path / "caller.f90"
-    src.write_text(FORTRAN_CALLER)
     import subprocess
-    r = subprocess.run([fc, "-c", "-J", str(tmp_path), "-I", mods, str(src), "-o", str(tmp_path / "caller.o")],
-                       capture_output=True, text=True)
-    assert r.returncode == 0, r.stderr
+    for name, text in (("caller", FORTRAN_CALLER), ("moving_step", FORTRAN_STEP)):
+        src = tmp_path / (name + ".f90")
+        src.write_text(text)
+        r = subprocess.run([fc, "-c", "-J", str(tmp_path), "-I", mods, str(src), "-o", str(tmp_path / (name + ".o"))],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, (name, r.stderr)
 
 
 def test_product_never_imports_the_oracle():
