@@ -202,6 +202,41 @@ def test_fortran_bindings_take_the_reference_arrays(tmp_path):
         assert r.returncode == 0, (name, r.stderr)
 
 
+C_CALLER = r"""
+#include "pureibm_gpu.h"
+#include "pureibm_dem.h"
+#include <stddef.h>
+int main(void) {
+  void* ctx = NULL;
+  void* dem = NULL;
+  pib_config c = {0};
+  pib_dem_config dc = {0};
+  double out[4];
+  int st = pib_create(&c, &ctx);
+  st |= pib_scalar_set(ctx, 0.1, 1.0, 1.0);
+  st |= pib_scalar_get(ctx, out);
+  st |= pib_scalstep_modes(ctx, 1);
+  st |= pib_dem_create(&dc, NULL, NULL, NULL, &dem);
+  return st;
+}
+"""
+
+
+def test_c_headers_compile_as_c99_and_link(tmp_path):
+    """include/*.h are plain C99 (no warnings under -pedantic) and a C caller
+    links against the HIP library (built, not run)."""
+    cc = shutil.which("gcc")
+    if not cc or not os.path.exists(_abi.gpu_lib_path()):
+        pytest.skip("gcc or the built library is missing")
+    src = tmp_path / "caller.c"
+    src.write_text(C_CALLER)
+    import subprocess
+    r = subprocess.run([cc, "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-I",
+                        os.path.join(ROOT, "include"), str(src), "-L", os.path.dirname(_abi.gpu_lib_path()),
+                        "-lpureibm_gpu", "-o", str(tmp_path / "caller")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 def test_product_never_imports_the_oracle():
     pkg = os.path.join(ROOT, "pureibm_ps2d_fd1d_amd")
     for dirpath, _, files in os.walk(pkg):
