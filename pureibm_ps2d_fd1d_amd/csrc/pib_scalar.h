@@ -51,6 +51,7 @@ __global__ void __launch_bounds__(PHI_T) k_phistep(ScalArgs A, int* status) {
   const double v1 = cn / beta, vn = a1 / alpha;
   const double b0 = b - alpha * v1, bn = b - beta * vn;
   const double f0 = 1.0 - dt * c1 * g * w2, dc3 = dt * c3, dc4 = dt * c4, dg1 = dt * g * c1;
+  const double gl = dg1 / dx2;  // weight of the explicit second difference
   const size_t ms = (size_t)E;
   const C2<double>* ph = A.phi + q;
   const C2<double>* __restrict__ nl = A.nlphi + q;
@@ -75,7 +76,7 @@ __global__ void __launch_bounds__(PHI_T) k_phistep(ScalArgs A, int* status) {
       lx = (pm.x + pp.x) - 2.0 * p0.x;
       ly = (pm.y + pp.y) - 2.0 * p0.y;
     }
-    return mk(rx + dg1 * lx / dx2, ry + dg1 * ly / dx2);
+    return mk(rx + gl * lx, ry + gl * ly);
   };
   int status_ = 0;
   // forward sweep (tridag3 of the rhs pair and auxu)
@@ -113,9 +114,10 @@ __global__ void __launch_bounds__(PHI_T) k_phistep(ScalArgs A, int* status) {
         if (bet == 0.0) { bet = 1.0e-30; status_ = PIB_ERR_PIVOT; }
         rv = rhs(i, pm, p0, pp, N[u], W[u], F[u]);
         const double u2 = i == nx - 1 ? beta : 0.0;
-        y1 = (rv.x - a * y1) / bet;
-        y2 = (u2 - a * y2) / bet;
-        y3 = (rv.y - a * y3) / bet;
+        const double ib = 1.0 / bet;  // one division a row for the three right-hand sides
+        y1 = (rv.x - a * y1) * ib;
+        y2 = (u2 - a * y2) * ib;
+        y3 = (rv.y - a * y3) * ib;
         const size_t o = (size_t)i * ms;
         gs[o] = gam;
         y1s[o] = y1;
@@ -158,8 +160,9 @@ __global__ void __launch_bounds__(PHI_T) k_phistep(ScalArgs A, int* status) {
   const double vdotyr = v1 * y1 + vn * yr_n;
   const double vdotyi = v1 * y3 + vn * yi_n;
   const double den = 1.0 + vdotz;
+  const double sr = vdotyr / den, si = vdotyi / den;
   C2<double>* pw = A.phi + q;
-  C2<double> first = mk(y1 - vdotyr * y2 / den, y3 - vdotyi * y2 / den);
+  C2<double> first = mk(y1 - sr * y2, y3 - si * y2);
   pw[0] = first;
   for (int i0 = 1; i0 < nx; i0 += PHI_K) {
     double Z[PHI_K], Y1[PHI_K], Y3[PHI_K];
@@ -175,7 +178,7 @@ __global__ void __launch_bounds__(PHI_T) k_phistep(ScalArgs A, int* status) {
 #pragma unroll
     for (int u = 0; u < PHI_K; ++u)
       if (i0 + u < nx)
-        pw[(size_t)(i0 + u) * ms] = mk(Y1[u] - vdotyr * Z[u] / den, Y3[u] - vdotyi * Z[u] / den);
+        pw[(size_t)(i0 + u) * ms] = mk(Y1[u] - sr * Z[u], Y3[u] - si * Z[u]);
   }
   pw[(size_t)nx * ms] = mk(first.x / A.hrn, first.y / A.hrn);
   if (status_) *status = status_;

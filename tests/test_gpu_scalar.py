@@ -7,7 +7,9 @@ On the 128^3 and 256^3 grids the Crank-Nicolson factor of cos modes is
 checked in closed form.
 
 Tolerance: relative 1e-12 to the field maximum (both sides FP64 Thomas +
-Sherman-Morrison; they differ only by fused multiply-adds).
+Sherman-Morrison; the GPU multiplies by one reciprocal pivot a row where the
+reference divides each right-hand side, and fuses multiply-adds, so the two
+round differently).
 """
 import ctypes
 import math
