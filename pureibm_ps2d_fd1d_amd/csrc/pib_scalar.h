@@ -11,7 +11,11 @@
 // A workgroup is one wavefront: a 128^3 grid has only 8320 modes, and
 // one-wave workgroups spread them over 130 CUs instead of 33.  Each sweep
 // loads PHI_K planes into registers before it runs their PHI_K steps of the
-// recurrence, so a lane keeps PHI_K plane loads in flight.
+// recurrence, so a lane keeps PHI_K plane loads in flight.  The three
+// right-hand sides of a row are multiplied by one reciprocal of the pivot,
+// and the correction factor (v.y)/(1+v.z) is formed once, where the
+// reference divides each: the divisions are a lane's longest dependent
+// chains.  This rounds differently from the reference by about an ulp.
 #pragma once
 
 #define PHI_T 64
