@@ -11,11 +11,12 @@
 // A workgroup is one wavefront: a 128^3 grid has only 8320 modes, and
 // one-wave workgroups spread them over 130 CUs instead of 33.  Each sweep
 // loads PHI_K planes into registers before it runs their PHI_K steps of the
-// recurrence, so a lane keeps PHI_K plane loads in flight.  The three
-// right-hand sides of a row are multiplied by one reciprocal of the pivot,
-// and the correction factor (v.y)/(1+v.z) is formed once, where the
-// reference divides each: the divisions are a lane's longest dependent
-// chains.  This rounds differently from the reference by about an ulp.
+// recurrence, so a lane keeps PHI_K plane loads in flight.  Each row takes
+// one reciprocal of its pivot, and the multiplier of the next row and the
+// three right-hand sides multiply by it; the correction factor (v.y)/(1+v.z)
+// is formed once.  The reference divides in each of these places; the
+// divisions are a lane's longest dependent chains.  This rounds differently
+// from the reference by about an ulp.
 #pragma once
 
 #define PHI_T 64
@@ -88,7 +89,8 @@ __global__ void __launch_bounds__(PHI_T) k_phistep(ScalArgs A, int* status) {
   double bet = b0;
   if (bet == 0.0) status_ = PIB_ERR_PIVOT;
   C2<double> rv = rhs(0, pm, p0, pp, nl[0], on[0], fp[0]);
-  double y1 = rv.x / bet, y2 = alpha / bet, y3 = rv.y / bet;
+  double ib = 1.0 / bet;  // the one division a row: every pivot use multiplies by it
+  double y1 = rv.x * ib, y2 = alpha * ib, y3 = rv.y * ib;
   y1s[0] = y1;
   y2s[0] = y2;
   y3s[0] = y3;
@@ -113,12 +115,12 @@ __global__ void __launch_bounds__(PHI_T) k_phistep(ScalArgs A, int* status) {
         pm = p0;
         p0 = pp;
         pp = P[u];
-        const double gam = c / bet;
+        const double gam = c * ib;
         bet = (i == nx - 1 ? bn : b) - a * gam;
         if (bet == 0.0) { bet = 1.0e-30; status_ = PIB_ERR_PIVOT; }
         rv = rhs(i, pm, p0, pp, N[u], W[u], F[u]);
         const double u2 = i == nx - 1 ? beta : 0.0;
-        const double ib = 1.0 / bet;  // one division a row for the three right-hand sides
+        ib = 1.0 / bet;
         y1 = (rv.x - a * y1) * ib;
         y2 = (u2 - a * y2) * ib;
         y3 = (rv.y - a * y3) * ib;
