@@ -115,3 +115,35 @@ def test_scalar_state_errors():
     assert g.fn("scalstep_modes")(g.ctx, 2) == _abi.PIB_ERR_ARG
     assert g.fn("scalstep_modes")(g.ctx, 1) == 0
     g.close()
+
+
+def test_scalstep_after_fluid_steps_uses_the_steps_dt(tmp_path, orc):
+    """Three graph-replayed fluid steps of a random bed (adaptive dt), then the
+    scalar solve: it reads dt from the context's device state, as the
+    reference's phistep reads the module's dt, and agrees with the oracle run
+    through the same calls."""
+    from conftest import make_setup
+    from pureibm_ps2d_fd1d_amd import solver
+    print("scalstep after 3 fluid steps, 24^3 bed", flush=True)
+    st = make_setup(tmp_path, "SC", kind="random", my=24, dbydx=6, nbody=3, seed=4, re=30.0)
+    g = solver.Solver(st)
+    o = solver.Solver(st, lib=orc.load(), prefix="orc_")
+    g.step(3)
+    o.step(3)
+    assert abs(g.scalars().dt - o.scalars().dt) <= 1e-12 * o.scalars().dt
+    shp = (st.nx + 1, st.my // 2 + 1, st.mz)
+    rng = np.random.default_rng(9)
+    for b in (g, o):
+        b.scalar_set(0.05, 1.02, 0.98)
+    for f in ("phi", "nlphi", "onlphi", "ffphi"):
+        a = rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+        g.set_field(f, a)
+        o.set_field(f, a)
+    g.scalstep_modes(1)
+    o.scalstep_modes(1)
+    e = rel_err(g.field("phi"), o.field("phi"))
+    print(f"  phi {e:.2e}", flush=True)
+    assert e < 1e-11
+    assert g.scalar_get() == o.scalar_get()
+    g.close()
+    o.close()
