@@ -196,7 +196,10 @@ int pib_scalar_get(void* ctx, double* out);
  * every (ky,kz) mode from phif, nlphif, onlphif and ffphi as they stand
  * (nlphi_graduphi, which forms them, is not part of this interface), then
  * heat_ratio_old = heat_ratio, heat_ratio = heat_ratio_new.  No source term;
- * serial x-periodic branch.  FP64 in both precisions. */
+ * serial x-periodic branch.  FP64 in both precisions.  The solve is enqueued
+ * on the context stream without waiting for it; a zero pivot
+ * (tridiagonal.F90:62,72-76) is raised on the device and returned as
+ * PIB_ERR_PIVOT by the next pib_sync, pib_step or step entry point. */
 int pib_scalstep_modes(void* ctx, int rks);
 
 /* ---- transform layer drop-in (fftw_interface.F90:256-338) ------------ */

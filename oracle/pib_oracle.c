@@ -787,6 +787,17 @@ int orc_compute_new_timestep(void* ctx, int rks) {
     dtnew = o->cfg.cfl_max * o->dx / umax_tmp;
   else
     dtnew = s->dt;
+  /* moving particles (dependent_functions.F90:1156-1165): when the fastest
+   * particle breaks the CFL bound the reference recomputes dtnew, still from
+   * the fluid's umax_tmp */
+  if (o->cfg.move_particles) {
+    double up_max = SMALL_NUMBER;
+    for (int m = 1; m <= o->nbody; ++m) {
+      double v1 = PB(o, velbdy, m, 1), v2 = PB(o, velbdy, m, 2), v3 = PB(o, velbdy, m, 3);
+      up_max = fmax(up_max, sqrt(v1 * v1 + v2 * v2 + v3 * v3));
+    }
+    if (up_max * s->dt / o->dx > o->cfg.cfl_max) dtnew = o->cfg.cfl_max * o->dx / umax_tmp;
+  }
   s->dtnew = dtnew;
   s->umax_tmp = umax_tmp;
   for (int n = 0; n < 3; ++n) {

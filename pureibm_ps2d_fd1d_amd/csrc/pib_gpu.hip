@@ -224,7 +224,8 @@ __global__ void __launch_bounds__(SCAL_T) k_scalar_cnt(DevState* stg, const doub
   state_get(sw, stg);
   double v[12];
   for (int q = 0; q < 12; ++q) v[q] = 0.0;
-  double mx[3] = {PIB_SMALL_NUMBER, PIB_SMALL_NUMBER, PIB_SMALL_NUMBER};
+  // mx[3]: the fastest particle |velbdy| (dependent_functions.F90:1156-1160)
+  double mx[4] = {PIB_SMALL_NUMBER, PIB_SMALL_NUMBER, PIB_SMALL_NUMBER, PIB_SMALL_NUMBER};
   for (int i = t; i < nx; i += SCAL_T)
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -234,14 +235,20 @@ __global__ void __launch_bounds__(SCAL_T) k_scalar_cnt(DevState* stg, const doub
       v[c * 3 + 2] += o[2];
       mx[c] = fmax(mx[c], o[3]);
     }
-  for (int m = t; m < nb; m += SCAL_T)
+  for (int m = t; m < nb; m += SCAL_T) {
+    double vb[3];
 #pragma unroll
-    for (int n = 0; n < 3; ++n) v[9 + n] += velbdy[(size_t)n * nb + m];
+    for (int n = 0; n < 3; ++n) {
+      vb[n] = velbdy[(size_t)n * nb + m];
+      v[9 + n] += vb[n];
+    }
+    mx[3] = fmax(mx[3], sqrt(vb[0] * vb[0] + vb[1] * vb[1] + vb[2] * vb[2]));
+  }
   state_put(&L, sw);
   DevState* st = &L;
   block_sums_w<12>(v, sh);
-  double mxc[3] = {mx[0], mx[1], mx[2]};
-  block_maxes_w<3>(mxc, sh);
+  double mxc[4] = {mx[0], mx[1], mx[2], mx[3]};
+  block_maxes_w<4>(mxc, sh);
   if (t == 0) {
     pib_scalars& s = st->s;
     double umax_tmp = fmax(fmax(mxc[0], mxc[1]), mxc[2]);
@@ -250,6 +257,8 @@ __global__ void __launch_bounds__(SCAL_T) k_scalar_cnt(DevState* stg, const doub
       dtnew = st->cfl_max * st->dx / umax_tmp;
     else
       dtnew = s.dt;
+    // moving particles (:1161-1164): the new dt still comes from the fluid's umax_tmp
+    if (st->move_particles && mxc[3] * s.dt / st->dx > st->cfl_max) dtnew = st->cfl_max * st->dx / umax_tmp;
     s.dtnew = dtnew;
     s.umax_tmp = umax_tmp;
     double ntot = (double)((long long)st->nx * st->my * st->mz);
@@ -676,7 +685,6 @@ struct Impl : Base {
   // scalar transport, one species (allocated by the first scalar_set)
   C2<double>* phi[4] = {nullptr, nullptr, nullptr, nullptr};  // phif, nlphif, onlphif, ffphi
   double* sscr = nullptr;
-  int* sstatus = nullptr;
   double sc_gamma = 0, heat_ratio = 1, heat_ratio_new = 1, heat_ratio_old = 1;
   // particles
   int nbody = 0, nbnd = 0;
@@ -1666,9 +1674,7 @@ struct Impl : Base {
       C2<double>* f[4];
       for (auto& p : f) p = alloc<C2<double>>(n);
       double* scr = alloc<double>((size_t)4 * nx * my2 * mz);
-      int* stat = alloc<int>(1);
       sscr = scr;
-      sstatus = stat;
       for (int q = 3; q >= 0; --q) phi[q] = f[q];
       heat_ratio_old = hr;
     }
@@ -1688,15 +1694,12 @@ struct Impl : Base {
     if (!phi[0]) throw PibFail(PIB_ERR_STATE, "pib_scalar_set first");
     const int E = my2 * mz;
     ScalArgs a{nx, my, mz, my2, rks, sc_gamma, heat_ratio, heat_ratio_new, dst, phi[0], phi[1], phi[2], phi[3], sscr};
-    HIPCHK(hipMemsetAsync(sstatus, 0, sizeof(int), stream));
-    k_phistep<<<(E + PHI_T - 1) / PHI_T, PHI_T, 0, stream>>>(a, sstatus);
+    // a zero pivot is raised in the device status word, like the fluid
+    // solves', and reported by the next call that waits on the stream
+    k_phistep<<<(E + PHI_T - 1) / PHI_T, PHI_T, 0, stream>>>(a, &dst->status);
     HIPCHK(hipGetLastError());
-    int st = 0;
-    HIPCHK(hipMemcpyAsync(&st, sstatus, sizeof(int), hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
     heat_ratio_old = heat_ratio;
     heat_ratio = heat_ratio_new;
-    if (st) throw PibFail(st, "zero pivot in the scalar solve");
   }
   void set_field(int f, const double* h) {
     if (f >= PIB_F_PHI && f <= PIB_F_FFPHI) return put_phi(scalar_field(f), h);

@@ -187,5 +187,6 @@ __global__ void __launch_bounds__(PHI_T) k_phistep(ScalArgs A, int* status) {
         pw[(size_t)(i0 + u) * ms] = mk(Y1[u] - sr * Z[u], Y3[u] - si * Z[u]);
   }
   pw[(size_t)nx * ms] = mk(first.x / A.hrn, first.y / A.hrn);
-  if (status_) *status = status_;
+  // raised into the context's device status word, kept if already raised
+  if (status_) atomicCAS(status, 0, status_);
 }

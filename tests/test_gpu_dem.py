@@ -267,3 +267,34 @@ def test_moving_bed_restart_continues(tmp_path):
     assert rel_err(b.field("u"), a.field("u")) < 1e-9
     for x in (a, b, da, db):
         x.close()
+
+
+def test_moving_particle_cfl_branch_matches_the_oracle(orc, tmp_path):
+    """compute_new_timestep's moving-particle branch (dependent_functions.F90:
+    1156-1165) on the GPU against the oracle: the fastest particle breaks the
+    CFL bound while the fluid keeps it, so the new dt comes from the branch
+    (cfl_max*dx/umax_tmp); one whole step then agrees with the oracle."""
+    st = make_setup(tmp_path, "MC", kind="random", my=24, dbydx=6, nbody=4, seed=12, re=20.0, adaptive=True)
+    st.config.move_particles = 1
+    st.config.rhos, st.config.rhof = 2.0, 1.0
+    g = solver.Solver(st)
+    o = solver.Solver(st, lib=orc.load(), prefix="orc_")
+    s0 = o.scalars()
+    cfl_max, dx = st.config.cfl_max, st.config.dx
+    up = 4.0 * cfl_max * dx / s0.dt
+    vel = np.zeros((st.nbody, 3))
+    vel[2] = [0.6 * up, 0.0, 0.8 * up]
+    vel[0] = [0.1 * up, 0.1 * up, 0.0]
+    for s_ in (g, o):
+        s_.update_particles(st.xc, vel)
+        s_.compute_new_timestep(1)
+    sg, so = g.scalars(), o.scalars()
+    assert so.umax_tmp * s0.dt / dx <= cfl_max and so.dt == cfl_max * dx / so.umax_tmp
+    assert abs(sg.dt - so.dt) <= 1e-14 * so.dt and abs(sg.t - so.t) <= 1e-14 * so.t
+    for s_ in (g, o):
+        s_.nonlinear(1)
+        s_.velstep(1)
+    assert rel_err(g.field("u"), o.field("u")) < 1e-10
+    assert rel_err(g.forces()["force"], o.forces()["force"]) < 1e-9
+    g.close()
+    o.close()

@@ -118,18 +118,22 @@ def test_scalar_state_errors():
 
 
 def test_scalstep_after_fluid_steps_uses_the_steps_dt(tmp_path, orc):
-    """Three graph-replayed fluid steps of a random bed (adaptive dt), then the
-    scalar solve: it reads dt from the context's device state, as the
-    reference's phistep reads the module's dt, and agrees with the oracle run
-    through the same calls."""
+    """Five fluid steps of a random bed (adaptive dt) in two pib_step calls --
+    steps 3 and 4 replay the captured step graph, step 5 (the last of a call)
+    runs eagerly from the state the replays wrote -- then the scalar solve: it
+    reads dt from the context's device state, as the reference's phistep reads
+    the module's dt, and agrees with the oracle run through the same calls.
+    The replayed path is the one pib_step takes from a context's third step on
+    (graph replays are bitwise equal to eager steps, tests/test_gpu_graph.py)."""
     from conftest import make_setup
     from pureibm_ps2d_fd1d_amd import solver
-    print("scalstep after 3 fluid steps, 24^3 bed", flush=True)
+    print("scalstep after 5 fluid steps (2 replayed from a graph), 24^3 bed", flush=True)
     st = make_setup(tmp_path, "SC", kind="random", my=24, dbydx=6, nbody=3, seed=4, re=30.0)
     g = solver.Solver(st)
     o = solver.Solver(st, lib=orc.load(), prefix="orc_")
+    g.step(2)
     g.step(3)
-    o.step(3)
+    o.step(5)
     assert abs(g.scalars().dt - o.scalars().dt) <= 1e-12 * o.scalars().dt
     shp = (st.nx + 1, st.my // 2 + 1, st.mz)
     rng = np.random.default_rng(9)
@@ -141,6 +145,7 @@ def test_scalstep_after_fluid_steps_uses_the_steps_dt(tmp_path, orc):
         o.set_field(f, a)
     g.scalstep_modes(1)
     o.scalstep_modes(1)
+    assert g._fn("sync")(g.ctx) == 0  # the solve's device status
     e = rel_err(g.field("phi"), o.field("phi"))
     print(f"  phi {e:.2e}", flush=True)
     assert e < 1e-11

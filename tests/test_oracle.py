@@ -389,3 +389,32 @@ def test_compute_new_timestep_masked_means(tmp_path, orc):
         assert abs(s.umean_temp[c] - ub[..., c].mean()) < 1e-13
     assert abs(s.umax_tmp - np.abs(ub).max()) < 1e-14
     assert s.ufmean_old[0] == s0.ufmean[0]
+
+
+@pytest.mark.parametrize("move", [0, 1])
+def test_new_timestep_moving_particle_cfl(tmp_path, orc, move):
+    """dependent_functions.F90:1150-1165: dtnew = cfl_max*dx/umax_tmp when the
+    fluid breaks the CFL bound; with moving particles the same formula (still
+    from the fluid's umax_tmp) also applies when the fastest particle breaks
+    it, even where the fluid alone keeps dt.  Case: up_max*dt/dx > cfl_max >=
+    umax_tmp*dt/dx, so the new dt differs between the two branches."""
+    st = make_setup(tmp_path, f"MV{move}", kind="random", my=24, dbydx=6, nbody=4, seed=12, re=20.0,
+                    adaptive=True)
+    st.config.move_particles = move
+    o = solver.Solver(st, lib=orc.load(), prefix="orc_")
+    s0 = o.scalars()
+    cfl_max, dx = st.config.cfl_max, st.config.dx
+    up = 4.0 * cfl_max * dx / s0.dt  # the fastest particle moves 4 cfl_max cells a step
+    vel = np.zeros((st.nbody, 3))
+    vel[2] = [0.6 * up, 0.0, 0.8 * up]  # |v| = up, composed over components
+    vel[0] = [0.1 * up, 0.1 * up, 0.0]
+    o.update_particles(st.xc, vel)
+    o.compute_new_timestep(1)
+    s = o.scalars()
+    assert s.umax_tmp * s0.dt / dx <= cfl_max  # the fluid alone keeps dt
+    if move:
+        assert s.dt == cfl_max * dx / s.umax_tmp and s.dt != s0.dt
+    else:
+        assert s.dt == s0.dt
+    assert s.dtnew == s.dt
+    o.close()
