@@ -222,6 +222,22 @@ int pib_ff2cr_planes(void* ctx, const double* b, long long ldb, int nplanes,
 int pib_tridiag_batched(void* ctx, int n, int nsys, const double* a, const double* b,
                         const double* c, const double* r, double* x, int cyclic);
 
+/* pib_tridiag_batched on device memory, allocating nothing.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops).  d_a..d_x are device
+ * pointers in the layout above; the kernel is enqueued on `stream` (a
+ * hipStream_t, NULL = the null stream) and the call returns without waiting.
+ * The workspace is a static device array of 4*PIB_TRIDIAG_NOALLOC_MAX doubles
+ * (32 MiB) that each device holds until the process ends, so n*nsys must not
+ * exceed PIB_TRIDIAG_NOALLOC_MAX (else PIB_ERR_ARG).  Every call on one device
+ * shares that array: calls that overlap on one device (other streams, other
+ * threads) race.  ctx may be NULL; when given it receives the error text.
+ * Its address is resolved per device at pib_create or at the first call. */
+#define PIB_TRIDIAG_NOALLOC_MAX (1 << 20)
+int pib_tridiag_batched_noalloc(void* ctx, int n, int nsys, const double* d_a, const double* d_b,
+                                const double* d_c, const double* d_r, double* d_x, int cyclic,
+                                void* stream);
+
 /* ---- timers ----------------------------------------------------------- */
 /* Milliseconds spent in each phase over the last pib_step call, measured with
  * HIP events on the context stream: [0] compute_new_timestep, [1] nonlinear

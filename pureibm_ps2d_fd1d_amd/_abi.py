@@ -23,6 +23,7 @@ PIB_ERR_BLOWUP = -5
 PIB_ERR_PIVOT = -6
 PIB_ERR_MOTION = -7
 PIB_ERR_CAPACITY = -8
+PIB_TRIDIAG_NOALLOC_MAX = 1 << 20  # rows of the no-alloc tridiagonal workspace
 
 PIB_TUNE_VELSTEP = 0
 PIB_TUNE_GRAPH = 1
@@ -140,6 +141,7 @@ _GPU_ONLY_SIGS = {
     "ff2rc_planes": ([_VP, _PD, ctypes.c_longlong, _I, _PD, ctypes.c_longlong], _I),
     "ff2cr_planes": ([_VP, _PD, ctypes.c_longlong, _I, _PD, ctypes.c_longlong], _I),
     "tridiag_batched": ([_VP, _I, _I, _PD, _PD, _PD, _PD, _PD, _I], _I),
+    "tridiag_batched_noalloc": ([_VP, _I, _I, _VP, _VP, _VP, _VP, _VP, _I, _VP], _I),
     "phase_times": ([_VP, _PD], _I),
     "set_timing": ([_VP, _I], _I),
     "kernel_times": ([_VP, _I, ctypes.c_char_p, _PD, _PI, _PI], _I),

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <string>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "pib_kernels.h"
@@ -593,6 +594,13 @@ __global__ void k_tridiag(int n, int nsys, const double* a, const double* b, con
     x[(size_t)i * ms + s] = cyclic ? yy - vy * y2[(size_t)i * ms + s] / (1.0 + vz) : yy;
   }
 }
+
+// Workspace of pib_tridiag_batched_noalloc: the four scratch vectors of
+// k_tridiag for at most PIB_TRIDIAG_NOALLOC_MAX rows in all.  Each device the
+// library runs on holds its own copy until the process ends.  Host code never
+// passes this name to a kernel: it uses the device address that
+// hipGetSymbolAddress returns, kept per device in scratch_dev.
+static __device__ __align__(256) double scratch[4 * (size_t)PIB_TRIDIAG_NOALLOC_MAX];
 
 // ===========================================================================
 // host side
@@ -1883,6 +1891,30 @@ int finish_status(void* ctx, int st) {
   return PIB_OK;
 }
 
+// Device address of `scratch` on each device, resolved once per device with
+// that device current (at pib_create, or at the first no-alloc call).
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double* scratch_dev[MAX_DEVICES] = {};
+int scratch_setup(int dev, double** out, std::string& err) {
+  if (dev < 0 || dev >= MAX_DEVICES) {
+    err = "device ordinal out of range";
+    return PIB_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return PIB_ERR_HIP;
+    }
+    scratch_dev[dev] = (double*)ptr;
+  }
+  *out = scratch_dev[dev];
+  return PIB_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1898,6 +1930,12 @@ int pib_create(const pib_config* cfg, void** ctx) {
     if (cfg->precision == PIB_FP64) c->impl = new Impl<double>(*cfg);
     else c->impl = new Impl<float>(*cfg);
   });
+  if (st == PIB_OK) {  // the no-alloc entry point's workspace on this device
+    int dev = 0;
+    double* ptr = nullptr;
+    st = hipGetDevice(&dev) == hipSuccess ? scratch_setup(dev, &ptr, c->err) : PIB_ERR_HIP;
+    if (st != PIB_OK) delete c->impl;
+  }
   if (st != PIB_OK) {
     fprintf(stderr, "pib_create: %s\n", c->err.c_str());
     delete c;
@@ -2090,6 +2128,40 @@ int pib_tridiag_batched(void* ctx, int n, int nsys, const double* a, const doubl
     for (auto& q : d) (void)hipFree(q);
     (void)hipFree(scr);
   });
+}
+int pib_tridiag_batched_noalloc(void* ctx, int n, int nsys, const double* d_a, const double* d_b,
+                                const double* d_c, const double* d_r, double* d_x, int cyclic, void* stream) {
+  Ctx* cc = (Ctx*)ctx;
+  std::string err;
+  int st = PIB_OK;
+  if (n < 3 || nsys < 1 || !d_a || !d_b || !d_c || !d_r || !d_x) {
+    err = "n >= 3, nsys >= 1 and five device pointers";
+    st = PIB_ERR_ARG;
+  } else if ((long long)n * nsys > (long long)PIB_TRIDIAG_NOALLOC_MAX) {
+    err = "n*nsys exceeds PIB_TRIDIAG_NOALLOC_MAX (" + std::to_string(PIB_TRIDIAG_NOALLOC_MAX) + " rows)";
+    st = PIB_ERR_ARG;
+  } else {
+    int dev = 0;
+    double* scr = nullptr;
+    if (hipGetDevice(&dev) != hipSuccess) {
+      err = "hipGetDevice failed";
+      st = PIB_ERR_HIP;
+    } else {
+      st = scratch_setup(dev, &scr, err);
+    }
+    if (st == PIB_OK) {
+      // every scratch entry the kernel reads it has written first in the same
+      // launch, so nothing is cleared
+      k_tridiag<<<(nsys + 255) / 256, 256, 0, (hipStream_t)stream>>>(n, nsys, d_a, d_b, d_c, d_r, d_x, cyclic, scr);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) {
+        err = std::string("k_tridiag launch: ") + hipGetErrorString(e);
+        st = PIB_ERR_HIP;
+      }
+    }
+  }
+  if (st != PIB_OK && cc) cc->err = err;
+  return st;
 }
 
 }  // extern "C"
