@@ -53,6 +53,8 @@ def load():
         lib.orc_quadgener.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_int, PD, PD]
         lib.orc_quadgener.restype = ctypes.c_int
+        lib.orc_pressure_modes.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        lib.orc_pressure_modes.restype = ctypes.c_int
         lib.orc_get_if_rev.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_ubyte)]
         lib.orc_get_if_rev.restype = ctypes.c_int
         _lib = lib

@@ -1744,6 +1744,20 @@ int orc_velstep(void* ctx, int rks) {
   return PIB_OK;
 }
 
+/* The pressure predictor alone (poisson.F90:55-239 for every mode), so that
+ * its right-hand side and symbol can be pinned: in a periodic box the later
+ * ustep/divcorr pair cancels whatever the predictor leaves in p, so the
+ * predictor cannot be seen in velstep's results. */
+int orc_pressure_modes(void* ctx, int rks) {
+  orc* o = (orc*)ctx;
+  if (!o || rks < 1 || rks > o->itrmax) return PIB_ERR_ARG;
+  int st = 0, s2;
+  for (int j = 1; j <= o->my2; ++j)
+    for (int k = 1; k <= o->mz; ++k)
+      if ((s2 = pressure(o, rks, j, k))) st = s2;
+  return st ? fail(o, st, "zero pivot in tridiagonal solve") : PIB_OK;
+}
+
 int orc_step(void* ctx, int nsteps) {
   orc* o = (orc*)ctx;
   if (!o || nsteps < 0) return PIB_ERR_ARG;

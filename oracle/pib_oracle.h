@@ -35,6 +35,7 @@ int orc_scalstep_modes(void* ctx, int rks);
 int orc_nonlinear_begin(void* ctx, int rks);
 int orc_nonlinear_finish(void* ctx, int rks);
 int orc_velstep(void* ctx, int rks);
+int orc_pressure_modes(void* ctx, int rks);
 int orc_step(void* ctx, int nsteps);
 
 /* transform layer: one plane each call, a(my,mz) real col-major, b(my2,mz) complex */
