@@ -327,13 +327,27 @@ def time_moving_bed(workload, steps=5, warmup=2):
             "ms_per_step": wall / steps * 1e3, "dem_substeps_per_step": info.factor}
 
 
-def cpu_baseline(setup, nsteps):
-    """Single-core oracle on the same workload (bounded sample: one untimed
-    first step, then nsteps timed steps)."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(setup, nsteps, threads):
+    """The oracle on the same workload, on ``threads`` host threads (its plane
+    transforms and mode solves run under OpenMP; sums stay serial): one
+    untimed first step, then nsteps timed steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     oracle.build()
-    o = solver.Solver(setup, lib=oracle.load(), prefix="orc_")
+    lib = oracle.load()
+    lib.orc_set_threads(threads)
+    o = solver.Solver(setup, lib=lib, prefix="orc_")
     o.step(1)
     t0 = time.perf_counter()
     o.step(nsteps)
@@ -450,10 +464,16 @@ def main():
             out["scalar_solve"] = {"error": str(e)}
     if not a.no_cpu:
         ncpu = max(1, a.cpu_steps)
-        tcpu = cpu_baseline(st64, ncpu)
-        out["cpu_baseline"] = {"value": N * ncpu / tcpu / 1e6, "unit": "Mgridpt-steps/s", "cores": 1,
-                               "kind": "port", "sample": f"{ncpu} FP64 step(s) of the same 128^3 bed "
-                                                         f"on the CPU oracle, {tcpu:.1f} s"}
+        nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        tall = cpu_baseline(st64, 2 * ncpu, nthr)
+        t1 = cpu_baseline(st64, ncpu, 1)
+        out["cpu_baseline"] = {"value": N * 2 * ncpu / tall / 1e6, "unit": "Mgridpt-steps/s", "cores": nthr,
+                               "kind": "port",
+                               "sample": f"{2 * ncpu} FP64 step(s) of the same 128^3 bed on the CPU oracle "
+                                         f"with {nthr} OpenMP threads, {tall:.1f} s",
+                               "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+                               "serial": {"value": N * ncpu / t1 / 1e6, "cores": 1,
+                                          "sample": f"{ncpu} FP64 step(s), one thread, {t1:.1f} s"}}
     print(json.dumps(out), flush=True)
 
 

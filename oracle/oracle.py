@@ -18,11 +18,13 @@ _lib = None
 
 
 def build(force=False):
-    """Compile the oracle with gcc (FP contraction off, so each op rounds as written)."""
+    """Compile the oracle with gcc (FP contraction off, so each op rounds as
+    written; OpenMP for the independent plane and mode loops)."""
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
     if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < max(map(os.path.getmtime, SRCS)):
-        subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared",
-                               "-o", LIB] + SRCS + ["-lm"])
+        subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared",
+                               "-o", LIB + ".tmp"] + SRCS + ["-lm"])
+        os.replace(LIB + ".tmp", LIB)
     return LIB
 
 
@@ -53,6 +55,10 @@ def load():
         lib.orc_quadgener.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_int, PD, PD]
         lib.orc_quadgener.restype = ctypes.c_int
+        lib.orc_set_threads.argtypes = [ctypes.c_int]
+        lib.orc_set_threads.restype = None
+        lib.orc_get_threads.argtypes = []
+        lib.orc_get_threads.restype = ctypes.c_int
         lib.orc_pressure_modes.argtypes = [ctypes.c_void_p, ctypes.c_int]
         lib.orc_pressure_modes.restype = ctypes.c_int
         lib.orc_get_if_rev.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_ubyte)]

@@ -1,18 +1,20 @@
 /*
  * pib_oracle.c -- TEST INFRASTRUCTURE ONLY (parity checker + CPU baseline).
  *
- * Serial FP64 restatement of PUReIBM-PS2D-FD1D's per-time-step fluid solve in
- * plain C.  Each routine follows the reference routine named in its comment
+ * FP64 restatement of PUReIBM-PS2D-FD1D's per-time-step fluid solve in plain
+ * C.  The plane transforms and the per-mode line solves, which are
+ * independent of each other, run under OpenMP (orc_set_threads); every sum
+ * over planes, points or bodies stays serial in the reference's order, so
+ * the results are the same bits for any thread count.  Each routine follows the reference routine named in its comment
  * (paths relative to PUReIBM_PS2D-FD1D/SRC/FLO), keeping the reference's loop
  * order and expression association so that the restatement can be read
  * against the Fortran line by line.  Arrays are held in the reference's
  * column-major layout with 1-based index helpers.  Build with
  * -ffp-contract=off so that every operation rounds as written.
  *
- * Parity unpinned (no independent check beyond these restatements): the
- * mpg controller of compute_mpg_at_n, the drag normalisation and ferror of
- * bcset, and the reflected-velocity formula of calc_inner_reversal_forcing.
- * Everything else is pinned by tests/test_oracle.py.
+ * Pinned by tests/test_oracle.py and tests/test_oracle_pins.py (closed forms,
+ * library routines, brute force).  Parity unpinned: the ferror history of
+ * bcset, a convergence diagnostic that no routine of the step reads.
  *
  * The only product-independent choices are (a) the FFT, a recursive
  * mixed-radix DIT DFT with directly evaluated twiddles standing in for FFTW
@@ -26,6 +28,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <omp.h>
 
 typedef struct { double re, im; } cpx;
 
@@ -109,18 +112,40 @@ static int small_factor(int n) {
 /* exp(i*2*pi*t/n) for t < n, one table per length (cached) */
 typedef struct { int n; cpx* tab; } twcache;
 static twcache g_tw[64];
-static int g_ntw = 0;
-static const cpx* tw_get(int n) {
-  for (int q = 0; q < g_ntw; ++q)
+static int g_ntw = 0; /* entries are complete before the count is published */
+static const cpx* tw_find(int n) {
+  int cnt = __atomic_load_n(&g_ntw, __ATOMIC_ACQUIRE);
+  for (int q = 0; q < cnt; ++q)
     if (g_tw[q].n == n) return g_tw[q].tab;
+  return NULL;
+}
+static const cpx* tw_get(int n) {
+  const cpx* found = tw_find(n);
+  if (found) return found;
   cpx* t = (cpx*)malloc(sizeof(cpx) * (size_t)n);
   for (int k = 0; k < n; ++k) {
     long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
     t[k] = C((double)cosl(a), (double)sinl(a));
   }
-  if (g_ntw < 64) { g_tw[g_ntw].n = n; g_tw[g_ntw].tab = t; ++g_ntw; }
-  return t;
+#pragma omp critical(orc_twiddles)
+  {
+    found = tw_find(n);
+    if (!found && g_ntw < 64) {
+      g_tw[g_ntw].n = n;
+      g_tw[g_ntw].tab = t;
+      __atomic_store_n(&g_ntw, g_ntw + 1, __ATOMIC_RELEASE);
+      found = t;
+      t = NULL;
+    }
+  }
+  if (!found) return t; /* table full: this call keeps its own copy (leaked) */
+  free(t);
+  return found;
 }
+
+/* threads for the plane and mode loops (0: OpenMP's default) */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : omp_get_num_procs()); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
 /* out[k] = sum_r in[r*s] exp(sign*2*pi*i*r*k/m); twiddles from the length-ntop
  * table (m divides ntop), indexed by the exact integer phase r*k mod m */
 static void dft_rec(const cpx* in, size_t s, cpx* out, int m, int sign, cpx* tmp, const cpx* tab, int ntop) {
@@ -207,6 +232,11 @@ void orc_ff2cr(int my, int mz, const double* bin, double* a) {
   free(row);
   free(res);
 }
+
+/* ---- per-thread plane buffers for the parallel plane loops ----------- */
+typedef struct { double *ra, *rb, *rc; cpx *ca, *cb, *cc; } planebuf;
+static planebuf pb_alloc(const orc* o);
+static void pb_free(planebuf* b) { free(b->ra); free(b->rb); free(b->rc); free(b->ca); free(b->cb); free(b->cc); }
 
 /* ---- plane gather/scatter around the transforms ---------------------- */
 /* real plane i of a (nx+1 or nx, my, mz) array with leading dim ld */
@@ -453,6 +483,18 @@ int orc_create(const pib_config* cfg, void** ctx) {
   o->count_fluid = (long long)o->nx * my * mz;
   *ctx = o;
   return PIB_OK;
+}
+
+static planebuf pb_alloc(const orc* o) {
+  planebuf b;
+  size_t nr = (size_t)o->my * o->mz, nc = (size_t)o->my2 * o->mz;
+  b.ra = (double*)xcalloc(nr, sizeof(double));
+  b.rb = (double*)xcalloc(nr, sizeof(double));
+  b.rc = (double*)xcalloc(nr, sizeof(double));
+  b.ca = (cpx*)xcalloc(nc, sizeof(cpx));
+  b.cb = (cpx*)xcalloc(nc, sizeof(cpx));
+  b.cc = (cpx*)xcalloc(nc, sizeof(cpx));
+  return b;
 }
 
 static void free_particles(orc* o) {
@@ -756,14 +798,21 @@ int orc_compute_new_timestep(void* ctx, int rks) {
   double umax_loc = SMALL_NUMBER, vmax_loc = SMALL_NUMBER, wmax_loc = SMALL_NUMBER;
   double ufl[3] = {0, 0, 0}, usl[3] = {0, 0, 0}, uml[3] = {0, 0, 0};
   double mesh_vel[3] = {0, 0, 0};
+#pragma omp parallel
+  {
+    planebuf w = pb_alloc(o);
+#pragma omp for schedule(static) collapse(2)
+    for (int i = 1; i <= nx; ++i)
+      for (int n = 1; n <= 3; ++n) {
+        for (int k = 1; k <= mz; ++k)
+          for (int j = 1; j <= my2; ++j) w.ca[(size_t)(k - 1) * my2 + (j - 1)] = o->u[IU(o, i, j, k, n)];
+        orc_ff2cr(my, mz, (const double*)w.ca, w.ra);
+        for (int k = 1; k <= mz; ++k)
+          for (int j = 1; j <= my; ++j) o->ubcp[IR(o, i, j, k, n)] = w.ra[(size_t)(k - 1) * my + (j - 1)];
+      }
+    pb_free(&w);
+  }
   for (int i = 1; i <= nx; ++i) {
-    for (int n = 1; n <= 3; ++n) {
-      for (int k = 1; k <= mz; ++k)
-        for (int j = 1; j <= my2; ++j) o->ca[(size_t)(k - 1) * my2 + (j - 1)] = o->u[IU(o, i, j, k, n)];
-      orc_ff2cr(my, mz, (const double*)o->ca, o->ra);
-      for (int k = 1; k <= mz; ++k)
-        for (int j = 1; j <= my; ++j) o->ubcp[IR(o, i, j, k, n)] = o->ra[(size_t)(k - 1) * my + (j - 1)];
-    }
     for (int k = 1; k <= mz; ++k)
       for (int j = 1; j <= my; ++j) {
         for (int n = 1; n <= 3; ++n) o->ubcp[IR(o, i, j, k, n)] = o->ubcp[IR(o, i, j, k, n)] + s->umean[n - 1];
@@ -832,11 +881,17 @@ int orc_compute_new_timestep(void* ctx, int rks) {
   s->mixmeanslipmod = sqrt(mixmeanslip[0] * mixmeanslip[0] + mixmeanslip[1] * mixmeanslip[1] +
                            mixmeanslip[2] * mixmeanslip[2]);
   /* calc_pressure */
-  for (int i = 1; i <= nx; ++i) {
-    for (int k = 1; k <= mz; ++k)
-      for (int j = 1; j <= my2; ++j) o->ca[(size_t)(k - 1) * my2 + (j - 1)] = o->p[IP(o, i, j, k)];
-    orc_ff2cr(my, mz, (const double*)o->ca, o->ra);
-    put_rplane(o, o->pbcp, nx, i, o->ra);
+#pragma omp parallel
+  {
+    planebuf w = pb_alloc(o);
+#pragma omp for schedule(static)
+    for (int i = 1; i <= nx; ++i) {
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my2; ++j) w.ca[(size_t)(k - 1) * my2 + (j - 1)] = o->p[IP(o, i, j, k)];
+      orc_ff2cr(my, mz, (const double*)w.ca, w.ra);
+      put_rplane(o, o->pbcp, nx, i, w.ra);
+    }
+    pb_free(&w);
   }
   if (rks == o->itrmax && o->cfg.adaptive) s->dt = dtnew;
   s->t = s->t + s->dt * (o->coef[rks - 1][0] + o->coef[rks - 1][1]);
@@ -868,12 +923,15 @@ static void form_nl(orc* o) {
   /* fdderiv(mx, my, mz, 3) */
   double* fd = (double*)xcalloc((size_t)(nx + 1) * pl * 3, sizeof(double));
 #define FD(i, j, k, n) fd[((((size_t)(n)-1) * mz + ((k)-1)) * my + ((j)-1)) * (nx + 1) + ((i)-1)]
+#pragma omp parallel
+  {
   double* pr[5];
   cpx* pf[5];
   for (int q = 0; q < 5; ++q) {
     pr[q] = (double*)xcalloc(pl, sizeof(double));
     pf[q] = (cpx*)xcalloc((size_t)my2 * mz, sizeof(cpx));
   }
+#pragma omp for schedule(static)
   for (int i = 1; i <= nx; ++i) {
     for (int k = 1; k <= mz; ++k)
       for (int j = 1; j <= my; ++j) {
@@ -902,18 +960,24 @@ static void form_nl(orc* o) {
         o->nl[IU(o, i, j, k, 3)] = cadd(cik(wy, pf[4][q]), cik(wz, pf[1][q]));
       }
   }
+  for (int q = 0; q < 5; ++q) { free(pr[q]); free(pf[q]); }
+  }
   if (center) {
     for (int n = 1; n <= 3; ++n)
       for (int k = 1; k <= mz; ++k)
         for (int j = 1; j <= my; ++j) FD(o->mx, j, k, n) = FD(1, j, k, n);
   } else {
-    /* MUSCL reconstruction + Rusanov flux, nl_allflow.F90:142-294 */
+    /* MUSCL reconstruction + Rusanov flux, nl_allflow.F90:142-294; the
+     * (j, k) lines are independent */
+#pragma omp parallel
+    {
     double* uplus = (double*)xcalloc((size_t)nx + 1, sizeof(double));
     double* uminus = (double*)xcalloc((size_t)nx + 1, sizeof(double));
     double* Uplus = (double*)xcalloc((size_t)nx + 1, sizeof(double));
     double* Uminus = (double*)xcalloc((size_t)nx + 1, sizeof(double));
     double* slope = (double*)xcalloc((size_t)nx + 1, sizeof(double));
     double slope_factor = 1.0;
+#pragma omp for schedule(static)
     for (int k = 1; k <= mz; ++k)
       for (int j = 1; j <= my; ++j)
         for (int idim = 1; idim <= 3; ++idim) {
@@ -953,8 +1017,13 @@ static void form_nl(orc* o) {
           FD(o->mx, j, k, idim) = FD(1, j, k, idim);
         }
     free(uplus); free(uminus); free(Uplus); free(Uminus); free(slope);
+    }
   }
+#pragma omp parallel
+  {
+  planebuf w = pb_alloc(o);
   for (int idim = 1; idim <= 3; ++idim) {
+#pragma omp for schedule(static)
     for (int i = 1; i <= nx; ++i) {
       for (int k = 1; k <= mz; ++k)
         for (int j = 1; j <= my; ++j) {
@@ -966,18 +1035,20 @@ static void form_nl(orc* o) {
             v = (FD(ip1, j, k, idim) - FD(im1, j, k, idim)) / (2.0 * dx);
           else
             v = (FD(i, j, k, idim) - FD(im1, j, k, idim)) / dx;
-          o->ra[(size_t)(k - 1) * my + (j - 1)] = v;
+          w.ra[(size_t)(k - 1) * my + (j - 1)] = v;
         }
-      orc_ff2rc(my, mz, o->ra, (double*)o->cb);
+      orc_ff2rc(my, mz, w.ra, (double*)w.cb);
       for (int k = 1; k <= mz; ++k)
         for (int j = 1; j <= my2; ++j)
-          o->nl[IU(o, i, j, k, idim)] = cadd(o->nl[IU(o, i, j, k, idim)], o->cb[(size_t)(k - 1) * my2 + (j - 1)]);
+          o->nl[IU(o, i, j, k, idim)] = cadd(o->nl[IU(o, i, j, k, idim)], w.cb[(size_t)(k - 1) * my2 + (j - 1)]);
     }
+#pragma omp single
     for (int k = 1; k <= mz; ++k)
       for (int j = 1; j <= my2; ++j) o->nl[IU(o, nx + 1, j, k, idim)] = o->nl[IU(o, 1, j, k, idim)];
   }
+  pb_free(&w);
+  }
 #undef FD
-  for (int q = 0; q < 5; ++q) { free(pr[q]); free(pf[q]); }
   free(fd);
 }
 
@@ -987,6 +1058,10 @@ static void form_nl(orc* o) {
 /* calc_visc :1767-1815 */
 static void calc_visc(orc* o) {
   int nx = o->nx, my = o->my, mz = o->mz, my2 = o->my2;
+#pragma omp parallel
+  {
+  planebuf w = pb_alloc(o);
+#pragma omp for schedule(static) collapse(2)
   for (int n = 1; n <= 3; ++n)
     for (int i = 1; i <= nx + 1; ++i) {
       int im = (i == 1) ? nx : i - 1;
@@ -998,11 +1073,13 @@ static void calc_visc(orc* o) {
           cpx f = cscal(1.0 / o->dx2, d2);
           f = csub(f, cscal(o->w2[(size_t)(k - 1) * my2 + (j - 1)], uc));
           f = cscal(o->vis, f);
-          o->ca[(size_t)(k - 1) * my2 + (j - 1)] = f;
+          w.ca[(size_t)(k - 1) * my2 + (j - 1)] = f;
         }
-      orc_ff2cr(my, mz, (const double*)o->ca, o->ra);
-      put_rplane(o, o->diffn + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, o->ra);
+      orc_ff2cr(my, mz, (const double*)w.ca, w.ra);
+      put_rplane(o, o->diffn + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, w.ra);
     }
+  pb_free(&w);
+  }
 }
 
 /* compute_omega :1648-1765 (omega stored in fr) */
@@ -1010,14 +1087,18 @@ static void compute_omega(orc* o) {
   int nx = o->nx, my = o->my, mz = o->mz, my2 = o->my2, mxf = o->mxf;
   double dx = o->dx;
   double* om = o->fr;
+#pragma omp parallel
+  {
+  planebuf w = pb_alloc(o);
+#pragma omp for schedule(static)
   for (int i = 1; i <= nx + 1; ++i) {
     for (int k = 1; k <= mz; ++k)
       for (int j = 1; j <= my2; ++j)
-        o->ca[(size_t)(k - 1) * my2 + (j - 1)] = cik(o->wy[j - 1], o->u[IU(o, i, j, k, 1)]);
-    orc_ff2cr(my, mz, (const double*)o->ca, o->ra); /* dudy */
+        w.ca[(size_t)(k - 1) * my2 + (j - 1)] = cik(o->wy[j - 1], o->u[IU(o, i, j, k, 1)]);
+    orc_ff2cr(my, mz, (const double*)w.ca, w.ra); /* dudy */
     for (int k = 1; k <= mz; ++k)
       for (int j = 1; j <= my; ++j) {
-        double dudy = o->ra[(size_t)(k - 1) * my + (j - 1)];
+        double dudy = w.ra[(size_t)(k - 1) * my + (j - 1)];
         double v;
         if (i == 1)
           v = (o->ubcp[IR(o, i + 1, j, k, 2)] - o->ubcp[IR(o, mxf - 1, j, k, 2)]) / (2.0 * dx) - dudy;
@@ -1028,14 +1109,15 @@ static void compute_omega(orc* o) {
         om[IR(o, i, j, k, 3)] = v;
       }
   }
+#pragma omp for schedule(static)
   for (int i = 1; i <= nx + 1; ++i) {
     for (int k = 1; k <= mz; ++k)
       for (int j = 1; j <= my2; ++j)
-        o->ca[(size_t)(k - 1) * my2 + (j - 1)] = cik(o->wz[k - 1], o->u[IU(o, i, j, k, 1)]);
-    orc_ff2cr(my, mz, (const double*)o->ca, o->ra); /* dudz */
+        w.ca[(size_t)(k - 1) * my2 + (j - 1)] = cik(o->wz[k - 1], o->u[IU(o, i, j, k, 1)]);
+    orc_ff2cr(my, mz, (const double*)w.ca, w.ra); /* dudz */
     for (int k = 1; k <= mz; ++k)
       for (int j = 1; j <= my; ++j) {
-        double dudz = o->ra[(size_t)(k - 1) * my + (j - 1)];
+        double dudz = w.ra[(size_t)(k - 1) * my + (j - 1)];
         double v;
         if (i == 1)
           v = dudz - (o->ubcp[IR(o, i + 1, j, k, 3)] - o->ubcp[IR(o, mxf - 1, j, k, 3)]) / (2.0 * dx);
@@ -1046,19 +1128,22 @@ static void compute_omega(orc* o) {
         om[IR(o, i, j, k, 2)] = v;
       }
   }
+#pragma omp for schedule(static)
   for (int i = 1; i <= nx + 1; ++i) {
     for (int k = 1; k <= mz; ++k)
       for (int j = 1; j <= my2; ++j) {
-        o->ca[(size_t)(k - 1) * my2 + (j - 1)] = cik(o->wz[k - 1], o->u[IU(o, i, j, k, 2)]); /* dvdz */
-        o->cb[(size_t)(k - 1) * my2 + (j - 1)] = cik(o->wy[j - 1], o->u[IU(o, i, j, k, 3)]); /* dwdy */
+        w.ca[(size_t)(k - 1) * my2 + (j - 1)] = cik(o->wz[k - 1], o->u[IU(o, i, j, k, 2)]); /* dvdz */
+        w.cb[(size_t)(k - 1) * my2 + (j - 1)] = cik(o->wy[j - 1], o->u[IU(o, i, j, k, 3)]); /* dwdy */
       }
-    orc_ff2cr(my, mz, (const double*)o->ca, o->ra);
-    orc_ff2cr(my, mz, (const double*)o->cb, o->rb);
+    orc_ff2cr(my, mz, (const double*)w.ca, w.ra);
+    orc_ff2cr(my, mz, (const double*)w.cb, w.rb);
     for (int k = 1; k <= mz; ++k)
       for (int j = 1; j <= my; ++j) {
         size_t q = (size_t)(k - 1) * my + (j - 1);
-        om[IR(o, i, j, k, 1)] = o->rb[q] - o->ra[q];
+        om[IR(o, i, j, k, 1)] = w.rb[q] - w.ra[q];
       }
+  }
+  pb_free(&w);
   }
   memcpy(o->omega_copy, o->fr, sizeof(double) * (size_t)(nx + 1) * my * mz * 3);
 }
@@ -1068,8 +1153,12 @@ static void calc_pgrad(orc* o) {
   int nx = o->nx, my = o->my, mz = o->mz, my2 = o->my2;
   double dx = o->dx;
   pib_scalars* s = &o->s;
-  cpx* f3 = (cpx*)xcalloc((size_t)my2 * mz, sizeof(cpx));
-  double* r3 = (double*)xcalloc((size_t)my * mz, sizeof(double));
+#pragma omp parallel
+  {
+  planebuf w = pb_alloc(o);
+  cpx* f3 = w.cc;
+  double* r3 = w.rc;
+#pragma omp for schedule(static)
   for (int i = 1; i <= nx; ++i) {
     for (int k = 1; k <= mz; ++k)
       for (int j = 1; j <= my2; ++j) {
@@ -1078,23 +1167,23 @@ static void calc_pgrad(orc* o) {
         if (i == 1) f1 = cdivr(csub(o->p[IP(o, i + 1, j, k)], o->p[IP(o, nx, j, k)]), 2.0 * dx);
         else if (i == nx) f1 = cdivr(csub(o->p[IP(o, 1, j, k)], o->p[IP(o, i - 1, j, k)]), 2.0 * dx);
         else f1 = cdivr(csub(o->p[IP(o, i + 1, j, k)], o->p[IP(o, i - 1, j, k)]), 2.0 * dx);
-        o->ca[q] = f1;
-        o->cb[q] = cik(o->wy[j - 1], o->p[IP(o, i, j, k)]);
+        w.ca[q] = f1;
+        w.cb[q] = cik(o->wy[j - 1], o->p[IP(o, i, j, k)]);
         f3[q] = cik(o->wz[k - 1], o->p[IP(o, i, j, k)]);
       }
-    orc_ff2cr(my, mz, (const double*)o->ca, o->ra);
-    orc_ff2cr(my, mz, (const double*)o->cb, o->rb);
+    orc_ff2cr(my, mz, (const double*)w.ca, w.ra);
+    orc_ff2cr(my, mz, (const double*)w.cb, w.rb);
     orc_ff2cr(my, mz, (const double*)f3, r3);
     for (int k = 1; k <= mz; ++k)
       for (int j = 1; j <= my; ++j) {
         size_t q = (size_t)(k - 1) * my + (j - 1);
-        o->ppr[IPPR(o, i, j, k, 1)] = o->ra[q] + s->mpg[0];
-        o->ppr[IPPR(o, i, j, k, 2)] = o->rb[q] + s->mpg[1];
+        o->ppr[IPPR(o, i, j, k, 1)] = w.ra[q] + s->mpg[0];
+        o->ppr[IPPR(o, i, j, k, 2)] = w.rb[q] + s->mpg[1];
         o->ppr[IPPR(o, i, j, k, 3)] = r3[q] + s->mpg[2];
       }
   }
-  free(f3);
-  free(r3);
+  pb_free(&w);
+  }
 }
 
 /* ---- trilinear ('lpi' order 2) interpolation -------------------------
@@ -1466,15 +1555,22 @@ static int bcset_forcing(orc* o, int rks) {
     s->frmean[n] = s->frmean[n] / (double)((long long)nx * my * mz);
     s->frmeanfluid[n] = s->frmeanfluid[n] / (double)o->count_fluid;
   }
+#pragma omp parallel
+  {
+    planebuf w = pb_alloc(o);
+#pragma omp for schedule(static) collapse(2)
+    for (int n = 1; n <= 3; ++n)
+      for (int i = 1; i <= nx; ++i) {
+        for (int j = 1; j <= my; ++j)
+          for (int k = 1; k <= mz; ++k) o->fr[IR(o, i, j, k, n)] = o->fr[IR(o, i, j, k, n)] - s->frmean[n - 1];
+        get_rplane(o, o->fr + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, w.ra);
+        orc_ff2rc(my, mz, w.ra, (double*)w.ca);
+        for (int k = 1; k <= mz; ++k)
+          for (int j = 1; j <= my2; ++j) o->ff[IU(o, i, j, k, n)] = w.ca[(size_t)(k - 1) * my2 + (j - 1)];
+      }
+    pb_free(&w);
+  }
   for (int n = 1; n <= 3; ++n) {
-    for (int i = 1; i <= nx; ++i) {
-      for (int j = 1; j <= my; ++j)
-        for (int k = 1; k <= mz; ++k) o->fr[IR(o, i, j, k, n)] = o->fr[IR(o, i, j, k, n)] - s->frmean[n - 1];
-      get_rplane(o, o->fr + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, o->ra);
-      orc_ff2rc(my, mz, o->ra, (double*)o->ca);
-      for (int k = 1; k <= mz; ++k)
-        for (int j = 1; j <= my2; ++j) o->ff[IU(o, i, j, k, n)] = o->ca[(size_t)(k - 1) * my2 + (j - 1)];
-    }
     for (int k = 1; k <= mz; ++k)
       for (int j = 1; j <= my2; ++j) o->ff[IU(o, nx + 1, j, k, n)] = o->ff[IU(o, 1, j, k, n)];
   }
@@ -1495,18 +1591,24 @@ static int nonlinear_part(orc* o, int rks, int part) {
   for (size_t q = 0; q < nc; ++q) { o->onl[q] = o->nl[q]; o->nl[q] = C(0.0, 0.0); }
   form_nl(o);
   for (size_t q = 0; q < nc; ++q) o->nl[q] = C(-o->nl[q].re, -o->nl[q].im);
-  for (int n = 1; n <= 3; ++n)
-    for (int i = 1; i <= nx + 1; ++i) {
-      for (int k = 1; k <= mz; ++k)
-        for (int j = 1; j <= my2; ++j) {
-          o->ca[(size_t)(k - 1) * my2 + (j - 1)] = o->nl[IU(o, i, j, k, n)];
-          o->cb[(size_t)(k - 1) * my2 + (j - 1)] = o->onl[IU(o, i, j, k, n)];
-        }
-      orc_ff2cr(my, mz, (const double*)o->ca, o->ra);
-      orc_ff2cr(my, mz, (const double*)o->cb, o->rb);
-      put_rplane(o, o->nlbc + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, o->ra);
-      put_rplane(o, o->onlbc + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, o->rb);
-    }
+#pragma omp parallel
+  {
+    planebuf w = pb_alloc(o);
+#pragma omp for schedule(static) collapse(2)
+    for (int n = 1; n <= 3; ++n)
+      for (int i = 1; i <= nx + 1; ++i) {
+        for (int k = 1; k <= mz; ++k)
+          for (int j = 1; j <= my2; ++j) {
+            w.ca[(size_t)(k - 1) * my2 + (j - 1)] = o->nl[IU(o, i, j, k, n)];
+            w.cb[(size_t)(k - 1) * my2 + (j - 1)] = o->onl[IU(o, i, j, k, n)];
+          }
+        orc_ff2cr(my, mz, (const double*)w.ca, w.ra);
+        orc_ff2cr(my, mz, (const double*)w.cb, w.rb);
+        put_rplane(o, o->nlbc + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, w.ra);
+        put_rplane(o, o->onlbc + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, w.rb);
+      }
+    pb_free(&w);
+  }
   if (o->nbody > 0) return part == 1 ? bcset_drag(o, rks) : bcset(o, rks);
   return PIB_OK;
 }
@@ -1711,16 +1813,26 @@ static int divcorr(orc* o, int rks, int j, int k) {
 int orc_velstep(void* ctx, int rks) {
   orc* o = (orc*)ctx;
   if (!o || rks < 1 || rks > o->itrmax) return PIB_ERR_ARG;
-  int st = 0, s2;
+  int st = 0;
+  /* the modes are independent; a zero pivot anywhere is reported */
+#pragma omp parallel for schedule(static) collapse(2) reduction(min : st)
   for (int j = 1; j <= o->my2; ++j)
-    for (int k = 1; k <= o->mz; ++k)
-      if ((s2 = pressure(o, rks, j, k))) st = s2;
+    for (int k = 1; k <= o->mz; ++k) {
+      int s2 = pressure(o, rks, j, k);
+      if (s2 < st) st = s2;
+    }
+#pragma omp parallel for schedule(static) collapse(2) reduction(min : st)
   for (int j = 1; j <= o->my2; ++j)
-    for (int k = 1; k <= o->mz; ++k)
-      if ((s2 = ustep(o, rks, j, k))) st = s2;
+    for (int k = 1; k <= o->mz; ++k) {
+      int s2 = ustep(o, rks, j, k);
+      if (s2 < st) st = s2;
+    }
+#pragma omp parallel for schedule(static) collapse(2) reduction(min : st)
   for (int j = 1; j <= o->my2; ++j)
-    for (int k = 1; k <= o->mz; ++k)
-      if ((s2 = divcorr(o, rks, j, k))) st = s2;
+    for (int k = 1; k <= o->mz; ++k) {
+      int s2 = divcorr(o, rks, j, k);
+      if (s2 < st) st = s2;
+    }
   /* check_divergence, steptheflow.F90:219-240 */
   double divmax = 0.0;
   for (int k = 1; k <= o->mz; ++k)
@@ -1751,10 +1863,13 @@ int orc_velstep(void* ctx, int rks) {
 int orc_pressure_modes(void* ctx, int rks) {
   orc* o = (orc*)ctx;
   if (!o || rks < 1 || rks > o->itrmax) return PIB_ERR_ARG;
-  int st = 0, s2;
+  int st = 0;
+#pragma omp parallel for schedule(static) collapse(2) reduction(min : st)
   for (int j = 1; j <= o->my2; ++j)
-    for (int k = 1; k <= o->mz; ++k)
-      if ((s2 = pressure(o, rks, j, k))) st = s2;
+    for (int k = 1; k <= o->mz; ++k) {
+      int s2 = pressure(o, rks, j, k);
+      if (s2 < st) st = s2;
+    }
   return st ? fail(o, st, "zero pivot in tridiagonal solve") : PIB_OK;
 }
 
@@ -1874,10 +1989,11 @@ int orc_scalstep_modes(void* ctx, int rks) {
   if (!o || rks < 1 || rks > o->itrmax) return PIB_ERR_ARG;
   if (!o->phif) return fail(o, PIB_ERR_STATE, "pib_scalar_set first");
   int st = 0;
+#pragma omp parallel for schedule(static) collapse(2) reduction(min : st)
   for (int j = 1; j <= o->my2; ++j)
     for (int k = 1; k <= o->mz; ++k) {
       int r = phistep(o, rks, j, k);
-      if (r) st = r;
+      if (r < st) st = r;
     }
   o->heat_ratio_old = o->heat_ratio;
   o->heat_ratio = o->heat_ratio_new;

@@ -36,6 +36,8 @@ int orc_nonlinear_begin(void* ctx, int rks);
 int orc_nonlinear_finish(void* ctx, int rks);
 int orc_velstep(void* ctx, int rks);
 int orc_pressure_modes(void* ctx, int rks);
+void orc_set_threads(int n);
+int orc_get_threads(void);
 int orc_step(void* ctx, int nsteps);
 
 /* transform layer: one plane each call, a(my,mz) real col-major, b(my2,mz) complex */

@@ -418,3 +418,25 @@ def test_new_timestep_moving_particle_cfl(tmp_path, orc, move):
         assert s.dt == s0.dt
     assert s.dtnew == s.dt
     o.close()
+
+
+@pytest.mark.parametrize("scheme", ["center", "upwind"])
+def test_threads_do_not_change_a_bit(tmp_path, orc, scheme):
+    """The oracle's plane transforms and mode solves run under OpenMP; the
+    sums stay serial, so one thread and four give the same bits."""
+    lib = orc.load()
+    kw = dict(kind="random", my=24, dbydx=6, nbody=3, seed=8, re=40.0, discr_scheme=scheme)
+    out = []
+    nt0 = lib.orc_get_threads()
+    try:
+        for nt in (1, 4):
+            lib.orc_set_threads(nt)
+            st = make_setup(tmp_path, f"TH{nt}", **kw)
+            o = solver.Solver(st, lib=lib, prefix="orc_")
+            o.step(3)
+            out.append((o.field("u"), o.field("p"), o.forces()["force"], o.scalars().norm_drag))
+            o.close()
+    finally:
+        lib.orc_set_threads(nt0)
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
