@@ -122,6 +122,11 @@ typedef struct pib_scalars {
 #define PIB_F_NLPHI 15  /* complex (nx+1,my2,mz): nlphif */
 #define PIB_F_ONLPHI 16 /* complex (nx+1,my2,mz): onlphif */
 #define PIB_F_FFPHI 17  /* complex (nx+1,my2,mz): ffphi, the scalar IBM forcing */
+/* adjoint state (pib_adjoint_begin first); plane nx+1 is the periodic copy of
+ * plane 1, so it is ignored on set and returned as a copy of plane 1 */
+#define PIB_F_AU 18     /* complex (nx+1,my2,mz,3): adjoint of u */
+#define PIB_F_AP 19     /* complex (nx,my2,mz): adjoint of p */
+#define PIB_F_ANL 20    /* complex (nx+1,my2,mz,3): adjoint of nl as the next step reads it (onl) */
 
 /* ---- lifecycle -------------------------------------------------------- */
 /* Replaces alloc_mem + the grid/wavenumber/coef part of initflo
@@ -237,6 +242,39 @@ int pib_tridiag_batched(void* ctx, int n, int nsys, const double* a, const doubl
 int pib_tridiag_batched_noalloc(void* ctx, int n, int nsys, const double* d_a, const double* d_b,
                                 const double* d_c, const double* d_r, double* d_x, int cyclic,
                                 void* stream);
+
+/* ---- adjoint of the fixed-bed step ------------------------------------ */
+/* No counterpart in the reference, which has no adjoint.  The discrete
+ * transpose of one step of this interface (compute_new_timestep, nonlinear
+ * with bcset, velstep) with respect to its state u, p, nl, umean, the
+ * particle centres xc and the mean pressure gradient mpg, for AB2/CN
+ * (rk3 = 0) with adaptive = 0, set_mpg = 1, the central x-flux, dorpr = 1,
+ * dobnd = 0, include_frmeanfluid = 0 and a fixed bed; else
+ * PIB_ERR_UNSUPPORTED.  The solid mask, the active reversal pairs, the
+ * stencil cells and nrpr_active are piecewise constant in xc and are held
+ * fixed.  Inner product: the re and im parts of every spectral entry of
+ * planes 1..nx, and umean.  The objective is J = sum over the reversed steps
+ * of sum_m wforce(m,:) . force(m,:), force as pib_get_forces returns it.
+ *
+ * pib_adjoint_begin allocates the adjoint state and zeroes it together with
+ * the accumulated gradients.  pib_adjoint_step reverses ONE step: the
+ * context's forward state must be the state at the start of that step (as
+ * pib_checkpoint_pop restores it); on return the adjoint state holds the
+ * adjoint of the step's inputs, the gradients have the step's part added, and
+ * the forward state has been advanced through the step's nonlinear part (it
+ * is scratch until the next pop).  wforce (nbody,3) is that step's weight on
+ * the per-particle force, or NULL.  pib_adjoint_get returns dJ/dxc (nbody,3),
+ * dJ/dmpg (3) and the adjoint of umean (3); any pointer may be NULL.
+ * pib_adjoint_set_umean seeds the adjoint of umean (3). */
+int pib_adjoint_begin(void* ctx);
+int pib_adjoint_step(void* ctx, const double* wforce);
+int pib_adjoint_get(void* ctx, double* dxc, double* dmpg, double* dumean);
+int pib_adjoint_set_umean(void* ctx, const double* dumean);
+/* Checkpoint stack of the forward state (u, p, nl, the physical copy of nl
+ * and the scalars): push copies it on the device, pop restores the top and
+ * drops it.  PIB_ERR_STATE when popping an empty stack. */
+int pib_checkpoint_push(void* ctx);
+int pib_checkpoint_pop(void* ctx);
 
 /* ---- timers ----------------------------------------------------------- */
 /* Milliseconds spent in each phase over the last pib_step call, measured with

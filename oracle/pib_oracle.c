@@ -76,6 +76,10 @@ typedef struct orc {
   long long count_solid, count_fluid;
   pib_scalars s;
   int have_points, have_particles;
+  /* adjoint state and the checkpoint stack (orc_adjoint_*, orc_checkpoint_*) */
+  void* adj;
+  void** ck;
+  int nck, ckcap;
   /* plane work arrays */
   double *ra, *rb;
   cpx *ca, *cb;
@@ -507,6 +511,7 @@ static void free_particles(orc* o) {
   o->nrpr_active = NULL;
 }
 
+static void adj_free(orc* o);
 int orc_destroy(void* ctx) {
   orc* o = (orc*)ctx;
   if (!o) return PIB_OK;
@@ -518,6 +523,7 @@ int orc_destroy(void* ctx) {
   free(o->ra); free(o->rb); free(o->ca); free(o->cb);
   free(o->bnd); free(o->cd);
   free_particles(o);
+  adj_free(o);
   free(o);
   return PIB_OK;
 }
@@ -710,6 +716,8 @@ int orc_get_if_rev(void* ctx, unsigned char* if_rev) {
 }
 
 /* ---- field transfer ---------------------------------------------------- */
+static int adj_field_info(orc* o, int field, void** ptr, size_t* ndoubles);
+static void adj_fix_copy_plane(orc* o, cpx* f, int copy);
 static int field_info(orc* o, int field, void** ptr, size_t* ndoubles) {
   size_t nc = (size_t)(o->nx + 1) * o->my2 * o->mz * 3;
   size_t nr = (size_t)(o->nx + 1) * o->my * o->mz * 3;
@@ -731,7 +739,7 @@ static int field_info(orc* o, int field, void** ptr, size_t* ndoubles) {
     case PIB_F_NLPHI: *ptr = o->nlphif; *ndoubles = 2 * nc / 3; return o->phif ? 0 : 1;
     case PIB_F_ONLPHI: *ptr = o->onlphif; *ndoubles = 2 * nc / 3; return o->phif ? 0 : 1;
     case PIB_F_FFPHI: *ptr = o->ffphi; *ndoubles = 2 * nc / 3; return o->phif ? 0 : 1;
-    default: return -1;
+    default: return adj_field_info(o, field, ptr, ndoubles);
   }
 }
 int orc_set_field(void* ctx, int field, const double* host) {
@@ -740,8 +748,10 @@ int orc_set_field(void* ctx, int field, const double* host) {
   size_t n;
   if (!o || !host) return PIB_ERR_ARG;
   int r = field_info(o, field, &ptr, &n); /* 1: a scalar field before orc_scalar_set */
-  if (r) return r > 0 ? fail(o, PIB_ERR_STATE, "pib_scalar_set first") : PIB_ERR_ARG;
+  if (r) return r > 0 ? fail(o, PIB_ERR_STATE, r == 2 ? "pib_adjoint_begin first" : "pib_scalar_set first")
+                      : PIB_ERR_ARG;
   memcpy(ptr, host, n * sizeof(double));
+  if (field == PIB_F_AU || field == PIB_F_ANL) adj_fix_copy_plane(o, (cpx*)ptr, 0);
   return PIB_OK;
 }
 int orc_get_field(void* ctx, int field, double* host) {
@@ -757,7 +767,9 @@ int orc_get_field(void* ctx, int field, double* host) {
   void* ptr;
   size_t n;
   int r = field_info(o, field, &ptr, &n);
-  if (r) return r > 0 ? fail(o, PIB_ERR_STATE, "pib_scalar_set first") : PIB_ERR_ARG;
+  if (r) return r > 0 ? fail(o, PIB_ERR_STATE, r == 2 ? "pib_adjoint_begin first" : "pib_scalar_set first")
+                      : PIB_ERR_ARG;
+  if (field == PIB_F_AU || field == PIB_F_ANL) adj_fix_copy_plane(o, (cpx*)ptr, 1);
   memcpy(host, ptr, n * sizeof(double));
   return PIB_OK;
 }
@@ -1998,5 +2010,721 @@ int orc_scalstep_modes(void* ctx, int rks) {
   o->heat_ratio_old = o->heat_ratio;
   o->heat_ratio = o->heat_ratio_new;
   if (st) return fail(o, st, "zero pivot in the scalar solve");
+  return PIB_OK;
+}
+
+/* ======================================================================
+ * Adjoint of the fixed-bed Euler step.  No counterpart in the reference,
+ * which has no adjoint: this is the discrete transpose of the step restated
+ * above (orc_compute_new_timestep, nonlinear_part, orc_velstep) for AB2/CN
+ * with a fixed dt, set_mpg, the central x-flux, dorpr and a bed at rest.
+ * The mask, the active reversal pairs, the stencil cells and nrpr_active
+ * are piecewise constant in the centres and are held fixed.  The variables
+ * are the re and im parts of the spectral entries of planes 1..nx (plane
+ * nx+1 is the periodic copy), umean, xc and mpg.  Every operator of the step
+ * is linear in the state except the products of form_nl, so the transpose
+ * of each is formed from the same fields:
+ *   c2r (row-column convention) x = Re sum c_ky Z e^{+i theta}, c_ky = 1 for
+ *     ky = 0, my/2 and 2 otherwise -> Z_bar = c_ky * my*mz * ff2rc(x_bar)
+ *   ff2rc  Y = (my*mz)^-1 sum x e^{-i theta} -> x_bar = ff2cr(Y_bar/c_ky)/(my*mz)
+ *   (i k) z -> (-i k) z_bar;  the symmetric Toeplitz line matrices are their
+ *   own transposes; the (0,0) pressure solve keeps rows 1..nx-1, re only.
+ * Pinned by finite differences of the forward step and by the tangent of the
+ * step in its state (tests/test_adjoint_oracle.py).
+ * ====================================================================== */
+typedef struct {
+  cpx *au, *aunew, *ap, *apnew, *anl, *anlp, *aff; /* IU (planes 1..nx) / IP */
+  double *aubcp, *anlbc, *aonlbc, *adiffn, *aom, *afr; /* IR (planes 1..nx) */
+  double *apbcp, *appr;                                /* IPR, IPPR */
+  double aumean[3], ampg[3];
+  double* axc; /* (nbody, 3) */
+  int nbody;
+} orc_adj;
+
+typedef struct {
+  cpx *u, *p, *nl, *onl;
+  pib_scalars s;
+} orc_ck;
+
+static void adj_free(orc* o) {
+  orc_adj* A = (orc_adj*)o->adj;
+  if (A) {
+    free(A->au); free(A->aunew); free(A->ap); free(A->apnew); free(A->anl); free(A->anlp); free(A->aff);
+    free(A->aubcp); free(A->anlbc); free(A->aonlbc); free(A->adiffn); free(A->aom); free(A->afr);
+    free(A->apbcp); free(A->appr); free(A->axc);
+    free(A);
+    o->adj = NULL;
+  }
+  for (int q = 0; q < o->nck; ++q) {
+    orc_ck* c = (orc_ck*)o->ck[q];
+    free(c->u); free(c->p); free(c->nl); free(c->onl); free(c);
+  }
+  free(o->ck);
+  o->ck = NULL;
+  o->nck = o->ckcap = 0;
+}
+
+static int adj_field_info(orc* o, int field, void** ptr, size_t* ndoubles) {
+  orc_adj* A = (orc_adj*)o->adj;
+  size_t nc = (size_t)(o->nx + 1) * o->my2 * o->mz * 3;
+  if (field != PIB_F_AU && field != PIB_F_AP && field != PIB_F_ANL) return -1;
+  if (!A) return 2;
+  if (field == PIB_F_AU) { *ptr = A->au; *ndoubles = 2 * nc; }
+  else if (field == PIB_F_AP) { *ptr = A->ap; *ndoubles = 2 * (size_t)o->nx * o->my2 * o->mz; }
+  else { *ptr = A->anl; *ndoubles = 2 * nc; }
+  return 0;
+}
+
+/* plane nx+1 of an adjoint spectral field is not a variable: zero it after a
+ * set, copy plane 1 into it before a get (as the forward fields hold it) */
+static void adj_fix_copy_plane(orc* o, cpx* f, int copy) {
+  for (int n = 1; n <= 3; ++n)
+    for (int k = 1; k <= o->mz; ++k)
+      for (int j = 1; j <= o->my2; ++j)
+        f[IU(o, o->nx + 1, j, k, n)] = copy ? f[IU(o, 1, j, k, n)] : C(0.0, 0.0);
+}
+
+int orc_adjoint_begin(void* ctx) {
+  orc* o = (orc*)ctx;
+  if (!o) return PIB_ERR_ARG;
+  if (!o->have_particles) return fail(o, PIB_ERR_STATE, "pib_set_particles first");
+  orc_adj* A = (orc_adj*)o->adj;
+  size_t nc = (size_t)(o->nx + 1) * o->my2 * o->mz * 3, np = (size_t)o->nx * o->my2 * o->mz;
+  size_t nr = (size_t)(o->nx + 1) * o->my * o->mz * 3, npr = (size_t)o->nx * o->my * o->mz;
+  if (A && A->nbody != o->nbody) {
+    free(A->axc);
+    A->axc = (double*)xcalloc((size_t)o->nbody * 3, sizeof(double));
+    A->nbody = o->nbody;
+  }
+  if (!A) {
+    A = (orc_adj*)calloc(1, sizeof(orc_adj));
+    A->au = (cpx*)xcalloc(nc, sizeof(cpx)); A->aunew = (cpx*)xcalloc(nc, sizeof(cpx));
+    A->anl = (cpx*)xcalloc(nc, sizeof(cpx)); A->anlp = (cpx*)xcalloc(nc, sizeof(cpx));
+    A->aff = (cpx*)xcalloc(nc, sizeof(cpx));
+    A->ap = (cpx*)xcalloc(np, sizeof(cpx)); A->apnew = (cpx*)xcalloc(np, sizeof(cpx));
+    A->aubcp = (double*)xcalloc(nr, sizeof(double)); A->anlbc = (double*)xcalloc(nr, sizeof(double));
+    A->aonlbc = (double*)xcalloc(nr, sizeof(double)); A->adiffn = (double*)xcalloc(nr, sizeof(double));
+    A->aom = (double*)xcalloc(nr, sizeof(double)); A->afr = (double*)xcalloc(nr, sizeof(double));
+    A->apbcp = (double*)xcalloc(npr, sizeof(double)); A->appr = (double*)xcalloc(3 * npr, sizeof(double));
+    A->axc = (double*)xcalloc((size_t)o->nbody * 3, sizeof(double));
+    A->nbody = o->nbody;
+    o->adj = A;
+  }
+  memset(A->au, 0, nc * sizeof(cpx));
+  memset(A->anl, 0, nc * sizeof(cpx));
+  memset(A->ap, 0, np * sizeof(cpx));
+  memset(A->axc, 0, (size_t)o->nbody * 3 * sizeof(double));
+  for (int n = 0; n < 3; ++n) A->aumean[n] = A->ampg[n] = 0.0;
+  return PIB_OK;
+}
+
+int orc_adjoint_get(void* ctx, double* dxc, double* dmpg, double* dumean) {
+  orc* o = (orc*)ctx;
+  if (!o) return PIB_ERR_ARG;
+  orc_adj* A = (orc_adj*)o->adj;
+  if (!A) return fail(o, PIB_ERR_STATE, "pib_adjoint_begin first");
+  if (dxc) memcpy(dxc, A->axc, (size_t)o->nbody * 3 * sizeof(double));
+  for (int n = 0; n < 3; ++n) {
+    if (dmpg) dmpg[n] = A->ampg[n];
+    if (dumean) dumean[n] = A->aumean[n];
+  }
+  return PIB_OK;
+}
+
+int orc_adjoint_set_umean(void* ctx, const double* dumean) {
+  orc* o = (orc*)ctx;
+  if (!o || !dumean) return PIB_ERR_ARG;
+  orc_adj* A = (orc_adj*)o->adj;
+  if (!A) return fail(o, PIB_ERR_STATE, "pib_adjoint_begin first");
+  for (int n = 0; n < 3; ++n) A->aumean[n] = dumean[n];
+  return PIB_OK;
+}
+
+int orc_checkpoint_push(void* ctx) {
+  orc* o = (orc*)ctx;
+  if (!o) return PIB_ERR_ARG;
+  size_t nc = (size_t)(o->nx + 1) * o->my2 * o->mz * 3, np = (size_t)o->nx * o->my2 * o->mz;
+  if (o->nck == o->ckcap) {
+    o->ckcap = o->ckcap ? 2 * o->ckcap : 16;
+    o->ck = (void**)realloc(o->ck, sizeof(void*) * (size_t)o->ckcap);
+  }
+  orc_ck* c = (orc_ck*)calloc(1, sizeof(orc_ck));
+  c->u = (cpx*)malloc(nc * sizeof(cpx)); memcpy(c->u, o->u, nc * sizeof(cpx));
+  c->nl = (cpx*)malloc(nc * sizeof(cpx)); memcpy(c->nl, o->nl, nc * sizeof(cpx));
+  c->onl = (cpx*)malloc(nc * sizeof(cpx)); memcpy(c->onl, o->onl, nc * sizeof(cpx));
+  c->p = (cpx*)malloc(np * sizeof(cpx)); memcpy(c->p, o->p, np * sizeof(cpx));
+  c->s = o->s;
+  o->ck[o->nck++] = c;
+  return PIB_OK;
+}
+
+int orc_checkpoint_pop(void* ctx) {
+  orc* o = (orc*)ctx;
+  if (!o) return PIB_ERR_ARG;
+  if (o->nck == 0) return fail(o, PIB_ERR_STATE, "checkpoint stack is empty");
+  size_t nc = (size_t)(o->nx + 1) * o->my2 * o->mz * 3, np = (size_t)o->nx * o->my2 * o->mz;
+  orc_ck* c = (orc_ck*)o->ck[--o->nck];
+  memcpy(o->u, c->u, nc * sizeof(cpx));
+  memcpy(o->nl, c->nl, nc * sizeof(cpx));
+  memcpy(o->onl, c->onl, nc * sizeof(cpx));
+  memcpy(o->p, c->p, np * sizeof(cpx));
+  o->s = c->s;
+  free(c->u); free(c->p); free(c->nl); free(c->onl); free(c);
+  return PIB_OK;
+}
+
+/* ---- transposed plane transforms ------------------------------------- */
+static inline double ky_weight(const orc* o, int j) { return (j == 1 || j == o->my2) ? 1.0 : 2.0; }
+/* out += (c2r)^T x for one real plane x(my, mz) */
+static void adj_c2r_T(const orc* o, const double* x, cpx* tmp, cpx* out, int accumulate) {
+  orc_ff2rc(o->my, o->mz, x, (double*)tmp);
+  double s = (double)o->my * (double)o->mz;
+  for (int k = 1; k <= o->mz; ++k)
+    for (int j = 1; j <= o->my2; ++j) {
+      size_t q = (size_t)(k - 1) * o->my2 + (j - 1);
+      cpx v = cscal(s * ky_weight(o, j), tmp[q]);
+      out[q] = accumulate ? cadd(out[q], v) : v;
+    }
+}
+/* x = (ff2rc)^T Y for one spectral plane Y(my2, mz) */
+static void adj_r2c_T(const orc* o, const cpx* y, cpx* tmp, double* x) {
+  double s = (double)o->my * (double)o->mz;
+  for (int k = 1; k <= o->mz; ++k)
+    for (int j = 1; j <= o->my2; ++j) {
+      size_t q = (size_t)(k - 1) * o->my2 + (j - 1);
+      tmp[q] = cdivr(y[q], s * ky_weight(o, j));
+    }
+  orc_ff2cr(o->my, o->mz, (const double*)tmp, x);
+}
+
+/* ---- velstep, transposed per mode ------------------------------------ */
+static inline int wrp(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+/* in place: (re, im) <- M^-1 (re, im), M the pressure/divcorr matrix of this
+ * mode (poisson.F90:81-83); the (0,0) mode solves rows 1..nx-1 of re only */
+static int adj_msolve(const orc* o, double w2, int zero_mode, double* re, double* im, double* w) {
+  int nx = o->nx;
+  double *a = w, *b = w + nx, *c = w + 2 * nx, *xr = w + 3 * nx, *xi = w + 4 * nx;
+  for (int i = 0; i < nx; ++i) {
+    a[i] = 1.0 / o->dx2 - 0.25 * w2;
+    b[i] = -2.0 / o->dx2 - 0.5 * w2;
+    c[i] = a[i];
+  }
+  int st;
+  if (zero_mode) {
+    st = orc_tridag(a, b, c, re, xr, nx - 1);
+    xr[nx - 1] = 0.0;
+    for (int i = 0; i < nx; ++i) xi[i] = 0.0;
+  } else {
+    st = sm_solve(a, b, c, re, im, xr, xi, nx);
+  }
+  memcpy(re, xr, sizeof(double) * nx);
+  memcpy(im, xi, sizeof(double) * nx);
+  return st;
+}
+static int adj_hsolve(const orc* o, double dt, double c2, double w2, double* re, double* im, double* w) {
+  int nx = o->nx;
+  double *a = w, *b = w + nx, *c = w + 2 * nx, *xr = w + 3 * nx, *xi = w + 4 * nx;
+  for (int i = 0; i < nx; ++i) {
+    a[i] = -dt * o->vis * c2 / o->dx2;
+    b[i] = 1.0 + dt * o->vis * c2 * (2.0 / o->dx2 + w2);
+    c[i] = a[i];
+  }
+  int st = sm_solve(a, b, c, re, im, xr, xi, nx);
+  memcpy(re, xr, sizeof(double) * nx);
+  memcpy(im, xi, sizeof(double) * nx);
+  return st;
+}
+
+static int adj_velstep_mode(orc* o, orc_adj* A, int j, int k) {
+  int nx = o->nx;
+  double dx = o->dx, dx2 = o->dx2, dt = o->s.dt, vis = o->vis;
+  double w2 = o->w2[(size_t)(k - 1) * o->my2 + (j - 1)], wy = o->wy[j - 1], wz = o->wz[k - 1];
+  double c1 = o->coef[0][0], c2 = o->coef[0][1], c3 = o->coef[0][2], c4 = o->coef[0][3];
+  double c12 = c1 + c2, alp = c12 * dt, cv = c2 * dt * vis;
+  int zero = (j == 1 && k == 1);
+  cpx* L = (cpx*)malloc(sizeof(cpx) * (size_t)nx * 8);
+  double* R = (double*)malloc(sizeof(double) * (size_t)nx * 7);
+  cpx *ub[3] = {L, L + nx, L + 2 * nx}, *pt = L + 3 * nx, *ph = L + 4 * nx, *s = L + 5 * nx, *q = L + 6 * nx;
+  double *re = R, *im = R + nx, *w = R + 2 * nx;
+  for (int d = 0; d < 3; ++d)
+    for (int i = 0; i < nx; ++i) ub[d][i] = A->au[IU(o, i + 1, j, k, d + 1)];
+  /* divcorr^T: p = p~ + phi - cv (L(phi)/dx2 - w2 phi), u = u* - alp G(phi) */
+  for (int i = 0; i < nx; ++i) {
+    cpx pb = A->ap[IP(o, i + 1, j, k)];
+    pt[i] = pb;
+    cpx lap = cadd(csub(A->ap[IP(o, wrp(i - 1, nx) + 1, j, k)], cscal(2.0, pb)), A->ap[IP(o, wrp(i + 1, nx) + 1, j, k)]);
+    ph[i] = csub(pb, cscal(cv, csub(cdivr(lap, dx2), cscal(w2, pb))));
+  }
+  for (int i = 0; i < nx; ++i) {
+    int ip = wrp(i + 1, nx);
+    cpx g = cdivr(csub(ub[0][i], ub[0][ip]), dx);
+    g = cadd(g, cscal(0.5, cik(-wy, cadd(ub[1][i], ub[1][ip]))));
+    g = cadd(g, cscal(0.5, cik(-wz, cadd(ub[2][i], ub[2][ip]))));
+    ph[i] = csub(ph[i], cscal(alp, g));
+  }
+  for (int i = 0; i < nx; ++i) { re[i] = ph[i].re; im[i] = ph[i].im; }
+  int st = adj_msolve(o, w2, zero, re, im, w);
+  for (int i = 0; i < nx; ++i) s[i] = cdivr(C(re[i], im[i]), alp);
+  /* u*_bar = u_bar + Dstag^T(s) */
+  for (int i = 0; i < nx; ++i) {
+    int im1 = wrp(i - 1, nx);
+    ub[0][i] = cadd(ub[0][i], cdivr(csub(s[im1], s[i]), dx));
+    ub[1][i] = cadd(ub[1][i], cscal(0.5, cik(-wy, cadd(s[im1], s[i]))));
+    ub[2][i] = cadd(ub[2][i], cscal(0.5, cik(-wz, cadd(s[im1], s[i]))));
+  }
+  /* ustep^T */
+  for (int d = 0; d < 3; ++d) {
+    for (int i = 0; i < nx; ++i) { re[i] = ub[d][i].re; im[i] = ub[d][i].im; }
+    int s2 = adj_hsolve(o, dt, c2, w2, re, im, w);
+    if (s2) st = s2;
+    for (int i = 0; i < nx; ++i) q[i] = C(re[i], im[i]);
+    for (int i = 0; i < nx; ++i) {
+      cpx lap = cadd(csub(q[wrp(i - 1, nx)], cscal(2.0, q[i])), q[wrp(i + 1, nx)]);
+      cpx v = cadd(cscal(1.0 - dt * vis * c1 * w2, q[i]), cdivr(cscal(dt * vis * c1, lap), dx2));
+      A->aunew[IU(o, i + 1, j, k, d + 1)] = v;
+      size_t ix = IU(o, i + 1, j, k, d + 1);
+      A->anl[ix] = cadd(A->anl[ix], cscal(dt * c3, q[i]));
+      A->aff[ix] = cscal(dt, q[i]);
+      A->anlp[ix] = cscal(dt * c4, q[i]);
+    }
+    /* p~_bar -= dt c12 G_d^T(q) */
+    for (int i = 0; i < nx; ++i) {
+      int ip = wrp(i + 1, nx);
+      cpx g;
+      if (d == 0) g = cdivr(csub(q[i], q[ip]), dx);
+      else g = cscal(0.5, cik(d == 1 ? -wy : -wz, cadd(q[i], q[ip])));
+      pt[i] = csub(pt[i], cscal(dt * c12, g));
+    }
+  }
+  /* pressure^T: p~ = M^-1 [c3 Dstag(nl) + Dstag(ff) + c4 Dstag(onl)]/c12 */
+  for (int i = 0; i < nx; ++i) { re[i] = pt[i].re; im[i] = pt[i].im; }
+  int s3 = adj_msolve(o, w2, zero, re, im, w);
+  if (s3) st = s3;
+  for (int i = 0; i < nx; ++i) s[i] = cdivr(C(re[i], im[i]), c12);
+  for (int i = 0; i < nx; ++i) {
+    int im1 = wrp(i - 1, nx);
+    cpx g[3];
+    g[0] = cdivr(csub(s[im1], s[i]), dx);
+    g[1] = cscal(0.5, cik(-wy, cadd(s[im1], s[i])));
+    g[2] = cscal(0.5, cik(-wz, cadd(s[im1], s[i])));
+    for (int d = 0; d < 3; ++d) {
+      size_t ix = IU(o, i + 1, j, k, d + 1);
+      A->anl[ix] = cadd(A->anl[ix], cscal(c3, g[d]));
+      A->aff[ix] = cadd(A->aff[ix], g[d]);
+      A->anlp[ix] = cadd(A->anlp[ix], cscal(c4, g[d]));
+    }
+  }
+  free(L);
+  free(R);
+  return st;
+}
+
+/* ---- IBM pieces, transposed ------------------------------------------ */
+/* gradient of the trilinear interpolant sum w f with respect to the position */
+static void interp_grad(const stencil* st, const double pos[3], const double v[8], double g[3]) {
+  double z[3] = {pos[0] - st->ib, pos[1] - st->jb, pos[2] - st->kb};
+  double xv[2] = {1.0 - z[0], z[0]}, yv[2] = {1.0 - z[1], z[1]}, zv[2] = {1.0 - z[2], z[2]};
+  g[0] = g[1] = g[2] = 0.0;
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i) {
+        double f = v[(k * 2 + j) * 2 + i];
+        g[0] += (i ? 1.0 : -1.0) * yv[j] * zv[k] * f;
+        g[1] += xv[i] * (j ? 1.0 : -1.0) * zv[k] * f;
+        g[2] += xv[i] * yv[j] * (k ? 1.0 : -1.0) * f;
+      }
+}
+static void sten_at(const orc* o, const double pos[3], stencil* st, int idx[8]) {
+  st->ib = cellof(pos[0]);
+  st->jb = cellof(pos[1]);
+  st->kb = cellof(pos[2]);
+  make_weights(st, pos);
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i)
+        idx[(k * 2 + j) * 2 + i] =
+            (int)((((size_t)(wrapz(o, st->kb + k) - 1)) * o->my + (wrapy(o, st->jb + j) - 1)));
+}
+/* node (i,j,k) of plane ii in an IR-layout (nx+1 planes) or IPR (nx planes) field */
+#define NODE_R(o, f, ld, q, ii, n) (f)[(((size_t)((n)-1) * (o)->mz * (o)->my) + (size_t)(q)) * (ld) + ((ii)-1)]
+/* out = sum_q w_q f(node q), grad: d/dpos; and the scatter f(node q) += w_q s */
+static double sten_get(const orc* o, const stencil* st, const int idx[8], const double* f, int ld, int n,
+                       const double pos[3], double grad[3], double gscale) {
+  double v[8], acc = 0.0;
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i) {
+        int q = (k * 2 + j) * 2 + i;
+        v[q] = NODE_R(o, f, ld, idx[q], wrapx(o, st->ib + i), n);
+        acc += st->w[q] * v[q];
+      }
+  if (grad) {
+    double g[3];
+    interp_grad(st, pos, v, g);
+    for (int d = 0; d < 3; ++d) grad[d] += gscale * g[d];
+  }
+  return acc;
+}
+static void sten_put(const orc* o, const stencil* st, const int idx[8], double* f, int ld, int n, double s) {
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i) {
+        int q = (k * 2 + j) * 2 + i;
+        NODE_R(o, f, ld, idx[q], wrapx(o, st->ib + i), n) += st->w[q] * s;
+      }
+}
+
+/* calc_inner_reversal_forcing, transposed (one body) */
+static void adj_forcing_body(orc* o, orc_adj* A, int m, const double sbar[3]) {
+  pib_scalars* s = &o->s;
+  int ld = o->nx + 1, ldp = o->nx;
+  double rb = o->radbdy[m - 1], ri = o->radibdy[m - 1], ro = o->radobdy[m - 1];
+  double c1 = o->coef[0][0], c2 = o->coef[0][1], c3 = o->coef[0][2], c4 = o->coef[0][3], c12 = c1 + c2;
+  double rdx = ri * o->dx;
+  double K = (4.0 * PI * (rdx * rdx) / (double)o->nrpr_active[m - 1]) * o->cfg.dr * o->dx;
+  double kap = (rb - ri) / (ro - rb), rn = o->cfg.revernorm == 1 ? 1.0 : 0.0;
+  double* gx = &A->axc[m - 1];
+  for (int l = 1; l <= o->nbnd; ++l) {
+    if (!o->if_rev[(size_t)(m - 1) * o->nbnd + (l - 1)]) continue;
+    double xli[3], xlo[3], rad = 0.0;
+    for (int n = 1; n <= 3; ++n) {
+      xli[n - 1] = XC(o, m, n) + BND(o, n, l) * ri;
+      xlo[n - 1] = XC(o, m, n) + BND(o, n, l) * ro;
+      rad += (BND(o, n, l) * ri) * (BND(o, n, l) * ri);
+    }
+    rad = sqrt(rad);
+    double nv[3];
+    for (int n = 0; n < 3; ++n) nv[n] = (BND(o, n + 1, l) * ri) / rad;
+    double xpi[3] = {xli[0] - 0.5, xli[1], xli[2]};
+    stencil si, so, sp;
+    int ii[8], io[8], ip[8];
+    sten_at(o, xli, &si, ii);
+    sten_at(o, xlo, &so, io);
+    sten_at(o, xpi, &sp, ip);
+    /* forward ft of this point (the same terms as the forward step) */
+    double ulo[3], ft[3];
+    for (int n = 0; n < 3; ++n) ulo[n] = sten_get(o, &so, io, o->ubcp, ld, n + 1, xlo, NULL, 0);
+    double un = nv[0] * ulo[0] + nv[1] * ulo[1] + nv[2] * ulo[2];
+    for (int d = 0; d < 3; ++d) {
+      double unorm = nv[d] * un, utang = ulo[d] - unorm;
+      double ucar = -kap * utang - rn * kap * unorm; /* bed at rest: linvel = 0 */
+      double uli = sten_get(o, &si, ii, o->ubcp, ld, d + 1, xli, NULL, 0);
+      double nll = sten_get(o, &si, ii, o->nlbc, ld, d + 1, xli, NULL, 0);
+      double onll = sten_get(o, &si, ii, o->onlbc, ld, d + 1, xli, NULL, 0);
+      double dfll = sten_get(o, &si, ii, o->diffn, ld, d + 1, xli, NULL, 0);
+      double ppll = sten_get(o, &sp, ip, o->ppr, ldp, d + 1, xpi, NULL, 0);
+      ft[d] = (s->cf * (uli - ucar) - c3 * nll - c4 * onll + c12 * (ppll - dfll)) * K;
+    }
+    /* spreading, transposed */
+    int solid[8], count_so = 0;
+    double wfl = 0.0;
+    for (int k = 0; k < 2; ++k)
+      for (int j = 0; j < 2; ++j)
+        for (int i = 0; i < 2; ++i) {
+          int q = (k * 2 + j) * 2 + i;
+          int fl = o->fluid[IFL(o, wrapx(o, si.ib + i), wrapy(o, si.jb + j), wrapz(o, si.kb + k))];
+          solid[q] = !fl;
+          if (fl) wfl += si.w[q] / o->dx3;
+          else count_so++;
+        }
+    double ftb[3], gq[8], Ad[3];
+    for (int d = 0; d < 3; ++d) {
+      double ssum = 0.0, direct = 0.0, wall = 0.0;
+      for (int q = 0; q < 8; ++q) {
+        int i = q & 1, j = (q >> 1) & 1, k = q >> 2;
+        double fb = NODE_R(o, A->afr, ld, (((size_t)(wrapz(o, si.kb + k) - 1)) * o->my + (wrapy(o, si.jb + j) - 1)),
+                           wrapx(o, si.ib + i), d + 1);
+        if (solid[q]) { ssum += fb; direct += si.w[q] * fb / o->dx3; }
+        wall += si.w[q] / o->dx3;
+      }
+      Ad[d] = count_so > 0 ? ssum / (double)count_so : 0.0;
+      ftb[d] = direct + (count_so > 0 ? wfl * Ad[d] : 0.0) + sbar[d] * wall;
+    }
+    for (int q = 0; q < 8; ++q) {
+      int i = q & 1, j = (q >> 1) & 1, k = q >> 2;
+      gq[q] = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        double fb = NODE_R(o, A->afr, ld, (((size_t)(wrapz(o, si.kb + k) - 1)) * o->my + (wrapy(o, si.jb + j) - 1)),
+                           wrapx(o, si.ib + i), d + 1);
+        gq[q] += ft[d] / o->dx3 * ((solid[q] ? fb : Ad[d]) + sbar[d]);
+      }
+    }
+    double gpos[3];
+    interp_grad(&si, xli, gq, gpos);
+    for (int d = 0; d < 3; ++d) gx[(size_t)d * o->nbody] += gpos[d];
+    /* ft, transposed onto the interpolated fields */
+    double h[3], Ph[3];
+    for (int d = 0; d < 3; ++d) h[d] = K * ftb[d];
+    double nh = nv[0] * h[0] + nv[1] * h[1] + nv[2] * h[2];
+    for (int d = 0; d < 3; ++d) Ph[d] = (h[d] - nv[d] * nh) + rn * nv[d] * nh;
+    double gi[3] = {0, 0, 0}, go[3] = {0, 0, 0}, gp[3] = {0, 0, 0};
+    for (int d = 0; d < 3; ++d) {
+      sten_get(o, &si, ii, o->ubcp, ld, d + 1, xli, gi, s->cf * h[d]);
+      sten_get(o, &si, ii, o->nlbc, ld, d + 1, xli, gi, -c3 * h[d]);
+      sten_get(o, &si, ii, o->onlbc, ld, d + 1, xli, gi, -c4 * h[d]);
+      sten_get(o, &si, ii, o->diffn, ld, d + 1, xli, gi, -c12 * h[d]);
+      sten_get(o, &so, io, o->ubcp, ld, d + 1, xlo, go, s->cf * kap * Ph[d]);
+      sten_get(o, &sp, ip, o->ppr, ldp, d + 1, xpi, gp, c12 * h[d]);
+      sten_put(o, &si, ii, A->aubcp, ld, d + 1, s->cf * h[d]);
+      sten_put(o, &si, ii, A->anlbc, ld, d + 1, -c3 * h[d]);
+      sten_put(o, &si, ii, A->aonlbc, ld, d + 1, -c4 * h[d]);
+      sten_put(o, &si, ii, A->adiffn, ld, d + 1, -c12 * h[d]);
+      sten_put(o, &so, io, A->aubcp, ld, d + 1, s->cf * kap * Ph[d]);
+      sten_put(o, &sp, ip, A->appr, ldp, d + 1, c12 * h[d]);
+    }
+    for (int d = 0; d < 3; ++d) gx[(size_t)d * o->nbody] += gi[d] + go[d] + gp[d];
+  }
+}
+
+/* compute_mpg_at_n's force(m) = c12 (pres + visc) - c12 mpg pi vol/6, transposed */
+static void adj_drag_body(orc* o, orc_adj* A, int m, const double* wf) {
+  int ld = o->nx + 1, ldp = o->nx;
+  double c12 = o->coef[0][0] + o->coef[0][1];
+  double rb = o->radbdy[m - 1], rdx = rb * o->dx;
+  double da1 = 4.0 * PI * (rdx * rdx) / (double)o->nbnd;
+  double w[3] = {wf[m - 1], wf[o->nbody + m - 1], wf[2 * o->nbody + m - 1]};
+  double vol = pow(2.0 * rb * o->dx, 3.0);
+  for (int d = 0; d < 3; ++d) A->ampg[d] -= c12 * w[d] * PI * vol / 6.0;
+  double* gx = &A->axc[m - 1];
+  for (int l = 1; l <= o->nbnd; ++l) {
+    double xl[3], xpb[3];
+    for (int n = 1; n <= 3; ++n) xl[n - 1] = XC(o, m, n) + BND(o, n, l) * rb;
+    xpb[0] = xl[0] - 0.5; xpb[1] = xl[1]; xpb[2] = xl[2];
+    double cd[3] = {CD(o, 1, l), CD(o, 2, l), CD(o, 3, l)};
+    stencil sp, sv;
+    int ip[8], iv[8];
+    sten_at(o, xpb, &sp, ip);
+    sten_at(o, xl, &sv, iv);
+    double plb = -c12 * da1 * (w[0] * cd[0] + w[1] * cd[1] + w[2] * cd[2]);
+    double cw[3] = {cd[1] * w[2] - cd[2] * w[1], cd[2] * w[0] - cd[0] * w[2], cd[0] * w[1] - cd[1] * w[0]};
+    double g[3] = {0, 0, 0};
+    /* pbcp is (nx, my, mz): one component */
+    sten_get(o, &sp, ip, o->pbcp, ldp, 1, xpb, g, plb);
+    sten_put(o, &sp, ip, A->apbcp, ldp, 1, plb);
+    for (int c = 0; c < 3; ++c) {
+      double vb = c12 * o->vis * da1 * cw[c];
+      sten_get(o, &sv, iv, o->omega_copy, ld, c + 1, xl, g, vb);
+      sten_put(o, &sv, iv, A->aom, ld, c + 1, vb);
+    }
+    for (int d = 0; d < 3; ++d) gx[(size_t)d * o->nbody] += g[d];
+  }
+}
+
+int orc_adjoint_step(void* ctx, const double* wforce) {
+  orc* o = (orc*)ctx;
+  if (!o) return PIB_ERR_ARG;
+  orc_adj* A = (orc_adj*)o->adj;
+  if (!A) return fail(o, PIB_ERR_STATE, "pib_adjoint_begin first");
+  if (A->nbody != o->nbody) return fail(o, PIB_ERR_STATE, "particle set changed since pib_adjoint_begin");
+  const pib_config* c = &o->cfg;
+  if (c->rk3 || c->adaptive || !c->set_mpg || c->discr_scheme != PIB_SCHEME_CENTER || !c->dorpr || c->dobnd ||
+      c->include_frmeanfluid || c->move_particles)
+    return fail(o, PIB_ERR_UNSUPPORTED, "adjoint: rk3=0, adaptive=0, set_mpg, center, dorpr, fixed bed only");
+  int nx = o->nx, my = o->my, mz = o->mz, my2 = o->my2;
+  size_t nc = (size_t)(nx + 1) * my2 * mz * 3, np = (size_t)nx * my2 * mz;
+  size_t nr = (size_t)(nx + 1) * my * mz * 3, npr = (size_t)nx * my * mz;
+  int st;
+  /* the step's forward fields */
+  if ((st = orc_compute_new_timestep(o, 1))) return st;
+  if ((st = nonlinear_part(o, 1, 0))) return st;
+  double dt = o->s.dt;
+  memset(A->aunew, 0, nc * sizeof(cpx));
+  memset(A->apnew, 0, np * sizeof(cpx));
+  memset(A->anlp, 0, nc * sizeof(cpx));
+  memset(A->aff, 0, nc * sizeof(cpx));
+  memset(A->aubcp, 0, nr * sizeof(double));
+  memset(A->anlbc, 0, nr * sizeof(double));
+  memset(A->aonlbc, 0, nr * sizeof(double));
+  memset(A->adiffn, 0, nr * sizeof(double));
+  memset(A->aom, 0, nr * sizeof(double));
+  memset(A->afr, 0, nr * sizeof(double));
+  memset(A->apbcp, 0, npr * sizeof(double));
+  memset(A->appr, 0, 3 * npr * sizeof(double));
+  /* velstep */
+  st = 0;
+#pragma omp parallel for schedule(static) collapse(2) reduction(min : st)
+  for (int j = 1; j <= my2; ++j)
+    for (int k = 1; k <= mz; ++k) {
+      int s2 = adj_velstep_mode(o, A, j, k);
+      if (s2 < st) st = s2;
+    }
+  if (st) return fail(o, st, "zero pivot in the transposed line solves");
+  /* umean += (-mpg + frmean) dt */
+  double frmb[3];
+  for (int n = 0; n < 3; ++n) {
+    frmb[n] = A->aumean[n] * dt;
+    A->ampg[n] -= A->aumean[n] * dt;
+  }
+  /* ff = ff2rc(fr - frmean) */
+  cpx* tc = (cpx*)xcalloc((size_t)my2 * mz, sizeof(cpx));
+  cpx* tc2 = (cpx*)xcalloc((size_t)my2 * mz, sizeof(cpx));
+  double* tr = (double*)xcalloc((size_t)my * mz, sizeof(double));
+  for (int n = 1; n <= 3; ++n)
+    for (int i = 1; i <= nx; ++i) {
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my2; ++j) tc2[(size_t)(k - 1) * my2 + (j - 1)] = A->aff[IU(o, i, j, k, n)];
+      adj_r2c_T(o, tc2, tc, tr);
+      put_rplane(o, A->afr + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, tr);
+      for (size_t q = 0; q < (size_t)my * mz; ++q) frmb[n - 1] -= tr[q];
+    }
+  /* frmean = (sum of the weighted shares)/(nx my mz) */
+  double sbar[3];
+  for (int n = 0; n < 3; ++n) sbar[n] = frmb[n] / (double)((long long)nx * my * mz);
+  for (int m = 1; m <= o->nbody; ++m) adj_forcing_body(o, A, m, sbar);
+  if (wforce)
+    for (int m = 1; m <= o->nbody; ++m) adj_drag_body(o, A, m, wforce);
+  /* calc_pgrad: ppr = c2r(G' p) + mpg */
+  {
+    cpx* Z = (cpx*)xcalloc(3 * np, sizeof(cpx));
+    for (int n = 1; n <= 3; ++n)
+      for (int i = 1; i <= nx; ++i) {
+        get_rplane(o, A->appr + (size_t)(n - 1) * nx * my * mz, nx, i, tr);
+        for (size_t q = 0; q < (size_t)my * mz; ++q) A->ampg[n - 1] += tr[q];
+        adj_c2r_T(o, tr, tc, Z + ((size_t)(n - 1) * nx + (i - 1)) * my2 * mz, 0);
+      }
+#define ZZ(n, i, j, k) Z[(((size_t)((n)-1) * nx + ((i)-1)) * mz + ((k)-1)) * my2 + ((j)-1)]
+    for (int i = 1; i <= nx; ++i) {
+      int im = wrp(i - 2, nx) + 1, ip = wrp(i, nx) + 1;
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my2; ++j) {
+          cpx v = cdivr(csub(ZZ(1, im, j, k), ZZ(1, ip, j, k)), 2.0 * o->dx);
+          v = cadd(v, cik(-o->wy[j - 1], ZZ(2, i, j, k)));
+          v = cadd(v, cik(-o->wz[k - 1], ZZ(3, i, j, k)));
+          A->apnew[IP(o, i, j, k)] = cadd(A->apnew[IP(o, i, j, k)], v);
+        }
+    }
+    /* compute_omega */
+    for (int n = 1; n <= 3; ++n)
+      for (int i = 1; i <= nx; ++i) {
+        get_rplane(o, A->aom + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, tr);
+        adj_c2r_T(o, tr, tc, Z + ((size_t)(n - 1) * nx + (i - 1)) * my2 * mz, 0);
+      }
+    for (int i = 1; i <= nx; ++i)
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my2; ++j) {
+          double wy = o->wy[j - 1], wz = o->wz[k - 1];
+          size_t i1 = IU(o, i, j, k, 1), i2 = IU(o, i, j, k, 2), i3 = IU(o, i, j, k, 3);
+          A->aunew[i1] = cadd(A->aunew[i1], cadd(cik(wy, ZZ(3, i, j, k)), cik(-wz, ZZ(2, i, j, k))));
+          A->aunew[i3] = cadd(A->aunew[i3], cik(-wy, ZZ(1, i, j, k)));
+          A->aunew[i2] = cadd(A->aunew[i2], cik(wz, ZZ(1, i, j, k)));
+        }
+    for (int i = 1; i <= nx; ++i) {
+      int im = wrp(i - 2, nx) + 1, ip = wrp(i, nx) + 1;
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my; ++j) {
+          A->aubcp[IR(o, i, j, k, 2)] += (A->aom[IR(o, im, j, k, 3)] - A->aom[IR(o, ip, j, k, 3)]) / (2.0 * o->dx);
+          A->aubcp[IR(o, i, j, k, 3)] -= (A->aom[IR(o, im, j, k, 2)] - A->aom[IR(o, ip, j, k, 2)]) / (2.0 * o->dx);
+        }
+    }
+    /* calc_visc */
+    for (int n = 1; n <= 3; ++n)
+      for (int i = 1; i <= nx; ++i) {
+        get_rplane(o, A->adiffn + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, tr);
+        adj_c2r_T(o, tr, tc, Z + ((size_t)(n - 1) * nx + (i - 1)) * my2 * mz, 0);
+      }
+    for (int n = 1; n <= 3; ++n)
+      for (int i = 1; i <= nx; ++i) {
+        int im = wrp(i - 2, nx) + 1, ip = wrp(i, nx) + 1;
+        for (int k = 1; k <= mz; ++k)
+          for (int j = 1; j <= my2; ++j) {
+            cpx lap = cadd(csub(ZZ(n, im, j, k), cscal(2.0, ZZ(n, i, j, k))), ZZ(n, ip, j, k));
+            cpx v = cscal(o->vis, csub(cdivr(lap, o->dx2), cscal(o->w2[(size_t)(k - 1) * my2 + (j - 1)], ZZ(n, i, j, k))));
+            size_t ix = IU(o, i, j, k, n);
+            A->aunew[ix] = cadd(A->aunew[ix], v);
+          }
+      }
+#undef ZZ
+    free(Z);
+  }
+  /* nlbc = c2r(nl), onlbc = c2r(onl) */
+  for (int n = 1; n <= 3; ++n)
+    for (int i = 1; i <= nx; ++i) {
+      get_rplane(o, A->anlbc + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, tr);
+      adj_c2r_T(o, tr, tc2, tc, 0);
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my2; ++j) {
+          size_t ix = IU(o, i, j, k, n);
+          A->anl[ix] = cadd(A->anl[ix], tc[(size_t)(k - 1) * my2 + (j - 1)]);
+        }
+      get_rplane(o, A->aonlbc + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, tr);
+      adj_c2r_T(o, tr, tc2, tc, 0);
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my2; ++j) {
+          size_t ix = IU(o, i, j, k, n);
+          A->anlp[ix] = cadd(A->anlp[ix], tc[(size_t)(k - 1) * my2 + (j - 1)]);
+        }
+    }
+  /* form_nl (central) and the sign of nonlinear: nl = -[(i wy) F(u_c v) + (i wz) F(u_c w) + F(dFx_c)] */
+  {
+    double* Q = (double*)xcalloc(8 * npr, sizeof(double)); /* Q12 Q13 Q22 Q23 Q33 E1 E2 E3 */
+#define QQ(t, i, j, k) Q[(((size_t)(t) * nx + ((i)-1)) * mz + ((k)-1)) * my + ((j)-1)]
+    for (int i = 1; i <= nx; ++i) {
+      for (int t = 0; t < 8; ++t) {
+        for (int k = 1; k <= mz; ++k)
+          for (int j = 1; j <= my2; ++j) {
+            double wy = o->wy[j - 1], wz = o->wz[k - 1];
+            cpx g1 = cscal(-1.0, A->anl[IU(o, i, j, k, 1)]), g2 = cscal(-1.0, A->anl[IU(o, i, j, k, 2)]),
+                g3 = cscal(-1.0, A->anl[IU(o, i, j, k, 3)]);
+            cpx v;
+            switch (t) {
+              case 0: v = cik(-wy, g1); break;
+              case 1: v = cik(-wz, g1); break;
+              case 2: v = cik(-wy, g2); break;
+              case 3: v = cadd(cik(-wz, g2), cik(-wy, g3)); break;
+              case 4: v = cik(-wz, g3); break;
+              case 5: v = g1; break;
+              case 6: v = g2; break;
+              default: v = g3;
+            }
+            tc2[(size_t)(k - 1) * my2 + (j - 1)] = v;
+          }
+        adj_r2c_T(o, tc2, tc, &QQ(t, i, 1, 1));
+      }
+    }
+    for (int i = 1; i <= nx; ++i) {
+      int im = wrp(i - 2, nx) + 1, ip = wrp(i, nx) + 1;
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my; ++j) {
+          double u1 = o->ubcp[IR(o, i, j, k, 1)], u2 = o->ubcp[IR(o, i, j, k, 2)], u3 = o->ubcp[IR(o, i, j, k, 3)];
+          double F1 = (QQ(5, im, j, k) - QQ(5, ip, j, k)) / (2.0 * o->dx);
+          double F2 = (QQ(6, im, j, k) - QQ(6, ip, j, k)) / (2.0 * o->dx);
+          double F3 = (QQ(7, im, j, k) - QQ(7, ip, j, k)) / (2.0 * o->dx);
+          double q12 = QQ(0, i, j, k), q13 = QQ(1, i, j, k), q22 = QQ(2, i, j, k), q23 = QQ(3, i, j, k),
+                 q33 = QQ(4, i, j, k);
+          A->aubcp[IR(o, i, j, k, 1)] += q12 * u2 + q13 * u3 + 2.0 * F1 * u1 + F2 * u2 + F3 * u3;
+          A->aubcp[IR(o, i, j, k, 2)] += q12 * u1 + 2.0 * q22 * u2 + q23 * u3 + F2 * u1;
+          A->aubcp[IR(o, i, j, k, 3)] += q13 * u1 + q23 * u2 + 2.0 * q33 * u3 + F3 * u1;
+        }
+    }
+#undef QQ
+    free(Q);
+  }
+  /* ubcp = c2r(u) + umean, pbcp = c2r(p) */
+  for (int n = 1; n <= 3; ++n)
+    for (int i = 1; i <= nx; ++i) {
+      get_rplane(o, A->aubcp + (size_t)(n - 1) * (nx + 1) * my * mz, nx + 1, i, tr);
+      for (size_t q = 0; q < (size_t)my * mz; ++q) A->aumean[n - 1] += tr[q];
+      adj_c2r_T(o, tr, tc2, tc, 0);
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my2; ++j) {
+          size_t ix = IU(o, i, j, k, n);
+          A->aunew[ix] = cadd(A->aunew[ix], tc[(size_t)(k - 1) * my2 + (j - 1)]);
+        }
+    }
+  for (int i = 1; i <= nx; ++i) {
+    get_rplane(o, A->apbcp, nx, i, tr);
+    adj_c2r_T(o, tr, tc2, tc, 0);
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my2; ++j) {
+        size_t ix = IP(o, i, j, k);
+        A->apnew[ix] = cadd(A->apnew[ix], tc[(size_t)(k - 1) * my2 + (j - 1)]);
+      }
+  }
+  free(tc); free(tc2); free(tr);
+  /* roll: the adjoint of this step's inputs */
+  cpx* t;
+  t = A->au; A->au = A->aunew; A->aunew = t;
+  t = A->ap; A->ap = A->apnew; A->apnew = t;
+  t = A->anl; A->anl = A->anlp; A->anlp = t;
   return PIB_OK;
 }

@@ -39,6 +39,13 @@ int orc_pressure_modes(void* ctx, int rks);
 void orc_set_threads(int n);
 int orc_get_threads(void);
 int orc_step(void* ctx, int nsteps);
+/* adjoint of the fixed-bed step and the checkpoint stack (pib_adjoint_*) */
+int orc_adjoint_begin(void* ctx);
+int orc_adjoint_step(void* ctx, const double* wforce);
+int orc_adjoint_get(void* ctx, double* dxc, double* dmpg, double* dumean);
+int orc_adjoint_set_umean(void* ctx, const double* dumean);
+int orc_checkpoint_push(void* ctx);
+int orc_checkpoint_pop(void* ctx);
 
 /* transform layer: one plane each call, a(my,mz) real col-major, b(my2,mz) complex */
 void orc_ff2rc(int my, int mz, const double* a, double* b);

@@ -42,8 +42,9 @@ FIELDS = {
     "u": 0, "p": 1, "nl": 2, "onl": 3, "ff": 4, "ubcp": 5, "pbcp": 6,
     "nlbcp": 7, "onlbcp": 8, "diffn": 9, "ppr": 10, "fr": 11, "omega": 12,
     "mask": 13, "phi": 14, "nlphi": 15, "onlphi": 16, "ffphi": 17,
+    "au": 18, "ap": 19, "anl": 20,
 }
-COMPLEX_FIELDS = {"u", "p", "nl", "onl", "ff", "phi", "nlphi", "onlphi", "ffphi"}
+COMPLEX_FIELDS = {"u", "p", "nl", "onl", "ff", "phi", "nlphi", "onlphi", "ffphi", "au", "ap", "anl"}
 
 
 class PibConfig(ctypes.Structure):
@@ -132,6 +133,12 @@ _STEP_SIGS = {
     "scalar_set": ([_VP, ctypes.c_double, ctypes.c_double, ctypes.c_double], _I),
     "scalar_get": ([_VP, _PD], _I),
     "scalstep_modes": ([_VP, _I], _I),
+    "adjoint_begin": ([_VP], _I),
+    "adjoint_step": ([_VP, _PD], _I),
+    "adjoint_get": ([_VP, _PD, _PD, _PD], _I),
+    "adjoint_set_umean": ([_VP, _PD], _I),
+    "checkpoint_push": ([_VP], _I),
+    "checkpoint_pop": ([_VP], _I),
 }
 
 _GPU_ONLY_SIGS = {

@@ -49,6 +49,7 @@ def build_gpu():
     out = os.path.join(BUILD, "libpureibm_gpu.so")
     _run([hipcc()] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"),
                                   os.path.join(CSRC, "pib_gpu.hip"), os.path.join(CSRC, "pib_dem.hip"),
+                                  os.path.join(CSRC, "pib_adjoint.hip"),
                                   "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out

@@ -180,7 +180,9 @@ def c3_realization(directory, k):
 WORKLOADS = {
     "C1": dict(kind="simple", my=64, dbydx=16, re=20.0, adaptive=False, ramp_frac_steps=0),
     "C2": dict(kind="random", my=128, dbydx=16, nbody=195, seed=20261202, re=50.0),
-    "C4": dict(kind="random", my=128, dbydx=16, nbody=50, seed=20261204, re=20.0),
+    # the adjoint workload: SET_MPG with a fixed dt, so that J(xc, mpg) is smooth
+    "C4": dict(kind="random", my=128, dbydx=16, nbody=50, seed=20261204, re=20.0, set_mpg=True,
+               adaptive=False),
 }
 
 

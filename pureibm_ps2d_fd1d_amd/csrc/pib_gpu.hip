@@ -16,6 +16,7 @@
 #include "pib_kernels.h"
 #include "pib_velstep.h"
 #include "pib_scalar.h"
+#include "pib_adjoint.h"
 
 namespace {
 
@@ -904,6 +905,8 @@ struct Impl : Base {
   ~Impl() override {
     if (stream) (void)hipStreamSynchronize(stream);
     drop_graphs();
+    for (auto& c : ck_stack) (void)hipFree(c.first);
+    for (void* q : ck_pool) (void)hipFree(q);
     for (auto& q : pallocs) (void)hipFree(q.first);
     drain_pool();
     for (void* q : allocs) (void)hipFree(q);
@@ -1719,6 +1722,9 @@ struct Impl : Base {
       case PIB_F_FF: put_spec(ff, 3, h, true); break;
       case PIB_F_UBCP: put_real(ubcp, 3, h, true); break;
       case PIB_F_PBCP: put_real(pbcp, 1, h, false); break;
+      case PIB_F_AU: need_adj(); put_spec(J.au, 3, h, true); break;
+      case PIB_F_AP: need_adj(); put_spec(J.ap, 1, h, false); break;
+      case PIB_F_ANL: need_adj(); put_spec(J.anl, 3, h, true); break;
       default: throw PibFail(PIB_ERR_ARG, "field cannot be set");
     }
   }
@@ -1737,6 +1743,9 @@ struct Impl : Base {
       case PIB_F_DIFFN: get_real(diffn, 3, h, true); break;
       case PIB_F_PPR: get_real(ppr, 3, h, false); break;
       case PIB_F_OMEGA: get_real(omega, 3, h, true); break;
+      case PIB_F_AU: need_adj(); get_spec(J.au, 3, h, true); break;
+      case PIB_F_AP: need_adj(); get_spec(J.ap, 1, h, false); break;
+      case PIB_F_ANL: need_adj(); get_spec(J.anl, 3, h, true); break;
       case PIB_F_FR: {
         DevState s;
         HIPCHK(hipMemcpy(&s, dst, sizeof s, hipMemcpyDeviceToHost));
@@ -1845,6 +1854,135 @@ struct Impl : Base {
       (void)hipFree(pallocs.back().first);
       pallocs.pop_back();
     }
+  }
+
+  // ---------------- adjoint of the fixed-bed step ----------------
+  AdjArgs<T> J{};
+  bool have_adj = false;
+  int adj_nbody = -1;
+  double* wdev = nullptr;
+  void adjoint_begin() {
+    if (!have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
+    const size_t E = (size_t)my2 * mz, nsc = (size_t)3 * nx * E, np = (size_t)nx * E;
+    const size_t npr = (size_t)nx * my * mz;
+    if (!have_adj) {
+      J.au = alloc<C2<T>>(nsc);
+      J.aunew = alloc<C2<T>>(nsc);
+      J.anl = alloc<C2<T>>(nsc);
+      J.anlp = alloc<C2<T>>(nsc);
+      J.aff = alloc<C2<T>>(nsc);
+      J.Z = alloc<C2<T>>(nsc);
+      J.ap = alloc<C2<T>>(np);
+      J.apnew = alloc<C2<T>>(np);
+      for (T** f : {&J.aubcp, &J.anlbc, &J.aonlbc, &J.adiffn, &J.aom, &J.afr, &J.appr}) *f = alloc<T>(3 * npr);
+      J.apbcp = alloc<T>(npr);
+      J.Q = alloc<T>(8 * npr);
+      J.sc = alloc<double>(12);
+      J.part = alloc<double>((size_t)3 * nx);
+      J.vs = alloc<double>((size_t)12 * nx * E);
+      have_adj = true;
+    }
+    if (adj_nbody != nbody) {
+      J.axc = alloc<double>((size_t)3 * std::max(nbody, 1));
+      wdev = alloc<double>((size_t)3 * std::max(nbody, 1));
+      adj_nbody = nbody;
+    }
+    HIPCHK(hipMemsetAsync(J.au, 0, nsc * sizeof(C2<T>), stream));
+    HIPCHK(hipMemsetAsync(J.anl, 0, nsc * sizeof(C2<T>), stream));
+    HIPCHK(hipMemsetAsync(J.ap, 0, np * sizeof(C2<T>), stream));
+    HIPCHK(hipMemsetAsync(J.sc, 0, 12 * sizeof(double), stream));
+    HIPCHK(hipMemsetAsync(J.axc, 0, (size_t)3 * nbody * sizeof(double), stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  void need_adj() const {
+    if (!have_adj) throw PibFail(PIB_ERR_STATE, "pib_adjoint_begin first");
+  }
+  void adjoint_step(const double* wforce) {
+    need_adj();
+    if (adj_nbody != nbody) throw PibFail(PIB_ERR_STATE, "particle set changed since pib_adjoint_begin");
+    const pib_config& c = cfg;
+    if (c.rk3 || c.adaptive || !c.set_mpg || c.discr_scheme != PIB_SCHEME_CENTER || !c.dorpr || c.dobnd ||
+        c.include_frmeanfluid || c.move_particles)
+      throw PibFail(PIB_ERR_UNSUPPORTED, "adjoint: rk3=0, adaptive=0, set_mpg, center, dorpr, fixed bed only");
+    // the step's forward fields
+    compute_new_timestep(1);
+    nonlinear_timed(1, false, 0);
+    J.wforce = nullptr;
+    if (wforce && nbody > 0) {
+      HIPCHK(hipMemcpyAsync(wdev, wforce, (size_t)3 * nbody * sizeof(double), hipMemcpyHostToDevice, stream));
+      J.wforce = wdev;
+    }
+    AdjGeom g{cur_sh(), nt_plane, plane_lds(), nbody, nbnd, stream};
+    adjoint_step_kernels<T>(args(1), parts(), J, g);
+    std::swap(J.au, J.aunew);
+    std::swap(J.ap, J.apnew);
+    std::swap(J.anl, J.anlp);
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  void adjoint_get(double* dxc, double* dmpg, double* dum) {
+    need_adj();
+    double h[12];
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(h, J.sc, sizeof h, hipMemcpyDeviceToHost));
+    if (dxc && nbody > 0) HIPCHK(hipMemcpy(dxc, J.axc, (size_t)3 * nbody * sizeof(double), hipMemcpyDeviceToHost));
+    for (int n = 0; n < 3; ++n) {
+      if (dmpg) dmpg[n] = h[3 + n];
+      if (dum) dum[n] = h[n];
+    }
+  }
+  void adjoint_set_umean(const double* dum) {
+    need_adj();
+    HIPCHK(hipMemcpy(J.sc, dum, 3 * sizeof(double), hipMemcpyHostToDevice));
+  }
+
+  // ---------------- checkpoint stack of the forward state ----------------
+  std::vector<std::pair<void*, bool>> ck_stack;  // device copy, nlbc_valid
+  std::vector<void*> ck_pool;
+  size_t part_bytes(int q) const {
+    const size_t E = (size_t)my2 * mz;
+    const size_t sz[5] = {(size_t)3 * nx * E * sizeof(C2<T>), (size_t)3 * nx * E * sizeof(C2<T>),
+                          (size_t)nx * E * sizeof(C2<T>), (size_t)3 * nx * my * mz * sizeof(T), sizeof(DevState)};
+    return sz[q];
+  }
+  // offsets of u, nl, p, nlbc and the device state in one checkpoint block
+  size_t ck_parts(size_t* off) const {
+    size_t o = 0;
+    for (int q = 0; q < 5; ++q) {
+      off[q] = o;
+      o += (part_bytes(q) + 255) / 256 * 256;
+    }
+    off[5] = o;
+    return o;
+  }
+  void checkpoint_push() {
+    size_t off[6];
+    const size_t bytes = ck_parts(off);
+    void* m = nullptr;
+    if (!ck_pool.empty()) {
+      m = ck_pool.back();
+      ck_pool.pop_back();
+    } else {
+      HIPCHK(hipMalloc(&m, bytes));
+    }
+    char* b = (char*)m;
+    const void* src[5] = {u, nl, p, nlbc, dst};
+    for (int q = 0; q < 5; ++q)
+      HIPCHK(hipMemcpyAsync(b + off[q], src[q], part_bytes(q), hipMemcpyDeviceToDevice, stream));
+    ck_stack.push_back({m, nlbc_valid});
+  }
+  void checkpoint_pop() {
+    if (ck_stack.empty()) throw PibFail(PIB_ERR_STATE, "checkpoint stack is empty");
+    size_t off[6];
+    ck_parts(off);
+    auto top = ck_stack.back();
+    ck_stack.pop_back();
+    const char* b = (const char*)top.first;
+    void* dstp[5] = {u, nl, p, nlbc, dst};
+    for (int q = 0; q < 5; ++q)
+      HIPCHK(hipMemcpyAsync(dstp[q], b + off[q], part_bytes(q), hipMemcpyDeviceToDevice, stream));
+    nlbc_valid = top.second;
+    ck_pool.push_back(top.first);
+    HIPCHK(hipStreamSynchronize(stream));
   }
 };
 
@@ -2041,6 +2179,26 @@ int pib_velstep(void* ctx, int rks) {
     I.velstep(rks);
   });
   return finish_status(ctx, st);
+}
+int pib_adjoint_begin(void* ctx) {
+  return dispatch(ctx, [&](auto& I) { I.adjoint_begin(); });
+}
+int pib_adjoint_step(void* ctx, const double* wforce) {
+  int st = dispatch(ctx, [&](auto& I) { I.adjoint_step(wforce); });
+  return finish_status(ctx, st);
+}
+int pib_adjoint_get(void* ctx, double* dxc, double* dmpg, double* dumean) {
+  return dispatch(ctx, [&](auto& I) { I.adjoint_get(dxc, dmpg, dumean); });
+}
+int pib_adjoint_set_umean(void* ctx, const double* dumean) {
+  if (!dumean) return PIB_ERR_ARG;
+  return dispatch(ctx, [&](auto& I) { I.adjoint_set_umean(dumean); });
+}
+int pib_checkpoint_push(void* ctx) {
+  return dispatch(ctx, [&](auto& I) { I.checkpoint_push(); });
+}
+int pib_checkpoint_pop(void* ctx) {
+  return dispatch(ctx, [&](auto& I) { I.checkpoint_pop(); });
 }
 int pib_step_async(void* ctx, int nsteps) {
   if (nsteps < 0) return PIB_ERR_ARG;

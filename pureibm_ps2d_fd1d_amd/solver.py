@@ -189,6 +189,39 @@ class Solver:
         self.nrpr_active = nr
         return cs.value, nr
 
+    # ---- adjoint of the fixed-bed step and the checkpoint stack ----
+    def checkpoint_push(self):
+        """Copy the forward state (u, p, nl and its physical copy, scalars)
+        onto the context's checkpoint stack."""
+        self._call("checkpoint_push", self.ctx)
+
+    def checkpoint_pop(self):
+        """Restore the forward state from the top of the stack and drop it."""
+        self._call("checkpoint_pop", self.ctx)
+
+    def adjoint_begin(self):
+        """Allocate and zero the adjoint state and the accumulated gradients."""
+        self._call("adjoint_begin", self.ctx)
+
+    def adjoint_step(self, wforce=None):
+        """Reverse the step that starts from the current forward state; wforce
+        (nbody, 3) weighs that step's per-particle force in the objective."""
+        w = None if wforce is None else np.asfortranarray(wforce, dtype=np.float64)
+        if w is not None and w.shape != (self.setup.nbody, 3):
+            raise ValueError("wforce must be (nbody, 3)")
+        self._call("adjoint_step", self.ctx, dptr(w) if w is not None else None)
+
+    def adjoint_get(self):
+        """-> (dJ/dxc (nbody, 3), dJ/dmpg (3,), adjoint of umean (3,))."""
+        dxc = np.zeros((self.setup.nbody, 3), order="F")
+        dmpg, dum = np.zeros(3), np.zeros(3)
+        self._call("adjoint_get", self.ctx, dptr(dxc), dptr(dmpg), dptr(dum))
+        return dxc, dmpg, dum
+
+    def adjoint_set_umean(self, dumean):
+        a = np.ascontiguousarray(dumean, dtype=np.float64)
+        self._call("adjoint_set_umean", self.ctx, dptr(a))
+
     # ---- restart files (restart_funcs.F90) ----
     def save_restart(self, xc=None, velbdy=None, ferror_array=None):
         """save_part_restart (restart_funcs.F90:69-128): writes the reference's
@@ -262,6 +295,7 @@ class Solver:
             "pbcp": (nx, my, mz), "ppr": (nx, my, mz, 3), "mask": (nx, my, mz),
             "phi": (nx + 1, my2, mz), "nlphi": (nx + 1, my2, mz), "onlphi": (nx + 1, my2, mz),
             "ffphi": (nx + 1, my2, mz),
+            "au": (nx + 1, my2, mz, 3), "ap": (nx, my2, mz), "anl": (nx + 1, my2, mz, 3),
         }[name]
 
     def field(self, name):
