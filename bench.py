@@ -280,8 +280,9 @@ def time_ibm_update(setup, reps=5):
 def time_scalar_solve(setup, reps=20):
     """One species' per-mode implicit solve (pib_scalstep_modes, FP64) on the
     workload's grid from random phif, nlphif, onlphif, ffphi.  Wall time a
-    call, its status read-back included; the algorithmic traffic is 4 complex
-    reads and 1 complex write a grid point."""
+    call: the calls are enqueued back to back and the stream drained at the
+    end (pib_sync, which also reads the status back); the algorithmic
+    traffic is 4 complex reads and 1 complex write a grid point."""
     g = solver.Solver(setup)
     g.scalar_set(0.02, 1.05, 0.97)
     rng = np.random.default_rng(5)
@@ -289,9 +290,12 @@ def time_scalar_solve(setup, reps=20):
     for f in ("phi", "nlphi", "onlphi", "ffphi"):
         g.set_field(f, rng.standard_normal(shp) + 1j * rng.standard_normal(shp))
     g.scalstep_modes(1)
+    g.lib.pib_sync(g.ctx)
     t0 = time.perf_counter()
     for _ in range(reps):
         g.scalstep_modes(1)
+    if g.lib.pib_sync(g.ctx) != 0:
+        raise RuntimeError(g.lib.pib_last_error(g.ctx).decode())
     t = (time.perf_counter() - t0) / reps
     g.close()
     pts = shp[0] * shp[1] * shp[2]
@@ -325,6 +329,47 @@ def time_moving_bed(workload, steps=5, warmup=2):
     dem.close()
     return {"bed": f"{st.nx}^3, {st.nbody} spheres, FP32-bulk fluid, FP64 DEM", "steps": steps,
             "ms_per_step": wall / steps * 1e3, "dem_substeps_per_step": info.factor}
+
+
+def run_adjoint(nsteps=64, every=8):
+    """Adjoint workload C4 (128^3, 50 spheres of 16 cells, Re_m = 20, SET_MPG
+    at the Stokes balance, fixed dt), FP32-bulk: J = total streamwise drag
+    after nsteps steps from the uniform stream.  Times the forward steps
+    alone (pib_step, graph replay), the forward with a checkpoint every
+    `every` steps, and the backward sweep (segment re-runs + adjoint steps)
+    that returns dJ/dxc and dJ/dmpg; adjoint_over_forward is the backward
+    sweep over the plain forward run."""
+    from pureibm_ps2d_fd1d_amd import adjoint
+    work = tempfile.mkdtemp(prefix="pib_c4_")
+    configs.write_workload(work, "C4")
+    st = solver.RunSetup(os.path.join(work, "C4"), precision=_abi.PIB_FP32_BULK)
+    P = adjoint.DragProblem(st, nsteps, every)
+    P.objective(None, None, keep=False)
+    mpg = adjoint.stokes_mpg(P.s)
+    P.objective(None, mpg, keep=False)  # warm: graphs, kernel attributes
+    s = P._fresh(None, mpg)
+    s.step(nsteps)
+    s = P._fresh(None, mpg)
+    s.lib.pib_sync(s.ctx)
+    t0 = time.perf_counter()
+    s.step(nsteps)
+    t_fwd = time.perf_counter() - t0
+    J = float(np.sum(P.wforce * s.forces()["force"]))
+    s = P._fresh(None, mpg)
+    t0 = time.perf_counter()
+    adjoint.forward_checkpointed(s, nsteps, every)
+    t_ck = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    dxc, dmpg = P.gradient()
+    t_bwd = time.perf_counter() - t0
+    P.close()
+    return {"workload": f"C4: {st.nx}^3, {st.nbody} spheres d=16 cells, Re_m=20, SET_MPG, fixed dt, FP32-bulk",
+            "steps": nsteps, "checkpoint_every": every, "J": J,
+            "forward_ms_per_step": t_fwd / nsteps * 1e3,
+            "forward_checkpointed_ms_per_step": t_ck / nsteps * 1e3,
+            "backward_ms_per_step": t_bwd / nsteps * 1e3,
+            "adjoint_over_forward": t_bwd / t_fwd,
+            "grad_norm_xc": float(np.linalg.norm(dxc)), "dJ_dmpg": [float(x) for x in dmpg]}
 
 
 def _cpu_model():
@@ -371,6 +416,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-realization figure")
     ap.add_argument("--batched-steps", type=int, default=100)
     ap.add_argument("--no-dem", action="store_true", help="skip the DEM sub-cycle figure")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (C4) figure")
     ap.add_argument("--dem-marches", type=int, default=10)
     ap.add_argument("--collect-traffic", action="store_true",
                     help="measure HBM bytes per dispatch with rocprofv3 counter passes (child processes)")
@@ -451,6 +497,11 @@ def main():
             out["batched_c3"] = run_batched(12, a.batched_steps, 5)
         except Exception as e:  # reported, never fatal for the headline figure
             out["batched_c3"] = {"error": str(e)}
+    if not a.no_adjoint and a.only in ("both", "fp32"):
+        try:
+            out["adjoint_c4"] = run_adjoint()
+        except Exception as e:  # reported, never fatal for the headline figure
+            out["adjoint_c4"] = {"error": str(e)}
     if not a.no_dem:
         try:
             out["dem"] = run_dem(a.dem_marches, 500, 0 if a.no_cpu else 2000)

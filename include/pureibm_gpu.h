@@ -62,7 +62,7 @@ typedef struct pib_config {
   int limiter;           /* PIB_LIMITER_* (LIMITER) */
   int revernorm;         /* REVERNORM */
   int dorpr;             /* DORPR */
-  int dobnd;             /* DOBND: calc_bnd_forcing, not implemented -> must be 0 */
+  int dobnd;             /* DOBND: calc_bnd_forcing (boundary_condition.F90:902-1102) */
   int include_frmeanfluid;
   int set_mpg;           /* SET_MPG: mpg held fixed */
   int soft_start;        /* initflo sets .true. (initialize_flo.F90:126) */
@@ -259,15 +259,27 @@ int pib_tridiag_batched_noalloc(void* ctx, int n, int nsys, const double* d_a, c
  * pib_adjoint_begin allocates the adjoint state and zeroes it together with
  * the accumulated gradients.  pib_adjoint_step reverses ONE step: the
  * context's forward state must be the state at the start of that step (as
- * pib_checkpoint_pop restores it); on return the adjoint state holds the
- * adjoint of the step's inputs, the gradients have the step's part added, and
- * the forward state has been advanced through the step's nonlinear part (it
- * is scratch until the next pop).  wforce (nbody,3) is that step's weight on
- * the per-particle force, or NULL.  pib_adjoint_get returns dJ/dxc (nbody,3),
- * dJ/dmpg (3) and the adjoint of umean (3); any pointer may be NULL.
+ * pib_checkpoint_pop restores it); once its work is done the adjoint state
+ * holds the adjoint of the step's inputs, the gradients have the step's part
+ * added, and the forward state has been advanced through the step's
+ * nonlinear part (it is scratch until the next pop).  The work is enqueued on
+ * the context stream without waiting for it.  wforce (nbody,3) is that step's
+ * weight on the per-particle force, or NULL.  pib_adjoint_get drains the
+ * stream (reporting a device-side error of the steps before it) and returns
+ * dJ/dxc (nbody,3), dJ/dmpg (3) and the adjoint of umean (3); any pointer may
+ * be NULL.
  * pib_adjoint_set_umean seeds the adjoint of umean (3). */
 int pib_adjoint_begin(void* ctx);
 int pib_adjoint_step(void* ctx, const double* wforce);
+/* nsteps steps as pib_step runs them, keeping after each the forward fields
+ * that the step's adjoint reads (ubcp, nlbc, onlbc, diffn, ppr and the
+ * step's t, dt, cf; velstep leaves them as the step's nonlinear part set
+ * them) on a record stack.  A pib_adjoint_step without a force weight takes
+ * the top record when there is one, instead of recomputing the fields from
+ * the forward state; with a force weight it always recomputes (it also needs
+ * the vorticity and pbcp) and leaves the record stack alone.  Enqueued
+ * without a final drain; pib_adjoint_begin empties the record stack. */
+int pib_adjoint_record(void* ctx, int nsteps);
 int pib_adjoint_get(void* ctx, double* dxc, double* dmpg, double* dumean);
 int pib_adjoint_set_umean(void* ctx, const double* dumean);
 /* Checkpoint stack of the forward state (u, p, nl, the physical copy of nl
@@ -275,6 +287,10 @@ int pib_adjoint_set_umean(void* ctx, const double* dumean);
  * drops it.  PIB_ERR_STATE when popping an empty stack. */
 int pib_checkpoint_push(void* ctx);
 int pib_checkpoint_pop(void* ctx);
+/* nsteps steps as pib_step runs them (graph replays), pushing the state at
+ * the start of each step: a segment's re-run before its steps are reversed.
+ * Enqueued without a final drain (pib_sync or pib_adjoint_get waits). */
+int pib_checkpoint_steps(void* ctx, int nsteps);
 
 /* ---- timers ----------------------------------------------------------- */
 /* Milliseconds spent in each phase over the last pib_step call, measured with

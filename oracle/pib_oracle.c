@@ -80,6 +80,8 @@ typedef struct orc {
   void* adj;
   void** ck;
   int nck, ckcap;
+  void** rec; /* recorded forward fields of steps to be reversed (orc_adjoint_record) */
+  int nrec, reccap;
   /* plane work arrays */
   double *ra, *rb;
   cpx *ca, *cb;
@@ -1509,6 +1511,63 @@ static void calc_inner_reversal_forcing(orc* o, int m, int rks, double da2) {
   }
 }
 
+/* calc_bnd_forcing :902-1102 (serial): the direct forcing at every surface
+ * point, interpolated as at an inner reversal point and spread with its
+ * stencil weights to all eight nodes (fluid or solid, no redistribution).
+ * linvel = velbdy + (angv x n) rb dx with n the point's unit normal cd; the
+ * reference takes n from the module variable SNORM, which this routine does
+ * not set (it holds a normal of the last reversal point handled), so the two
+ * agree only for angv = 0 -- the case of every reference run, whose DEM
+ * never updates OMEGA (collision_mod.F90:1386). */
+static void calc_bnd_forcing(orc* o, int m, int rks, double da1) {
+  pib_scalars* s = &o->s;
+  double rb = o->radbdy[m - 1];
+  double drm = o->cfg.dr;
+  double c1 = o->coef[rks - 1][0], c2 = o->coef[rks - 1][1], c3 = o->coef[rks - 1][2], c4 = o->coef[rks - 1][3];
+  for (int l = 1; l <= o->nbnd; ++l) {
+    double xl[3];
+    for (int n = 1; n <= 3; ++n) xl[n - 1] = XC(o, m, n) + BND(o, n, l) * rb;
+    double xpb[3] = {xl[0] - 0.5, xl[1], xl[2]};
+    int pcell[3], vcell[3];
+    for (int n = 0; n < 3; ++n) {
+      pcell[n] = cellof(xpb[n]);
+      vcell[n] = cellof(xl[n]);
+    }
+    double ppll[3], pl;
+    interpolate_pdata(o, pcell, xpb, ppll, &pl);
+    stencil st;
+    double ul[3], nll[3], onll[3], dfll[3];
+    interpolate_udata(o, vcell, xl, &st, ul, nll, onll, dfll, 1);
+    double w1 = PB(o, angv, m, 1), w2v = PB(o, angv, m, 2), w3 = PB(o, angv, m, 3);
+    double sn[3] = {CD(o, 1, l), CD(o, 2, l), CD(o, 3, l)};
+    double crossp[3] = {w2v * sn[2] - w3 * sn[1], w3 * sn[0] - w1 * sn[2], w1 * sn[1] - w2v * sn[0]};
+    double ft[3];
+    for (int n = 0; n < 3; ++n) {
+      double linvel = PB(o, velbdy, m, n + 1) + crossp[n] * rb * o->dx;
+      ft[n] = s->cf * (-linvel + ul[n]);
+      ft[n] = ft[n] - c3 * nll[n] - c4 * onll[n];
+      ft[n] = ft[n] + (c1 + c2) * (ppll[n] - dfll[n]);
+      ft[n] = ft[n] * da1 * drm * o->dx;
+    }
+    for (int k = 0; k < 2; ++k) {
+      int kk = wrapz(o, st.kb + k);
+      for (int j = 0; j < 2; ++j) {
+        int jj = wrapy(o, st.jb + j);
+        for (int i = 0; i < 2; ++i) {
+          int ii = wrapx(o, st.ib + i);
+          for (int n = 1; n <= 3; ++n) {
+            double tmppa = st.w[(k * 2 + j) * 2 + i] * ft[n - 1];
+            o->fr[IR(o, ii, jj, kk, n)] = o->fr[IR(o, ii, jj, kk, n)] + tmppa / o->dx3;
+            if (o->fluid[IFL(o, ii, jj, kk)] && o->cfg.include_frmeanfluid)
+              s->frmeanfluid[n - 1] = s->frmeanfluid[n - 1] + tmppa / o->dx3;
+            s->frmean[n - 1] = s->frmean[n - 1] + tmppa / o->dx3;
+          }
+        }
+      }
+    }
+  }
+}
+
 /* bcset :467-900, serial.  The first half (:517-538 with the diagnostics of
  * :723-884, which read only force, force_chem and mpg) runs before the point
  * where the reference moves particles (:539-554); the second half is
@@ -1559,8 +1618,11 @@ static int bcset_forcing(orc* o, int rks) {
   for (int n = 0; n < 3; ++n) { s->frmean[n] = 0.0; s->frmeanfluid[n] = 0.0; }
   memset(o->fr, 0, sizeof(double) * (size_t)(nx + 1) * my * mz * 3);
   for (int m = 1; m <= o->nbody; ++m) {
+    double rbx = o->radbdy[m - 1] * o->dx;
+    double da1 = 4.0 * PI * (rbx * rbx) / (double)o->nbnd;
     double rdx = o->radibdy[m - 1] * o->dx;
     double da2 = 4.0 * PI * (rdx * rdx) / (double)o->nrpr_active[m - 1];
+    if (o->cfg.dobnd) calc_bnd_forcing(o, m, rks, da1);
     if (o->cfg.dorpr) calc_inner_reversal_forcing(o, m, rks, da2);
   }
   for (int n = 0; n < 3; ++n) {
@@ -2062,6 +2124,49 @@ static void adj_free(orc* o) {
   free(o->ck);
   o->ck = NULL;
   o->nck = o->ckcap = 0;
+  for (int q = 0; q < o->nrec; ++q) free(o->rec[q]);
+  free(o->rec);
+  o->rec = NULL;
+  o->nrec = o->reccap = 0;
+}
+
+/* After a whole step, ubcp, nlbc, onlbc, diffn and ppr still hold that step's
+ * values (velstep writes only u and p), with its t, dt and cf in the
+ * scalars: what the adjoint of the step reads besides the bed.  A record is
+ * those five fields and the scalars. */
+static size_t rec_doubles(const orc* o) {
+  size_t nr = (size_t)(o->nx + 1) * o->my * o->mz * 3;
+  return 4 * nr + (size_t)o->nx * o->my * o->mz * 3;
+}
+static void rec_copy(orc* o, double* b, int push) {
+  size_t nr = (size_t)(o->nx + 1) * o->my * o->mz * 3, np = (size_t)o->nx * o->my * o->mz * 3;
+  double* f[4] = {o->ubcp, o->nlbc, o->onlbc, o->diffn};
+  for (int q = 0; q < 4; ++q) {
+    if (push) memcpy(b + q * nr, f[q], nr * sizeof(double));
+    else memcpy(f[q], b + q * nr, nr * sizeof(double));
+  }
+  if (push) memcpy(b + 4 * nr, o->ppr, np * sizeof(double));
+  else memcpy(o->ppr, b + 4 * nr, np * sizeof(double));
+  pib_scalars* sp = (pib_scalars*)(b + 4 * nr + np);
+  if (push) *sp = o->s;
+  else o->s = *sp;
+}
+int orc_adjoint_record(void* ctx, int nsteps) {
+  orc* o = (orc*)ctx;
+  if (!o || nsteps < 0) return PIB_ERR_ARG;
+  if (!o->adj) return fail(o, PIB_ERR_STATE, "pib_adjoint_begin first");
+  for (int n = 0; n < nsteps; ++n) {
+    int st = orc_step(o, 1);
+    if (st) return st;
+    if (o->nrec == o->reccap) {
+      o->reccap = o->reccap ? 2 * o->reccap : 16;
+      o->rec = (void**)realloc(o->rec, sizeof(void*) * (size_t)o->reccap);
+    }
+    double* b = (double*)malloc(sizeof(double) * rec_doubles(o) + sizeof(pib_scalars));
+    rec_copy(o, b, 1);
+    o->rec[o->nrec++] = b;
+  }
+  return PIB_OK;
 }
 
 static int adj_field_info(orc* o, int field, void** ptr, size_t* ndoubles) {
@@ -2110,6 +2215,8 @@ int orc_adjoint_begin(void* ctx) {
     A->nbody = o->nbody;
     o->adj = A;
   }
+  for (int q = 0; q < o->nrec; ++q) free(o->rec[q]);
+  o->nrec = 0;
   memset(A->au, 0, nc * sizeof(cpx));
   memset(A->anl, 0, nc * sizeof(cpx));
   memset(A->ap, 0, np * sizeof(cpx));
@@ -2170,6 +2277,17 @@ int orc_checkpoint_pop(void* ctx) {
   memcpy(o->p, c->p, np * sizeof(cpx));
   o->s = c->s;
   free(c->u); free(c->p); free(c->nl); free(c->onl); free(c);
+  return PIB_OK;
+}
+
+int orc_checkpoint_steps(void* ctx, int nsteps) {
+  orc* o = (orc*)ctx;
+  if (!o || nsteps < 0) return PIB_ERR_ARG;
+  for (int n = 0; n < nsteps; ++n) {
+    int st = orc_checkpoint_push(o);
+    if (!st) st = orc_step(o, 1);
+    if (st) return st;
+  }
   return PIB_OK;
 }
 
@@ -2524,9 +2642,15 @@ int orc_adjoint_step(void* ctx, const double* wforce) {
   size_t nc = (size_t)(nx + 1) * my2 * mz * 3, np = (size_t)nx * my2 * mz;
   size_t nr = (size_t)(nx + 1) * my * mz * 3, npr = (size_t)nx * my * mz;
   int st;
-  /* the step's forward fields */
-  if ((st = orc_compute_new_timestep(o, 1))) return st;
-  if ((st = nonlinear_part(o, 1, 0))) return st;
+  /* the step's forward fields: recorded ones (no force weight), or run */
+  if (!wforce && o->nrec > 0) {
+    double* b = (double*)o->rec[--o->nrec];
+    rec_copy(o, b, 0);
+    free(b);
+  } else {
+    if ((st = orc_compute_new_timestep(o, 1))) return st;
+    if ((st = nonlinear_part(o, 1, 0))) return st;
+  }
   double dt = o->s.dt;
   memset(A->aunew, 0, nc * sizeof(cpx));
   memset(A->apnew, 0, np * sizeof(cpx));

@@ -46,6 +46,8 @@ int orc_adjoint_get(void* ctx, double* dxc, double* dmpg, double* dumean);
 int orc_adjoint_set_umean(void* ctx, const double* dumean);
 int orc_checkpoint_push(void* ctx);
 int orc_checkpoint_pop(void* ctx);
+int orc_checkpoint_steps(void* ctx, int nsteps);
+int orc_adjoint_record(void* ctx, int nsteps);
 
 /* transform layer: one plane each call, a(my,mz) real col-major, b(my2,mz) complex */
 void orc_ff2rc(int my, int mz, const double* a, double* b);

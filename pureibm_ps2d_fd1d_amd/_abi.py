@@ -135,10 +135,12 @@ _STEP_SIGS = {
     "scalstep_modes": ([_VP, _I], _I),
     "adjoint_begin": ([_VP], _I),
     "adjoint_step": ([_VP, _PD], _I),
+    "adjoint_record": ([_VP, _I], _I),
     "adjoint_get": ([_VP, _PD, _PD, _PD], _I),
     "adjoint_set_umean": ([_VP, _PD], _I),
     "checkpoint_push": ([_VP], _I),
     "checkpoint_pop": ([_VP], _I),
+    "checkpoint_steps": ([_VP, _I], _I),
 }
 
 _GPU_ONLY_SIGS = {

@@ -4,9 +4,9 @@ The reference has no adjoint.  This module drives the hand-written adjoint
 step of include/pureibm_gpu.h (``pib_adjoint_step``) with the checkpoint
 stack (``pib_checkpoint_push``/``_pop``): the forward run keeps the state at
 the start of every ``every``-th step; the backward sweep restores each
-segment, re-runs it forward keeping the state at the start of each of its
-steps, and reverses the steps last to first.  At most
-``ceil(nsteps/every) + every - 1`` states are held at once.
+segment, re-runs it forward keeping the forward fields of each of its steps
+(``pib_adjoint_record``), and reverses the steps last to first.  At most
+``ceil(nsteps/every)`` states and ``every`` records are held at once.
 
 ``DragObjective`` wraps the pair as a ``torch.autograd.Function`` of the
 particle centres and the mean pressure gradient: a thin test harness over the
@@ -32,29 +32,46 @@ def stokes_mpg(s):
 def forward_checkpointed(s, nsteps, every):
     """Run ``nsteps`` steps of Solver ``s``, pushing the state at the start of
     steps 0, every, 2*every, ..."""
-    for n in range(nsteps):
-        if n % every == 0:
-            s.checkpoint_push()
-        s.step(1)
+    for n0 in range(0, nsteps, every):
+        s.checkpoint_push()
+        s.step(min(every, nsteps - n0))
 
 
 def backward(s, nsteps, every, wforce_of_step):
     """Reverse ``nsteps`` steps whose segment checkpoints ``forward_checkpointed``
     pushed.  ``wforce_of_step(n)`` gives the (nbody, 3) weight of step n's
-    per-particle force in the objective, or None.  Returns (dJ/dxc, dJ/dmpg,
+    per-particle force in the objective, or None.  Each segment is restored
+    and re-run: a step without a weight keeps its forward fields on the
+    record stack (pib_adjoint_record), one with a weight keeps the state at
+    its start (its adjoint recomputes the fields, the vorticity included);
+    then its steps are reversed last to first.  Returns (dJ/dxc, dJ/dmpg,
     adjoint of the initial umean)."""
     s.adjoint_begin()
     nseg = math.ceil(nsteps / every)
     for seg in reversed(range(nseg)):
         n0, n1 = seg * every, min((seg + 1) * every, nsteps)
         s.checkpoint_pop()  # the state at the start of step n0
-        for n in range(n0, n1 - 1):
-            s.checkpoint_push()
-            s.step(1)
-        s.adjoint_step(wforce_of_step(n1 - 1))  # the state at the start of step n1-1 is current
-        for n in reversed(range(n0, n1 - 1)):
-            s.checkpoint_pop()
-            s.adjoint_step(wforce_of_step(n))
+        last_w = wforce_of_step(n1 - 1) is not None
+        stop = n1 - 1 if last_w else n1
+        n = n0
+        while n < stop:
+            if wforce_of_step(n) is not None:
+                s.checkpoint_push()
+                s.step(1)
+                n += 1
+                continue
+            k = 1
+            while n + k < stop and wforce_of_step(n + k) is None:
+                k += 1
+            s.adjoint_record(k)
+            n += k
+        if last_w:  # the state at the start of step n1-1 is current
+            s.adjoint_step(wforce_of_step(n1 - 1))
+        for n in reversed(range(n0, stop)):
+            w = wforce_of_step(n)
+            if w is not None:
+                s.checkpoint_pop()
+            s.adjoint_step(w)
     return s.adjoint_get()
 
 

@@ -45,12 +45,21 @@ def flang():
 
 
 def build_gpu():
+    """Each translation unit compiled on its own process (they build in
+    parallel), then linked into the shared library."""
     os.makedirs(BUILD, exist_ok=True)
     out = os.path.join(BUILD, "libpureibm_gpu.so")
-    _run([hipcc()] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"),
-                                  os.path.join(CSRC, "pib_gpu.hip"), os.path.join(CSRC, "pib_dem.hip"),
-                                  os.path.join(CSRC, "pib_adjoint.hip"),
-                                  "-o", out + ".tmp"])
+    srcs = ["pib_gpu.hip", "pib_dem.hip", "pib_adjoint.hip"]
+    objs = [os.path.join(BUILD, os.path.splitext(f)[0] + ".o") for f in srcs]
+    flags = [f for f in HIP_FLAGS if f != "-shared"] + ["-I", os.path.join(ROOT, "include")]
+    procs = []
+    for src, obj in zip(srcs, objs):
+        cmd = [hipcc()] + flags + ["-c", os.path.join(CSRC, src), "-o", obj]
+        print("+", " ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+    if any(p.wait() != 0 for p in procs):
+        raise subprocess.CalledProcessError(1, "hipcc")
+    _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out
 

@@ -95,7 +95,7 @@ def write_case(directory, run_name, *, my, lybyd, dbydx, centers=None, input_typ
                vol_frac1=None, re=20.0, vis=0.012, cfl=0.1, cfl_max=0.5, adaptive=True,
                discr_scheme="center", limiter="none", rk3=False, ramp_frac_steps=0,
                set_mpg=False, mpg=(0.0, 0.0, 0.0), revernorm=1, dr=1.0, f2=1.6,
-               include_frmeanfluid=False, dorpr=True, imove=0, rhos=1.0, coeff_rest=1.0):
+               include_frmeanfluid=False, dorpr=True, dobnd=False, imove=0, rhos=1.0, coeff_rest=1.0):
     """Write ``<directory>/<run_name>_floparam.in`` (+ sphr_center file).
     imove=1 makes a freely moving bed (soft-sphere collisions, particle
     density rhos, restitution coeff_rest)."""
@@ -122,7 +122,7 @@ def write_case(directory, run_name, *, my, lybyd, dbydx, centers=None, input_typ
         "PART_PROPT": part,
         "FLAGS": {
             "IMOVE": int(imove), "ISCALON": 0, "IRESTART": 0, "ADAPTIVE": bool(adaptive),
-            "RK3": bool(rk3), "DORPR": bool(dorpr), "DOBND": False,
+            "RK3": bool(rk3), "DORPR": bool(dorpr), "DOBND": bool(dobnd),
             "INCLUDE_FRMEANFLUID": bool(include_frmeanfluid), "SET_MPG": bool(set_mpg),
             "REVERNORM": int(revernorm),
         },
@@ -158,18 +158,27 @@ def random_bed(directory, run_name, *, my, dbydx, nbody, seed, **kw):
 C3_PHI_CAP = 0.35  # random sequential addition jams near 0.38
 
 
+C3_CELL_RE_MAX = 4.5  # Re_m / d: the resolution the scheme needs (measured)
+
+
 def c3_realization(directory, k):
     """Realization k (0..11) of the batched workload: grids 96^3/128^3/160^3 in
     turn, sphere diameter d ~ U{12..20} cells, solid fraction ~ U[0.1, 0.4]
-    capped at 0.35 (RSA jams near 0.38), Re_m ~ U[10, 100], all drawn from
-    seed 1000 + k.  The x-fluxes use the MUSCL/Rusanov 'upwind' scheme, as the
-    reference's own input decks do (sample_inputs/floparam.in,
-    MIS1_floparam.in: DISCR_SCHEME = 'upwind')."""
+    capped at 0.35 (RSA jams near 0.38), Re_m ~ U[10, min(100, 4.5 d)], all
+    drawn from seed 1000 + k.  The x-fluxes use the MUSCL/Rusanov 'upwind'
+    scheme, as the reference's own input decks do (sample_inputs/floparam.in,
+    MIS1_floparam.in: DISCR_SCHEME = 'upwind').  The cap on Re_m is the grid
+    resolution the method needs: with the pseudo-spectral y-z derivatives of
+    the products (no dealiasing), beds whose cell Reynolds number Re_m/d
+    exceeds 5 diverge within a few hundred steps in FP64 and FP32
+    alike, with any time step, with RK3 and with a ramped start (Re_m/d =
+    5.1 and 5.7 diverged; 4.6 and below ran 320 steps); d = 20 reaches
+    Re_m = 90."""
     rng = np.random.Generator(np.random.PCG64(1000 + k))
     my = (96, 128, 160)[k % 3]
     d = int(rng.integers(12, 21))
     phi = min(float(rng.uniform(0.1, 0.4)), C3_PHI_CAP)
-    re = float(rng.uniform(10.0, 100.0))
+    re = 10.0 + (min(100.0, C3_CELL_RE_MAX * d) - 10.0) * float(rng.uniform(0.0, 1.0))
     lybyd = my / d
     nbody = max(1, int(round(phi * lybyd ** 3 / (math.pi / 6.0))))
     return random_bed(directory, f"C3_{k}", my=my, dbydx=d, nbody=nbody, seed=k + 1, re=re,

@@ -7,11 +7,14 @@
 // interpolations scatter with atomic adds, so their sums (and the gradient
 // sums) are not bit-reproducible from run to run, unlike the forward step.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 
 #include "pib_adjoint.h"
+#include "pib_velstep.h"
 
 namespace {
 
@@ -34,10 +37,21 @@ __device__ __forceinline__ int wrp(int i, int n) { return i < 0 ? i + n : (i >= 
 template <typename T>
 __device__ __forceinline__ double ky_weight(const KArgs<T>& A, int ky) { return (ky == 0 || ky == A.my2 - 1) ? 1.0 : 2.0; }
 
+// up to six buffers zeroed in one launch (grid.y = buffer); a fill kernel,
+// so that it may be captured in a graph
 template <typename T>
-__global__ void k_adj_zero(T* p, size_t n) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = (T)0;
+struct ZeroSet {
+  T* p[6];
+  size_t n[6];
+  int k;
+};
+template <typename T>
+__global__ void k_adj_zero_set(ZeroSet<T> z) {
+  const int b = blockIdx.y;
+  T* p = z.p[b];
+  const size_t n = z.n[b];
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = (T)0;
 }
 
 // ===========================================================================
@@ -224,6 +238,153 @@ __global__ void __launch_bounds__(64) k_adj_velstep(KArgs<T> A, AdjArgs<T> J, in
   }
 }
 
+// The same transposes on the tiles of the forward velstep (pib_velstep.h):
+// M modes per workgroup, L x rows per lane in registers, every line solved
+// by the partition method with the forward's symmetric matrices (their own
+// transposes).  The (0,0) mode runs in double in the last workgroup (L = 2,
+// one mode); its pressure/divcorr solves keep rows 1..nx-1 of the real part,
+// which is the transpose of the forward's pinned solve.
+// the last row of the previous segment for NL lines, through LDS
+template <int NL, typename T, typename R, int L, int M>
+__device__ __forceinline__ void prev_last(const VTile<T, R, L, M>& V, C2<R> (*x)[L], C2<R>* out) {
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NL; ++c) V.S.el[c * V.bd + V.tid] = x[c][L - 1];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NL; ++c) out[c] = V.S.el[c * V.bd + V.nbl(V.s - 1)];
+}
+template <typename T, typename R, int L, int M>
+__device__ void vel_tile_adj(const KArgs<T>& A, const AdjArgs<T>& J, int q0, bool mean, unsigned char* raw) {
+  VTile<T, R, L, M> V(A, q0, mean, raw);
+  const DevState* st = A.st;
+  const R c1 = (R)st->coef[0][0], c2 = (R)st->coef[0][1], c3 = (R)st->coef[0][2], c4 = (R)st->coef[0][3];
+  const R dx = (R)st->dx, dx2 = dx * dx, dt = (R)st->s.dt, vis = (R)st->vis;
+  const R idx = (R)1 / dx, idx2 = (R)1 / dx2;
+  const R c12 = c1 + c2, alp = c12 * dt, cv = (c2 * dt) * vis;
+  const int i0 = V.s * L;
+  const R wy = V.wy, wz = V.wz, w2 = V.w2;
+  const R ap = idx2 - (R)0.25 * w2, bp = (R)-2 * idx2 - (R)0.5 * w2;
+  C2<R> u[3][L], pb[1][L];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+#pragma unroll
+    for (int k = 0; k < L; ++k) u[c][k] = V.ld(J.au, c, i0 + k);
+#pragma unroll
+    for (int k = 0; k < L; ++k) pin(u[c][k]);
+  }
+#pragma unroll
+  for (int k = 0; k < L; ++k) pb[0][k] = V.ld(J.ap, 0, i0 + k);
+  // ---- divcorr^T: phi_bar = p_bar - cv (L(p_bar)/dx2 - w2 p_bar) - alp G^T(u_bar) ----
+  C2<R> phi[1][L];
+  {
+    C2<R> un[3], pn[1], pp[1];
+    V.template next_first<3>(u, un);
+    V.template next_first<1>(pb, pn);
+    prev_last<1>(V, pb, pp);
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      C2<R> pm = k > 0 ? pb[0][k - 1] : pp[0];
+      C2<R> pq = k < L - 1 ? pb[0][k + 1] : pn[0];
+      C2<R> n0 = k < L - 1 ? u[0][k + 1] : un[0];
+      C2<R> n1 = k < L - 1 ? u[1][k + 1] : un[1];
+      C2<R> n2 = k < L - 1 ? u[2][k + 1] : un[2];
+      C2<R> lap = (pm - cscal((R)2, pb[0][k])) + pq;
+      C2<R> ph = pb[0][k] - cscal(cv, cscal(idx2, lap) - cscal(w2, pb[0][k]));
+      C2<R> g = (cscal(idx, u[0][k] - n0) + cik((R)-0.5 * wy, u[1][k] + n1)) + cik((R)-0.5 * wz, u[2][k] + n2);
+      phi[0][k] = ph - cscal(alp, g);
+      if (mean) phi[0][k].y = (R)0;
+    }
+  }
+  {
+    Toep<R, L> tp;
+    toep_init(tp, ap, bp, V.st);
+    V.template stage<1>(tp);
+    V.template solve<1>(tp, phi, mean);
+  }
+  if (mean)
+#pragma unroll
+    for (int k = 0; k < L; ++k) phi[0][k].y = (R)0;
+  // u*_bar = u_bar + D^T(dv_bar/alp)
+  {
+    C2<R> fp[1];
+    prev_last<1>(V, phi, fp);
+    const R ia = (R)1 / alp;
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      C2<R> sm = cscal(ia, k > 0 ? phi[0][k - 1] : fp[0]), sk = cscal(ia, phi[0][k]);
+      u[0][k] = u[0][k] + cscal(idx, sm - sk);
+      u[1][k] = u[1][k] + cik((R)-0.5 * wy, sm + sk);
+      u[2][k] = u[2][k] + cik((R)-0.5 * wz, sm + sk);
+    }
+  }
+  // ---- ustep^T ----
+  {
+    Toep<R, L> th;
+    const R ah = -dt * vis * c2 * idx2;
+    const R bh = (R)1 + dt * vis * c2 * ((R)2 * idx2 + w2);
+    toep_init(th, ah, bh, V.st);
+    V.template stage<3>(th);
+    V.template solve<3>(th, u, false);
+  }
+  {
+    C2<R> qn[3], qp[3];
+    V.template next_first<3>(u, qn);
+    prev_last<3>(V, u, qp);
+    const R e1 = (R)1 - dt * vis * c1 * w2, e2 = dt * vis * c1 * idx2;
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      C2<R> g = mk((R)0, (R)0);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        C2<R> qm = k > 0 ? u[c][k - 1] : qp[c];
+        C2<R> qq = k < L - 1 ? u[c][k + 1] : qn[c];
+        C2<R> qc = u[c][k];
+        V.stv(J.aunew, c, i0 + k, cscal(e1, qc) + cscal(e2, (qm - cscal((R)2, qc)) + qq));
+        V.stv(J.anl, c, i0 + k, V.ld(J.anl, c, i0 + k) + cscal(dt * c3, qc));
+        V.stv(J.aff, c, i0 + k, cscal(dt, qc));
+        V.stv(J.anlp, c, i0 + k, cscal(dt * c4, qc));
+        if (c == 0) g = g + cscal(idx, qc - qq);
+        else g = g + cik((R)-0.5 * (c == 1 ? wy : wz), qc + qq);
+      }
+      pb[0][k] = pb[0][k] - cscal(dt * c12, g);
+      if (mean) pb[0][k].y = (R)0;
+    }
+  }
+  // ---- pressure^T ----
+  {
+    Toep<R, L> tp;
+    toep_init(tp, ap, bp, V.st);
+    V.template stage<1>(tp);
+    V.template solve<1>(tp, pb, mean);
+  }
+  if (mean)
+#pragma unroll
+    for (int k = 0; k < L; ++k) pb[0][k].y = (R)0;
+  {
+    C2<R> fp[1];
+    prev_last<1>(V, pb, fp);
+    const R ic = (R)1 / c12;
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+      C2<R> sm = cscal(ic, k > 0 ? pb[0][k - 1] : fp[0]), sk = cscal(ic, pb[0][k]);
+      C2<R> g[3] = {cscal(idx, sm - sk), cik((R)-0.5 * wy, sm + sk), cik((R)-0.5 * wz, sm + sk)};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        V.stv(J.anl, c, i0 + k, V.ld(J.anl, c, i0 + k) + cscal(c3, g[c]));
+        V.stv(J.aff, c, i0 + k, V.ld(J.aff, c, i0 + k) + g[c]);
+        V.stv(J.anlp, c, i0 + k, V.ld(J.anlp, c, i0 + k) + cscal(c4, g[c]));
+      }
+    }
+  }
+}
+template <typename T, int L, int M, int NT>
+__global__ void __launch_bounds__(NT, NT <= 256 ? 3 : 1) k_adj_velstep_tiled(KArgs<T> A, AdjArgs<T> J, int ntiles) {
+  extern __shared__ __align__(16) unsigned char vlds[];
+  if ((int)blockIdx.x == ntiles) vel_tile_adj<T, double, 2, 1>(A, J, 0, true, vlds);
+  else vel_tile_adj<T, T, L, M>(A, J, (int)blockIdx.x * M, false, vlds);
+}
+
 // umean += (-mpg + frmean) dt and ff = r2c(fr - frmean), transposed for the
 // mean terms; with the force weights, the mpg term of
 // force = c12 (pres + visc) - c12 mpg pi vol/6.  One workgroup.
@@ -347,7 +508,7 @@ __global__ void __launch_bounds__(256) k_adj_sum_part(const double* part, int nx
 // ===========================================================================
 // element-wise transposes on the spectral grid, (i, mode) per lane
 // ===========================================================================
-enum { ADJ_PGRAD = 0, ADJ_OMEGA = 1, ADJ_VISC = 2 };
+enum { ADJ_PGRAD = 0, ADJ_OMEGA = 1, ADJ_SFIELD = 2 };
 template <typename T, int KIND>
 __global__ void k_adj_modes(KArgs<T> A, AdjArgs<T> J) {
   const size_t E = (size_t)A.my2 * A.mz;
@@ -364,18 +525,24 @@ __global__ void k_adj_modes(KArgs<T> A, AdjArgs<T> J) {
     // calc_pgrad: ppr = c2r((p_{i+1} - p_{i-1})/2dx, (i wy) p, (i wz) p) + mpg
     const double dx = (double)A.dx;
     C2<double> v = sc(1.0 / (2.0 * dx), Z(0, im) - Z(0, ip)) + ik(-wy, Z(1, i)) + ik(-wz, Z(2, i));
-    J.apnew[at(0)] = t2<T>(d2(J.apnew[at(0)]) + v);
+    J.apnew[at(0)] = t2<T>(v);  // the first term of the step's adjoint of p
   } else if constexpr (KIND == ADJ_OMEGA) {
     // compute_omega: w1 = c2r((i wy) u3 - (i wz) u2), w2 = c2r((i wz) u1) - .., w3 = .. - c2r((i wy) u1)
     J.aunew[at(0)] = t2<T>(d2(J.aunew[at(0)]) + ik(wy, Z(2, i)) + ik(-wz, Z(1, i)));
     J.aunew[at(1)] = t2<T>(d2(J.aunew[at(1)]) + ik(wz, Z(0, i)));
     J.aunew[at(2)] = t2<T>(d2(J.aunew[at(2)]) + ik(-wy, Z(0, i)));
   } else {
-    // calc_visc: diffn = c2r(vis (L(u)/dx2 - w2 u))
+    // Z = c2r^T(S): nlbc = c2r(nl) and onlbc = c2r(onl) with the adjoints
+    // -c3 S, -c4 S; calc_visc, diffn = c2r(vis (L(u)/dx2 - w2 u)), with -c12 S
+    const DevState* st = A.st;
+    const double c3 = st->coef[0][2], c4 = st->coef[0][3], c12 = st->coef[0][0] + st->coef[0][1];
     const double dx2 = (double)A.dx * (double)A.dx, w2 = (double)A.w2[q], vis = (double)A.vis;
     for (int c = 0; c < 3; ++c) {
-      C2<double> lap = (Z(c, im) - sc(2.0, Z(c, i))) + Z(c, ip);
-      J.aunew[at(c)] = t2<T>(d2(J.aunew[at(c)]) + sc(vis, sc(1.0 / dx2, lap) - sc(w2, Z(c, i))));
+      const C2<double> zc = Z(c, i);
+      C2<double> lap = (Z(c, im) - sc(2.0, zc)) + Z(c, ip);
+      J.aunew[at(c)] = t2<T>(d2(J.aunew[at(c)]) + sc(-c12 * vis, sc(1.0 / dx2, lap) - sc(w2, zc)));
+      J.anl[at(c)] = t2<T>(d2(J.anl[at(c)]) + sc(-c3, zc));
+      J.anlp[at(c)] = t2<T>(d2(J.anlp[at(c)]) + sc(-c4, zc));
     }
   }
 }
@@ -383,10 +550,11 @@ __global__ void k_adj_modes(KArgs<T> A, AdjArgs<T> J) {
 // ===========================================================================
 // element-wise transposes on the physical grid, one node per lane
 // ===========================================================================
-// compute_omega's x-differences of ubcp and form_nl's products (central),
-// transposed into aubcp
+// form_nl's products (central), the inner-point share cf S of the reversal
+// forcing and, when the step's drag is weighed, compute_omega's
+// x-differences of ubcp, transposed into aubcp
 template <typename T>
-__global__ void k_adj_phys(KArgs<T> A, AdjArgs<T> J) {
+__global__ void k_adj_phys(KArgs<T> A, AdjArgs<T> J, int drag) {
   const size_t P = (size_t)A.my * A.mz, cs = (size_t)A.nx * P;
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= cs) return;
@@ -402,11 +570,17 @@ __global__ void k_adj_phys(KArgs<T> A, AdjArgs<T> J) {
   double a1 = q12 * u2 + q13 * u3 + 2.0 * F1 * u1 + F2 * u2 + F3 * u3;
   double a2 = q12 * u1 + 2.0 * q22 * u2 + q23 * u3 + F2 * u1;
   double a3 = q13 * u1 + q23 * u2 + 2.0 * q33 * u3 + F3 * u1;
-  a2 += ((double)J.aom[2 * cs + om] - (double)J.aom[2 * cs + op]) / tdx;
-  a3 -= ((double)J.aom[cs + om] - (double)J.aom[cs + op]) / tdx;
-  J.aubcp[e] = (T)((double)J.aubcp[e] + a1);
-  J.aubcp[cs + e] = (T)((double)J.aubcp[cs + e] + a2);
-  J.aubcp[2 * cs + e] = (T)((double)J.aubcp[2 * cs + e] + a3);
+  if (drag) {
+    a2 += ((double)J.aom[2 * cs + om] - (double)J.aom[2 * cs + op]) / tdx;
+    a3 -= ((double)J.aom[cs + om] - (double)J.aom[cs + op]) / tdx;
+  }
+  const double cf = A.st->s.cf;
+  a1 += cf * (double)J.S[e];
+  a2 += cf * (double)J.S[cs + e];
+  a3 += cf * (double)J.S[2 * cs + e];
+  J.aubcp[e] = (T)a1;  // the first terms of the step's adjoint of ubcp
+  J.aubcp[cs + e] = (T)a2;
+  J.aubcp[2 * cs + e] = (T)a3;
 }
 
 // ===========================================================================
@@ -431,14 +605,6 @@ __device__ double interp_g(const Sten& s, const T* f, double coef, double g[3]) 
   return acc;
 }
 template <typename T>
-__device__ double interp_v(const Sten& s, const T* f) {
-  double acc = 0.0;
-  for (int k = 0; k < 2; ++k)
-    for (int j = 0; j < 2; ++j)
-      for (int i = 0; i < 2; ++i) acc += (double)f[s.node(i, j, k)] * s.w(i, j, k);
-  return acc;
-}
-template <typename T>
 __device__ void scatter8(const Sten& s, T* f, double v) {
   for (int k = 0; k < 2; ++k)
     for (int j = 0; j < 2; ++j)
@@ -450,11 +616,35 @@ __device__ void add_grad(double* axc, int nbody, int m, double* g, double* sh) {
     for (int d = 0; d < 3; ++d) unsafeAtomicAdd(axc + (size_t)d * nbody + m, g[d]);
 }
 
+// value of the trilinear interpolant of f; its position gradient in g
+template <typename T>
+__device__ __forceinline__ double interp_vg(const Sten& s, const T* f, double g[3]) {
+  double acc = 0.0;
+  g[0] = g[1] = g[2] = 0.0;
+  const double x1 = s.z[0], y1 = s.z[1], z1 = s.z[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const double v = (double)f[s.node(i, j, k)];
+        const double X = i ? x1 : 1.0 - x1, Y = j ? y1 : 1.0 - y1, Z = k ? z1 : 1.0 - z1;
+        acc += v * X * Y * Z;
+        g[0] += v * (i ? 1.0 : -1.0) * Y * Z;
+        g[1] += v * X * (j ? 1.0 : -1.0) * Z;
+        g[2] += v * X * Y * (k ? 1.0 : -1.0);
+      }
+  return acc;
+}
+
 // calc_inner_reversal_forcing (boundary_condition.F90:1104-1479) transposed:
-// per active reversal pair, the adjoint of force_tmp from the spread nodes
-// (solid share, the force_dist redistribution, frmean), the position
-// gradient of the spreading weights and of the interpolations, and the
-// scatter into the adjoints of ubcp, nlbc, onlbc, diffn and ppr.
+// per active reversal pair, one pass of gathers gives force_tmp and the
+// position gradients of its interpolations; the adjoint of force_tmp comes
+// from the spread nodes (solid share, the force_dist redistribution,
+// frmean); the position gradient of the spreading weights; and the scatters:
+// h at the inner point into S, cf kappa P h at the outer point into aubcp,
+// c12 h at the pressure point into appr.
 template <typename T>
 __global__ void __launch_bounds__(256) k_adj_forcing_points(KArgs<T> A, Particles P, AdjArgs<T> J, int G) {
   __shared__ double sh[3 * 16];
@@ -484,9 +674,14 @@ __global__ void __launch_bounds__(256) k_adj_forcing_points(KArgs<T> A, Particle
     make_sten(so, xlo, A.nx, A.my, A.mz);
     make_sten(sp, xpi, A.nx, A.my, A.mz);
     make_sten(si, xli, A.nx, A.my, A.mz);
-    // the forward force_tmp of this pair
-    double ulo[3];
-    for (int n = 0; n < 3; ++n) ulo[n] = interp_v(so, A.ubcp + n * cs);
+    // force_tmp of this pair with the gradients of its interpolations:
+    // Gi[d] = d/dpos of [cf uli - c3 nll - c4 onll - c12 dfll + c12 ppll]_d,
+    // Go[d] = d/dpos of ulo_d
+    double ulo[3], Go[3][3], Gi[3][3], g[3];
+    for (int n = 0; n < 3; ++n) {
+      ulo[n] = interp_vg(so, A.ubcp + n * cs, g);
+      for (int e = 0; e < 3; ++e) Go[n][e] = g[e];
+    }
     const double w1 = P.angv[m], w2v = P.angv[P.nbody + m], w3 = P.angv[2 * P.nbody + m];
     const double crossp[3] = {w2v * nv[2] - w3 * nv[1], w3 * nv[0] - w1 * nv[2], w1 * nv[1] - w2v * nv[0]};
     double linvel[3], unmag = 0.0, ubnmag = 0.0;
@@ -504,20 +699,33 @@ __global__ void __launch_bounds__(256) k_adj_forcing_points(KArgs<T> A, Particle
       const double ubnorm = nv[d] * ubnmag, ubtang = linvel[d] - ubnorm;
       const double utangi = (ubtang * (ro - ri) - utang * (rb - ri)) / (ro - rb);
       const double unormi = rn * (ubnorm * (ro - ri) - unorm * (rb - ri)) / (ro - rb);
-      double f = cf * (interp_v(si, A.ubcp + d * cs) - (utangi + unormi));
-      f = f - c3 * interp_v(si, A.nlbc + d * cs) - c4 * interp_v(si, A.onlbc + d * cs);
-      f = f + c12 * (interp_v(sp, A.ppr + d * cs) - interp_v(si, A.diffn + d * cs));
+      double f = cf * (interp_vg(si, A.ubcp + d * cs, g) - (utangi + unormi));
+      for (int e = 0; e < 3; ++e) Gi[d][e] = cf * g[e];
+      f = f - c3 * interp_vg(si, A.nlbc + d * cs, g);
+      for (int e = 0; e < 3; ++e) Gi[d][e] -= c3 * g[e];
+      f = f - c4 * interp_vg(si, A.onlbc + d * cs, g);
+      for (int e = 0; e < 3; ++e) Gi[d][e] -= c4 * g[e];
+      const double pp = interp_vg(sp, A.ppr + d * cs, g);
+      for (int e = 0; e < 3; ++e) Gi[d][e] += c12 * g[e];
+      f = f + c12 * (pp - interp_vg(si, A.diffn + d * cs, g));
+      for (int e = 0; e < 3; ++e) Gi[d][e] -= c12 * g[e];
       ft[d] = f * K;
     }
     // spreading, transposed
     bool solid[8];
+    double fb[3][8];
     int count_so = 0;
     double wfl = 0.0, wall = 0.0;
+#pragma unroll
     for (int k = 0; k < 2; ++k)
+#pragma unroll
       for (int j = 0; j < 2; ++j)
+#pragma unroll
         for (int i = 0; i < 2; ++i) {
           const int q = (k * 2 + j) * 2 + i;
-          solid[q] = A.mask[si.node(i, j, k)] == 0;
+          const unsigned nd = si.node(i, j, k);
+          solid[q] = A.mask[nd] == 0;
+          for (int d = 0; d < 3; ++d) fb[d][q] = (double)J.afr[d * cs + nd];
           const double wq = si.w(i, j, k) / dx3;
           wall += wq;
           if (solid[q]) ++count_so;
@@ -526,28 +734,32 @@ __global__ void __launch_bounds__(256) k_adj_forcing_points(KArgs<T> A, Particle
     double ftb[3], Ad[3];
     for (int d = 0; d < 3; ++d) {
       double ssum = 0.0, direct = 0.0;
+#pragma unroll
       for (int k = 0; k < 2; ++k)
+#pragma unroll
         for (int j = 0; j < 2; ++j)
-          for (int i = 0; i < 2; ++i)
-            if (solid[(k * 2 + j) * 2 + i]) {
-              const double fb = (double)J.afr[d * cs + si.node(i, j, k)];
-              ssum += fb;
-              direct += si.w(i, j, k) / dx3 * fb;
+#pragma unroll
+          for (int i = 0; i < 2; ++i) {
+            const int q = (k * 2 + j) * 2 + i;
+            if (solid[q]) {
+              ssum += fb[d][q];
+              direct += si.w(i, j, k) / dx3 * fb[d][q];
             }
+          }
       Ad[d] = count_so > 0 ? ssum / (double)count_so : 0.0;
       ftb[d] = direct + (count_so > 0 ? wfl * Ad[d] : 0.0) + J.sc[6 + d] * wall;
     }
     {
       const double x1 = si.z[0], y1 = si.z[1], z1 = si.z[2];
+#pragma unroll
       for (int k = 0; k < 2; ++k)
+#pragma unroll
         for (int j = 0; j < 2; ++j)
+#pragma unroll
           for (int i = 0; i < 2; ++i) {
-            const bool so_ = solid[(k * 2 + j) * 2 + i];
+            const int q = (k * 2 + j) * 2 + i;
             double gq = 0.0;
-            for (int d = 0; d < 3; ++d) {
-              const double fb = so_ ? (double)J.afr[d * cs + si.node(i, j, k)] : Ad[d];
-              gq += ft[d] / dx3 * (fb + J.sc[6 + d]);
-            }
+            for (int d = 0; d < 3; ++d) gq += ft[d] / dx3 * ((solid[q] ? fb[d][q] : Ad[d]) + J.sc[6 + d]);
             const double X = i ? x1 : 1.0 - x1, Y = j ? y1 : 1.0 - y1, Z = k ? z1 : 1.0 - z1;
             gx[0] += gq * (i ? 1.0 : -1.0) * Y * Z;
             gx[1] += gq * X * (j ? 1.0 : -1.0) * Z;
@@ -559,23 +771,75 @@ __global__ void __launch_bounds__(256) k_adj_forcing_points(KArgs<T> A, Particle
     for (int d = 0; d < 3; ++d) h[d] = K * ftb[d];
     const double nh = nv[0] * h[0] + nv[1] * h[1] + nv[2] * h[2];
     for (int d = 0; d < 3; ++d) Ph[d] = (h[d] - nv[d] * nh) + rn * nv[d] * nh;
+    for (int d = 0; d < 3; ++d)
+      for (int e = 0; e < 3; ++e) gx[e] += h[d] * Gi[d][e] + cf * kap * Ph[d] * Go[d][e];
+    double* co = J.coef + ((size_t)m * P.nbnd + l) * 6;
     for (int d = 0; d < 3; ++d) {
-      const double a_i = cf * h[d], a_o = cf * kap * Ph[d];
-      interp_g(si, A.ubcp + d * cs, a_i, gx);
-      interp_g(si, A.nlbc + d * cs, -c3 * h[d], gx);
-      interp_g(si, A.onlbc + d * cs, -c4 * h[d], gx);
-      interp_g(si, A.diffn + d * cs, -c12 * h[d], gx);
-      interp_g(so, A.ubcp + d * cs, a_o, gx);
-      interp_g(sp, A.ppr + d * cs, c12 * h[d], gx);
-      scatter8(si, J.aubcp + d * cs, a_i);
-      scatter8(si, J.anlbc + d * cs, -c3 * h[d]);
-      scatter8(si, J.aonlbc + d * cs, -c4 * h[d]);
-      scatter8(si, J.adiffn + d * cs, -c12 * h[d]);
-      scatter8(so, J.aubcp + d * cs, a_o);
-      scatter8(sp, J.appr + d * cs, c12 * h[d]);
+      co[d] = h[d];
+      co[3 + d] = cf * kap * Ph[d];
     }
   }
   add_grad(J.axc, P.nbody, m, gx, sh);
+}
+
+// The spread of the per-point adjoints through node-sorted records, one
+// lane per target node (no atomics; each node's records in a fixed order).
+// two record sets in one launch: set 0 spreads h into S (inner points), set 1
+// cf kappa P h into aubcp (outer points), set 2 c12 h into appr (pressure
+// points)
+template <typename T>
+__global__ void k_adj_rec_gather3(AdjArgs<T> J, int k0, int k1, int b1, double c12, size_t cs) {
+  // workgroups [0, b1) take record set k0, the rest set k1
+  const int b = blockIdx.x;
+  const int q = b < b1 ? k0 : k1;
+  const int t = (b < b1 ? b : b - b1) * blockDim.x + threadIdx.x;
+  const AdjRecords& R = J.rec[q];
+  if (t >= R.ntarget) return;
+  const int off = q == 1 ? 3 : 0;
+  const double scale = q == 2 ? c12 : 1.0;
+  T* dst = q == 0 ? J.S : (q == 1 ? J.aubcp : J.appr);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int r = R.toff[t]; r < R.toff[t + 1]; ++r) {
+    const double* c = J.coef + (size_t)R.rpt[r] * 6 + off;
+    const double w = R.rw[r];
+    a0 += w * c[0];
+    a1 += w * c[1];
+    a2 += w * c[2];
+  }
+  const size_t n = (size_t)R.tnode[t];
+  dst[n] = (T)((double)dst[n] + scale * a0);
+  dst[cs + n] = (T)((double)dst[cs + n] + scale * a1);
+  dst[2 * cs + n] = (T)((double)dst[2 * cs + n] + scale * a2);
+}
+
+// (point, corner) records of one stencil kind for every active reversal pair
+__global__ void k_adj_records(Particles P, int kind, int nx, int my, int mz, unsigned int sentinel, unsigned int* key,
+                              int* rec, double* w) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= P.nbody * P.nbnd) return;
+  const int m = idx / P.nbnd, l = idx - m * P.nbnd;
+  const bool act = P.if_rev[idx] != 0;
+  const double r = kind == ADJ_REC_OUTER ? P.rado[m] : P.radi[m];
+  double x[3];
+  for (int n = 0; n < 3; ++n) x[n] = P.xc[n * P.nbody + m] + P.bnd[l * 3 + n] * r;
+  if (kind == ADJ_REC_PRES) x[0] -= 0.5;
+  Sten s;
+  make_sten(s, x, nx, my, mz);
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i) {
+        const size_t q = (size_t)idx * 8 + (k * 2 + j) * 2 + i;
+        key[q] = act ? s.node(i, j, k) : sentinel;
+        rec[q] = (int)q;
+        w[q] = s.w(i, j, k);
+      }
+}
+__global__ void k_adj_rec_gather_fields(int n, const int* rec_sorted, const double* w, int* rpt, double* rw) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int r = rec_sorted[t];
+  rpt[t] = r / 8;
+  rw[t] = w[r];
 }
 
 // calc_pres_visc_drag (boundary_condition.F90:1481-1645) transposed for the
@@ -620,18 +884,14 @@ __global__ void __launch_bounds__(256) k_adj_drag_points(KArgs<T> A, Particles P
 // host side: the launches of one adjoint step
 // ===========================================================================
 template <typename K>
-void set_lds(K kern, size_t bytes) {
+void set_lds(K kern, size_t bytes, bool capturing = false) {
+  if (capturing) return;  // set by the eager step that precedes any capture
   ADJCHK(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-}
-template <typename T>
-void zero(T* p, size_t n, hipStream_t s) {
-  k_adj_zero<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, n);
-  ADJCHK(hipGetLastError());
 }
 template <typename T, int KIND, int SH>
 void r2c_t(const KArgs<T>& A, const AdjArgs<T>& J, const AdjGeom& g, int ncomp) {
   auto kern = k_adj_r2c_t<T, KIND, SH>;
-  set_lds(kern, g.lds);
+  set_lds(kern, g.lds, g.capturing);
   kern<<<dim3(A.nx, ncomp), SH ? shape_nt(SH) : g.nt_plane, g.lds, g.stream>>>(A, J);
   ADJCHK(hipGetLastError());
 }
@@ -647,7 +907,7 @@ void r2c_t_any(const KArgs<T>& A, const AdjArgs<T>& J, const AdjGeom& g, int nco
 template <typename T, int SH>
 void c2r_t(const KArgs<T>& A, const AdjGeom& g, int ncomp, const T* src, C2<T>* dst, double* part, bool add) {
   auto kern = k_adj_c2r_t<T, SH>;
-  set_lds(kern, g.lds);
+  set_lds(kern, g.lds, g.capturing);
   kern<<<dim3(A.nx, ncomp), SH ? shape_nt(SH) : g.nt_plane, g.lds, g.stream>>>(A, src, dst, part, add ? 1 : 0);
   ADJCHK(hipGetLastError());
 }
@@ -661,7 +921,83 @@ void c2r_t_any(const KArgs<T>& A, const AdjGeom& g, int ncomp, const T* src, C2<
   }
 }
 
+// the tiled transposed velstep when the shape fits it (as the forward's)
+template <typename T, int L, int M>
+bool vel_adj_tiled(const KArgs<T>& A, const AdjArgs<T>& J, hipStream_t s, bool capturing) {
+  const int nx = A.nx;
+  if (nx % L != 0 || nx % 2 != 0) return false;
+  const int bd = M * (nx / L);
+  if (bd > 512 || nx / 2 > bd) return false;
+  const int ntiles = (A.my2 * A.mz + M - 1) / M;
+  const size_t lds = std::max(vel_lds_bytes(bd), (size_t)bd * sizeof(double));
+  auto kern = bd <= 256 ? k_adj_velstep_tiled<T, L, M, 256> : k_adj_velstep_tiled<T, L, M, 512>;
+  set_lds(kern, lds, capturing);
+  kern<<<ntiles + 1, bd, lds, s>>>(A, J, ntiles);
+  ADJCHK(hipGetLastError());
+  return true;
+}
+
 }  // namespace
+
+void adjoint_build_records(const Particles& P, int nx, int my, int mz, hipStream_t s, AdjRecords rec[3],
+                           std::vector<void*>& mem) {
+  auto dev = [&](size_t bytes) {
+    void* p = nullptr;
+    ADJCHK(hipMalloc(&p, bytes + 64));
+    mem.push_back(p);
+    return p;
+  };
+  const size_t nrec = (size_t)P.nbody * P.nbnd * 8, N = (size_t)nx * my * mz;
+  const unsigned int sentinel = (unsigned int)N;
+  int bits = 1;
+  while ((1ull << bits) <= (unsigned long long)N) ++bits;
+  unsigned int* key = (unsigned int*)dev(nrec * 4);
+  unsigned int* key_s = (unsigned int*)dev(nrec * 4);
+  int* recn = (int*)dev(nrec * 4);
+  int* rec_s = (int*)dev(nrec * 4);
+  double* w = (double*)dev(nrec * 8);
+  int* counts = (int*)dev((nrec + 1) * 4);
+  int* nruns = (int*)dev(4);
+  size_t tb1 = 0, tb2 = 0, tb3 = 0;
+  ADJCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb1, key, key_s, recn, rec_s, (int)nrec, 0, bits, s));
+  ADJCHK(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb2, key_s, key, counts, nruns, (int)nrec, s));
+  ADJCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, counts, recn, (int)nrec + 1, s));
+  void* tmp = dev(std::max(tb1, std::max(tb2, tb3)));
+  const size_t npts = (size_t)P.nbody * P.nbnd;
+  for (int kind = 0; kind < 3; ++kind) {
+    AdjRecords& R = rec[kind];
+    R.ntarget = 0;
+    if (nrec == 0) continue;
+    unsigned int* ukeys = (unsigned int*)dev((nrec + 1) * 4);
+    int* toff = (int*)dev((nrec + 2) * 4);
+    k_adj_records<<<(unsigned)((npts + 255) / 256), 256, 0, s>>>(P, kind, nx, my, mz, sentinel, key, recn, w);
+    ADJCHK(hipGetLastError());
+    ADJCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tb1, key, key_s, recn, rec_s, (int)nrec, 0, bits, s));
+    ADJCHK(hipcub::DeviceRunLengthEncode::Encode(tmp, tb2, key_s, ukeys, counts, nruns, (int)nrec, s));
+    int hruns = 0;
+    ADJCHK(hipMemcpyAsync(&hruns, nruns, sizeof(int), hipMemcpyDeviceToHost, s));
+    ADJCHK(hipStreamSynchronize(s));
+    unsigned int lastkey = 0;
+    if (hruns > 0) ADJCHK(hipMemcpy(&lastkey, ukeys + hruns - 1, sizeof(unsigned int), hipMemcpyDeviceToHost));
+    const int nt = hruns - (hruns > 0 && lastkey >= sentinel ? 1 : 0);
+    ADJCHK(hipcub::DeviceScan::ExclusiveSum(tmp, tb3, counts, toff, nt + 1, s));
+    int acc = 0;
+    ADJCHK(hipMemcpyAsync(&acc, toff + nt, sizeof(int), hipMemcpyDeviceToHost, s));
+    ADJCHK(hipStreamSynchronize(s));
+    int* rpt = (int*)dev((size_t)(acc + 1) * 4);
+    double* rw = (double*)dev((size_t)(acc + 1) * 8);
+    if (acc > 0) {
+      k_adj_rec_gather_fields<<<(acc + 255) / 256, 256, 0, s>>>(acc, rec_s, w, rpt, rw);
+      ADJCHK(hipGetLastError());
+    }
+    R.ntarget = nt;
+    R.tnode = (const int*)ukeys;
+    R.toff = toff;
+    R.rpt = rpt;
+    R.rw = rw;
+  }
+  ADJCHK(hipStreamSynchronize(s));
+}
 
 template <typename T>
 void adjoint_step_kernels(const KArgs<T>& A, const Particles& P, const AdjArgs<T>& J, const AdjGeom& g) {
@@ -669,13 +1005,36 @@ void adjoint_step_kernels(const KArgs<T>& A, const Particles& P, const AdjArgs<T
   const size_t E = (size_t)A.my2 * A.mz, N = (size_t)A.nx * A.my * A.mz;
   const size_t nspec = (size_t)A.nx * E;
   const unsigned ebl = (unsigned)((nspec + 255) / 256), pbl = (unsigned)((N + 255) / 256);
-  for (T* f : {J.aubcp, J.anlbc, J.aonlbc, J.adiffn, J.aom, J.appr}) zero(f, 3 * N, s);
-  zero(J.apbcp, N, s);
-  zero((T*)J.apnew, 2 * nspec, s);
+  const bool drag = J.wforce != nullptr && P.nbody > 0;
+  {
+    // aubcp and apnew are first written by k_adj_phys and k_adj_modes<PGRAD>
+    ZeroSet<T> z{};
+    z.p[0] = J.S;
+    z.p[1] = J.appr;
+    z.n[0] = z.n[1] = 3 * N;
+    z.k = 2;
+    if (drag) {
+      z.p[2] = J.aom;
+      z.p[3] = J.apbcp;
+      z.n[2] = 3 * N;
+      z.n[3] = N;
+      z.k = 4;
+    }
+    k_adj_zero_set<T><<<dim3((unsigned)((3 * N + 255) / 256), z.k), 256, 0, s>>>(z);
+    ADJCHK(hipGetLastError());
+  }
   // velstep
   const int nm = (int)E;
-  k_adj_velstep<T><<<(nm + 63) / 64, 64, 0, s>>>(A, J, nm);
-  ADJCHK(hipGetLastError());
+  bool tiled;
+  const bool cap = g.capturing;
+  if constexpr (sizeof(T) == 4)
+    tiled = vel_adj_tiled<T, 8, 16>(A, J, s, cap) || vel_adj_tiled<T, 4, 16>(A, J, s, cap);
+  else
+    tiled = vel_adj_tiled<T, 4, 8>(A, J, s, cap) || vel_adj_tiled<T, 2, 8>(A, J, s, cap);
+  if (!tiled) {
+    k_adj_velstep<T><<<(nm + 63) / 64, 64, 0, s>>>(A, J, nm);
+    ADJCHK(hipGetLastError());
+  }
   // umean update and the mean of fr; the drag's mpg term
   k_adj_scal<T><<<1, 256, 0, s>>>(A, J, P.nbody, P.rad);
   ADJCHK(hipGetLastError());
@@ -687,7 +1046,12 @@ void adjoint_step_kernels(const KArgs<T>& A, const Particles& P, const AdjArgs<T
   if (P.nbody > 0) {
     k_adj_forcing_points<T><<<nbg, 256, 0, s>>>(A, P, J, G);
     ADJCHK(hipGetLastError());
-    if (J.wforce) {
+    const int bi = (J.rec[ADJ_REC_INNER].ntarget + 255) / 256, bp = (J.rec[ADJ_REC_PRES].ntarget + 255) / 256;
+    if (bi + bp > 0) {
+      k_adj_rec_gather3<T><<<bi + bp, 256, 0, s>>>(J, ADJ_REC_INNER, ADJ_REC_PRES, bi, g.c12, N);
+      ADJCHK(hipGetLastError());
+    }
+    if (drag) {
       k_adj_drag_points<T><<<nbg, 256, 0, s>>>(A, P, J, G);
       ADJCHK(hipGetLastError());
     }
@@ -699,25 +1063,31 @@ void adjoint_step_kernels(const KArgs<T>& A, const Particles& P, const AdjArgs<T
   k_adj_modes<T, ADJ_PGRAD><<<ebl, 256, 0, s>>>(A, J);
   ADJCHK(hipGetLastError());
   // compute_omega -> u (its x-differences of ubcp go with the products below)
-  c2r_t_any<T>(A, g, 3, J.aom, J.Z, nullptr, false);
-  k_adj_modes<T, ADJ_OMEGA><<<ebl, 256, 0, s>>>(A, J);
+  if (drag) {
+    c2r_t_any<T>(A, g, 3, J.aom, J.Z, nullptr, false);
+    k_adj_modes<T, ADJ_OMEGA><<<ebl, 256, 0, s>>>(A, J);
+    ADJCHK(hipGetLastError());
+  }
+  // nlbc, onlbc, diffn at the inner points: -> nl, onl, u
+  c2r_t_any<T>(A, g, 3, J.S, J.Z, nullptr, false);
+  k_adj_modes<T, ADJ_SFIELD><<<ebl, 256, 0, s>>>(A, J);
   ADJCHK(hipGetLastError());
-  // calc_visc -> u
-  c2r_t_any<T>(A, g, 3, J.adiffn, J.Z, nullptr, false);
-  k_adj_modes<T, ADJ_VISC><<<ebl, 256, 0, s>>>(A, J);
-  ADJCHK(hipGetLastError());
-  // nlbc = c2r(nl), onlbc = c2r(onl)
-  c2r_t_any<T>(A, g, 3, J.anlbc, J.anl, nullptr, true);
-  c2r_t_any<T>(A, g, 3, J.aonlbc, J.anlp, nullptr, true);
   // form_nl and the sign of nonlinear
   r2c_t_any<T, ADJ_NLQ>(A, J, g, 8);
-  k_adj_phys<T><<<pbl, 256, 0, s>>>(A, J);
+  k_adj_phys<T><<<pbl, 256, 0, s>>>(A, J, drag ? 1 : 0);
   ADJCHK(hipGetLastError());
+  {
+    const int bo = (J.rec[ADJ_REC_OUTER].ntarget + 255) / 256;
+    if (P.nbody > 0 && bo > 0) {
+      k_adj_rec_gather3<T><<<bo, 256, 0, s>>>(J, ADJ_REC_OUTER, ADJ_REC_OUTER, bo, g.c12, N);
+      ADJCHK(hipGetLastError());
+    }
+  }
   // ubcp = c2r(u) + umean, pbcp = c2r(p)
   c2r_t_any<T>(A, g, 3, J.aubcp, J.aunew, J.part, true);
   k_adj_sum_part<<<1, 256, 0, s>>>(J.part, A.nx, J.sc, 0);
   ADJCHK(hipGetLastError());
-  c2r_t_any<T>(A, g, 1, J.apbcp, J.apnew, nullptr, true);
+  if (drag) c2r_t_any<T>(A, g, 1, J.apbcp, J.apnew, nullptr, true);
 }
 
 template void adjoint_step_kernels<float>(const KArgs<float>&, const Particles&, const AdjArgs<float>&, const AdjGeom&);

@@ -29,6 +29,10 @@ __global__ void k_fill_u8(uint8_t* p, size_t n, uint8_t v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+__global__ void k_zero_f64(double* p, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
 
 // grid_nodes_insphere (dependent_functions.F90:783-963): one workgroup per
 // body walks its bounding box; duplicate solid flags are counted as the
@@ -145,37 +149,60 @@ __global__ void k_count_active(const uint8_t* if_rev, int nbnd, int* nrpr_active
 // Spreading records: for each active reversal pair and each of the 8 corners
 // of its inner-point stencil, the grid node it deposits on (sentinel when it
 // deposits nowhere), its weight and whether the redistributed fluid share is
-// added (boundary_condition.F90:1379-1462).
+// added (boundary_condition.F90:1379-1462).  With dobnd, each body's surface
+// points come first (calc_bnd_forcing, all eight nodes, no redistribution):
+// record ((m*2 + kind)*nbnd + l)*8 + corner, kind 0 surface, 1 reversal, so
+// that the stable sort keeps the reference's (body, routine, point, corner)
+// order within a node.  rec_point maps a record to its row of the per-point
+// forces: reversal pairs first (m*nbnd + l), surface points after them.
+__device__ __host__ inline int rec_point(int r8, int nbody, int nbnd, int dobnd) {
+  if (!dobnd) return r8;
+  const int m = r8 / (2 * nbnd), rest = r8 - m * 2 * nbnd, kind = rest / nbnd, l = rest - kind * nbnd;
+  return kind ? m * nbnd + l : nbody * nbnd + m * nbnd + l;
+}
 __global__ void k_spread_records(Particles P, const uint8_t* mask, int nx, int my, int mz, int frmeanfluid,
-                                 unsigned int sentinel, unsigned int* key, int* rec, double* w, uint8_t* dist) {
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= P.nbody * P.nbnd) return;
-  int m = idx / P.nbnd, l = idx - m * P.nbnd;
-  bool act = P.if_rev[idx] != 0;
-  double xli[3];
-  for (int n = 0; n < 3; ++n) xli[n] = P.xc[n * P.nbody + m] + P.bnd[l * 3 + n] * P.radi[m];
+                                 int dorpr, int dobnd, unsigned int sentinel, unsigned int* key, int* rec, double* w,
+                                 uint8_t* dist) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int npk = P.nbody * P.nbnd;
+  if (idx >= (dobnd ? 2 * npk : npk)) return;
+  int m, l, kind;
+  if (dobnd) {
+    m = idx / (2 * P.nbnd);
+    const int rest = idx - m * 2 * P.nbnd;
+    kind = rest / P.nbnd;
+    l = rest - kind * P.nbnd;
+  } else {
+    m = idx / P.nbnd;
+    l = idx - m * P.nbnd;
+    kind = 1;
+  }
+  const bool act = kind == 0 || (dorpr && P.if_rev[(size_t)m * P.nbnd + l] != 0);
+  const double r = kind ? P.radi[m] : P.rad[m];
+  double x[3];
+  for (int n = 0; n < 3; ++n) x[n] = P.xc[n * P.nbody + m] + P.bnd[l * 3 + n] * r;
   Sten s;
-  make_sten(s, xli, nx, my, mz);
+  make_sten(s, x, nx, my, mz);
   for (int k = 0; k < 2; ++k)
     for (int j = 0; j < 2; ++j)
       for (int i = 0; i < 2; ++i) {
         int cidx = (k * 2 + j) * 2 + i;
-        size_t r = (size_t)idx * 8 + cidx;
+        size_t q = (size_t)idx * 8 + cidx;
         unsigned int node = s.node(i, j, k);
         bool fl = mask[node] != 0;
-        bool dep = act && (frmeanfluid ? true : !fl);
-        key[r] = dep ? node : sentinel;
-        rec[r] = (int)r;
-        w[r] = s.w(i, j, k);
-        dist[r] = (uint8_t)(frmeanfluid ? 0 : 1);
+        bool dep = act && (kind == 0 || frmeanfluid ? true : !fl);
+        key[q] = dep ? node : sentinel;
+        rec[q] = (int)q;
+        w[q] = s.w(i, j, k);
+        dist[q] = (uint8_t)(kind == 0 || frmeanfluid ? 0 : 1);
       }
 }
-__global__ void k_gather_records(int n, const int* rec_sorted, const double* w, const uint8_t* dist, int* rpt,
-                                 double* rw, uint8_t* rd) {
+__global__ void k_gather_records(int n, const int* rec_sorted, const double* w, const uint8_t* dist, int nbody,
+                                 int nbnd, int dobnd, int* rpt, double* rw, uint8_t* rd) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   int r = rec_sorted[t];
-  rpt[t] = r / 8;
+  rpt[t] = rec_point(r / 8, nbody, nbnd, dobnd);
   rw[t] = w[r];
   rd[t] = dist[r];
 }
@@ -907,6 +934,8 @@ struct Impl : Base {
     drop_graphs();
     for (auto& c : ck_stack) (void)hipFree(c.first);
     for (void* q : ck_pool) (void)hipFree(q);
+    for (void* q : rec_blocks) (void)hipFree(q);
+    free_adj_records();
     for (auto& q : pallocs) (void)hipFree(q.first);
     drain_pool();
     for (void* q : allocs) (void)hipFree(q);
@@ -1053,6 +1082,7 @@ struct Impl : Base {
     if (nb > 0 && !have_points) throw PibFail(PIB_ERR_STATE, "set_points must precede set_particles");
     HIPCHK(hipStreamSynchronize(stream));
     drop_graphs();
+    ++part_gen;
     // the per-particle force sums of the step live on when the same bed
     // moves: the reference's grid_nodes_insphere leaves them alone
     std::vector<double> keep;
@@ -1142,8 +1172,9 @@ struct Impl : Base {
       HIPCHK(hipMemcpyAsync(hn.data(), nrpr, nb * sizeof(int), hipMemcpyDeviceToHost, stream));
       const size_t G = (nbnd + 255) / 256;
       drag_part = alloc<double>((size_t)nb * G * 9, true);
-      frc_part = alloc<double>((size_t)nb * G * 6, true);
-      frc_pts = alloc<double>(npts * 6, true);
+      const size_t kinds = cfg.dobnd ? 2 : 1;  // reversal pairs, and the surface points of calc_bnd_forcing
+      frc_part = alloc<double>(kinds * nb * G * 6, true);
+      frc_pts = alloc<double>(kinds * npts * 6, true);
       build_spreading();
     }
     HIPCHK(hipStreamSynchronize(stream));
@@ -1166,7 +1197,8 @@ struct Impl : Base {
     have_particles = true;
   }
   void build_spreading() {
-    size_t nrec = (size_t)nbody * nbnd * 8;
+    const int dobnd = cfg.dobnd ? 1 : 0;
+    size_t nrec = (size_t)nbody * nbnd * 8 * (dobnd ? 2 : 1);
     size_t N = (size_t)nx * my * mz;
     unsigned int sentinel = (unsigned int)N;
     int bits = 1;
@@ -1178,10 +1210,10 @@ struct Impl : Base {
     double* w = alloc<double>(nrec, true);
     uint8_t* dist = alloc<uint8_t>(nrec, true);
     Particles P = parts();
-    size_t npts = (size_t)nbody * nbnd;
+    size_t npts = (size_t)nbody * nbnd * (dobnd ? 2 : 1);
     k_spread_records<<<(unsigned)((npts + 255) / 256), 256, 0, stream>>>(P, mask, nx, my, mz,
-                                                                          cfg.include_frmeanfluid, sentinel, key,
-                                                                          rec, w, dist);
+                                                                          cfg.include_frmeanfluid, cfg.dorpr, dobnd,
+                                                                          sentinel, key, rec, w, dist);
     HIPCHK(hipGetLastError());
     size_t tmp_bytes = 0;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key, key_s, rec, rec_s, (int)nrec, 0, bits,
@@ -1236,7 +1268,8 @@ struct Impl : Base {
     rw = alloc<double>(nrec + 1, true);
     rdist = alloc<uint8_t>(nrec + 1, true);
     if (acc > 0) {
-      k_gather_records<<<(acc + 255) / 256, 256, 0, stream>>>(acc, rec_s, w, dist, rpt, rw, rdist);
+      k_gather_records<<<(acc + 255) / 256, 256, 0, stream>>>(acc, rec_s, w, dist, nbody, nbnd, dobnd, rpt, rw,
+                                                                rdist);
       HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(stream));
@@ -1327,13 +1360,25 @@ struct Impl : Base {
     const unsigned nbg = (unsigned)((nbody * G + 7) / 8 * 8);
     c2r<C2R_PGRAD>(rks, 3);
     join_branches();
-    if (cfg.dorpr) {
-      kb("forcing_points");
-      k_forcing_points<T><<<nbg, 256, 0, stream>>>(args(rks), P, frc_pts, frc_part, G);
-      HIPCHK(hipGetLastError());
-      ke();
+    if (cfg.dorpr || cfg.dobnd) {
+      if (cfg.dorpr) {
+        kb("forcing_points");
+        k_forcing_points<T><<<nbg, 256, 0, stream>>>(args(rks), P, frc_pts, frc_part, G);
+        HIPCHK(hipGetLastError());
+        ke();
+      } else {
+        // the reversal rows of the partials and forces stay zero
+        k_zero_f64<<<(unsigned)((nbody * G * 6 + 255) / 256), 256, 0, stream>>>(frc_part, (size_t)nbody * G * 6);
+        HIPCHK(hipGetLastError());
+      }
+      if (cfg.dobnd) {
+        kb("bnd_points");
+        k_bnd_points<T><<<nbg, 256, 0, stream>>>(args(rks), P, frc_pts, frc_part, G);
+        HIPCHK(hipGetLastError());
+        ke();
+      }
       kb("reduce_frmean");
-      k_reduce_frmean<<<1, SCAL_T, 0, stream>>>(dst, frc_part, G * nbody);
+      k_reduce_frmean<<<1, SCAL_T, 0, stream>>>(dst, frc_part, G * nbody * (cfg.dobnd ? 2 : 1));
       HIPCHK(hipGetLastError());
       ke();
       if (ntarget > 0) {
@@ -1422,8 +1467,17 @@ struct Impl : Base {
         (void)hipGraphExecDestroy(g);
         g = nullptr;
       }
+    for (auto& g : agexec)
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+    for (auto& kv : agexec_rec) (void)hipGraphExecDestroy(kv.second);
+    agexec_rec.clear();
+    adj_eager = 0;
   }
   void eager_step(bool rec) {
+    ++state_gen;
     for (int rks = 1; rks <= itrmax; ++rks) {
       bool r = rec && rks == itrmax;
       if (r) HIPCHK(hipEventRecord(ev[0], stream));
@@ -1468,6 +1522,7 @@ struct Impl : Base {
       nlbc_valid = v0;
     }
     HIPCHK(hipGraphLaunch(gexec[par], stream));
+    ++state_gen;
     std::swap(nl, onl);
     std::swap(nlbc, onlbc);
     nlbc_valid = true;
@@ -1713,6 +1768,7 @@ struct Impl : Base {
     heat_ratio = heat_ratio_new;
   }
   void set_field(int f, const double* h) {
+    ++state_gen;
     if (f >= PIB_F_PHI && f <= PIB_F_FFPHI) return put_phi(scalar_field(f), h);
     switch (f) {
       case PIB_F_U: put_spec(u, 3, h, true); break;
@@ -1765,6 +1821,7 @@ struct Impl : Base {
     }
   }
   void set_scalars(const pib_scalars* s) {
+    ++state_gen;
     DevState h;
     HIPCHK(hipMemcpy(&h, dst, sizeof h, hipMemcpyDeviceToHost));
     double cs = h.s.count_solid, cf = h.s.count_fluid, mv = h.s.maxvolfrac;
@@ -1858,9 +1915,19 @@ struct Impl : Base {
 
   // ---------------- adjoint of the fixed-bed step ----------------
   AdjArgs<T> J{};
+  hipGraphExec_t agexec[4] = {};  // (nl buffer, adjoint buffer) assignments
+  std::map<int, hipGraphExec_t> agexec_rec;  // steps on recorded fields: (record depth, adjoint buffer)
+  C2<T>* adj_au0 = nullptr;  // the au buffer of adjoint assignment 0
+  int adj_eager = 0;
   bool have_adj = false;
   int adj_nbody = -1;
   double* wdev = nullptr;
+  int part_gen = 0, adj_gen = -1;  // particle sets seen, and the one the adjoint's records are of
+  std::vector<void*> adj_rec_mem;
+  void free_adj_records() {
+    for (void* q : adj_rec_mem) (void)hipFree(q);
+    adj_rec_mem.clear();
+  }
   void adjoint_begin() {
     if (!have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
     const size_t E = (size_t)my2 * mz, nsc = (size_t)3 * nx * E, np = (size_t)nx * E;
@@ -1874,7 +1941,7 @@ struct Impl : Base {
       J.Z = alloc<C2<T>>(nsc);
       J.ap = alloc<C2<T>>(np);
       J.apnew = alloc<C2<T>>(np);
-      for (T** f : {&J.aubcp, &J.anlbc, &J.aonlbc, &J.adiffn, &J.aom, &J.afr, &J.appr}) *f = alloc<T>(3 * npr);
+      for (T** f : {&J.aubcp, &J.aom, &J.afr, &J.appr, &J.S}) *f = alloc<T>(3 * npr);
       J.apbcp = alloc<T>(npr);
       J.Q = alloc<T>(8 * npr);
       J.sc = alloc<double>(12);
@@ -1885,8 +1952,16 @@ struct Impl : Base {
     if (adj_nbody != nbody) {
       J.axc = alloc<double>((size_t)3 * std::max(nbody, 1));
       wdev = alloc<double>((size_t)3 * std::max(nbody, 1));
+      J.coef = alloc<double>((size_t)6 * std::max(nbody * nbnd, 1));
       adj_nbody = nbody;
     }
+    if (adj_gen != part_gen) {
+      HIPCHK(hipStreamSynchronize(stream));
+      free_adj_records();
+      adjoint_build_records(parts(), nx, my, mz, stream, J.rec, adj_rec_mem);
+      adj_gen = part_gen;
+    }
+    rec_depth = 0;
     HIPCHK(hipMemsetAsync(J.au, 0, nsc * sizeof(C2<T>), stream));
     HIPCHK(hipMemsetAsync(J.anl, 0, nsc * sizeof(C2<T>), stream));
     HIPCHK(hipMemsetAsync(J.ap, 0, np * sizeof(C2<T>), stream));
@@ -1899,25 +1974,114 @@ struct Impl : Base {
   }
   void adjoint_step(const double* wforce) {
     need_adj();
-    if (adj_nbody != nbody) throw PibFail(PIB_ERR_STATE, "particle set changed since pib_adjoint_begin");
+    if (adj_nbody != nbody || adj_gen != part_gen)
+      throw PibFail(PIB_ERR_STATE, "particle set changed since pib_adjoint_begin");
     const pib_config& c = cfg;
     if (c.rk3 || c.adaptive || !c.set_mpg || c.discr_scheme != PIB_SCHEME_CENTER || !c.dorpr || c.dobnd ||
         c.include_frmeanfluid || c.move_particles)
       throw PibFail(PIB_ERR_UNSUPPORTED, "adjoint: rk3=0, adaptive=0, set_mpg, center, dorpr, fixed bed only");
-    // the step's forward fields
-    compute_new_timestep(1);
-    nonlinear_timed(1, false, 0);
+    const bool drag = wforce != nullptr && nbody > 0;
     J.wforce = nullptr;
-    if (wforce && nbody > 0) {
+    if (drag) {
       HIPCHK(hipMemcpyAsync(wdev, wforce, (size_t)3 * nbody * sizeof(double), hipMemcpyHostToDevice, stream));
       J.wforce = wdev;
     }
-    AdjGeom g{cur_sh(), nt_plane, plane_lds(), nbody, nbnd, stream};
-    adjoint_step_kernels<T>(args(1), parts(), J, g);
+    // without a force weight, the recorded fields of the step when there are
+    // any: the kernels read them where they were recorded
+    const void* rec = nullptr;
+    if (!drag && rec_depth > 0) rec = rec_blocks[--rec_depth];
+    // steps without a force weight replay from HIP graphs, one per (nl buffer,
+    // adjoint buffer) assignment, or per (record, adjoint buffer) on recorded
+    // fields, once an eager adjoint step has set every kernel attribute
+    const bool graph = use_graph && !drag && !ktime && adj_eager >= 1 && (rec || nlbc_valid);
+    if (!graph) {
+      adjoint_body(drag, rec);
+      ++adj_eager;
+      return;
+    }
+    if (!nl_a) nl_a = nl;
+    if (!adj_au0) adj_au0 = J.au;
+    const int apar = J.au == adj_au0 ? 0 : 1;
+    hipGraphExec_t* slot = rec ? &agexec_rec[4 * rec_depth + 2 * (rec_owns[rec_depth] ? 1 : 0) + apar]
+                               : &agexec[(nl == nl_a ? 0 : 1) + 2 * apar];
+    if (!*slot) {
+      C2<T>*nl0 = nl, *onl0 = onl;
+      T *nb0 = nlbc, *onb0 = onlbc;
+      const bool v0 = nlbc_valid;
+      const AdjArgs<T> J0 = J;
+      auto restore = [&] {
+        nl = nl0;
+        onl = onl0;
+        nlbc = nb0;
+        onlbc = onb0;
+        nlbc_valid = v0;
+        J = J0;
+      };
+      hipGraph_t g;
+      HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      capturing = true;
+      try {
+        adjoint_body(false, rec);
+      } catch (...) {
+        capturing = false;
+        (void)hipStreamEndCapture(stream, &g);
+        restore();
+        throw;
+      }
+      capturing = false;
+      HIPCHK(hipStreamEndCapture(stream, &g));
+      HIPCHK(hipGraphInstantiate(slot, g, nullptr, nullptr, 0));
+      HIPCHK(hipGraphDestroy(g));
+      restore();
+    }
+    HIPCHK(hipGraphLaunch(*slot, stream));
+    if (!rec) {
+      std::swap(nl, onl);
+      std::swap(nlbc, onlbc);
+      nlbc_valid = true;
+    }
     std::swap(J.au, J.aunew);
     std::swap(J.ap, J.apnew);
     std::swap(J.anl, J.anlp);
-    HIPCHK(hipStreamSynchronize(stream));
+  }
+  // the work of one adjoint step on the stream: the step's forward fields
+  // that the adjoint reads -- compute_new_timestep (ubcp, pbcp, t, cf),
+  // nonlinear's onl <- nl, form_nl and nlbc, and bcset's diffn, ppr (and
+  // omega when the step's drag is weighed); the forward drag, forcing and ff
+  // are not needed -- then the transposed kernels.  rec: a record block whose
+  // fields and device state the kernels read instead.
+  void adjoint_body(bool drag, const void* rec) {
+    if (!rec) {
+      compute_new_timestep(1);
+      std::swap(nl, onl);
+      std::swap(nlbc, onlbc);
+      const bool onlbc_ok = nlbc_valid;
+      form_nl(1);
+      c2r<C2R_NLBC>(1, 3);
+      if (!onlbc_ok) c2r<C2R_ONLBC>(1, 3);
+      nlbc_valid = true;
+      if (nbody > 0) {
+        if (drag) c2r_pair<C2R_VISC, 3, C2R_OMEGA, 3>(1, "c2r_visc_omega");
+        else c2r<C2R_VISC>(1, 3);
+        c2r<C2R_PGRAD>(1, 3);
+      }
+    }
+    KArgs<T> A = args(1);
+    if (rec) {
+      const size_t cs3 = (size_t)3 * nx * my * mz;
+      T* b = (T*)rec;
+      A.ubcp = b;
+      A.nlbc = b + cs3;
+      A.onlbc = rec_owns[rec_depth] ? b + 2 * cs3 : (T*)rec_blocks[rec_depth - 1] + cs3;
+      A.diffn = b + 3 * cs3;
+      A.ppr = b + 4 * cs3;
+      A.st = (DevState*)(b + 5 * cs3);
+    }
+    AdjGeom g{cur_sh(), nt_plane, plane_lds(), nbody, nbnd, stream, coef[0][0] + coef[0][1], capturing};
+    adjoint_step_kernels<T>(A, parts(), J, g);
+    std::swap(J.au, J.aunew);
+    std::swap(J.ap, J.apnew);
+    std::swap(J.anl, J.anlp);
   }
   void adjoint_get(double* dxc, double* dmpg, double* dum) {
     need_adj();
@@ -1970,6 +2134,68 @@ struct Impl : Base {
       HIPCHK(hipMemcpyAsync(b + off[q], src[q], part_bytes(q), hipMemcpyDeviceToDevice, stream));
     ck_stack.push_back({m, nlbc_valid});
   }
+  // one whole step, replayed from the step graphs once they exist
+  void one_step() {
+    if (use_graph && !ktime && eager_steps >= 2 && nlbc_valid) {
+      graph_step();
+      return;
+    }
+    eager_step(false);
+    ++eager_steps;
+  }
+  void checkpoint_steps(int nsteps) {
+    if (!have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
+    for (int n = 0; n < nsteps; ++n) {
+      checkpoint_push();
+      one_step();
+    }
+  }
+  // ---- the recorded forward fields of steps to be reversed ----
+  // After a whole step, ubcp, nlbc, onlbc, diffn and ppr still hold that
+  // step's values (velstep writes only u and p), together with the step's t,
+  // dt and cf in the device state: what the adjoint of the step reads
+  // besides the mask and the bed.  adjoint_record runs steps and keeps those
+  // fields; adjoint_step then takes them instead of recomputing them.
+  // record blocks by stack depth (a segment's records are pushed and taken
+  // in stack order, so a depth's block always holds the same step position
+  // and the graphs of steps on recorded fields stay valid); each block is
+  // the five 3-component fields, then the device state
+  std::vector<void*> rec_blocks;
+  std::vector<bool> rec_owns;  // the record holds its own onlbc
+  int rec_depth = 0;
+  // state_gen counts whatever changes the forward state (steps, restores,
+  // sets); rec_gen is its value after the last recorded step
+  unsigned long long state_gen = 0, rec_gen = ~0ull;
+  size_t rec_bytes() const {
+    return (size_t)15 * nx * my * mz * sizeof(T) + sizeof(DevState);
+  }
+  void adjoint_record(int nsteps) {
+    need_adj();
+    if (nbody == 0) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
+    const size_t cs3 = (size_t)3 * nx * my * mz;
+    for (int n = 0; n < nsteps; ++n) {
+      one_step();
+      if (rec_depth == (int)rec_blocks.size()) {
+        void* m = nullptr;
+        HIPCHK(hipMalloc(&m, rec_bytes()));
+        rec_blocks.push_back(m);
+      }
+      // a step's onlbc is the nlbc of the step before it, which the record
+      // below this one holds when that record is of the step just before
+      // (nothing ran or restored the state since); else it keeps its own
+      const bool own_onlbc = rec_depth == 0 || state_gen != rec_gen + 1;
+      rec_gen = state_gen;
+      if ((int)rec_owns.size() <= rec_depth) rec_owns.resize(rec_depth + 1);
+      rec_owns[rec_depth] = own_onlbc;
+      char* b = (char*)rec_blocks[rec_depth++];
+      T* f[5] = {ubcp, nlbc, onlbc, diffn, ppr};
+      for (int q = 0; q < 5; ++q)
+        if (q != 2 || own_onlbc)
+          HIPCHK(hipMemcpyAsync(b + (size_t)q * cs3 * sizeof(T), f[q], cs3 * sizeof(T), hipMemcpyDeviceToDevice,
+                                stream));
+      HIPCHK(hipMemcpyAsync(b + 5 * cs3 * sizeof(T), dst, sizeof(DevState), hipMemcpyDeviceToDevice, stream));
+    }
+  }
   void checkpoint_pop() {
     if (ck_stack.empty()) throw PibFail(PIB_ERR_STATE, "checkpoint stack is empty");
     size_t off[6];
@@ -1981,8 +2207,8 @@ struct Impl : Base {
     for (int q = 0; q < 5; ++q)
       HIPCHK(hipMemcpyAsync(dstp[q], b + off[q], part_bytes(q), hipMemcpyDeviceToDevice, stream));
     nlbc_valid = top.second;
-    ck_pool.push_back(top.first);
-    HIPCHK(hipStreamSynchronize(stream));
+    ++state_gen;
+    ck_pool.push_back(top.first);  // the copies out of it precede any reuse on the stream
   }
 };
 
@@ -2061,7 +2287,6 @@ int pib_create(const pib_config* cfg, void** ctx) {
   if (!cfg || !ctx) return PIB_ERR_ARG;
   *ctx = nullptr;
   if (cfg->nx < 4 || cfg->my < 4 || cfg->mz < 4 || (cfg->my & 1) || (cfg->mz & 1)) return PIB_ERR_ARG;
-  if (cfg->dobnd) return PIB_ERR_UNSUPPORTED;
   if (cfg->precision != PIB_FP64 && cfg->precision != PIB_FP32_BULK) return PIB_ERR_ARG;
   Ctx* c = new Ctx{cfg->precision, nullptr, ""};
   int st = guard(c, [&] {
@@ -2184,11 +2409,11 @@ int pib_adjoint_begin(void* ctx) {
   return dispatch(ctx, [&](auto& I) { I.adjoint_begin(); });
 }
 int pib_adjoint_step(void* ctx, const double* wforce) {
-  int st = dispatch(ctx, [&](auto& I) { I.adjoint_step(wforce); });
-  return finish_status(ctx, st);
+  return dispatch(ctx, [&](auto& I) { I.adjoint_step(wforce); });
 }
 int pib_adjoint_get(void* ctx, double* dxc, double* dmpg, double* dumean) {
-  return dispatch(ctx, [&](auto& I) { I.adjoint_get(dxc, dmpg, dumean); });
+  int st = dispatch(ctx, [&](auto& I) { I.adjoint_get(dxc, dmpg, dumean); });
+  return finish_status(ctx, st);
 }
 int pib_adjoint_set_umean(void* ctx, const double* dumean) {
   if (!dumean) return PIB_ERR_ARG;
@@ -2199,6 +2424,14 @@ int pib_checkpoint_push(void* ctx) {
 }
 int pib_checkpoint_pop(void* ctx) {
   return dispatch(ctx, [&](auto& I) { I.checkpoint_pop(); });
+}
+int pib_adjoint_record(void* ctx, int nsteps) {
+  if (nsteps < 0) return PIB_ERR_ARG;
+  return dispatch(ctx, [&](auto& I) { I.adjoint_record(nsteps); });
+}
+int pib_checkpoint_steps(void* ctx, int nsteps) {
+  if (nsteps < 0) return PIB_ERR_ARG;
+  return dispatch(ctx, [&](auto& I) { I.checkpoint_steps(nsteps); });
 }
 int pib_step_async(void* ctx, int nsteps) {
   if (nsteps < 0) return PIB_ERR_ARG;

@@ -896,6 +896,74 @@ __device__ void forcing_point(const KArgs<T>& A, const Particles& P, int m, int 
   }
 }
 
+// calc_bnd_forcing (boundary_condition.F90:902-1102): the direct forcing at
+// every surface point, interpolated as at an inner reversal point and
+// spread to all eight nodes of its stencil (fluid or solid, no
+// redistribution).  Work item (m, g) as above; out[(nbody*nbnd + m*nbnd + l)*6
+// + 0..2] force_tmp, [3..5] zero; the workgroup sums of the frmean /
+// frmeanfluid parts go to fpart[(nbody*G + m*G + g)*6 + 0..5].  The
+// rotation term uses the point's own normal cd (the reference's stale SNORM
+// gives the same for angv = 0, the case of every reference run).
+template <typename T>
+__global__ void __launch_bounds__(256) k_bnd_points(KArgs<T> A, Particles P, double* out, double* fpart, int G) {
+  __shared__ double sh[6 * 16];
+  const int w = xcd_linear(blockIdx.x, gridDim.x);
+  const int m = w / G, g = w - m * G;
+  if (m >= P.nbody) return;  // padding of the grid to a multiple of 8
+  const int l = g * blockDim.x + threadIdx.x;
+  double fs[6] = {0, 0, 0, 0, 0, 0};
+  if (l < P.nbnd) {
+    const DevState* st = A.st;
+    const int rks = A.rks;
+    const double c1 = st->coef[rks - 1][0], c2 = st->coef[rks - 1][1], c3 = st->coef[rks - 1][2],
+                 c4 = st->coef[rks - 1][3];
+    const double rb = P.rad[m];
+    double xl[3];
+    for (int n = 0; n < 3; ++n) xl[n] = P.xc[n * P.nbody + m] + P.bnd[l * 3 + n] * rb;
+    const double xpb[3] = {xl[0] - 0.5, xl[1], xl[2]};
+    const size_t cs = (size_t)A.nx * A.mz * A.my;
+    Sten sv, sp;
+    make_sten(sv, xl, A.nx, A.my, A.mz);
+    make_sten(sp, xpb, A.nx, A.my, A.mz);
+    const double w1 = P.angv[m], w2 = P.angv[P.nbody + m], w3 = P.angv[2 * P.nbody + m];
+    const double sn[3] = {P.cd[l * 3], P.cd[l * 3 + 1], P.cd[l * 3 + 2]};
+    const double crossp[3] = {w2 * sn[2] - w3 * sn[1], w3 * sn[0] - w1 * sn[2], w1 * sn[1] - w2 * sn[0]};
+    double ft[3];
+    for (int n = 0; n < 3; ++n) {
+      const double linvel = P.vel[n * P.nbody + m] + crossp[n] * rb * st->dx;
+      ft[n] = st->s.cf * (-linvel + interp_kji(sv, A.ubcp + n * cs));
+    }
+    for (int n = 0; n < 3; ++n) pin_reg(ft[n]);
+    for (int n = 0; n < 3; ++n) ft[n] = ft[n] - c3 * interp_ijk(sv, A.nlbc + n * cs) - c4 * interp_ijk(sv, A.onlbc + n * cs);
+    for (int n = 0; n < 3; ++n) pin_reg(ft[n]);
+    for (int n = 0; n < 3; ++n)
+      ft[n] = ft[n] + (c1 + c2) * (interp_kji(sp, A.ppr + n * cs) - interp_ijk(sv, A.diffn + n * cs));
+    const double rdx = rb * st->dx;
+    const double da1 = 4.0 * 3.14159265358979323846 * (rdx * rdx) / (double)P.nbnd;
+    for (int n = 0; n < 3; ++n) ft[n] = ft[n] * da1 * st->dr * st->dx;
+    uint8_t mk8[2][2][2];
+    gather8(sv, A.mask, mk8);
+    for (int k = 0; k < 2; ++k)
+      for (int j = 0; j < 2; ++j)
+        for (int i = 0; i < 2; ++i) {
+          const double wq = sv.w(i, j, k);
+          for (int n = 0; n < 3; ++n) {
+            const double tp = wq * ft[n] / st->dx3;
+            fs[n] += tp;
+            if (mk8[k][i][j] && st->include_frmeanfluid) fs[3 + n] += tp;
+          }
+        }
+    double* o = out + ((size_t)P.nbody * P.nbnd + (size_t)m * P.nbnd + l) * 6;
+    for (int n = 0; n < 3; ++n) {
+      o[n] = ft[n];
+      o[3 + n] = 0.0;
+    }
+  }
+  block_sums_w<6>(fs, sh);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 6; ++q) fpart[((size_t)P.nbody * G + (size_t)m * G + g) * 6 + q] = fs[q];
+}
+
 // Atomics-free spreading.  The records (point, corner) that deposit on a
 // grid node were binned to that node at set-up (stable sort: the reference's
 // (body, point, corner) order within a node) and the target nodes were cut

@@ -331,9 +331,6 @@ contains
        last_err = 'IMOVE with CAGE_SIMUL or ONLY_DEM: run the DEM alone through pib_dem_* instead'
        st = PIB_ERR_UNSUPPORTED; return
     end if
-    if (dobnd) then
-       last_err = 'DOBND = T (calc_bnd_forcing) is not supported'; st = PIB_ERR_UNSUPPORTED; return
-    end if
     if (impose_grav .or. mean_vel_to_particles .or. movingcv .or. irestart /= 0 .or. only_dem) then
        last_err = 'IMPOSE_GRAV/MEAN_VEL_TO_PARTICLES/MOVINGCV/IRESTART/ONLY_DEM are not supported'
        st = PIB_ERR_UNSUPPORTED; return

@@ -199,6 +199,10 @@ class Solver:
         """Restore the forward state from the top of the stack and drop it."""
         self._call("checkpoint_pop", self.ctx)
 
+    def checkpoint_steps(self, nsteps):
+        """``nsteps`` steps, pushing the state at the start of each."""
+        self._call("checkpoint_steps", self.ctx, nsteps)
+
     def adjoint_begin(self):
         """Allocate and zero the adjoint state and the accumulated gradients."""
         self._call("adjoint_begin", self.ctx)
@@ -210,6 +214,11 @@ class Solver:
         if w is not None and w.shape != (self.setup.nbody, 3):
             raise ValueError("wforce must be (nbody, 3)")
         self._call("adjoint_step", self.ctx, dptr(w) if w is not None else None)
+
+    def adjoint_record(self, nsteps):
+        """``nsteps`` steps, keeping each step's forward fields for its
+        adjoint step (pib_adjoint_record)."""
+        self._call("adjoint_record", self.ctx, nsteps)
 
     def adjoint_get(self):
         """-> (dJ/dxc (nbody, 3), dJ/dmpg (3,), adjoint of umean (3,))."""

@@ -158,6 +158,12 @@ CASES = {
                             revernorm=0, include_frmeanfluid=True),
     "set_mpg": dict(kind="random", my=24, dbydx=8, nbody=2, seed=4, re=20.0, set_mpg=True,
                     mpg=(-0.02, 0.0, 0.0)),
+    # calc_bnd_forcing beside the reversal pairs (the reference's compiled
+    # default DOBND = T), once on the LDS plane path and once on a compiled shape
+    "dobnd": dict(kind="random", my=24, dbydx=6, nbody=3, seed=7, re=30.0, dobnd=True, discr_scheme="upwind"),
+    "dobnd_frmeanfluid_32": dict(kind="random", my=32, dbydx=8, nbody=2, seed=3, re=20.0, dobnd=True,
+                                 include_frmeanfluid=True),
+    "dobnd_only": dict(kind="random", my=24, dbydx=8, nbody=2, seed=6, re=20.0, dobnd=True, dorpr=False),
 }
 
 

@@ -433,3 +433,54 @@ def test_mpg_controller_without_surface_drag(tmp_path, orc):
     for n in range(3):
         ref = s3.cf * (s3.ufmean_des[n] - s3.ufmean[n]) - visc[n] / vf
         assert abs(s3.mpg[n] - ref) < 1e-12 * max(abs(ref), abs(visc[1]) / vf), n
+
+
+def test_surface_forcing_of_a_sphere_at_rest(tmp_path, orc):
+    """calc_bnd_forcing (boundary_condition.F90:902-1102), DOBND = T with the
+    reversal pairs off.  In a uniform flow U past a sphere at rest (no
+    pressure, viscous or nonlinear terms, mpg held at zero) every surface
+    point carries F = cf U dA1 dr dx with dA1 = 4 pi (r dx)^2 / nbnd, spread
+    with its trilinear weights to all eight nodes of its stencil, fluid or
+    solid.  So frmean = nbnd F/(dx^3 N) in closed form, and fr is the
+    deposit of those point forces, formed here by brute force over the
+    points."""
+    st, o = _sphere(tmp_path, orc, "BF", set_mpg=True, mpg=(0.0, 0.0, 0.0), dorpr=False, dobnd=True)
+    U = (0.7, -0.2, 0.4)
+    _uniform_flow(o, U)
+    o.compute_new_timestep(1)
+    o.nonlinear(1)
+    s = o.scalars()
+    nx, my, mz, dx, dr = st.nx, st.my, st.mz, st.config.dx, st.config.dr
+    rb = st.radbdy[0]
+    N = nx * my * mz
+    F = np.array([s.cf * U[n] * 4 * np.pi * (rb * dx) ** 2 / st.nbnd * dr * dx for n in range(3)])
+    for n in range(3):
+        ref = st.nbnd * F[n] / (dx ** 3 * N)
+        assert abs(s.frmean[n] - ref) < 1e-12 * abs(ref), n
+    dep = np.zeros((nx, my, mz, 3))
+    pos = st.xc[0][:, None] + st.bndpts * rb
+    for l in range(st.nbnd):
+        p = pos[:, l]
+        c = np.floor(p).astype(int)
+        z = p - c
+        for i in (0, 1):
+            for j in (0, 1):
+                for k in (0, 1):
+                    w = (z[0] if i else 1 - z[0]) * (z[1] if j else 1 - z[1]) * (z[2] if k else 1 - z[2])
+                    dep[(c[0] + i - 1) % nx, (c[1] + j - 1) % my, (c[2] + k - 1) % mz] += w * F / dx ** 3
+    fr = o.field("fr")[:nx]  # fr - frmean
+    assert rel_err(fr + np.array(s.frmean), dep) < 1e-12
+
+
+def test_surface_forcing_of_a_sphere_moving_with_the_flow(tmp_path, orc):
+    """The same flow with the sphere moving at U: u - linvel = 0 at every
+    surface point, so calc_bnd_forcing deposits nothing."""
+    st, o = _sphere(tmp_path, orc, "BM", set_mpg=True, mpg=(0.0, 0.0, 0.0), dorpr=False, dobnd=True)
+    U = (0.7, -0.2, 0.4)
+    _uniform_flow(o, U)
+    o.update_particles(st.xc, np.array([U]))
+    o.compute_new_timestep(1)
+    o.nonlinear(1)
+    s = o.scalars()
+    assert np.max(np.abs(o.field("fr"))) < 1e-12 * abs(s.cf)
+    assert max(abs(x) for x in s.frmean) < 1e-14 * abs(s.cf)
