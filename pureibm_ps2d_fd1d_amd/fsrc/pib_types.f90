@@ -14,6 +14,7 @@ module pib_types
   integer(c_int), parameter :: PIB_LIMITER_NONE = 0, PIB_LIMITER_MINMOD = 1
   integer(c_int), parameter :: PIB_F_U = 0, PIB_F_P = 1
   integer(c_int), parameter :: PIB_F_PHI = 14, PIB_F_NLPHI = 15, PIB_F_ONLPHI = 16, PIB_F_FFPHI = 17
+  integer(c_int), parameter :: PIB_F_AU = 18, PIB_F_AP = 19, PIB_F_ANL = 20
 
   type, bind(C) :: pib_config
      integer(c_int) :: nx, my, mz, precision
@@ -263,5 +264,52 @@ module pib_types
        integer(c_int), value :: nsteps
        integer(c_int) :: st
      end function pib_step
+     ! the adjoint of the fixed-bed step and its checkpoint/record stacks
+     ! (no counterpart in the reference)
+     function pib_adjoint_begin(ctx) bind(C, name="pib_adjoint_begin") result(st)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int) :: st
+     end function pib_adjoint_begin
+     function pib_adjoint_step(ctx, wforce) bind(C, name="pib_adjoint_step") result(st)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       type(c_ptr), value :: wforce     ! c_loc of real(8) (nbody, 3), or c_null_ptr
+       integer(c_int) :: st
+     end function pib_adjoint_step
+     function pib_adjoint_record(ctx, nsteps) bind(C, name="pib_adjoint_record") result(st)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nsteps
+       integer(c_int) :: st
+     end function pib_adjoint_record
+     function pib_adjoint_get(ctx, dxc, dmpg, dumean) bind(C, name="pib_adjoint_get") result(st)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: dxc(*), dmpg(3), dumean(3)
+       integer(c_int) :: st
+     end function pib_adjoint_get
+     function pib_adjoint_set_umean(ctx, dumean) bind(C, name="pib_adjoint_set_umean") result(st)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: dumean(3)
+       integer(c_int) :: st
+     end function pib_adjoint_set_umean
+     function pib_checkpoint_push(ctx) bind(C, name="pib_checkpoint_push") result(st)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int) :: st
+     end function pib_checkpoint_push
+     function pib_checkpoint_pop(ctx) bind(C, name="pib_checkpoint_pop") result(st)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int) :: st
+     end function pib_checkpoint_pop
+     function pib_checkpoint_steps(ctx, nsteps) bind(C, name="pib_checkpoint_steps") result(st)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nsteps
+       integer(c_int) :: st
+     end function pib_checkpoint_steps
   end interface
 end module pib_types

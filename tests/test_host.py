@@ -130,6 +130,45 @@ program caller
 end program caller
 """
 
+# the adjoint sweep of the Fortran binding example: forward run checkpointed
+# every 8 steps, then each segment recorded and reversed
+FORTRAN_ADJOINT = """
+program adjoint_sweep
+  use iso_c_binding
+  use pib_types
+  implicit none
+  integer, parameter :: nbody = 2, nseg = 4, every = 8
+  type(c_ptr) :: ctx
+  integer(c_int) :: st, k, j
+  real(c_double), target :: wforce(nbody, 3)
+  real(c_double) :: dxc(nbody, 3), dmpg(3), dumean(3)
+  ctx = c_null_ptr
+  do k = 1, nseg
+     st = pib_checkpoint_push(ctx)
+     st = pib_checkpoint_steps(ctx, every)
+  end do
+  st = pib_adjoint_begin(ctx)
+  dumean = 0
+  st = pib_adjoint_set_umean(ctx, dumean)
+  do k = nseg, 1, -1
+     st = pib_checkpoint_pop(ctx)
+     if (k == nseg) then          ! the drag of the last step is in J
+        st = pib_adjoint_record(ctx, every - 1)
+        st = pib_adjoint_step(ctx, c_loc(wforce))
+        do j = every - 1, 1, -1
+           st = pib_adjoint_step(ctx, c_null_ptr)
+        end do
+     else
+        st = pib_adjoint_record(ctx, every)
+        do j = every, 1, -1
+           st = pib_adjoint_step(ctx, c_null_ptr)
+        end do
+     end if
+  end do
+  st = pib_adjoint_get(ctx, dxc, dmpg, dumean)
+end program adjoint_sweep
+"""
+
 # the fixed-bed and moving-bed call sequences of the Fortran binding example
 FORTRAN_STEP = """
 program moving_step
@@ -187,14 +226,14 @@ end program moving_step
 def test_fortran_bindings_take_the_reference_arrays(tmp_path):
     """A Fortran caller passes the reference's complex(8) and real(8) arrays
     to the field, scalar, transform and tridiagonal entry points, and runs the
-    fixed-bed and moving-bed call sequences, through the interfaces of
-    fsrc/pib_types.f90 (compile only, no call)."""
+    fixed-bed, moving-bed and adjoint call sequences, through the interfaces
+    of fsrc/pib_types.f90 (compile only, no call)."""
     fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
     mods = os.path.join(ROOT, "pureibm_ps2d_fd1d_amd", "_build")
     if not os.path.exists(fc) or not os.path.exists(os.path.join(mods, "pib_types.mod")):
         pytest.skip("amdflang or the built pib_types module is missing")
     import subprocess
-    for name, text in (("caller", FORTRAN_CALLER), ("moving_step", FORTRAN_STEP)):
+    for name, text in (("caller", FORTRAN_CALLER), ("moving_step", FORTRAN_STEP), ("adjoint", FORTRAN_ADJOINT)):
         src = tmp_path / (name + ".f90")
         src.write_text(text)
         r = subprocess.run([fc, "-c", "-J", str(tmp_path), "-I", mods, str(src), "-o", str(tmp_path / (name + ".o"))],
