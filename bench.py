@@ -182,16 +182,24 @@ def traffic_of(kernel):
         return None
 
 
-def run_batched(nreal, steps, warmup):
+def run_batched(nreal, steps, warmup, tune_steps=10):
     """Batched realizations (FP32-bulk): nreal independent contexts of the C3
-    mix (96^3/128^3/160^3 beds), each on its own stream.  Times the contexts
-    one after the other and then all launched together (pib_step_async on
-    every context, then pib_sync on every context)."""
+    mix (96^3/128^3/160^3 beds).  Each shape's step form is picked by the
+    autotuner (pib_autotune, cached per shape).  Times the contexts one after
+    the other (pib_step each), launched together on their own streams
+    (pib_step_async on every context, then pib_sync), and as one batch
+    (pib_batch_step: one graph launch a step for all of them) -- the value."""
     work = tempfile.mkdtemp(prefix="pib_c3_")
     sts = [solver.RunSetup(configs.c3_realization(work, k), precision=_abi.PIB_FP32_BULK) for k in range(nreal)]
     gs = [solver.Solver(st) for st in sts]
-    for g in gs:
-        g.step(warmup)
+    picks = {}
+    for g, st in zip(gs, sts):
+        g.step(2)
+        p, ms = g.autotune(tune_steps)
+        if ms > 0:
+            picks[f"{st.nx}^3"] = {"fft": p[0], "velstep": p[1], "streams": p[2], "ms_per_step": ms}
+    B = solver.Batch(gs)
+    B.step(warmup)
     alone = 0.0
     for g in gs:
         t0 = time.perf_counter()
@@ -205,13 +213,21 @@ def run_batched(nreal, steps, warmup):
     for g in gs:
         if g.lib.pib_sync(g.ctx) != 0:
             raise RuntimeError(g.lib.pib_last_error(g.ctx).decode())
-    together = time.perf_counter() - t0
+    streams = time.perf_counter() - t0
+    B.step(2)  # both batch graphs captured before the timed region
+    t0 = time.perf_counter()
+    B.step(steps)
+    batch = time.perf_counter() - t0
     pts = sum(st.nx * st.my * st.mz for st in sts)
+    B.close()
     for g in gs:
         g.close()
     return {"realizations": nreal, "steps": steps, "grid_points": pts,
-            "value": pts * steps / together / 1e6, "unit": "Mgridpt-steps/s",
+            "value": pts * steps / batch / 1e6, "unit": "Mgridpt-steps/s",
+            "ms_per_batch_step": batch / steps * 1e3,
+            "streams": pts * steps / streams / 1e6,
             "one_after_another": pts * steps / alone / 1e6,
+            "autotuned": picks,
             "sizes": [f"{st.nx}^3/{st.nbody}" for st in sts]}
 
 

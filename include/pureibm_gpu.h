@@ -314,7 +314,8 @@ int pib_kernel_times(void* ctx, int maxn, char* names, double* total_ms, int* la
  *     the next form down.
  *   PIB_TUNE_FFT: 1 = plane transforms of a compiled shape (my/2, mz) in
  *     {(64,128), (32,64), (16,32)} use the two-stage kernels (default); 0 =
- *     always the generic mixed-radix transform.
+ *     always the generic mixed-radix transform; 2 = the generic transform in
+ *     HBM slots (the form of planes too large for LDS).
  *   PIB_TUNE_STREAMS: 1 = with particles, the form_nl branch of a stage
  *     (form_nl and its transforms) runs on a second stream beside the bcset
  *     branch up to the pressure gradient, joined before the forcing (default
@@ -329,6 +330,38 @@ int pib_kernel_times(void* ctx, int maxn, char* names, double* total_ms, int* la
 #define PIB_TUNE_FFT 2
 #define PIB_TUNE_STREAMS 4
 int pib_set_tuning(void* ctx, int key, int value);
+
+/* Autotuner (no reference counterpart).  Times PIB_TUNE_FFT (0, and 1 where
+ * the plane has a compiled shape), then PIB_TUNE_VELSTEP (0, 1, 2), then
+ * PIB_TUNE_STREAMS (0, 1), each over nsteps graph-replayed steps of this
+ * context's bed started from the same saved state (pib_checkpoint_push/pop),
+ * and keeps the fastest.  The state after the call is the state before it.
+ * The picks are cached for the process per (nx, my, mz, precision): a later
+ * call for a context of the same shape applies them without timing unless
+ * force != 0.  picks (may be NULL) receives (fft, velstep, streams) and ms
+ * (may be NULL) the GPU ms per step of the picked form (0 when cached).
+ * Needs pib_set_particles first; waits on the stream. */
+int pib_autotune(void* ctx, int nsteps, int force, int* picks, double* ms);
+
+/* ---- batched realizations -------------------------------------------------
+ * No reference counterpart: the reference runs one realization per MPI job
+ * (its MIS decks are swept by scripts).  A batch steps n independent contexts
+ * (any sizes and precisions, each with its particles set) in lockstep.  Once
+ * every context has run its two eager steps, each batch step is ONE
+ * hipGraphLaunch on the batch's stream: a graph holding each context's whole
+ * step as a child graph, with no edges between contexts.  Two graphs are
+ * kept, one per nl/onl buffer assignment; they are rebuilt when a context
+ * drops its graphs (new particles, a new kernel form).  pib_batch_step
+ * enqueues and orders the contexts' own streams after the batch's work, so a
+ * context may be used on its own afterwards; pib_batch_sync drains and
+ * returns the first non-zero status of the contexts (its message through
+ * pib_batch_last_error).  The contexts must outlive the batch; a context
+ * belongs to at most one batch at a time. */
+int pib_batch_create(void* const* ctxs, int n, void** batch);
+int pib_batch_step(void* batch, int nsteps);
+int pib_batch_sync(void* batch);
+int pib_batch_destroy(void* batch);
+const char* pib_batch_last_error(void* batch);
 
 #ifdef __cplusplus
 }

@@ -155,6 +155,12 @@ _GPU_ONLY_SIGS = {
     "set_timing": ([_VP, _I], _I),
     "kernel_times": ([_VP, _I, ctypes.c_char_p, _PD, _PI, _PI], _I),
     "set_tuning": ([_VP, _I, _I], _I),
+    "autotune": ([_VP, _I, _I, _PI, _PD], _I),
+    "batch_create": ([ctypes.POINTER(_VP), _I, ctypes.POINTER(_VP)], _I),
+    "batch_step": ([_VP, _I], _I),
+    "batch_sync": ([_VP], _I),
+    "batch_destroy": ([_VP], _I),
+    "batch_last_error": ([_VP], ctypes.c_char_p),
 }
 
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -683,8 +684,20 @@ static std::vector<C2<T>> tw_table(int n, int count) {
   return t;
 }
 
+// What a batch of contexts (pib_batch_*) needs of each context, whatever its
+// precision.
 struct Base {
   virtual ~Base() {}
+  virtual hipStream_t main_stream() const = 0;
+  virtual bool graph_ready() const = 0;  // eager steps have set every kernel attribute
+  virtual int graph_parity() = 0;        // the nl/onl buffer assignment of the next step
+  virtual hipGraph_t capture_step() = 0; // the next step as a graph; the host state is kept
+  // the next step enqueued on the context's stream while that stream takes
+  // part in a capture begun elsewhere; the host state is kept
+  virtual void step_into_capture() = 0;
+  virtual void advance_step() = 0;       // the host side of a step replayed elsewhere
+  virtual void batch_eager_step() = 0;   // one eager step on the context's own streams
+  virtual unsigned long long graph_generation() const = 0;  // bumped when captured graphs go stale
 };
 
 template <typename T>
@@ -1461,7 +1474,9 @@ struct Impl : Base {
   int eager_steps = 0;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   C2<T>* nl_a = nullptr;  // the nl buffer of assignment 0
+  unsigned long long graph_gen = 0;
   void drop_graphs() {
+    ++graph_gen;
     for (auto& g : gexec)
       if (g) {
         (void)hipGraphExecDestroy(g);
@@ -1489,43 +1504,71 @@ struct Impl : Base {
       if (r) HIPCHK(hipEventRecord(ev[4], stream));
     }
   }
-  void graph_step() {
+  hipStream_t main_stream() const override { return stream; }
+  bool graph_ready() const override { return use_graph && !ktime && eager_steps >= 2 && nlbc_valid; }
+  int graph_parity() override {
     if (!nl_a) nl_a = nl;
-    int par = nl == nl_a ? 0 : 1;
-    if (!gexec[par]) {
-      C2<T>*nl0 = nl, *onl0 = onl;
-      T *nb0 = nlbc, *onb0 = onlbc;
-      bool v0 = nlbc_valid;
-      hipGraph_t g;
-      HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-      capturing = true;
-      try {
-        eager_step(false);
-      } catch (...) {
-        capturing = false;
-        (void)hipStreamEndCapture(stream, &g);
-        nl = nl0;
-        onl = onl0;
-        nlbc = nb0;
-        onlbc = onb0;
-        nlbc_valid = v0;
-        throw;
-      }
+    return nl == nl_a ? 0 : 1;
+  }
+  unsigned long long graph_generation() const override { return graph_gen; }
+  void step_into_capture() override {
+    C2<T>*nl0 = nl, *onl0 = onl;
+    T *nb0 = nlbc, *onb0 = onlbc;
+    bool v0 = nlbc_valid;
+    auto gen0 = state_gen;
+    auto restore = [&] {
       capturing = false;
-      HIPCHK(hipStreamEndCapture(stream, &g));
-      HIPCHK(hipGraphInstantiate(&gexec[par], g, nullptr, nullptr, 0));
-      HIPCHK(hipGraphDestroy(g));
       nl = nl0;
       onl = onl0;
       nlbc = nb0;
       onlbc = onb0;
       nlbc_valid = v0;
+      state_gen = gen0;
+    };
+    capturing = true;
+    try {
+      eager_step(false);
+    } catch (...) {
+      restore();
+      throw;
     }
-    HIPCHK(hipGraphLaunch(gexec[par], stream));
+    restore();
+  }
+  hipGraph_t capture_step() override {
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      step_into_capture();
+    } catch (...) {
+      (void)hipStreamEndCapture(stream, &g);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(stream, &g));
+    return g;
+  }
+  void advance_step() override {
     ++state_gen;
     std::swap(nl, onl);
     std::swap(nlbc, onlbc);
     nlbc_valid = true;
+    timed = false;
+  }
+  void batch_eager_step() override {
+    if (!have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
+    eager_step(false);
+    ++eager_steps;
+    timed = false;
+  }
+  void graph_step() {
+    int par = graph_parity();
+    if (!gexec[par]) {
+      hipGraph_t g = capture_step();
+      hipError_t e = hipGraphInstantiate(&gexec[par], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPCHK(e);
+    }
+    HIPCHK(hipGraphLaunch(gexec[par], stream));
+    advance_step();
   }
   void step_async(int nsteps) {
     if (!have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
@@ -2196,6 +2239,88 @@ struct Impl : Base {
       HIPCHK(hipMemcpyAsync(b + 5 * cs3 * sizeof(T), dst, sizeof(DevState), hipMemcpyDeviceToDevice, stream));
     }
   }
+  // ---------------- autotuner ----------------
+  // a step form: (PIB_TUNE_FFT, PIB_TUNE_VELSTEP, PIB_TUNE_STREAMS)
+  std::array<int, 3> form() const { return {plane_hbm && !hbm_needed ? 2 : (fft_compiled ? 1 : 0), vel_variant, use_fork ? 1 : 0}; }
+  void set_form(const std::array<int, 3>& f) {
+    HIPCHK(hipStreamSynchronize(stream));
+    drop_graphs();
+    fft_compiled = f[0] == 1;
+    set_plane_hbm(f[0] == 2 || hbm_needed);
+    vel_variant = f[1];
+    use_fork = f[2] == 1;
+  }
+  // GPU ms per graph-replayed step of the current form from the current
+  // state, which is restored afterwards
+  double time_form(int nsteps) {
+    checkpoint_push();
+    for (int n = 0; n < 2; ++n) {  // kernel attributes, then both graphs captured
+      eager_step(false);
+      ++eager_steps;
+    }
+    for (int n = 0; n < 2; ++n) graph_step();
+    HIPCHK(hipEventRecord(ev[0], stream));
+    for (int n = 0; n < nsteps; ++n) graph_step();
+    HIPCHK(hipEventRecord(ev[1], stream));
+    HIPCHK(hipEventSynchronize(ev[1]));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    checkpoint_pop();
+    timed = false;
+    return ms / nsteps;
+  }
+  void autotune(int nsteps, bool force, int* picks, double* ms_out) {
+    if (!have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
+    if (nsteps < 1) throw PibFail(PIB_ERR_ARG, "nsteps < 1");
+    if (!use_graph) throw PibFail(PIB_ERR_STATE, "the autotuner times graph-replayed steps (PIB_TUNE_GRAPH = 1)");
+    const std::array<int, 4> key{nx, my, mz, (int)sizeof(T)};
+    std::array<int, 3> pick = form();
+    double best = 0;
+    bool cached = false;
+    {
+      std::lock_guard<std::mutex> lk(tune_mutex());
+      auto it = tune_cache().find(key);
+      if (it != tune_cache().end() && !force) {
+        pick = it->second;
+        cached = true;
+      }
+    }
+    if (!cached) {
+      if (pick[0] == 2) pick[0] = 1;
+      set_form(pick);
+      best = time_form(nsteps);
+      auto axis = [&](int a, std::initializer_list<int> vals) {
+        for (int v : vals) {
+          if (v == pick[a]) continue;
+          auto f = pick;
+          f[a] = v;
+          set_form(f);
+          double t = time_form(nsteps);
+          if (t < best) {
+            best = t;
+            pick = f;
+          }
+        }
+      };
+      if (sh != 0 && !hbm_needed) axis(0, {0, 1});
+      axis(1, {0, 1, 2});
+      if (nbody > 0) axis(2, {0, 1});
+      std::lock_guard<std::mutex> lk(tune_mutex());
+      tune_cache()[key] = pick;
+    }
+    set_form(pick);
+    if (picks)
+      for (int q = 0; q < 3; ++q) picks[q] = pick[q];
+    if (ms_out) *ms_out = best;
+  }
+  static std::mutex& tune_mutex() {
+    static std::mutex m;
+    return m;
+  }
+  static std::map<std::array<int, 4>, std::array<int, 3>>& tune_cache() {
+    static std::map<std::array<int, 4>, std::array<int, 3>> c;
+    return c;
+  }
   void checkpoint_pop() {
     if (ck_stack.empty()) throw PibFail(PIB_ERR_STATE, "checkpoint stack is empty");
     size_t off[6];
@@ -2475,6 +2600,175 @@ int pib_set_tuning(void* ctx, int key, int value) {
   }
   return PIB_ERR_ARG;
 }
+int pib_autotune(void* ctx, int nsteps, int force, int* picks, double* ms) {
+  return dispatch(ctx, [&](auto& I) { I.autotune(nsteps, force != 0, picks, ms); });
+}
+
+// ---------------- batched realizations ----------------
+// Each batch step is one launch of a graph captured across the batch's and
+// the contexts' streams: a fork from the batch stream into every context's
+// stream, each context's whole step on its own stream, and a join back, so
+// the contexts' steps are independent branches of the one graph.
+struct Batch {
+  std::vector<Ctx*> ctx;
+  hipStream_t stream = nullptr;
+  std::vector<hipEvent_t> ev;  // a context's stream -> the batch stream
+  hipEvent_t ev_done = nullptr;  // the batch stream -> every context's stream
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  std::vector<int> key[2];  // nl/onl assignment of every context
+  std::vector<unsigned long long> gen[2];
+  bool joined = false;
+  std::string err;
+  ~Batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& e : exec)
+      if (e) (void)hipGraphExecDestroy(e);
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (ev_done) (void)hipEventDestroy(ev_done);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void join() {  // the batch stream waits for the contexts' earlier work
+    for (size_t i = 0; i < ctx.size(); ++i) {
+      HIPCHK(hipEventRecord(ev[i], ctx[i]->impl->main_stream()));
+      HIPCHK(hipStreamWaitEvent(stream, ev[i], 0));
+    }
+    joined = true;
+  }
+  void release() {  // the contexts' later work waits for the batch's
+    HIPCHK(hipEventRecord(ev_done, stream));
+    for (auto c : ctx) HIPCHK(hipStreamWaitEvent(c->impl->main_stream(), ev_done, 0));
+    joined = false;
+  }
+  void build(int slot, const std::vector<int>& k, const std::vector<unsigned long long>& g) {
+    if (exec[slot]) {
+      HIPCHK(hipStreamSynchronize(stream));
+      (void)hipGraphExecDestroy(exec[slot]);
+      exec[slot] = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      HIPCHK(hipEventRecord(ev_done, stream));
+      for (size_t i = 0; i < ctx.size(); ++i) {
+        hipStream_t cs = ctx[i]->impl->main_stream();
+        HIPCHK(hipStreamWaitEvent(cs, ev_done, 0));
+        ctx[i]->impl->step_into_capture();
+        HIPCHK(hipEventRecord(ev[i], cs));
+        HIPCHK(hipStreamWaitEvent(stream, ev[i], 0));
+      }
+    } catch (...) {
+      (void)hipStreamEndCapture(stream, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(stream, &graph));
+    hipError_t e = hipGraphInstantiate(&exec[slot], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(e);
+    key[slot] = k;
+    gen[slot] = g;
+  }
+  void step(int nsteps) {
+    const size_t n = ctx.size();
+    std::vector<int> k(n);
+    std::vector<unsigned long long> g(n);
+    for (int s = 0; s < nsteps; ++s) {
+      bool ready = true;
+      for (auto c : ctx) ready = ready && c->impl->graph_ready();
+      if (!ready) {  // the eager steps that precede any capture, per context
+        if (joined) release();
+        for (auto c : ctx) c->impl->batch_eager_step();
+        continue;
+      }
+      if (!joined) join();
+      for (size_t i = 0; i < n; ++i) {
+        k[i] = ctx[i]->impl->graph_parity();
+        g[i] = ctx[i]->impl->graph_generation();
+      }
+      int slot = -1;
+      for (int q = 0; q < 2; ++q)
+        if (exec[q] && key[q] == k && gen[q] == g) slot = q;
+      if (slot < 0) {  // keep the other assignment's graph while it is current
+        slot = (exec[0] && gen[0] == g) ? 1 : 0;
+        build(slot, k, g);
+      }
+      HIPCHK(hipGraphLaunch(exec[slot], stream));
+      for (auto c : ctx) c->impl->advance_step();
+    }
+    if (joined) release();
+  }
+};
+
+int pib_batch_create(void* const* ctxs, int n, void** batch) {
+  if (!ctxs || n < 1 || !batch) return PIB_ERR_ARG;
+  *batch = nullptr;
+  Batch* b = new Batch;
+  for (int i = 0; i < n; ++i) {
+    Ctx* c = (Ctx*)ctxs[i];
+    if (!c || !c->impl) {
+      delete b;
+      return PIB_ERR_ARG;
+    }
+    for (auto o : b->ctx)
+      if (o == c) {
+        delete b;
+        return PIB_ERR_ARG;
+      }
+    b->ctx.push_back(c);
+  }
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_done, hipEventDisableTiming) != hipSuccess) {
+    delete b;
+    return PIB_ERR_HIP;
+  }
+  b->ev.assign(n, nullptr);
+  for (auto& e : b->ev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      delete b;
+      return PIB_ERR_HIP;
+    }
+  *batch = b;
+  return PIB_OK;
+}
+int pib_batch_step(void* batch, int nsteps) {
+  Batch* b = (Batch*)batch;
+  if (!b || nsteps < 0) return PIB_ERR_ARG;
+  try {
+    b->step(nsteps);
+    return PIB_OK;
+  } catch (const PibFail& e) {
+    b->err = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    b->err = e.what();
+    return PIB_ERR_HIP;
+  }
+}
+int pib_batch_sync(void* batch) {
+  Batch* b = (Batch*)batch;
+  if (!b) return PIB_ERR_ARG;
+  if (hipStreamSynchronize(b->stream) != hipSuccess) {
+    b->err = "hipStreamSynchronize failed";
+    return PIB_ERR_HIP;
+  }
+  for (size_t i = 0; i < b->ctx.size(); ++i) {
+    int st = finish_status(b->ctx[i], PIB_OK);
+    if (st != PIB_OK) {
+      b->err = "context " + std::to_string(i) + ": " + b->ctx[i]->err;
+      return st;
+    }
+  }
+  return PIB_OK;
+}
+int pib_batch_destroy(void* batch) {
+  delete (Batch*)batch;
+  return PIB_OK;
+}
+const char* pib_batch_last_error(void* batch) {
+  return batch ? ((Batch*)batch)->err.c_str() : "null batch";
+}
+
 int pib_set_timing(void* ctx, int on) {
   return dispatch(ctx, [&](auto& I) { I.set_timing(on); });
 }

@@ -71,6 +71,45 @@ class RunSetup:
         return s
 
 
+class Batch:
+    """Independent realizations (Solver objects on the HIP library) stepped
+    in lockstep by pib_batch_step: one graph launch a step for all of them."""
+
+    def __init__(self, solvers):
+        self.lib = _abi.load_gpu()
+        self.solvers = list(solvers)
+        arr = (ctypes.c_void_p * len(self.solvers))(*[g.ctx.value for g in self.solvers])
+        self.b = ctypes.c_void_p()
+        st = self.lib.pib_batch_create(arr, len(self.solvers), ctypes.byref(self.b))
+        if st != PIB_OK:
+            raise PibError(st, "pib_batch_create")
+
+    def _check(self, st):
+        if st != PIB_OK:
+            raise PibError(st, (self.lib.pib_batch_last_error(self.b) or b"").decode(errors="replace"))
+
+    def step_async(self, nsteps=1):
+        self._check(self.lib.pib_batch_step(self.b, nsteps))
+
+    def sync(self):
+        self._check(self.lib.pib_batch_sync(self.b))
+
+    def step(self, nsteps=1):
+        self.step_async(nsteps)
+        self.sync()
+
+    def close(self):
+        if self.b:
+            self.lib.pib_batch_destroy(self.b)
+            self.b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Solver:
     """One realization on one device context.
 
@@ -329,6 +368,16 @@ class Solver:
 
     def set_scalars(self, s):
         self._call("set_scalars", self.ctx, ctypes.byref(s))
+
+    def autotune(self, nsteps=10, force=False):
+        """pib_autotune: time the step forms of this bed and keep the
+        fastest (cached per shape for the process).  -> (picks, ms): picks
+        = (PIB_TUNE_FFT, PIB_TUNE_VELSTEP, PIB_TUNE_STREAMS), ms the GPU ms
+        per step of the pick (0 when it came from the cache)."""
+        picks = (ctypes.c_int * 3)()
+        ms = ctypes.c_double()
+        self._call("autotune", self.ctx, nsteps, 1 if force else 0, picks, ctypes.byref(ms))
+        return tuple(picks), ms.value
 
     def forces(self):
         nb = self.setup.nbody
