@@ -195,17 +195,45 @@ int pib_sync(void* ctx);
  * onlphif, ffphi (PIB_F_PHI..PIB_F_FFPHI, zero) and sets heat_ratio_old.
  * All three must be > 0. */
 int pib_scalar_set(void* ctx, double gamma, double heat_ratio, double heat_ratio_new);
-/* out[4] = heat_ratio, heat_ratio_new, heat_ratio_old, gamma */
+/* out[4] = heat_ratio, heat_ratio_new, heat_ratio_old, gamma (the ratios
+ * live on the device; this call waits on the stream) */
 int pib_scalar_get(void* ctx, double* out);
 /* scalstep's mode loop (stepthescalar.F90:67-91): phistep (:97-241) for
- * every (ky,kz) mode from phif, nlphif, onlphif and ffphi as they stand
- * (nlphi_graduphi, which forms them, is not part of this interface), then
- * heat_ratio_old = heat_ratio, heat_ratio = heat_ratio_new.  No source term;
- * serial x-periodic branch.  FP64 in both precisions.  The solve is enqueued
- * on the context stream without waiting for it; a zero pivot
+ * every (ky,kz) mode from phif, nlphif, onlphif and ffphi as they stand,
+ * then heat_ratio_old = heat_ratio, heat_ratio = heat_ratio_new.  No source
+ * term; serial x-periodic branch.  FP64 in both precisions.  The solve is
+ * enqueued on the context stream without waiting for it; a zero pivot
  * (tridiagonal.F90:62,72-76) is raised on the device and returned as
  * PIB_ERR_PIVOT by the next pib_sync, pib_step or step entry point. */
 int pib_scalstep_modes(void* ctx, int rks);
+/* The surface value of the species on each body (phisurfall(1:nbody,1),
+ * scalar_init.F90:101), nbody values; after pib_set_particles and again
+ * after any pib_set_particles that changes nbody. */
+int pib_scalar_surface(void* ctx, const double* phisurfall);
+/* The whole scalstep of stage rks (stepthescalar.F90:51-95), after the
+ * stage's velstep (dtibm_main_scal_flo.F90:489-515):
+ *   nlphi_graduphi (nlphi.F90:56-155): onlphif <- nlphif, form_nlphi
+ *     (nl_allphi.F90:59-430: phi = ff2cr(phif), the y/z products with the
+ *     stage's ubcp, the x-flux -- central or MUSCL/Rusanov upwind per
+ *     DISCR_SCHEME -- with the heat-ratio twist at the x wrap, the sign),
+ *     compute_phim_heat_ratio (:431-487: heat_ratio_new = 1/phim(nx+1));
+ *   bcsetscal (bc_scalar.F90:65-383): calc_diffn (:935-1033), per body
+ *     calc_bndrev_data (:386-556: the surface flux, and with DOBND the
+ *     surface forcing) and calc_inner_rev_data (:558-932) when DORPR,
+ *     ffphi = ff2rc(fr);
+ *   the mode loop and the ratio roll of pib_scalstep_modes.
+ * One species, serial branch, x periodic, no source term.  Needs
+ * pib_scalar_set, pib_set_particles and pib_scalar_surface.  Enqueued
+ * without waiting; errors as pib_scalstep_modes. */
+int pib_scalstep(void* ctx, int rks);
+/* After pib_scalstep: flux_body (nbody) = the sum over each body's surface
+ * points of n . grad phi (bc_scalar.F90:524-551), out4 = (flux_global = sum
+ * of flux_body dA1 (:185), phi_fluid_mean, phi_solid_mean
+ * (nl_allphi.F90:380-419), 0).  Either may be NULL.  Waits on the stream.
+ * The Nusselt number of the reference's output is -flux_global
+ * char_length^2 / (6 voldom maxvolfrac (phisurf - phistream))
+ * (bc_scalar.F90:272-273). */
+int pib_scalar_flux(void* ctx, double* flux_body, double* out4);
 
 /* ---- transform layer drop-in (fftw_interface.F90:256-338) ------------ */
 /* ff2rc over nplanes planes: a(my,mz) real -> b(my2,mz) complex, forward

@@ -61,6 +61,13 @@ typedef struct orc {
   /* scalar transport, one species: phif, nlphif, onlphif, ffphi (nx+1, my2, mz) */
   cpx *phif, *nlphif, *onlphif, *ffphi;
   double sc_gamma, heat_ratio, heat_ratio_new, heat_ratio_old;
+  /* the physical scalar arrays of the whole scalstep, planes 0..nx+1 (IS0):
+   * phi, its nl and onl, diffn, the three gradient components, the forcing
+   * and the x-flux of form_nlphi; per-body surface value and flux */
+  double *phir, *nlphir, *onlphir, *dphi, *gphi, *frphi, *fdd;
+  double *phisurfall, *flux_body;
+  int nsurf;
+  double phi_fluid_mean, phi_solid_mean, flux_global;
   cpx *p;                 /* (nx, my2, mz) */
   double *ubcp, *nlbc, *onlbc, *diffn, *fr; /* (nx+1, my, mz, 3); fr doubles as omega */
   double *omega_copy;     /* last omega, kept for parity dumps */
@@ -520,6 +527,8 @@ int orc_destroy(void* ctx) {
   free(o->wy); free(o->wz); free(o->w2);
   free(o->u); free(o->nl); free(o->onl); free(o->ff); free(o->p);
   free(o->phif); free(o->nlphif); free(o->onlphif); free(o->ffphi);
+  free(o->phir); free(o->nlphir); free(o->onlphir); free(o->dphi); free(o->gphi); free(o->frphi);
+  free(o->fdd); free(o->phisurfall); free(o->flux_body);
   free(o->ubcp); free(o->nlbc); free(o->onlbc); free(o->diffn); free(o->fr); free(o->omega_copy);
   free(o->pbcp); free(o->ppr); free(o->fluid);
   free(o->ra); free(o->rb); free(o->ca); free(o->cb);
@@ -2073,6 +2082,502 @@ int orc_scalstep_modes(void* ctx, int rks) {
   o->heat_ratio = o->heat_ratio_new;
   if (st) return fail(o, st, "zero pivot in the scalar solve");
   return PIB_OK;
+}
+
+/* ======================================================================
+ * The whole scalstep (stepthescalar.F90:51-95), serial, x periodic with the
+ * heat-ratio twist phi(x + L) = phi(x) / heat_ratio, one species (the
+ * reference's nscalmax = 1), no source term:
+ *   nlphi_graduphi (nlphi.F90:56-155): onlphif <- nlphif, form_nlphi
+ *     (nl_allphi.F90:59-430) with compute_phim_heat_ratio (:431-536), the
+ *     physical nl and onl of phi;
+ *   bcsetscal (bc_scalar.F90:65-383): calc_diffn (:935-1033), per body
+ *     calc_bndrev_data (:385-556: the surface flux, and the surface forcing
+ *     when DOBND) and calc_inner_rev_data (:558-932) when dorpr, then
+ *     ffphi = ff2rc(fr);
+ *   the mode solve and the heat-ratio roll (orc_scalstep_modes).
+ * Physical arrays hold planes 0..nx+1 of the reference (IS0).  The velocity
+ * is ubcp of the stage (compute_new_timestep), as in the reference, where
+ * velstep leaves it in place.
+ * ====================================================================== */
+#define IS0(o, i, j, k) ((((size_t)((k)-1)) * (o)->my + ((j)-1)) * ((o)->nx + 2) + (i))
+
+int orc_scalar_surface(void* ctx, const double* phisurfall) {
+  orc* o = (orc*)ctx;
+  if (!o || (!phisurfall && o->nbody > 0)) return PIB_ERR_ARG;
+  if (!o->have_particles) return fail(o, PIB_ERR_STATE, "pib_set_particles first");
+  free(o->phisurfall);
+  free(o->flux_body);
+  o->phisurfall = (double*)xcalloc((size_t)o->nbody, sizeof(double));
+  o->flux_body = (double*)xcalloc((size_t)o->nbody, sizeof(double));
+  for (int m = 0; m < o->nbody; ++m) o->phisurfall[m] = phisurfall[m];
+  o->nsurf = o->nbody;
+  return PIB_OK;
+}
+
+int orc_scalar_flux(void* ctx, double* flux_body, double* out4) {
+  orc* o = (orc*)ctx;
+  if (!o) return PIB_ERR_ARG;
+  if (!o->phir) return fail(o, PIB_ERR_STATE, "no scalar step has run");
+  if (flux_body)
+    for (int m = 0; m < o->nsurf; ++m) flux_body[m] = o->flux_body[m];
+  if (out4) {
+    out4[0] = o->flux_global;
+    out4[1] = o->phi_fluid_mean;
+    out4[2] = o->phi_solid_mean;
+    out4[3] = 0.0;
+  }
+  return PIB_OK;
+}
+
+/* the physical arrays of the last scalstep for the tests: which = 0 phi,
+ * 1 nl, 2 onl, 3 diffn, 4..6 grad phi, 7 fr; (nx+2, my, mz), plane 0 first */
+int orc_scalar_debug(void* ctx, int which, double* out) {
+  orc* o = (orc*)ctx;
+  if (!o || !out || which < 0 || which > 7) return PIB_ERR_ARG;
+  if (!o->phir) return fail(o, PIB_ERR_STATE, "no scalar step has run");
+  const size_t np = (size_t)(o->nx + 2) * o->my * o->mz;
+  const double* src[8] = {o->phir, o->nlphir, o->onlphir, o->dphi, o->gphi, o->gphi + np, o->gphi + 2 * np, o->frphi};
+  memcpy(out, src[which], np * sizeof(double));
+  return PIB_OK;
+}
+
+/* scalar plane i (1..mx) of a spectral (nx+1, my2, mz) array to physical
+ * plane i of an IS0 array */
+static void phi_c2r(orc* o, const cpx* f, int i, double* dst, planebuf* w) {
+  for (int k = 1; k <= o->mz; ++k)
+    for (int j = 1; j <= o->my2; ++j) w->ca[(size_t)(k - 1) * o->my2 + (j - 1)] = f[IS(o, i, j, k)];
+  orc_ff2cr(o->my, o->mz, (const double*)w->ca, w->ra);
+  for (int k = 1; k <= o->mz; ++k)
+    for (int j = 1; j <= o->my; ++j) dst[IS0(o, i, j, k)] = w->ra[(size_t)(k - 1) * o->my + (j - 1)];
+}
+/* ff2rc of the (my, mz) plane in w->ra into w->ca */
+static void phi_r2c(orc* o, planebuf* w) { orc_ff2rc(o->my, o->mz, w->ra, (double*)w->ca); }
+
+/* form_nlphi, nl_allphi.F90:59-430, with compute_phim_heat_ratio(.TRUE.) */
+static void form_nlphi(orc* o) {
+  int nx = o->nx, my = o->my, mz = o->mz, my2 = o->my2, mx = o->mx, mxf = o->mxf;
+  const int center = o->cfg.discr_scheme == PIB_SCHEME_CENTER;
+  const double h = o->heat_ratio, dx = o->dx;
+  /* phi, the y/z products and their transforms, planes 1..mx (:97-160) */
+#pragma omp parallel
+  {
+    planebuf w = pb_alloc(o);
+#pragma omp for schedule(static)
+    for (int i = 1; i <= mx; ++i) {
+      phi_c2r(o, o->phif, i, o->phir, &w);
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my; ++j) {
+          double ph = o->phir[IS0(o, i, j, k)];
+          o->fdd[IS0(o, i, j, k)] = center ? o->ubcp[IR(o, i, j, k, 1)] * ph : 0.0;
+          w.ra[(size_t)(k - 1) * my + (j - 1)] = o->ubcp[IR(o, i, j, k, 2)] * ph;
+          w.rb[(size_t)(k - 1) * my + (j - 1)] = o->ubcp[IR(o, i, j, k, 3)] * ph;
+        }
+      orc_ff2rc(my, mz, w.ra, (double*)w.ca);
+      orc_ff2rc(my, mz, w.rb, (double*)w.cb);
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my2; ++j) {
+          size_t q = (size_t)(k - 1) * my2 + (j - 1);
+          o->nlphif[IS(o, i, j, k)] = cadd(cik(o->wy[j - 1], w.ca[q]), cik(o->wz[k - 1], w.cb[q]));
+        }
+    }
+    pb_free(&w);
+  }
+  /* compute_phim_heat_ratio (:431-487): only plane mx sets heat_ratio_new */
+  {
+    double phim = 0.0, um = 0.0;
+    int cnt = 0;
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my; ++j)
+        if (o->fluid[IFL(o, mx, j, k)]) {
+          phim = phim + o->ubcp[IR(o, mx, j, k, 1)] * o->phir[IS0(o, mx, j, k)];
+          um = um + o->ubcp[IR(o, mx, j, k, 1)];
+          cnt = cnt + 1;
+        }
+    um = um / (double)cnt;
+    phim = phim / (um * (double)cnt);
+    o->heat_ratio_new = 1.0 / phim;
+  }
+  if (center) {
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my; ++j) o->fdd[IS0(o, 0, j, k)] = o->fdd[IS0(o, nx, j, k)] * h;
+  } else {
+    /* MUSCL slopes and the Rusanov flux at the faces i+1/2, i = 0..nx (:185-344) */
+    double *uplus = (double*)xcalloc((size_t)nx + 2, sizeof(double)), *uminus = (double*)xcalloc((size_t)nx + 2, sizeof(double));
+    double *Up = (double*)xcalloc((size_t)nx + 2, sizeof(double)), *Um = (double*)xcalloc((size_t)nx + 2, sizeof(double));
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my; ++j) {
+#define PH(i) o->phir[IS0(o, (i), j, k)]
+#define U1(i) o->ubcp[IR(o, (i), j, k, 1)]
+        double sphi0 = (PH(1) - PH(nx - 1) * h) / (2.0 * dx);
+        Um[0] = PH(nx) * h + sphi0 * (dx / 2.0);
+        double sphim = (PH(2) / h - PH(nx)) / (2.0 * dx);
+        Up[nx] = PH(1) / h - sphim * (dx / 2.0);
+        double su0 = (U1(1) - U1(nx - 1)) / (2.0 * dx);
+        uminus[0] = U1(nx) + su0 * (dx / 2.0);
+        double sum = (U1(2) - U1(nx)) / (2.0 * dx);
+        uplus[nx] = U1(1) - sum * (dx / 2.0);
+        for (int i = 1; i <= nx; ++i) {
+          int im1 = i - 1, ip1 = i + 1;
+          double pfm = 1.0, pfp = 1.0;
+          if (im1 < 1) {
+            im1 = mxf + im1 - 1;
+            pfm = h;
+          }
+          if (ip1 > mxf) {
+            ip1 = ip1 - (mxf - 1);
+            pfp = 1.0 / h;
+          }
+          double sphi = (pfp * PH(ip1) - pfm * PH(im1)) / (2.0 * dx);
+          double su = (U1(ip1) - U1(im1)) / (2.0 * dx);
+          uplus[i - 1] = U1(i) - su * (dx / 2.0);
+          uminus[i] = U1(i) + su * (dx / 2.0);
+          Up[i - 1] = PH(i) - sphi * (dx / 2.0);
+          Um[i] = PH(i) + sphi * (dx / 2.0);
+        }
+        for (int i = 0; i <= nx; ++i) {
+          double ms = fmax(fabs(uminus[i]), fabs(uplus[i]));
+          double f = 0.5 * (uplus[i] * Up[i]) + 0.5 * (uminus[i] * Um[i]);
+          o->fdd[IS0(o, i, j, k)] = f - 0.5 * ms * (Up[i] - Um[i]);
+        }
+#undef PH
+#undef U1
+      }
+    free(uplus); free(uminus); free(Up); free(Um);
+  }
+  /* x-derivative of the flux, transformed into nlphif (:347-375) */
+#pragma omp parallel
+  {
+    planebuf w = pb_alloc(o);
+#pragma omp for schedule(static)
+    for (int i = 1; i <= nx; ++i) {
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my; ++j) {
+          double d = center ? (o->fdd[IS0(o, i + 1, j, k)] - o->fdd[IS0(o, i - 1, j, k)]) / (2.0 * dx)
+                            : (o->fdd[IS0(o, i, j, k)] - o->fdd[IS0(o, i - 1, j, k)]) / dx;
+          w.ra[(size_t)(k - 1) * my + (j - 1)] = d;
+        }
+      phi_r2c(o, &w);
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my2; ++j)
+          o->nlphif[IS(o, i, j, k)] = cadd(o->nlphif[IS(o, i, j, k)], w.ca[(size_t)(k - 1) * my2 + (j - 1)]);
+    }
+    pb_free(&w);
+  }
+  for (int k = 1; k <= mz; ++k)
+    for (int j = 1; j <= my2; ++j) o->nlphif[IS(o, mx, j, k)] = cdivr(o->nlphif[IS(o, 1, j, k)], h);
+  /* the sign of nl, the phase means, phi of plane mx (:380-419) */
+  double pf = 0.0, ps = 0.0;
+  for (int i = 1; i <= nx; ++i)
+    for (int k = 1; k <= mz; ++k)
+      for (int j = 1; j <= my; ++j) {
+        if (j <= my2) o->nlphif[IS(o, i, j, k)] = cscal(-1.0, o->nlphif[IS(o, i, j, k)]);
+        if (o->fluid[IFL(o, i, j, k)]) pf = pf + o->phir[IS0(o, i, j, k)];
+        else ps = ps + o->phir[IS0(o, i, j, k)];
+      }
+  for (int k = 1; k <= mz; ++k) {
+    for (int j = 1; j <= my2; ++j) o->nlphif[IS(o, mx, j, k)] = cscal(-1.0, o->nlphif[IS(o, mx, j, k)]);
+    for (int j = 1; j <= my; ++j) o->phir[IS0(o, mx, j, k)] = o->phir[IS0(o, 1, j, k)] / h;
+  }
+  o->phi_fluid_mean = pf / (double)o->count_fluid;
+  o->phi_solid_mean = o->count_solid > 0 ? ps / (double)o->count_solid : 0.0;
+}
+
+/* calc_diffn, bc_scalar.F90:935-1033: diffn and grad phi of planes 1..mx */
+static void calc_diffn_phi(orc* o) {
+  int nx = o->nx, my = o->my, mz = o->mz, my2 = o->my2, mx = o->mx;
+  const double h = o->heat_ratio, dx = o->dx, g = o->sc_gamma;
+#pragma omp parallel
+  {
+    planebuf w = pb_alloc(o);
+    cpx* f4 = (cpx*)xcalloc((size_t)my2 * mz * 4, sizeof(cpx));
+#pragma omp for schedule(static)
+    for (int i = 1; i <= mx; ++i) {
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my2; ++j) {
+          const cpx* P = o->phif;
+          cpx pm, pp, p0 = P[IS(o, i, j, k)];
+          if (i == 1) {
+            pm = cscal(h, P[IS(o, nx, j, k)]);
+            pp = P[IS(o, 2, j, k)];
+          } else if (i == nx) {
+            pm = P[IS(o, i - 1, j, k)];
+            pp = cdivr(P[IS(o, 1, j, k)], h);
+          } else if (i == nx + 1) {
+            pm = P[IS(o, i - 1, j, k)];
+            pp = cdivr(P[IS(o, 2, j, k)], h);
+          } else {
+            pm = P[IS(o, i - 1, j, k)];
+            pp = P[IS(o, i + 1, j, k)];
+          }
+          cpx f1 = cscal(1.0 / o->dx2, cadd(csub(pm, cscal(2.0, p0)), pp));
+          f1 = csub(f1, cscal(o->w2[(size_t)(k - 1) * my2 + (j - 1)], p0));
+          f1 = cscal(g, f1);
+          size_t q = (size_t)(k - 1) * my2 + (j - 1);
+          f4[q] = f1;
+          f4[(size_t)my2 * mz + q] = cdivr(csub(pp, pm), 2.0 * dx);
+          f4[(size_t)2 * my2 * mz + q] = cik(o->wy[j - 1], p0);
+          f4[(size_t)3 * my2 * mz + q] = cik(o->wz[k - 1], p0);
+        }
+      double* dst[4] = {o->dphi, o->gphi, o->gphi + (size_t)(nx + 2) * my * mz, o->gphi + (size_t)2 * (nx + 2) * my * mz};
+      for (int q = 0; q < 4; ++q) {
+        orc_ff2cr(my, mz, (const double*)(f4 + (size_t)q * my2 * mz), w.ra);
+        for (int k = 1; k <= mz; ++k)
+          for (int j = 1; j <= my; ++j) dst[q][IS0(o, i, j, k)] = w.ra[(size_t)(k - 1) * my + (j - 1)];
+      }
+    }
+    free(f4);
+    pb_free(&w);
+  }
+}
+
+/* interpolate_phidata, bc_scalar.F90:1036-1153 (order-2 stencil, serial):
+ * x nodes below 1 wrap up a period with the factor heat_ratio (heat_ratio_old
+ * for onl), above mxf down a period with 1/heat_ratio; ul by the
+ * interpolator's (k, j, i) sum, the others by array_dot_product's (i, j, k) */
+static void interp_phi(const orc* o, const int pc[3], const double pos[3], stencil* st, double* ul,
+                       double* nll, double* onll, double* dfll, int flag) {
+  st->ib = pc[0]; st->jb = pc[1]; st->kb = pc[2];
+  make_weights(st, pos);
+  double vs[8], ns[8], os[8], ds[8];
+  for (int k = 0; k < 2; ++k)
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 2; ++i) {
+        int ii = st->ib + i, jj = wrapy(o, st->jb + j), kk = wrapz(o, st->kb + k);
+        double pf = 1.0, pfo = 1.0;
+        if (ii < 1) {
+          ii = o->mxf + ii - 1;
+          pf = o->heat_ratio;
+          pfo = o->heat_ratio_old;
+        } else if (ii > o->mxf) {
+          ii = ii - o->mxf + 1;
+          pf = 1.0 / o->heat_ratio;
+        }
+        int q = (k * 2 + j) * 2 + i;
+        vs[q] = o->phir[IS0(o, ii, jj, kk)] * pf;
+        if (flag == 1) {
+          ns[q] = o->nlphir[IS0(o, ii, jj, kk)] * pf;
+          os[q] = o->onlphir[IS0(o, ii, jj, kk)] * pfo;
+          ds[q] = o->dphi[IS0(o, ii, jj, kk)] * pf;
+        }
+      }
+  *ul = interp_kji(st, vs);
+  if (flag == 1) {
+    *nll = dot_ijk(st, ns);
+    *onll = dot_ijk(st, os);
+    *dfll = dot_ijk(st, ds);
+  }
+}
+
+/* calc_bndrev_data, bc_scalar.F90:385-556: the surface flux of body m and,
+ * with DOBND, the surface forcing (its x nodes wrap into 1..mxf: a share on
+ * plane mxf is not transformed) */
+static void calc_bndrev_data(orc* o, int m, int rks, double da1) {
+  const double drm = 1.0, rb = o->radbdy[m - 1], cf = o->s.cf, h = o->heat_ratio;
+  const double c1 = o->coef[rks - 1][0], c2 = o->coef[rks - 1][1], c3 = o->coef[rks - 1][2], c4 = o->coef[rks - 1][3];
+  const size_t gs = (size_t)(o->nx + 2) * o->my * o->mz;
+  double fluxloc = 0.0;
+  for (int l = 1; l <= o->nbnd; ++l) {
+    double xl[3];
+    int is[3];
+    for (int n = 1; n <= 3; ++n) {
+      xl[n - 1] = XC(o, m, n) + BND(o, n, l) * rb;
+      is[n - 1] = cellof(xl[n - 1]);
+    }
+    double xlt = xl[0];
+    int ct = is[0];
+    if (!(1 <= ct && ct <= o->nx)) {
+      if (is[0] < 1) { ct = o->mxf + is[0] - 1; xlt = o->mxf + xl[0] - 1; }
+      if (is[0] >= o->mxf) { ct = is[0] - (o->mxf - 1); xlt = xl[0] - (o->mxf - 1); }
+    }
+    xl[0] = xlt;
+    is[0] = ct;
+    stencil st;
+    double phil = 0.0, nlphil = 0.0, onlphil = 0.0, dfll = 0.0;
+    interp_phi(o, is, xl, &st, &phil, NULL, NULL, NULL, 0);
+    double tempor = 0.0;
+    if (o->cfg.dobnd) {
+      tempor = tempor + cf * (phil - o->phisurfall[m - 1]);
+      tempor = tempor - c3 * nlphil - c4 * onlphil;
+      tempor = tempor - (c1 + c2) * dfll;
+      tempor = tempor * da1 * drm * o->dx;
+    }
+    double gsten[3][8];
+    for (int k = 0; k < 2; ++k) {
+      int kk = wrapz(o, st.kb + k);
+      for (int j = 0; j < 2; ++j) {
+        int jj = wrapy(o, st.jb + j);
+        for (int i = 0; i < 2; ++i) {
+          int ii = st.ib + i;
+          double pf = 1.0;
+          if (ii < 1) {
+            ii = o->mxf + ii - 1;
+            pf = h;
+          }
+          if (ii > o->mxf) {
+            ii = ii - o->mxf + 1;
+            pf = 1.0 / h;
+          }
+          int q = (k * 2 + j) * 2 + i;
+          if (o->cfg.dobnd) o->frphi[IS0(o, ii, jj, kk)] = o->frphi[IS0(o, ii, jj, kk)] + (st.w[q] * tempor) / o->dx3;
+          for (int n = 0; n < 3; ++n) gsten[n][q] = o->gphi[(size_t)n * gs + IS0(o, ii, jj, kk)] * pf;
+        }
+      }
+    }
+    double gb[3];
+    for (int n = 0; n < 3; ++n) gb[n] = dot_ijk(&st, gsten[n]);
+    fluxloc = fluxloc + (gb[0] * BND(o, 1, l) + gb[1] * BND(o, 2, l) + gb[2] * BND(o, 3, l));
+  }
+  o->flux_body[m - 1] = fluxloc;
+}
+
+/* calc_inner_rev_data, bc_scalar.F90:558-932 (serial): a reversal pair
+ * whose inner point lies in cell nx is evaluated twice, the second time one
+ * period down (cell 0); the stencil's x nodes stay in 0..nx+1 and the shares
+ * on planes 0 and nx+1 are not transformed */
+static void calc_inner_rev_data(orc* o, int m, int rks, double da2) {
+  const double drm = 1.0, cf = o->s.cf;
+  const double rb = o->radbdy[m - 1], ri = o->radibdy[m - 1], ro = o->radobdy[m - 1];
+  const double c1 = o->coef[rks - 1][0], c2 = o->coef[rks - 1][1], c3 = o->coef[rks - 1][2], c4 = o->coef[rks - 1][3];
+  for (int l = 1; l <= o->nbnd; ++l) {
+    if (!o->if_rev[(size_t)(m - 1) * o->nbnd + (l - 1)]) continue;
+    double xli[3];
+    int iii[3];
+    for (int n = 1; n <= 3; ++n) {
+      xli[n - 1] = XC(o, m, n) + BND(o, n, l) * ri;
+      iii[n - 1] = cellof(xli[n - 1]);
+    }
+    int ct = iii[0];
+    double xlt = xli[0];
+    if (!(1 <= ct && ct <= o->nx)) {
+      if (iii[0] < 1) { ct = o->mxf + iii[0] - 1; xlt = o->mxf + xli[0] - 1; }
+      if (iii[0] >= o->mxf) { ct = iii[0] - (o->mxf - 1); xlt = xli[0] - (o->mxf - 1); }
+    }
+    int nev = ct == o->nx ? 2 : 1;
+    for (int ev = 1; ev <= nev; ++ev) {
+      if (ev == 1) {
+        iii[0] = ct;
+        xli[0] = xlt;
+      } else if (iii[0] >= o->mxf - 1) {
+        iii[0] = iii[0] - (o->mxf - 1);
+        xli[0] = xli[0] - (o->mxf - 1);
+      }
+      double xlo[3];
+      int io[3];
+      for (int n = 1; n <= 3; ++n) {
+        xlo[n - 1] = xli[n - 1] + BND(o, n, l) * (ro - ri);
+        io[n - 1] = cellof(xlo[n - 1]);
+      }
+      stencil st;
+      double philo, phili, nlphil, onlphil, dfll;
+      interp_phi(o, io, xlo, &st, &philo, NULL, NULL, NULL, 0);
+      interp_phi(o, iii, xli, &st, &phili, &nlphil, &onlphil, &dfll, 1);
+      philo = o->phisurfall[m - 1] * (ro - ri) - philo * (rb - ri);
+      philo = philo / (ro - rb);
+      double phil = cf * (-philo + phili);
+      double tempor = 0.0;
+      tempor = tempor + phil;
+      tempor = tempor - c3 * nlphil - c4 * onlphil;
+      tempor = tempor - (c2 + c1) * dfll;
+      tempor = tempor * da2 * drm * o->dx;
+      int count_fl = 0;
+      double force_fl = 0.0;
+      for (int k = 0; k < 2; ++k) {
+        int kk = wrapz(o, st.kb + k);
+        for (int j = 0; j < 2; ++j) {
+          int jj = wrapy(o, st.jb + j);
+          for (int i = 0; i < 2; ++i) {
+            int ii = st.ib + i;
+            if (ii < 0) ii = o->mxf + ii - 1;
+            if (o->fluid[IFL(o, ii, jj, kk)]) {
+              count_fl = count_fl + 1;
+              force_fl = force_fl + st.w[(k * 2 + j) * 2 + i] * tempor / o->dx3;
+            }
+          }
+        }
+      }
+      int count_so = 8 - count_fl;
+      double force_dist = force_fl / (double)count_so;
+      for (int k = 0; k < 2; ++k) {
+        int kk = wrapz(o, st.kb + k);
+        for (int j = 0; j < 2; ++j) {
+          int jj = wrapy(o, st.jb + j);
+          for (int i = 0; i < 2; ++i) {
+            int ii = st.ib + i;
+            if (ii < 0) ii = o->mxf + ii - 1;
+            double add = st.w[(k * 2 + j) * 2 + i] * tempor / o->dx3;
+            size_t q = IS0(o, ii, jj, kk);
+            if (o->cfg.include_frmeanfluid) o->frphi[q] = o->frphi[q] + add;
+            else if (!o->fluid[IFL(o, ii, jj, kk)]) o->frphi[q] = o->frphi[q] + add + force_dist;
+          }
+        }
+      }
+    }
+  }
+}
+
+int orc_scalstep(void* ctx, int rks) {
+  orc* o = (orc*)ctx;
+  if (!o || rks < 1 || rks > o->itrmax) return PIB_ERR_ARG;
+  if (!o->phif) return fail(o, PIB_ERR_STATE, "pib_scalar_set first");
+  if (!o->have_particles) return fail(o, PIB_ERR_STATE, "pib_set_particles first");
+  if (o->nbody > 0 && o->nsurf != o->nbody) return fail(o, PIB_ERR_STATE, "pib_scalar_surface first");
+  if (o->nx < 3) return fail(o, PIB_ERR_ARG, "the scalar step needs nx >= 3");
+  int nx = o->nx, my = o->my, mz = o->mz, my2 = o->my2, mx = o->mx;
+  const size_t np = (size_t)(nx + 2) * my * mz;
+  if (!o->phir) {
+    o->phir = (double*)xcalloc(np, sizeof(double));
+    o->nlphir = (double*)xcalloc(np, sizeof(double));
+    o->onlphir = (double*)xcalloc(np, sizeof(double));
+    o->dphi = (double*)xcalloc(np, sizeof(double));
+    o->gphi = (double*)xcalloc(3 * np, sizeof(double));
+    o->frphi = (double*)xcalloc(np, sizeof(double));
+    o->fdd = (double*)xcalloc(np, sizeof(double));
+  }
+  /* nlphi_graduphi (nlphi.F90:73-132) */
+  for (int k = 1; k <= mz; ++k)
+    for (int j = 1; j <= my2; ++j)
+      for (int i = 1; i <= nx + 1; ++i) {
+        o->onlphif[IS(o, i, j, k)] = o->nlphif[IS(o, i, j, k)];
+        o->nlphif[IS(o, i, j, k)] = C(0.0, 0.0);
+      }
+  form_nlphi(o);
+#pragma omp parallel
+  {
+    planebuf w = pb_alloc(o);
+#pragma omp for schedule(static)
+    for (int i = 1; i <= mx; ++i) {
+      phi_c2r(o, o->nlphif, i, o->nlphir, &w);
+      phi_c2r(o, o->onlphif, i, o->onlphir, &w);
+    }
+    pb_free(&w);
+  }
+  /* bcsetscal (bc_scalar.F90:65-383) */
+  memset(o->frphi, 0, np * sizeof(double));
+  calc_diffn_phi(o);
+  o->flux_global = 0.0;
+  for (int m = 1; m <= o->nbody; ++m) {
+    double rb = o->radbdy[m - 1], ri = o->radibdy[m - 1];
+    double da1 = 4.0 * PI * (rb * o->dx) * (rb * o->dx) / (double)(float)o->nbnd;
+    double da2 = 4.0 * PI * (ri * o->dx) * (ri * o->dx) / (double)(float)o->nrpr_active[m - 1];
+    calc_bndrev_data(o, m, rks, da1);
+    if (o->cfg.dorpr) calc_inner_rev_data(o, m, rks, da2);
+    o->flux_global = o->flux_global + o->flux_body[m - 1] * da1;
+  }
+#pragma omp parallel
+  {
+    planebuf w = pb_alloc(o);
+#pragma omp for schedule(static)
+    for (int i = 1; i <= nx; ++i) {
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my; ++j) w.ra[(size_t)(k - 1) * my + (j - 1)] = o->frphi[IS0(o, i, j, k)];
+      phi_r2c(o, &w);
+      for (int k = 1; k <= mz; ++k)
+        for (int j = 1; j <= my2; ++j) o->ffphi[IS(o, i, j, k)] = w.ca[(size_t)(k - 1) * my2 + (j - 1)];
+    }
+    pb_free(&w);
+  }
+  return orc_scalstep_modes(ctx, rks);
 }
 
 /* ======================================================================

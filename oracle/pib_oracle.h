@@ -32,6 +32,10 @@ int orc_nonlinear(void* ctx, int rks);
 int orc_scalar_set(void* ctx, double gamma, double heat_ratio, double heat_ratio_new);
 int orc_scalar_get(void* ctx, double* out);
 int orc_scalstep_modes(void* ctx, int rks);
+int orc_scalstep(void* ctx, int rks);
+int orc_scalar_surface(void* ctx, const double* phisurfall);
+int orc_scalar_flux(void* ctx, double* flux_body, double* out4);
+int orc_scalar_debug(void* ctx, int which, double* out);
 int orc_nonlinear_begin(void* ctx, int rks);
 int orc_nonlinear_finish(void* ctx, int rks);
 int orc_velstep(void* ctx, int rks);

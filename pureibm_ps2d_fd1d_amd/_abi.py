@@ -133,6 +133,9 @@ _STEP_SIGS = {
     "scalar_set": ([_VP, ctypes.c_double, ctypes.c_double, ctypes.c_double], _I),
     "scalar_get": ([_VP, _PD], _I),
     "scalstep_modes": ([_VP, _I], _I),
+    "scalstep": ([_VP, _I], _I),
+    "scalar_surface": ([_VP, _PD], _I),
+    "scalar_flux": ([_VP, _PD, _PD], _I),
     "adjoint_begin": ([_VP], _I),
     "adjoint_step": ([_VP, _PD], _I),
     "adjoint_record": ([_VP, _I], _I),

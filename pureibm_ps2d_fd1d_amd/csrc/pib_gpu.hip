@@ -17,6 +17,7 @@
 #include "pib_kernels.h"
 #include "pib_velstep.h"
 #include "pib_scalar.h"
+#include "pib_scalar_nl.h"
 #include "pib_adjoint.h"
 
 namespace {
@@ -734,6 +735,16 @@ struct Impl : Base {
   // scalar transport, one species (allocated by the first scalar_set)
   C2<double>* phi[4] = {nullptr, nullptr, nullptr, nullptr};  // phif, nlphif, onlphif, ffphi
   double* sscr = nullptr;
+  // heat_ratio, heat_ratio_new, heat_ratio_old, gamma, phi_fluid_mean,
+  // phi_solid_mean on the device (the host values are those last set)
+  double* sratio = nullptr;
+  double sratio_h[8] = {};
+  // the whole scalar step: physical arrays (planes 0..nx+1) and per-body data
+  T* sphys = nullptr;
+  double* spart = nullptr;
+  double *sflux = nullptr, *sphisurf = nullptr;
+  int ssurf_n = -1;
+  std::vector<double> host_rad;
   double sc_gamma = 0, heat_ratio = 1, heat_ratio_new = 1, heat_ratio_old = 1;
   // particles
   int nbody = 0, nbnd = 0;
@@ -1138,6 +1149,7 @@ struct Impl : Base {
       hri[m] = hrad[m] - cfg.dr;
       hro[m] = hrad[m] + cfg.dr;
     }
+    host_rad = hr;
     if (nb > 0) {
       HIPCHK(hipMemcpyAsync(xc, hxc, n3 * sizeof(double), hipMemcpyHostToDevice, stream));
       HIPCHK(hipMemcpyAsync(rad, hr.data(), nb * sizeof(double), hipMemcpyHostToDevice, stream));
@@ -1784,31 +1796,177 @@ struct Impl : Base {
       for (auto& p : f) p = alloc<C2<double>>(n);
       double* scr = alloc<double>((size_t)4 * nx * my2 * mz);
       sscr = scr;
+      sratio = alloc<double>(8);
       for (int q = 3; q >= 0; --q) phi[q] = f[q];
       heat_ratio_old = hr;
+    } else {
+      scalar_get(sratio_h);  // heat_ratio_old as the device holds it
+      heat_ratio_old = sratio_h[2];
     }
     sc_gamma = gamma;
     heat_ratio = hr;
     heat_ratio_new = hrn;
+    sratio_h[0] = hr;
+    sratio_h[1] = hrn;
+    sratio_h[2] = heat_ratio_old;
+    sratio_h[3] = gamma;
+    HIPCHK(hipMemcpyAsync(sratio, sratio_h, 4 * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
   }
   void scalar_get(double* out) {
     if (!phi[0]) throw PibFail(PIB_ERR_STATE, "pib_scalar_set first");
-    out[0] = heat_ratio;
-    out[1] = heat_ratio_new;
-    out[2] = heat_ratio_old;
-    out[3] = sc_gamma;
+    double r[8];
+    HIPCHK(hipMemcpyAsync(r, sratio, sizeof r, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int q = 0; q < 4; ++q) out[q] = r[q];
   }
   void scalstep_modes(int rks) {
     if (rks < 1 || rks > itrmax) throw PibFail(PIB_ERR_ARG, "rks out of range");
     if (!phi[0]) throw PibFail(PIB_ERR_STATE, "pib_scalar_set first");
     const int E = my2 * mz;
-    ScalArgs a{nx, my, mz, my2, rks, sc_gamma, heat_ratio, heat_ratio_new, dst, phi[0], phi[1], phi[2], phi[3], sscr};
+    ScalArgs a{nx, my, mz, my2, rks, sc_gamma, sratio, dst, phi[0], phi[1], phi[2], phi[3], sscr};
     // a zero pivot is raised in the device status word, like the fluid
     // solves', and reported by the next call that waits on the stream
+    kb("phistep");
     k_phistep<<<(E + PHI_T - 1) / PHI_T, PHI_T, 0, stream>>>(a, &dst->status);
     HIPCHK(hipGetLastError());
-    heat_ratio_old = heat_ratio;
-    heat_ratio = heat_ratio_new;
+    ke();
+    k_ratio_roll<<<1, 64, 0, stream>>>(sratio);
+    HIPCHK(hipGetLastError());
+  }
+  // ---- the whole scalar step (stepthescalar.F90:51-95) ----
+  void scalar_surface(const double* ps) {
+    if (!have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
+    if (nbody > 0 && !ps) throw PibFail(PIB_ERR_ARG, "phisurfall is null");
+    const int n = nbody > 0 ? nbody : 1;
+    sphisurf = alloc<double>(n);
+    sflux = alloc<double>(n);
+    if (nbody > 0) HIPCHK(hipMemcpyAsync(sphisurf, ps, nbody * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    ssurf_n = nbody;
+  }
+  template <int SH>
+  void scal_launch(int rks, const ScalStep<T>& S) {
+    const int nt = plane_nt(SH);
+    const size_t lds = plane_lds();
+    const KArgs<T> A = args(rks);
+    {
+      auto kern = k_scal_phi<T, SH>;
+      set_lds(kern, lds);
+      kb("scal_phi");
+      kern<<<nx + 1, nt, lds, stream>>>(A, S);
+      HIPCHK(hipGetLastError());
+      ke();
+    }
+    k_scal_ratio<T><<<1, 64, 0, stream>>>(A, S);
+    HIPCHK(hipGetLastError());
+    {
+      auto kern = cfg.discr_scheme == PIB_SCHEME_UPWIND ? k_scal_xflux<T, SH, true> : k_scal_xflux<T, SH, false>;
+      set_lds(kern, lds);
+      kb("scal_xflux");
+      kern<<<nx, nt, lds, stream>>>(A, S);
+      HIPCHK(hipGetLastError());
+      ke();
+    }
+    {
+      auto kern = k_scal_phys<T, SH>;
+      set_lds(kern, lds);
+      kb("scal_phys");
+      kern<<<dim3(nx + 1, 6), nt, lds, stream>>>(A, S);
+      HIPCHK(hipGetLastError());
+      ke();
+    }
+    k_fill_t<T><<<(unsigned)((S.np + 255) / 256), 256, 0, stream>>>(S.fr, S.np);
+    HIPCHK(hipGetLastError());
+    if (nbody > 0) {
+      kb("scal_bnd");
+      k_scal_bnd<T><<<nbody, 256, 0, stream>>>(A, parts(), S);
+      HIPCHK(hipGetLastError());
+      ke();
+      if (cfg.dorpr) {
+        const int n = nbody * nbnd;
+        kb("scal_rev");
+        k_scal_rev<T><<<(n + 255) / 256, 256, 0, stream>>>(A, parts(), S, rks);
+        HIPCHK(hipGetLastError());
+        ke();
+      }
+    }
+    {
+      auto kern = k_scal_ff<T, SH>;
+      set_lds(kern, lds);
+      kb("scal_ff");
+      kern<<<nx, nt, lds, stream>>>(A, S);
+      HIPCHK(hipGetLastError());
+      ke();
+    }
+  }
+  void scalstep(int rks) {
+    if (rks < 1 || rks > itrmax) throw PibFail(PIB_ERR_ARG, "rks out of range");
+    if (!phi[0]) throw PibFail(PIB_ERR_STATE, "pib_scalar_set first");
+    if (!have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
+    if (ssurf_n != nbody) throw PibFail(PIB_ERR_STATE, "pib_scalar_surface first");
+    if (nx < 3) throw PibFail(PIB_ERR_ARG, "the scalar step needs nx >= 3");
+    if (nl_pending) throw PibFail(PIB_ERR_STATE, "pib_nonlinear_finish first");
+    const size_t np = (size_t)(nx + 2) * my * mz;
+    if (!sphys) {
+      sphys = alloc<T>(9 * np);
+      spart = alloc<double>((size_t)4 * (nx + 1));
+    }
+    // onlphif <- nlphif (nlphi.F90:73-86): the buffers swap roles; nlphif
+    // is rewritten whole
+    std::swap(phi[1], phi[2]);
+    ScalStep<T> S;
+    S.phif = phi[0];
+    S.nlphif = phi[1];
+    S.onlphif = phi[2];
+    S.ffphi = phi[3];
+    S.ph = sphys;
+    S.nlp = sphys + np;
+    S.onp = sphys + 2 * np;
+    S.df = sphys + 3 * np;
+    S.g = sphys + 4 * np;
+    S.fr = sphys + 7 * np;
+    S.fdd = sphys + 8 * np;
+    S.np = np;
+    S.part = spart;
+    S.ratio = sratio;
+    S.flux_body = sflux;
+    S.phisurf = sphisurf;
+    S.dx = cfg.dx;
+    S.upwind = cfg.discr_scheme == PIB_SCHEME_UPWIND;
+    S.dobnd = cfg.dobnd;
+    S.dorpr = cfg.dorpr;
+    S.frmeanfluid = cfg.include_frmeanfluid;
+    switch (cur_sh()) {
+      case 1: scal_launch<1>(rks, S); break;
+      case 2: scal_launch<2>(rks, S); break;
+      case 3: scal_launch<3>(rks, S); break;
+      default: scal_launch<0>(rks, S);
+    }
+    scalstep_modes(rks);
+  }
+  // flux_body (nbody), and (flux_global, phi_fluid_mean, phi_solid_mean, 0)
+  void scalar_flux(double* fb, double* out4) {
+    if (!sphys || ssurf_n != nbody) throw PibFail(PIB_ERR_STATE, "no scalar step has run");
+    std::vector<double> f(nbody > 0 ? nbody : 1);
+    double r[8];
+    if (nbody > 0) HIPCHK(hipMemcpyAsync(f.data(), sflux, nbody * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(r, sratio, sizeof r, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    // flux_global = sum of flux_body dA1 in body order (bc_scalar.F90:185)
+    double fg = 0.0;
+    for (int m = 0; m < nbody; ++m) {
+      const double rb = host_rad[m];
+      const double da1 = 4.0 * 3.14159265358979323846264338327950288 * (rb * cfg.dx) * (rb * cfg.dx) / (double)(float)nbnd;
+      fg = fg + f[m] * da1;
+      if (fb) fb[m] = f[m];
+    }
+    if (out4) {
+      out4[0] = fg;
+      out4[1] = r[4];
+      out4[2] = r[5];
+      out4[3] = 0.0;
+    }
   }
   void set_field(int f, const double* h) {
     ++state_gen;
@@ -2473,6 +2631,15 @@ int pib_scalar_set(void* ctx, double gamma, double heat_ratio, double heat_ratio
 int pib_scalar_get(void* ctx, double* out) {
   if (!out) return PIB_ERR_ARG;
   return dispatch(ctx, [&](auto& I) { I.scalar_get(out); });
+}
+int pib_scalstep(void* ctx, int rks) {
+  return dispatch(ctx, [&](auto& I) { I.scalstep(rks); });
+}
+int pib_scalar_surface(void* ctx, const double* phisurfall) {
+  return dispatch(ctx, [&](auto& I) { I.scalar_surface(phisurfall); });
+}
+int pib_scalar_flux(void* ctx, double* flux_body, double* out4) {
+  return dispatch(ctx, [&](auto& I) { I.scalar_flux(flux_body, out4); });
 }
 int pib_scalstep_modes(void* ctx, int rks) {
   return dispatch(ctx, [&](auto& I) { I.scalstep_modes(rks); });

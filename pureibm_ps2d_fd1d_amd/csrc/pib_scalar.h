@@ -24,7 +24,8 @@
 
 struct ScalArgs {
   int nx, my, mz, my2, rks;
-  double gamma, hr, hrn;
+  double gamma;
+  const double* ratio;        // device: heat_ratio, heat_ratio_new (read at the kernel's start)
   const DevState* st;
   C2<double>* phi;            // [i][kz][ky], i = 0..nx
   const C2<double>* nlphi;    // nlphif, onlphif, ffphi: same layout, i < nx read
@@ -51,8 +52,9 @@ __global__ void __launch_bounds__(PHI_T) k_phistep(ScalArgs A, int* status) {
   // tridiagonal rows (:118-124, 133-134) and Sherman-Morrison vectors (:178-183)
   const double a = -(dt * g * c2 / dx2), c = a;
   const double b = 1.0 + dt * g * c2 * (2.0 / dx2 + w2);
-  const double alpha = A.hrn, beta = 1.0 / A.hrn;
-  const double a1 = a * A.hrn, cn = c / A.hrn;
+  const double HR = A.ratio[0], HRN = A.ratio[1];
+  const double alpha = HRN, beta = 1.0 / HRN;
+  const double a1 = a * HRN, cn = c / HRN;
   const double v1 = cn / beta, vn = a1 / alpha;
   const double b0 = b - alpha * v1, bn = b - beta * vn;
   const double f0 = 1.0 - dt * c1 * g * w2, dc3 = dt * c3, dc4 = dt * c4, dg1 = dt * g * c1;
@@ -75,8 +77,8 @@ __global__ void __launch_bounds__(PHI_T) k_phistep(ScalArgs A, int* status) {
     ry = ry + dc4 * w.y;
     double lx, ly;
     if (i == 0) {                       // x-periodic first row (:136-140)
-      lx = (pp.x - 2.0 * p0.x) + A.hr * pm.x;
-      ly = (pp.y - 2.0 * p0.y) + A.hr * pm.y;
+      lx = (pp.x - 2.0 * p0.x) + HR * pm.x;
+      ly = (pp.y - 2.0 * p0.y) + HR * pm.y;
     } else {
       lx = (pm.x + pp.x) - 2.0 * p0.x;
       ly = (pm.y + pp.y) - 2.0 * p0.y;
@@ -186,7 +188,16 @@ __global__ void __launch_bounds__(PHI_T) k_phistep(ScalArgs A, int* status) {
       if (i0 + u < nx)
         pw[(size_t)(i0 + u) * ms] = mk(Y1[u] - sr * Z[u], Y3[u] - si * Z[u]);
   }
-  pw[(size_t)nx * ms] = mk(first.x / A.hrn, first.y / A.hrn);
+  pw[(size_t)nx * ms] = mk(first.x / HRN, first.y / HRN);
   // raised into the context's device status word, kept if already raised
   if (status_) atomicCAS(status, 0, status_);
+}
+
+// scalstep's heat-ratio roll (stepthescalar.F90:89-90) on the device ratios
+// (heat_ratio, heat_ratio_new, heat_ratio_old)
+__global__ void k_ratio_roll(double* r) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    r[2] = r[0];
+    r[0] = r[1];
+  }
 }

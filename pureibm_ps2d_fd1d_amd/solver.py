@@ -179,6 +179,26 @@ class Solver:
         """The per-mode implicit solve of scalstep (stepthescalar.F90:67-91)."""
         self._call("scalstep_modes", self.ctx, rks)
 
+    def scalar_surface(self, phisurf):
+        """Surface value of the scalar on each body (phisurfall, one species;
+        scalar_init.F90:101 sets PHISURF on every body)."""
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(phisurf, dtype=np.float64), (self.setup.nbody,)))
+        self._call("scalar_surface", self.ctx, dptr(a))
+
+    def scalstep(self, rks=1):
+        """The whole scalstep of stage rks (stepthescalar.F90:51-95): the
+        nonlinear term, the immersed-boundary forcing and the mode solve.
+        Call after the stage's velstep."""
+        self._call("scalstep", self.ctx, rks)
+
+    def scalar_flux(self):
+        """-> (flux_body (nbody,), flux_global, phi_fluid_mean, phi_solid_mean)
+        of the last scalstep (bc_scalar.F90:524-551, 185; nl_allphi.F90:380-419)."""
+        fb = np.zeros(max(self.setup.nbody, 1))
+        out = np.zeros(4)
+        self._call("scalar_flux", self.ctx, dptr(fb), dptr(out))
+        return fb[: self.setup.nbody], out[0], out[1], out[2]
+
     def nonlinear_begin(self, rks=1):
         """nonlinear(rks) up to where bcset moves particles (form_nl, the
         transforms, calc_visc, compute_mpg_at_n)."""
