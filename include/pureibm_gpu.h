@@ -198,6 +198,9 @@ int pib_scalar_set(void* ctx, double gamma, double heat_ratio, double heat_ratio
 /* out[4] = heat_ratio, heat_ratio_new, heat_ratio_old, gamma (the ratios
  * live on the device; this call waits on the stream) */
 int pib_scalar_get(void* ctx, double* out);
+/* Sets heat_ratio, heat_ratio_new, heat_ratio_old (hr3), as a scalar
+ * restart restores them (restart_funcs.F90:596); each > 0. */
+int pib_scalar_ratios(void* ctx, const double* hr3);
 /* scalstep's mode loop (stepthescalar.F90:67-91): phistep (:97-241) for
  * every (ky,kz) mode from phif, nlphif, onlphif and ffphi as they stand,
  * then heat_ratio_old = heat_ratio, heat_ratio = heat_ratio_new.  No source

@@ -1997,6 +1997,16 @@ int orc_scalar_set(void* ctx, double gamma, double heat_ratio, double heat_ratio
   return PIB_OK;
 }
 
+int orc_scalar_ratios(void* ctx, const double* r) {
+  orc* o = (orc*)ctx;
+  if (!o || !r || !(r[0] > 0) || !(r[1] > 0) || !(r[2] > 0)) return PIB_ERR_ARG;
+  if (!o->phif) return fail(o, PIB_ERR_STATE, "pib_scalar_set first");
+  o->heat_ratio = r[0];
+  o->heat_ratio_new = r[1];
+  o->heat_ratio_old = r[2];
+  return PIB_OK;
+}
+
 int orc_scalar_get(void* ctx, double* out) {
   orc* o = (orc*)ctx;
   if (!o || !out) return PIB_ERR_ARG;

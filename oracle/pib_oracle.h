@@ -31,6 +31,7 @@ int orc_compute_new_timestep(void* ctx, int rks);
 int orc_nonlinear(void* ctx, int rks);
 int orc_scalar_set(void* ctx, double gamma, double heat_ratio, double heat_ratio_new);
 int orc_scalar_get(void* ctx, double* out);
+int orc_scalar_ratios(void* ctx, const double* r);
 int orc_scalstep_modes(void* ctx, int rks);
 int orc_scalstep(void* ctx, int rks);
 int orc_scalar_surface(void* ctx, const double* phisurfall);

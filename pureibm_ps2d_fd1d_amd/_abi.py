@@ -136,6 +136,7 @@ _STEP_SIGS = {
     "scalstep": ([_VP, _I], _I),
     "scalar_surface": ([_VP, _PD], _I),
     "scalar_flux": ([_VP, _PD, _PD], _I),
+    "scalar_ratios": ([_VP, _PD], _I),
     "adjoint_begin": ([_VP], _I),
     "adjoint_step": ([_VP, _PD], _I),
     "adjoint_record": ([_VP, _I], _I),
@@ -257,6 +258,9 @@ _HOST_SIGS = {
                       _PD, _PD, _PD, _PD, _PI], _I),
     "read_restart": ([ctypes.c_char_p, _I, ctypes.POINTER(PibScalars), _I, _PD, _PD, _PD, _I, _I, _I,
                       _PD, _PD, _PD, _PD, _PI], _I),
+    "get_scalar": ([_PD], _I),
+    "save_scal_restart": ([ctypes.c_char_p, _I, _I, _I, _I, _I, _I, _PD, _PD, _PD, _PD, _PD, _PD], _I),
+    "read_scal_restart": ([ctypes.c_char_p, _I, _I, _I, _I, _I, _I, _PD, _PD, _PD, _PD, _PD, _PD], _I),
 }
 
 

@@ -95,10 +95,12 @@ def write_case(directory, run_name, *, my, lybyd, dbydx, centers=None, input_typ
                vol_frac1=None, re=20.0, vis=0.012, cfl=0.1, cfl_max=0.5, adaptive=True,
                discr_scheme="center", limiter="none", rk3=False, ramp_frac_steps=0,
                set_mpg=False, mpg=(0.0, 0.0, 0.0), revernorm=1, dr=1.0, f2=1.6,
-               include_frmeanfluid=False, dorpr=True, dobnd=False, imove=0, rhos=1.0, coeff_rest=1.0):
+               include_frmeanfluid=False, dorpr=True, dobnd=False, imove=0, rhos=1.0, coeff_rest=1.0,
+               iscalon=0, phisurf=0.0, phistream=1.0, pr=0.7):
     """Write ``<directory>/<run_name>_floparam.in`` (+ sphr_center file).
     imove=1 makes a freely moving bed (soft-sphere collisions, particle
-    density rhos, restitution coeff_rest)."""
+    density rhos, restitution coeff_rest).  iscalon=1 adds the scalar
+    (SCAL_PROPT: one species, PHISURF, PHISTREAM, PR_OR_SC_NU = pr)."""
     os.makedirs(directory, exist_ok=True)
     base = os.path.join(directory, run_name)
     if input_type is None:
@@ -121,7 +123,7 @@ def write_case(directory, run_name, *, my, lybyd, dbydx, centers=None, input_typ
         },
         "PART_PROPT": part,
         "FLAGS": {
-            "IMOVE": int(imove), "ISCALON": 0, "IRESTART": 0, "ADAPTIVE": bool(adaptive),
+            "IMOVE": int(imove), "ISCALON": int(iscalon), "IRESTART": 0, "ADAPTIVE": bool(adaptive),
             "RK3": bool(rk3), "DORPR": bool(dorpr), "DOBND": bool(dobnd),
             "INCLUDE_FRMEANFLUID": bool(include_frmeanfluid), "SET_MPG": bool(set_mpg),
             "REVERNORM": int(revernorm),
@@ -130,6 +132,10 @@ def write_case(directory, run_name, *, my, lybyd, dbydx, centers=None, input_typ
         "PSD": {"PSD_TYPE": "discrete", "NPHASES": 1},
         "GCG": {"COEFF_REST": float(coeff_rest)},
     }
+    if iscalon:
+        groups["FLAGS"]["ISCAL_RESTART"] = 0
+        groups["SCAL_PROPT"] = {"NSPMX": 1, "PHISTREAM": float(phistream), "PHISURF": float(phisurf),
+                                "PR_OR_SC_NU": float(pr), "SOURCEPRESENT": False, "SETPHIMEAN": False}
     write_namelists(base + "_floparam.in", groups)
     if centers is not None:
         with open(base + "_sphr_center.inp", "w") as f:

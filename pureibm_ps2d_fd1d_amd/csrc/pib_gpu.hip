@@ -1813,6 +1813,18 @@ struct Impl : Base {
     HIPCHK(hipMemcpyAsync(sratio, sratio_h, 4 * sizeof(double), hipMemcpyHostToDevice, stream));
     HIPCHK(hipStreamSynchronize(stream));
   }
+  void scalar_ratios(const double* r) {
+    if (!phi[0]) throw PibFail(PIB_ERR_STATE, "pib_scalar_set first");
+    if (!r || !(r[0] > 0) || !(r[1] > 0) || !(r[2] > 0)) throw PibFail(PIB_ERR_ARG, "heat ratios must be > 0");
+    double cur[4];
+    scalar_get(cur);
+    sratio_h[0] = r[0];
+    sratio_h[1] = r[1];
+    sratio_h[2] = r[2];
+    sratio_h[3] = cur[3];
+    HIPCHK(hipMemcpyAsync(sratio, sratio_h, 4 * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
   void scalar_get(double* out) {
     if (!phi[0]) throw PibFail(PIB_ERR_STATE, "pib_scalar_set first");
     double r[8];
@@ -2631,6 +2643,9 @@ int pib_scalar_set(void* ctx, double gamma, double heat_ratio, double heat_ratio
 int pib_scalar_get(void* ctx, double* out) {
   if (!out) return PIB_ERR_ARG;
   return dispatch(ctx, [&](auto& I) { I.scalar_get(out); });
+}
+int pib_scalar_ratios(void* ctx, const double* hr3) {
+  return dispatch(ctx, [&](auto& I) { I.scalar_ratios(hr3); });
 }
 int pib_scalstep(void* ctx, int rks) {
   return dispatch(ctx, [&](auto& I) { I.scalstep(rks); });

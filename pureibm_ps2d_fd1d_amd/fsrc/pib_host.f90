@@ -102,7 +102,7 @@ module pib_host
        pib_host_get_config, pib_host_get_particles, pib_host_get_points, &
        pib_host_last_error, pib_host_quadgener, pib_host_get_nsteps_hint, &
        pib_host_set_option, pib_host_time_scales, pib_host_save_restart, pib_host_read_restart, &
-       pib_host_get_dem_config
+       pib_host_get_dem_config, pib_host_get_scalar, pib_host_save_scal_restart, pib_host_read_scal_restart
 contains
 
   subroutine calculate_constants
@@ -324,8 +324,19 @@ contains
        last_err = 'ALIASFLAG = 1 is not supported'; st = PIB_ERR_UNSUPPORTED; return
     end if
     if (iscalon /= 0) then
-       last_err = 'ISCALON: scalar transport is not built'
-       st = PIB_ERR_UNSUPPORTED; return
+       ! one species, no source term (the GPU step's scope; scalar_init.F90:58-68, 284-350)
+       if (nspmx /= 1) then
+          last_err = 'NSPMX > 1: one scalar species is supported'; st = PIB_ERR_UNSUPPORTED; return
+       end if
+       if (sourcepresent) then
+          last_err = 'SOURCEPRESENT = T (add_source) is not supported'; st = PIB_ERR_UNSUPPORTED; return
+       end if
+       if (.not. (pr_or_sc_nu > zero)) then
+          last_err = 'PR_OR_SC_NU must be > 0'; st = PIB_ERR_ARG; return
+       end if
+       if (imove == 1) then
+          last_err = 'ISCALON with IMOVE = 1: the scalar step is for fixed beds'; st = PIB_ERR_UNSUPPORTED; return
+       end if
     end if
     if (imove == 1 .and. (cage_simul .or. only_dem)) then
        last_err = 'IMOVE with CAGE_SIMUL or ONLY_DEM: run the DEM alone through pib_dem_* instead'
@@ -531,6 +542,8 @@ contains
     if (t_vis - t_min < small_number) t_min = t_vis
     lchar = dx
     if (re < one) dt_tmp_vis = re*dbydx*dt_tmp_conv/(cfl*vfl)
+    ! initscal (scalar_init.F90:73-98): gamma = vis/Pr, dt_tmp_diff = dx^2 (1 - maxvolfrac)/gamma
+    if (iscalon == 1) dt_tmp_diff = (dx*dx*(one-maxvolfrac))/(vis/pr_or_sc_nu)
     dt_tmp_grav = large_number
     dtcoll = large_number
     dt = min(dt_tmp_conv, dt_tmp_vis, dt_tmp_grav, dt_tmp_diff)
@@ -549,7 +562,29 @@ contains
     ts_conv = t_conv
     ts_vis = (char_length*char_length)/vis
     ts_diff = large_number
+    if (iscalon == 1) ts_diff = (char_length*char_length)/(vis/pr_or_sc_nu)
   end function pib_host_finish
+
+  ! The scalar set-up of the run (scal_propt, scalar_init.F90:58-204):
+  ! out = (iscalon, gamma = vis/Pr, phisurf, phistream, Pr, iscal_restart,
+  ! voldom, char_length); valid after pib_host_configure.
+  function pib_host_get_scalar(out) bind(C, name="pib_host_get_scalar") result(st)
+    real(c_double), intent(out) :: out(8)
+    integer(c_int) :: st
+    st = PIB_OK
+    out = zero
+    if (.not. configured) then
+       st = PIB_ERR_STATE; last_err = 'pib_host_configure first'; return
+    end if
+    out(1) = real(iscalon, prcn)
+    out(2) = vis/pr_or_sc_nu
+    out(3) = phisurf
+    out(4) = phistream
+    out(5) = pr_or_sc_nu
+    out(6) = merge(real(iscal_restart, prcn), -one, iscal_restart /= undefined_i)
+    out(7) = doml(1)*doml(2)*doml(3)
+    out(8) = char_length
+  end function pib_host_get_scalar
 
   ! t_conv, t_vis, t_diff of the _norm_drag.dat time columns
   ! (boundary_condition.F90:861); valid after pib_host_finish
@@ -693,6 +728,77 @@ contains
       end if
     end subroutine write_config
   end function pib_host_save_restart
+
+  ! The scalar's restart file <RUN>_scal_<c>.rst (restart_funcs.F90:315-333,
+  ! 384-387, 572-596), serial records in the reference's order: nu_error,
+  ! nu_old; nu_error_array(1:nerr_steps); phif; nlphif(1:nx+1); phisurfall;
+  ! phirmean, heat_ratio, heat_ratio_old.  count is the flow files' count
+  ! (pib_host_save_restart's count_out); the other count's scalar file is
+  ! deleted.  nu3 = (nu_error, nu_old, phirmean), hr2 = (heat_ratio,
+  ! heat_ratio_old), nuerr(1:nerr_steps).
+  function pib_host_save_scal_restart(run, n, count, nx_, my2_, mz_, nb, phif, nlphif, phisurfall, &
+       nu3, hr2, nuerr) bind(C, name="pib_host_save_scal_restart") result(st)
+    character(kind=c_char), intent(in) :: run(*)
+    integer(c_int), value :: n, count, nx_, my2_, mz_, nb
+    complex(c_double_complex), intent(in) :: phif(nx_ + 1, my2_, mz_), nlphif(nx_ + 1, my2_, mz_)
+    real(c_double), intent(in) :: phisurfall(*), nu3(3), hr2(2), nuerr(*)
+    integer(c_int) :: st
+    character(len=1024) :: run_name
+    character(len=10) :: c1
+    integer :: ru, ios
+    st = PIB_OK
+    call cstr(run, n, run_name)
+    if (count /= 1 .and. count /= 2) then
+       last_err = 'save_scal_restart: count must be 1 or 2'; st = PIB_ERR_ARG; return
+    end if
+    write(c1, '(I1)') count
+    open(newunit=ru, file=trim(run_name)//'_scal_'//trim(c1)//'.rst', form=trim(rstsave), status='replace', &
+         iostat=ios)
+    if (ios /= 0) then
+       last_err = 'save_scal_restart: cannot open the scalar restart file'; st = PIB_ERR_ARG; return
+    end if
+    write(ru) nu3(1), nu3(2)
+    write(ru) nuerr(1:nerr_steps)
+    write(ru) phif
+    write(ru) nlphif
+    write(ru) phisurfall(1:nb)
+    write(ru) nu3(3), hr2(1), hr2(2)
+    close(ru, status='keep')
+    write(c1, '(I1)') 3 - count
+    call remove_file(trim(run_name)//'_scal_'//trim(c1)//'.rst')
+  end function pib_host_save_scal_restart
+
+  ! read_unformatted's scalar part (restart_funcs.F90:572-596) for count c.
+  function pib_host_read_scal_restart(run, n, count, nx_, my2_, mz_, nb, phif, nlphif, phisurfall, &
+       nu3, hr2, nuerr) bind(C, name="pib_host_read_scal_restart") result(st)
+    character(kind=c_char), intent(in) :: run(*)
+    integer(c_int), value :: n, count, nx_, my2_, mz_, nb
+    complex(c_double_complex), intent(out) :: phif(nx_ + 1, my2_, mz_), nlphif(nx_ + 1, my2_, mz_)
+    real(c_double), intent(out) :: phisurfall(*), nu3(3), hr2(2), nuerr(*)
+    integer(c_int) :: st
+    character(len=1024) :: run_name
+    character(len=10) :: c1
+    integer :: ru, ios
+    st = PIB_OK
+    call cstr(run, n, run_name)
+    write(c1, '(I1)') count
+    open(newunit=ru, file=trim(run_name)//'_scal_'//trim(c1)//'.rst', form=trim(rstsave), status='old', &
+         action='read', iostat=ios)
+    if (ios /= 0) then
+       last_err = 'read_scal_restart: cannot open '//trim(run_name)//'_scal_'//trim(c1)//'.rst'
+       st = PIB_ERR_ARG; return
+    end if
+    read(ru, iostat=ios) nu3(1), nu3(2)
+    if (ios == 0) read(ru, iostat=ios) nuerr(1:nerr_steps)
+    if (ios == 0) read(ru, iostat=ios) phif
+    if (ios == 0) read(ru, iostat=ios) nlphif
+    if (ios == 0) read(ru, iostat=ios) phisurfall(1:nb)
+    if (ios == 0) read(ru, iostat=ios) nu3(3), hr2(1), hr2(2)
+    close(ru)
+    if (ios /= 0) then
+       last_err = 'read_scal_restart: short or mismatched scalar restart file'; st = PIB_ERR_ARG
+    end if
+  end function pib_host_read_scal_restart
 
   subroutine remove_file(name)
     character(len=*), intent(in) :: name

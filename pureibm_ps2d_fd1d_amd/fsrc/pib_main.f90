@@ -12,7 +12,13 @@
 ! (boundary_condition.F90:874-884).  With IMOVE = 1 the particles move: each
 ! stage runs nonlinear to the drag, des_time_march over dt (the GPU DEM),
 ! grid_nodes_insphere/update_nrpr_array for the new centres, the forcing and
-! velstep (boundary_condition.F90:539-554).
+! velstep (boundary_condition.F90:539-554).  With ISCALON = 1 each stage
+! ends with the scalar step (dtibm_main_scal_flo.F90:489-515), and once per
+! step the scalar records are written (bc_scalar.F90:366-376):
+!   RUN_NAME_scalflux.dat        tscal/t_conv, tscal/t_vis, tscal/t_diff,
+!                                -flux_body/(phisurf-phistream) per body,
+!                                -flux_global/(phisurf-phistream)
+!   RUN_NAME_nusselt_comps.dat   the same times, the Nusselt number, heat_ratio
 program pureibm_gpu
   use iso_c_binding
   use pib_types
@@ -32,6 +38,11 @@ program pureibm_gpu
   integer(c_long_long) :: nsolid
   real(c_double) :: tc, tv, td, ms4(4)
   character(len=600) :: fname
+  real(c_double) :: scal(8), tscal, fg4(4), hr4(4), nusselt
+  real(c_double), allocatable :: fbody(:)
+  complex(c_double_complex), allocatable :: phi0(:,:,:)
+  logical :: scalar_on
+  integer :: uflux, unuss, i
 
   call get_command_argument(1, run_name)
   if (len_trim(run_name) == 0) then
@@ -67,6 +78,22 @@ program pureibm_gpu
   write(*,'(A,3I6,A,I6,A,I8)') 'GRID ', nx, my, mz, '  NBODY ', nb, '  SOLID NODES ', nsolid
 
   itrmax = merge(3, 1, cfg%rk3 /= 0)
+  call check(pib_host_get_scalar(scal), 'scalar set-up')
+  scalar_on = nint(scal(1)) == 1
+  if (scalar_on) then
+     ! initscal (scalar_init.F90:73, 101, 117-171): phi = 1, ratios 1, phisurf on every body
+     call gpu(pib_scalar_set(ctx, scal(2), 1.d0, 1.d0), 'pib_scalar_set')
+     allocate(fbody(max(nb,1)), phi0(nx+1, my/2+1, mz))
+     fbody = scal(3)
+     call gpu(pib_scalar_surface(ctx, fbody), 'pib_scalar_surface')
+     phi0 = (0.d0, 0.d0)
+     phi0(:,1,1) = (1.d0, 0.d0)
+     call gpu(pib_set_field(ctx, PIB_F_PHI, phi0), 'pib_set_field(phi)')
+     tscal = 0.d0
+     open(newunit=uflux, file=trim(run_name)//'_scalflux.dat', form='formatted', status='replace')
+     open(newunit=unuss, file=trim(run_name)//'_nusselt_comps.dat', form='formatted', status='replace')
+     write(*,'(A,g17.8,A,g17.8)') 'SCALAR ON: GAMMA ', scal(2), '  PR ', scal(5)
+  end if
   if (cfg%move_particles /= 0) then
      ! des_time_march(.TRUE.) (initialize_flo.F90:630-637)
      call check(pib_host_get_dem_config(dc), 'DEM constants')
@@ -99,6 +126,17 @@ program pureibm_gpu
            call gpu(pib_nonlinear_finish(ctx, rks), 'pib_nonlinear_finish')
            call gpu(pib_velstep(ctx, rks), 'pib_velstep')
         end do
+     else if (scalar_on) then
+        st = PIB_OK
+        do rks = 1, itrmax
+           call gpu(pib_compute_new_timestep(ctx, rks), 'pib_compute_new_timestep')
+           call gpu(pib_get_scalars(ctx, sc), 'pib_get_scalars')
+           tscal = tscal + sc%dt               ! dependent_functions.F90:1257
+           st = pib_nonlinear(ctx, rks)
+           if (st /= PIB_OK) exit
+           call gpu(pib_velstep(ctx, rks), 'pib_velstep')
+           call gpu(pib_scalstep(ctx, rks), 'pib_scalstep')
+        end do
      else
         st = pib_step(ctx, 1)
      end if
@@ -118,9 +156,24 @@ program pureibm_gpu
           sc%norm_drag, sc%usmean_act(1:3), sc%ufmean(1:3)/cfg%ucharmod
      write(ucomp,'(20(2x, e20.12))') sc%pres_drag, sc%visc_drag, sc%total_drag, sc%total_drag_mpg, &
           sc%norm_drag, abs(sc%total_drag - sc%total_drag_mpg)/(sc%total_drag + 1.0e-35)
+     if (scalar_on) then
+        call gpu(pib_scalar_flux(ctx, fbody, fg4), 'pib_scalar_flux')
+        call gpu(pib_scalar_get(ctx, hr4), 'pib_scalar_get')
+        ! bc_scalar.F90:272-273: nusselt_avg(3)
+        nusselt = -(fg4(1)*scal(8)*scal(8))/(6.d0*scal(7)*sc%maxvolfrac*(scal(3)-scal(4)))
+        write(uflux,'(400(2x,e20.12))') tscal/tc, tscal/tv, tscal/td, &
+             (-fbody(i)/(scal(3)-scal(4)), i=1,nb), -fg4(1)/(scal(3)-scal(4))
+        ! heat_ratio of the step (rolled to heat_ratio_old at its end)
+        write(unuss,'(20(2x,e20.12))') tscal/tc, tscal/tv, tscal/td, nusselt, hr4(3)
+        write(*,'(A25,4(2x,g17.8))') 'NUSSELT, HEAT RATIO', nusselt, hr4(1)
+     end if
   end do
   close(unorm)
   close(ucomp)
+  if (scalar_on) then
+     close(uflux)
+     close(unuss)
+  end if
   call gpu(pib_phase_times(ctx, ms4), 'pib_phase_times')
   write(*,'(A,4F10.4)') 'LAST STEP MS (HYDRO NL BC VP) ', ms4
   if (cfg%move_particles /= 0) st = pib_dem_destroy(dem)

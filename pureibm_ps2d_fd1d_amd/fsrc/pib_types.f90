@@ -247,6 +247,30 @@ module pib_types
        integer(c_int), value :: rks
        integer(c_int) :: st
      end function pib_scalstep_modes
+     function pib_scalar_ratios(ctx, hr3) bind(C, name="pib_scalar_ratios") result(st)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: hr3(3)
+       integer(c_int) :: st
+     end function pib_scalar_ratios
+     function pib_scalstep(ctx, rks) bind(C, name="pib_scalstep") result(st)
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: rks
+       integer(c_int) :: st
+     end function pib_scalstep
+     function pib_scalar_surface(ctx, phisurfall) bind(C, name="pib_scalar_surface") result(st)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: phisurfall(*)
+       integer(c_int) :: st
+     end function pib_scalar_surface
+     function pib_scalar_flux(ctx, flux_body, out4) bind(C, name="pib_scalar_flux") result(st)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: flux_body(*), out4(4)
+       integer(c_int) :: st
+     end function pib_scalar_flux
      function pib_phase_times(ctx, ms4) bind(C, name="pib_phase_times") result(st)
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx

@@ -48,6 +48,11 @@ class RunSetup:
         self.bndpts = np.zeros((3, self.nbnd), order="F")
         self.cd = np.zeros((3, self.nbnd), order="F")
         host.pib_host_get_points(dptr(self.bndpts), dptr(self.cd))
+        sc = np.zeros(8)
+        host.pib_host_get_scalar(dptr(sc))
+        # scal_propt of an ISCALON = 1 deck (scalar_init.F90:58-204)
+        self.scalar = dict(iscalon=int(sc[0]), gamma=sc[1], phisurf=sc[2], phistream=sc[3], pr=sc[4],
+                           iscal_restart=int(sc[5]), voldom=sc[6], char_length=sc[7])
 
     def _load(self):
         """(Re)read the namelists and configure; the Fortran host holds one run."""
@@ -137,6 +142,15 @@ class Solver:
         self.nx, self.my, self.mz = c.nx, c.my, c.mz
         self.my2 = c.my // 2 + 1
         self.itrmax = 3 if c.rk3 else 1
+        sc = getattr(setup, "scalar", None)
+        if sc and sc["iscalon"] == 1:
+            # initscal (scalar_init.F90:73, 101, 117-171): gamma = vis/Pr, the
+            # ratios 1, phisurf on every body, phi = 1 on planes 1..nx+1
+            self.scalar_set(sc["gamma"], 1.0, 1.0)
+            self.scalar_surface(np.full(setup.nbody, sc["phisurf"]))
+            phi = np.zeros((self.nx + 1, self.my2, self.mz), dtype=np.complex128)
+            phi[:, 0, 0] = 1.0
+            self.set_field("phi", phi)
 
     def _fn(self, name):
         return getattr(self.lib, self.p + name)
@@ -295,7 +309,8 @@ class Solver:
         """save_part_restart (restart_funcs.F90:69-128): writes the reference's
         restart files <RUN>_u_<c>.rst, _nl_<c>.rst, _p_<c>.rst,
         _sphere_config.rst and <RUN>_RESTART for this context's state, the
-        count c alternating 1, 2.  xc/velbdy default to the set-up's centres
+        count c alternating 1, 2 (and <RUN>_scal_<c>.rst for an ISCALON
+        deck).  xc/velbdy default to the set-up's centres
         at rest (pass the current ones for a moving bed); ferror_array is the
         NERR_STEPS history of the caller's convergence check (ones when not
         kept).  Returns c."""
@@ -317,12 +332,15 @@ class Solver:
                                         dptr(ferr), ctypes.byref(cnt))
         if rc != PIB_OK:
             raise PibError(rc, _abi.host_error())
+        if st.scalar["iscalon"] == 1:
+            self._save_scal_restart(cnt.value, nerr)
         return cnt.value
 
     def load_restart(self):
         """read_restart (restart_funcs.F90:411-616): reads the files of the
         count named in <RUN>_RESTART into this context (u, nl, p and the
-        restart scalars; the centres too when they differ from the set-up's).
+        restart scalars; the centres too when they differ from the set-up's;
+        the scalar's file too for an ISCALON deck with ISCAL_RESTART = 1).
         Returns (count, ferror_array, xc, velbdy)."""
         st = self.setup
         st._load()
@@ -349,7 +367,50 @@ class Solver:
         self.set_scalars(s)
         if not np.array_equal(xc, st.xc) or vel.any():
             self.update_particles(xc, vel)
+        if st.scalar["iscalon"] == 1 and st.scalar["iscal_restart"] == 1:
+            self._load_scal_restart(cnt.value, nerr)
         return cnt.value, ferr, xc, vel
+
+    def _save_scal_restart(self, count, nerr, nu_error_array=None, nu=(1.0, 0.0)):
+        """<RUN>_scal_<count>.rst (restart_funcs.F90:315-333): phif, nlphif,
+        phisurfall and the heat ratios; the Nusselt history as given (ones
+        when not kept)."""
+        st = self.setup
+        host = _abi.load_host()
+        phif, nlphif = (np.asfortranarray(self.field(k)) for k in ("phi", "nlphi"))
+        hr = self.scalar_get()
+        ps = np.full(max(st.nbody, 1), st.scalar["phisurf"])
+        nue = np.ones(nerr) if nu_error_array is None else np.ascontiguousarray(nu_error_array, dtype=np.float64)
+        nu3 = np.array([nu[0], nu[1], 0.0])
+        hr2 = np.array([hr[0], hr[2]])
+        rn = st.run_name.encode()
+        rc = host.pib_host_save_scal_restart(rn, len(rn), count, self.nx, self.my2, self.mz, st.nbody,
+                                             dptr(phif), dptr(nlphif), dptr(ps), dptr(nu3), dptr(hr2), dptr(nue))
+        if rc != PIB_OK:
+            raise PibError(rc, _abi.host_error())
+
+    def _load_scal_restart(self, count, nerr):
+        st = self.setup
+        host = _abi.load_host()
+        nx, my2, mz = self.nx, self.my2, self.mz
+        phif = np.zeros((nx + 1, my2, mz), dtype=np.complex128, order="F")
+        nlphif = np.zeros_like(phif)
+        ps = np.zeros(max(st.nbody, 1))
+        nu3, hr2, nue = np.zeros(3), np.zeros(2), np.zeros(nerr)
+        rn = st.run_name.encode()
+        rc = host.pib_host_read_scal_restart(rn, len(rn), count, nx, my2, mz, st.nbody, dptr(phif), dptr(nlphif),
+                                             dptr(ps), dptr(nu3), dptr(hr2), dptr(nue))
+        if rc != PIB_OK:
+            raise PibError(rc, _abi.host_error())
+        self.set_field("phi", phif)
+        self.set_field("nlphi", nlphif)
+        self.scalar_surface(ps[: st.nbody])
+        self.scalar_ratios(hr2[0], hr2[0], hr2[1])
+        return nue, nu3
+
+    def scalar_ratios(self, heat_ratio, heat_ratio_new, heat_ratio_old):
+        r = np.array([heat_ratio, heat_ratio_new, heat_ratio_old], dtype=np.float64)
+        self._call("scalar_ratios", self.ctx, dptr(r))
 
     # ---- state access in the reference layout ----
     def _shape(self, name):

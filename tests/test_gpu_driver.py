@@ -85,3 +85,42 @@ def test_driver_moves_an_imove_bed(tmp_path, orc):
     hist = np.array(hist)
     assert np.max(np.abs(nd[:, 3] - hist) / np.abs(hist)) < 1e-9
     assert np.any(np.abs(nd[-1, 5:8]) > 0)       # usmean_act of the moving particles
+
+
+def test_driver_runs_a_scalar_deck(tmp_path, orc):
+    """ISCALON = 1: each stage ends with the scalar step; the driver's
+    _scalflux.dat and _nusselt_comps.dat records equal the oracle chain's
+    (same stage calls, tscal advanced by dt each stage) to 1e-9."""
+    print("fortran driver, 24^3 bed with the scalar, 5 steps", flush=True)
+    kw = dict(kind="random", my=24, dbydx=6, nbody=3, seed=9, re=20.0, discr_scheme="upwind", iscalon=1,
+              phisurf=0.0, phistream=1.0, pr=0.7)
+    st = make_setup(tmp_path, "DSC", **kw)
+    nsteps = 5
+    run = os.path.join(str(tmp_path), "DSC")
+    r = subprocess.run([DRIVER, run, str(nsteps), "fp64"], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0 and "SCALAR ON" in r.stdout
+    flux = np.loadtxt(run + "_scalflux.dat", ndmin=2)
+    nuss = np.loadtxt(run + "_nusselt_comps.dat", ndmin=2)
+    assert flux.shape == (nsteps, 3 + st.nbody + 1) and nuss.shape == (nsteps, 5)
+    host = _abi.load_host()
+    tc, tv, td = (ctypes.c_double() for _ in range(3))
+    o = solver.Solver(st, lib=orc.load(), prefix="orc_")
+    host.pib_host_time_scales(ctypes.byref(tc), ctypes.byref(tv), ctypes.byref(td))
+    sc = st.scalar
+    dphi = sc["phisurf"] - sc["phistream"]
+    tscal = 0.0
+    for n in range(nsteps):
+        o.compute_new_timestep(1)
+        tscal += o.scalars().dt
+        o.nonlinear(1)
+        o.velstep(1)
+        o.scalstep(1)
+        fb, fg, _, _ = o.scalar_flux()
+        nus = -(fg * sc["char_length"] ** 2) / (6 * sc["voldom"] * o.scalars().maxvolfrac * dphi)
+        ref = np.concatenate([[tscal / tc.value, tscal / tv.value, tscal / td.value], -fb / dphi, [-fg / dphi]])
+        assert np.max(np.abs(flux[n] - ref) / np.maximum(np.abs(ref), 1e-30)) < 1e-9, n
+        assert abs(nuss[n, 3] - nus) <= 1e-9 * abs(nus)
+        assert abs(nuss[n, 4] - o.scalar_get()[2]) <= 1e-11 * abs(o.scalar_get()[2])  # e20.12 text
+    print(f"  Nusselt after {nsteps} steps: {nuss[-1, 3]:.10e}, heat ratio {nuss[-1, 4]:.12f}", flush=True)
+    o.close()

@@ -41,3 +41,28 @@ def test_gpu_restart_files_continue_on_the_oracle(orc, tmp_path):
     assert abs(g.scalars().norm_drag - o.scalars().norm_drag) <= 1e-9 * abs(o.scalars().norm_drag)
     g.close()
     o.close()
+
+
+def test_gpu_scalar_restart_continues(tmp_path):
+    """An ISCALON run saved with its scalar file and read back into a fresh
+    GPU context continues like the uninterrupted run (to 1e-12: the
+    scalar's spreading adds with atomics, so two GPU runs agree to
+    round-off rather than bitwise)."""
+    from test_restart import _scal_steps, _with_iscal_restart
+    st = make_setup(tmp_path, "GS", kind="random", my=32, dbydx=8, nbody=4, seed=9, discr_scheme="upwind",
+                    iscalon=1)
+    a = solver.Solver(st)
+    _scal_steps(a, 3)
+    assert a.save_restart() == 1
+    _scal_steps(a, 3)
+    b = solver.Solver(_with_iscal_restart(st))
+    b.load_restart()
+    _scal_steps(b, 3)
+    for k in ("phi", "nlphi"):
+        e = rel_err(b.field(k), a.field(k))
+        print(f"  {k} {e:.1e}", flush=True)
+        assert e < 1e-12, k
+    ra, rb = a.scalar_get(), b.scalar_get()
+    assert all(abs(x - y) <= 1e-13 * abs(x) for x, y in zip(ra[:3], rb[:3]))
+    a.close()
+    b.close()

@@ -299,3 +299,37 @@ def test_batched_realizations_are_reproducible(tmp_path):
         nl = open(b1 + "_floparam.in").read()
         sizes.append(int(nl.split("MY = ")[1].split(",")[0]))
     assert sizes == [96, 128, 160, 96, 128, 160]
+
+
+def test_scalar_deck_is_configured(tmp_path):
+    """An ISCALON = 1 deck: gamma = vis/Pr (scalar_init.F90:73), the initial
+    dt takes the diffusive limit dx^2 (1 - maxvolfrac)/gamma beside the
+    convective dx/umax (initialize_flo.F90:620-666), t_diff = d^2/gamma."""
+    from conftest import make_setup
+    vis, pr = 0.012, 0.05          # a small Pr makes the diffusive limit the smaller one
+    st = make_setup(tmp_path, "SC", kind="simple", my=16, dbydx=4, re=20.0, iscalon=1, pr=pr, vis=vis)
+    sc = st.scalar
+    assert sc["iscalon"] == 1 and abs(sc["gamma"] - vis / pr) < 1e-15 and sc["phisurf"] == 0.0
+    s = st.finish(100)
+    dx = st.config.dx
+    phi = 100 / (st.nx * st.my * st.mz)
+    dt_diff = dx * dx * (1 - phi) / (vis / pr)
+    assert abs(s.dt - 0.1 * dt_diff) < 1e-15 * dt_diff   # CFL = 0.1 of write_case
+    host = _abi.load_host()
+    tc, tv, td = (ctypes.c_double() for _ in range(3))
+    host.pib_host_time_scales(ctypes.byref(tc), ctypes.byref(tv), ctypes.byref(td))
+    assert abs(td.value - 1.0 / (vis / pr)) < 1e-12   # DIA_PHYS = 1
+
+
+def test_scalar_deck_limits(tmp_path):
+    from conftest import make_setup
+    from pureibm_ps2d_fd1d_amd import configs
+    d = str(tmp_path)
+    configs.simple_cubic(d, "S2", my=16, dbydx=4, iscalon=1)
+    p = os.path.join(d, "S2_floparam.in")
+    txt = open(p).read().replace("NSPMX = 1", "NSPMX = 2")
+    open(p, "w").write(txt)
+    with pytest.raises(solver.PibError) as e:
+        solver.RunSetup(os.path.join(d, "S2"))
+    assert e.value.code == _abi.PIB_ERR_UNSUPPORTED
+

@@ -103,3 +103,51 @@ def test_missing_restart_is_an_error(oracle_run):
     st, o = oracle_run
     with pytest.raises(solver.PibError):
         o.load_restart()
+
+
+def _scal_steps(s, n):
+    for _ in range(n):
+        s.compute_new_timestep(1)
+        s.nonlinear(1)
+        s.velstep(1)
+        s.scalstep(1)
+
+
+def _with_iscal_restart(st):
+    p = st.run_name + "_floparam.in"
+    txt = open(p).read()
+    assert "ISCAL_RESTART = 0" in txt
+    open(p, "w").write(txt.replace("ISCAL_RESTART = 0", "ISCAL_RESTART = 1"))
+    return solver.RunSetup(st.run_name)
+
+
+def test_scalar_restart_records_and_continuation(orc, tmp_path):
+    """<RUN>_scal_<c>.rst (restart_funcs.F90:315-333): nu_error, nu_old;
+    the NERR_STEPS history; phif; nlphif(1:nx+1); phisurfall; phirmean,
+    heat_ratio, heat_ratio_old.  Read back with ISCAL_RESTART = 1 into a
+    fresh context, the scalar run continues bit for bit; the next save takes
+    count 2 and removes the count-1 scalar file."""
+    st = make_setup(tmp_path, "SR", kind="random", my=24, dbydx=6, nbody=3, seed=8, discr_scheme="upwind",
+                    iscalon=1)
+    a = solver.Solver(st, lib=orc.load(), prefix="orc_")
+    _scal_steps(a, 3)
+    c = a.save_restart()
+    from pureibm_ps2d_fd1d_amd import _abi
+    nerr = _abi.load_host().pib_host_get_nsteps_hint()
+    nx, my2, mz, nb = a.nx, a.my2, a.mz, st.nbody
+    r = records(st.run_name + f"_scal_{c}.rst")
+    assert [len(x) for x in r] == [16, 8 * nerr, 16 * (nx + 1) * my2 * mz, 16 * (nx + 1) * my2 * mz, 8 * nb, 24]
+    assert r[2] == np.asfortranarray(a.field("phi")).tobytes(order="F")
+    assert r[3] == np.asfortranarray(a.field("nlphi")).tobytes(order="F")
+    hr = a.scalar_get()
+    assert struct.unpack("<3d", r[5]) == (0.0, hr[0], hr[2]) and hr[0] != 1.0
+    assert struct.unpack(f"<{nb}d", r[4]) == (0.0,) * nb
+    _scal_steps(a, 2)
+    b = solver.Solver(_with_iscal_restart(st), lib=orc.load(), prefix="orc_")
+    b.load_restart()
+    _scal_steps(b, 2)
+    for k in ("u", "phi", "nlphi", "onlphi", "ffphi"):
+        assert np.array_equal(b.field(k), a.field(k)), k
+    assert b.scalar_get()[:3] == a.scalar_get()[:3]
+    assert b.save_restart() == 2
+    assert os.path.exists(st.run_name + "_scal_2.rst") and not os.path.exists(st.run_name + "_scal_1.rst")
