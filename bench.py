@@ -319,6 +319,73 @@ def time_scalar_solve(setup, reps=20):
             "algorithmic_GB_s": 80.0 * pts / t / 1e9}
 
 
+def _kernel_times(g):
+    lib = g.lib
+    maxn = 64
+    names = ctypes.create_string_buffer(32 * maxn)
+    tot = np.zeros(maxn)
+    cnt = np.zeros(maxn, dtype=np.int32)
+    n = ctypes.c_int()
+    lib.pib_kernel_times(g.ctx, maxn, names, _abi.dptr(tot),
+                         cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.byref(n))
+    out = {}
+    for q in range(n.value):
+        nm = names.raw[32 * q:32 * q + 32].split(b"\0")[0].decode()
+        out[nm] = (float(tot[q]), int(cnt[q]))
+    return out
+
+
+def run_scalar(steps=20, warmup=3, ksteps=3):
+    """Scalar transport (ISCALON = 1) on the C2 bed (128^3, 195 spheres):
+    FP32-bulk fluid, scalar transforms in FP32, scalar fields and mode solve
+    in FP64.  Each step is the fluid step's stage calls and then pib_scalstep
+    (the whole scalstep: nlphi, the reversal-pair forcing, calc_diffn and the
+    surface flux, the mode solve), enqueued back to back and drained at the
+    end; the same steps without the scalar give the fluid's share.  Per-kernel
+    times of the scalar step from HIP events over ksteps more steps."""
+    work = tempfile.mkdtemp(prefix="pib_scal_")
+    configs.write_workload(work, "C2", iscalon=1, discr_scheme="upwind")
+    st = solver.RunSetup(os.path.join(work, "C2"), precision=_abi.PIB_FP32_BULK)
+    g = solver.Solver(st)
+
+    def stage(scal):
+        g.compute_new_timestep(1)
+        g.nonlinear(1)
+        g.velstep(1)
+        if scal:
+            g.scalstep(1)
+
+    for _ in range(warmup):
+        stage(True)
+    g.lib.pib_sync(g.ctx)
+    res = {}
+    for scal in (True, False):
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            stage(scal)
+        if g.lib.pib_sync(g.ctx) != 0:
+            raise RuntimeError(g.lib.pib_last_error(g.ctx).decode())
+        res[scal] = (time.perf_counter() - t0) / steps
+    g.lib.pib_set_timing(g.ctx, 1)
+    for _ in range(ksteps):
+        stage(True)
+    g.lib.pib_sync(g.ctx)
+    kern = {k: v[0] / max(v[1], 1) * 1e3 for k, v in _kernel_times(g).items()
+            if k.startswith("scal_") or k == "phistep"}
+    g.lib.pib_set_timing(g.ctx, 0)
+    fb, fg, pf, ps = g.scalar_flux()
+    sc = st.scalar
+    nus = -(fg * sc["char_length"] ** 2) / (6 * sc["voldom"] * g.scalars().maxvolfrac * (sc["phisurf"] - sc["phistream"]))
+    hr = g.scalar_get()[0]
+    g.close()
+    pts = st.nx * st.my * st.mz
+    return {"workload": "C2 bed with the scalar (ISCALON = 1, Pr 0.7, phisurf 0, phistream 1), upwind x-fluxes",
+            "steps": steps, "ms_per_step_with_scalar": 1e3 * res[True], "ms_per_step_fluid_only": 1e3 * res[False],
+            "scalar_ms_per_step": 1e3 * (res[True] - res[False]),
+            "value": pts / (res[True] - res[False]) / 1e6, "unit": "Mgridpt-scalar-steps/s",
+            "kernel_us_per_launch": kern, "nusselt": nus, "heat_ratio": hr, "phi_fluid_mean": pf}
+
+
 def time_moving_bed(workload, steps=5, warmup=2):
     """Freely moving bed on the default workload's 128^3/195-sphere bed,
     FP32-bulk fluid with the FP64 DEM: per step nonlinear to the drag, the DEM
@@ -433,6 +500,7 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=100)
     ap.add_argument("--no-dem", action="store_true", help="skip the DEM sub-cycle figure")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (C4) figure")
+    ap.add_argument("--no-scalar", action="store_true", help="skip the scalar-step (ISCALON) figure")
     ap.add_argument("--dem-marches", type=int, default=10)
     ap.add_argument("--collect-traffic", action="store_true",
                     help="measure HBM bytes per dispatch with rocprofv3 counter passes (child processes)")
@@ -529,6 +597,11 @@ def main():
             out["scalar_solve"] = time_scalar_solve(st32)
         except Exception as e:  # reported, never fatal for the headline figure
             out["scalar_solve"] = {"error": str(e)}
+    if not a.no_scalar and a.only in ("both", "fp32"):
+        try:
+            out["scalar_step_c2"] = run_scalar()
+        except Exception as e:  # reported, never fatal for the headline figure
+            out["scalar_step_c2"] = {"error": str(e)}
     if not a.no_cpu:
         ncpu = max(1, a.cpu_steps)
         nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)

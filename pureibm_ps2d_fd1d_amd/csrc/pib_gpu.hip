@@ -741,6 +741,8 @@ struct Impl : Base {
   double sratio_h[8] = {};
   // the whole scalar step: physical arrays (planes 0..nx+1) and per-body data
   T* sphys = nullptr;
+  T *s_nlp = nullptr, *s_onp = nullptr;  // the physical nl and onl of phi (they swap roles a step)
+  bool snlp_valid = false;                // s_nlp = ff2cr(nlphif) of the fields as they stand
   double* spart = nullptr;
   double *sflux = nullptr, *sphisurf = nullptr;
   int ssurf_n = -1;
@@ -1872,6 +1874,13 @@ struct Impl : Base {
     }
     k_scal_ratio<T><<<1, 64, 0, stream>>>(A, S);
     HIPCHK(hipGetLastError());
+    if (S.upwind) {
+      const size_t n = (size_t)(nx + 1) * my * mz;
+      kb("scal_faces");
+      k_scal_faces<T><<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(A, S);
+      HIPCHK(hipGetLastError());
+      ke();
+    }
     {
       auto kern = cfg.discr_scheme == PIB_SCHEME_UPWIND ? k_scal_xflux<T, SH, true> : k_scal_xflux<T, SH, false>;
       set_lds(kern, lds);
@@ -1884,7 +1893,7 @@ struct Impl : Base {
       auto kern = k_scal_phys<T, SH>;
       set_lds(kern, lds);
       kb("scal_phys");
-      kern<<<dim3(nx + 1, 6), nt, lds, stream>>>(A, S);
+      kern<<<dim3(nx + 1, S.skip_onl ? 5 : 6), nt, lds, stream>>>(A, S);
       HIPCHK(hipGetLastError());
       ke();
     }
@@ -1923,18 +1932,23 @@ struct Impl : Base {
     if (!sphys) {
       sphys = alloc<T>(9 * np);
       spart = alloc<double>((size_t)4 * (nx + 1));
+      s_nlp = sphys + np;
+      s_onp = sphys + 2 * np;
+      snlp_valid = false;
     }
     // onlphif <- nlphif (nlphi.F90:73-86): the buffers swap roles; nlphif
-    // is rewritten whole
+    // is rewritten whole.  The physical onl is then the last step's nl.
     std::swap(phi[1], phi[2]);
+    std::swap(s_nlp, s_onp);
     ScalStep<T> S;
     S.phif = phi[0];
     S.nlphif = phi[1];
     S.onlphif = phi[2];
     S.ffphi = phi[3];
     S.ph = sphys;
-    S.nlp = sphys + np;
-    S.onp = sphys + 2 * np;
+    S.nlp = s_nlp;
+    S.onp = s_onp;
+    S.skip_onl = snlp_valid ? 1 : 0;
     S.df = sphys + 3 * np;
     S.g = sphys + 4 * np;
     S.fr = sphys + 7 * np;
@@ -1955,6 +1969,7 @@ struct Impl : Base {
       case 3: scal_launch<3>(rks, S); break;
       default: scal_launch<0>(rks, S);
     }
+    snlp_valid = true;
     scalstep_modes(rks);
   }
   // flux_body (nbody), and (flux_global, phi_fluid_mean, phi_solid_mean, 0)
@@ -1982,7 +1997,10 @@ struct Impl : Base {
   }
   void set_field(int f, const double* h) {
     ++state_gen;
-    if (f >= PIB_F_PHI && f <= PIB_F_FFPHI) return put_phi(scalar_field(f), h);
+    if (f >= PIB_F_PHI && f <= PIB_F_FFPHI) {
+      snlp_valid = false;
+      return put_phi(scalar_field(f), h);
+    }
     switch (f) {
       case PIB_F_U: put_spec(u, 3, h, true); break;
       case PIB_F_P: put_spec(p, 1, h, false); break;

@@ -12,7 +12,8 @@
 // heat_ratio_new, heat_ratio_old, gamma), so no step waits on the host.
 //
 // Launch order (one stream): k_scal_phi (phi, products, y/z terms, plane
-// sums) -> k_scal_ratio (heat_ratio_new, phase means) -> k_scal_xflux (the
+// sums) -> k_scal_ratio (heat_ratio_new, phase means) -> [k_scal_faces, the
+// upwind face fluxes] -> k_scal_xflux (the
 // x-flux derivative, the sign, plane nx+1) -> k_scal_phys (nl, onl, diffn and
 // grad phi in physical space) -> fill fr -> k_scal_bnd (surface flux and
 // DOBND forcing, one workgroup per body) -> k_scal_rev (reversal pairs, one
@@ -35,6 +36,7 @@ struct ScalStep {
   const double* phisurf;  // (nbody)
   double dx;
   int upwind, dobnd, dorpr, frmeanfluid;
+  int skip_onl;  // onp already holds ff2cr(onlphif): the previous step's nlp
 };
 
 template <typename T>
@@ -210,8 +212,8 @@ struct OpPhiFlux {
   }
   __device__ T val(int y, int z) const {
     const int i = p + 1;
-    if constexpr (UP) {
-      return (T)((face(i, y, z) - face(i - 1, y, z)) / S.dx);
+    if constexpr (UP) {  // the faces are in fdd (k_scal_faces)
+      return (T)(((double)S.fdd[sps(A, i, y, z)] - (double)S.fdd[sps(A, i - 1, y, z)]) / S.dx);
     } else {
       const double fm = i == 1 ? (double)S.fdd[sps(A, A.nx, y, z)] * h : (double)S.fdd[sps(A, i - 1, y, z)];
       return (T)(((double)S.fdd[sps(A, i + 1, y, z)] - fm) / (2.0 * S.dx));
@@ -229,6 +231,18 @@ struct OpPhiFlux {
     if (p == 0) S.nlphif[sss(A, A.nx, ky, kz)] = mk(r.x / h, r.y / h);
   }
 };
+// the upwind flux of every face i + 1/2, i = 0..nx, into fdd (the
+// reference's fdderiv(0:nx), nl_allphi.F90:325-340), one lane a point
+template <typename T>
+__global__ void __launch_bounds__(256) k_scal_faces(KArgs<T> A, ScalStep<T> S) {
+  const size_t plane = (size_t)A.my * A.mz;
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (size_t)(A.nx + 1) * plane) return;
+  const int i = (int)(q / plane);
+  const int r = (int)(q - (size_t)i * plane), z = r / A.my, y = r - z * A.my;
+  OpPhiFlux<T, true> op{A, S, 0, S.ratio[0]};
+  S.fdd[q] = (T)op.face(i, y, z);
+}
 template <typename T, int SH, bool UP>
 __global__ void __launch_bounds__(1024) k_scal_xflux(KArgs<T> A, ScalStep<T> S) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -239,7 +253,9 @@ __global__ void __launch_bounds__(1024) k_scal_xflux(KArgs<T> A, ScalStep<T> S) 
 // ---------------------------------------------------------------------------
 // k_scal_phys, grid (planes 1..nx+1, 6): the physical nl and onl of phi
 // (nlphi.F90:119-132) and calc_diffn's diffn and grad phi (bc_scalar.F90:
-// 935-1033), whose x differences wrap with the heat ratio.
+// 935-1033), whose x differences wrap with the heat ratio.  onlphif is the
+// previous step's nlphif, so its physical field is the previous step's nl
+// when nothing changed the fields in between (grid (planes, 5), skip_onl).
 template <typename T>
 struct OpPhiPhys {
   const KArgs<T>& A;
@@ -294,7 +310,8 @@ struct OpPhiPhys {
 template <typename T, int SH>
 __global__ void __launch_bounds__(1024) k_scal_phys(KArgs<T> A, ScalStep<T> S) {
   extern __shared__ __align__(16) unsigned char smem[];
-  OpPhiPhys<T> op{A, S, (int)blockIdx.x, (int)blockIdx.y, S.ratio[0], S.ratio[3]};
+  const int kind = (int)blockIdx.y + (S.skip_onl && blockIdx.y >= 1 ? 1 : 0);
+  OpPhiPhys<T> op{A, S, (int)blockIdx.x, kind, S.ratio[0], S.ratio[3]};
   plane_c2r_sh<SH, 0>(plane_slot(A), A.G, A.tb, op, smem);
 }
 

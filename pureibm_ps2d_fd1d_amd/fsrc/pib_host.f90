@@ -334,9 +334,6 @@ contains
        if (.not. (pr_or_sc_nu > zero)) then
           last_err = 'PR_OR_SC_NU must be > 0'; st = PIB_ERR_ARG; return
        end if
-       if (imove == 1) then
-          last_err = 'ISCALON with IMOVE = 1: the scalar step is for fixed beds'; st = PIB_ERR_UNSUPPORTED; return
-       end if
     end if
     if (imove == 1 .and. (cage_simul .or. only_dem)) then
        last_err = 'IMOVE with CAGE_SIMUL or ONLY_DEM: run the DEM alone through pib_dem_* instead'

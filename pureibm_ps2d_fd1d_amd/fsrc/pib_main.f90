@@ -112,6 +112,10 @@ program pureibm_gpu
         st = PIB_OK
         do rks = 1, itrmax
            call gpu(pib_compute_new_timestep(ctx, rks), 'pib_compute_new_timestep')
+           if (scalar_on) then
+              call gpu(pib_get_scalars(ctx, sc), 'pib_get_scalars')
+              tscal = tscal + sc%dt               ! dependent_functions.F90:1257
+           end if
            st = pib_nonlinear_begin(ctx, rks)
            if (st /= PIB_OK) exit
            call gpu(pib_get_forces(ctx, frc, pres, visc, torq), 'pib_get_forces')
@@ -125,6 +129,7 @@ program pureibm_gpu
                 'pib_set_particles')
            call gpu(pib_nonlinear_finish(ctx, rks), 'pib_nonlinear_finish')
            call gpu(pib_velstep(ctx, rks), 'pib_velstep')
+           if (scalar_on) call gpu(pib_scalstep(ctx, rks), 'pib_scalstep')
         end do
      else if (scalar_on) then
         st = PIB_OK

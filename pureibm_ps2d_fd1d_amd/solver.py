@@ -143,7 +143,8 @@ class Solver:
         self.my2 = c.my // 2 + 1
         self.itrmax = 3 if c.rk3 else 1
         sc = getattr(setup, "scalar", None)
-        if sc and sc["iscalon"] == 1:
+        self.scalar_on = bool(sc and sc["iscalon"] == 1)
+        if self.scalar_on:
             # initscal (scalar_init.F90:73, 101, 117-171): gamma = vis/Pr, the
             # ratios 1, phisurf on every body, phi = 1 on planes 1..nx+1
             self.scalar_set(sc["gamma"], 1.0, 1.0)
@@ -227,7 +228,8 @@ class Solver:
         nonlinear up to the drag, des_time_march over dt with the step's
         PRES/VISC/mpg/frame_accln (``dem``, a ``Dem``), grid_nodes_insphere and
         update_nrpr_array for the moved centres, the forcing, velstep
-        (dtibm_main_scal_flo.F90:426-527 with boundary_condition.F90:539-554).
+        (dtibm_main_scal_flo.F90:426-527 with boundary_condition.F90:539-554),
+        then the scalar step when the deck has ISCALON = 1 (:489-515).
         The context must have been created with move_particles set."""
         if not self.cfg.move_particles:
             raise ValueError("the run set-up must set move_particles")
@@ -241,6 +243,8 @@ class Solver:
                 self.update_particles(xc, vel, angv)
                 self.nonlinear_finish(rks)
                 self.velstep(rks)
+                if self.scalar_on:
+                    self.scalstep(rks)
 
     def update_particles(self, xc, velbdy=None, angv=None):
         """Moved particles: what bcset does after des_time_march when particles

@@ -199,9 +199,9 @@ def test_moving_bed_steps_match_the_oracle(orc, tmp_path):
         x.close()
 
 
-def _moving(tmp_path, name, prec=_abi.PIB_FP64, rk3=False, lib=None, prefix="pib_", vel_seed=5):
+def _moving(tmp_path, name, prec=_abi.PIB_FP64, rk3=False, lib=None, prefix="pib_", vel_seed=5, **kw):
     st = make_setup(tmp_path, name, kind="random", precision=prec, my=32, dbydx=8, nbody=6, seed=3,
-                    imove=1, rhos=2.0, coeff_rest=0.8, rk3=rk3)
+                    imove=1, rhos=2.0, coeff_rest=0.8, rk3=rk3, **kw)
     assert st.config.move_particles == 1 and st.config.rhos == 2.0
     s = solver.Solver(st, lib=lib, prefix=prefix)
     dx = st.config.dx
@@ -225,6 +225,27 @@ def test_moving_bed_rk3_matches_the_oracle(orc, tmp_path):
         assert rel_err(g.field("u"), o.field("u")) < 1e-10
         assert abs(g.scalars().norm_drag - o.scalars().norm_drag) <= 1e-9 * abs(o.scalars().norm_drag)
     assert dg.state()[4].substeps == do.state()[4].substeps > 0
+    for x in (g, o, dg, do):
+        x.close()
+
+
+def test_moving_bed_with_the_scalar_matches_the_oracle(orc, tmp_path):
+    """IMOVE = 1 with ISCALON = 1 (the reference's MIS deck combination):
+    each stage ends with the scalar step on the moved bed's mask and
+    reversal pairs; phi, the surface fluxes and the heat ratio follow the
+    oracle chain (1e-10 on phi, 1e-9 on the fluxes)."""
+    st, g, dg = _moving(tmp_path, "MS", iscalon=1, discr_scheme="upwind")
+    _, o, do = _moving(tmp_path, "MS", lib=orc.load(), prefix="orc_", iscalon=1, discr_scheme="upwind")
+    assert g.scalar_on and o.scalar_on
+    for k in range(4):
+        g.step_moving(dg, 1)
+        o.step_moving(do, 1)
+        e = rel_err(g.field("phi"), o.field("phi"))
+        fg, fo = g.scalar_flux(), o.scalar_flux()
+        ef = rel_err(fg[0], fo[0])
+        print(f"  step {k + 1}: phi {e:.1e} flux {ef:.1e} heat ratio {g.scalar_get()[0]:.12f}", flush=True)
+        assert e < 1e-10 and ef < 1e-9
+        assert abs(g.scalar_get()[0] - o.scalar_get()[0]) <= 1e-12
     for x in (g, o, dg, do):
         x.close()
 
