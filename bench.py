@@ -58,12 +58,18 @@ def shell_fraction(st):
     return len(nodes) / float(nx * my * mz)
 
 
-def run_gpu(setup, steps, warmup, ksteps):
+def run_gpu(setup, steps, warmup, ksteps, autotune=True):
     """Timed run (whole steps replayed from HIP graphs, no per-launch timing),
     then ksteps more steps with HIP events around every launch for the
-    per-kernel figures."""
+    per-kernel figures.  With autotune the step form is first picked by
+    pib_autotune (the state is left as it was)."""
     g = solver.Solver(setup)
     lib = g.lib
+    picks = None
+    if autotune:
+        g.step(2)
+        p, ms = g.autotune(10)
+        picks = {"fft": p[0], "velstep": p[1], "streams": p[2], "ms_per_step": ms}
     g.step(warmup)
     lib.pib_sync(g.ctx)
     t0 = time.perf_counter()
@@ -87,7 +93,8 @@ def run_gpu(setup, steps, warmup, ksteps):
     for q in range(n.value):
         nm = names.raw[32 * q:32 * q + 32].split(b"\0")[0].decode()
         kern[nm] = (float(tot[q]), int(cnt[q]))
-    res = dict(wall_s=t1 - t0, kernels=kern, ksteps=ksteps, scalars=g.scalars(), forces=g.forces()["force"])
+    res = dict(wall_s=t1 - t0, kernels=kern, ksteps=ksteps, scalars=g.scalars(), forces=g.forces()["force"],
+               autotuned=picks)
     g.close()
     return res
 
@@ -501,6 +508,7 @@ def main():
     ap.add_argument("--no-dem", action="store_true", help="skip the DEM sub-cycle figure")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint (C4) figure")
     ap.add_argument("--no-scalar", action="store_true", help="skip the scalar-step (ISCALON) figure")
+    ap.add_argument("--no-autotune", action="store_true", help="time the default step form, not the autotuned one")
     ap.add_argument("--dem-marches", type=int, default=10)
     ap.add_argument("--collect-traffic", action="store_true",
                     help="measure HBM bytes per dispatch with rocprofv3 counter passes (child processes)")
@@ -531,8 +539,8 @@ def main():
     N = st64.nx * st64.my * st64.mz
     s = shell_fraction(st64)
 
-    r32 = run_gpu(st32, a.steps, a.warmup, a.kernel_steps) if a.only in ("both", "fp32") else None
-    r64 = run_gpu(st64, a.steps, a.warmup, a.kernel_steps) if a.only in ("both", "fp64") else None
+    r32 = run_gpu(st32, a.steps, a.warmup, a.kernel_steps, not a.no_autotune) if a.only in ("both", "fp32") else None
+    r64 = run_gpu(st64, a.steps, a.warmup, a.kernel_steps, not a.no_autotune) if a.only in ("both", "fp64") else None
     head = r32 if r32 is not None else r64
     w = 4 if r32 is not None else 8
     ms_step = head["wall_s"] / a.steps * 1e3
@@ -565,6 +573,7 @@ def main():
         "roofline": roof,
         "step_roofline": step_roof,
         "kernel_ms_per_step": {k: v[0] / head["ksteps"] for k, v in sorted(kern.items(), key=lambda kv: -kv[1][0])},
+        "autotuned": head["autotuned"],
     }
     if r64 is not None and r32 is not None:
         v64 = N * a.steps / r64["wall_s"] / 1e6
@@ -575,7 +584,7 @@ def main():
                        "mean_drag_rel_err_fp32_vs_fp64": abs(d32 - d64) / abs(d64),
                        "max_particle_drag_rel_err": float(np.max(np.abs(f32[:, 0] - f64[:, 0])) /
                                                           np.max(np.abs(f64[:, 0]))),
-                       "norm_drag_fp64": d64, "norm_drag_fp32": d32}
+                       "norm_drag_fp64": d64, "norm_drag_fp32": d32, "autotuned": r64["autotuned"]}
     if not a.no_batched and a.only in ("both", "fp32"):
         try:
             out["batched_c3"] = run_batched(12, a.batched_steps, 5)
