@@ -1872,7 +1872,7 @@ struct Impl : Base {
       HIPCHK(hipGetLastError());
       ke();
     }
-    k_scal_ratio<T><<<1, 64, 0, stream>>>(A, S);
+    k_scal_ratio<T><<<1, 256, 0, stream>>>(A, S);
     HIPCHK(hipGetLastError());
     if (S.upwind) {
       const size_t n = (size_t)(nx + 1) * my * mz;

@@ -140,21 +140,24 @@ __global__ void __launch_bounds__(1024) k_scal_phi(KArgs<T> A, ScalStep<T> S) {
 
 // heat_ratio_new = 1/phim(nx+1), phim = sum u phi / (um count), um = sum u /
 // count over the fluid of plane nx+1 (nl_allphi.F90:448-483); the phase
-// means of phi (:380-419).  One lane, planes in order.
+// means of phi (:380-419).  One workgroup of 256 lanes; the plane sums in a
+// fixed order.
 template <typename T>
-__global__ void k_scal_ratio(KArgs<T> A, ScalStep<T> S) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double pf = 0.0, ps = 0.0;
-  for (int p = 0; p < A.nx; ++p) {
-    pf += S.part[(size_t)p * 4];
-    ps += S.part[(size_t)p * 4 + 1];
+__global__ void __launch_bounds__(256) k_scal_ratio(KArgs<T> A, ScalStep<T> S) {
+  __shared__ double sh[2 * 16];
+  double v[2] = {0.0, 0.0};
+  for (int p = threadIdx.x; p < A.nx; p += blockDim.x) {
+    v[0] += S.part[(size_t)p * 4];
+    v[1] += S.part[(size_t)p * 4 + 1];
   }
+  block_sums_w<2>(v, sh);
+  if (threadIdx.x != 0) return;
   const double* q = S.part + (size_t)A.nx * 4;
   const double um = q[1] / q[2];
   const double phim = q[0] / (um * q[2]);
   S.ratio[1] = 1.0 / phim;
-  S.ratio[4] = pf / A.st->s.count_fluid;
-  S.ratio[5] = A.st->s.count_solid > 0 ? ps / A.st->s.count_solid : 0.0;
+  S.ratio[4] = v[0] / A.st->s.count_fluid;
+  S.ratio[5] = A.st->s.count_solid > 0 ? v[1] / A.st->s.count_solid : 0.0;
 }
 
 // ---------------------------------------------------------------------------
