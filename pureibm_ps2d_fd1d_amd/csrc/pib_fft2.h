@@ -52,6 +52,22 @@ template <>
 struct Split2<256> {
   static constexpr int A = 16, B = 16;
 };
+template <>
+struct Split2<48> {
+  static constexpr int A = 6, B = 8;
+};
+template <>
+struct Split2<96> {
+  static constexpr int A = 8, B = 12;
+};
+template <>
+struct Split2<80> {
+  static constexpr int A = 10, B = 8;
+};
+template <>
+struct Split2<160> {
+  static constexpr int A = 10, B = 16;
+};
 
 constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 
@@ -96,11 +112,74 @@ __device__ __forceinline__ void dft16(C2<T>* v) {
     for (int k1 = 0; k1 < 4; ++k1) v[k2 + 4 * k1] = b[k1];
   }
 }
+// e^{-2 pi i j/N} for the products j = n1*k2 of the composite splits below
+// (6 = 2 x 3, 10 = 2 x 5, 12 = 4 x 3)
+template <typename T, int N>
+__device__ __forceinline__ C2<T> wN(int j) {
+  if constexpr (N == 6) {
+    switch (j) {
+      case 1: return mk((T)0.5, (T)-0.86602540378443864676372317075293618);
+      case 2: return mk((T)-0.5, (T)-0.86602540378443864676372317075293618);
+      default: return mk((T)1, (T)0);
+    }
+  } else if constexpr (N == 10) {
+    switch (j) {
+      case 1: return mk((T)0.80901699437494742410229341718281906, (T)-0.58778525229247312916870595463907277);
+      case 2: return mk((T)0.30901699437494742410229341718281906, (T)-0.95105651629515357211643933337938214);
+      case 3: return mk((T)-0.30901699437494742410229341718281906, (T)-0.95105651629515357211643933337938214);
+      case 4: return mk((T)-0.80901699437494742410229341718281906, (T)-0.58778525229247312916870595463907277);
+      default: return mk((T)1, (T)0);
+    }
+  } else {
+    switch (j) {
+      case 1: return mk((T)0.86602540378443864676372317075293618, (T)-0.5);
+      case 2: return mk((T)0.5, (T)-0.86602540378443864676372317075293618);
+      case 3: return mk((T)0, (T)-1);
+      case 4: return mk((T)-0.5, (T)-0.86602540378443864676372317075293618);
+      case 6: return mk((T)-1, (T)0);
+      default: return mk((T)1, (T)0);
+    }
+  }
+}
+template <bool INV, int R, typename T>
+__device__ __forceinline__ void dft2n(C2<T>* v);
+// N = R1 R2 as a four-step DFT in registers: n = n1 + R1 n2, k = k2 + R2 k1
+template <bool INV, int R1, int R2, typename T>
+__device__ __forceinline__ void dft_ct(C2<T>* v) {
+  C2<T> a[R1][R2];
+#pragma unroll
+  for (int n1 = 0; n1 < R1; ++n1) {
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) a[n1][n2] = v[n1 + R1 * n2];
+    dft2n<INV, R2>(a[n1]);
+#pragma unroll
+    for (int k2 = 1; k2 < R2; ++k2)
+      if (n1 > 0) {
+        C2<T> w = wN<T, R1 * R2>(n1 * k2);
+        if (INV) w.y = -w.y;
+        a[n1][k2] = cmul(a[n1][k2], w);
+      }
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < R2; ++k2) {
+    C2<T> b[R1];
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) b[k1] = a[k1][k2];
+    dft2n<INV, R1>(b);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) v[k2 + R2 * k1] = b[k1];
+  }
+}
 template <bool INV, int R, typename T>
 __device__ __forceinline__ void dft2n(C2<T>* v) {
   if constexpr (R == 2) dft2<INV>(v);
+  else if constexpr (R == 3) dft3<INV>(v);
   else if constexpr (R == 4) dft4<INV>(v);
+  else if constexpr (R == 5) dft5<INV>(v);
+  else if constexpr (R == 6) dft_ct<INV, 2, 3>(v);
   else if constexpr (R == 8) dft8<INV>(v);
+  else if constexpr (R == 10) dft_ct<INV, 2, 5>(v);
+  else if constexpr (R == 12) dft_ct<INV, 4, 3>(v);
   else dft16<INV>(v);
 }
 // v * W^j (forward) or v * conj(W^j) (inverse), W^j from a table in LDS
@@ -375,18 +454,47 @@ template <>
 struct Shape<3> {
   static constexpr int HY = 16, MZ = 32, NT = 128;
 };
+// the 96^2 and 160^2 planes (radix 3 and 5 in the split lengths)
+template <>
+struct Shape<4> {
+  static constexpr int HY = 48, MZ = 96, NT = 512;
+};
+template <>
+struct Shape<5> {
+  static constexpr int HY = 80, MZ = 160, NT = 1024;
+};
 static inline int shape_of(int hy, int mz) {
   if (hy == 64 && mz == 128) return 1;
   if (hy == 32 && mz == 64) return 2;
   if (hy == 16 && mz == 32) return 3;
+  if (hy == 48 && mz == 96) return 4;
+  if (hy == 80 && mz == 160) return 5;
   return 0;
+}
+template <int SH>
+constexpr int shape_ld_c() {
+  return Plane2<Shape<SH>::HY, Shape<SH>::MZ, Shape<SH>::NT>::LD;
 }
 // LDS row stride (in complex entries) of a compiled shape's plane buffer
 static inline int shape_ld(int sh) {
-  return sh == 1 ? Plane2<64, 128, 1024>::LD : (sh == 2 ? Plane2<32, 64, 256>::LD : (sh == 3 ? Plane2<16, 32, 128>::LD : 0));
+  switch (sh) {
+    case 1: return shape_ld_c<1>();
+    case 2: return shape_ld_c<2>();
+    case 3: return shape_ld_c<3>();
+    case 4: return shape_ld_c<4>();
+    case 5: return shape_ld_c<5>();
+    default: return 0;
+  }
 }
 static inline int shape_nt(int sh) {
-  return sh == 1 ? Shape<1>::NT : (sh == 2 ? Shape<2>::NT : (sh == 3 ? Shape<3>::NT : 0));
+  switch (sh) {
+    case 1: return Shape<1>::NT;
+    case 2: return Shape<2>::NT;
+    case 3: return Shape<3>::NT;
+    case 4: return Shape<4>::NT;
+    case 5: return Shape<5>::NT;
+    default: return 0;
+  }
 }
 // lds: the workgroup's LDS, which holds the plane (and the tables); hbm: the
 // workgroup's plane slot in HBM, or null.  The generic transform (SH = 0)

@@ -1080,6 +1080,8 @@ struct Impl : Base {
       case 1: launch_c2r_pair<KA, NA, KB, NB, 1>(rks, name); break;
       case 2: launch_c2r_pair<KA, NA, KB, NB, 2>(rks, name); break;
       case 3: launch_c2r_pair<KA, NA, KB, NB, 3>(rks, name); break;
+      case 4: launch_c2r_pair<KA, NA, KB, NB, 4>(rks, name); break;
+      case 5: launch_c2r_pair<KA, NA, KB, NB, 5>(rks, name); break;
       default: launch_c2r_pair<KA, NA, KB, NB, 0>(rks, name);
     }
   }
@@ -1089,6 +1091,8 @@ struct Impl : Base {
       case 1: launch_c2r<KIND, 1>(rks, ncomp); break;
       case 2: launch_c2r<KIND, 2>(rks, ncomp); break;
       case 3: launch_c2r<KIND, 3>(rks, ncomp); break;
+      case 4: launch_c2r<KIND, 4>(rks, ncomp); break;
+      case 5: launch_c2r<KIND, 5>(rks, ncomp); break;
       default: launch_c2r<KIND, 0>(rks, ncomp);
     }
   }
@@ -1319,6 +1323,8 @@ struct Impl : Base {
       case 1: return launch_nl_sh<1>(up0, mm0, rks);
       case 2: return launch_nl_sh<2>(up0, mm0, rks);
       case 3: return launch_nl_sh<3>(up0, mm0, rks);
+      case 4: return launch_nl_sh<4>(up0, mm0, rks);
+      case 5: return launch_nl_sh<5>(up0, mm0, rks);
       default: break;
     }
     size_t E = (size_t)my2 * mz;
@@ -1421,6 +1427,8 @@ struct Impl : Base {
       case 1: r2c_force<1>(rks); break;
       case 2: r2c_force<2>(rks); break;
       case 3: r2c_force<3>(rks); break;
+      case 4: r2c_force<4>(rks); break;
+      case 5: r2c_force<5>(rks); break;
       default: r2c_force<0>(rks);
     }
   }
@@ -1967,6 +1975,8 @@ struct Impl : Base {
       case 1: scal_launch<1>(rks, S); break;
       case 2: scal_launch<2>(rks, S); break;
       case 3: scal_launch<3>(rks, S); break;
+      case 4: scal_launch<4>(rks, S); break;
+      case 5: scal_launch<5>(rks, S); break;
       default: scal_launch<0>(rks, S);
     }
     snlp_valid = true;

@@ -238,13 +238,17 @@ def test_blowup_detected_like_the_reference(tmp_path, orc):
     assert eo.value.code == _abi.PIB_ERR_BLOWUP == eg.value.code
 
 
-@pytest.mark.parametrize("my,dbydx,nbody", [(32, 8, 6), (64, 16, 3), (128, 16, 8)])
-def test_compiled_plane_shapes_match_generic_and_oracle(tmp_path, orc, my, dbydx, nbody):
+@pytest.mark.parametrize("my,dbydx,nbody,prec", [(32, 8, 6, 0), (64, 16, 3, 0), (128, 16, 8, 0), (96, 12, 6, 0),
+                                                 (96, 12, 6, 1), (160, 20, 5, 1)])
+def test_compiled_plane_shapes_match_generic_and_oracle(tmp_path, orc, my, dbydx, nbody, prec):
     """The two-stage plane transforms of the compiled shapes (my/2, mz) =
-    (16,32), (32,64), (64,128) against the generic mixed-radix transforms
-    (PIB_TUNE_FFT = 0) and against the oracle, over whole steps."""
-    print(f"compiled shapes my={my}", flush=True)
-    kw = dict(kind="random", my=my, dbydx=dbydx, nbody=nbody, seed=31, re=30.0)
+    (16,32), (32,64), (64,128), (48,96), (80,160) -- the last two with the
+    radix-6/12 and radix-10 splits -- against the generic mixed-radix
+    transforms (PIB_TUNE_FFT = 0) and against the oracle, over whole steps.
+    FP64: 1e-10 on u, 1e-9 on the forces, compiled vs generic nl 1e-12.
+    FP32-bulk (the 160^2 FP64 plane does not fit the LDS): 1e-5, 1e-4, 1e-5."""
+    print(f"compiled shapes my={my} prec={prec}", flush=True)
+    kw = dict(kind="random", my=my, dbydx=dbydx, nbody=nbody, seed=31, re=30.0, precision=prec)
     st = make_setup(tmp_path, "SH", **kw)
     o = solver.Solver(st, lib=orc.load(), prefix="orc_")
     g1 = solver.Solver(st)
@@ -257,9 +261,11 @@ def test_compiled_plane_shapes_match_generic_and_oracle(tmp_path, orc, my, dbydx
     uo = o.field("u")
     e1, e0 = rel_err(g1.field("u"), uo), rel_err(g0.field("u"), uo)
     f1 = rel_err(g1.forces()["force"], o.forces()["force"])
-    print(f"  u err compiled {e1:.2e} generic {e0:.2e} force {f1:.2e}", flush=True)
-    assert e1 < 1e-10 and e0 < 1e-10 and f1 < 1e-9
-    assert rel_err(g1.field("nl"), g0.field("nl")) < 1e-12
+    en = rel_err(g1.field("nl"), g0.field("nl"))
+    print(f"  u err compiled {e1:.2e} generic {e0:.2e} force {f1:.2e} nl compiled vs generic {en:.2e}", flush=True)
+    tu, tf, tn = (1e-10, 1e-9, 1e-12) if prec == 0 else (1e-5, 1e-4, 1e-5)
+    assert e1 < tu and e0 < tu and f1 < tf
+    assert en < tn
 
 
 @pytest.mark.parametrize("my,dbydx,nbody,scheme", [(24, 6, 3, "center"), (40, 10, 2, "upwind")])
