@@ -206,7 +206,7 @@ __global__ void k_gather_records(int n, const int* rec_sorted, const double* w, 
   int r = rec_sorted[t];
   rpt[t] = rec_point(r / 8, nbody, nbnd, dobnd);
   rw[t] = w[r];
-  rd[t] = dist[r];
+  rd[t] = (uint8_t)(dist[r] | ((r & 1) << 1));  // bit 1: the stencil's upper x node (corner i = 1)
 }
 
 // ===========================================================================
@@ -746,6 +746,8 @@ struct Impl : Base {
   double* spart = nullptr;
   double *sflux = nullptr, *sphisurf = nullptr;
   int ssurf_n = -1;
+  double* sspt = nullptr;  // per spreading-record point: the scalar forcing's two evaluations
+  size_t sspt_n = 0;
   std::vector<double> host_rad;
   double sc_gamma = 0, heat_ratio = 1, heat_ratio_new = 1, heat_ratio_old = 1;
   // particles
@@ -1919,6 +1921,13 @@ struct Impl : Base {
         HIPCHK(hipGetLastError());
         ke();
       }
+      if (nspread_blk > 0) {
+        kb("scal_spread");
+        k_scal_spread<T><<<(nspread_blk + 7) / 8 * 8, 256, 0, stream>>>(A, S, spread_bt, nspread_blk, tnode, toff,
+                                                                         rpt, rw, rdist);
+        HIPCHK(hipGetLastError());
+        ke();
+      }
     }
     {
       auto kern = k_scal_ff<T, SH>;
@@ -1966,6 +1975,12 @@ struct Impl : Base {
     S.ratio = sratio;
     S.flux_body = sflux;
     S.phisurf = sphisurf;
+    const size_t nspt = (size_t)(cfg.dobnd ? 2 : 1) * nbody * nbnd * 4;
+    if (nspt > sspt_n) {
+      sspt = alloc<double>(nspt);
+      sspt_n = nspt;
+    }
+    S.spt = sspt;
     S.dx = cfg.dx;
     S.upwind = cfg.discr_scheme == PIB_SCHEME_UPWIND;
     S.dobnd = cfg.dobnd;

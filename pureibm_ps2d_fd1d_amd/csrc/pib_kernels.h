@@ -998,7 +998,7 @@ __global__ void __launch_bounds__(256) k_spread(KArgs<T> A, const int* bt, int n
     if (rp[k] >= 0) {
       const double* o = pts + (size_t)rp[k] * 6;
       double wt = rw[r];
-      bool d = rdist[r] != 0;
+      bool d = (rdist[r] & 1) != 0;
       int q = r - r0;
       for (int n = 0; n < 3; ++n) cv[n * SPREAD_CAP + q] = (wt * o[n]) / dx3 + (d ? o[3 + n] : 0.0);
     }

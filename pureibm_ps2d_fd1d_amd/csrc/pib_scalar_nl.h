@@ -34,6 +34,10 @@ struct ScalStep {
   double* ratio;      // heat_ratio, heat_ratio_new, heat_ratio_old, gamma, phi_fluid_mean, phi_solid_mean
   double* flux_body;  // (nbody)
   const double* phisurf;  // (nbody)
+  // per point of the spreading records: (tempor, force_dist) of the first
+  // evaluation and of the one a period down; rows as the fluid's records
+  // number them (reversal pairs m*nbnd + l, surface points after them)
+  double* spt;
   double dx;
   int upwind, dobnd, dorpr, frmeanfluid;
   int skip_onl;  // onp already holds ff2cr(onlphif): the previous step's nlp
@@ -422,8 +426,8 @@ __device__ inline int cellof_d(double x) { return x < 0.0 ? (int)(x - 1.0) : (in
 
 // calc_bndrev_data (bc_scalar.F90:385-556), one workgroup per body: the
 // surface flux sum of n . grad phi, and with DOBND the surface forcing
-// cf (phi - phisurf) dA1 dx spread to the eight nodes (x nodes wrapped into
-// 1..nx+1; plane nx+1 is not transformed).
+// cf (phi - phisurf) dA1 dx of each point for k_scal_spread (its x nodes
+// wrap into 1..nx+1; plane nx+1 is not transformed).
 template <typename T>
 __global__ void __launch_bounds__(256) k_scal_bnd(KArgs<T> A, Particles P, ScalStep<T> S) {
   __shared__ double sh[64];
@@ -458,7 +462,16 @@ __global__ void __launch_bounds__(256) k_scal_bnd(KArgs<T> A, Particles P, ScalS
     double phil, dum;
     interp_phi(A, S, stn, false, h, 1.0, &phil, &dum, &dum, &dum);
     double tempor = 0.0;
-    if (S.dobnd) tempor = ((cf * (phil - S.phisurf[m])) * da1 * 1.0) * dx;
+    if (S.dobnd) {
+      tempor = ((cf * (phil - S.phisurf[m])) * da1 * 1.0) * dx;
+      // the share of x node nx+1 (a point in cell nx) stays on a plane that
+      // is not transformed: the second set carries it as zero
+      double* o = S.spt + ((size_t)P.nbody * nb + (size_t)m * nb + l) * 4;
+      o[0] = tempor;
+      o[1] = 0.0;
+      o[2] = is[0] == nx ? 0.0 : tempor;
+      o[3] = 0.0;
+    }
     double gb[3] = {0.0, 0.0, 0.0};
     for (int i = 0; i < 2; ++i)
       for (int j = 0; j < 2; ++j)
@@ -477,7 +490,6 @@ __global__ void __launch_bounds__(256) k_scal_bnd(KArgs<T> A, Particles P, ScalS
           ii = xfold(ii, nx);
           const double w = stn.w[(k * 2 + j) * 2 + i];
           for (int n = 0; n < 3; ++n) gb[n] = gb[n] + (double)S.g[(size_t)n * S.np + sps(A, ii, y, z)] * pf * w;
-          if (S.dobnd) atomicAdd(S.fr + sps(A, ii, y, z), (T)((w * tempor) / dx3));
         }
     fl = fl + (gb[0] * P.bnd[(size_t)l * 3] + gb[1] * P.bnd[(size_t)l * 3 + 1] + gb[2] * P.bnd[(size_t)l * 3 + 2]);
   }
@@ -491,7 +503,10 @@ __global__ void __launch_bounds__(256) k_scal_bnd(KArgs<T> A, Particles P, ScalS
 // time one period down (cell 0); its forcing cf (phi_i - phi_ref) - c3 nl -
 // c4 onl - (c1 + c2) diffn, times dA2 dx, goes to the stencil's solid nodes
 // with the fluid nodes' share redistributed evenly over them (to every node
-// with include_frmeanfluid); x nodes stay in 0..nx+1.
+// with include_frmeanfluid).  Of the two evaluations, only the first's x node
+// nx and the second's x node 1 land on transformed planes: the pair's row
+// holds (tempor, force_dist) of each, and k_scal_spread takes the second for
+// the stencil's upper x nodes.
 template <typename T>
 __global__ void __launch_bounds__(256) k_scal_rev(KArgs<T> A, Particles P, ScalStep<T> S, int rks) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -566,14 +581,38 @@ __global__ void __launch_bounds__(256) k_scal_rev(KArgs<T> A, Particles P, ScalS
           }
         }
     const double force_dist = count_fl < 8 ? force_fl / (double)(8 - count_fl) : 0.0;
-    for (int k = 0; k < 2; ++k)
-      for (int j = 0; j < 2; ++j)
-        for (int i = 0; i < 2; ++i) {
-          const int ii = xfold(si.ib + i, nx);
-          const int y = wrap1y(si.jb + j, A.my) - 1, z = wrap1y(si.kb + k, A.mz) - 1;
-          const double add = si.w[(k * 2 + j) * 2 + i] * tempor / dx3;
-          if (S.frmeanfluid) atomicAdd(S.fr + sps(A, ii, y, z), (T)add);
-          else if (!fluid_at(A, ii, y, z)) atomicAdd(S.fr + sps(A, ii, y, z), (T)(add + force_dist));
-        }
+    double* o = S.spt + ((size_t)m * P.nbnd + l) * 4;
+    if (ev == 1) {
+      o[0] = tempor;
+      o[1] = force_dist;
+    }
+    o[2] = tempor;  // a single evaluation: its values for the upper nodes too
+    o[3] = force_dist;
+  }
+}
+
+// The scalar forcing on the grid from the fluid's node-sorted spreading
+// records (the same stencils, modulo the x period): one lane per target node
+// adds its records in their sorted order, each (w tempor)/dx^3 plus the
+// redistributed share when the record carries it, with the upper-x-node
+// records taking the pair's second evaluation.  No atomics: the sums are
+// deterministic.  rdist bit 0: redistributed share; bit 1: upper x node.
+template <typename T>
+__global__ void __launch_bounds__(256) k_scal_spread(KArgs<T> A, ScalStep<T> S, const int* bt, int nrange,
+                                                     const int* tnode, const int* toff, const int* rpt,
+                                                     const double* rw, const uint8_t* rdist) {
+  const int w = xcd_linear(blockIdx.x, gridDim.x);
+  if (w >= nrange) return;
+  const int t0 = bt[w], t1 = bt[w + 1];
+  const double dx3 = A.st->dx3;
+  const size_t plane = (size_t)A.my * A.mz;
+  for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x) {
+    double acc = 0.0;
+    for (int r = toff[t]; r < toff[t + 1]; ++r) {
+      const uint8_t f = rdist[r];
+      const double* o = S.spt + (size_t)rpt[r] * 4 + ((f & 2) ? 2 : 0);
+      acc = acc + ((rw[r] * o[0]) / dx3 + ((f & 1) ? o[1] : 0.0));
+    }
+    S.fr[(size_t)tnode[t] + plane] = (T)acc;
   }
 }
