@@ -1375,6 +1375,10 @@ struct Impl : Base {
   // bcset up to where the reference moves particles (boundary_condition.F90:
   // 517-538): calc_visc, compute_omega, compute_mpg_at_n and the diagnostics
   void bcset_drag(int rks) {
+    if (skip_drag) {  // a recorded step: calc_visc only
+      c2r<C2R_VISC>(rks, 3);
+      return;
+    }
     Particles P = parts();
     c2r_pair<C2R_VISC, 3, C2R_OMEGA, 3>(rks, "c2r_visc_omega");
     const int G = (nbnd + 255) / 256;
@@ -1513,7 +1517,10 @@ struct Impl : Base {
       }
     for (auto& kv : agexec_rec) (void)hipGraphExecDestroy(kv.second);
     agexec_rec.clear();
+    for (auto& kv : rgexec) (void)hipGraphExecDestroy(kv.second);
+    rgexec.clear();
     adj_eager = 0;
+    lite_eager = 0;
   }
   void eager_step(bool rec) {
     ++state_gen;
@@ -2410,47 +2417,110 @@ struct Impl : Base {
   // After a whole step, ubcp, nlbc, onlbc, diffn and ppr still hold that
   // step's values (velstep writes only u and p), together with the step's t,
   // dt and cf in the device state: what the adjoint of the step reads
-  // besides the mask and the bed.  adjoint_record runs steps and keeps those
-  // fields; adjoint_step then takes them instead of recomputing them.
+  // besides the mask and the bed.  adjoint_record runs steps whose ubcp,
+  // nlbc, diffn and ppr are written straight into the record block (the
+  // context's buffers are pointed there for the step) and copies the device
+  // state after it; adjoint_step then takes them instead of recomputing them.
   // record blocks by stack depth (a segment's records are pushed and taken
   // in stack order, so a depth's block always holds the same step position
   // and the graphs of steps on recorded fields stay valid); each block is
-  // the five 3-component fields, then the device state
+  // the five 3-component fields, then the device state, then GATHER_PAD
+  // elements for the stencil loads past the last field
   std::vector<void*> rec_blocks;
   std::vector<bool> rec_owns;  // the record holds its own onlbc
   int rec_depth = 0;
   // state_gen counts whatever changes the forward state (steps, restores,
   // sets); rec_gen is its value after the last recorded step
   unsigned long long state_gen = 0, rec_gen = ~0ull;
+  // recorded steps skip the drag and its diagnostics (the vorticity, the
+  // per-particle forces, norm_drag): with the mean pressure gradient set and
+  // the bed fixed nothing of the state depends on them
+  bool skip_drag = false;
+  int lite_eager = 0;  // eager recorded steps since the graphs were dropped
+  // graphs of recorded steps, by the buffers they read and write
+  std::map<std::array<const void*, 7>, hipGraphExec_t> rgexec;
   size_t rec_bytes() const {
-    return (size_t)15 * nx * my * mz * sizeof(T) + sizeof(DevState);
+    return (size_t)15 * nx * my * mz * sizeof(T) + sizeof(DevState) + GATHER_PAD * sizeof(T);
+  }
+  void rec_step() {
+    if (use_graph && !ktime && eager_steps >= 2 && nlbc_valid && (!skip_drag || lite_eager >= 1)) {
+      const std::array<const void*, 7> key{nl, nlbc, onlbc, ubcp, diffn, ppr, skip_drag ? this : nullptr};
+      hipGraphExec_t& slot = rgexec[key];
+      if (!slot) {
+        hipGraph_t g = capture_step();
+        hipError_t e = hipGraphInstantiate(&slot, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) rgexec.erase(key);
+        HIPCHK(e);
+      }
+      HIPCHK(hipGraphLaunch(slot, stream));
+      advance_step();
+      return;
+    }
+    eager_step(false);
+    if (skip_drag) ++lite_eager;
+    else ++eager_steps;
   }
   void adjoint_record(int nsteps) {
     need_adj();
     if (nbody == 0) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
-    const size_t cs3 = (size_t)3 * nx * my * mz;
-    for (int n = 0; n < nsteps; ++n) {
-      one_step();
-      if (rec_depth == (int)rec_blocks.size()) {
-        void* m = nullptr;
-        HIPCHK(hipMalloc(&m, rec_bytes()));
-        rec_blocks.push_back(m);
+    if (nsteps <= 0) return;
+    const size_t cs3 = (size_t)3 * nx * my * mz, fb = cs3 * sizeof(T);
+    // the context's own buffers: nl/onl swap every step, nlbc/onlbc with
+    // them, so after nsteps steps nlbc is own_nb[nsteps & 1]
+    T* const own_nb[2] = {nlbc, onlbc};
+    T *const ub0 = ubcp, *const df0 = diffn, *const pp0 = ppr;
+    auto back = [&](int done) {
+      skip_drag = false;
+      if (done > 0) {
+        T* const last = nlbc;
+        nlbc = own_nb[done & 1];
+        onlbc = own_nb[(done & 1) ^ 1];
+        HIPCHK(hipMemcpyAsync(nlbc, last, fb, hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(ub0, ubcp, fb, hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(df0, diffn, fb, hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(pp0, ppr, fb, hipMemcpyDeviceToDevice, stream));
+      } else {
+        nlbc = own_nb[0];
+        onlbc = own_nb[1];
       }
-      // a step's onlbc is the nlbc of the step before it, which the record
-      // below this one holds when that record is of the step just before
-      // (nothing ran or restored the state since); else it keeps its own
-      const bool own_onlbc = rec_depth == 0 || state_gen != rec_gen + 1;
-      rec_gen = state_gen;
-      if ((int)rec_owns.size() <= rec_depth) rec_owns.resize(rec_depth + 1);
-      rec_owns[rec_depth] = own_onlbc;
-      char* b = (char*)rec_blocks[rec_depth++];
-      T* f[5] = {ubcp, nlbc, onlbc, diffn, ppr};
-      for (int q = 0; q < 5; ++q)
-        if (q != 2 || own_onlbc)
-          HIPCHK(hipMemcpyAsync(b + (size_t)q * cs3 * sizeof(T), f[q], cs3 * sizeof(T), hipMemcpyDeviceToDevice,
-                                stream));
-      HIPCHK(hipMemcpyAsync(b + 5 * cs3 * sizeof(T), dst, sizeof(DevState), hipMemcpyDeviceToDevice, stream));
+      ubcp = ub0;
+      diffn = df0;
+      ppr = pp0;
+    };
+    skip_drag = cfg.set_mpg && !cfg.move_particles;
+    int done = 0;
+    try {
+      for (; done < nsteps; ++done) {
+        if (rec_depth == (int)rec_blocks.size()) {
+          void* m = nullptr;
+          HIPCHK(hipMalloc(&m, rec_bytes()));
+          rec_blocks.push_back(m);
+        }
+        T* b = (T*)rec_blocks[rec_depth];
+        // nonlinear swaps nlbc and onlbc first: the step's nlbc lands where
+        // onlbc points
+        ubcp = b;
+        onlbc = b + cs3;
+        diffn = b + 3 * cs3;
+        ppr = b + 4 * cs3;
+        rec_step();
+        // a step's onlbc is the nlbc of the step before it, which the record
+        // below this one holds when that record is of the step just before
+        // (nothing ran or restored the state since); else it keeps its own
+        const bool own_onlbc = rec_depth == 0 || state_gen != rec_gen + 1;
+        rec_gen = state_gen;
+        if ((int)rec_owns.size() <= rec_depth) rec_owns.resize(rec_depth + 1);
+        rec_owns[rec_depth] = own_onlbc;
+        if (own_onlbc) HIPCHK(hipMemcpyAsync(b + 2 * cs3, onlbc, fb, hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(b + 5 * cs3, dst, sizeof(DevState), hipMemcpyDeviceToDevice, stream));
+        ++rec_depth;
+      }
+    } catch (...) {
+      back(done);
+      throw;
     }
+    back(done);
   }
   // ---------------- autotuner ----------------
   // a step form: (PIB_TUNE_FFT, PIB_TUNE_VELSTEP, PIB_TUNE_STREAMS)

@@ -80,6 +80,52 @@ def test_gpu_adjoint_step_is_the_transpose_of_the_gpu_step(tmp_path):
     s.close()
 
 
+@pytest.mark.parametrize("prec", [_abi.PIB_FP64, _abi.PIB_FP32_BULK])
+def test_backward_sweep_on_recorded_steps_matches_the_oracle(tmp_path, orc, prec):
+    """The whole backward sweep (checkpoint pops, pib_adjoint_record, adjoint
+    steps on recorded fields and on recomputed ones) against the oracle's,
+    with force weights on steps inside segments as well as at their ends, so
+    records that keep their own onlbc sit above the bottom of the stack.
+    Then, on fresh contexts, steps recorded between plain steps: the state
+    after them must be that of plain steps (the context's buffers hold the
+    last recorded step's fields).  FP64: relative 1e-9; FP32-bulk (GPU
+    only, the oracle is FP64): 1e-4."""
+    st = _setup(tmp_path, my=24, dbydx=6, nbody=4, seed=7)
+    nsteps, every = 14, 5
+    rng = np.random.default_rng(5)
+    w = {n: np.asfortranarray(rng.standard_normal((st.nbody, 3))) for n in (3, 9, 13)}
+    out = []
+    for lib in ("gpu", "orc"):
+        st.config.precision = prec if lib == "gpu" else _abi.PIB_FP64
+
+        def fresh():
+            s = solver.Solver(st) if lib == "gpu" else solver.Solver(st, lib=orc.load(), prefix="orc_")
+            sc = s.scalars()
+            for n in range(3):
+                sc.umean[n] = sc.ufmean_des[n]
+            s.set_scalars(sc)
+            return s
+        s = fresh()
+        adjoint.forward_checkpointed(s, nsteps, every)
+        grad = adjoint.backward(s, nsteps, every, w.get)
+        s.close()
+        s = fresh()
+        s.step(3)
+        s.adjoint_begin()
+        s.adjoint_record(3)
+        s.step(1)
+        s.adjoint_record(4)
+        s.step(2)
+        out.append((grad, s.field("u"), s.field("p"), s.field("nl")))
+        s.close()
+    (gg, gu, gp, gn), (og, ou, op, on) = out
+    tol = 1e-9 if prec == _abi.PIB_FP64 else 1e-4
+    errs = [rel_err(a, b) for a, b in zip(gg, og)] + [rel_err(gu, ou), rel_err(gp, op), rel_err(gn, on)]
+    print("backward sweep: dxc {:.1e} dmpg {:.1e} dumean {:.1e}; recorded then plain steps: u {:.1e} p {:.1e} "
+          "nl {:.1e}".format(*errs), flush=True)
+    assert max(errs) < tol
+
+
 def test_gpu_gradient_matches_finite_differences_small_bed(tmp_path):
     st = _setup(tmp_path, my=24, dbydx=6, nbody=3, seed=2)
     P = adjoint.DragProblem(st, nsteps=6, every=4)

@@ -14,8 +14,8 @@ HBM plane form, beds with spheres across the x wrap.
 
 Tolerances, FP64: relative 1e-10 to the field maximum on phif, nlphif,
 onlphif and ffphi, 1e-12 on the heat ratios, 1e-9 on the surface fluxes
-(the spreading adds with atomics, in no fixed order).  FP32-bulk against
-FP64 on the GPU: 1e-4 relative on phif and the mean surface flux.
+(the transforms and the spreading sum in orders of their own).  FP32-bulk
+against FP64 on the GPU: 1e-4 relative on phif and the mean surface flux.
 """
 import numpy as np
 import pytest
