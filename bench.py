@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 # spectral and physical fields both counted as N words).
 PASSES = {
     "c2r_ubcp_pbcp": 8, "form_nl": 6, "form_nl_yz": 6, "form_nl_x": 9, "c2r_nlbc": 6, "c2r_onlbc": 6,
-    "c2r_visc_omega": 14, "c2r_pgrad": 4, "r2c_force": 6, "velstep": 17,
+    "c2r_visc_omega": 14, "c2r_pgrad": 4, "r2c_force": 6, "velstep": 17, "c2r_inner": 12, "c2r_omega": 8,
 }
 
 
@@ -104,6 +104,7 @@ KERNEL_SYMBOLS = {
     "form_nl": "k_form_nl<float", "velstep": "k_velstep_tiled<float", "forcing_points": "k_forcing_points<float",
     "spread": "k_spread<float", "c2r_visc_omega": "k_c2r_pair<float, 4,", "c2r_ubcp_pbcp": "k_c2r_pair<float, 0,",
     "c2r_pgrad": "k_c2r<float, 6,", "c2r_nlbc": "k_c2r<float, 2,", "r2c_force": "k_r2c_force<float",
+    "c2r_inner": "k_c2r<float, 7,", "c2r_omega": "k_c2r<float, 5,",
     "drag_points": "k_drag_points<float",
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_fluid_step_128cube_fp32.md")

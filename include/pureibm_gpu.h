@@ -110,9 +110,13 @@ typedef struct pib_scalars {
 #define PIB_F_FF 4     /* complex (nx+1,my2,mz,3) */
 #define PIB_F_UBCP 5   /* real (nx+1,my,mz,3) */
 #define PIB_F_PBCP 6   /* real (nx,my,mz) */
-#define PIB_F_NLBCP 7  /* real (nx+1,my,mz,3) */
-#define PIB_F_ONLBCP 8 /* real (nx+1,my,mz,3) */
-#define PIB_F_DIFFN 9  /* real (nx+1,my,mz,3) */
+/* nlbcp, onlbcp and diffn are not formed by the step (the reversal forcing
+ * interpolates one combination of them, transformed from the spectra);
+ * pib_get_field transforms them on request from nl, onl and u as they stand,
+ * so diffn read after velstep is that of the new u, not of the stage's. */
+#define PIB_F_NLBCP 7  /* real (nx+1,my,mz,3), get only */
+#define PIB_F_ONLBCP 8 /* real (nx+1,my,mz,3), get only */
+#define PIB_F_DIFFN 9  /* real (nx+1,my,mz,3), get only */
 #define PIB_F_PPR 10   /* real (nx,my,mz,3) */
 #define PIB_F_FR 11    /* real (nx+1,my,mz,3): IBM forcing before frmean removal */
 #define PIB_F_OMEGA 12 /* real (nx+1,my,mz,3): vorticity from compute_omega */
@@ -303,9 +307,12 @@ int pib_tridiag_batched_noalloc(void* ctx, int n, int nsys, const double* d_a, c
 int pib_adjoint_begin(void* ctx);
 int pib_adjoint_step(void* ctx, const double* wforce);
 /* nsteps steps as pib_step runs them, keeping after each the forward fields
- * that the step's adjoint reads (ubcp, nlbc, onlbc, diffn, ppr and the
+ * that the step's adjoint reads (ubcp, the inner-point combination of
+ * ubcp, nlbcp, onlbcp and diffn that the forcing interpolates, ppr and the
  * step's t, dt, cf; velstep leaves them as the step's nonlinear part set
- * them) on a record stack.  A pib_adjoint_step without a force weight takes
+ * them) on a record stack.  With SET_MPG and a fixed bed the recorded steps
+ * skip the drag and its diagnostics (the per-particle forces after them
+ * are those of the last full step).  A pib_adjoint_step without a force weight takes
  * the top record when there is one, instead of recomputing the fields from
  * the forward state; with a force weight it always recomputes (it also needs
  * the vorticity and pbcp) and leaves the record stack alone.  Enqueued
@@ -313,8 +320,7 @@ int pib_adjoint_step(void* ctx, const double* wforce);
 int pib_adjoint_record(void* ctx, int nsteps);
 int pib_adjoint_get(void* ctx, double* dxc, double* dmpg, double* dumean);
 int pib_adjoint_set_umean(void* ctx, const double* dumean);
-/* Checkpoint stack of the forward state (u, p, nl, the physical copy of nl
- * and the scalars): push copies it on the device, pop restores the top and
+/* Checkpoint stack of the forward state (u, p, nl and the scalars): push copies it on the device, pop restores the top and
  * drops it.  PIB_ERR_STATE when popping an empty stack. */
 int pib_checkpoint_push(void* ctx);
 int pib_checkpoint_pop(void* ctx);

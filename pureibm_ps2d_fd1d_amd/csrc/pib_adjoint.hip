@@ -655,7 +655,7 @@ __global__ void __launch_bounds__(256) k_adj_forcing_points(KArgs<T> A, Particle
   double gx[3] = {0, 0, 0};
   if (l < P.nbnd && P.if_rev[(size_t)m * P.nbnd + l]) {
     const DevState* st = A.st;
-    const double c1 = st->coef[0][0], c2 = st->coef[0][1], c3 = st->coef[0][2], c4 = st->coef[0][3], c12 = c1 + c2;
+    const double c12 = st->coef[0][0] + st->coef[0][1];
     const double cf = st->s.cf, dx3 = st->dx3;
     const double rb = P.rad[m], ri = P.radi[m], ro = P.rado[m];
     double xli[3], xlo[3], rad = 0.0;
@@ -675,7 +675,8 @@ __global__ void __launch_bounds__(256) k_adj_forcing_points(KArgs<T> A, Particle
     make_sten(sp, xpi, A.nx, A.my, A.mz);
     make_sten(si, xli, A.nx, A.my, A.mz);
     // force_tmp of this pair with the gradients of its interpolations:
-    // Gi[d] = d/dpos of [cf uli - c3 nll - c4 onll - c12 dfll + c12 ppll]_d,
+    // Gi[d] = d/dpos of [cf uli - c3 nll - c4 onll - c12 dfll + c12 ppll]_d
+    // (the first four terms are the one field qin at the inner point),
     // Go[d] = d/dpos of ulo_d
     double ulo[3], Go[3][3], Gi[3][3], g[3];
     for (int n = 0; n < 3; ++n) {
@@ -699,16 +700,11 @@ __global__ void __launch_bounds__(256) k_adj_forcing_points(KArgs<T> A, Particle
       const double ubnorm = nv[d] * ubnmag, ubtang = linvel[d] - ubnorm;
       const double utangi = (ubtang * (ro - ri) - utang * (rb - ri)) / (ro - rb);
       const double unormi = rn * (ubnorm * (ro - ri) - unorm * (rb - ri)) / (ro - rb);
-      double f = cf * (interp_vg(si, A.ubcp + d * cs, g) - (utangi + unormi));
-      for (int e = 0; e < 3; ++e) Gi[d][e] = cf * g[e];
-      f = f - c3 * interp_vg(si, A.nlbc + d * cs, g);
-      for (int e = 0; e < 3; ++e) Gi[d][e] -= c3 * g[e];
-      f = f - c4 * interp_vg(si, A.onlbc + d * cs, g);
-      for (int e = 0; e < 3; ++e) Gi[d][e] -= c4 * g[e];
+      double f = cf * -(utangi + unormi) + interp_vg(si, A.qin + d * cs, g);
+      for (int e = 0; e < 3; ++e) Gi[d][e] = g[e];
       const double pp = interp_vg(sp, A.ppr + d * cs, g);
       for (int e = 0; e < 3; ++e) Gi[d][e] += c12 * g[e];
-      f = f + c12 * (pp - interp_vg(si, A.diffn + d * cs, g));
-      for (int e = 0; e < 3; ++e) Gi[d][e] -= c12 * g[e];
+      f = f + c12 * pp;
       ft[d] = f * K;
     }
     // spreading, transposed

@@ -246,7 +246,7 @@ __device__ inline void state_store(DevState* st, const DevState* L) {
 // compute_new_timestep tail (dependent_functions.F90:1143-1277); part holds
 // per-(component, plane) fluid/solid/total sums and max|u| from c2r_ubcp
 __global__ void __launch_bounds__(SCAL_T) k_scalar_cnt(DevState* stg, const double* part, int nx, int rks,
-                                                       const double* velbdy, int nb) {
+                                                       const double* velbdy, int nb, double* sconst) {
   __shared__ DevState L;
   __shared__ double sh[12 * 16];
   const int t = threadIdx.x;
@@ -322,6 +322,11 @@ __global__ void __launch_bounds__(SCAL_T) k_scalar_cnt(DevState* stg, const doub
       s.cf = -1.0 / s.dt;
       s.cforig = s.cf;
     }
+    // the stage's constants for kernels that run beside the drag's state
+    // update: cf, umean, the stage coefficients
+    sconst[0] = s.cf;
+    for (int n = 0; n < 3; ++n) sconst[1 + n] = s.umean[n];
+    for (int q = 0; q < 4; ++q) sconst[4 + q] = st->coef[rks - 1][q];
   }
   state_store(stg, &L);
 }
@@ -724,7 +729,8 @@ struct Impl : Base {
   std::vector<void*> allocs;
   T *wy, *wz, *w2;
   C2<T> *u, *p, *nl, *onl, *ff;
-  T *ubcp, *pbcp, *nlbc, *onlbc, *diffn, *omega, *fr, *ppr;
+  T *ubcp, *pbcp, *nlbc, *onlbc, *diffn, *omega, *fr, *ppr, *qin;
+  double* sconst;  // cf, umean and the coefficients of the current stage
   uint8_t* mask;
   DevState* dst;
   double* part;
@@ -917,6 +923,8 @@ struct Impl : Base {
     nlbc = alloc<T>(nsr + GATHER_PAD);
     onlbc = alloc<T>(nsr + GATHER_PAD);
     diffn = alloc<T>(nsr + GATHER_PAD);
+    qin = alloc<T>(nsr + GATHER_PAD);
+    sconst = alloc<double>(8);
     omega = alloc<T>(nsr + GATHER_PAD);
     fr = alloc<T>(nsr);
     ppr = alloc<T>(nsr + GATHER_PAD);
@@ -1022,6 +1030,8 @@ struct Impl : Base {
     A.omega = omega;
     A.fr = fr;
     A.ppr = ppr;
+    A.qin = qin;
+    A.sconst = sconst;
     A.mask = mask;
     A.st = dst;
     A.part = part;
@@ -1059,7 +1069,8 @@ struct Impl : Base {
   int plane_nt(int s) const { return s ? shape_nt(s) : nt_plane; }
   template <int KIND, int SH>
   void launch_c2r(int rks, int ncomp) {
-    static const char* names[] = {"c2r_ubcp", "c2r_pbcp", "c2r_nlbc", "c2r_onlbc", "c2r_visc", "c2r_omega", "c2r_pgrad"};
+    static const char* names[] = {"c2r_ubcp", "c2r_pbcp",  "c2r_nlbc",  "c2r_onlbc",
+                                  "c2r_visc", "c2r_omega", "c2r_pgrad", "c2r_inner"};
     auto kern = k_c2r<T, KIND, SH>;
     set_lds(kern, plane_lds());
     kb(names[KIND]);
@@ -1312,7 +1323,7 @@ struct Impl : Base {
   void compute_new_timestep(int rks) {
     c2r_pair<C2R_UBCP, 3, C2R_PBCP, 1>(rks, "c2r_ubcp_pbcp");
     kb("scalar_cnt");
-    k_scalar_cnt<<<1, SCAL_T, 0, stream>>>(dst, part, nx, rks, vel ? vel : part, vel ? nbody : 0);
+    k_scalar_cnt<<<1, SCAL_T, 0, stream>>>(dst, part, nx, rks, vel ? vel : part, vel ? nbody : 0, sconst);
     HIPCHK(hipGetLastError());
     ke();
   }
@@ -1373,14 +1384,12 @@ struct Impl : Base {
     bcset_forcing(rks);
   }
   // bcset up to where the reference moves particles (boundary_condition.F90:
-  // 517-538): calc_visc, compute_omega, compute_mpg_at_n and the diagnostics
+  // 517-538): compute_omega, compute_mpg_at_n and the diagnostics (calc_visc's
+  // diffn goes into qin, on the form_nl branch)
   void bcset_drag(int rks) {
-    if (skip_drag) {  // a recorded step: calc_visc only
-      c2r<C2R_VISC>(rks, 3);
-      return;
-    }
+    if (skip_drag) return;  // a recorded step
     Particles P = parts();
-    c2r_pair<C2R_VISC, 3, C2R_OMEGA, 3>(rks, "c2r_visc_omega");
+    c2r<C2R_OMEGA>(rks, 3);
     const int G = (nbnd + 255) / 256;
     kb("drag_points");
     const unsigned nbg = (unsigned)((nbody * G + 7) / 8 * 8);
@@ -1544,15 +1553,12 @@ struct Impl : Base {
   unsigned long long graph_generation() const override { return graph_gen; }
   void step_into_capture() override {
     C2<T>*nl0 = nl, *onl0 = onl;
-    T *nb0 = nlbc, *onb0 = onlbc;
     bool v0 = nlbc_valid;
     auto gen0 = state_gen;
     auto restore = [&] {
       capturing = false;
       nl = nl0;
       onl = onl0;
-      nlbc = nb0;
-      onlbc = onb0;
       nlbc_valid = v0;
       state_gen = gen0;
     };
@@ -1580,7 +1586,6 @@ struct Impl : Base {
   void advance_step() override {
     ++state_gen;
     std::swap(nl, onl);
-    std::swap(nlbc, onlbc);
     nlbc_valid = true;
     timed = false;
   }
@@ -1615,12 +1620,14 @@ struct Impl : Base {
     timed = nsteps > 0;
   }
   // nonlinear + bcset of one stage.  onl <- nl (nl.F90:99-108): the buffers
-  // swap roles; the physical copy of onl is the previous nlbc when that one
-  // is current.  With particles, the form_nl branch (form_nl and the nlbc
-  // transforms) runs on stream2 beside the bcset branch up to the pressure
-  // gradient (visc/omega transforms, drag, drag scalars, pgrad transform):
-  // the two touch disjoint buffers and only the latter reads or writes the
-  // device state.  The branches join before the forcing.
+  // swap roles.  With particles, the form_nl branch (form_nl and the qin
+  // transform, which takes nl, onl and u as spectra) runs on stream2 beside
+  // the bcset branch up to the pressure gradient (omega transform, drag,
+  // drag scalars, pgrad transform): the two write disjoint buffers, and the
+  // form_nl branch reads the stage's constants from sconst, not from the
+  // device state the drag scalars rewrite.  The branches join before the
+  // forcing.  nlbcp, onlbcp and diffn are not formed by the step; the
+  // field getters transform them on request.
   // part 0: the whole of nonlinear; 1: up to the drag (nonlinear_begin);
   // 2: the forcing (nonlinear_finish)
   void nonlinear_timed(int rks, bool rec, int part = 0) {
@@ -1629,8 +1636,6 @@ struct Impl : Base {
       return;
     }
     std::swap(nl, onl);
-    std::swap(nlbc, onlbc);
-    bool onlbc_ok = nlbc_valid;
     forked = use_fork && !ktime && nbody > 0 && part == 0;
     if (forked) {
       HIPCHK(hipEventRecord(evfork, stream));
@@ -1638,8 +1643,7 @@ struct Impl : Base {
       std::swap(stream, stream2);
     }
     form_nl(rks);
-    c2r<C2R_NLBC>(rks, 3);
-    if (!onlbc_ok) c2r<C2R_ONLBC>(rks, 3);
+    if (nbody > 0) c2r<C2R_INNER>(rks, 3);
     if (forked) {
       std::swap(stream, stream2);
       HIPCHK(hipEventRecord(evjoin, stream2));
@@ -2057,9 +2061,20 @@ struct Impl : Base {
       case PIB_F_FF: get_spec(ff, 3, h, true); break;
       case PIB_F_UBCP: get_real(ubcp, 3, h, true); break;
       case PIB_F_PBCP: get_real(pbcp, 1, h, false); break;
-      case PIB_F_NLBCP: get_real(nlbc, 3, h, true); break;
-      case PIB_F_ONLBCP: get_real(onlbc, 3, h, true); break;
-      case PIB_F_DIFFN: get_real(diffn, 3, h, true); break;
+      // not formed by the step (the forcing takes qin): transformed here
+      // from nl, onl and u as they stand
+      case PIB_F_NLBCP:
+        c2r<C2R_NLBC>(1, 3);
+        get_real(nlbc, 3, h, true);
+        break;
+      case PIB_F_ONLBCP:
+        c2r<C2R_ONLBC>(1, 3);
+        get_real(onlbc, 3, h, true);
+        break;
+      case PIB_F_DIFFN:
+        c2r<C2R_VISC>(1, 3);
+        get_real(diffn, 3, h, true);
+        break;
       case PIB_F_PPR: get_real(ppr, 3, h, false); break;
       case PIB_F_OMEGA: get_real(omega, 3, h, true); break;
       case PIB_F_AU: need_adj(); get_spec(J.au, 3, h, true); break;
@@ -2265,18 +2280,14 @@ struct Impl : Base {
     if (!nl_a) nl_a = nl;
     if (!adj_au0) adj_au0 = J.au;
     const int apar = J.au == adj_au0 ? 0 : 1;
-    hipGraphExec_t* slot = rec ? &agexec_rec[4 * rec_depth + 2 * (rec_owns[rec_depth] ? 1 : 0) + apar]
-                               : &agexec[(nl == nl_a ? 0 : 1) + 2 * apar];
+    hipGraphExec_t* slot = rec ? &agexec_rec[2 * rec_depth + apar] : &agexec[(nl == nl_a ? 0 : 1) + 2 * apar];
     if (!*slot) {
       C2<T>*nl0 = nl, *onl0 = onl;
-      T *nb0 = nlbc, *onb0 = onlbc;
       const bool v0 = nlbc_valid;
       const AdjArgs<T> J0 = J;
       auto restore = [&] {
         nl = nl0;
         onl = onl0;
-        nlbc = nb0;
-        onlbc = onb0;
         nlbc_valid = v0;
         J = J0;
       };
@@ -2300,7 +2311,6 @@ struct Impl : Base {
     HIPCHK(hipGraphLaunch(*slot, stream));
     if (!rec) {
       std::swap(nl, onl);
-      std::swap(nlbc, onlbc);
       nlbc_valid = true;
     }
     std::swap(J.au, J.aunew);
@@ -2309,23 +2319,19 @@ struct Impl : Base {
   }
   // the work of one adjoint step on the stream: the step's forward fields
   // that the adjoint reads -- compute_new_timestep (ubcp, pbcp, t, cf),
-  // nonlinear's onl <- nl, form_nl and nlbc, and bcset's diffn, ppr (and
-  // omega when the step's drag is weighed); the forward drag, forcing and ff
-  // are not needed -- then the transposed kernels.  rec: a record block whose
-  // fields and device state the kernels read instead.
+  // nonlinear's onl <- nl, form_nl and qin, bcset's ppr (and omega when the
+  // step's drag is weighed); the forward drag, forcing and ff are not needed
+  // -- then the transposed kernels.  rec: a record block whose fields and
+  // device state the kernels read instead.
   void adjoint_body(bool drag, const void* rec) {
     if (!rec) {
       compute_new_timestep(1);
       std::swap(nl, onl);
-      std::swap(nlbc, onlbc);
-      const bool onlbc_ok = nlbc_valid;
       form_nl(1);
-      c2r<C2R_NLBC>(1, 3);
-      if (!onlbc_ok) c2r<C2R_ONLBC>(1, 3);
       nlbc_valid = true;
       if (nbody > 0) {
-        if (drag) c2r_pair<C2R_VISC, 3, C2R_OMEGA, 3>(1, "c2r_visc_omega");
-        else c2r<C2R_VISC>(1, 3);
+        c2r<C2R_INNER>(1, 3);
+        if (drag) c2r<C2R_OMEGA>(1, 3);
         c2r<C2R_PGRAD>(1, 3);
       }
     }
@@ -2334,11 +2340,9 @@ struct Impl : Base {
       const size_t cs3 = (size_t)3 * nx * my * mz;
       T* b = (T*)rec;
       A.ubcp = b;
-      A.nlbc = b + cs3;
-      A.onlbc = rec_owns[rec_depth] ? b + 2 * cs3 : (T*)rec_blocks[rec_depth - 1] + cs3;
-      A.diffn = b + 3 * cs3;
-      A.ppr = b + 4 * cs3;
-      A.st = (DevState*)(b + 5 * cs3);
+      A.qin = b + cs3;
+      A.ppr = b + 2 * cs3;
+      A.st = (DevState*)(b + 3 * cs3);
     }
     AdjGeom g{cur_sh(), nt_plane, plane_lds(), nbody, nbnd, stream, coef[0][0] + coef[0][1], capturing};
     adjoint_step_kernels<T>(A, parts(), J, g);
@@ -2367,22 +2371,22 @@ struct Impl : Base {
   std::vector<void*> ck_pool;
   size_t part_bytes(int q) const {
     const size_t E = (size_t)my2 * mz;
-    const size_t sz[5] = {(size_t)3 * nx * E * sizeof(C2<T>), (size_t)3 * nx * E * sizeof(C2<T>),
-                          (size_t)nx * E * sizeof(C2<T>), (size_t)3 * nx * my * mz * sizeof(T), sizeof(DevState)};
+    const size_t sz[4] = {(size_t)3 * nx * E * sizeof(C2<T>), (size_t)3 * nx * E * sizeof(C2<T>),
+                          (size_t)nx * E * sizeof(C2<T>), sizeof(DevState)};
     return sz[q];
   }
-  // offsets of u, nl, p, nlbc and the device state in one checkpoint block
+  // offsets of u, nl, p and the device state in one checkpoint block
   size_t ck_parts(size_t* off) const {
     size_t o = 0;
-    for (int q = 0; q < 5; ++q) {
+    for (int q = 0; q < 4; ++q) {
       off[q] = o;
       o += (part_bytes(q) + 255) / 256 * 256;
     }
-    off[5] = o;
+    off[4] = o;
     return o;
   }
   void checkpoint_push() {
-    size_t off[6];
+    size_t off[5];
     const size_t bytes = ck_parts(off);
     void* m = nullptr;
     if (!ck_pool.empty()) {
@@ -2392,8 +2396,8 @@ struct Impl : Base {
       HIPCHK(hipMalloc(&m, bytes));
     }
     char* b = (char*)m;
-    const void* src[5] = {u, nl, p, nlbc, dst};
-    for (int q = 0; q < 5; ++q)
+    const void* src[4] = {u, nl, p, dst};
+    for (int q = 0; q < 4; ++q)
       HIPCHK(hipMemcpyAsync(b + off[q], src[q], part_bytes(q), hipMemcpyDeviceToDevice, stream));
     ck_stack.push_back({m, nlbc_valid});
   }
@@ -2414,37 +2418,36 @@ struct Impl : Base {
     }
   }
   // ---- the recorded forward fields of steps to be reversed ----
-  // After a whole step, ubcp, nlbc, onlbc, diffn and ppr still hold that
-  // step's values (velstep writes only u and p), together with the step's t,
-  // dt and cf in the device state: what the adjoint of the step reads
-  // besides the mask and the bed.  adjoint_record runs steps whose ubcp,
-  // nlbc, diffn and ppr are written straight into the record block (the
-  // context's buffers are pointed there for the step) and copies the device
-  // state after it; adjoint_step then takes them instead of recomputing them.
-  // record blocks by stack depth (a segment's records are pushed and taken
-  // in stack order, so a depth's block always holds the same step position
-  // and the graphs of steps on recorded fields stay valid); each block is
-  // the five 3-component fields, then the device state, then GATHER_PAD
-  // elements for the stencil loads past the last field
+  // After a whole step, ubcp, qin and ppr still hold that step's values
+  // (velstep writes only u and p), together with the step's t, dt and cf in
+  // the device state: what the adjoint of the step reads besides the mask
+  // and the bed.  adjoint_record runs steps whose ubcp, qin and ppr are
+  // written straight into the record block (the context's buffers are
+  // pointed there for the step) and copies the device state after it;
+  // adjoint_step then takes them instead of recomputing them.  Record blocks
+  // by stack depth (a segment's records are pushed and taken in stack order,
+  // so a depth's block always holds the same step position and the graphs
+  // of steps on recorded fields stay valid); each block is the three
+  // 3-component fields, then the device state, then GATHER_PAD elements for
+  // the stencil loads past the last field.
   std::vector<void*> rec_blocks;
-  std::vector<bool> rec_owns;  // the record holds its own onlbc
   int rec_depth = 0;
   // state_gen counts whatever changes the forward state (steps, restores,
-  // sets); rec_gen is its value after the last recorded step
-  unsigned long long state_gen = 0, rec_gen = ~0ull;
+  // sets)
+  unsigned long long state_gen = 0;
   // recorded steps skip the drag and its diagnostics (the vorticity, the
   // per-particle forces, norm_drag): with the mean pressure gradient set and
   // the bed fixed nothing of the state depends on them
   bool skip_drag = false;
   int lite_eager = 0;  // eager recorded steps since the graphs were dropped
   // graphs of recorded steps, by the buffers they read and write
-  std::map<std::array<const void*, 7>, hipGraphExec_t> rgexec;
+  std::map<std::array<const void*, 5>, hipGraphExec_t> rgexec;
   size_t rec_bytes() const {
-    return (size_t)15 * nx * my * mz * sizeof(T) + sizeof(DevState) + GATHER_PAD * sizeof(T);
+    return (size_t)9 * nx * my * mz * sizeof(T) + sizeof(DevState) + GATHER_PAD * sizeof(T);
   }
   void rec_step() {
     if (use_graph && !ktime && eager_steps >= 2 && nlbc_valid && (!skip_drag || lite_eager >= 1)) {
-      const std::array<const void*, 7> key{nl, nlbc, onlbc, ubcp, diffn, ppr, skip_drag ? this : nullptr};
+      const std::array<const void*, 5> key{nl, ubcp, qin, ppr, skip_drag ? this : nullptr};
       hipGraphExec_t& slot = rgexec[key];
       if (!slot) {
         hipGraph_t g = capture_step();
@@ -2466,26 +2469,17 @@ struct Impl : Base {
     if (nbody == 0) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
     if (nsteps <= 0) return;
     const size_t cs3 = (size_t)3 * nx * my * mz, fb = cs3 * sizeof(T);
-    // the context's own buffers: nl/onl swap every step, nlbc/onlbc with
-    // them, so after nsteps steps nlbc is own_nb[nsteps & 1]
-    T* const own_nb[2] = {nlbc, onlbc};
-    T *const ub0 = ubcp, *const df0 = diffn, *const pp0 = ppr;
+    T *const ub0 = ubcp, *const qi0 = qin, *const pp0 = ppr;
+    // back to the context's own buffers, holding the last step's fields
     auto back = [&](int done) {
       skip_drag = false;
       if (done > 0) {
-        T* const last = nlbc;
-        nlbc = own_nb[done & 1];
-        onlbc = own_nb[(done & 1) ^ 1];
-        HIPCHK(hipMemcpyAsync(nlbc, last, fb, hipMemcpyDeviceToDevice, stream));
         HIPCHK(hipMemcpyAsync(ub0, ubcp, fb, hipMemcpyDeviceToDevice, stream));
-        HIPCHK(hipMemcpyAsync(df0, diffn, fb, hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(qi0, qin, fb, hipMemcpyDeviceToDevice, stream));
         HIPCHK(hipMemcpyAsync(pp0, ppr, fb, hipMemcpyDeviceToDevice, stream));
-      } else {
-        nlbc = own_nb[0];
-        onlbc = own_nb[1];
       }
       ubcp = ub0;
-      diffn = df0;
+      qin = qi0;
       ppr = pp0;
     };
     skip_drag = cfg.set_mpg && !cfg.move_particles;
@@ -2498,22 +2492,11 @@ struct Impl : Base {
           rec_blocks.push_back(m);
         }
         T* b = (T*)rec_blocks[rec_depth];
-        // nonlinear swaps nlbc and onlbc first: the step's nlbc lands where
-        // onlbc points
         ubcp = b;
-        onlbc = b + cs3;
-        diffn = b + 3 * cs3;
-        ppr = b + 4 * cs3;
+        qin = b + cs3;
+        ppr = b + 2 * cs3;
         rec_step();
-        // a step's onlbc is the nlbc of the step before it, which the record
-        // below this one holds when that record is of the step just before
-        // (nothing ran or restored the state since); else it keeps its own
-        const bool own_onlbc = rec_depth == 0 || state_gen != rec_gen + 1;
-        rec_gen = state_gen;
-        if ((int)rec_owns.size() <= rec_depth) rec_owns.resize(rec_depth + 1);
-        rec_owns[rec_depth] = own_onlbc;
-        if (own_onlbc) HIPCHK(hipMemcpyAsync(b + 2 * cs3, onlbc, fb, hipMemcpyDeviceToDevice, stream));
-        HIPCHK(hipMemcpyAsync(b + 5 * cs3, dst, sizeof(DevState), hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(b + 3 * cs3, dst, sizeof(DevState), hipMemcpyDeviceToDevice, stream));
         ++rec_depth;
       }
     } catch (...) {
@@ -2606,13 +2589,13 @@ struct Impl : Base {
   }
   void checkpoint_pop() {
     if (ck_stack.empty()) throw PibFail(PIB_ERR_STATE, "checkpoint stack is empty");
-    size_t off[6];
+    size_t off[5];
     ck_parts(off);
     auto top = ck_stack.back();
     ck_stack.pop_back();
     const char* b = (const char*)top.first;
-    void* dstp[5] = {u, nl, p, nlbc, dst};
-    for (int q = 0; q < 5; ++q)
+    void* dstp[4] = {u, nl, p, dst};
+    for (int q = 0; q < 4; ++q)
       HIPCHK(hipMemcpyAsync(dstp[q], b + off[q], part_bytes(q), hipMemcpyDeviceToDevice, stream));
     nlbc_valid = top.second;
     ++state_gen;

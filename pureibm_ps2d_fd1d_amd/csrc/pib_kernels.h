@@ -52,6 +52,13 @@ struct KArgs {
   T* omega;
   T* fr;
   T* ppr;
+  // the inner-point combination of the reversal forcing, one field for the
+  // four that calc_inner_reversal_forcing interpolates at the same point:
+  // qin = cf ubcp - c3 nlbc - c4 onlbc - (c1 + c2) diffn
+  T* qin;
+  // the stage's cf, umean[3] and coefficients c1..c4, written by the step's
+  // first scalar kernel; read where the device state may be rewritten beside
+  const double* sconst;
   const uint8_t* mask;
   DevState* st;
   double* part;  // per-workgroup partial results
@@ -356,8 +363,43 @@ struct OpPgrad {
   }
 };
 
+// The inner-point combination of calc_inner_reversal_forcing
+// (boundary_condition.F90:1104-1479) and calc_bnd_forcing (:902-1102): the
+// forcing interpolates cf ubcp, nlbcp, onlbcp and diffn at one point and
+// combines them linearly with coefficients fixed for the stage, so the
+// combination is formed on the spectra and transformed once:
+// qin = c2r(cf u - c3 nl - c4 onl - c12 vis (delta_x^2 u / dx2 - w2 u))
+//       + cf umean
+template <typename T>
+struct OpInner {
+  const KArgs<T>& A;
+  int c, i;
+  T cf, c3, c4, c12v, add;
+  __device__ OpInner(const KArgs<T>& a, int c_, int i_) : A(a), c(c_), i(i_) {
+    const double* k = A.sconst;
+    cf = (T)k[0];
+    c3 = (T)k[6];
+    c4 = (T)k[7];
+    c12v = (T)((k[4] + k[5]) * (double)A.vis);
+    add = (T)(k[0] * k[1 + c]);
+  }
+  __device__ C2<T> load(int ky, int kz) {
+    int im = wrapn(i - 1, A.nx), ip = wrapn(i + 1, A.nx);
+    const size_t o = sidx(A, c, i, ky, kz);
+    C2<T> um = A.u[sidx(A, c, im, ky, kz)], uc = A.u[o], up = A.u[sidx(A, c, ip, ky, kz)];
+    C2<T> d2 = (um - cscal((T)2, uc)) + up;
+    C2<T> v = cscal((T)1 / A.dx2, d2) - cscal(A.w2[(size_t)kz * A.my2 + ky], uc);
+    return ((cscal(cf, uc) - cscal(c3, A.nl[o])) - cscal(c4, A.onl[o])) - cscal(c12v, v);
+  }
+  __device__ void store2(int, int y, int z, T a, T b) {
+    size_t o = ridx(A, c, i, y, z);
+    A.qin[o] = a + add;
+    A.qin[o + 1] = b + add;
+  }
+};
+
 // generic c2r launcher body: grid (nplanes, ncomp)
-enum { C2R_UBCP = 0, C2R_PBCP, C2R_NLBC, C2R_ONLBC, C2R_VISC, C2R_OMEGA, C2R_PGRAD };
+enum { C2R_UBCP = 0, C2R_PBCP, C2R_NLBC, C2R_ONLBC, C2R_VISC, C2R_OMEGA, C2R_PGRAD, C2R_INNER };
 
 template <typename T, int KIND, int SH>
 __device__ __forceinline__ void c2r_plane(const KArgs<T>& A, unsigned char* smem, int c, int i) {
@@ -386,6 +428,9 @@ __device__ __forceinline__ void c2r_plane(const KArgs<T>& A, unsigned char* smem
     plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op, smem);
   } else if constexpr (KIND == C2R_OMEGA) {
     OpOmega<T> op{A, c, i};
+    plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op, smem);
+  } else if constexpr (KIND == C2R_INNER) {
+    OpInner<T> op(A, c, i);
     plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op, smem);
   } else {
     OpPgrad<T> op{A, c, i, (T)A.st->s.mpg[c]};
@@ -812,8 +857,7 @@ template <typename T>
 __device__ void forcing_point(const KArgs<T>& A, const Particles& P, int m, int l, double* o, double* fsum) {
   const DevState* st = A.st;
   const int rks = A.rks;
-  const double c1 = st->coef[rks - 1][0], c2 = st->coef[rks - 1][1], c3 = st->coef[rks - 1][2],
-               c4 = st->coef[rks - 1][3];
+  const double c1 = st->coef[rks - 1][0], c2 = st->coef[rks - 1][1];
   double rb = P.rad[m], ri = P.radi[m], ro = P.rado[m];
   double xli[3], xlo[3], rad = 0.0;
   for (int n = 0; n < 3; ++n) {
@@ -856,16 +900,11 @@ __device__ void forcing_point(const KArgs<T>& A, const Particles& P, int m, int 
     if (st->revernorm == 1) unormi = (ubnorm * (ro - ri) - unorm * (rb - ri)) / (ro - rb);
     ft[d] = -(utangi + unormi);  // -ucar
   }
-  for (int n = 0; n < 3; ++n) ft[n] = st->s.cf * (ft[n] + interp_kji(si, A.ubcp + n * cs));
+  // cf (uli - ucar) - c3 nll - c4 onll + c12 (ppll - dfll), the four
+  // inner-point terms as one interpolation of qin
+  for (int n = 0; n < 3; ++n) ft[n] = st->s.cf * ft[n] + interp_kji(si, A.qin + n * cs);
   for (int n = 0; n < 3; ++n) pin_reg(ft[n]);
-  for (int n = 0; n < 3; ++n) ft[n] = ft[n] - c3 * interp_ijk(si, A.nlbc + n * cs);
-  for (int n = 0; n < 3; ++n) pin_reg(ft[n]);
-  for (int n = 0; n < 3; ++n) ft[n] = ft[n] - c4 * interp_ijk(si, A.onlbc + n * cs);
-  for (int n = 0; n < 3; ++n) pin_reg(ft[n]);
-  double ppll[3];
-  for (int n = 0; n < 3; ++n) ppll[n] = interp_kji(sp, A.ppr + n * cs);
-  for (int n = 0; n < 3; ++n) pin_reg(ppll[n]);
-  for (int n = 0; n < 3; ++n) ft[n] = ft[n] + (c1 + c2) * (ppll[n] - interp_ijk(si, A.diffn + n * cs));
+  for (int n = 0; n < 3; ++n) ft[n] = ft[n] + (c1 + c2) * interp_kji(sp, A.ppr + n * cs);
   for (int n = 0; n < 3; ++n) pin_reg(ft[n]);
   double rdi = ri * st->dx;
   double da2 = 4.0 * 3.14159265358979323846 * (rdi * rdi) / (double)P.nrpr_active[m];
@@ -915,8 +954,7 @@ __global__ void __launch_bounds__(256) k_bnd_points(KArgs<T> A, Particles P, dou
   if (l < P.nbnd) {
     const DevState* st = A.st;
     const int rks = A.rks;
-    const double c1 = st->coef[rks - 1][0], c2 = st->coef[rks - 1][1], c3 = st->coef[rks - 1][2],
-                 c4 = st->coef[rks - 1][3];
+    const double c1 = st->coef[rks - 1][0], c2 = st->coef[rks - 1][1];
     const double rb = P.rad[m];
     double xl[3];
     for (int n = 0; n < 3; ++n) xl[n] = P.xc[n * P.nbody + m] + P.bnd[l * 3 + n] * rb;
@@ -931,13 +969,10 @@ __global__ void __launch_bounds__(256) k_bnd_points(KArgs<T> A, Particles P, dou
     double ft[3];
     for (int n = 0; n < 3; ++n) {
       const double linvel = P.vel[n * P.nbody + m] + crossp[n] * rb * st->dx;
-      ft[n] = st->s.cf * (-linvel + interp_kji(sv, A.ubcp + n * cs));
+      ft[n] = st->s.cf * (-linvel) + interp_kji(sv, A.qin + n * cs);
     }
     for (int n = 0; n < 3; ++n) pin_reg(ft[n]);
-    for (int n = 0; n < 3; ++n) ft[n] = ft[n] - c3 * interp_ijk(sv, A.nlbc + n * cs) - c4 * interp_ijk(sv, A.onlbc + n * cs);
-    for (int n = 0; n < 3; ++n) pin_reg(ft[n]);
-    for (int n = 0; n < 3; ++n)
-      ft[n] = ft[n] + (c1 + c2) * (interp_kji(sp, A.ppr + n * cs) - interp_ijk(sv, A.diffn + n * cs));
+    for (int n = 0; n < 3; ++n) ft[n] = ft[n] + (c1 + c2) * interp_kji(sp, A.ppr + n * cs);
     const double rdx = rb * st->dx;
     const double da1 = 4.0 * 3.14159265358979323846 * (rdx * rdx) / (double)P.nbnd;
     for (int n = 0; n < 3; ++n) ft[n] = ft[n] * da1 * st->dr * st->dx;

@@ -17,9 +17,9 @@
 // x-flux derivative, the sign, plane nx+1) -> k_scal_phys (nl, onl, diffn and
 // grad phi in physical space) -> fill fr -> k_scal_bnd (surface flux and
 // DOBND forcing, one workgroup per body) -> k_scal_rev (reversal pairs, one
-// lane per pair) -> k_scal_ff (ff2rc of fr) -> k_phistep -> k_ratio_roll.
-// The spreading adds with atomics (no fixed order: the sums agree with the
-// reference's to round-off, not bitwise).
+// lane per pair) -> k_scal_spread (the per-point values gathered onto the
+// grid through the fluid's node-sorted records) -> k_scal_ff (ff2rc of fr)
+// -> k_phistep -> k_ratio_roll.  No atomics: every sum has a fixed order.
 #pragma once
 
 template <typename T>
@@ -601,18 +601,24 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_scal_spread(KArgs<T> A, ScalStep<T> S, const int* bt, int nrange,
                                                      const int* tnode, const int* toff, const int* rpt,
                                                      const double* rw, const uint8_t* rdist) {
+  __shared__ double cv[SPREAD_CAP];
   const int w = xcd_linear(blockIdx.x, gridDim.x);
   if (w >= nrange) return;
   const int t0 = bt[w], t1 = bt[w + 1];
+  const int r0 = toff[t0], r1 = toff[t1];
   const double dx3 = A.st->dx3;
+  // each record's value staged in LDS (a range holds at most SPREAD_CAP
+  // records), then one lane per node sums its records in record order
+  for (int r = r0 + (int)threadIdx.x; r < r1; r += 256) {
+    const uint8_t f = rdist[r];
+    const double* o = S.spt + (size_t)rpt[r] * 4 + ((f & 2) ? 2 : 0);
+    cv[r - r0] = (rw[r] * o[0]) / dx3 + ((f & 1) ? o[1] : 0.0);
+  }
+  __syncthreads();
   const size_t plane = (size_t)A.my * A.mz;
-  for (int t = t0 + (int)threadIdx.x; t < t1; t += blockDim.x) {
+  for (int t = t0 + (int)threadIdx.x; t < t1; t += 256) {
     double acc = 0.0;
-    for (int r = toff[t]; r < toff[t + 1]; ++r) {
-      const uint8_t f = rdist[r];
-      const double* o = S.spt + (size_t)rpt[r] * 4 + ((f & 2) ? 2 : 0);
-      acc = acc + ((rw[r] * o[0]) / dx3 + ((f & 1) ? o[1] : 0.0));
-    }
+    for (int r = toff[t]; r < toff[t + 1]; ++r) acc = acc + cv[r - r0];
     S.fr[(size_t)tnode[t] + plane] = (T)acc;
   }
 }
