@@ -165,9 +165,13 @@ __device__ __host__ inline int rec_point(int r8, int nbody, int nbnd, int dobnd)
 __global__ void k_spread_records(Particles P, const uint8_t* mask, int nx, int my, int mz, int frmeanfluid,
                                  int dorpr, int dobnd, unsigned int sentinel, unsigned int* key, int* rec, double* w,
                                  uint8_t* dist) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  // one lane per record (point idx, corner cidx = (k*2 + j)*2 + i), so that
+  // the four record arrays are written coalesced
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int npk = P.nbody * P.nbnd;
-  if (idx >= (dobnd ? 2 * npk : npk)) return;
+  if (q >= (size_t)(dobnd ? 2 * npk : npk) * 8) return;
+  const int idx = (int)(q >> 3), cidx = (int)(q & 7);
+  const int i = cidx & 1, j = (cidx >> 1) & 1, k = cidx >> 2;
   int m, l, kind;
   if (dobnd) {
     m = idx / (2 * P.nbnd);
@@ -185,19 +189,13 @@ __global__ void k_spread_records(Particles P, const uint8_t* mask, int nx, int m
   for (int n = 0; n < 3; ++n) x[n] = P.xc[n * P.nbody + m] + P.bnd[l * 3 + n] * r;
   Sten s;
   make_sten(s, x, nx, my, mz);
-  for (int k = 0; k < 2; ++k)
-    for (int j = 0; j < 2; ++j)
-      for (int i = 0; i < 2; ++i) {
-        int cidx = (k * 2 + j) * 2 + i;
-        size_t q = (size_t)idx * 8 + cidx;
-        unsigned int node = s.node(i, j, k);
-        bool fl = mask[node] != 0;
-        bool dep = act && (kind == 0 || frmeanfluid ? true : !fl);
-        key[q] = dep ? node : sentinel;
-        rec[q] = (int)q;
-        w[q] = s.w(i, j, k);
-        dist[q] = (uint8_t)(kind == 0 || frmeanfluid ? 0 : 1);
-      }
+  const unsigned int node = s.node(i, j, k);
+  const bool fl = mask[node] != 0;
+  const bool dep = act && (kind == 0 || frmeanfluid ? true : !fl);
+  key[q] = dep ? node : sentinel;
+  rec[q] = (int)q;
+  w[q] = s.w(i, j, k);
+  dist[q] = (uint8_t)(kind == 0 || frmeanfluid ? 0 : 1);
 }
 __global__ void k_gather_records(int n, const int* rec_sorted, const double* w, const uint8_t* dist, int nbody,
                                  int nbnd, int dobnd, int* rpt, double* rw, uint8_t* rd) {
@@ -207,6 +205,33 @@ __global__ void k_gather_records(int n, const int* rec_sorted, const double* w, 
   rpt[t] = rec_point(r / 8, nbody, nbnd, dobnd);
   rw[t] = w[r];
   rd[t] = (uint8_t)(dist[r] | ((r & 1) << 1));  // bit 1: the stencil's upper x node (corner i = 1)
+}
+// The run of the sentinel key (records that deposit nowhere) sorts last:
+// its count is cleared so that the scan and the maximum over the whole count
+// array see the target runs only; meta[0] = the number of target nodes.
+__global__ void k_runs_fix(const unsigned int* ukeys, int* counts, const int* nruns, unsigned int sentinel, int* meta) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int n = *nruns;
+  int nt = n;
+  if (n > 0 && ukeys[n - 1] >= sentinel) {
+    counts[n - 1] = 0;
+    nt = n - 1;
+  }
+  meta[0] = nt;
+}
+// meta[1] = the number of target records (offset past the last target)
+__global__ void k_runs_total(const int* toff, int* meta) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  meta[1] = toff[meta[0]];
+}
+// the mask-dependent scalars of the device state after a new particle set
+__global__ void k_state_counts(DevState* st, int nb, int nbnd, double cs, double ntot) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  st->nbody = nb;
+  st->nbnd = nbnd;
+  st->s.count_solid = cs;
+  st->s.count_fluid = ntot - cs;
+  st->s.maxvolfrac = cs / ntot;
 }
 
 // ===========================================================================
@@ -731,6 +756,8 @@ struct Impl : Base {
   C2<T> *u, *p, *nl, *onl, *ff;
   T *ubcp, *pbcp, *nlbc, *onlbc, *diffn, *omega, *fr, *ppr, *qin;
   double* sconst;  // cf, umean and the coefficients of the current stage
+  double* keep_dev = nullptr;  // per-particle force sums parked across a particle update
+  size_t keep_n = 0;
   uint8_t* mask;
   DevState* dst;
   double* part;
@@ -805,7 +832,8 @@ struct Impl : Base {
   }
 
   template <typename V>
-  V* alloc(size_t n, bool part_dep = false) {
+  // zero = false for a buffer that its producer writes whole before any read
+  V* alloc(size_t n, bool part_dep = false, bool zero = true) {
     const size_t bytes = n * sizeof(V) + 64;
     void* ptr = nullptr;
     if (part_dep) {
@@ -816,7 +844,7 @@ struct Impl : Base {
       }
     }
     if (!ptr) HIPCHK(hipMalloc(&ptr, bytes));
-    HIPCHK(hipMemsetAsync(ptr, 0, bytes, stream));
+    if (zero) HIPCHK(hipMemsetAsync(ptr, 0, bytes, stream));
     if (part_dep)
       pallocs.push_back({ptr, bytes});
     else
@@ -971,6 +999,7 @@ struct Impl : Base {
     for (auto& c : ck_stack) (void)hipFree(c.first);
     for (void* q : ck_pool) (void)hipFree(q);
     for (void* q : rec_blocks) (void)hipFree(q);
+    if (keep_dev) (void)hipFree(keep_dev);
     free_adj_records();
     for (auto& q : pallocs) (void)hipFree(q.first);
     drain_pool();
@@ -1123,38 +1152,8 @@ struct Impl : Base {
   void set_particles(int nb, const double* hxc, const double* hrad, const double* hvel, const double* hangv,
                      long long* count_solid, int* nrpr_active) {
     if (nb > 0 && !have_points) throw PibFail(PIB_ERR_STATE, "set_points must precede set_particles");
-    HIPCHK(hipStreamSynchronize(stream));
-    drop_graphs();
-    ++part_gen;
-    // the per-particle force sums of the step live on when the same bed
-    // moves: the reference's grid_nodes_insphere leaves them alone
-    std::vector<double> keep;
-    if (have_particles && nb == nbody && nb > 0) {
-      keep.resize(15 * (size_t)nb);
-      double* src[5] = {pres, visc, torq, force, force_chem};
-      for (int a = 0; a < 5; ++a)
-        HIPCHK(hipMemcpy(keep.data() + (size_t)a * 3 * nb, src[a], 3 * nb * sizeof(double), hipMemcpyDeviceToHost));
-    }
-    // a moved bed of the same size reuses the previous set's buffers
-    for (auto& q : pallocs) ppool.insert({q.second, q.first});
-    pallocs.clear();
-    nbody = nb;
-    size_t n3 = (size_t)3 * (nb > 0 ? nb : 1);
-    xc = alloc<double>(n3, true);
-    rad = alloc<double>(n3, true);
-    radi = alloc<double>(n3, true);
-    rado = alloc<double>(n3, true);
-    vel = alloc<double>(n3, true);
-    angv = alloc<double>(n3, true);
-    pres = alloc<double>(n3, true);
-    visc = alloc<double>(n3, true);
-    torq = alloc<double>(n3, true);
-    force = alloc<double>(n3, true);
-    force_chem = alloc<double>(n3, true);
-    nrpr = alloc<int>(nb > 0 ? nb : 1, true);
-    long long* cnt = alloc<long long>(nb > 0 ? nb : 1, true);
-    std::vector<double> hr(nb), hri(nb), hro(nb);
     const int nmin = std::min(nx, std::min(my, mz));
+    std::vector<double> hr(nb), hri(nb), hro(nb);
     for (int m = 0; m < nb; ++m) {
       if (!(hrad[m] > cfg.dr && hrad[m] + cfg.dr + 2 < 0.5 * nmin))
         throw PibFail(PIB_ERR_ARG, "radbdy must exceed dr and stay below half the box");
@@ -1168,6 +1167,43 @@ struct Impl : Base {
       hri[m] = hrad[m] - cfg.dr;
       hro[m] = hrad[m] + cfg.dr;
     }
+    HIPCHK(hipStreamSynchronize(stream));
+    drop_graphs();
+    ++part_gen;
+    // the per-particle force sums of the step live on when the same bed
+    // moves (the reference's grid_nodes_insphere leaves them alone): parked
+    // on the device while the buffers are handed out again
+    const bool keep = have_particles && nb == nbody && nb > 0;
+    if (keep) {
+      if (keep_n < (size_t)15 * nb) {
+        if (keep_dev) HIPCHK(hipFree(keep_dev));
+        keep_dev = nullptr;
+        HIPCHK(hipMalloc(&keep_dev, (size_t)15 * nb * sizeof(double)));
+        keep_n = (size_t)15 * nb;
+      }
+      const double* src[5] = {pres, visc, torq, force, force_chem};
+      for (int a = 0; a < 5; ++a)
+        HIPCHK(hipMemcpyAsync(keep_dev + (size_t)a * 3 * nb, src[a], 3 * nb * sizeof(double),
+                              hipMemcpyDeviceToDevice, stream));
+    }
+    // a moved bed of the same size reuses the previous set's buffers
+    for (auto& q : pallocs) ppool.insert({q.second, q.first});
+    pallocs.clear();
+    nbody = nb;
+    size_t n3 = (size_t)3 * (nb > 0 ? nb : 1);
+    xc = alloc<double>(n3, true, nb == 0);
+    rad = alloc<double>(n3, true, nb == 0);
+    radi = alloc<double>(n3, true, nb == 0);
+    rado = alloc<double>(n3, true, nb == 0);
+    vel = alloc<double>(n3, true, !hvel);
+    angv = alloc<double>(n3, true, !hangv);
+    pres = alloc<double>(n3, true, !keep);
+    visc = alloc<double>(n3, true, !keep);
+    torq = alloc<double>(n3, true, !keep);
+    force = alloc<double>(n3, true, !keep);
+    force_chem = alloc<double>(n3, true, !keep);
+    nrpr = alloc<int>(nb > 0 ? nb : 1, true);
+    long long* cnt = alloc<long long>(nb > 0 ? nb : 1, true);
     host_rad = hr;
     if (nb > 0) {
       HIPCHK(hipMemcpyAsync(xc, hxc, n3 * sizeof(double), hipMemcpyHostToDevice, stream));
@@ -1176,38 +1212,30 @@ struct Impl : Base {
       HIPCHK(hipMemcpyAsync(rado, hro.data(), nb * sizeof(double), hipMemcpyHostToDevice, stream));
       if (hvel) HIPCHK(hipMemcpyAsync(vel, hvel, n3 * sizeof(double), hipMemcpyHostToDevice, stream));
       if (hangv) HIPCHK(hipMemcpyAsync(angv, hangv, n3 * sizeof(double), hipMemcpyHostToDevice, stream));
-      if (!keep.empty()) {
+      if (keep) {
         double* dstp[5] = {pres, visc, torq, force, force_chem};
         for (int a = 0; a < 5; ++a)
-          HIPCHK(hipMemcpyAsync(dstp[a], keep.data() + (size_t)a * 3 * nb, 3 * nb * sizeof(double),
-                                hipMemcpyHostToDevice, stream));
+          HIPCHK(hipMemcpyAsync(dstp[a], keep_dev + (size_t)a * 3 * nb, 3 * nb * sizeof(double),
+                                hipMemcpyDeviceToDevice, stream));
       }
     }
     // solid mask and counts
     fill_mask();
-    long long cs = 0;
+    std::vector<long long> hc(nb > 0 ? nb : 1, 0);
+    std::vector<int> hn(nb > 0 ? nb : 1, 0), hcnt(nb > 0 ? nb : 1, 0);
     if (nb > 0) {
       k_mask_body<<<nb, 256, 0, stream>>>(mask, xc, rad, nb, nx, my, mz, cnt);
       HIPCHK(hipGetLastError());
-      std::vector<long long> hc(nb);
       HIPCHK(hipMemcpyAsync(hc.data(), cnt, nb * sizeof(long long), hipMemcpyDeviceToHost, stream));
-      HIPCHK(hipStreamSynchronize(stream));
-      for (long long v : hc) cs += v;
-    }
-    size_t npts = (size_t)nb * nbnd;
-    std::vector<int> hn(nb > 0 ? nb : 1, 0);
-    if (nb > 0) {
-      // neighbour sets, active reversal pairs
-      int* nbl = alloc<int>((size_t)nb * MAXNB, true);
-      int* nbc = alloc<int>(nb, true);
+      // neighbour sets (k_if_rev reads at most MAXNB members; a larger count
+      // is refused below), active reversal pairs
+      int* nbl = alloc<int>((size_t)nb * MAXNB, true, false);
+      int* nbc = alloc<int>(nb, true, false);
       k_neighbours<<<nb, 256, 0, stream>>>(xc, rad, nb, cfg.dx, nx * cfg.dx, my * cfg.dx, mz * cfg.dx, nbl, nbc);
       HIPCHK(hipGetLastError());
-      std::vector<int> hcnt(nb);
       HIPCHK(hipMemcpyAsync(hcnt.data(), nbc, nb * sizeof(int), hipMemcpyDeviceToHost, stream));
-      HIPCHK(hipStreamSynchronize(stream));
-      for (int v : hcnt)
-        if (v > MAXNB) throw PibFail(PIB_ERR_ARG, "more than 96 neighbours of one sphere (overlapping bed?)");
-      if_rev = alloc<uint8_t>(npts, true);
+      size_t npts = (size_t)nb * nbnd;
+      if_rev = alloc<uint8_t>(npts, true, false);
       Particles P = parts();
       k_if_rev<<<(unsigned)((npts + 255) / 256), 256, 0, stream>>>(P, nx, my, mz, nbl, nbc, if_rev);
       HIPCHK(hipGetLastError());
@@ -1215,22 +1243,20 @@ struct Impl : Base {
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(hn.data(), nrpr, nb * sizeof(int), hipMemcpyDeviceToHost, stream));
       const size_t G = (nbnd + 255) / 256;
-      drag_part = alloc<double>((size_t)nb * G * 9, true);
+      // written whole by their kernels every step before any read
+      drag_part = alloc<double>((size_t)nb * G * 9, true, false);
       const size_t kinds = cfg.dobnd ? 2 : 1;  // reversal pairs, and the surface points of calc_bnd_forcing
-      frc_part = alloc<double>(kinds * nb * G * 6, true);
-      frc_pts = alloc<double>(kinds * npts * 6, true);
-      build_spreading();
+      frc_part = alloc<double>(kinds * nb * G * 6, true, false);
+      frc_pts = alloc<double>(kinds * npts * 6, true, false);
+      build_spreading();  // waits for the stream: the copies above are in
+      for (int m = 0; m < nb; ++m)
+        if (hcnt[m] > MAXNB) throw PibFail(PIB_ERR_ARG, "more than 96 neighbours of one sphere (overlapping bed?)");
     }
-    HIPCHK(hipStreamSynchronize(stream));
-    // scalars that depend on the mask
-    DevState h;
-    HIPCHK(hipMemcpy(&h, dst, sizeof h, hipMemcpyDeviceToHost));
-    h.nbody = nb;
-    h.nbnd = nbnd;
-    h.s.count_solid = (double)cs;
-    h.s.count_fluid = (double)((long long)nx * my * mz) - (double)cs;
-    h.s.maxvolfrac = (double)cs / (double)((long long)nx * my * mz);
-    HIPCHK(hipMemcpy(dst, &h, sizeof h, hipMemcpyHostToDevice));
+    long long cs = 0;
+    for (int m = 0; m < nb; ++m) cs += hc[m];
+    // scalars that depend on the mask, set on the device
+    k_state_counts<<<1, 64, 0, stream>>>(dst, nb, nbnd, (double)cs, (double)((long long)nx * my * mz));
+    HIPCHK(hipGetLastError());
     if (count_solid) *count_solid = cs;
     if (nrpr_active)
       for (int m = 0; m < nb; ++m) nrpr_active[m] = hn[m];
@@ -1247,55 +1273,53 @@ struct Impl : Base {
     unsigned int sentinel = (unsigned int)N;
     int bits = 1;
     while ((1ull << bits) <= (unsigned long long)N) ++bits;
-    unsigned int* key = alloc<unsigned int>(nrec, true);
-    unsigned int* key_s = alloc<unsigned int>(nrec, true);
-    int* rec = alloc<int>(nrec, true);
-    int* rec_s = alloc<int>(nrec, true);
-    double* w = alloc<double>(nrec, true);
-    uint8_t* dist = alloc<uint8_t>(nrec, true);
+    // every buffer but the run counts is written whole by its producer
+    unsigned int* key = alloc<unsigned int>(nrec, true, false);
+    unsigned int* key_s = alloc<unsigned int>(nrec, true, false);
+    int* rec = alloc<int>(nrec, true, false);
+    int* rec_s = alloc<int>(nrec, true, false);
+    double* w = alloc<double>(nrec, true, false);
+    uint8_t* dist = alloc<uint8_t>(nrec, true, false);
     Particles P = parts();
-    size_t npts = (size_t)nbody * nbnd * (dobnd ? 2 : 1);
-    k_spread_records<<<(unsigned)((npts + 255) / 256), 256, 0, stream>>>(P, mask, nx, my, mz,
+    k_spread_records<<<(unsigned)((nrec + 255) / 256), 256, 0, stream>>>(P, mask, nx, my, mz,
                                                                           cfg.include_frmeanfluid, cfg.dorpr, dobnd,
                                                                           sentinel, key, rec, w, dist);
     HIPCHK(hipGetLastError());
     size_t tmp_bytes = 0;
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key, key_s, rec, rec_s, (int)nrec, 0, bits,
                                               stream));
-    void* tmp = alloc<unsigned char>(tmp_bytes + 16, true);
+    void* tmp = alloc<unsigned char>(tmp_bytes + 16, true, false);
     HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key, key_s, rec, rec_s, (int)nrec, 0, bits, stream));
     // run-length encode the sorted keys
-    unsigned int* ukeys = alloc<unsigned int>(nrec + 1, true);
-    int* counts = alloc<int>(nrec + 1, true);
-    int* nruns = alloc<int>(1, true);
+    unsigned int* ukeys = alloc<unsigned int>(nrec + 1, true, false);
+    int* counts = alloc<int>(nrec + 1, true);  // zero past the runs: the scan and maximum read it whole
+    int* nruns = alloc<int>(1, true, false);
     size_t tb2 = 0;
     HIPCHK(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb2, key_s, ukeys, counts, nruns, (int)nrec, stream));
-    void* tmp2 = alloc<unsigned char>(tb2 + 16, true);
+    void* tmp2 = alloc<unsigned char>(tb2 + 16, true, false);
     HIPCHK(hipcub::DeviceRunLengthEncode::Encode(tmp2, tb2, key_s, ukeys, counts, nruns, (int)nrec, stream));
-    int hruns = 0;
-    HIPCHK(hipMemcpyAsync(&hruns, nruns, sizeof(int), hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
-    // the sentinel key (records that deposit nowhere) sorts last
-    unsigned int lastkey = 0;
-    if (hruns > 0) HIPCHK(hipMemcpy(&lastkey, ukeys + hruns - 1, sizeof(unsigned int), hipMemcpyDeviceToHost));
-    ntarget = hruns - (hruns > 0 && lastkey >= sentinel ? 1 : 0);
+    // target count, scan and maximum on the device, one wait for the sizes
+    int* meta = alloc<int>(4, true, false);
+    k_runs_fix<<<1, 64, 0, stream>>>(ukeys, counts, nruns, sentinel, meta);
+    HIPCHK(hipGetLastError());
     // target nodes are the run keys; record offsets their exclusive prefix sum
     tnode = (int*)ukeys;
-    toff = alloc<int>(nrec + 2, true);
-    int* dmax = alloc<int>(1, true);
+    toff = alloc<int>(nrec + 2, true, false);
+    int* dmax = alloc<int>(1, true, false);
     size_t tb3 = 0, tb4 = 0;
-    // temporary storage sized for the record bound, so that it is reused too
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, counts, toff, (int)nrec + 1, stream));
     HIPCHK(hipcub::DeviceReduce::Max(nullptr, tb4, counts, dmax, (int)nrec, stream));
-    void* tmp3 = alloc<unsigned char>(std::max(tb3, tb4) + 16, true);
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(tmp3, tb3, counts, toff, ntarget + 1, stream));
-    int acc = 0, maxcnt = 0;
-    if (ntarget > 0) {
-      HIPCHK(hipcub::DeviceReduce::Max(tmp3, tb4, counts, dmax, ntarget, stream));
-      HIPCHK(hipMemcpyAsync(&maxcnt, dmax, sizeof(int), hipMemcpyDeviceToHost, stream));
-      HIPCHK(hipMemcpyAsync(&acc, toff + ntarget, sizeof(int), hipMemcpyDeviceToHost, stream));
-    }
+    void* tmp3 = alloc<unsigned char>(std::max(tb3, tb4) + 16, true, false);
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(tmp3, tb3, counts, toff, (int)nrec + 1, stream));
+    HIPCHK(hipcub::DeviceReduce::Max(tmp3, tb4, counts, dmax, (int)nrec, stream));
+    k_runs_total<<<1, 64, 0, stream>>>(toff, meta);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(meta + 2, dmax, sizeof(int), hipMemcpyDeviceToDevice, stream));
+    int hm[3] = {0, 0, 0};
+    HIPCHK(hipMemcpyAsync(hm, meta, sizeof hm, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
+    ntarget = hm[0];
+    const int acc = hm[1], maxcnt = ntarget > 0 ? hm[2] : 0;
     if (maxcnt > SPREAD_CAP)
       throw PibFail(PIB_ERR_UNSUPPORTED, "more reversal records on one node than a workgroup holds");
     // workgroup ranges: the targets whose first record falls in one bucket of
@@ -1304,19 +1328,18 @@ struct Impl : Base {
     // bed of the same size finds them in the reuse pool)
     const int B = SPREAD_CAP - maxcnt + 1;
     nspread_blk = ntarget > 0 ? (acc + B - 1) / B : 0;
-    spread_bt = alloc<int>(nrec + 2, true);
+    spread_bt = alloc<int>(nrec + 2, true, false);
     k_bucket_starts<<<(unsigned)((ntarget + 1 + 255) / 256), 256, 0, stream>>>(toff, ntarget, B, nspread_blk,
                                                                                  spread_bt);
     HIPCHK(hipGetLastError());
-    rpt = alloc<int>(nrec + 1, true);
-    rw = alloc<double>(nrec + 1, true);
-    rdist = alloc<uint8_t>(nrec + 1, true);
+    rpt = alloc<int>(nrec + 1, true, false);
+    rw = alloc<double>(nrec + 1, true, false);
+    rdist = alloc<uint8_t>(nrec + 1, true, false);
     if (acc > 0) {
       k_gather_records<<<(acc + 255) / 256, 256, 0, stream>>>(acc, rec_s, w, dist, nbody, nbnd, dobnd, rpt, rw,
                                                                 rdist);
       HIPCHK(hipGetLastError());
     }
-    HIPCHK(hipStreamSynchronize(stream));
   }
 
   // ---------------- per-step routines ----------------
@@ -1635,8 +1658,11 @@ struct Impl : Base {
       if (nbody > 0) bcset_forcing(rks);
       return;
     }
+    join_branches();
     std::swap(nl, onl);
-    forked = use_fork && !ktime && nbody > 0 && part == 0;
+    // part 1 leaves the branch running for the caller's DEM and particle
+    // update; nonlinear_finish joins it before the forcing
+    forked = use_fork && !ktime && nbody > 0;
     if (forked) {
       HIPCHK(hipEventRecord(evfork, stream));
       HIPCHK(hipStreamWaitEvent(stream2, evfork, 0));
@@ -2111,20 +2137,22 @@ struct Impl : Base {
     }
     HIPCHK(hipMemcpy(dst, &h, sizeof h, hipMemcpyHostToDevice));
   }
+  // copies ordered on the main stream only (a form_nl branch in flight on
+  // the second stream writes neither)
   void get_scalars(pib_scalars* s) {
     DevState h;
+    HIPCHK(hipMemcpyAsync(&h, dst, sizeof h, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
-    HIPCHK(hipMemcpy(&h, dst, sizeof h, hipMemcpyDeviceToHost));
     *s = h.s;
   }
   void get_forces(double* f, double* pr, double* vi, double* tq) {
-    HIPCHK(hipStreamSynchronize(stream));
     size_t n = (size_t)3 * nbody * sizeof(double);
     if (nbody == 0) return;
-    if (f) HIPCHK(hipMemcpy(f, force, n, hipMemcpyDeviceToHost));
-    if (pr) HIPCHK(hipMemcpy(pr, pres, n, hipMemcpyDeviceToHost));
-    if (vi) HIPCHK(hipMemcpy(vi, visc, n, hipMemcpyDeviceToHost));
-    if (tq) HIPCHK(hipMemcpy(tq, torq, n, hipMemcpyDeviceToHost));
+    if (f) HIPCHK(hipMemcpyAsync(f, force, n, hipMemcpyDeviceToHost, stream));
+    if (pr) HIPCHK(hipMemcpyAsync(pr, pres, n, hipMemcpyDeviceToHost, stream));
+    if (vi) HIPCHK(hipMemcpyAsync(vi, visc, n, hipMemcpyDeviceToHost, stream));
+    if (tq) HIPCHK(hipMemcpyAsync(tq, torq, n, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
   }
 
   // ---------------- drop-in transforms ----------------
@@ -2624,19 +2652,31 @@ int guard(Ctx* c, F f) {
   }
 }
 
+// Entry points that may run while the form_nl branch of pib_nonlinear_begin
+// is still in flight on the second stream (it writes nl and qin only): the
+// drag results, the device state, the particle update and the finish, which
+// joins the branch itself before the forcing.
 template <typename F>
-int dispatch(void* ctx, F f) {
+int dispatch_keep_fork(void* ctx, F f) {
   Ctx* c = (Ctx*)ctx;
   return guard(c, [&] {
     if (c->prec == PIB_FP64) f(*static_cast<Impl<double>*>(c->impl));
     else f(*static_cast<Impl<float>*>(c->impl));
   });
 }
+// Every other entry point first makes the main stream wait for that branch.
+template <typename F>
+int dispatch(void* ctx, F f) {
+  return dispatch_keep_fork(ctx, [&](auto& I) {
+    I.join_branches();
+    f(I);
+  });
+}
 
 int finish_status(void* ctx, int st) {
   if (st != PIB_OK) return st;
   int dev = 0;
-  int r = dispatch(ctx, [&](auto& I) { dev = I.check_status(); });
+  int r = dispatch_keep_fork(ctx, [&](auto& I) { dev = I.check_status(); });
   if (r != PIB_OK) return r;
   if (dev) {
     Ctx* c = (Ctx*)ctx;
@@ -2723,7 +2763,8 @@ int pib_set_points(void* ctx, int nbnd, const double* bndpts, const double* cd) 
 int pib_set_particles(void* ctx, int nbody, const double* xc, const double* radbdy, const double* velbdy,
                       const double* angv, long long* count_solid, int* nrpr_active) {
   if (nbody < 0 || (nbody > 0 && (!xc || !radbdy))) return PIB_ERR_ARG;
-  return dispatch(ctx, [&](auto& I) { I.set_particles(nbody, xc, radbdy, velbdy, angv, count_solid, nrpr_active); });
+  return dispatch_keep_fork(ctx,
+                            [&](auto& I) { I.set_particles(nbody, xc, radbdy, velbdy, angv, count_solid, nrpr_active); });
 }
 int pib_set_field(void* ctx, int field, const double* host) {
   if (!host) return PIB_ERR_ARG;
@@ -2761,10 +2802,10 @@ int pib_set_scalars(void* ctx, const pib_scalars* s) {
 }
 int pib_get_scalars(void* ctx, pib_scalars* s) {
   if (!s) return PIB_ERR_ARG;
-  return dispatch(ctx, [&](auto& I) { I.get_scalars(s); });
+  return dispatch_keep_fork(ctx, [&](auto& I) { I.get_scalars(s); });
 }
 int pib_get_forces(void* ctx, double* force, double* pres, double* visc, double* torq) {
-  return dispatch(ctx, [&](auto& I) { I.get_forces(force, pres, visc, torq); });
+  return dispatch_keep_fork(ctx, [&](auto& I) { I.get_forces(force, pres, visc, torq); });
 }
 int pib_compute_new_timestep(void* ctx, int rks) {
   int st = dispatch(ctx, [&](auto& I) {
@@ -2792,7 +2833,7 @@ int pib_nonlinear_begin(void* ctx, int rks) {
   return finish_status(ctx, st);
 }
 int pib_nonlinear_finish(void* ctx, int rks) {
-  int st = dispatch(ctx, [&](auto& I) {
+  int st = dispatch_keep_fork(ctx, [&](auto& I) {
     if (I.nl_pending != rks) throw PibFail(PIB_ERR_STATE, "pib_nonlinear_begin with this rks first");
     if (!I.have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
     I.nonlinear_timed(rks, false, 2);
