@@ -32,6 +32,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 PASSES = {
     "c2r_ubcp_pbcp": 8, "form_nl": 6, "form_nl_yz": 6, "form_nl_x": 9, "c2r_nlbc": 6, "c2r_onlbc": 6,
     "c2r_visc_omega": 14, "c2r_pgrad": 4, "r2c_force": 6, "velstep": 17, "c2r_inner": 12, "c2r_omega": 8,
+    "nl_products": 9, "c2r_inner_nl": 18,
 }
 
 
@@ -69,7 +70,8 @@ def run_gpu(setup, steps, warmup, ksteps, autotune=True):
     if autotune:
         g.step(2)
         p, ms = g.autotune(10)
-        picks = {"fft": p[0], "velstep": p[1], "streams": p[2], "ms_per_step": ms}
+        picks = {"fft": p[0], "velstep": p[1], "streams": p[2], "nl": g.tuning(_abi.PIB_TUNE_NL),
+                 "ms_per_step": ms}
     g.step(warmup)
     lib.pib_sync(g.ctx)
     t0 = time.perf_counter()
@@ -104,7 +106,8 @@ KERNEL_SYMBOLS = {
     "form_nl": "k_form_nl<float", "velstep": "k_velstep_tiled<float", "forcing_points": "k_forcing_points<float",
     "spread": "k_spread<float", "c2r_visc_omega": "k_c2r_pair<float, 4,", "c2r_ubcp_pbcp": "k_c2r_pair<float, 0,",
     "c2r_pgrad": "k_c2r<float, 6,", "c2r_nlbc": "k_c2r<float, 2,", "r2c_force": "k_r2c_force<float",
-    "c2r_inner": "k_c2r<float, 7,", "c2r_omega": "k_c2r<float, 5,",
+    "c2r_inner": "k_c2r<float, 7,", "c2r_omega": "k_c2r<float, 5,", "nl_products": "k_nl_products<float",
+    "c2r_inner_nl": "k_c2r<float, 8,",
     "drag_points": "k_drag_points<float",
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_fluid_step_128cube_fp32.md")
@@ -205,7 +208,8 @@ def run_batched(nreal, steps, warmup, tune_steps=10):
         g.step(2)
         p, ms = g.autotune(tune_steps)
         if ms > 0:
-            picks[f"{st.nx}^3"] = {"fft": p[0], "velstep": p[1], "streams": p[2], "ms_per_step": ms}
+            picks[f"{st.nx}^3"] = {"fft": p[0], "velstep": p[1], "streams": p[2],
+                                   "nl": g.tuning(_abi.PIB_TUNE_NL), "ms_per_step": ms}
     B = solver.Batch(gs)
     B.step(warmup)
     alone = 0.0

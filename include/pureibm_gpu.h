@@ -358,6 +358,11 @@ int pib_kernel_times(void* ctx, int maxn, char* names, double* total_ms, int* la
  *     branch up to the pressure gradient, joined before the forcing (default
  *     for FP64); 0 = one stream (default for FP32-bulk).  Steps under
  *     pib_set_timing use one stream.  Results are the same either way.
+ *   PIB_TUNE_NL: 1 = form_nl as eight separate transforms a plane (the five
+ *     y/z-flux products and the three x-flux derivatives), nl formed once
+ *     where the qin transform reads it (default, except for the FP32-bulk
+ *     128^2 plane); 0 = three transforms a (plane, component) updating nl in
+ *     place.  The same sums in the same order either way.
  *   PIB_TUNE_GRAPH: 1 = pib_step replays whole steps from HIP graphs (default;
  *     the last step of each call and steps under pib_set_timing run eagerly);
  *     0 = every step is launched kernel by kernel.
@@ -366,16 +371,21 @@ int pib_kernel_times(void* ctx, int maxn, char* names, double* total_ms, int* la
 #define PIB_TUNE_GRAPH 1
 #define PIB_TUNE_FFT 2
 #define PIB_TUNE_STREAMS 4
+#define PIB_TUNE_NL 5
 int pib_set_tuning(void* ctx, int key, int value);
+/* The current value of a PIB_TUNE_* key (after pib_autotune: its picks,
+ * PIB_TUNE_NL included).  PIB_ERR_ARG for an unknown key. */
+int pib_get_tuning(void* ctx, int key, int* value);
 
 /* Autotuner (no reference counterpart).  Times PIB_TUNE_FFT (0, and 1 where
  * the plane has a compiled shape), then PIB_TUNE_VELSTEP (0, 1, 2), then
- * PIB_TUNE_STREAMS (0, 1), each over nsteps graph-replayed steps of this
+ * PIB_TUNE_STREAMS (0, 1), then PIB_TUNE_NL (0, 1), each over nsteps graph-replayed steps of this
  * context's bed started from the same saved state (pib_checkpoint_push/pop),
  * and keeps the fastest.  The state after the call is the state before it.
  * The picks are cached for the process per (nx, my, mz, precision): a later
  * call for a context of the same shape applies them without timing unless
- * force != 0.  picks (may be NULL) receives (fft, velstep, streams) and ms
+ * force != 0.  picks (may be NULL) receives (fft, velstep, streams; the
+ * PIB_TUNE_NL pick through pib_get_tuning) and ms
  * (may be NULL) the GPU ms per step of the picked form (0 when cached).
  * Needs pib_set_particles first; waits on the stream. */
 int pib_autotune(void* ctx, int nsteps, int force, int* picks, double* ms);

@@ -28,6 +28,7 @@ PIB_TRIDIAG_NOALLOC_MAX = 1 << 20  # rows of the no-alloc tridiagonal workspace
 PIB_TUNE_VELSTEP = 0
 PIB_TUNE_GRAPH = 1
 PIB_TUNE_STREAMS = 4
+PIB_TUNE_NL = 5
 PIB_TUNE_FFT = 2
 
 PIB_FP64 = 0
@@ -159,6 +160,7 @@ _GPU_ONLY_SIGS = {
     "set_timing": ([_VP, _I], _I),
     "kernel_times": ([_VP, _I, ctypes.c_char_p, _PD, _PI, _PI], _I),
     "set_tuning": ([_VP, _I, _I], _I),
+    "get_tuning": ([_VP, _I, _PI], _I),
     "autotune": ([_VP, _I, _I, _PI, _PD], _I),
     "batch_create": ([ctypes.POINTER(_VP), _I, ctypes.POINTER(_VP)], _I),
     "batch_step": ([_VP, _I], _I),

@@ -756,6 +756,7 @@ struct Impl : Base {
   C2<T> *u, *p, *nl, *onl, *ff;
   T *ubcp, *pbcp, *nlbc, *onlbc, *diffn, *omega, *fr, *ppr, *qin;
   double* sconst;  // cf, umean and the coefficients of the current stage
+  C2<T>* nlp = nullptr;  // the eight spectra of form_nl by separate transforms
   double* keep_dev = nullptr;  // per-particle force sums parked across a particle update
   size_t keep_n = 0;
   uint8_t* mask;
@@ -899,6 +900,10 @@ struct Impl : Base {
     size_t E = (size_t)my2 * mz;
     nt_plane = E <= 256 ? 64 : (E <= 2500 ? 256 : (E <= 5000 ? 512 : 1024));
     sh = shape_of(hy, mz);
+    // form_nl by separate transforms, except on the FP32 128^2 plane, whose
+    // in-place form runs its three transforms at four waves per SIMD and
+    // measured faster (the autotuner times both)
+    nl_sep = !(sizeof(T) == 4 && sh == 1);
     // rows of the generic transform, or of the compiled shape's (padded) layout
     lds_plane = (size_t)mz * std::max(G.ld, shape_ld(sh)) * sizeof(C2<T>);
     size_t red = (size_t)std::max(nt_plane, shape_nt(sh)) * sizeof(double);
@@ -952,6 +957,7 @@ struct Impl : Base {
     onlbc = alloc<T>(nsr + GATHER_PAD);
     diffn = alloc<T>(nsr + GATHER_PAD);
     qin = alloc<T>(nsr + GATHER_PAD);
+    nlp = alloc<C2<T>>((size_t)8 * nx * E);
     sconst = alloc<double>(8);
     omega = alloc<T>(nsr + GATHER_PAD);
     fr = alloc<T>(nsr);
@@ -1024,7 +1030,7 @@ struct Impl : Base {
   C2<T>* pscr = nullptr;
   size_t pslot = 0, lds_hbm = 0;
   void set_plane_hbm(bool on) {
-    if (on && !pscr) pscr = alloc<C2<T>>((size_t)9 * nx * pslot);
+    if (on && !pscr) pscr = alloc<C2<T>>((size_t)14 * nx * pslot);
     plane_hbm = on;
   }
   size_t plane_lds() const { return plane_hbm ? lds_hbm : lds_plane; }
@@ -1042,6 +1048,8 @@ struct Impl : Base {
     A.wz = wz;
     A.w2 = w2;
     A.G = G;
+    // six transforms a launch on the main stream, eight (the separate
+    // transforms of form_nl) on the second
     A.pscr = plane_hbm ? pscr + (stream == stream_main ? 0 : (size_t)6 * nx * pslot) : nullptr;
     A.pslot = pslot;
     if (plane_hbm) A.G.tab_off = 0;
@@ -1051,6 +1059,7 @@ struct Impl : Base {
     A.nl = nl;
     A.onl = onl;
     A.ff = ff;
+    A.nlp = nlp;
     A.ubcp = ubcp;
     A.pbcp = pbcp;
     A.nlbc = nlbc;
@@ -1098,8 +1107,8 @@ struct Impl : Base {
   int plane_nt(int s) const { return s ? shape_nt(s) : nt_plane; }
   template <int KIND, int SH>
   void launch_c2r(int rks, int ncomp) {
-    static const char* names[] = {"c2r_ubcp", "c2r_pbcp",  "c2r_nlbc",  "c2r_onlbc",
-                                  "c2r_visc", "c2r_omega", "c2r_pgrad", "c2r_inner"};
+    static const char* names[] = {"c2r_ubcp",  "c2r_pbcp",  "c2r_nlbc",  "c2r_onlbc", "c2r_visc",
+                                  "c2r_omega", "c2r_pgrad", "c2r_inner", "c2r_inner_nl"};
     auto kern = k_c2r<T, KIND, SH>;
     set_lds(kern, plane_lds());
     kb(names[KIND]);
@@ -1353,6 +1362,45 @@ struct Impl : Base {
   // form_nl keeps NIT spectral accumulators per lane across its three
   // transforms; at most 512 lanes per workgroup leave each lane 256 VGPRs
   int nt_nl() const { return nt_plane < 512 ? nt_plane : 512; }
+  // form_nl (and, with particles, the qin transform) of a stage: eight
+  // separate transforms a plane, nl written by the qin transform or
+  // k_nl_combine (nl_sep); else the in-place three-stage form
+  bool nl_sep = true;  // form_nl by separate transforms (PIB_TUNE_NL)
+  void nl_terms(int rks, bool inner) {
+    if (nl_sep) {
+      switch (cur_sh()) {
+        case 1: nl_products<1>(rks); break;
+        case 2: nl_products<2>(rks); break;
+        case 3: nl_products<3>(rks); break;
+        case 4: nl_products<4>(rks); break;
+        case 5: nl_products<5>(rks); break;
+        default: nl_products<0>(rks);
+      }
+      if (inner) {
+        c2r<C2R_INNER_NL>(rks, 3);
+      } else {
+        kb("nl_combine");
+        const size_t n = (size_t)3 * nx * my2 * mz;
+        k_nl_combine<T><<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(args(rks));
+        HIPCHK(hipGetLastError());
+        ke();
+      }
+      return;
+    }
+    form_nl(rks);
+    if (inner) c2r<C2R_INNER>(rks, 3);
+  }
+  template <int SH>
+  void nl_products(int rks) {
+    const bool up = cfg.discr_scheme == PIB_SCHEME_UPWIND, mm = cfg.limiter == PIB_LIMITER_MINMOD;
+    auto kern = !up ? k_nl_products<T, false, false, SH> : (!mm ? k_nl_products<T, true, false, SH>
+                                                                : k_nl_products<T, true, true, SH>);
+    set_lds(kern, plane_lds());
+    kb("nl_products");
+    kern<<<dim3(nx, 8), plane_nt(SH), plane_lds(), stream>>>(args(rks));
+    HIPCHK(hipGetLastError());
+    ke();
+  }
   void form_nl(int rks) {
     bool up0 = cfg.discr_scheme == PIB_SCHEME_UPWIND, mm0 = cfg.limiter == PIB_LIMITER_MINMOD;
     switch (cur_sh()) {
@@ -1668,8 +1716,7 @@ struct Impl : Base {
       HIPCHK(hipStreamWaitEvent(stream2, evfork, 0));
       std::swap(stream, stream2);
     }
-    form_nl(rks);
-    if (nbody > 0) c2r<C2R_INNER>(rks, 3);
+    nl_terms(rks, nbody > 0);
     if (forked) {
       std::swap(stream, stream2);
       HIPCHK(hipEventRecord(evjoin, stream2));
@@ -2355,10 +2402,9 @@ struct Impl : Base {
     if (!rec) {
       compute_new_timestep(1);
       std::swap(nl, onl);
-      form_nl(1);
+      nl_terms(1, nbody > 0);
       nlbc_valid = true;
       if (nbody > 0) {
-        c2r<C2R_INNER>(1, 3);
         if (drag) c2r<C2R_OMEGA>(1, 3);
         c2r<C2R_PGRAD>(1, 3);
       }
@@ -2534,15 +2580,18 @@ struct Impl : Base {
     back(done);
   }
   // ---------------- autotuner ----------------
-  // a step form: (PIB_TUNE_FFT, PIB_TUNE_VELSTEP, PIB_TUNE_STREAMS)
-  std::array<int, 3> form() const { return {plane_hbm && !hbm_needed ? 2 : (fft_compiled ? 1 : 0), vel_variant, use_fork ? 1 : 0}; }
-  void set_form(const std::array<int, 3>& f) {
+  // a step form: (PIB_TUNE_FFT, PIB_TUNE_VELSTEP, PIB_TUNE_STREAMS, PIB_TUNE_NL)
+  std::array<int, 4> form() const {
+    return {plane_hbm && !hbm_needed ? 2 : (fft_compiled ? 1 : 0), vel_variant, use_fork ? 1 : 0, nl_sep ? 1 : 0};
+  }
+  void set_form(const std::array<int, 4>& f) {
     HIPCHK(hipStreamSynchronize(stream));
     drop_graphs();
     fft_compiled = f[0] == 1;
     set_plane_hbm(f[0] == 2 || hbm_needed);
     vel_variant = f[1];
     use_fork = f[2] == 1;
+    nl_sep = f[3] == 1;
   }
   // GPU ms per graph-replayed step of the current form from the current
   // state, which is restored afterwards
@@ -2568,7 +2617,7 @@ struct Impl : Base {
     if (nsteps < 1) throw PibFail(PIB_ERR_ARG, "nsteps < 1");
     if (!use_graph) throw PibFail(PIB_ERR_STATE, "the autotuner times graph-replayed steps (PIB_TUNE_GRAPH = 1)");
     const std::array<int, 4> key{nx, my, mz, (int)sizeof(T)};
-    std::array<int, 3> pick = form();
+    std::array<int, 4> pick = form();
     double best = 0;
     bool cached = false;
     {
@@ -2599,6 +2648,7 @@ struct Impl : Base {
       if (sh != 0 && !hbm_needed) axis(0, {0, 1});
       axis(1, {0, 1, 2});
       if (nbody > 0) axis(2, {0, 1});
+      axis(3, {0, 1});
       std::lock_guard<std::mutex> lk(tune_mutex());
       tune_cache()[key] = pick;
     }
@@ -2611,8 +2661,8 @@ struct Impl : Base {
     static std::mutex m;
     return m;
   }
-  static std::map<std::array<int, 4>, std::array<int, 3>>& tune_cache() {
-    static std::map<std::array<int, 4>, std::array<int, 3>> c;
+  static std::map<std::array<int, 4>, std::array<int, 4>>& tune_cache() {
+    static std::map<std::array<int, 4>, std::array<int, 4>> c;
     return c;
   }
   void checkpoint_pop() {
@@ -2909,6 +2959,14 @@ int pib_set_tuning(void* ctx, int key, int value) {
       I.use_fork = value == 1;
     });
   }
+  if (key == PIB_TUNE_NL) {
+    if (value < 0 || value > 1) return PIB_ERR_ARG;
+    return dispatch(ctx, [&](auto& I) {
+      HIPCHK(hipStreamSynchronize(I.stream));
+      I.drop_graphs();
+      I.nl_sep = value == 1;
+    });
+  }
   if (key == PIB_TUNE_GRAPH) {
     if (value < 0 || value > 1) return PIB_ERR_ARG;
     return dispatch(ctx, [&](auto& I) {
@@ -2918,6 +2976,20 @@ int pib_set_tuning(void* ctx, int key, int value) {
     });
   }
   return PIB_ERR_ARG;
+}
+int pib_get_tuning(void* ctx, int key, int* value) {
+  if (!value) return PIB_ERR_ARG;
+  if (key != PIB_TUNE_VELSTEP && key != PIB_TUNE_GRAPH && key != PIB_TUNE_FFT && key != PIB_TUNE_STREAMS &&
+      key != PIB_TUNE_NL)
+    return PIB_ERR_ARG;
+  return dispatch(ctx, [&](auto& I) {
+    const auto f = I.form();
+    *value = key == PIB_TUNE_FFT ? f[0]
+             : key == PIB_TUNE_VELSTEP ? f[1]
+             : key == PIB_TUNE_STREAMS ? f[2]
+             : key == PIB_TUNE_NL ? f[3]
+                                  : (I.use_graph ? 1 : 0);
+  });
 }
 int pib_autotune(void* ctx, int nsteps, int force, int* picks, double* ms) {
   return dispatch(ctx, [&](auto& I) { I.autotune(nsteps, force != 0, picks, ms); });

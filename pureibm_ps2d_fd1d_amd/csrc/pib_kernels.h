@@ -44,6 +44,9 @@ struct KArgs {
   C2<T>* nl;
   C2<T>* onl;
   C2<T>* ff;
+  // the six product spectra of the central form_nl, [q][i][kz][ky] with q:
+  // 0 uu, 1 uv, 2 uw, 3 vv, 4 vw, 5 ww
+  C2<T>* nlp;
   T* ubcp;
   T* pbcp;
   T* nlbc;
@@ -371,6 +374,8 @@ struct OpPgrad {
 // qin = c2r(cf u - c3 nl - c4 onl - c12 vis (delta_x^2 u / dx2 - w2 u))
 //       + cf umean
 template <typename T>
+__device__ C2<T> nl_from_products(const KArgs<T>& A, int c, int i, int ky, int kz);
+template <typename T, bool NLP>
 struct OpInner {
   const KArgs<T>& A;
   int c, i;
@@ -389,7 +394,14 @@ struct OpInner {
     C2<T> um = A.u[sidx(A, c, im, ky, kz)], uc = A.u[o], up = A.u[sidx(A, c, ip, ky, kz)];
     C2<T> d2 = (um - cscal((T)2, uc)) + up;
     C2<T> v = cscal((T)1 / A.dx2, d2) - cscal(A.w2[(size_t)kz * A.my2 + ky], uc);
-    return ((cscal(cf, uc) - cscal(c3, A.nl[o])) - cscal(c4, A.onl[o])) - cscal(c12v, v);
+    C2<T> nlv;
+    if constexpr (NLP) {  // nl formed here from the product spectra, and stored
+      nlv = nl_from_products(A, c, i, ky, kz);
+      A.nl[o] = nlv;
+    } else {
+      nlv = A.nl[o];
+    }
+    return ((cscal(cf, uc) - cscal(c3, nlv)) - cscal(c4, A.onl[o])) - cscal(c12v, v);
   }
   __device__ void store2(int, int y, int z, T a, T b) {
     size_t o = ridx(A, c, i, y, z);
@@ -399,7 +411,7 @@ struct OpInner {
 };
 
 // generic c2r launcher body: grid (nplanes, ncomp)
-enum { C2R_UBCP = 0, C2R_PBCP, C2R_NLBC, C2R_ONLBC, C2R_VISC, C2R_OMEGA, C2R_PGRAD, C2R_INNER };
+enum { C2R_UBCP = 0, C2R_PBCP, C2R_NLBC, C2R_ONLBC, C2R_VISC, C2R_OMEGA, C2R_PGRAD, C2R_INNER, C2R_INNER_NL };
 
 template <typename T, int KIND, int SH>
 __device__ __forceinline__ void c2r_plane(const KArgs<T>& A, unsigned char* smem, int c, int i) {
@@ -429,8 +441,8 @@ __device__ __forceinline__ void c2r_plane(const KArgs<T>& A, unsigned char* smem
   } else if constexpr (KIND == C2R_OMEGA) {
     OpOmega<T> op{A, c, i};
     plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op, smem);
-  } else if constexpr (KIND == C2R_INNER) {
-    OpInner<T> op(A, c, i);
+  } else if constexpr (KIND == C2R_INNER || KIND == C2R_INNER_NL) {
+    OpInner<T, KIND == C2R_INNER_NL> op(A, c, i);
     plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op, smem);
   } else {
     OpPgrad<T> op{A, c, i, (T)A.st->s.mpg[c]};
@@ -484,6 +496,7 @@ __global__ void __launch_bounds__(1024, (PlaneOcc<T, SH>::WPE)) k_r2c_force(KArg
   OpForce<T> op{A, c, i, (T)A.st->s.frmean[c]};
   plane_r2c_sh<SH, 0>(buf, A.G, A.tb, op, smem);
 }
+
 
 // ===========================================================================
 // form_nl (nl_allflow.F90:71-327) + the sign flip of nonlinear (nl.F90:141-153)
@@ -667,6 +680,67 @@ __global__ void __launch_bounds__((NlOcc<T, SH>::NT), (NlOcc<T, SH>::WPE)) k_for
     OpNlStage<2, O, T> o2{op};
     plane_r2c2<HY, MZ, NT>(buf, A.G, A.tb, o2);
   }
+}
+
+// ===========================================================================
+// form_nl (nl_allflow.F90:71-327) by separate transforms: per plane the five
+// distinct products of the y- and z-fluxes, uv, uw, vv, vw, ww (as the
+// reference forms them, u_c u_s), and the x-flux derivative D_c of each
+// component (central or MUSCL/Rusanov, as the in-place form has it) are
+// transformed once each -- eight transforms a plane where the in-place form
+// runs nine -- and
+//   nl_c(i) = -((wy F(u_c v) + wz F(u_c w)) + F(D_c))
+// is formed where it is consumed (the qin transform, or k_nl_combine), in
+// the in-place form's order of terms, so nl is written once.
+// ===========================================================================
+template <typename T, bool UPWIND, bool MINMOD>
+struct OpProd {
+  const KArgs<T>& A;
+  int a, b, i, q;
+  NlFlux<T, UPWIND, MINMOD> fx;
+  __device__ T val(int y, int z) const {
+    if (q >= 5) return fx.dx_flux(q - 5, i, y, z);
+    return A.ubcp[ridx(A, a, i, y, z)] * A.ubcp[ridx(A, b, i, y, z)];
+  }
+  __device__ void load2(int y, int z, T& va, T& vb) {
+    va = val(y, z);
+    vb = val(y + 1, z);
+  }
+  __device__ void store(int, int ky, int kz, C2<T> v) { A.nlp[sidx(A, q, i, ky, kz)] = v; }
+};
+// grid (planes, 8 spectra): q = 0 uv, 1 uw, 2 vv, 3 vw, 4 ww, 5 + c: D_c
+template <typename T, bool UPWIND, bool MINMOD, int SH>
+__global__ void __launch_bounds__(1024, (PlaneOcc<T, SH>::WPE)) k_nl_products(KArgs<T> A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  C2<T>* buf = SH == 0 ? plane_slot(A) : nullptr;
+  const int i = plane_of(A);
+  const int q = blockIdx.y;
+  const int a = q < 2 ? 0 : (q < 4 ? 1 : 2);
+  const int b = q < 2 ? q + 1 : (q < 4 ? q - 1 : 2);
+  OpProd<T, UPWIND, MINMOD> op{A, a, b, i, q, {A}};
+  plane_r2c_sh<SH, 0>(buf, A.G, A.tb, op, smem);
+}
+template <typename T>
+__device__ C2<T> nl_from_products(const KArgs<T>& A, int c, int i, int ky, int kz) {
+  // F(u_c v) and F(u_c w) of component c: uv, uw / vv, vw / vw, ww
+  const int qy = c == 0 ? 0 : (c == 1 ? 2 : 3), qz = c == 0 ? 1 : (c == 1 ? 3 : 4);
+  const C2<T> py = A.nlp[sidx(A, qy, i, ky, kz)];
+  const C2<T> pz = A.nlp[sidx(A, qz, i, ky, kz)];
+  const C2<T> px = A.nlp[sidx(A, 5 + c, i, ky, kz)];
+  const C2<T> a = (cik(A.wy[ky], py) + cik(A.wz[kz], pz)) + px;
+  return mk(-a.x, -a.y);
+}
+// nl from the products where no qin transform follows (a bed without
+// particles): one lane per spectral entry (c, i, ky, kz)
+template <typename T>
+__global__ void __launch_bounds__(256) k_nl_combine(KArgs<T> A) {
+  const size_t E = (size_t)A.my2 * A.mz;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 3 * (size_t)A.nx * E) return;
+  const size_t ci = e / E, q = e - ci * E;
+  const int c = (int)(ci / A.nx), i = (int)(ci - (size_t)c * A.nx);
+  const int kz = (int)(q / A.my2), ky = (int)(q - (size_t)kz * A.my2);
+  A.nl[e] = nl_from_products(A, c, i, ky, kz);
 }
 
 // ===========================================================================

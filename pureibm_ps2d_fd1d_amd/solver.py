@@ -464,6 +464,12 @@ class Solver:
         self._call("autotune", self.ctx, nsteps, 1 if force else 0, picks, ctypes.byref(ms))
         return tuple(picks), ms.value
 
+    def tuning(self, key):
+        """pib_get_tuning: the current value of a PIB_TUNE_* key."""
+        v = ctypes.c_int()
+        self._call("get_tuning", self.ctx, key, ctypes.byref(v))
+        return v.value
+
     def forces(self):
         nb = self.setup.nbody
         out = [np.zeros((nb, 3), order="F") for _ in range(4)]

@@ -123,9 +123,13 @@ def test_autotune_keeps_the_state_and_caches_the_pick(tmp_path, orc):
         assert np.array_equal(g.field(f), a), f
     sc1 = g.scalars()
     assert sc1.t == sc0.t and sc1.dt == sc0.dt and sc1.norm_drag == sc0.norm_drag
+    nl = g.tuning(_abi.PIB_TUNE_NL)
+    assert nl in (0, 1)
+    assert (g.tuning(_abi.PIB_TUNE_FFT), g.tuning(_abi.PIB_TUNE_VELSTEP), g.tuning(_abi.PIB_TUNE_STREAMS)) == picks
     g2 = solver.Solver(st)
+    assert g2.lib.pib_set_tuning(g2.ctx, _abi.PIB_TUNE_NL, 1 - nl) == 0 and g2.tuning(_abi.PIB_TUNE_NL) == 1 - nl
     picks2, ms2 = g2.autotune(nsteps=4)
-    assert picks2 == picks and ms2 == 0.0
+    assert picks2 == picks and ms2 == 0.0 and g2.tuning(_abi.PIB_TUNE_NL) == nl
     g2.close()
     g.step(3)
     o.step(3)
