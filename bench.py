@@ -140,7 +140,8 @@ def collect_traffic():
     what the counter reads for plain device-to-device copies of known size.
     Writes TRAFFIC_FILE.  Runs the GPU only in child processes."""
     work = tempfile.mkdtemp(prefix="pib_pmc_")
-    run_args = ["--only", "fp32", "--steps", "3", "--warmup", "2", "--kernel-steps", "1", "--no-cpu"]
+    run_args = ["--only", "fp32", "--steps", "3", "--warmup", "2", "--kernel-steps", "1", "--no-cpu", "--no-batched",
+                "--no-dem", "--no-adjoint", "--no-scalar", "--no-autotune"]
     copy_args = ["--calib-copy", str(COPY_MIB)]
     fetch = _pmc_pass("FETCH_SIZE", run_args, os.path.join(work, "f"))
     write = _pmc_pass("WRITE_SIZE", run_args, os.path.join(work, "w"))
