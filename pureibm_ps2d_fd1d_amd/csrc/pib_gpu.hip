@@ -781,6 +781,7 @@ struct Impl : Base {
   double *sflux = nullptr, *sphisurf = nullptr;
   int ssurf_n = -1;
   double* sspt = nullptr;  // per spreading-record point: the scalar forcing's two evaluations
+  C2<double>* snlz = nullptr;  // the w phi term of nlphif between k_scal_phi and k_scal_xflux
   size_t sspt_n = 0;
   std::vector<double> host_rad;
   double sc_gamma = 0, heat_ratio = 1, heat_ratio_new = 1, heat_ratio_old = 1;
@@ -1962,7 +1963,7 @@ struct Impl : Base {
       auto kern = k_scal_phi<T, SH>;
       set_lds(kern, lds);
       kb("scal_phi");
-      kern<<<nx + 1, nt, lds, stream>>>(A, S);
+      kern<<<dim3(nx + 1, 2), nt, lds, stream>>>(A, S);
       HIPCHK(hipGetLastError());
       ke();
     }
@@ -2065,6 +2066,8 @@ struct Impl : Base {
       sspt_n = nspt;
     }
     S.spt = sspt;
+    if (!snlz) snlz = alloc<C2<double>>((size_t)nx * my2 * mz);
+    S.nlz = snlz;
     S.dx = cfg.dx;
     S.upwind = cfg.discr_scheme == PIB_SCHEME_UPWIND;
     S.dobnd = cfg.dobnd;

@@ -26,6 +26,7 @@ template <typename T>
 struct ScalStep {
   C2<double>* phif;
   C2<double>* nlphif;
+  C2<double>* nlz;  // the wz ff2rc(w phi) term of nlphif, planes 1..nx, until k_scal_xflux adds it
   const C2<double>* onlphif;
   C2<double>* ffphi;
   T *ph, *nlp, *onp, *df, *g, *fr, *fdd;  // physical, planes 0..nx+1; g holds 3 arrays
@@ -76,24 +77,30 @@ __global__ void k_fill_t(T* p, size_t n) {
 // the phase means (planes 1..nx) and of compute_phim_heat_ratio (plane
 // nx+1), then nlphif = wy ff2rc(v phi) + wz ff2rc(w phi) (planes 1..nx; the
 // plane nx+1 value is set by k_scal_xflux).
-template <typename T>
+// LAST: plane nx+1 (its sums weigh phi with u for heat_ratio_new); a
+// functor of its own, so that the other planes' functor carries no branch
+// for it (with the branch, the compiler kept the kernel arguments in
+// scratch memory)
+template <typename T, bool LAST>
 struct OpPhiC2R {
   const KArgs<T>& A;
   const ScalStep<T>& S;
   int p;
   double s0, s1, s2;
+  bool own;  // the workgroup that writes fdd and the plane sums
   __device__ C2<T> load(int ky, int kz) {
     C2<double> v = S.phif[sss(A, p, ky, kz)];
     return mk((T)v.x, (T)v.y);
   }
   __device__ void one(int y, int z, T v) {
     const size_t o = sps(A, p + 1, y, z);
-    const int pu = p == A.nx ? 0 : p;
-    const T u1 = A.ubcp[ridx(A, 0, pu, y, z)];
     S.ph[o] = v;
+    if (!own) return;
+    const int pu = LAST ? 0 : p;
+    const T u1 = A.ubcp[ridx(A, 0, pu, y, z)];
     S.fdd[o] = S.upwind ? (T)0 : u1 * v;
     const bool fl = A.mask[((size_t)pu * A.mz + z) * A.my + y] != 0;
-    if (p < A.nx) {
+    if (!LAST) {
       if (fl) s0 += (double)v;
       else s1 += (double)v;
     } else if (fl) {
@@ -118,28 +125,46 @@ struct OpPhiProd {  // ST 0: ff2rc(v phi) times wy (assigned), 1: ff2rc(w phi) t
     b = val(y + 1, z);
   }
   __device__ void store(int, int ky, int kz, C2<T> v) {
-    C2<double>* o = S.nlphif + sss(A, p, ky, kz);
-    if (ST == 0) *o = cikd((double)A.wy[ky], dbl(v));
-    else *o = *o + cikd((double)A.wz[kz], dbl(v));
+    if (ST == 0) S.nlphif[sss(A, p, ky, kz)] = cikd((double)A.wy[ky], dbl(v));
+    else S.nlz[sss(A, p, ky, kz)] = cikd((double)A.wz[kz], dbl(v));
   }
 };
-template <typename T, int SH>
-__global__ void __launch_bounds__(1024) k_scal_phi(KArgs<T> A, ScalStep<T> S) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int p = blockIdx.x;
-  C2<T>* buf = plane_slot(A);
-  OpPhiC2R<T> op{A, S, p, 0.0, 0.0, 0.0};
+// ff2cr of plane p of phif into ph, with fdd and the plane sums when own
+template <typename T, int SH, bool LAST>
+__device__ void phi_plane(const KArgs<T>& A, const ScalStep<T>& S, int p, bool own, C2<T>* buf,
+                          unsigned char* smem) {
+  OpPhiC2R<T, LAST> op{A, S, p, 0.0, 0.0, 0.0, own};
   plane_c2r_sh<SH, 0>(buf, A.G, A.tb, op, smem);
-  double sm[3] = {op.s0, op.s1, op.s2};
-  block_sums_w<3>(sm, (double*)smem);
-  if (threadIdx.x == 0)
-    for (int q = 0; q < 3; ++q) S.part[(size_t)p * 4 + q] = sm[q];
-  if (p == A.nx) return;
+  if (own) {
+    double sm[3] = {op.s0, op.s1, op.s2};
+    block_sums_w<3>(sm, (double*)smem);
+    if (threadIdx.x == 0)
+      for (int q = 0; q < 3; ++q) S.part[(size_t)p * 4 + q] = sm[q];
+  }
+}
+template <typename T, int SH>
+__global__ void __launch_bounds__(1024, (PlaneOcc<T, SH>::WPE)) k_scal_phi(KArgs<T> A, ScalStep<T> S) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  // grid (nx + 1, 2): workgroup row 0 transforms phi, keeps fdd and the plane
+  // sums and forms the v phi term; row 1 transforms phi again (the same
+  // values into ph) and forms the w phi term, so a plane's three transforms
+  // run as two in each of two workgroups
+  const int p = blockIdx.x, half = blockIdx.y;
+  if (half == 1 && p == A.nx) return;
+  C2<T>* buf = plane_slot(A);
+  if (p == A.nx) {
+    phi_plane<T, SH, true>(A, S, p, true, buf, smem);
+    return;
+  }
+  phi_plane<T, SH, false>(A, S, p, half == 0, buf, smem);
   __syncthreads();  // this workgroup's phi stores are visible to it
-  OpPhiProd<T, 0> o0{A, S, p};
-  plane_r2c_sh<SH, 0>(buf, A.G, A.tb, o0, smem);
-  OpPhiProd<T, 1> o1{A, S, p};
-  plane_r2c_sh<SH, 0>(buf, A.G, A.tb, o1, smem);
+  if (half == 0) {
+    OpPhiProd<T, 0> o0{A, S, p};
+    plane_r2c_sh<SH, 0>(buf, A.G, A.tb, o0, smem);
+  } else {
+    OpPhiProd<T, 1> o1{A, S, p};
+    plane_r2c_sh<SH, 0>(buf, A.G, A.tb, o1, smem);
+  }
 }
 
 // heat_ratio_new = 1/phim(nx+1), phim = sum u phi / (um count), um = sum u /
@@ -232,7 +257,7 @@ struct OpPhiFlux {
   }
   __device__ void store(int, int ky, int kz, C2<T> v) {
     C2<double>* o = S.nlphif + sss(A, p, ky, kz);
-    C2<double> s = *o + dbl(v);
+    C2<double> s = (*o + S.nlz[sss(A, p, ky, kz)]) + dbl(v);
     C2<double> r = mk(-s.x, -s.y);
     *o = r;
     if (p == 0) S.nlphif[sss(A, A.nx, ky, kz)] = mk(r.x / h, r.y / h);
