@@ -382,7 +382,8 @@ int pib_get_tuning(void* ctx, int key, int* value);
  * PIB_TUNE_STREAMS (0, 1), then PIB_TUNE_NL (0, 1), each over nsteps graph-replayed steps of this
  * context's bed started from the same saved state (pib_checkpoint_push/pop),
  * and keeps the fastest.  The state after the call is the state before it.
- * The picks are cached for the process per (nx, my, mz, precision): a later
+ * The picks are cached for the process per (nx, my, mz, precision, x-flux
+ * scheme and limiter): a later
  * call for a context of the same shape applies them without timing unless
  * force != 0.  picks (may be NULL) receives (fft, velstep, streams; the
  * PIB_TUNE_NL pick through pib_get_tuning) and ms

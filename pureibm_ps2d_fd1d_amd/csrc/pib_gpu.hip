@@ -2619,7 +2619,8 @@ struct Impl : Base {
     if (!have_particles) throw PibFail(PIB_ERR_STATE, "pib_set_particles first");
     if (nsteps < 1) throw PibFail(PIB_ERR_ARG, "nsteps < 1");
     if (!use_graph) throw PibFail(PIB_ERR_STATE, "the autotuner times graph-replayed steps (PIB_TUNE_GRAPH = 1)");
-    const std::array<int, 4> key{nx, my, mz, (int)sizeof(T)};
+    // the x-flux scheme too: the forms' costs differ between central and upwind
+    const std::array<int, 6> key{nx, my, mz, (int)sizeof(T), (int)cfg.discr_scheme, (int)cfg.limiter};
     std::array<int, 4> pick = form();
     double best = 0;
     bool cached = false;
@@ -2664,8 +2665,8 @@ struct Impl : Base {
     static std::mutex m;
     return m;
   }
-  static std::map<std::array<int, 4>, std::array<int, 4>>& tune_cache() {
-    static std::map<std::array<int, 4>, std::array<int, 4>> c;
+  static std::map<std::array<int, 6>, std::array<int, 4>>& tune_cache() {
+    static std::map<std::array<int, 6>, std::array<int, 4>> c;
     return c;
   }
   void checkpoint_pop() {
