@@ -139,6 +139,12 @@ __global__ void k_bucket_starts(const int* toff, int ntarget, int B, int nblk, i
   for (int k = bp + 1; k <= b; ++k) bt[k] = t;
 }
 
+// The first record of each spreading range: rs[k] = toff[bt[k]], k = 0..nblk.
+__global__ void k_range_starts(const int* toff, const int* bt, int nblk, int* rs) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k <= nblk) rs[k] = toff[bt[k]];
+}
+
 __global__ void k_count_active(const uint8_t* if_rev, int nbnd, int* nrpr_active) {
   __shared__ double sh[256];
   int m = blockIdx.x;
@@ -795,6 +801,7 @@ struct Impl : Base {
   double *drag_part = nullptr, *frc_part = nullptr, *frc_pts = nullptr;
   int ntarget = 0, nspread_blk = 0;
   int* spread_bt = nullptr;  // target range of each spreading workgroup
+  int* spread_rs = nullptr;  // its first record (toff of its first target)
   int *tnode = nullptr, *toff = nullptr, *rpt = nullptr;
   double* rw = nullptr;
   uint8_t* rdist = nullptr;
@@ -1342,6 +1349,10 @@ struct Impl : Base {
     k_bucket_starts<<<(unsigned)((ntarget + 1 + 255) / 256), 256, 0, stream>>>(toff, ntarget, B, nspread_blk,
                                                                                  spread_bt);
     HIPCHK(hipGetLastError());
+    spread_rs = alloc<int>(nrec + 2, true, false);
+    k_range_starts<<<(unsigned)((nspread_blk + 1 + 255) / 256), 256, 0, stream>>>(toff, spread_bt, nspread_blk,
+                                                                                    spread_rs);
+    HIPCHK(hipGetLastError());
     rpt = alloc<int>(nrec + 1, true, false);
     rw = alloc<double>(nrec + 1, true, false);
     rdist = alloc<uint8_t>(nrec + 1, true, false);
@@ -1503,7 +1514,8 @@ struct Impl : Base {
       ke();
       if (ntarget > 0) {
         kb("spread");
-        k_spread<T><<<(nspread_blk + 7) / 8 * 8, 256, 0, stream>>>(args(rks), spread_bt, nspread_blk, tnode, toff,
+        k_spread<T><<<(nspread_blk + 7) / 8 * 8, 256, 0, stream>>>(args(rks), spread_bt, spread_rs, nspread_blk,
+                                                                  tnode, toff,
                                                                   rpt, rw, rdist,
                                                                frc_pts);
         HIPCHK(hipGetLastError());
@@ -2008,7 +2020,8 @@ struct Impl : Base {
       }
       if (nspread_blk > 0) {
         kb("scal_spread");
-        k_scal_spread<T><<<(nspread_blk + 7) / 8 * 8, 256, 0, stream>>>(A, S, spread_bt, nspread_blk, tnode, toff,
+        k_scal_spread<T><<<(nspread_blk + 7) / 8 * 8, 256, 0, stream>>>(A, S, spread_bt, spread_rs, nspread_blk,
+                                                                         tnode, toff,
                                                                          rpt, rw, rdist);
         HIPCHK(hipGetLastError());
         ke();

@@ -1085,14 +1085,14 @@ __global__ void __launch_bounds__(256) k_bnd_points(KArgs<T> A, Particles P, dou
 // (a rounding-level difference).
 #define SPREAD_CAP 1024
 template <typename T>
-__global__ void __launch_bounds__(256) k_spread(KArgs<T> A, const int* bt, int nrange, const int* tnode,
-                                                const int* toff, const int* rpt, const double* rw,
+__global__ void __launch_bounds__(256) k_spread(KArgs<T> A, const int* bt, const int* rs, int nrange,
+                                                const int* tnode, const int* toff, const int* rpt, const double* rw,
                                                 const uint8_t* rdist, const double* pts) {
   __shared__ double cv[3 * SPREAD_CAP];
   const int w = xcd_linear(blockIdx.x, gridDim.x);  // neighbouring ranges share an L2
   if (w >= nrange) return;
   const int t0 = bt[w], t1 = bt[w + 1];
-  const int r0 = toff[t0], r1 = toff[t1];
+  const int r0 = rs[w], r1 = rs[w + 1];  // toff[t0], toff[t1]
   const double dx3 = A.st->dx3;
   constexpr int CH = SPREAD_CAP / 256;
   int rp[CH];
@@ -1100,6 +1100,17 @@ __global__ void __launch_bounds__(256) k_spread(KArgs<T> A, const int* bt, int n
   for (int k = 0; k < CH; ++k) {
     int r = r0 + (int)threadIdx.x + k * 256;
     rp[k] = r < r1 ? rpt[r] : -1;
+  }
+  // this lane's nodes (a range holds at most SPREAD_CAP records, so at most
+  // SPREAD_CAP nodes): their record bounds, loaded before the barrier
+  int tb[CH], te[CH], tn[CH];
+#pragma unroll
+  for (int k = 0; k < CH; ++k) {
+    const int t = t0 + (int)threadIdx.x + k * 256;
+    const bool ok = t < t1;
+    tb[k] = ok ? toff[t] : 0;
+    te[k] = ok ? toff[t + 1] : 0;
+    tn[k] = ok ? tnode[t] : 0;
   }
 #pragma unroll
   for (int k = 0; k < CH; ++k) {
@@ -1114,14 +1125,15 @@ __global__ void __launch_bounds__(256) k_spread(KArgs<T> A, const int* bt, int n
   }
   __syncthreads();
   const size_t cs = (size_t)A.nx * A.mz * A.my;
-  for (int t = t0 + (int)threadIdx.x; t < t1; t += 256) {
+#pragma unroll
+  for (int k = 0; k < CH; ++k) {
+    if (t0 + (int)threadIdx.x + k * 256 >= t1) break;
     double acc[3] = {0, 0, 0};
-    for (int r = toff[t]; r < toff[t + 1]; ++r) {
+    for (int r = tb[k]; r < te[k]; ++r) {
       int q = r - r0;
       for (int n = 0; n < 3; ++n) acc[n] = acc[n] + cv[n * SPREAD_CAP + q];
     }
-    int node = tnode[t];
-    for (int n = 0; n < 3; ++n) A.fr[n * cs + node] = (T)acc[n];
+    for (int n = 0; n < 3; ++n) A.fr[n * cs + tn[k]] = (T)acc[n];
   }
 }
 

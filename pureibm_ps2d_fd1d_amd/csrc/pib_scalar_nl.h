@@ -623,14 +623,14 @@ __global__ void __launch_bounds__(256) k_scal_rev(KArgs<T> A, Particles P, ScalS
 // records taking the pair's second evaluation.  No atomics: the sums are
 // deterministic.  rdist bit 0: redistributed share; bit 1: upper x node.
 template <typename T>
-__global__ void __launch_bounds__(256) k_scal_spread(KArgs<T> A, ScalStep<T> S, const int* bt, int nrange,
-                                                     const int* tnode, const int* toff, const int* rpt,
+__global__ void __launch_bounds__(256) k_scal_spread(KArgs<T> A, ScalStep<T> S, const int* bt, const int* rs,
+                                                     int nrange, const int* tnode, const int* toff, const int* rpt,
                                                      const double* rw, const uint8_t* rdist) {
   __shared__ double cv[SPREAD_CAP];
   const int w = xcd_linear(blockIdx.x, gridDim.x);
   if (w >= nrange) return;
   const int t0 = bt[w], t1 = bt[w + 1];
-  const int r0 = toff[t0], r1 = toff[t1];
+  const int r0 = rs[w], r1 = rs[w + 1];  // toff[t0], toff[t1]
   const double dx3 = A.st->dx3;
   // each record's value staged in LDS (a range holds at most SPREAD_CAP
   // records), then one lane per node sums its records in record order
