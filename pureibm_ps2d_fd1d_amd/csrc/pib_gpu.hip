@@ -788,6 +788,7 @@ struct Impl : Base {
   int ssurf_n = -1;
   double* sspt = nullptr;  // per spreading-record point: the scalar forcing's two evaluations
   C2<double>* snlz = nullptr;  // the w phi term of nlphif between k_scal_phi and k_scal_xflux
+  T* sph2 = nullptr;           // the second copy of phi that k_scal_phi's w phi workgroups transform
   size_t sspt_n = 0;
   std::vector<double> host_rad;
   double sc_gamma = 0, heat_ratio = 1, heat_ratio_new = 1, heat_ratio_old = 1;
@@ -2079,8 +2080,12 @@ struct Impl : Base {
       sspt_n = nspt;
     }
     S.spt = sspt;
-    if (!snlz) snlz = alloc<C2<double>>((size_t)nx * my2 * mz);
+    if (!snlz) {
+      snlz = alloc<C2<double>>((size_t)nx * my2 * mz);
+      sph2 = alloc<T>(np);
+    }
     S.nlz = snlz;
+    S.ph2 = sph2;
     S.dx = cfg.dx;
     S.upwind = cfg.discr_scheme == PIB_SCHEME_UPWIND;
     S.dobnd = cfg.dobnd;

@@ -27,6 +27,7 @@ struct ScalStep {
   C2<double>* phif;
   C2<double>* nlphif;
   C2<double>* nlz;  // the wz ff2rc(w phi) term of nlphif, planes 1..nx, until k_scal_xflux adds it
+  T* ph2;           // phi of planes 1..nx as the second k_scal_phi workgroup of a plane transforms it
   const C2<double>* onlphif;
   C2<double>* ffphi;
   T *ph, *nlp, *onp, *df, *g, *fr, *fdd;  // physical, planes 0..nx+1; g holds 3 arrays
@@ -94,8 +95,11 @@ struct OpPhiC2R {
   }
   __device__ void one(int y, int z, T v) {
     const size_t o = sps(A, p + 1, y, z);
+    if (!own) {  // the w phi workgroup: its own copy
+      S.ph2[o] = v;
+      return;
+    }
     S.ph[o] = v;
-    if (!own) return;
     const int pu = LAST ? 0 : p;
     const T u1 = A.ubcp[ridx(A, 0, pu, y, z)];
     S.fdd[o] = S.upwind ? (T)0 : u1 * v;
@@ -115,11 +119,13 @@ struct OpPhiC2R {
   }
 };
 template <typename T, int ST>
-struct OpPhiProd {  // ST 0: ff2rc(v phi) times wy (assigned), 1: ff2rc(w phi) times wz (added)
+struct OpPhiProd {  // ST 0: ff2rc(v phi) times wy into nlphif, 1: ff2rc(w phi) times wz into nlz
   const KArgs<T>& A;
   const ScalStep<T>& S;
   int p;
-  __device__ T val(int y, int z) const { return A.ubcp[ridx(A, ST + 1, p, y, z)] * S.ph[sps(A, p + 1, y, z)]; }
+  __device__ T val(int y, int z) const {
+    return A.ubcp[ridx(A, ST + 1, p, y, z)] * (ST == 0 ? S.ph : S.ph2)[sps(A, p + 1, y, z)];
+  }
   __device__ void load2(int y, int z, T& a, T& b) {
     a = val(y, z);
     b = val(y + 1, z);
@@ -145,9 +151,9 @@ __device__ void phi_plane(const KArgs<T>& A, const ScalStep<T>& S, int p, bool o
 template <typename T, int SH>
 __global__ void __launch_bounds__(1024, (PlaneOcc<T, SH>::WPE)) k_scal_phi(KArgs<T> A, ScalStep<T> S) {
   extern __shared__ __align__(16) unsigned char smem[];
-  // grid (nx + 1, 2): workgroup row 0 transforms phi, keeps fdd and the plane
-  // sums and forms the v phi term; row 1 transforms phi again (the same
-  // values into ph) and forms the w phi term, so a plane's three transforms
+  // grid (nx + 1, 2): workgroup row 0 transforms phi into ph, keeps fdd and
+  // the plane sums and forms the v phi term; row 1 transforms phi again into
+  // its own copy ph2 and forms the w phi term, so a plane's three transforms
   // run as two in each of two workgroups
   const int p = blockIdx.x, half = blockIdx.y;
   if (half == 1 && p == A.nx) return;
