@@ -178,7 +178,11 @@ int pib_nonlinear(void* ctx, int rks);
  * transforms, calc_visc and compute_mpg_at_n (drag, mpg, frame_accln, the
  * bcset diagnostics); _finish runs calc_pgrad and the reversal forcing.  In
  * between, a caller moves the bed: pib_get_forces, pib_dem_march, then
- * pib_set_particles with the new centres and velocities. */
+ * pib_set_particles with the new centres and velocities.  With
+ * PIB_TUNE_STREAMS = 1 the form_nl branch (form_nl and the inner-point
+ * transform) may still be running on a second stream when _begin returns;
+ * pib_get_forces, pib_get_scalars and pib_set_particles leave it running,
+ * every other entry point (pib_sync included) waits for it first. */
 int pib_nonlinear_begin(void* ctx, int rks);
 int pib_nonlinear_finish(void* ctx, int rks);
 /* velstep (steptheflow.F90:52-160): pressure, ustep, divcorr per (ky,kz) mode,

@@ -2953,7 +2953,10 @@ int pib_step_async(void* ctx, int nsteps) {
   if (nsteps < 0) return PIB_ERR_ARG;
   return dispatch(ctx, [&](auto& I) { I.step_async(nsteps); });
 }
-int pib_sync(void* ctx) { return finish_status(ctx, PIB_OK); }
+int pib_sync(void* ctx) {
+  // a form_nl branch left in flight by pib_nonlinear_begin is drained too
+  return finish_status(ctx, dispatch(ctx, [&](auto& I) { (void)I; }));
+}
 int pib_step(void* ctx, int nsteps) { return finish_status(ctx, pib_step_async(ctx, nsteps)); }
 int pib_set_tuning(void* ctx, int key, int value) {
   if (key == PIB_TUNE_VELSTEP) {
