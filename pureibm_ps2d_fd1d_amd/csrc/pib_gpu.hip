@@ -509,23 +509,7 @@ __global__ void __launch_bounds__(DRAG_T) k_scalar_drag(DevState* stg, const dou
 // from the (body, chunk) partials of k_forcing_points
 __global__ void __launch_bounds__(SCAL_T) k_reduce_frmean(DevState* st, const double* fpart, int nrec) {
   __shared__ double sh[6 * 16];
-  double a[6] = {0, 0, 0, 0, 0, 0};
-  for (int p = threadIdx.x; p < nrec; p += SCAL_T)
-    for (int q = 0; q < 6; ++q) a[q] += fpart[(size_t)p * 6 + q];
-  block_sums_w<6>(a, sh);
-  if (threadIdx.x == 0) {
-    double ntot = (double)((long long)st->nx * st->my * st->mz);
-    double cfl = st->s.count_fluid;
-    double r[6];
-    for (int n = 0; n < 3; ++n) {
-      r[n] = a[n] / ntot;
-      r[3 + n] = a[3 + n] / cfl;
-    }
-    for (int n = 0; n < 3; ++n) {
-      st->s.frmean[n] = r[n];
-      st->s.frmeanfluid[n] = r[3 + n];
-    }
-  }
+  reduce_frmean_body(st, fpart, nrec, sh);
 }
 
 // velstep tail: check_divergence max, umean update (steptheflow.F90:93-99)
@@ -1509,16 +1493,18 @@ struct Impl : Base {
         HIPCHK(hipGetLastError());
         ke();
       }
-      kb("reduce_frmean");
-      k_reduce_frmean<<<1, SCAL_T, 0, stream>>>(dst, frc_part, G * nbody * (cfg.dobnd ? 2 : 1));
-      HIPCHK(hipGetLastError());
-      ke();
+      const int nfrec = G * nbody * (cfg.dobnd ? 2 : 1);
       if (ntarget > 0) {
+        // the spreading ranges, then one workgroup for the frmean reduction
         kb("spread");
-        k_spread<T><<<(nspread_blk + 7) / 8 * 8, 256, 0, stream>>>(args(rks), spread_bt, spread_rs, nspread_blk,
-                                                                  tnode, toff,
-                                                                  rpt, rw, rdist,
-                                                               frc_pts);
+        k_spread<T><<<(nspread_blk + 1 + 7) / 8 * 8, 256, 0, stream>>>(args(rks), spread_bt, spread_rs,
+                                                                      nspread_blk, tnode, toff, rpt, rw, rdist,
+                                                                      frc_pts, dst, frc_part, nfrec);
+        HIPCHK(hipGetLastError());
+        ke();
+      } else {
+        kb("reduce_frmean");
+        k_reduce_frmean<<<1, SCAL_T, 0, stream>>>(dst, frc_part, nfrec);
         HIPCHK(hipGetLastError());
         ke();
       }

@@ -1084,13 +1084,42 @@ __global__ void __launch_bounds__(256) k_bnd_points(KArgs<T> A, Particles P, dou
 // two shares to fr one after the other; here they are added together first
 // (a rounding-level difference).
 #define SPREAD_CAP 1024
+// frmean and frmeanfluid (boundary_condition.F90:678-701) from the
+// (body, chunk) partials of the forcing: one workgroup, fixed order
+__device__ inline void reduce_frmean_body(DevState* st, const double* fpart, int nrec, double* sh) {
+  double a[6] = {0, 0, 0, 0, 0, 0};
+  for (int p = threadIdx.x; p < nrec; p += blockDim.x)
+    for (int q = 0; q < 6; ++q) a[q] += fpart[(size_t)p * 6 + q];
+  block_sums_w<6>(a, sh);
+  if (threadIdx.x == 0) {
+    double ntot = (double)((long long)st->nx * st->my * st->mz);
+    double cfl = st->s.count_fluid;
+    double r[6];
+    for (int n = 0; n < 3; ++n) {
+      r[n] = a[n] / ntot;
+      r[3 + n] = a[3 + n] / cfl;
+    }
+    for (int n = 0; n < 3; ++n) {
+      st->s.frmean[n] = r[n];
+      st->s.frmeanfluid[n] = r[3 + n];
+    }
+  }
+}
+// The spreading, and (work index nrange) the frmean reduction, which reads
+// only the forcing's partials and writes only frmean/frmeanfluid, so it
+// runs beside the spreading workgroups instead of as a launch of its own.
 template <typename T>
 __global__ void __launch_bounds__(256) k_spread(KArgs<T> A, const int* bt, const int* rs, int nrange,
                                                 const int* tnode, const int* toff, const int* rpt, const double* rw,
-                                                const uint8_t* rdist, const double* pts) {
+                                                const uint8_t* rdist, const double* pts, DevState* stw,
+                                                const double* fpart, int nfrec) {
   __shared__ double cv[3 * SPREAD_CAP];
   const int w = xcd_linear(blockIdx.x, gridDim.x);  // neighbouring ranges share an L2
-  if (w >= nrange) return;
+  if (w == nrange) {
+    reduce_frmean_body(stw, fpart, nfrec, cv);
+    return;
+  }
+  if (w > nrange) return;
   const int t0 = bt[w], t1 = bt[w + 1];
   const int r0 = rs[w], r1 = rs[w + 1];  // toff[t0], toff[t1]
   const double dx3 = A.st->dx3;
