@@ -48,7 +48,7 @@ def _compare(g, o, tag):
 
 
 def test_c1_200_steps_against_the_oracle(tmp_path, orc):
-    """C1 as the survey sets it for parity: ADAPTIVE = F, RAMP_FRAC_STEPS = 0."""
+    """The C1 workload: a simple cubic bed of one sphere, ADAPTIVE = F, RAMP_FRAC_STEPS = 0."""
     print("C1 64^3 one sphere, 200 steps", flush=True)
     st = _setup(tmp_path, "C1", adaptive=False, ramp_frac_steps=0)
     o = solver.Solver(st, lib=orc.load(), prefix="orc_")
