@@ -163,6 +163,19 @@ int pib_set_field(void* ctx, int field, const double* host);
 int pib_get_field(void* ctx, int field, double* host);
 int pib_set_scalars(void* ctx, const pib_scalars* s);
 int pib_get_scalars(void* ctx, pib_scalars* s);
+/* The drag-error history of bcset (boundary_condition.F90:828-833, the
+ * reference's ferror_array and ferror_hist, which its time loop compares
+ * with TOL_FERROR, dtibm_main_scal_flo.F90:471-476).  The last stage of each
+ * step enters that stage's ferror; the array comes back newest first, and
+ * ferror_hist = SUM(ferror_array(1:n))/n in that order.  n is NERR_STEPS:
+ * PIB_NERR_DEFAULT entries of 1 at pib_create (initialize_flo.F90:695-696);
+ * pib_set_ferror_history sets n (1..PIB_NERR_MAX) and the entries (NULL: ones;
+ * a restart passes the saved ferror_array, restart_funcs.F90:270-274).  get: n must
+ * equal the current length; either output may be NULL. */
+#define PIB_NERR_DEFAULT 100
+#define PIB_NERR_MAX 4096
+int pib_get_ferror_history(void* ctx, int n, double* ferror_array, double* ferror_hist);
+int pib_set_ferror_history(void* ctx, int n, const double* ferror_array);
 /* Per-particle force(nbody,3), pres, visc, torq (boundary_condition.F90:430-435,
  * 1613-1634); any pointer may be NULL. */
 int pib_get_forces(void* ctx, double* force, double* pres, double* visc, double* torq);

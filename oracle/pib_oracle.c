@@ -13,8 +13,8 @@
  * -ffp-contract=off so that every operation rounds as written.
  *
  * Pinned by tests/test_oracle.py and tests/test_oracle_pins.py (closed forms,
- * library routines, brute force).  Parity unpinned: the ferror history of
- * bcset, a convergence diagnostic that no routine of the step reads.
+ * library routines, brute force, and bcset's ferror history against the
+ * norm_drag sequence it is formed from).
  *
  * The only product-independent choices are (a) the FFT, a recursive
  * mixed-radix DIT DFT with directly evaluated twiddles standing in for FFTW
@@ -56,6 +56,10 @@ typedef struct orc {
   double dx, dx2, dx3, vis;
   double coef[3][4];
   int itrmax;
+  /* bcset's drag-error history (boundary_condition.F90:828-833) */
+  int nerr_steps;
+  double ferror_hist;
+  double ferror_array[PIB_NERR_MAX];
   double *wy, *wz, *w2; /* wy, wz hold the imaginary parts; w2(my2,mz) */
   cpx *u, *nl, *onl, *ff; /* (nx+1, my2, mz, 3) */
   /* scalar transport, one species: phif, nlphif, onlphif, ffphi (nx+1, my2, mz) */
@@ -454,6 +458,10 @@ int orc_create(const pib_config* cfg, void** ctx) {
     o->coef[0][0] = 0.5; o->coef[0][1] = 0.5; o->coef[0][2] = 3.0 / 2.0; o->coef[0][3] = -0.5;
     o->itrmax = 1;
   }
+  /* initialize_flo.F90:695-696 */
+  o->nerr_steps = PIB_NERR_DEFAULT;
+  for (int k = 0; k < PIB_NERR_MAX; ++k) o->ferror_array[k] = 1.0;
+  o->ferror_hist = 1.0;
   /* wavenumbers, initialize_flo.F90:735-763 (dy = dz = dx) */
   int my = o->my, mz = o->mz, my2 = o->my2;
   double twopi = 2.0 * PI;
@@ -790,6 +798,30 @@ int orc_set_scalars(void* ctx, const pib_scalars* s) {
   double ms = o->s.maxvolfrac, cs = o->s.count_solid, cf = o->s.count_fluid;
   o->s = *s;
   if (o->have_particles) { o->s.maxvolfrac = ms; o->s.count_solid = cs; o->s.count_fluid = cf; }
+  return PIB_OK;
+}
+int orc_get_ferror_history(void* ctx, int n, double* ferror_array, double* ferror_hist) {
+  orc* o = (orc*)ctx;
+  if (!o) return PIB_ERR_ARG;
+  if (n != o->nerr_steps) return fail(o, PIB_ERR_ARG, "n must equal NERR_STEPS");
+  if (ferror_array) memcpy(ferror_array, o->ferror_array, sizeof(double) * (size_t)n);
+  if (ferror_hist) *ferror_hist = o->ferror_hist;
+  return PIB_OK;
+}
+/* the history a run starts from: ones (initialize_flo.F90:695-696) or the
+ * array of a restart file (restart_funcs.F90:270-274), whose mean is the
+ * starting ferror_hist */
+int orc_set_ferror_history(void* ctx, int n, const double* ferror_array) {
+  orc* o = (orc*)ctx;
+  if (!o) return PIB_ERR_ARG;
+  if (n < 1 || n > PIB_NERR_MAX) return fail(o, PIB_ERR_ARG, "NERR_STEPS out of range");
+  o->nerr_steps = n;
+  double sum = 0.0;
+  for (int k = 0; k < n; ++k) {
+    o->ferror_array[k] = ferror_array ? ferror_array[k] : 1.0;
+    sum += o->ferror_array[k];
+  }
+  o->ferror_hist = sum / (double)n;
   return PIB_OK;
 }
 int orc_get_scalars(void* ctx, pib_scalars* s) {
@@ -1616,6 +1648,18 @@ static int bcset_drag(orc* o, int rks) {
   if (s->norm_drag > 0.0) s->ferror = fabs(s->norm_drag - s->fold) / s->norm_drag;
   else s->ferror = 1.0;
   s->fold = s->norm_drag;
+  /* :828-833, once a step: the array shifts by one, the new ferror enters
+   * at its head, and ferror_hist is its mean */
+  if (rks == o->itrmax) {
+    int n = o->nerr_steps;
+    double tmp[PIB_NERR_MAX];
+    for (int k = 0; k < n; ++k) tmp[k] = o->ferror_array[k];
+    for (int k = 1; k < n; ++k) o->ferror_array[k] = tmp[k - 1];
+    o->ferror_array[0] = s->ferror;
+    double sum = 0.0;
+    for (int k = 0; k < n; ++k) sum += o->ferror_array[k];
+    o->ferror_hist = sum / (double)n;
+  }
   if (s->norm_drag > (double)1.0E+06f) return fail(o, PIB_ERR_BLOWUP, "norm_drag > 1e6 (blow-up)");
   return PIB_OK;
 }

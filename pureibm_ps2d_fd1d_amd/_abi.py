@@ -15,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD = os.path.join(HERE, "_build")
 
 PIB_OK = 0
+PIB_NERR_DEFAULT = 100  # drag-error history length at creation (pureibm_gpu.h)
+PIB_NERR_MAX = 4096
 PIB_ERR_ARG = -1
 PIB_ERR_UNSUPPORTED = -2
 PIB_ERR_HIP = -3
@@ -124,6 +126,8 @@ _STEP_SIGS = {
     "get_field": ([_VP, _I, _PD], _I),
     "set_scalars": ([_VP, ctypes.POINTER(PibScalars)], _I),
     "get_scalars": ([_VP, ctypes.POINTER(PibScalars)], _I),
+    "get_ferror_history": ([_VP, _I, _PD, _PD], _I),
+    "set_ferror_history": ([_VP, _I, _PD], _I),
     "get_forces": ([_VP, _PD, _PD, _PD, _PD], _I),
     "compute_new_timestep": ([_VP, _I], _I),
     "nonlinear": ([_VP, _I], _I),

@@ -96,7 +96,7 @@ def write_case(directory, run_name, *, my, lybyd, dbydx, centers=None, input_typ
                discr_scheme="center", limiter="none", rk3=False, ramp_frac_steps=0,
                set_mpg=False, mpg=(0.0, 0.0, 0.0), revernorm=1, dr=1.0, f2=1.6,
                include_frmeanfluid=False, dorpr=True, dobnd=False, imove=0, rhos=1.0, coeff_rest=1.0,
-               iscalon=0, phisurf=0.0, phistream=1.0, pr=0.7):
+               iscalon=0, phisurf=0.0, phistream=1.0, pr=0.7, nerr_steps=100, tol_ferror=None):
     """Write ``<directory>/<run_name>_floparam.in`` (+ sphr_center file).
     imove=1 makes a freely moving bed (soft-sphere collisions, particle
     density rhos, restitution coeff_rest).  iscalon=1 adds the scalar
@@ -116,7 +116,7 @@ def write_case(directory, run_name, *, my, lybyd, dbydx, centers=None, input_typ
         part["VOL_FRAC1"] = float(vol_frac1)
     groups = {
         "FLOANDGRID": {
-            "MY": int(my), "TEND": 1.0, "NERR_STEPS": 100, "RAMP_FRAC_STEPS": int(ramp_frac_steps),
+            "MY": int(my), "TEND": 1.0, "NERR_STEPS": int(nerr_steps), "RAMP_FRAC_STEPS": int(ramp_frac_steps),
             "VIS": float(vis), "RE": float(re), "RHOF": 1.0, "CFL": float(cfl),
             "CFL_MAX": float(cfl_max), "MPG": [float(x) for x in mpg],
             "FLO_ANG": [0.0, 90.0, 90.0],
@@ -132,6 +132,8 @@ def write_case(directory, run_name, *, my, lybyd, dbydx, centers=None, input_typ
         "PSD": {"PSD_TYPE": "discrete", "NPHASES": 1},
         "GCG": {"COEFF_REST": float(coeff_rest)},
     }
+    if tol_ferror is not None:
+        groups["FLOANDGRID"]["TOL_FERROR"] = float(tol_ferror)
     if iscalon:
         groups["FLAGS"]["ISCAL_RESTART"] = 0
         groups["SCAL_PROPT"] = {"NSPMX": 1, "PHISTREAM": float(phistream), "PHISURF": float(phisurf),

@@ -371,7 +371,7 @@ __global__ void __launch_bounds__(DRAG_T) k_scalar_drag(DevState* stg, const dou
                                                         int nb, int rks, const double* __restrict__ rad,
                                                         double* __restrict__ pres, double* __restrict__ visc,
                                                         double* __restrict__ torq, double* __restrict__ force,
-                                                        double* __restrict__ force_chem) {
+                                                        double* __restrict__ force_chem, double* __restrict__ ferr_ring) {
   __shared__ DevState L;
   __shared__ double sh[12 * 16];
   __shared__ double mpg_s[3];
@@ -500,6 +500,13 @@ __global__ void __launch_bounds__(DRAG_T) k_scalar_drag(DevState* stg, const dou
     if (s.norm_drag > 0.0) s.ferror = fabs(s.norm_drag - s.fold) / s.norm_drag;
     else s.ferror = 1.0;
     s.fold = s.norm_drag;
+    // the step's last stage enters its ferror in the history (:828-833): a
+    // ring of nerr slots in place of the reference's shifted array
+    if (rks == st->itrmax) {
+      const int q = st->ferr_pos;
+      ferr_ring[q] = s.ferror;
+      st->ferr_pos = q + 1 == st->nerr ? 0 : q + 1;
+    }
     if (!(s.norm_drag <= (double)1.0E+06f) && st->status == 0) st->status = PIB_ERR_BLOWUP;
   }
   state_store(stg, &L);
@@ -784,6 +791,8 @@ struct Impl : Base {
   int* nrpr = nullptr;
   double *pres = nullptr, *visc = nullptr, *torq = nullptr, *force = nullptr, *force_chem = nullptr;
   double *drag_part = nullptr, *frc_part = nullptr, *frc_pts = nullptr;
+  // ferror history ring, PIB_NERR_MAX slots (nerr of them in use)
+  double* ferr_ring = nullptr;
   int ntarget = 0, nspread_blk = 0;
   int* spread_bt = nullptr;  // target range of each spreading workgroup
   int* spread_rs = nullptr;  // its first record (toff of its first target)
@@ -959,6 +968,7 @@ struct Impl : Base {
     mask = alloc<uint8_t>((size_t)nx * my * mz + GATHER_PAD);
     fill_mask();
     dst = alloc<DevState>(1);
+    ferr_ring = alloc<double>(PIB_NERR_MAX, false, false);
     part_n = std::max<size_t>((size_t)3 * nx * 4, (size_t)(E / 8 + 2));
     part = alloc<double>(part_n);
     vscr = alloc<T>((size_t)6 * nx * E);
@@ -989,7 +999,12 @@ struct Impl : Base {
     h.rhos = c.rhos;
     h.rhof = c.rhof;
     h.s.count_fluid = (double)((long long)nx * my * mz);
+    // initialize_flo.F90:695: ferror_array = 1 over the default NERR_STEPS
+    h.nerr = PIB_NERR_DEFAULT;
+    h.ferr_pos = 0;
+    std::vector<double> ones(PIB_NERR_MAX, 1.0);
     HIPCHK(hipMemcpyAsync(dst, &h, sizeof h, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(ferr_ring, ones.data(), sizeof(double) * PIB_NERR_MAX, hipMemcpyHostToDevice, stream));
     HIPCHK(hipStreamSynchronize(stream));
   }
   ~Impl() override {
@@ -1465,7 +1480,8 @@ struct Impl : Base {
     HIPCHK(hipGetLastError());
     ke();
     kb("scalar_drag");
-    k_scalar_drag<<<1, DRAG_T, 0, stream>>>(dst, drag_part, G, nbody, rks, rad, pres, visc, torq, force, force_chem);
+    k_scalar_drag<<<1, DRAG_T, 0, stream>>>(dst, drag_part, G, nbody, rks, rad, pres, visc, torq, force, force_chem,
+                                            ferr_ring);
     HIPCHK(hipGetLastError());
     ke();
   }
@@ -2191,6 +2207,39 @@ struct Impl : Base {
     }
     HIPCHK(hipMemcpy(dst, &h, sizeof h, hipMemcpyHostToDevice));
   }
+  // ferror_array(1:n) newest first, as the reference shifts it
+  // (boundary_condition.F90:828-833), and ferror_hist = SUM(...)/n summed in
+  // that order
+  void get_ferror_history(int n, double* arr, double* hist) {
+    DevState h;
+    HIPCHK(hipMemcpyAsync(&h, dst, sizeof h, hipMemcpyDeviceToHost, stream));
+    std::vector<double> ring(PIB_NERR_MAX);
+    HIPCHK(hipMemcpyAsync(ring.data(), ferr_ring, sizeof(double) * PIB_NERR_MAX, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    if (n != h.nerr) throw PibFail(PIB_ERR_ARG, "the history holds NERR_STEPS entries: n must equal it");
+    double sum = 0.0;
+    for (int k = 0; k < n; ++k) {
+      const double v = ring[(size_t)((h.ferr_pos - 1 - k + 2 * n) % n)];
+      if (arr) arr[k] = v;
+      sum += v;
+    }
+    if (hist) *hist = sum / n;
+  }
+  // a new history of n entries, newest first (ones when arr is null, as
+  // initialize_flo.F90:695 sets it; a restart passes the saved array)
+  void set_ferror_history(int n, const double* arr) {
+    if (n < 1 || n > PIB_NERR_MAX) throw PibFail(PIB_ERR_ARG, "NERR_STEPS out of range (1..PIB_NERR_MAX)");
+    std::vector<double> ring(PIB_NERR_MAX, 1.0);
+    // slot n-1-k holds entry k, so the next ferror (slot 0) drops the oldest
+    for (int k = 0; k < n; ++k) ring[(size_t)(n - 1 - k)] = arr ? arr[k] : 1.0;
+    ++state_gen;
+    DevState h;
+    HIPCHK(hipMemcpy(&h, dst, sizeof h, hipMemcpyDeviceToHost));
+    h.nerr = n;
+    h.ferr_pos = 0;
+    HIPCHK(hipMemcpy(dst, &h, sizeof h, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ferr_ring, ring.data(), sizeof(double) * PIB_NERR_MAX, hipMemcpyHostToDevice));
+  }
   // copies ordered on the main stream only (a form_nl branch in flight on
   // the second stream writes neither)
   void get_scalars(pib_scalars* s) {
@@ -2861,6 +2910,12 @@ int pib_set_scalars(void* ctx, const pib_scalars* s) {
 int pib_get_scalars(void* ctx, pib_scalars* s) {
   if (!s) return PIB_ERR_ARG;
   return dispatch_keep_fork(ctx, [&](auto& I) { I.get_scalars(s); });
+}
+int pib_get_ferror_history(void* ctx, int n, double* ferror_array, double* ferror_hist) {
+  return dispatch_keep_fork(ctx, [&](auto& I) { I.get_ferror_history(n, ferror_array, ferror_hist); });
+}
+int pib_set_ferror_history(void* ctx, int n, const double* ferror_array) {
+  return dispatch(ctx, [&](auto& I) { I.set_ferror_history(n, ferror_array); });
 }
 int pib_get_forces(void* ctx, double* force, double* pres, double* visc, double* torq) {
   return dispatch_keep_fork(ctx, [&](auto& I) { I.get_forces(force, pres, visc, torq); });

@@ -26,7 +26,10 @@ struct DevState {
   int nx, my, mz, nbody, nbnd, adaptive, soft_start, set_mpg, include_frmeanfluid, itrmax;
   int revernorm;
   int status;  // 0 ok, else a PIB_ERR_* code raised on the device
-  int move_particles, pad_;
+  int move_particles;
+  // the drag-error history (boundary_condition.F90:828-833): its length
+  // nerr_steps and the ring slot the next step's ferror goes to
+  int nerr, ferr_pos, pad_;
   double rhos, rhof;
 };
 

@@ -100,7 +100,7 @@ module pib_host
 
   public :: pib_host_read, pib_host_configure, pib_host_finish, pib_host_counts, &
        pib_host_get_config, pib_host_get_particles, pib_host_get_points, &
-       pib_host_last_error, pib_host_quadgener, pib_host_get_nsteps_hint, &
+       pib_host_last_error, pib_host_quadgener, pib_host_get_nsteps_hint, pib_host_get_tol_ferror, &
        pib_host_set_option, pib_host_time_scales, pib_host_save_restart, pib_host_read_restart, &
        pib_host_get_dem_config, pib_host_get_scalar, pib_host_save_scal_restart, pib_host_read_scal_restart
 contains
@@ -594,12 +594,19 @@ contains
     st = PIB_OK
   end function pib_host_time_scales
 
-  ! Number of steps a run of TEND flow-through times would take at the initial
-  ! dt (initflo prints this as the minimum number of steps).
+  ! NERR_STEPS (floandgrid): the length of the drag-error history whose mean
+  ! the time loop compares with TOL_FERROR (boundary_condition.F90:828-833).
   function pib_host_get_nsteps_hint() bind(C, name="pib_host_get_nsteps_hint") result(ns)
     integer(c_int) :: ns
     ns = nerr_steps
   end function pib_host_get_nsteps_hint
+
+  ! TOL_FERROR (floandgrid): the convergence tolerance of that mean
+  ! (dtibm_main_scal_flo.F90:471-476).
+  function pib_host_get_tol_ferror() bind(C, name="pib_host_get_tol_ferror") result(t)
+    real(c_double) :: t
+    t = tol_ferror
+  end function pib_host_get_tol_ferror
 
   ! ---- restart files (restart_funcs.F90) ------------------------------------
   ! save_part_restart + save_unformatted (:69-128, 188-362) for the flow of a

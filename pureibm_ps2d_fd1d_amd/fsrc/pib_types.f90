@@ -142,6 +142,23 @@ module pib_types
        type(pib_scalars), intent(out) :: s
        integer(c_int) :: st
      end function pib_get_scalars
+     ! bcset's drag-error history (boundary_condition.F90:828-833), newest first
+     function pib_get_ferror_history(ctx, n, ferror_array, ferror_hist) &
+          bind(C, name="pib_get_ferror_history") result(st)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n
+       real(c_double), intent(out) :: ferror_array(*)
+       real(c_double), intent(out) :: ferror_hist
+       integer(c_int) :: st
+     end function pib_get_ferror_history
+     function pib_set_ferror_history(ctx, n, ferror_array) bind(C, name="pib_set_ferror_history") result(st)
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n
+       real(c_double), intent(in) :: ferror_array(*)
+       integer(c_int) :: st
+     end function pib_set_ferror_history
      function pib_get_forces(ctx, force, pres, visc, torq) bind(C, name="pib_get_forces") result(st)
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx

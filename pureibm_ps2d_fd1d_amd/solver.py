@@ -316,8 +316,9 @@ class Solver:
         count c alternating 1, 2 (and <RUN>_scal_<c>.rst for an ISCALON
         deck).  xc/velbdy default to the set-up's centres
         at rest (pass the current ones for a moving bed); ferror_array is the
-        NERR_STEPS history of the caller's convergence check (ones when not
-        kept).  Returns c."""
+        NERR_STEPS drag-error history (by default the context's own, from
+        pib_get_ferror_history, when its length is NERR_STEPS; ones when it is
+        not).  Returns c."""
         st = self.setup
         st._load()
         host = _abi.load_host()
@@ -326,7 +327,12 @@ class Solver:
         s = self.scalars()
         xc = np.asfortranarray(st.xc if xc is None else xc, dtype=np.float64)
         vel = np.asfortranarray(np.zeros((st.nbody, 3)) if velbdy is None else velbdy, dtype=np.float64)
-        ferr = np.ones(nerr) if ferror_array is None else np.ascontiguousarray(ferror_array, dtype=np.float64)
+        if ferror_array is not None:
+            ferr = np.ascontiguousarray(ferror_array, dtype=np.float64)
+        elif nerr <= _abi.PIB_NERR_MAX and self._ferror_len() == nerr:
+            ferr = self.ferror_history(nerr)[0]
+        else:
+            ferr = np.ones(nerr)
         if ferr.size < nerr:
             raise ValueError(f"ferror_array needs NERR_STEPS = {nerr} entries")
         cnt = ctypes.c_int()
@@ -373,6 +379,8 @@ class Solver:
             self.update_particles(xc, vel)
         if st.scalar["iscalon"] == 1 and st.scalar["iscal_restart"] == 1:
             self._load_scal_restart(cnt.value, nerr)
+        if nerr <= _abi.PIB_NERR_MAX:  # the history the next steps extend
+            self.set_ferror_history(nerr, ferr)
         return cnt.value, ferr, xc, vel
 
     def _save_scal_restart(self, count, nerr, nu_error_array=None, nu=(1.0, 0.0)):
@@ -463,6 +471,33 @@ class Solver:
         ms = ctypes.c_double()
         self._call("autotune", self.ctx, nsteps, 1 if force else 0, picks, ctypes.byref(ms))
         return tuple(picks), ms.value
+
+    def ferror_history(self, n=None):
+        """bcset's drag-error history (boundary_condition.F90:828-833):
+        (ferror_array newest first, ferror_hist).  n is NERR_STEPS, by
+        default the length last set (PIB_NERR_DEFAULT from creation)."""
+        if n is None:
+            n = getattr(self, "_nerr", _abi.PIB_NERR_DEFAULT)
+        a = np.empty(n, dtype=np.float64)
+        h = ctypes.c_double()
+        self._call("get_ferror_history", self.ctx, n, a.ctypes.data_as(_abi._PD), ctypes.byref(h))
+        return a, h.value
+
+    def _ferror_len(self):
+        """Length of the context's drag-error history (the n that
+        pib_get_ferror_history accepts), found by asking for NULL outputs."""
+        for n in (getattr(self, "_nerr", None), _abi.PIB_NERR_DEFAULT):
+            if n and self._fn("get_ferror_history")(self.ctx, n, None, None) == PIB_OK:
+                return n
+        return None
+
+    def set_ferror_history(self, n, ferror_array=None):
+        """A history of n entries, newest first (None: ones, as at start-up)."""
+        a = None if ferror_array is None else np.ascontiguousarray(ferror_array, dtype=np.float64)
+        if a is not None and a.size != n:
+            raise ValueError("ferror_array needs n entries")
+        self._call("set_ferror_history", self.ctx, n, None if a is None else a.ctypes.data_as(_abi._PD))
+        self._nerr = n
 
     def tuning(self, key):
         """pib_get_tuning: the current value of a PIB_TUNE_* key."""

@@ -124,3 +124,32 @@ def test_driver_runs_a_scalar_deck(tmp_path, orc):
         assert abs(nuss[n, 4] - o.scalar_get()[2]) <= 1e-11 * abs(o.scalar_get()[2])  # e20.12 text
     print(f"  Nusselt after {nsteps} steps: {nuss[-1, 3]:.10e}, heat ratio {nuss[-1, 4]:.12f}", flush=True)
     o.close()
+
+
+def test_driver_stops_when_the_drag_error_history_converges(tmp_path, orc):
+    """dtibm_main_scal_flo.F90:471-476, 652-658: a fixed bed stops once the
+    mean of the NERR_STEPS drag-error history is below TOL_FERROR, after
+    writing that step's records and RUN_NAME_CONVERGED.  NERR_STEPS = 2 and
+    TOL_FERROR = 0.99 make the stop fall inside the run; the step it falls on
+    is the first one whose oracle history mean is below the tolerance."""
+    print("fortran driver, 24^3 bed, convergence stop", flush=True)
+    assert os.path.exists(DRIVER), "driver not built"
+    kw = dict(kind="random", my=24, dbydx=6, nbody=3, seed=5, re=40.0, nerr_steps=2, tol_ferror=0.99)
+    st = make_setup(tmp_path, "DCV", **kw)
+    o = solver.Solver(st, lib=orc.load(), prefix="orc_")
+    o.set_ferror_history(2)
+    nsteps, stop = 8, None
+    for n in range(nsteps):
+        o.step(1)
+        if o.ferror_history(2)[1] < 0.99:
+            stop = n + 1
+            break
+    assert stop is not None and stop < nsteps
+    run = os.path.join(str(tmp_path), "DCV")
+    r = subprocess.run([DRIVER, run, str(nsteps), "fp64"], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    nd = np.loadtxt(run + "_norm_drag.dat", ndmin=2)
+    assert nd.shape[0] == stop
+    assert os.path.exists(run + "_CONVERGED")
+    assert "FERROR_HIST AND TOL" in r.stdout

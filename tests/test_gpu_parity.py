@@ -316,3 +316,35 @@ def test_c5_size_bed_runs_and_precisions_agree(tmp_path):
     print(f"  norm_drag fp64 {s64.norm_drag:.8e} fp32 {s32.norm_drag:.8e} divmax {s64.divmax:.2e}", flush=True)
     assert s64.divmax < 1e-10
     assert abs(s32.norm_drag - s64.norm_drag) < 1e-4 * abs(s64.norm_drag)
+
+
+@pytest.mark.parametrize("rk3", [False, True])
+def test_ferror_history_through_graph_replayed_steps(tmp_path, orc, rk3):
+    """bcset's drag-error history (boundary_condition.F90:828-833): the GPU
+    keeps it as a ring that the drag scalars write once a step, the oracle
+    shifts the reference's array.  Seven steps in two pib_step calls (steps 3
+    to 6 replay the captured graph) over a 3-entry history that starts from
+    given values, so the history wraps twice; with RK3 only the last stage's
+    ferror enters."""
+    print(f"ferror history, 24^3 random bed, rk3={rk3}", flush=True)
+    st = make_setup(tmp_path, "FH", kind="random", my=24, dbydx=6, nbody=3, seed=4, re=30.0, rk3=rk3)
+    g = solver.Solver(st)
+    o = solver.Solver(st, lib=orc.load(), prefix="orc_")
+    a0, h0 = g.ferror_history()
+    assert (a0 == 1.0).all() and h0 == 1.0  # NERR_STEPS entries of 1 from creation
+    start = [0.5, 0.25, 0.125]
+    for b in (g, o):
+        b.set_ferror_history(3, start)
+    ag, hg = g.ferror_history(3)
+    assert list(ag) == start and hg == o.ferror_history(3)[1]
+    g.step(2)
+    g.step(5)
+    o.step(7)
+    ag, hg = g.ferror_history(3)
+    ao, ho = o.ferror_history(3)
+    print(f"  gpu {ag} {hg:.6e}; oracle {ao} {ho:.6e}", flush=True)
+    assert rel_err(ag, ao) < 1e-9
+    assert abs(hg - ho) <= 1e-9 * abs(ho)
+    assert ag[0] == g.scalars().ferror
+    with pytest.raises(solver.PibError):
+        g.ferror_history(4)

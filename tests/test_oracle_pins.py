@@ -484,3 +484,57 @@ def test_surface_forcing_of_a_sphere_moving_with_the_flow(tmp_path, orc):
     s = o.scalars()
     assert np.max(np.abs(o.field("fr"))) < 1e-12 * abs(s.cf)
     assert max(abs(x) for x in s.frmean) < 1e-14 * abs(s.cf)
+
+
+@pytest.mark.parametrize("rk3", [False, True])
+def test_ferror_and_its_history(tmp_path, orc, rk3):
+    """boundary_condition.F90:802-833.  Every bcset call sets
+    ferror = |norm_drag - fold|/norm_drag and then fold = norm_drag, starting
+    from fold = 0 (initialize_flo.F90:679-680), so the first ferror is exactly
+    1; only the step's last stage enters its ferror at the head of
+    ferror_array, whose older entries shift down by one (the oldest drops out),
+    and ferror_hist is the mean of the array, summed from its head.  Checked
+    stage by stage against the norm_drag sequence, with a history short
+    enough (4) to wrap."""
+    st, o = _sphere(tmp_path, orc, "FE", rk3=rk3)
+    n = 4
+    o.set_ferror_history(n)
+    a, h = o.ferror_history(n)
+    assert (a == 1.0).all() and h == 1.0
+    assert o.scalars().fold == 0.0
+    itrmax = 3 if rk3 else 1
+    nd_prev, last = 0.0, []
+    for step in range(6):
+        for rks in range(1, itrmax + 1):
+            o.compute_new_timestep(rks)
+            o.nonlinear(rks)
+            s = o.scalars()
+            assert s.norm_drag > 0.0
+            assert s.ferror == abs(s.norm_drag - nd_prev) / s.norm_drag
+            assert s.fold == s.norm_drag
+            nd_prev = s.norm_drag
+            o.velstep(rks)
+        last.append(s.ferror)
+        want = (last[::-1] + [1.0] * n)[:n]
+        a, h = o.ferror_history(n)
+        assert list(a) == want, step
+        acc = 0.0
+        for v in want:
+            acc += v
+        assert h == acc / n
+    with pytest.raises(solver.PibError):
+        o.ferror_history(n + 1)
+
+
+def test_ferror_of_a_bed_without_drag_is_one(tmp_path, orc):
+    """boundary_condition.F90:807-811: norm_drag = 0 (a sphere moving with a
+    uniform flow and no pressure carries no force) gives ferror = 1."""
+    st, o = _sphere(tmp_path, orc, "FZ", set_mpg=True, mpg=(0.0, 0.0, 0.0))
+    U = (0.7, -0.2, 0.4)
+    _uniform_flow(o, U)
+    o.update_particles(st.xc, np.array([U]))
+    o.compute_new_timestep(1)
+    o.nonlinear(1)
+    s = o.scalars()
+    assert s.norm_drag == 0.0
+    assert s.ferror == 1.0 and s.fold == 0.0

@@ -87,16 +87,23 @@ def test_continuation_equals_the_uninterrupted_run(orc, tmp_path):
     st = make_setup(tmp_path, "RC", kind="random", my=32, dbydx=8, nbody=5, seed=4)
     a = solver.Solver(st, lib=orc.load(), prefix="orc_")
     a.step(3)
+    saved = a.ferror_history()[0]
     a.save_restart()
     a.step(2)
     b = solver.Solver(st, lib=orc.load(), prefix="orc_")
     cnt, ferr, xc, vel = b.load_restart()
-    assert cnt == 1 and np.all(ferr == 1.0)
+    # the drag-error history travels with the restart (restart_funcs.F90:
+    # 270-274): three entries of the run (the first exactly 1, fold starting
+    # at 0), the rest the initial ones
+    assert cnt == 1 and np.array_equal(ferr, saved)
+    assert ferr[2] == 1.0 and np.all(ferr[3:] == 1.0) and ferr[0] != 1.0
     b.step(2)
     for k in ("u", "p", "nl"):
         assert np.array_equal(b.field(k), a.field(k)), k
     sa, sb = a.scalars(), b.scalars()
     assert sa.t == sb.t and sa.norm_drag == sb.norm_drag
+    ha, hb = a.ferror_history(), b.ferror_history()
+    assert np.array_equal(ha[0], hb[0]) and ha[1] == hb[1]
 
 
 def test_missing_restart_is_an_error(oracle_run):
