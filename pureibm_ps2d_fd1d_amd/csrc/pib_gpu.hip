@@ -2196,8 +2196,10 @@ struct Impl : Base {
   }
   void set_scalars(const pib_scalars* s) {
     ++state_gen;
+    // ordered after the work already queued on the main stream
     DevState h;
-    HIPCHK(hipMemcpy(&h, dst, sizeof h, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(&h, dst, sizeof h, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
     double cs = h.s.count_solid, cf = h.s.count_fluid, mv = h.s.maxvolfrac;
     h.s = *s;
     if (have_particles) {
@@ -2205,7 +2207,8 @@ struct Impl : Base {
       h.s.count_fluid = cf;
       h.s.maxvolfrac = mv;
     }
-    HIPCHK(hipMemcpy(dst, &h, sizeof h, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(dst, &h, sizeof h, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
   }
   // ferror_array(1:n) newest first, as the reference shifts it
   // (boundary_condition.F90:828-833), and ferror_hist = SUM(...)/n summed in
@@ -2233,12 +2236,15 @@ struct Impl : Base {
     // slot n-1-k holds entry k, so the next ferror (slot 0) drops the oldest
     for (int k = 0; k < n; ++k) ring[(size_t)(n - 1 - k)] = arr ? arr[k] : 1.0;
     ++state_gen;
+    // ordered after the work already queued on the main stream
     DevState h;
-    HIPCHK(hipMemcpy(&h, dst, sizeof h, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(&h, dst, sizeof h, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
     h.nerr = n;
     h.ferr_pos = 0;
-    HIPCHK(hipMemcpy(dst, &h, sizeof h, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ferr_ring, ring.data(), sizeof(double) * PIB_NERR_MAX, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(dst, &h, sizeof h, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(ferr_ring, ring.data(), sizeof(double) * PIB_NERR_MAX, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
   }
   // copies ordered on the main stream only (a form_nl branch in flight on
   // the second stream writes neither)
