@@ -2178,13 +2178,15 @@ struct Impl : Base {
       case PIB_F_ANL: need_adj(); get_spec(J.anl, 3, h, true); break;
       case PIB_F_FR: {
         DevState s;
-        HIPCHK(hipMemcpy(&s, dst, sizeof s, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(&s, dst, sizeof s, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
         get_real(fr, 3, h, true, s.s.frmean);
         break;
       }
       case PIB_F_MASK: {
         std::vector<uint8_t> t((size_t)nx * my * mz);
-        HIPCHK(hipMemcpy(t.data(), mask, t.size(), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(t.data(), mask, t.size(), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
         for (int k = 0; k < mz; ++k)
           for (int j = 0; j < my; ++j)
             for (int i = 0; i < nx; ++i)
@@ -2499,7 +2501,8 @@ struct Impl : Base {
   }
   void adjoint_set_umean(const double* dum) {
     need_adj();
-    HIPCHK(hipMemcpy(J.sc, dum, 3 * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(J.sc, dum, 3 * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
   }
 
   // ---------------- checkpoint stack of the forward state ----------------
