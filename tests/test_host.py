@@ -258,6 +258,8 @@ int main(void) {
   st |= pib_scalar_set(ctx, 0.1, 1.0, 1.0);
   st |= pib_scalar_get(ctx, out);
   st |= pib_scalstep_modes(ctx, 1);
+  st |= pib_set_ferror_history(ctx, PIB_NERR_DEFAULT, NULL);
+  st |= pib_get_ferror_history(ctx, PIB_NERR_DEFAULT, NULL, out);
   st |= pib_dem_create(&dc, NULL, NULL, NULL, &dem);
   return st;
 }
